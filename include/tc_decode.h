@@ -1,0 +1,89 @@
+/*
+ * tc_decode.h -- C ABI of the JPEG 2000 decode path on MI355X.
+ *
+ * Decoder entry points replace Decoder::decode (TC/Decoder.h:14-33,
+ * TC/Decoder.cpp:94-216) for callers that bind the path through an FFI; the
+ * C++ drop-in classes (include/tc/Decoder.h and the stage classes) sit on the
+ * same implementation.  Three forms are offered:
+ *   - per image:  tc_decoder_decode (host codestream in, host image out);
+ *   - resident:   tc_decoder_prepare once, tc_decoder_run any number of
+ *                 times on the device-resident data, tc_decoder_fetch;
+ *   - batched:    tc_decoder_prepare with n codestreams, one launch per
+ *                 stage for the whole batch, optionally replayed from a
+ *                 captured graph.
+ * Output images are interleaved (pixel-major), 1 byte per sample when every
+ * component has at most 8 bits, else 2 bytes (little-endian); signed
+ * components are stored as two's complement of that width.
+ */
+#ifndef TC_DECODE_H
+#define TC_DECODE_H
+
+#include "tc_device.h"
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+typedef struct tc_decoder_s *tc_decoder;
+
+typedef struct {
+    int32_t x0, y0, width, height;
+    int32_t ncomp;
+    int32_t max_prec;
+    int32_t bytes_per_sample; /* of the decoded output: 1 or 2 */
+    int32_t reversible;       /* component 0: 1 = 5/3, 0 = 9/7 */
+    int32_t nlevels, ntx, nty, cbw, cbh, nlayers, mct, progression;
+    int32_t prec[4], sgnd[4]; /* first four components */
+} tc_image_info;
+
+/* tier-2 view of one code-block, for checking the parser */
+typedef struct {
+    int32_t tile, comp, res, band; /* band: 0 LL, 1 HL, 2 LH, 3 HH */
+    int32_t x0, y0, x1, y1;
+    int32_t npasses, zero_bp, mb, nbytes;
+    uint32_t crc; /* FNV-1a of the codeword bytes */
+} tc_cblk_info;
+
+typedef struct {
+    float fill_ms, t1_ms, dwt_ms, final_ms, total_ms; /* of the last timed run */
+    int64_t cblks, codeword_bytes, samples, pixels;
+    int64_t dwt_bytes, final_bytes; /* algorithmic HBM bytes of the DWT levels / final level */
+    int32_t bpw_small, bpw_large;   /* blocks per wave used (0: class absent) */
+    int32_t images;
+    int64_t device_bytes;
+} tc_decoder_stats;
+
+/* Main-header information (SIZ/COD/QCD) of a raw codestream or JP2 file. */
+tc_status tc_image_probe(const uint8_t *cs, size_t len, tc_image_info *info);
+
+/* Parses the whole codestream and lists its code-blocks (tile, component,
+ * resolution, band, raster order).  *count receives the total; at most max are
+ * written to out (which may be NULL). */
+tc_status tc_image_list_cblks(const uint8_t *cs, size_t len, tc_cblk_info *out, int max, int *count);
+
+tc_status tc_decoder_create(tc_device dev, tc_decoder *out);
+tc_status tc_decoder_destroy(tc_decoder dec);
+
+/* Code-block-to-wave packing: blocks per wave (1, 8, 16, 32, 64; 0 = automatic)
+ * for blocks up to 32x32 (small_blocks = 1) or up to 64x64 (small_blocks = 0). */
+tc_status tc_decoder_set_blocks_per_wave(tc_decoder dec, int small_blocks, int bpw);
+
+/* Per-image decode: parse, upload, decode, download (blocking). */
+tc_status tc_decoder_decode(tc_decoder dec, const uint8_t *cs, size_t len, void *out, size_t out_cap,
+                            tc_image_info *info);
+
+/* Resident / batched form. */
+tc_status tc_decoder_prepare(tc_decoder dec, int n, const uint8_t *const *cs, const size_t *lens);
+tc_status tc_decoder_run(tc_decoder dec, int timed);
+tc_status tc_decoder_sync(tc_decoder dec);
+tc_status tc_decoder_fetch(tc_decoder dec, int i, void *out, size_t out_cap);
+tc_status tc_decoder_output(tc_decoder dec, int i, const void **device_ptr, size_t *bytes);
+tc_status tc_decoder_capture_graph(tc_decoder dec);
+tc_status tc_decoder_launch_graph(tc_decoder dec);
+tc_status tc_decoder_get_stats(tc_decoder dec, tc_decoder_stats *stats);
+tc_stream tc_decoder_stream(tc_decoder dec);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

@@ -1,0 +1,94 @@
+"""Writes the committed golden fixtures of tests/golden/.
+
+1. reference_tables.json: the constant tables the reference holds
+   (TC/coefficient_coder.cl:36-60 MQ tables, :330-415 SPCXLUT, :437-451
+   signcxlut; TC/dwt_r97.cl:5-8 and TC/dwt_common.cl:10-11 9/7 constants),
+   parsed as numbers from the reference sources.  Run in the authoring
+   container, where /root/reference exists.
+2. openjpeg_*.j2k with openjpeg_*.npz: small codestreams coded by OpenJPEG
+   2.5.4 (bundled with Pillow 12.2.0) from seeded images, with the source
+   image and OpenJPEG's own decode of each, an independent decoder's output
+   for the oracle and the GPU path to match.
+"""
+import io
+import json
+import os
+import re
+import sys
+
+import numpy as np
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+ROOT = os.path.dirname(os.path.dirname(HERE))
+sys.path.insert(0, ROOT)
+REF = "/root/reference/ThousandthChicken"
+
+
+def c_array(text, name):
+    m = re.search(name + r"[^=]*=\s*\{(.*?)\};", text, re.S)
+    body = re.sub(r"/\*.*?\*/", "", m.group(1), flags=re.S)
+    return [int(v, 0) for v in re.findall(r"0x[0-9A-Fa-f]+|\d+", body)]
+
+
+def tables():
+    cc = open(os.path.join(REF, "coefficient_coder.cl")).read()
+    out = {
+        "source": "TC/coefficient_coder.cl, TC/dwt_r97.cl, TC/dwt_common.cl",
+        "Qe": c_array(cc, r"unsigned short Qe\[\]"),
+        "NMPS": c_array(cc, r"unsigned char NMPS\[\]"),
+        "NLPS": c_array(cc, r"unsigned char NLPS\[\]"),
+        "SWITCH": c_array(cc, r"unsigned char SWITCH\[\]"),
+        "signcxlut": c_array(cc, r"unsigned char signcxlut\[256\]"),
+    }
+    sp = c_array(cc, r"unsigned char SPCXLUT\[3\]\[512\]")
+    out["SPCXLUT"] = [sp[0:512], sp[512:1024], sp[1024:1536]]
+    r97 = open(os.path.join(REF, "dwt_r97.cl")).read() + open(os.path.join(REF, "dwt_common.cl")).read()
+    consts = {}
+    for m in re.finditer(r"CONSTANT float (\w+)\s*=\s*([-+]?\d*\.\d+)f?\s*;", r97):
+        consts[m.group(1)] = float(m.group(2))
+    out["dwt97_constants"] = consts
+    assert len(out["Qe"]) == 47 and len(out["signcxlut"]) == 256
+    return out
+
+
+CASES = [
+    # name, (h, w, c, bits), seed, pillow options
+    ("ll_rgb64", (64, 64, 3, 8), 21, dict(num_resolutions=3, mct=1)),
+    ("ll_rgb100x77_cb32", (100, 77, 3, 8), 22, dict(num_resolutions=4, codeblock_size=(32, 32), mct=1)),
+    ("ll_tiles_offset", (150, 110, 3, 8), 23, dict(num_resolutions=4, tile_size=(64, 64), offset=(3, 5), mct=1)),
+    ("ll_rpcl", (97, 131, 3, 8), 24, dict(num_resolutions=5, progression="RPCL", mct=1)),
+    ("ll_gray16", (120, 90, 1, 16), 25, dict(num_resolutions=5)),
+    ("ll_gray8_nomct", (70, 50, 1, 8), 26, dict(num_resolutions=2)),
+    ("ly_rgb128", (128, 128, 3, 8), 27, dict(irreversible=True, quality_mode="rates", quality_layers=[24],
+                                               num_resolutions=5, mct=1)),
+    ("ly_rgb_layers", (111, 143, 3, 8), 28, dict(irreversible=True, quality_mode="rates",
+                                                   quality_layers=[40, 20, 8], num_resolutions=4, mct=1)),
+    ("ly_gray", (96, 100, 1, 8), 29, dict(irreversible=True, quality_mode="rates", quality_layers=[6],
+                                          num_resolutions=4)),
+]
+
+
+def codestreams():
+    from PIL import Image
+    from thousandthchicken_amd import synth
+    for name, (h, w, c, bits), seed, kw in CASES:
+        a = synth.image(h, w, c, seed, bits)
+        src = a[..., 0] if c == 1 else a
+        b = io.BytesIO()
+        if bits == 16:
+            Image.fromarray(src, "I;16").save(b, "JPEG2000", no_jp2=True, **kw)
+        else:
+            Image.fromarray(src, "RGB" if c == 3 else "L").save(b, "JPEG2000", no_jp2=True, **kw)
+        cs = b.getvalue()
+        dec = np.asarray(Image.open(io.BytesIO(cs)))
+        with open(os.path.join(HERE, f"openjpeg_{name}.j2k"), "wb") as f:
+            f.write(cs)
+        np.savez_compressed(os.path.join(HERE, f"openjpeg_{name}.npz"), source=src, openjpeg=dec)
+        print(name, len(cs))
+
+
+if __name__ == "__main__":
+    if os.path.isdir(REF):
+        with open(os.path.join(HERE, "reference_tables.json"), "w") as f:
+            json.dump(tables(), f)
+    codestreams()

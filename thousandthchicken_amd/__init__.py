@@ -1,0 +1,236 @@
+"""MI355X-native JPEG 2000 (T.800 Part 1) decode path.
+
+Python view of the C ABI in include/tc_device.h and include/tc_decode.h,
+loaded from the in-tree libtc.so.  The library holds the HIP kernels (gfx950)
+and the C++ host path; nothing here computes image data.  Importing works
+without a GPU (parser entry points run on the host); decoding needs one, and
+fails loudly when the library or the device is missing.
+"""
+import ctypes
+import os
+
+import numpy as np
+
+_HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(_HERE, "libtc.so")
+
+_lib = None
+
+
+class TCError(RuntimeError):
+    def __init__(self, status, msg):
+        super().__init__(f"{msg} (status {status})")
+        self.status = status
+
+
+class ImageInfo(ctypes.Structure):
+    _fields_ = [(n, ctypes.c_int32) for n in (
+        "x0", "y0", "width", "height", "ncomp", "max_prec", "bytes_per_sample", "reversible",
+        "nlevels", "ntx", "nty", "cbw", "cbh", "nlayers", "mct", "progression")] + [
+        ("prec", ctypes.c_int32 * 4), ("sgnd", ctypes.c_int32 * 4)]
+
+
+class CblkInfo(ctypes.Structure):
+    _fields_ = [(n, ctypes.c_int32) for n in (
+        "tile", "comp", "res", "band", "x0", "y0", "x1", "y1", "npasses", "zero_bp", "mb",
+        "nbytes")] + [("crc", ctypes.c_uint32)]
+
+
+class DecoderStats(ctypes.Structure):
+    _fields_ = [(n, ctypes.c_float) for n in ("fill_ms", "t1_ms", "dwt_ms", "final_ms", "total_ms")] + [
+        (n, ctypes.c_int64) for n in ("cblks", "codeword_bytes", "samples", "pixels", "dwt_bytes",
+                                      "final_bytes")] + [
+        ("bpw_small", ctypes.c_int32), ("bpw_large", ctypes.c_int32), ("images", ctypes.c_int32),
+        ("device_bytes", ctypes.c_int64)]
+
+
+class DeviceProps(ctypes.Structure):
+    _fields_ = [("name", ctypes.c_char * 64), ("arch", ctypes.c_char * 32),
+                ("compute_units", ctypes.c_int32), ("wavefront_size", ctypes.c_int32),
+                ("xcds", ctypes.c_int32), ("lds_bytes_per_workgroup", ctypes.c_uint64),
+                ("global_mem_bytes", ctypes.c_uint64), ("mem_base_align", ctypes.c_uint32),
+                ("clock_khz", ctypes.c_int32), ("memory_clock_khz", ctypes.c_int32),
+                ("memory_bus_bits", ctypes.c_int32)]
+
+
+def lib():
+    """The loaded libtc.so; raises if it has not been built."""
+    global _lib
+    if _lib is not None:
+        return _lib
+    if not os.path.exists(LIB_PATH):
+        raise TCError(-1, f"{LIB_PATH} is missing: run thousandthchicken_amd/build.py (or __graft_entry__.build())")
+    L = ctypes.CDLL(LIB_PATH)
+    vp, i32, sz = ctypes.c_void_p, ctypes.c_int, ctypes.c_size_t
+    P = ctypes.POINTER
+    sigs = {
+        "tc_last_error": (ctypes.c_char_p, []),
+        "tc_status_string": (ctypes.c_char_p, [i32]),
+        "tc_device_count": (i32, [P(i32)]),
+        "tc_device_open": (i32, [i32, P(vp)]),
+        "tc_device_close": (i32, [vp]),
+        "tc_device_get_props": (i32, [vp, P(DeviceProps)]),
+        "tc_image_probe": (i32, [ctypes.c_char_p, sz, P(ImageInfo)]),
+        "tc_image_list_cblks": (i32, [ctypes.c_char_p, sz, vp, i32, P(i32)]),
+        "tc_decoder_create": (i32, [vp, P(vp)]),
+        "tc_decoder_destroy": (i32, [vp]),
+        "tc_decoder_set_blocks_per_wave": (i32, [vp, i32, i32]),
+        "tc_decoder_decode": (i32, [vp, ctypes.c_char_p, sz, vp, sz, P(ImageInfo)]),
+        "tc_decoder_prepare": (i32, [vp, i32, vp, vp]),
+        "tc_decoder_run": (i32, [vp, i32]),
+        "tc_decoder_sync": (i32, [vp]),
+        "tc_decoder_fetch": (i32, [vp, i32, vp, sz]),
+        "tc_decoder_output": (i32, [vp, i32, P(vp), P(sz)]),
+        "tc_decoder_capture_graph": (i32, [vp]),
+        "tc_decoder_launch_graph": (i32, [vp]),
+        "tc_decoder_get_stats": (i32, [vp, P(DecoderStats)]),
+        "tc_decoder_stream": (vp, [vp]),
+    }
+    for name, (res, args) in sigs.items():
+        f = getattr(L, name)
+        f.restype = res
+        f.argtypes = args
+    _lib = L
+    return L
+
+
+def check(status, what=""):
+    if status != 0:
+        msg = lib().tc_last_error().decode(errors="replace")
+        raise TCError(status, f"{what}: {msg}" if what else msg)
+
+
+def exported_symbols():
+    """Names of every tc_* function declared in include/*.h."""
+    import re
+    inc = os.path.join(_HERE, "..", "include")
+    names = set()
+    for root, _, files in os.walk(inc):
+        for f in files:
+            if f.endswith(".h"):
+                txt = open(os.path.join(root, f)).read()
+                txt = re.sub(r"/\*.*?\*/", "", txt, flags=re.S)
+                txt = re.sub(r"//[^\n]*", "", txt)
+                names.update(re.findall(r"\b(tc_[a-z0-9_]+)\s*\(", txt))
+    return sorted(names)
+
+
+def probe(cs: bytes) -> ImageInfo:
+    info = ImageInfo()
+    check(lib().tc_image_probe(cs, len(cs), ctypes.byref(info)), "tc_image_probe")
+    return info
+
+
+def list_cblks(cs: bytes):
+    n = ctypes.c_int(0)
+    check(lib().tc_image_list_cblks(cs, len(cs), None, 0, ctypes.byref(n)), "tc_image_list_cblks")
+    arr = (CblkInfo * max(n.value, 1))()
+    check(lib().tc_image_list_cblks(cs, len(cs), arr, n.value, ctypes.byref(n)), "tc_image_list_cblks")
+    return [arr[i] for i in range(n.value)]
+
+
+class Device:
+    def __init__(self, ordinal=0):
+        h = ctypes.c_void_p()
+        check(lib().tc_device_open(ordinal, ctypes.byref(h)), "tc_device_open")
+        self.handle = h
+
+    def props(self):
+        p = DeviceProps()
+        check(lib().tc_device_get_props(self.handle, ctypes.byref(p)), "tc_device_get_props")
+        return p
+
+    def close(self):
+        if self.handle:
+            lib().tc_device_close(self.handle)
+            self.handle = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
+_default_device = None
+
+
+def default_device():
+    global _default_device
+    if _default_device is None:
+        _default_device = Device(0)
+    return _default_device
+
+
+def _shape(info: ImageInfo):
+    dt = np.uint8 if info.bytes_per_sample == 1 else np.uint16
+    return (info.height, info.width, info.ncomp), dt
+
+
+class Decoder:
+    """Decoder over the C ABI: per-image decode, and the resident/batched form."""
+
+    def __init__(self, device: Device = None):
+        self.device = device or default_device()
+        h = ctypes.c_void_p()
+        check(lib().tc_decoder_create(self.device.handle, ctypes.byref(h)), "tc_decoder_create")
+        self.handle = h
+        self._infos = []
+        self._keep = None
+
+    def close(self):
+        if getattr(self, "handle", None):
+            lib().tc_decoder_destroy(self.handle)
+            self.handle = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def set_blocks_per_wave(self, small, bpw):
+        check(lib().tc_decoder_set_blocks_per_wave(self.handle, 1 if small else 0, bpw), "set_blocks_per_wave")
+
+    def decode(self, cs: bytes) -> np.ndarray:
+        info = probe(cs)
+        shape, dt = _shape(info)
+        out = np.empty(shape, dt)
+        check(lib().tc_decoder_decode(self.handle, cs, len(cs), out.ctypes.data, out.nbytes, None), "decode")
+        return out
+
+    def prepare(self, streams):
+        streams = [bytes(s) for s in streams]
+        self._infos = [probe(s) for s in streams]
+        n = len(streams)
+        ptrs = (ctypes.c_char_p * n)(*streams)
+        lens = (ctypes.c_size_t * n)(*[len(s) for s in streams])
+        self._keep = (streams, ptrs, lens)
+        check(lib().tc_decoder_prepare(self.handle, n, ctypes.cast(ptrs, ctypes.c_void_p),
+                                       ctypes.cast(lens, ctypes.c_void_p)), "prepare")
+
+    def run(self, timed=False):
+        check(lib().tc_decoder_run(self.handle, 1 if timed else 0), "run")
+
+    def sync(self):
+        check(lib().tc_decoder_sync(self.handle), "sync")
+
+    def capture_graph(self):
+        check(lib().tc_decoder_capture_graph(self.handle), "capture_graph")
+
+    def launch_graph(self):
+        check(lib().tc_decoder_launch_graph(self.handle), "launch_graph")
+
+    def fetch(self, i) -> np.ndarray:
+        shape, dt = _shape(self._infos[i])
+        out = np.empty(shape, dt)
+        check(lib().tc_decoder_fetch(self.handle, i, out.ctypes.data, out.nbytes), "fetch")
+        return out
+
+    def stats(self) -> DecoderStats:
+        s = DecoderStats()
+        check(lib().tc_decoder_get_stats(self.handle, ctypes.byref(s)), "stats")
+        return s
+
+    def stream_handle(self):
+        return lib().tc_decoder_stream(self.handle)

@@ -1,0 +1,791 @@
+// Tier-2 codestream parsing and Annex B geometry (see codestream.hpp).
+#include "codestream.hpp"
+
+#include <algorithm>
+#include <cmath>
+#include <cstring>
+
+namespace tc {
+
+namespace {
+
+constexpr int kMaxLevels = 32;
+
+struct CodingStyle {
+    int nlev = 0, xcb = 6, ycb = 6, cbsty = 0, xform = 1;
+    uint8_t ppx[kMaxLevels + 1], ppy[kMaxLevels + 1];
+    CodingStyle() {
+        std::fill(ppx, ppx + kMaxLevels + 1, 15);
+        std::fill(ppy, ppy + kMaxLevels + 1, 15);
+    }
+};
+
+struct Quant {
+    int style = 0, guard = 2;
+    std::vector<uint16_t> v;  // style 0: exponent only (eps); 1/2: eps<<11|mu
+};
+
+struct Params {
+    int prog = 0, nlayers = 1, mct = 0;
+    bool sop = false, eph = false;
+    std::vector<CodingStyle> cs;
+    std::vector<Quant> q;
+};
+
+struct Range {
+    size_t off, len;
+};
+
+inline int64_t floor_div(int64_t a, int64_t b) {
+    int64_t q = a / b;
+    return (a % b != 0 && ((a < 0) != (b < 0))) ? q - 1 : q;
+}
+inline int64_t ceil_div(int64_t a, int64_t b) { return -floor_div(-a, b); }
+
+class Reader {
+  public:
+    Reader(const uint8_t* p, size_t n, size_t pos = 0) : p_(p), n_(n), pos_(pos) {}
+    bool ok() const { return ok_; }
+    size_t pos() const { return pos_; }
+    void seek(size_t p) { pos_ = p; }
+    size_t size() const { return n_; }
+    uint32_t u8() {
+        if (pos_ >= n_) {
+            ok_ = false;
+            return 0;
+        }
+        return p_[pos_++];
+    }
+    uint32_t u16() {
+        uint32_t a = u8();
+        return (a << 8) | u8();
+    }
+    uint32_t u32() {
+        uint32_t a = u16();
+        return (a << 16) | u16();
+    }
+
+  private:
+    const uint8_t* p_;
+    size_t n_, pos_;
+    bool ok_ = true;
+};
+
+// Packet-header bit reader with the stuffing rule of T.800 B.10.1: after an
+// 0xFF byte only 7 bits of the next byte carry data.
+class BitReader {
+  public:
+    BitReader(const uint8_t* p, size_t end, size_t pos) : p_(p), end_(end), pos_(pos) {}
+    int bit() {
+        if (avail_ == 0) {
+            if (pos_ >= end_) {
+                bad_ = true;
+                return 0;
+            }
+            avail_ = (last_ == 0xFF) ? 7 : 8;
+            last_ = p_[pos_++];
+        }
+        --avail_;
+        return (last_ >> avail_) & 1;
+    }
+    uint32_t bits(int n) {
+        uint32_t v = 0;
+        while (n-- > 0) v = (v << 1) | (uint32_t)bit();
+        return v;
+    }
+    // End of header: a final 0xFF owns the following stuffed byte.
+    void align() {
+        if (last_ == 0xFF) {
+            if (pos_ < end_)
+                ++pos_;
+            else
+                bad_ = true;
+        }
+        avail_ = 0;
+        last_ = 0;
+    }
+    size_t pos() const { return pos_; }
+    bool bad() const { return bad_; }
+
+  private:
+    const uint8_t* p_;
+    size_t end_, pos_;
+    uint32_t last_ = 0;
+    int avail_ = 0;
+    bool bad_ = false;
+};
+
+// Tag tree over a grid of leaves (T.800 B.10.2).  Each node keeps its lower
+// bound and whether its value is known; parents are found by halving.
+class TagTree {
+  public:
+    void init(int w, int h) {
+        levels_.clear();
+        int lw = w, lh = h;
+        size_t total = 0;
+        for (;;) {
+            levels_.push_back({lw, lh, total});
+            total += (size_t)lw * lh;
+            if (lw == 1 && lh == 1) break;
+            lw = (lw + 1) / 2;
+            lh = (lh + 1) / 2;
+        }
+        low_.assign(total, 0);
+        known_.assign(total, 0);
+    }
+    // true when the value of leaf (x,y) is below thr
+    bool decode(BitReader& br, int x, int y, int thr) {
+        int path[40];
+        int nl = (int)levels_.size();
+        for (int l = 0; l < nl; l++) {
+            path[l] = (int)(levels_[l].base + (size_t)y * levels_[l].w + x);
+            x >>= 1;
+            y >>= 1;
+        }
+        int low = 0;
+        for (int l = nl - 1; l >= 0; l--) {
+            int q = path[l];
+            if (low_[q] < low) low_[q] = low;
+            while (!known_[q] && low_[q] < thr) {
+                if (br.bit())
+                    known_[q] = 1;
+                else
+                    ++low_[q];
+                if (br.bad()) return false;
+            }
+            low = low_[q];
+            if (!known_[q]) return false;
+        }
+        return low < thr;
+    }
+    // full value of leaf (x,y)
+    int value(BitReader& br, int x, int y) {
+        int t = 1;
+        while (!decode(br, x, y, t)) {
+            if (br.bad() || t > 64) return -1;
+            ++t;
+        }
+        return t - 1;
+    }
+
+  private:
+    struct Lvl {
+        int w, h;
+        size_t base;
+    };
+    std::vector<Lvl> levels_;
+    std::vector<int> low_;
+    std::vector<uint8_t> known_;
+};
+
+struct BandState {
+    TagTree incl, zbp;
+};
+
+Status read_cs_params(Reader& r, CodingStyle& c, bool user_prec) {
+    c.nlev = (int)r.u8();
+    c.xcb = (int)r.u8() + 2;
+    c.ycb = (int)r.u8() + 2;
+    c.cbsty = (int)r.u8();
+    c.xform = (int)r.u8();
+    if (!r.ok()) return Status::Truncated;
+    if (c.nlev > kMaxLevels || c.xcb > 10 || c.ycb > 10 || c.xcb + c.ycb > 12 || c.xform > 1)
+        return Status::Unsupported;
+    for (int i = 0; i <= c.nlev; i++) {
+        if (user_prec) {
+            uint32_t b = r.u8();
+            c.ppx[i] = (uint8_t)(b & 15);
+            c.ppy[i] = (uint8_t)(b >> 4);
+        } else {
+            c.ppx[i] = c.ppy[i] = 15;
+        }
+    }
+    return r.ok() ? Status::Ok : Status::Truncated;
+}
+
+Status read_quant(Reader& r, size_t end, Quant& q) {
+    uint32_t s = r.u8();
+    q.style = (int)(s & 31);
+    q.guard = (int)(s >> 5);
+    q.v.clear();
+    if (q.style == 0) {
+        while (r.pos() < end) q.v.push_back((uint16_t)((r.u8() >> 3) << 11));
+    } else if (q.style == 1 || q.style == 2) {
+        while (r.pos() + 2 <= end) q.v.push_back((uint16_t)r.u16());
+    } else {
+        return Status::Unsupported;
+    }
+    return r.ok() ? Status::Ok : Status::Truncated;
+}
+
+// One parameter marker segment (COD/COC/QCD/QCC) or a skippable one.
+Status read_segment(Reader& r, uint32_t marker, int ncomp, Params& P, std::vector<uint8_t>& coc_set,
+                    std::vector<uint8_t>& qcc_set, std::string& err) {
+    size_t lpos = r.pos();
+    uint32_t L = r.u16();
+    if (!r.ok() || L < 2 || lpos + L > r.size()) return Status::Truncated;
+    size_t end = lpos + L;
+    Status st = Status::Ok;
+    switch (marker) {
+    case 0xFF52: {  // COD
+        uint32_t scod = r.u8();
+        P.prog = (int)r.u8();
+        P.nlayers = (int)r.u16();
+        P.mct = (int)r.u8();
+        P.sop = (scod & 2) != 0;
+        P.eph = (scod & 4) != 0;
+        CodingStyle c;
+        st = read_cs_params(r, c, (scod & 1) != 0);
+        if (st != Status::Ok) break;
+        for (int i = 0; i < ncomp; i++)
+            if (!coc_set[i]) P.cs[i] = c;
+        if (P.prog > 4 || P.nlayers < 1) st = Status::Unsupported;
+        break;
+    }
+    case 0xFF53: {  // COC
+        uint32_t ci = ncomp < 257 ? r.u8() : r.u16();
+        uint32_t scoc = r.u8();
+        if ((int)ci >= ncomp) return Status::BadMarker;
+        st = read_cs_params(r, P.cs[ci], (scoc & 1) != 0);
+        coc_set[ci] = 1;
+        break;
+    }
+    case 0xFF5C: {  // QCD
+        Quant q;
+        st = read_quant(r, end, q);
+        if (st != Status::Ok) break;
+        for (int i = 0; i < ncomp; i++)
+            if (!qcc_set[i]) P.q[i] = q;
+        break;
+    }
+    case 0xFF5D: {  // QCC
+        uint32_t ci = ncomp < 257 ? r.u8() : r.u16();
+        if ((int)ci >= ncomp) return Status::BadMarker;
+        st = read_quant(r, end, P.q[ci]);
+        qcc_set[ci] = 1;
+        break;
+    }
+    case 0xFF64:  // COM
+    case 0xFF55:  // TLM
+    case 0xFF57:  // PLM
+    case 0xFF58:  // PLT
+    case 0xFF63:  // CRG
+        break;
+    default: {
+        char b[64];
+        snprintf(b, sizeof b, "unsupported marker 0x%04X", marker);
+        err = b;
+        return Status::Unsupported;  // RGN, POC, PPM, PPT and unknown markers
+    }
+    }
+    if (st != Status::Ok) return st;
+    r.seek(end);
+    return Status::Ok;
+}
+
+// Locates the codestream inside a JP2 file (signature box, then the 'jp2c'
+// box, as TC/boxes.c:245-307 does for the reference), or returns the input.
+bool locate_codestream(const uint8_t* d, size_t n, const uint8_t** cs, size_t* cl) {
+    static const uint8_t sig[12] = {0, 0, 0, 12, 'j', 'P', ' ', ' ', 0x0D, 0x0A, 0x87, 0x0A};
+    if (n >= 12 && std::memcmp(d, sig, 12) == 0) {
+        size_t pos = 0;
+        while (pos + 8 <= n) {
+            uint64_t L = ((uint64_t)d[pos] << 24) | ((uint64_t)d[pos + 1] << 16) | ((uint64_t)d[pos + 2] << 8) | d[pos + 3];
+            size_t hdr = 8;
+            if (L == 1) {
+                if (pos + 16 > n) return false;
+                L = 0;
+                for (int i = 0; i < 8; i++) L = (L << 8) | d[pos + 8 + i];
+                hdr = 16;
+            } else if (L == 0) {
+                L = n - pos;
+            }
+            if (L < hdr) return false;
+            if (std::memcmp(d + pos + 4, "jp2c", 4) == 0) {
+                *cs = d + pos + hdr;
+                *cl = std::min<uint64_t>(L, n - pos) - hdr;
+                return true;
+            }
+            pos += (size_t)L;
+        }
+        return false;
+    }
+    *cs = d;
+    *cl = n;
+    return true;
+}
+
+Status parse_main_header(Reader& r, Image& img, Params& P, std::string& err) {
+    if (r.u16() != 0xFF4F) {
+        err = "missing SOC";
+        return Status::BadMarker;
+    }
+    if (r.u16() != 0xFF51) {
+        err = "missing SIZ";
+        return Status::BadMarker;
+    }
+    size_t lpos = r.pos();
+    uint32_t L = r.u16();
+    r.u16();  // Rsiz
+    img.x1 = (int)r.u32();
+    img.y1 = (int)r.u32();
+    img.x0 = (int)r.u32();
+    img.y0 = (int)r.u32();
+    img.tw = (int)r.u32();
+    img.th = (int)r.u32();
+    img.tx0 = (int)r.u32();
+    img.ty0 = (int)r.u32();
+    img.ncomp = (int)r.u16();
+    if (!r.ok()) return Status::Truncated;
+    if (img.ncomp < 1 || img.ncomp > 16384) return Status::Unsupported;
+    img.prec.resize(img.ncomp);
+    img.sgnd.resize(img.ncomp);
+    for (int i = 0; i < img.ncomp; i++) {
+        uint32_t s = r.u8();
+        img.prec[i] = (int)(s & 0x7F) + 1;
+        img.sgnd[i] = (int)(s >> 7);
+        uint32_t dx = r.u8(), dy = r.u8();
+        if (dx != 1 || dy != 1) {
+            err = "component sub-sampling is not supported";
+            return Status::Unsupported;
+        }
+        if (img.prec[i] > 16) {
+            err = "bit depth above 16";
+            return Status::Unsupported;
+        }
+    }
+    if (!r.ok()) return Status::Truncated;
+    r.seek(lpos + L);
+    if (img.x1 <= img.x0 || img.y1 <= img.y0 || img.tw <= 0 || img.th <= 0 || img.tx0 > img.x0 ||
+        img.ty0 > img.y0 || (int64_t)img.tx0 + img.tw <= img.x0 || (int64_t)img.ty0 + img.th <= img.y0) {
+        err = "bad image/tile geometry";
+        return Status::BadGeometry;
+    }
+    img.ntx = (int)ceil_div(img.x1 - img.tx0, img.tw);
+    img.nty = (int)ceil_div(img.y1 - img.ty0, img.th);
+    if ((int64_t)img.ntx * img.nty > 65535) {  // Isot is 16 bits (A.4.2)
+        err = "more than 65535 tiles";
+        return Status::BadGeometry;
+    }
+    if ((int64_t)img.width() * img.height() > ((int64_t)1 << 34)) {
+        err = "image area beyond 2^34 samples";
+        return Status::Capacity;
+    }
+    P.cs.assign(img.ncomp, CodingStyle());
+    P.q.assign(img.ncomp, Quant());
+    std::vector<uint8_t> coc(img.ncomp, 0), qcc(img.ncomp, 0);
+    bool cod = false, qcd = false;
+    for (;;) {
+        uint32_t m = r.u16();
+        if (!r.ok()) return Status::Truncated;
+        if (m == 0xFF90) {
+            r.seek(r.pos() - 2);
+            break;
+        }
+        cod |= m == 0xFF52;
+        qcd |= m == 0xFF5C;
+        Status s = read_segment(r, m, img.ncomp, P, coc, qcc, err);
+        if (s != Status::Ok) return s;
+    }
+    if (!cod || !qcd) {
+        err = "main header lacks COD or QCD";
+        return Status::BadMarker;
+    }
+    img.progression = P.prog;
+    img.nlayers = P.nlayers;
+    img.mct = P.mct;
+    return Status::Ok;
+}
+
+// Geometry of one tile and its code-blocks (Annex B.3-B.7).
+Status build_tile(Image& img, int t, const Params& P, std::vector<std::vector<BandState>>& bstate, std::string& err) {
+    Tile& T = img.tiles[t];
+    int p = t % img.ntx, q = t / img.ntx;
+    T.x0 = std::max(img.tx0 + p * img.tw, img.x0);
+    T.y0 = std::max(img.ty0 + q * img.th, img.y0);
+    T.x1 = std::min((int64_t)img.tx0 + (int64_t)(p + 1) * img.tw, (int64_t)img.x1);
+    T.y1 = std::min((int64_t)img.ty0 + (int64_t)(q + 1) * img.th, (int64_t)img.y1);
+    T.mct = P.mct;
+    T.comps.resize(img.ncomp);
+    bstate.resize(img.ncomp);
+    for (int c = 0; c < img.ncomp; c++) {
+        const CodingStyle& C = P.cs[c];
+        const Quant& Q = P.q[c];
+        TileComp& tc = T.comps[c];
+        tc.x0 = T.x0;
+        tc.y0 = T.y0;
+        tc.x1 = T.x1;
+        tc.y1 = T.y1;
+        tc.nlev = C.nlev;
+        tc.reversible = C.xform;
+        tc.cbsty = C.cbsty;
+        if (C.cbsty != 0) {
+            err = "code-block style flags are not supported";
+            return Status::Unsupported;
+        }
+        int nq = Q.style == 1 ? 1 : 3 * C.nlev + 1;
+        if ((int)Q.v.size() < nq) {
+            err = "quantization marker too short";
+            return Status::BadMarker;
+        }
+        tc.res.assign(C.nlev + 1, Resolution());
+        bstate[c].assign((C.nlev + 1) * 3, BandState());
+        for (int r = 0; r <= C.nlev; r++) {
+            Resolution& R = tc.res[r];
+            int64_t s = (int64_t)1 << (C.nlev - r);
+            R.x0 = (int)ceil_div(tc.x0, s);
+            R.y0 = (int)ceil_div(tc.y0, s);
+            R.x1 = (int)ceil_div(tc.x1, s);
+            R.y1 = (int)ceil_div(tc.y1, s);
+            if (R.x1 > R.x0 && R.y1 > R.y0) {
+                int64_t ppw = (int64_t)1 << C.ppx[r], pph = (int64_t)1 << C.ppy[r];
+                if (ceil_div(R.x1, ppw) - floor_div(R.x0, ppw) != 1 || ceil_div(R.y1, pph) - floor_div(R.y0, pph) != 1) {
+                    err = "more than one precinct per resolution is not supported";
+                    return Status::Unsupported;
+                }
+            }
+            R.nbands = r == 0 ? 1 : 3;
+            int nb = r == 0 ? C.nlev : C.nlev - r + 1;
+            for (int k = 0; k < R.nbands; k++) {
+                Band& B = R.band[k];
+                B.orient = r == 0 ? 0 : k + 1;
+                int xo = (B.orient == 1 || B.orient == 3), yo = (B.orient == 2 || B.orient == 3);
+                if (nb == 0) {
+                    B.x0 = tc.x0;
+                    B.y0 = tc.y0;
+                    B.x1 = tc.x1;
+                    B.y1 = tc.y1;
+                } else {
+                    int64_t bs = (int64_t)1 << nb, hs = (int64_t)1 << (nb - 1);
+                    B.x0 = (int)ceil_div(tc.x0 - hs * xo, bs);
+                    B.y0 = (int)ceil_div(tc.y0 - hs * yo, bs);
+                    B.x1 = (int)ceil_div(tc.x1 - hs * xo, bs);
+                    B.y1 = (int)ceil_div(tc.y1 - hs * yo, bs);
+                }
+                B.xcb = std::min<int>(C.xcb, r == 0 ? C.ppx[r] : C.ppx[r] - 1);
+                B.ycb = std::min<int>(C.ycb, r == 0 ? C.ppy[r] : C.ppy[r] - 1);
+                // quantization (E.1): derived style scales the LL exponent per level
+                int qi = r == 0 ? 0 : 3 * (r - 1) + k + 1;
+                uint16_t qv = Q.style == 1 ? Q.v[0] : Q.v[qi];
+                B.eps = qv >> 11;
+                B.mu = qv & 0x7FF;
+                if (Q.style == 1 && r > 0) B.eps = B.eps - C.nlev + nb;
+                B.mb = Q.guard + B.eps - 1;
+                int gain = B.orient == 0 ? 0 : (B.orient == 3 ? 2 : 1);
+                B.delta = (float)std::ldexp(1.0 + B.mu / 2048.0, img.prec[c] + gain - B.eps);
+                B.first_cblk = (int)img.cblks.size();
+                if (B.x1 > B.x0 && B.y1 > B.y0) {
+                    int64_t cw = (int64_t)1 << B.xcb, ch = (int64_t)1 << B.ycb;
+                    int gx0 = (int)floor_div(B.x0, cw), gy0 = (int)floor_div(B.y0, ch);
+                    B.ncx = (int)(ceil_div(B.x1, cw) - gx0);
+                    B.ncy = (int)(ceil_div(B.y1, ch) - gy0);
+                    if ((int64_t)img.cblks.size() + (int64_t)B.ncx * B.ncy > ((int64_t)1 << 24)) {
+                        err = "more than 2^24 code-blocks";
+                        return Status::Capacity;
+                    }
+                    for (int j = 0; j < B.ncy; j++)
+                        for (int i = 0; i < B.ncx; i++) {
+                            CodeBlock cb;
+                            cb.tile = t;
+                            cb.comp = c;
+                            cb.res = r;
+                            cb.band = k;
+                            cb.x0 = std::max<int64_t>(B.x0, (gx0 + i) * cw);
+                            cb.y0 = std::max<int64_t>(B.y0, (gy0 + j) * ch);
+                            cb.x1 = std::min<int64_t>(B.x1, (gx0 + i + 1) * cw);
+                            cb.y1 = std::min<int64_t>(B.y1, (gy0 + j + 1) * ch);
+                            img.cblks.push_back(cb);
+                        }
+                    bstate[c][r * 3 + k].incl.init(B.ncx, B.ncy);
+                    bstate[c][r * 3 + k].zbp.init(B.ncx, B.ncy);
+                } else {
+                    B.ncx = B.ncy = 0;
+                }
+            }
+        }
+    }
+    return Status::Ok;
+}
+
+int pass_count(BitReader& br) {  // B.10.6
+    if (!br.bit()) return 1;
+    if (!br.bit()) return 2;
+    uint32_t v = br.bits(2);
+    if (v != 3) return 3 + (int)v;
+    v = br.bits(5);
+    if (v != 31) return 6 + (int)v;
+    return 37 + (int)br.bits(7);
+}
+
+int ilog2(int v) {
+    int l = 0;
+    while (v > 1) {
+        v >>= 1;
+        ++l;
+    }
+    return l;
+}
+
+// Reads one packet of (component c, resolution r, layer l) at *pos within the
+// tile data ranges.
+Status read_packet(Image& img, int t, int c, int r, int layer, const Params& P,
+                   std::vector<std::vector<BandState>>& bstate, const std::vector<Range>& ranges, size_t& ri,
+                   size_t& pos, std::string& err) {
+    const TileComp& tc = img.tiles[t].comps[c];
+    const Resolution& R = tc.res[r];
+    if (R.x1 <= R.x0 || R.y1 <= R.y0) return Status::Ok;  // empty resolution: no precinct
+    while (ri < ranges.size() && pos >= ranges[ri].off + ranges[ri].len) {
+        ++ri;
+        if (ri < ranges.size()) pos = ranges[ri].off;
+    }
+    if (ri >= ranges.size()) return Status::Ok;  // stream ends at a packet boundary
+    const uint8_t* d = img.cs;
+    size_t end = ranges[ri].off + ranges[ri].len;
+    if (P.sop && pos + 6 <= end && d[pos] == 0xFF && d[pos + 1] == 0x91) pos += 6;
+    BitReader br(d, end, pos);
+    struct Inc {
+        uint32_t cb;
+        uint32_t len;
+    };
+    Inc incs[512];
+    std::vector<Inc> more;
+    int ninc = 0;
+    if (br.bit()) {
+        for (int k = 0; k < R.nbands; k++) {
+            const Band& B = R.band[k];
+            BandState& S = bstate[c][r * 3 + k];
+            for (int j = 0; j < B.ncy; j++)
+                for (int i = 0; i < B.ncx; i++) {
+                    uint32_t id = (uint32_t)(B.first_cblk + j * B.ncx + i);
+                    CodeBlock& cb = img.cblks[id];
+                    bool inc = cb.included ? br.bit() != 0 : S.incl.decode(br, i, j, layer + 1);
+                    if (br.bad()) break;
+                    if (!inc) continue;
+                    if (!cb.included) {
+                        int z = S.zbp.value(br, i, j);
+                        if (z < 0) {
+                            err = "bad zero bit-plane tag tree";
+                            return Status::Truncated;
+                        }
+                        cb.zbp = z;
+                        cb.included = true;
+                        cb.numbps = B.mb - z;
+                    }
+                    int np = pass_count(br);
+                    while (br.bit()) {
+                        if (br.bad() || ++cb.lblock > 32) break;
+                    }
+                    uint32_t L = br.bits(cb.lblock + ilog2(np));
+                    cb.npasses += np;
+                    Inc e{id, L};
+                    if (ninc < 512)
+                        incs[ninc] = e;
+                    else
+                        more.push_back(e);
+                    ++ninc;
+                }
+        }
+    }
+    br.align();
+    if (br.bad()) {
+        err = "truncated packet header";
+        return Status::Truncated;
+    }
+    pos = br.pos();
+    if (P.eph) {
+        if (pos + 2 <= end && d[pos] == 0xFF && d[pos + 1] == 0x92)
+            pos += 2;
+        else {
+            err = "missing EPH";
+            return Status::BadMarker;
+        }
+    }
+    for (int s = 0; s < ninc; s++) {
+        const Inc& e = s < 512 ? incs[s] : more[s - 512];
+        if (pos + e.len > end) {
+            err = "truncated packet body";
+            return Status::Truncated;
+        }
+        CodeBlock& cb = img.cblks[e.cb];
+        if (e.len) {
+            if (cb.nseg == 0) cb.seg0_off = (uint32_t)pos;
+            img.segs.push_back(Segment{e.cb, (uint32_t)pos, e.len});
+            cb.nseg++;
+            cb.nbytes += e.len;
+        }
+        pos += e.len;
+    }
+    return Status::Ok;
+}
+
+Status read_packets(Image& img, int t, const Params& P, std::vector<std::vector<BandState>>& bstate,
+                    const std::vector<Range>& ranges, std::string& err) {
+    if (ranges.empty()) return Status::Ok;
+    size_t ri = 0, pos = ranges[0].off;
+    int C = img.ncomp, L = P.nlayers, maxr = 0;
+    for (int c = 0; c < C; c++) maxr = std::max(maxr, img.tiles[t].comps[c].nlev + 1);
+    auto pk = [&](int c, int r, int l) -> Status {
+        if (r > img.tiles[t].comps[c].nlev) return Status::Ok;
+        return read_packet(img, t, c, r, l, P, bstate, ranges, ri, pos, err);
+    };
+    Status s = Status::Ok;
+    switch (P.prog) {
+    case 0:  // LRCP
+        for (int l = 0; l < L; l++)
+            for (int r = 0; r < maxr; r++)
+                for (int c = 0; c < C; c++)
+                    if ((s = pk(c, r, l)) != Status::Ok) return s;
+        break;
+    case 1:  // RLCP
+        for (int r = 0; r < maxr; r++)
+            for (int l = 0; l < L; l++)
+                for (int c = 0; c < C; c++)
+                    if ((s = pk(c, r, l)) != Status::Ok) return s;
+        break;
+    case 2:  // RPCL (one precinct per resolution)
+        for (int r = 0; r < maxr; r++)
+            for (int c = 0; c < C; c++)
+                for (int l = 0; l < L; l++)
+                    if ((s = pk(c, r, l)) != Status::Ok) return s;
+        break;
+    default:  // PCRL, CPRL (one precinct per resolution)
+        for (int c = 0; c < C; c++)
+            for (int r = 0; r < maxr; r++)
+                for (int l = 0; l < L; l++)
+                    if ((s = pk(c, r, l)) != Status::Ok) return s;
+        break;
+    }
+    return Status::Ok;
+}
+
+Status parse_impl(const uint8_t* data, size_t len, Image& img, std::string& err, bool packets) {
+    img = Image();
+    const uint8_t* cs;
+    size_t cl;
+    if (!data || !locate_codestream(data, len, &cs, &cl)) {
+        err = "no codestream found";
+        return Status::BadMarker;
+    }
+    img.cs = cs;
+    img.cs_len = cl;
+    Reader r(cs, cl);
+    Params main;
+    Status s = parse_main_header(r, img, main, err);
+    if (s != Status::Ok || !packets) {
+        if (s == Status::Ok) {
+            // geometry of tile 0 only, for probing
+            img.tiles.resize((size_t)img.ntx * img.nty);
+            std::vector<std::vector<BandState>> bs;
+            s = build_tile(img, 0, main, bs, err);
+            img.cblks.clear();
+        }
+        return s;
+    }
+    int nt = img.ntx * img.nty;
+    img.tiles.resize(nt);
+    std::vector<Params> tp(nt);
+    std::vector<uint8_t> seen(nt, 0);
+    std::vector<std::vector<Range>> ranges(nt);
+    // tile-parts (A.4.2)
+    for (;;) {
+        if (r.pos() + 2 > cl) break;  // tolerate a missing EOC
+        size_t sot = r.pos();
+        uint32_t m = r.u16();
+        if (m == 0xFFD9) break;
+        if (m != 0xFF90) {
+            err = "expected SOT";
+            return Status::BadMarker;
+        }
+        uint32_t Lsot = r.u16();
+        uint32_t I = r.u16();
+        uint32_t Psot = r.u32();
+        uint32_t TP = r.u8();
+        r.u8();
+        if (!r.ok()) return Status::Truncated;
+        if (Lsot != 10 || (int)I >= nt) {
+            err = "bad SOT";
+            return Status::BadMarker;
+        }
+        size_t end = Psot ? sot + Psot : (cl >= 2 ? cl - 2 : cl);
+        if (end > cl || end < r.pos()) {
+            err = "tile-part runs past the codestream";
+            return Status::Truncated;
+        }
+        if (!seen[I]) {
+            tp[I] = main;
+            seen[I] = 1;
+        }
+        std::vector<uint8_t> coc(img.ncomp, 0), qcc(img.ncomp, 0);
+        for (;;) {
+            uint32_t mk = r.u16();
+            if (!r.ok()) return Status::Truncated;
+            if (mk == 0xFF93) break;
+            if (TP != 0 && (mk == 0xFF52 || mk == 0xFF53 || mk == 0xFF5C || mk == 0xFF5D)) {
+                err = "coding parameters in a later tile-part";
+                return Status::BadMarker;
+            }
+            s = read_segment(r, mk, img.ncomp, tp[I], coc, qcc, err);
+            if (s != Status::Ok) return s;
+        }
+        if (end > r.pos()) ranges[I].push_back(Range{r.pos(), end - r.pos()});
+        r.seek(end);
+    }
+    for (int t = 0; t < nt; t++) {
+        const Params& P = seen[t] ? tp[t] : main;
+        std::vector<std::vector<BandState>> bs;
+        s = build_tile(img, t, P, bs, err);
+        if (s != Status::Ok) return s;
+        s = read_packets(img, t, P, bs, ranges[t], err);
+        if (s != Status::Ok) return s;
+    }
+    for (auto& cb : img.cblks) {
+        if (!cb.included) continue;
+        if (cb.numbps > 31 || cb.numbps < 0) {
+            err = "bit-plane count out of range";
+            return Status::Unsupported;
+        }
+        int maxp = cb.numbps > 0 ? 3 * cb.numbps - 2 : 0;
+        if (cb.npasses > maxp) cb.npasses = maxp;
+    }
+    return Status::Ok;
+}
+
+}  // namespace
+
+int Image::max_prec() const {
+    int m = 0;
+    for (int p : prec) m = std::max(m, p);
+    return m;
+}
+
+bool Image::all_reversible() const {
+    for (const auto& t : tiles)
+        for (const auto& c : t.comps)
+            if (!c.reversible) return false;
+    return true;
+}
+
+Status parse_codestream(const uint8_t* data, size_t len, Image& img, std::string& err) {
+    return parse_impl(data, len, img, err, true);
+}
+
+Status probe_codestream(const uint8_t* data, size_t len, Image& img, std::string& err) {
+    return parse_impl(data, len, img, err, false);
+}
+
+const char* status_name(Status s) {
+    switch (s) {
+    case Status::Ok: return "ok";
+    case Status::InvalidArgument: return "invalid argument";
+    case Status::Truncated: return "truncated codestream";
+    case Status::BadMarker: return "bad marker";
+    case Status::Unsupported: return "unsupported feature";
+    case Status::BadGeometry: return "bad geometry";
+    case Status::OutOfMemory: return "out of memory";
+    case Status::Device: return "device error";
+    case Status::Capacity: return "capacity exceeded";
+    }
+    return "unknown";
+}
+
+}  // namespace tc

@@ -1,0 +1,116 @@
+// Tier-2 codestream parsing and Annex B geometry for the decode path.
+//
+// Replaces the reference's host C parser TC/codestream.c (main header
+// :233-258, COD :90-124, QCD :140-230, tile header :260-298, packet header
+// :336-453, packet body :455-470), its geometry TC/codestream_image.c:14-277
+// and tag trees TC/codestream_tag_tree_encode.c:23-132.  Differences: the parse
+// returns a flat table of code-blocks with codeword segments (offsets into the
+// codestream, no per-block malloc/memcpy as in TC/Decoder.cpp:71-79), reports
+// errors instead of exit(0) (TC/io_buffered_stream.c:42-46), and covers what
+// the reference lacks: expounded quantization, COC/QCC, tile-part headers,
+// COM/TLM/PLM/PLT/CRG skipping, several layers and tile-parts, the five
+// progression orders with one precinct per resolution, empty packets,
+// SOP/EPH, edge tiles and per-component precision and signedness.
+#pragma once
+#include <stddef.h>
+#include <stdint.h>
+
+#include <string>
+#include <vector>
+
+namespace tc {
+
+enum class Status : int32_t {
+    Ok = 0,
+    InvalidArgument = 1,
+    Truncated = 2,
+    BadMarker = 3,
+    Unsupported = 4,
+    BadGeometry = 5,
+    OutOfMemory = 6,
+    Device = 7,
+    Capacity = 8,
+};
+
+struct Band {
+    int orient = 0;  // 0 LL, 1 HL, 2 LH, 3 HH
+    int x0 = 0, y0 = 0, x1 = 0, y1 = 0;
+    int xcb = 6, ycb = 6;      // code-block exponents after the precinct limit
+    int ncx = 0, ncy = 0;      // code-blocks across / down
+    int eps = 0, mu = 0, mb = 0;
+    float delta = 1.0f;        // quantization step (irreversible)
+    int first_cblk = 0;        // index into Image::cblks
+};
+
+struct Resolution {
+    int x0 = 0, y0 = 0, x1 = 0, y1 = 0;
+    int nbands = 0;
+    Band band[3];
+    int width() const { return x1 - x0; }
+    int height() const { return y1 - y0; }
+};
+
+struct TileComp {
+    int x0 = 0, y0 = 0, x1 = 0, y1 = 0;
+    int nlev = 0;
+    int reversible = 1;  // 1: 5/3, 0: 9/7
+    int cbsty = 0;
+    std::vector<Resolution> res;  // nlev + 1
+    int width() const { return x1 - x0; }
+    int height() const { return y1 - y0; }
+};
+
+struct Tile {
+    int x0 = 0, y0 = 0, x1 = 0, y1 = 0;
+    int mct = 0;
+    std::vector<TileComp> comps;
+};
+
+struct CodeBlock {
+    int tile = 0, comp = 0, res = 0, band = 0;  // band index within the resolution
+    int x0 = 0, y0 = 0, x1 = 0, y1 = 0;         // subband coordinates
+    int npasses = 0;
+    int zbp = 0;          // missing most-significant bit-planes
+    int numbps = 0;       // bit-planes present: M_b - zbp
+    int lblock = 3;
+    bool included = false;
+    uint32_t nbytes = 0;  // codeword bytes over all segments
+    int nseg = 0;
+    uint32_t seg0_off = 0;  // first segment offset in the codestream
+};
+
+struct Segment {
+    uint32_t cblk;
+    uint32_t off;  // offset in the codestream
+    uint32_t len;
+};
+
+struct Image {
+    // SIZ
+    int x0 = 0, y0 = 0, x1 = 0, y1 = 0;
+    int tx0 = 0, ty0 = 0, tw = 0, th = 0, ntx = 0, nty = 0;
+    int ncomp = 0;
+    std::vector<int> prec, sgnd;
+    // COD (main header)
+    int progression = 0, nlayers = 1, mct = 0;
+    std::vector<Tile> tiles;
+    std::vector<CodeBlock> cblks;
+    std::vector<Segment> segs;  // in codestream order; contiguous per block when nseg == 1
+    const uint8_t* cs = nullptr;  // codestream (inside a JP2 wrapper if any)
+    size_t cs_len = 0;
+    int width() const { return x1 - x0; }
+    int height() const { return y1 - y0; }
+    int max_prec() const;
+    bool all_reversible() const;
+};
+
+// Parses a raw codestream or a JP2 file.  The image keeps a pointer into
+// `data`, which must outlive it.
+Status parse_codestream(const uint8_t* data, size_t len, Image& img, std::string& err);
+
+// Main header only (SIZ/COD/QCD), no packets.
+Status probe_codestream(const uint8_t* data, size_t len, Image& img, std::string& err);
+
+const char* status_name(Status s);
+
+}  // namespace tc

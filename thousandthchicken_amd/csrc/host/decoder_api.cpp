@@ -1,0 +1,243 @@
+// C ABI of the decode path (include/tc_decode.h).
+#include <cstring>
+#include <memory>
+#include <string>
+#include <vector>
+
+#include "../../../include/tc_decode.h"
+#include "../tc_internal.hpp"
+#include "codestream.hpp"
+#include "pipeline.hpp"
+
+using namespace tc;
+
+struct tc_decoder_s {
+    tc_device dev;
+    std::unique_ptr<Pipeline> pipe;
+    std::vector<std::vector<uint8_t>> data;  // owned copies of the prepared codestreams
+    std::vector<std::unique_ptr<Image>> imgs;
+};
+
+namespace {
+
+tc_status to_status(Status s) {
+    switch (s) {
+    case Status::Ok: return TC_SUCCESS;
+    case Status::InvalidArgument: return TC_ERR_INVALID_ARG;
+    case Status::Truncated: return TC_ERR_TRUNCATED;
+    case Status::BadMarker: return TC_ERR_BAD_MARKER;
+    case Status::Unsupported: return TC_ERR_UNSUPPORTED;
+    case Status::BadGeometry: return TC_ERR_GEOMETRY;
+    case Status::OutOfMemory: return TC_ERR_OUT_OF_MEMORY;
+    case Status::Device: return TC_ERR_DEVICE;
+    case Status::Capacity: return TC_ERR_CAPACITY;
+    }
+    return TC_ERR_DEVICE;
+}
+
+tc_status report(Status s, const std::string& err) {
+    if (s == Status::Ok) return TC_SUCCESS;
+    set_error(err.empty() ? status_name(s) : err);
+    return to_status(s);
+}
+
+void fill_info(const Image& im, tc_image_info* info) {
+    std::memset(info, 0, sizeof *info);
+    info->x0 = im.x0;
+    info->y0 = im.y0;
+    info->width = im.width();
+    info->height = im.height();
+    info->ncomp = im.ncomp;
+    info->max_prec = im.max_prec();
+    info->bytes_per_sample = info->max_prec > 8 ? 2 : 1;
+    info->ntx = im.ntx;
+    info->nty = im.nty;
+    info->nlayers = im.nlayers;
+    info->mct = im.mct;
+    info->progression = im.progression;
+    if (!im.tiles.empty() && !im.tiles[0].comps.empty()) {
+        const TileComp& tc = im.tiles[0].comps[0];
+        info->reversible = tc.reversible;
+        info->nlevels = tc.nlev;
+        const Band& b = tc.res[tc.nlev].band[0];
+        info->cbw = 1 << b.xcb;
+        info->cbh = 1 << b.ycb;
+    }
+    for (int c = 0; c < 4 && c < im.ncomp; c++) {
+        info->prec[c] = im.prec[c];
+        info->sgnd[c] = im.sgnd[c];
+    }
+}
+
+uint32_t fnv1a(const uint8_t* p, size_t n, uint32_t h) {
+    for (size_t i = 0; i < n; i++) {
+        h ^= p[i];
+        h *= 16777619u;
+    }
+    return h;
+}
+
+}  // namespace
+
+extern "C" {
+
+tc_status tc_image_probe(const uint8_t* cs, size_t len, tc_image_info* info) {
+    if (!cs || !info) return report(Status::InvalidArgument, "tc_image_probe: null");
+    Image im;
+    std::string err;
+    Status s = probe_codestream(cs, len, im, err);
+    if (s != Status::Ok) return report(s, err);
+    fill_info(im, info);
+    return TC_SUCCESS;
+}
+
+tc_status tc_image_list_cblks(const uint8_t* cs, size_t len, tc_cblk_info* out, int max, int* count) {
+    if (!cs || !count) return report(Status::InvalidArgument, "tc_image_list_cblks: null");
+    Image im;
+    std::string err;
+    Status s = parse_codestream(cs, len, im, err);
+    if (s != Status::Ok) return report(s, err);
+    // crc over each block's segments in order
+    std::vector<uint32_t> crc(im.cblks.size(), 2166136261u);
+    for (const Segment& g : im.segs) crc[g.cblk] = fnv1a(im.cs + g.off, g.len, crc[g.cblk]);
+    *count = (int)im.cblks.size();
+    for (size_t k = 0; out && k < im.cblks.size() && (int)k < max; k++) {
+        const CodeBlock& cb = im.cblks[k];
+        const Band& B = im.tiles[cb.tile].comps[cb.comp].res[cb.res].band[cb.band];
+        tc_cblk_info& o = out[k];
+        o.tile = cb.tile;
+        o.comp = cb.comp;
+        o.res = cb.res;
+        o.band = B.orient;
+        o.x0 = cb.x0;
+        o.y0 = cb.y0;
+        o.x1 = cb.x1;
+        o.y1 = cb.y1;
+        o.npasses = cb.npasses;
+        o.zero_bp = cb.zbp;
+        o.mb = B.mb;
+        o.nbytes = (int32_t)cb.nbytes;
+        o.crc = crc[k];
+    }
+    return TC_SUCCESS;
+}
+
+tc_status tc_decoder_create(tc_device dev, tc_decoder* out) {
+    if (!dev || !out) return report(Status::InvalidArgument, "tc_decoder_create: null");
+    auto d = new tc_decoder_s();
+    d->dev = dev;
+    d->pipe.reset(new Pipeline(dev));
+    if (!d->pipe->stream()) {
+        delete d;
+        return report(Status::Device, std::string("stream creation failed: ") + tc_last_error());
+    }
+    *out = d;
+    return TC_SUCCESS;
+}
+
+tc_status tc_decoder_destroy(tc_decoder d) {
+    delete d;
+    return TC_SUCCESS;
+}
+
+tc_status tc_decoder_set_blocks_per_wave(tc_decoder d, int small, int bpw) {
+    if (!d || (bpw != 0 && bpw != 1 && bpw != 8 && bpw != 16 && bpw != 32 && bpw != 64))
+        return report(Status::InvalidArgument, "blocks per wave must be 0, 1, 8, 16, 32 or 64");
+    Tuning t = d->pipe->tuning();
+    (small ? t.bpw32 : t.bpw64) = bpw;
+    d->pipe->set_tuning(t);
+    return TC_SUCCESS;
+}
+
+tc_status tc_decoder_prepare(tc_decoder d, int n, const uint8_t* const* cs, const size_t* lens) {
+    if (!d || n < 0 || (n > 0 && (!cs || !lens))) return report(Status::InvalidArgument, "tc_decoder_prepare: args");
+    d->pipe->sync();
+    d->data.clear();
+    d->imgs.clear();
+    std::vector<const Image*> v;
+    for (int i = 0; i < n; i++) {
+        if (!cs[i]) return report(Status::InvalidArgument, "tc_decoder_prepare: null codestream");
+        d->data.emplace_back(cs[i], cs[i] + lens[i]);
+        d->imgs.emplace_back(new Image());
+        std::string err;
+        Status s = parse_codestream(d->data.back().data(), d->data.back().size(), *d->imgs.back(), err);
+        if (s != Status::Ok) return report(s, "image " + std::to_string(i) + ": " + err);
+        v.push_back(d->imgs.back().get());
+    }
+    std::string err;
+    return report(d->pipe->prepare(v, err), err);
+}
+
+tc_status tc_decoder_run(tc_decoder d, int timed) {
+    if (!d) return report(Status::InvalidArgument, "tc_decoder_run: null");
+    Status s = d->pipe->run(timed != 0);
+    return report(s, s == Status::Ok ? "" : std::string("launch failed: ") + tc_last_error());
+}
+
+tc_status tc_decoder_sync(tc_decoder d) {
+    if (!d) return report(Status::InvalidArgument, "tc_decoder_sync: null");
+    Status s = d->pipe->sync();
+    return report(s, s == Status::Ok ? "" : std::string("stream sync failed: ") + tc_last_error());
+}
+
+tc_status tc_decoder_fetch(tc_decoder d, int i, void* out, size_t cap) {
+    if (!d || !out) return report(Status::InvalidArgument, "tc_decoder_fetch: null");
+    std::string err;
+    return report(d->pipe->fetch(i, out, cap, err), err);
+}
+
+tc_status tc_decoder_output(tc_decoder d, int i, const void** ptr, size_t* bytes) {
+    if (!d || !ptr || !bytes || i < 0 || i >= d->pipe->num_images())
+        return report(Status::InvalidArgument, "tc_decoder_output: args");
+    *ptr = d->pipe->output_ptr(i);
+    *bytes = d->pipe->output_bytes(i);
+    return TC_SUCCESS;
+}
+
+tc_status tc_decoder_decode(tc_decoder d, const uint8_t* cs, size_t len, void* out, size_t cap, tc_image_info* info) {
+    if (!d || !cs || !out) return report(Status::InvalidArgument, "tc_decoder_decode: null");
+    tc_status s = tc_decoder_prepare(d, 1, &cs, &len);
+    if (s) return s;
+    if (info) fill_info(*d->imgs[0], info);
+    s = tc_decoder_run(d, 0);
+    if (s) return s;
+    return tc_decoder_fetch(d, 0, out, cap);
+}
+
+tc_status tc_decoder_capture_graph(tc_decoder d) {
+    if (!d) return report(Status::InvalidArgument, "tc_decoder_capture_graph: null");
+    std::string err;
+    return report(d->pipe->capture_graph(err), err);
+}
+
+tc_status tc_decoder_launch_graph(tc_decoder d) {
+    if (!d) return report(Status::InvalidArgument, "tc_decoder_launch_graph: null");
+    Status s = d->pipe->launch_graph();
+    return report(s, s == Status::Ok ? "" : std::string("graph launch failed: ") + tc_last_error());
+}
+
+tc_status tc_decoder_get_stats(tc_decoder d, tc_decoder_stats* st) {
+    if (!d || !st) return report(Status::InvalidArgument, "tc_decoder_get_stats: null");
+    std::memset(st, 0, sizeof *st);
+    StageTimes t = d->pipe->times();
+    st->fill_ms = t.fill_ms;
+    st->t1_ms = t.t1_ms;
+    st->dwt_ms = t.dwt_ms;
+    st->final_ms = t.final_ms;
+    st->total_ms = t.total_ms;
+    st->cblks = d->pipe->num_cblks();
+    st->codeword_bytes = d->pipe->codeword_bytes();
+    st->samples = d->pipe->sample_count();
+    st->pixels = d->pipe->pixel_count();
+    st->dwt_bytes = d->pipe->dwt_bytes();
+    st->final_bytes = d->pipe->final_bytes();
+    st->bpw_small = d->pipe->blocks_per_wave(0);
+    st->bpw_large = d->pipe->blocks_per_wave(1);
+    st->images = d->pipe->num_images();
+    st->device_bytes = (int64_t)d->pipe->device_bytes();
+    return TC_SUCCESS;
+}
+
+tc_stream tc_decoder_stream(tc_decoder d) { return d ? d->pipe->stream() : nullptr; }
+
+}  // extern "C"

@@ -1,0 +1,502 @@
+// Device work planning and execution (see pipeline.hpp).
+#include "pipeline.hpp"
+
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <cstring>
+
+#include "../tc_internal.hpp"
+
+namespace tc {
+
+int idwt_tile_w();
+int idwt_tile_h();
+
+namespace {
+
+uint64_t align_up(uint64_t v, uint64_t a) { return (v + a - 1) / a * a; }
+
+template <typename T>
+uint64_t put(std::vector<uint8_t>& blob, const std::vector<T>& v) {
+    uint64_t off = align_up(blob.size(), 256);
+    blob.resize(off + v.size() * sizeof(T));
+    if (!v.empty()) std::memcpy(blob.data() + off, v.data(), v.size() * sizeof(T));
+    return off;
+}
+
+Status from_tc(tc_status s) { return s == TC_SUCCESS ? Status::Ok : (s == TC_ERR_OUT_OF_MEMORY ? Status::OutOfMemory : Status::Device); }
+
+}  // namespace
+
+int pick_blocks_per_wave(int n, bool small, int cus, const Tuning& t) {
+    int fixed = small ? t.bpw32 : t.bpw64;
+    if (fixed) return fixed;
+    int per = n / std::max(cus, 1);
+    int g = 1;
+    for (int c : {8, 16, 32, 64})
+        if (per >= c) g = c;
+    return g;
+}
+
+Pipeline::Pipeline(tc_device dev) : dev_(dev) {
+    if (tc_stream_create(dev, &stream_) != TC_SUCCESS) stream_ = nullptr;
+    for (auto& e : ev_) tc_event_create(&e);
+}
+
+Pipeline::~Pipeline() {
+    if (stream_) tc_stream_sync(stream_);
+    drop_graph();
+    for (auto& e : ev_) tc_event_destroy(e);
+    tc_buf_free(upload_);
+    tc_buf_free(samples_);
+    tc_buf_free(out_);
+    if (pinned_) tc_host_free_pinned(pinned_);
+    tc_stream_destroy(stream_);
+}
+
+Status Pipeline::ensure(tc_buf& b, size_t bytes, const char* what, std::string& err) {
+    if (b && tc_buf_size(b) >= bytes) return Status::Ok;
+    if (b) {
+        tc_stream_sync(stream_);
+        tc_buf_free(b);
+        b = nullptr;
+    }
+    size_t want = std::max<size_t>(bytes + bytes / 8, 1 << 20);
+    tc_status s = tc_buf_alloc(dev_, want, &b);
+    if (s != TC_SUCCESS) {
+        err = std::string("device allocation of ") + what + " failed: " + tc_last_error();
+        b = nullptr;
+        return from_tc(s);
+    }
+    return Status::Ok;
+}
+
+Status Pipeline::ensure_pinned(size_t bytes, std::string& err) {
+    if (pinned_ && pinned_cap_ >= bytes) return Status::Ok;
+    if (pinned_) tc_host_free_pinned(pinned_);
+    pinned_ = nullptr;
+    size_t want = std::max<size_t>(bytes + bytes / 8, 1 << 20);
+    tc_status s = tc_host_alloc_pinned(want, &pinned_);
+    if (s != TC_SUCCESS) {
+        err = std::string("pinned allocation failed: ") + tc_last_error();
+        pinned_ = nullptr;
+        return from_tc(s);
+    }
+    pinned_cap_ = want;
+    return Status::Ok;
+}
+
+Status Pipeline::prepare(const std::vector<const Image*>& imgs, std::string& err) {
+    if (!stream_) {
+        err = "no stream";
+        return Status::Device;
+    }
+    drop_graph();
+    tc_stream_sync(stream_);
+    const int TX = idwt_tile_w(), TY = idwt_tile_h();
+    groups_.clear();
+    levels_.clear();
+    finals_.clear();
+    images_.clear();
+    ncblk_ = cw_bytes_ = nsamples_ = npixels_ = dwt_bytes_ = final_bytes_ = 0;
+
+    // ---- sample arena layout: coefficient planes, then work buffers ----
+    struct TCPos {
+        uint64_t plane = 0, work[2] = {0, 0};
+    };
+    std::vector<std::vector<std::vector<TCPos>>> pos(imgs.size());
+    uint64_t coef = 0;
+    for (size_t i = 0; i < imgs.size(); i++) {
+        const Image& im = *imgs[i];
+        pos[i].resize(im.tiles.size());
+        for (size_t t = 0; t < im.tiles.size(); t++) {
+            pos[i][t].resize(im.ncomp);
+            for (int c = 0; c < im.ncomp; c++) {
+                const TileComp& tc = im.tiles[t].comps[c];
+                pos[i][t][c].plane = coef;
+                coef += align_up((uint64_t)tc.width() * tc.height(), 64);
+            }
+        }
+    }
+    uint64_t work = coef;
+    for (size_t i = 0; i < imgs.size(); i++) {
+        const Image& im = *imgs[i];
+        for (size_t t = 0; t < im.tiles.size(); t++)
+            for (int c = 0; c < im.ncomp; c++) {
+                const TileComp& tc = im.tiles[t].comps[c];
+                uint64_t sz[2] = {0, 0};
+                for (int n = 2; n <= tc.nlev; n++) {
+                    const Resolution& R = tc.res[tc.nlev - n + 1];
+                    uint64_t e = (uint64_t)R.width() * R.height();
+                    sz[(tc.nlev - n) % 2] = std::max(sz[(tc.nlev - n) % 2], e);
+                }
+                for (int k = 0; k < 2; k++) {
+                    pos[i][t][c].work[k] = work;
+                    work += align_up(sz[k], 64);
+                }
+            }
+    }
+    coef_elems_ = coef;
+
+    // ---- codewords and tier-1 descriptors ----
+    std::vector<uint8_t> cw;
+    std::vector<tc_cblk_desc> descs;
+    std::vector<uint32_t> cls_idx[2];
+    for (size_t i = 0; i < imgs.size(); i++) {
+        const Image& im = *imgs[i];
+        // segments grouped per code-block, in codestream order
+        std::vector<uint32_t> seg_start(im.cblks.size() + 1, 0), seg_list(im.segs.size());
+        for (const Segment& s : im.segs) seg_start[s.cblk + 1]++;
+        for (size_t k = 0; k < im.cblks.size(); k++) seg_start[k + 1] += seg_start[k];
+        {
+            std::vector<uint32_t> fillp(seg_start.begin(), seg_start.end() - 1);
+            for (uint32_t s = 0; s < im.segs.size(); s++) seg_list[fillp[im.segs[s].cblk]++] = s;
+        }
+        for (size_t k = 0; k < im.cblks.size(); k++) {
+            const CodeBlock& cb = im.cblks[k];
+            int w = cb.x1 - cb.x0, h = cb.y1 - cb.y0;
+            if (!cb.included || cb.numbps <= 0 || cb.npasses <= 0 || w <= 0 || h <= 0) continue;
+            if (w > 64 || h > 64) {
+                err = "code-blocks larger than 64x64 are not supported";
+                return Status::Unsupported;
+            }
+            const TileComp& tc = im.tiles[cb.tile].comps[cb.comp];
+            const Band& B = tc.res[cb.res].band[cb.band];
+            int ox = 0, oy = 0;
+            if (cb.res > 0) {
+                if (B.orient == 1 || B.orient == 3) ox = tc.res[cb.res - 1].width();
+                if (B.orient == 2 || B.orient == 3) oy = tc.res[cb.res - 1].height();
+            }
+            tc_cblk_desc d;
+            std::memset(&d, 0, sizeof d);
+            d.data_off = (uint32_t)cw.size();
+            d.data_len = cb.nbytes;
+            for (uint32_t s = seg_start[k]; s < seg_start[k + 1]; s++) {
+                const Segment& sg = im.segs[seg_list[s]];
+                cw.insert(cw.end(), im.cs + sg.off, im.cs + sg.off + sg.len);
+            }
+            d.out_off = pos[i][cb.tile][cb.comp].plane + (uint64_t)(oy + cb.y0 - B.y0) * tc.width() +
+                        (uint64_t)(ox + cb.x0 - B.x0);
+            d.out_pitch = (uint32_t)tc.width();
+            d.w = (uint16_t)w;
+            d.h = (uint16_t)h;
+            d.numbps = (uint8_t)cb.numbps;
+            d.npasses = (uint8_t)std::min(cb.npasses, 255);
+            d.cls = (uint8_t)(B.orient == 1 ? 1 : (B.orient == 3 ? 2 : 0));
+            d.mode = tc.reversible ? 0 : 1;
+            d.delta = B.delta;
+            cls_idx[(w <= 32 && h <= 32) ? 0 : 1].push_back((uint32_t)descs.size());
+            descs.push_back(d);
+        }
+    }
+    cw.resize(cw.size() + 16, 0xFF);  // slack for look-ahead reads
+    cw_bytes_ = (int64_t)cw.size() - 16;
+    ncblk_ = (int64_t)descs.size();
+
+    std::vector<uint8_t> blob;
+    uint64_t cw_off = put(blob, cw);
+    descs_off_ = put(blob, descs);
+    int cus = device_cu_count(dev_);
+    for (int k = 0; k < 2; k++) {
+        auto& v = cls_idx[k];
+        bpw_[k] = 0;
+        if (v.empty()) continue;
+        // packing: blocks with similar pass counts and shapes share a wave
+        std::stable_sort(v.begin(), v.end(), [&](uint32_t a, uint32_t b) {
+            const tc_cblk_desc &x = descs[a], &y = descs[b];
+            if (x.npasses != y.npasses) return x.npasses > y.npasses;
+            if (x.h != y.h) return x.h > y.h;
+            return x.w > y.w;
+        });
+        Group g;
+        g.small = k == 0;
+        g.bpw = pick_blocks_per_wave((int)v.size(), g.small, cus, tuning_);
+        g.n = (int)v.size();
+        g.order_off = put(blob, v);
+        bpw_[k] = g.bpw;
+        groups_.push_back(g);
+    }
+
+    // ---- inverse DWT levels ----
+    int maxlev = 0;
+    for (auto* im : imgs)
+        for (auto& t : im->tiles)
+            for (auto& c : t.comps) maxlev = std::max(maxlev, c.nlev);
+    for (int s = 0; s + 2 <= maxlev; s++) {
+        std::vector<tc_dwt_job> jobs[2];
+        std::vector<uint32_t> maps[2];
+        for (size_t i = 0; i < imgs.size(); i++) {
+            const Image& im = *imgs[i];
+            for (size_t t = 0; t < im.tiles.size(); t++)
+                for (int c = 0; c < im.ncomp; c++) {
+                    const TileComp& tc = im.tiles[t].comps[c];
+                    int n = tc.nlev - s;
+                    if (n < 2) continue;
+                    int r = tc.nlev - n + 1;
+                    const Resolution &R = tc.res[r], &Rl = tc.res[r - 1];
+                    if (R.width() <= 0 || R.height() <= 0) continue;
+                    const TCPos& P = pos[i][t][c];
+                    int lw = Rl.width(), lh = Rl.height();
+                    tc_dwt_job j;
+                    std::memset(&j, 0, sizeof j);
+                    if (n == tc.nlev) {
+                        j.ll_off = P.plane;
+                        j.ll_pitch = (uint32_t)tc.width();
+                    } else {
+                        j.ll_off = P.work[(tc.nlev - n - 1) % 2];
+                        j.ll_pitch = (uint32_t)lw;
+                    }
+                    j.hl_off = P.plane + lw;
+                    j.lh_off = P.plane + (uint64_t)lh * tc.width();
+                    j.hh_off = P.plane + (uint64_t)lh * tc.width() + lw;
+                    j.band_pitch = (uint32_t)tc.width();
+                    j.out_off = P.work[(tc.nlev - n) % 2];
+                    j.out_pitch = (uint32_t)R.width();
+                    j.u0 = R.x0;
+                    j.u1 = R.x1;
+                    j.v0 = R.y0;
+                    j.v1 = R.y1;
+                    int irr = tc.reversible ? 0 : 1;
+                    j.tiles_x = (R.width() + TX - 1) / TX;
+                    int tiles_y = (R.height() + TY - 1) / TY;
+                    j.wg_base = (int32_t)maps[irr].size();
+                    uint32_t jid = (uint32_t)jobs[irr].size();
+                    for (int q = 0; q < j.tiles_x * tiles_y; q++) maps[irr].push_back(jid);
+                    jobs[irr].push_back(j);
+                    dwt_bytes_ += (int64_t)R.width() * R.height() * 8;
+                }
+        }
+        for (int irr = 0; irr < 2; irr++) {
+            if (jobs[irr].empty()) continue;
+            LevelLaunch L;
+            L.irr = irr != 0;
+            L.jobs_off = put(blob, jobs[irr]);
+            L.map_off = put(blob, maps[irr]);
+            L.nwg = (int)maps[irr].size();
+            levels_.push_back(L);
+        }
+    }
+
+    // ---- final level + colour + pack, and output placement ----
+    uint64_t out_bytes_total = 0;
+    struct FKey {
+        int irr, nc, ob;
+    };
+    std::vector<tc_final_job> fjobs[8];
+    std::vector<uint32_t> fmaps[8];
+    auto key_index = [](int irr, int nc, int ob) { return irr * 4 + (nc == 3 ? 2 : 0) + (ob == 2 ? 1 : 0); };
+    for (size_t i = 0; i < imgs.size(); i++) {
+        const Image& im = *imgs[i];
+        ImageOut io;
+        io.bps = im.max_prec() > 8 ? 2 : 1;
+        io.off = align_up(out_bytes_total, 256);
+        io.bytes = (uint64_t)im.width() * im.height() * im.ncomp * io.bps;
+        out_bytes_total = io.off + io.bytes;
+        images_.push_back(io);
+        npixels_ += (int64_t)im.width() * im.height();
+        nsamples_ += (int64_t)im.width() * im.height() * im.ncomp;
+        uint64_t out_elem_base = io.off / io.bps;
+        for (size_t t = 0; t < im.tiles.size(); t++) {
+            const Tile& T = im.tiles[t];
+            std::vector<std::vector<int>> sets;
+            bool mct3 = T.mct && im.ncomp >= 3 && T.comps[0].reversible == T.comps[1].reversible &&
+                        T.comps[0].reversible == T.comps[2].reversible && T.comps[0].nlev == T.comps[1].nlev &&
+                        T.comps[0].nlev == T.comps[2].nlev;
+            int c0 = 0;
+            if (mct3) {
+                sets.push_back({0, 1, 2});
+                c0 = 3;
+            }
+            for (int c = c0; c < im.ncomp; c++) sets.push_back({c});
+            for (auto& set : sets) {
+                const TileComp& tc0 = T.comps[set[0]];
+                const Resolution& R = tc0.res[tc0.nlev];
+                if (R.width() <= 0 || R.height() <= 0) continue;
+                tc_final_job f;
+                std::memset(&f, 0, sizeof f);
+                f.u0 = R.x0;
+                f.u1 = R.x1;
+                f.v0 = R.y0;
+                f.v1 = R.y1;
+                f.levels = tc0.nlev;
+                f.mct = set.size() == 3 ? 1 : 0;
+                for (size_t k = 0; k < set.size(); k++) {
+                    int c = set[k];
+                    const TileComp& tc = T.comps[c];
+                    const TCPos& P = pos[i][t][c];
+                    if (tc.nlev == 0) {
+                        f.ll_off[k] = f.hl_off[k] = f.lh_off[k] = f.hh_off[k] = P.plane;
+                        f.ll_pitch[k] = f.band_pitch[k] = (uint32_t)tc.width();
+                    } else {
+                        const Resolution& Rl = tc.res[tc.nlev - 1];
+                        int lw = Rl.width(), lh = Rl.height();
+                        if (tc.nlev == 1) {
+                            f.ll_off[k] = P.plane;
+                            f.ll_pitch[k] = (uint32_t)tc.width();
+                        } else {
+                            f.ll_off[k] = P.work[(tc.nlev - 2) % 2];
+                            f.ll_pitch[k] = (uint32_t)lw;
+                        }
+                        f.hl_off[k] = P.plane + lw;
+                        f.lh_off[k] = P.plane + (uint64_t)lh * tc.width();
+                        f.hh_off[k] = P.plane + (uint64_t)lh * tc.width() + lw;
+                        f.band_pitch[k] = (uint32_t)tc.width();
+                    }
+                    int prec = im.prec[c];
+                    if (im.sgnd[c]) {
+                        f.shift[k] = 0;
+                        f.minv[k] = -(1 << (prec - 1));
+                        f.maxv[k] = (1 << (prec - 1)) - 1;
+                    } else {
+                        f.shift[k] = 1 << (prec - 1);
+                        f.minv[k] = 0;
+                        f.maxv[k] = (1 << prec) - 1;
+                    }
+                }
+                f.out_off = out_elem_base + ((uint64_t)(R.y0 - im.y0) * im.width() + (uint64_t)(R.x0 - im.x0)) * im.ncomp +
+                            (uint64_t)set[0];
+                f.out_pitch = (uint32_t)((uint64_t)im.width() * im.ncomp);
+                f.out_cstride = (uint32_t)im.ncomp;
+                int irr = tc0.reversible ? 0 : 1, nc = (int)set.size();
+                f.tiles_x = (R.width() + TX - 1) / TX;
+                int tiles_y = (R.height() + TY - 1) / TY;
+                int key = key_index(irr, nc, io.bps);
+                f.wg_base = (int32_t)fmaps[key].size();
+                uint32_t jid = (uint32_t)fjobs[key].size();
+                for (int q = 0; q < f.tiles_x * tiles_y; q++) fmaps[key].push_back(jid);
+                fjobs[key].push_back(f);
+                final_bytes_ += (int64_t)R.width() * R.height() * nc * (4 + io.bps);
+            }
+        }
+    }
+    for (int irr = 0; irr < 2; irr++)
+        for (int nc : {1, 3})
+            for (int ob : {1, 2}) {
+                int key = key_index(irr, nc, ob);
+                if (fjobs[key].empty()) continue;
+                FinalLaunch F;
+                F.irr = irr != 0;
+                F.nc = nc;
+                F.ob = ob;
+                F.jobs_off = put(blob, fjobs[key]);
+                F.map_off = put(blob, fmaps[key]);
+                F.nwg = (int)fmaps[key].size();
+                finals_.push_back(F);
+            }
+
+    // ---- device buffers and upload ----
+    Status st = ensure(upload_, blob.size(), "descriptors", err);
+    if (st == Status::Ok) st = ensure(samples_, std::max<uint64_t>(work, 1) * 4, "sample planes", err);
+    if (st == Status::Ok) st = ensure(out_, std::max<uint64_t>(out_bytes_total, 1), "output images", err);
+    if (st == Status::Ok) st = ensure_pinned(blob.size(), err);
+    if (st != Status::Ok) return st;
+    std::memcpy(pinned_, blob.data(), blob.size());
+    tc_status ts = tc_buf_copy_h2d(stream_, upload_, 0, pinned_, blob.size());
+    if (ts != TC_SUCCESS) {
+        err = tc_last_error();
+        return from_tc(ts);
+    }
+    (void)cw_off;
+    return Status::Ok;
+}
+
+Status Pipeline::run(bool timed) {
+    uint8_t* up = (uint8_t*)tc_buf_device_ptr(upload_);
+    void* base = tc_buf_device_ptr(samples_);
+    void* out = tc_buf_device_ptr(out_);
+    if (!up || !base || !out) return Status::InvalidArgument;
+    timed_ = timed;
+    tc_status s = TC_SUCCESS;
+    if (timed) tc_event_record(ev_[0], stream_);
+    s = tc_launch_fill32(stream_, base, 0u, coef_elems_);
+    if (timed) tc_event_record(ev_[1], stream_);
+    for (const Group& g : groups_) {
+        if (s != TC_SUCCESS) break;
+        s = tc_launch_t1_decode(stream_, g.small ? 1 : 0, g.bpw, up + descs_off_, (const uint32_t*)(up + g.order_off),
+                                g.n, up, base);
+    }
+    if (timed) tc_event_record(ev_[2], stream_);
+    for (const LevelLaunch& L : levels_) {
+        if (s != TC_SUCCESS) break;
+        s = tc_launch_idwt_level(stream_, L.irr ? 1 : 0, up + L.jobs_off, (const uint32_t*)(up + L.map_off), L.nwg, base);
+    }
+    if (timed) tc_event_record(ev_[3], stream_);
+    for (const FinalLaunch& F : finals_) {
+        if (s != TC_SUCCESS) break;
+        s = tc_launch_idwt_final(stream_, F.irr ? 1 : 0, F.nc, F.ob, up + F.jobs_off, (const uint32_t*)(up + F.map_off),
+                                 F.nwg, base, out);
+    }
+    if (timed) tc_event_record(ev_[4], stream_);
+    return s == TC_SUCCESS ? Status::Ok : Status::Device;
+}
+
+Status Pipeline::sync() { return tc_stream_sync(stream_) == TC_SUCCESS ? Status::Ok : Status::Device; }
+
+StageTimes Pipeline::times() const {
+    StageTimes t;
+    if (!timed_) return t;
+    tc_event_elapsed_ms(ev_[0], ev_[1], &t.fill_ms);
+    tc_event_elapsed_ms(ev_[1], ev_[2], &t.t1_ms);
+    tc_event_elapsed_ms(ev_[2], ev_[3], &t.dwt_ms);
+    tc_event_elapsed_ms(ev_[3], ev_[4], &t.final_ms);
+    tc_event_elapsed_ms(ev_[0], ev_[4], &t.total_ms);
+    return t;
+}
+
+Status Pipeline::fetch(int i, void* dst, size_t cap, std::string& err) {
+    if (i < 0 || i >= (int)images_.size()) {
+        err = "image index out of range";
+        return Status::InvalidArgument;
+    }
+    const ImageOut& o = images_[i];
+    if (cap < o.bytes) {
+        err = "output buffer too small";
+        return Status::Capacity;
+    }
+    tc_status s = tc_buf_copy_d2h(stream_, dst, out_, o.off, o.bytes);
+    if (s == TC_SUCCESS) s = tc_stream_sync(stream_);
+    if (s != TC_SUCCESS) {
+        err = tc_last_error();
+        return Status::Device;
+    }
+    return Status::Ok;
+}
+
+const void* Pipeline::output_ptr(int i) const {
+    return (const uint8_t*)tc_buf_device_ptr(out_) + images_[i].off;
+}
+
+size_t Pipeline::output_bytes(int i) const { return images_[i].bytes; }
+
+Status Pipeline::capture_graph(std::string& err) {
+    drop_graph();
+    if (tc_graph_begin(stream_) != TC_SUCCESS) {
+        err = tc_last_error();
+        return Status::Device;
+    }
+    Status s = run(false);
+    tc_graph g = nullptr;
+    tc_status e = tc_graph_end(stream_, &g);
+    if (s != Status::Ok || e != TC_SUCCESS) {
+        err = tc_last_error();
+        if (g) tc_graph_destroy(g);
+        return Status::Device;
+    }
+    graph_ = g;
+    return Status::Ok;
+}
+
+Status Pipeline::launch_graph() {
+    if (!graph_) return Status::InvalidArgument;
+    return tc_graph_launch(graph_, stream_) == TC_SUCCESS ? Status::Ok : Status::Device;
+}
+
+void Pipeline::drop_graph() {
+    if (graph_) tc_graph_destroy(graph_);
+    graph_ = nullptr;
+}
+
+size_t Pipeline::device_bytes() const { return tc_buf_size(upload_) + tc_buf_size(samples_) + tc_buf_size(out_); }
+
+}  // namespace tc

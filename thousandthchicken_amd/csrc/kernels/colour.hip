@@ -1,0 +1,51 @@
+// Standalone inverse colour transforms and DC level shift over planes in
+// place, for callers that run the reference's Preprocessor stage on its own
+// (TC/Preprocessor.cpp:44-157; kernels TC/preprocess_rct_inverse.cl:18-36,
+// TC/preprocess_ict_inverse.cl:22-41 with T.800 G.3 constants,
+// TC/preprocess_dc_level_shift_inverse.cl:16-19).  The decode pipeline itself
+// fuses this work into the final inverse-DWT level.
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+namespace tc {
+namespace {
+
+__device__ __forceinline__ int clampi(int v, int lo, int hi) { return v < lo ? lo : (v > hi ? hi : v); }
+
+__global__ void colour_kernel(int kind, int32_t* __restrict__ c0, int32_t* __restrict__ c1,
+                              int32_t* __restrict__ c2, size_t n, int shift, int lo, int hi) {
+    size_t stride = (size_t)gridDim.x * blockDim.x;
+    for (size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += stride) {
+        if (kind == 0) {
+            c0[i] = clampi(c0[i] + shift, lo, hi);
+        } else if (kind == 1) {
+            int y0 = c0[i], y1 = c1[i], y2 = c2[i];
+            int g = y0 - ((y1 + y2) >> 2);
+            c0[i] = clampi(y2 + g + shift, lo, hi);
+            c1[i] = clampi(g + shift, lo, hi);
+            c2[i] = clampi(y1 + g + shift, lo, hi);
+        } else {
+            float y0 = __int_as_float(c0[i]), y1 = __int_as_float(c1[i]), y2 = __int_as_float(c2[i]);
+            float r = fmaf(1.402f, y2, y0);
+            float g = fmaf(-0.71414f, y2, fmaf(-0.34413f, y1, y0));
+            float b = fmaf(1.772f, y1, y0);
+            c0[i] = clampi((int)rintf(r) + shift, lo, hi);
+            c1[i] = clampi((int)rintf(g) + shift, lo, hi);
+            c2[i] = clampi((int)rintf(b) + shift, lo, hi);
+        }
+    }
+}
+
+}  // namespace
+
+hipError_t launch_colour(hipStream_t s, int kind, void* c0, void* c1, void* c2, size_t n, int shift, int lo,
+                         int hi) {
+    if (!n) return hipSuccess;
+    size_t blocks = (n + 255) / 256;
+    if (blocks > 8192) blocks = 8192;
+    hipLaunchKernelGGL(colour_kernel, dim3((unsigned)blocks), dim3(256), 0, s, kind, (int32_t*)c0, (int32_t*)c1,
+                       (int32_t*)c2, n, shift, lo, hi);
+    return hipGetLastError();
+}
+
+}  // namespace tc

@@ -1,0 +1,59 @@
+// Device-side work descriptors shared by the HIP kernels and the host planner.
+// Plain structs, no HIP types: the host fills arrays of them and uploads them.
+#pragma once
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+// One code-block for tier-1 decoding (the reference's per-block descriptor is
+// CodeBlockAdditionalInfo, TC/coefficientcoder_common.h:5-24; here with 16-bit
+// dimensions, the pass count, the codeword location and the dequantization
+// step folded in).  32 bytes.
+typedef struct tc_cblk_desc {
+    uint32_t data_off;   // byte offset of the codeword in the codeword arena
+    uint32_t data_len;   // codeword bytes
+    uint64_t out_off;    // element offset of coefficient (0,0) of the block
+    uint32_t out_pitch;  // elements per row of the coefficient plane
+    uint16_t w, h;       // block size (<= 64 x 64)
+    uint8_t numbps;      // bit-planes to decode: M_b minus missing MSBs
+    uint8_t npasses;     // coding passes present (<= 3*numbps-2)
+    uint8_t cls;         // context class: 0 = LL/LH, 1 = HL, 2 = HH
+    uint8_t mode;        // 0 = reversible (int32 out), 1 = irreversible (float out)
+    float delta;         // quantization step Delta_b (irreversible)
+} tc_cblk_desc;
+
+// One level of a 2-D inverse DWT over one tile-component: rebuilds the
+// resolution rectangle [u0,u1) x [v0,v1) from its LL (previous level's output
+// or the coefficient plane) and the HL/LH/HH quadrants of the coefficient plane.
+// Offsets are in elements from the kernel's base pointer.
+typedef struct tc_dwt_job {
+    uint64_t ll_off, hl_off, lh_off, hh_off, out_off;
+    uint32_t ll_pitch, band_pitch, out_pitch;
+    int32_t u0, u1, v0, v1;
+    int32_t tiles_x;     // work-group tiles across
+    int32_t wg_base;     // first work-group index of this job in the launch
+    int32_t pad;
+} tc_dwt_job;
+
+// Final level fused with the inverse component transform, DC level shift,
+// rounding, clamping and packing into the interleaved output image.
+// Components 0..nc-1 of the job share the rectangle.  levels == 0 means the
+// tile-component has no DWT (the LL band is the image).
+typedef struct tc_final_job {
+    uint64_t ll_off[3], hl_off[3], lh_off[3], hh_off[3];
+    uint32_t ll_pitch[3], band_pitch[3];
+    int32_t shift[3], minv[3], maxv[3];
+    int32_t u0, u1, v0, v1;
+    uint64_t out_off;       // output element index of (u0,v0), first component
+    uint32_t out_pitch;     // output elements per image row
+    uint32_t out_cstride;   // output elements between adjacent pixels
+    int32_t tiles_x, wg_base;
+    int32_t levels;         // 0: copy through
+    int32_t mct;            // 1: apply RCT (5/3) / ICT (9/7) to components 0..2
+} tc_final_job;
+
+#ifdef __cplusplus
+}
+#endif

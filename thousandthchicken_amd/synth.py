@@ -1,0 +1,117 @@
+"""Seeded synthetic images for the benchmark workloads and tests.
+
+Recipe (per component, every value drawn from a SplitMix64-seeded PCG32):
+  * 8 sinusoids a*sin(2*pi*(fx*x + fy*y) + phi), a ~ U[10,40],
+    fx ~ U[0.5,16]/width, fy ~ U[0.5,16]/height, phi ~ U[0, 2*pi);
+  * Gaussian noise with standard deviation 12 (the "sigma 12"), blurred with a
+    Gaussian of sigma 1 pixel (7-tap separable kernel, edges clamped);
+  * 20 axis-aligned sharp-edged rectangles, each side ~ U[4%, 40%] of the
+    image side, that pull the area halfway towards a flat value ~ U[0,255].
+Samples sum around 128, are rounded and clipped to [0,255], then scaled by
+(2^B - 1)/255 and rounded for B-bit images.
+"""
+import math
+
+import numpy as np
+
+_M64 = np.uint64(0xFFFFFFFFFFFFFFFF)
+
+
+def _splitmix64(x):
+    with np.errstate(over="ignore"):
+        x = np.asarray(x, dtype=np.uint64) + np.uint64(0x9E3779B97F4A7C15)
+        z = x
+        z = (z ^ (z >> np.uint64(30))) * np.uint64(0xBF58476D1CE4E5B9)
+        z = (z ^ (z >> np.uint64(27))) * np.uint64(0x94D049BB133111EB)
+        return z ^ (z >> np.uint64(31))
+
+
+class PCG32:
+    """Vector of independent PCG32 (XSH-RR) streams."""
+
+    MUL = np.uint64(6364136223846793005)
+
+    def __init__(self, seeds, streams):
+        seeds = np.atleast_1d(np.asarray(seeds, dtype=np.uint64))
+        streams = np.atleast_1d(np.asarray(streams, dtype=np.uint64))
+        with np.errstate(over="ignore"):
+            self.inc = (streams << np.uint64(1)) | np.uint64(1)
+            self.state = np.zeros_like(seeds)
+            self.next_u32()
+            self.state = self.state + seeds
+            self.next_u32()
+
+    def next_u32(self):
+        with np.errstate(over="ignore"):
+            old = self.state
+            self.state = old * self.MUL + self.inc
+            xs = (((old >> np.uint64(18)) ^ old) >> np.uint64(27)) & np.uint64(0xFFFFFFFF)
+            rot = old >> np.uint64(59)
+            r = ((xs >> rot) | (xs << ((np.uint64(32) - rot) & np.uint64(31)))) & np.uint64(0xFFFFFFFF)
+        return r
+
+    def uniform(self):
+        """U[0,1) with 32-bit resolution, one per stream."""
+        return self.next_u32().astype(np.float64) / 4294967296.0
+
+
+def _scalar_stream(seed, tag):
+    s = _splitmix64(np.uint64(seed) ^ _splitmix64(np.uint64(tag)))
+    return PCG32(s, np.uint64(tag))
+
+
+def _gauss_kernel(sigma=1.0, radius=3):
+    x = np.arange(-radius, radius + 1, dtype=np.float64)
+    k = np.exp(-0.5 * (x / sigma) ** 2)
+    return k / k.sum()
+
+
+def _blur(a, k):
+    r = len(k) // 2
+    p = np.pad(a, ((0, 0), (r, r)), mode="edge")
+    a = sum(k[i] * p[:, i:i + a.shape[1]] for i in range(len(k)))
+    p = np.pad(a, ((r, r), (0, 0)), mode="edge")
+    return sum(k[i] * p[i:i + a.shape[0], :] for i in range(len(k)))
+
+
+def component(height, width, seed, comp):
+    """One float64 component of the recipe, before rounding and clipping."""
+    g = _scalar_stream(seed, 1000 + comp)
+    y = np.arange(height, dtype=np.float64)[:, None]
+    x = np.arange(width, dtype=np.float64)[None, :]
+    img = np.full((height, width), 128.0)
+    for _ in range(8):
+        a = 10 + 30 * g.uniform()[0]
+        fx = (0.5 + 15.5 * g.uniform()[0]) / width
+        fy = (0.5 + 15.5 * g.uniform()[0]) / height
+        ph = 2 * math.pi * g.uniform()[0]
+        img += a * np.sin(2 * math.pi * (fx * x + fy * y) + ph)
+    for _ in range(20):
+        rw = max(1, int(round(width * (0.04 + 0.36 * g.uniform()[0]))))
+        rh = max(1, int(round(height * (0.04 + 0.36 * g.uniform()[0]))))
+        rx = int(g.uniform()[0] * max(1, width - rw + 1))
+        ry = int(g.uniform()[0] * max(1, height - rh + 1))
+        val = 255 * g.uniform()[0]
+        img[ry:ry + rh, rx:rx + rw] = 0.5 * img[ry:ry + rh, rx:rx + rw] + 0.5 * val
+    # per-row noise streams, Box-Muller
+    rows = PCG32(_splitmix64(np.arange(height, dtype=np.uint64) + np.uint64(seed * 7919 + comp * 104729)),
+                 np.arange(height, dtype=np.uint64) + np.uint64(comp))
+    noise = np.empty((height, width))
+    for j in range(0, width, 2):
+        u1 = (rows.next_u32().astype(np.float64) + 1.0) / 4294967297.0
+        u2 = rows.uniform()
+        r = np.sqrt(-2.0 * np.log(u1))
+        noise[:, j] = r * np.cos(2 * math.pi * u2)
+        if j + 1 < width:
+            noise[:, j + 1] = r * np.sin(2 * math.pi * u2)
+    img += _blur(12.0 * noise, _gauss_kernel(1.0))
+    return img
+
+
+def image(height, width, ncomp, seed, bits=8):
+    """Seeded image [height, width, ncomp] (uint8 for 8-bit, else uint16)."""
+    planes = [np.clip(np.rint(component(height, width, seed, c)), 0, 255) for c in range(ncomp)]
+    a = np.stack(planes, axis=-1)
+    if bits == 8:
+        return a.astype(np.uint8)
+    return np.rint(a * ((1 << bits) - 1) / 255.0).astype(np.uint16)
