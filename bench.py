@@ -65,6 +65,8 @@ def main():
     ap.add_argument("--warmup", type=int, default=5)
     ap.add_argument("--workload", default="cfg2_lossy_2048", choices=sorted(WORKLOADS))
     ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--bpw-large", type=int, default=0, help="blocks per wave, blocks up to 64x64 (0 auto, 1 wave/block)")
+    ap.add_argument("--bpw-small", type=int, default=0, help="blocks per wave, blocks up to 32x32 (0 auto, 1 wave/block)")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py measures one GPU: this build has no multi-GPU path (asked for {args.gpus})",
@@ -77,6 +79,8 @@ def main():
     info = tca.probe(cs)
     pixels = info.width * info.height
     dec = tca.Decoder()
+    dec.set_blocks_per_wave(False, args.bpw_large)
+    dec.set_blocks_per_wave(True, args.bpw_small)
     dec.prepare([cs])
     for _ in range(args.warmup):
         dec.run()
@@ -86,6 +90,12 @@ def main():
     h, w, c, bits, seed, lossless, lev, cb, bpp = WORKLOADS[args.workload]
     if lossless and not np.array_equal(out.reshape(src.shape), src):
         raise SystemExit("benchmark decode is not bit-exact")
+    if not lossless:
+        from oracle import oracle
+        ref = oracle.decode(cs)
+        dmax = int(np.abs(out.transpose(2, 0, 1).astype(np.int32) - ref).max())
+        if dmax > 1:
+            raise SystemExit(f"benchmark decode differs from the oracle by {dmax} LSB")
 
     dec.sync()
     t0 = time.perf_counter()

@@ -111,9 +111,17 @@ tc_status tc_graph_destroy(tc_graph g);
  * descs: device array of tc_cblk_desc (thousandthchicken_amd/csrc/tc_kernels.h),
  * order: device array of n indices into descs (packing order) or NULL,
  * arena: codewords, coef: zero-filled coefficient planes (int32 or float).
- * small_blocks: every block fits 32x32; blocks_per_wave: 1, 8, 16, 32 or 64. */
+ * small_blocks: every block fits 32x32; blocks_per_wave: 8, 16, 32 or 64 (one lane per block). */
 tc_status tc_launch_t1_decode(tc_stream s, int small_blocks, int blocks_per_wave, const void *descs,
                               const uint32_t *order, int n, const uint8_t *arena, void *coef);
+
+/* Same work with one wavefront per code-block (latency-bound launches with few
+ * blocks): grid work-groups loop over the n blocks in order.  Codewords must
+ * start at 4-byte aligned offsets; arena_bytes bounds every codeword read.
+ * small_blocks: every block is at most 32 wide.  Writes every coefficient of each
+ * decoded block (no zero-fill needed for them). */
+tc_status tc_launch_t1_decode_wave(tc_stream s, int small_blocks, int grid, const void *descs, const uint32_t *order,
+                                   int n, const uint8_t *arena, size_t arena_bytes, void *coef);
 
 /* One inverse DWT level over a job list (run, TC/dwt_r53.cl:183 / TC/dwt_r97.cl:229).
  * jobs: device array of tc_dwt_job; wg_job: work-group -> job map of nwg entries. */

@@ -42,6 +42,7 @@ def lib():
         P = ctypes.POINTER
         L.or_read_info.argtypes = [ctypes.c_char_p, ctypes.c_size_t, P(Info)]
         L.or_list_cblks.argtypes = [ctypes.c_char_p, ctypes.c_size_t, P(Cblk), ctypes.c_int]
+        L.or_t1_symbol_counts.argtypes = [ctypes.c_char_p, ctypes.c_size_t, ctypes.c_void_p, ctypes.c_int]
         L.or_decode.argtypes = [ctypes.c_char_p, ctypes.c_size_t, ctypes.c_void_p, ctypes.c_void_p]
         L.or_t1_decode.argtypes = [ctypes.c_char_p, ctypes.c_int, ctypes.c_int, ctypes.c_int,
                                    ctypes.c_int, ctypes.c_int, ctypes.c_int,
@@ -89,6 +90,15 @@ def list_cblks(cs: bytes):
     n2 = lib().or_list_cblks(cs, len(cs), arr, n)
     assert n2 == n
     return [arr[k] for k in range(n)]
+
+
+def symbol_counts(cs: bytes):
+    n = lib().or_list_cblks(cs, len(cs), None, 0)
+    out = np.zeros(max(n, 1), np.int32)
+    k = lib().or_t1_symbol_counts(cs, len(cs), out.ctypes.data, n)
+    if k < 0:
+        raise OracleError(f"or_t1_symbol_counts failed ({k})")
+    return out[:n]
 
 
 def t1_decode(data: bytes, w, h, orient, numbps, npasses):

@@ -1347,6 +1347,7 @@ static int walk_tiles(const uint8_t *cs0, size_t len0, tile_fn fn, void *user) {
 typedef struct {
     or_cblk *out;
     int max, n;
+    int32_t *nsym; /* when set: MQ symbols decoded per code-block */
 } list_ctx;
 
 static int list_tile(const hdr_t *h, int t, tc_t *tcs, void *user) {
@@ -1357,7 +1358,19 @@ static int list_tile(const hdr_t *h, int t, tc_t *tcs, void *user) {
                 band_t *B = &tcs[c].r[r].b[k];
                 for (int i = 0; i < B->ncx * B->ncy; i++) {
                     cb_t *C = &B->cb[i];
-                    if (L->n < L->max) {
+                    if (L->nsym && L->n < L->max) {
+                        int w = C->x1 - C->x0, hh = C->y1 - C->y0, k = 0;
+                        if (w > 0 && hh > 0 && C->npasses > 0) {
+                            int32_t *mag = (int32_t *)malloc(sizeof(int32_t) * (size_t)w * hh);
+                            int8_t *sg = (int8_t *)malloc((size_t)w * hh), *lp = (int8_t *)malloc((size_t)w * hh);
+                            if (mag && sg && lp)
+                                k = or_t1_decode(C->data, C->len, w, hh, B->orient, B->mb - C->zbp, C->npasses, mag, sg, lp);
+                            free(mag);
+                            free(sg);
+                            free(lp);
+                        }
+                        L->nsym[L->n] = k;
+                    } else if (L->out && L->n < L->max) {
                         or_cblk *o = &L->out[L->n];
                         o->tile = t;
                         o->comp = c;
@@ -1380,7 +1393,13 @@ static int list_tile(const hdr_t *h, int t, tc_t *tcs, void *user) {
 }
 
 int or_list_cblks(const uint8_t *cs, size_t len, or_cblk *out, int max) {
-    list_ctx L = {out, max, 0};
+    list_ctx L = {out, max, 0, NULL};
+    int rc = walk_tiles(cs, len, list_tile, &L);
+    return rc ? -rc : L.n;
+}
+
+int or_t1_symbol_counts(const uint8_t *cs, size_t len, int32_t *nsym, int max) {
+    list_ctx L = {NULL, max, 0, nsym};
     int rc = walk_tiles(cs, len, list_tile, &L);
     return rc ? -rc : L.n;
 }

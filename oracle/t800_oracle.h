@@ -71,6 +71,9 @@ int or_read_info(const uint8_t *cs, size_t len, or_info *info);
  * resolution, band, raster.  Returns the count (or -error); writes at most max. */
 int or_list_cblks(const uint8_t *cs, size_t len, or_cblk *out, int max);
 
+/* MQ symbols decoded per code-block (tier-2 order); returns the count. */
+int or_t1_symbol_counts(const uint8_t *cs, size_t len, int32_t *nsym, int max);
+
 /* Full decode.  out: planar int32 [ncomp][y1-y0][x1-x0] after colour transform,
  * DC shift, rounding and clamping.  out_f (optional): the same values before
  * rounding/clamping (FP64 for the 9/7 path, exact integers for 5/3). */

@@ -17,6 +17,7 @@ OBJ = os.path.join(HERE, "_objs")
 SOURCES = [
     "device.hip",
     "kernels/t1_decode.hip",
+    "kernels/t1_wave.hip",
     "kernels/idwt.hip",
     "kernels/colour.hip",
     "host/codestream.cpp",

@@ -12,6 +12,8 @@
 namespace tc {
 hipError_t launch_t1(hipStream_t s, bool cls32, int g, const tc_cblk_desc* descs, const uint32_t* order, int n,
                      const uint8_t* arena, uint32_t* coef);
+hipError_t launch_t1_wave(hipStream_t s, bool narrow, int grid, const tc_cblk_desc* descs, const uint32_t* order,
+                          int n, const uint8_t* arena, uint32_t arena_bytes, uint32_t* coef);
 hipError_t launch_idwt_level(hipStream_t s, bool is97, const tc_dwt_job* jobs, const uint32_t* wg_job, int nwg,
                              void* base);
 hipError_t launch_idwt_final(hipStream_t s, bool is97, int nc, int out_bytes, const tc_final_job* jobs,
@@ -317,11 +319,22 @@ tc_status tc_graph_destroy(tc_graph g) {
 tc_status tc_launch_t1_decode(tc_stream s, int small_blocks, int g, const void* descs, const uint32_t* order, int n,
                               const uint8_t* arena, void* coef) {
     if (!s || (n > 0 && (!descs || !arena || !coef)) || n < 0) return fail(TC_ERR_INVALID_ARG, "tc_launch_t1_decode");
-    if (g != 1 && g != 8 && g != 16 && g != 32 && g != 64)
-        return fail(TC_ERR_INVALID_ARG, "tc_launch_t1_decode: blocks_per_wave must be 1, 8, 16, 32 or 64");
+    if (g != 8 && g != 16 && g != 32 && g != 64)
+        return fail(TC_ERR_INVALID_ARG, "tc_launch_t1_decode: blocks_per_wave must be 8, 16, 32 or 64");
     return hip_fail(launch_t1(s->s, small_blocks != 0, g, (const tc_cblk_desc*)descs, order, n, arena,
                               (uint32_t*)coef),
                     "t1_decode launch");
+}
+
+tc_status tc_launch_t1_decode_wave(tc_stream s, int small_blocks, int grid, const void* descs, const uint32_t* order,
+                                   int n, const uint8_t* arena, size_t arena_bytes, void* coef) {
+    if (!s || n < 0 || grid <= 0 || (n > 0 && (!descs || !arena || !coef)) || (arena_bytes & 3) ||
+        arena_bytes > 0xFFFFFFFFull || ((uintptr_t)arena & 3))
+        return fail(TC_ERR_INVALID_ARG, "tc_launch_t1_decode_wave");
+    return hip_fail(launch_t1_wave(s->s, small_blocks != 0, grid, (const tc_cblk_desc*)descs, order, n, arena,
+                                   (uint32_t)arena_bytes,
+                                   (uint32_t*)coef),
+                    "t1_decode_wave launch");
 }
 
 tc_status tc_launch_idwt_level(tc_stream s, int irr, const void* jobs, const uint32_t* wg_job, int nwg, void* base) {

@@ -142,7 +142,7 @@ tc_status tc_decoder_destroy(tc_decoder d) {
 
 tc_status tc_decoder_set_blocks_per_wave(tc_decoder d, int small, int bpw) {
     if (!d || (bpw != 0 && bpw != 1 && bpw != 8 && bpw != 16 && bpw != 32 && bpw != 64))
-        return report(Status::InvalidArgument, "blocks per wave must be 0, 1, 8, 16, 32 or 64");
+        return report(Status::InvalidArgument, "blocks per wave must be 0 (auto), 1 (one wave per block), 8, 16, 32 or 64");
     Tuning t = d->pipe->tuning();
     (small ? t.bpw32 : t.bpw64) = bpw;
     d->pipe->set_tuning(t);

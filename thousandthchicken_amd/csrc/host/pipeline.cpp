@@ -170,6 +170,7 @@ Status Pipeline::prepare(const std::vector<const Image*>& imgs, std::string& err
             }
             tc_cblk_desc d;
             std::memset(&d, 0, sizeof d);
+            cw.resize((cw.size() + 3) & ~(size_t)3, 0);  // codewords start 4-byte aligned
             d.data_off = (uint32_t)cw.size();
             d.data_len = cb.nbytes;
             for (uint32_t s = seg_start[k]; s < seg_start[k + 1]; s++) {
@@ -190,8 +191,9 @@ Status Pipeline::prepare(const std::vector<const Image*>& imgs, std::string& err
             descs.push_back(d);
         }
     }
-    cw.resize(cw.size() + 16, 0xFF);  // slack for look-ahead reads
-    cw_bytes_ = (int64_t)cw.size() - 16;
+    cw_bytes_ = (int64_t)cw.size();
+    cw.resize(((cw.size() + 3) & ~(size_t)3) + 16, 0xFF);  // slack for look-ahead reads
+    cw_size_ = cw.size();
     ncblk_ = (int64_t)descs.size();
 
     std::vector<uint8_t> blob;
@@ -413,8 +415,12 @@ Status Pipeline::run(bool timed) {
     if (timed) tc_event_record(ev_[1], stream_);
     for (const Group& g : groups_) {
         if (s != TC_SUCCESS) break;
-        s = tc_launch_t1_decode(stream_, g.small ? 1 : 0, g.bpw, up + descs_off_, (const uint32_t*)(up + g.order_off),
-                                g.n, up, base);
+        if (g.bpw == 1)
+            s = tc_launch_t1_decode_wave(stream_, g.small ? 1 : 0, std::min(g.n, device_cu_count(dev_) * 16), up + descs_off_,
+                                         (const uint32_t*)(up + g.order_off), g.n, up, cw_size_, base);
+        else
+            s = tc_launch_t1_decode(stream_, g.small ? 1 : 0, g.bpw, up + descs_off_,
+                                    (const uint32_t*)(up + g.order_off), g.n, up, base);
     }
     if (timed) tc_event_record(ev_[2], stream_);
     for (const LevelLaunch& L : levels_) {

@@ -111,7 +111,7 @@ class Pipeline {
     std::vector<LevelLaunch> levels_;
     std::vector<FinalLaunch> finals_;
     std::vector<ImageOut> images_;
-    uint64_t descs_off_ = 0, coef_elems_ = 0;
+    uint64_t descs_off_ = 0, coef_elems_ = 0, cw_size_ = 0;
     int64_t ncblk_ = 0, cw_bytes_ = 0, nsamples_ = 0, npixels_ = 0, dwt_bytes_ = 0, final_bytes_ = 0;
     int bpw_[2] = {0, 0};
     Tuning tuning_;

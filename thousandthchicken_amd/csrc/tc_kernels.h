@@ -57,3 +57,7 @@ typedef struct tc_final_job {
 #ifdef __cplusplus
 }
 #endif
+
+#ifdef __cplusplus
+static_assert(sizeof(tc_cblk_desc) == 32, "tc_cblk_desc is read as 8 dwords by the kernels");
+#endif
