@@ -83,6 +83,13 @@ tc_status tc_decoder_launch_graph(tc_decoder dec);
 tc_status tc_decoder_get_stats(tc_decoder dec, tc_decoder_stats *stats);
 tc_stream tc_decoder_stream(tc_decoder dec);
 
+/* Decodes by running the drop-in stage classes one after the other, as
+ * TC/Decoder.cpp:153-180 does (CoefficientCoder::decode_tile,
+ * Quantizer::dequantize_tile, DWT::iwt per tile, then Preprocessor), with each
+ * tile-component mapped to the host and packed into the same interleaved layout
+ * as tc_decoder_decode.  Slower than the fused path; for checking the stages. */
+tc_status tc_decode_staged(tc_device dev, const uint8_t *cs, size_t len, void *out, size_t out_cap);
+
 #ifdef __cplusplus
 }
 #endif

@@ -71,12 +71,14 @@ tc_status tc_stream_create(tc_device dev, tc_stream *out);
 tc_status tc_stream_destroy(tc_stream s);
 tc_status tc_stream_sync(tc_stream s);  /* clFinish */
 tc_status tc_stream_flush(tc_stream s); /* clFlush: work is already submitted on HIP */
+tc_device tc_stream_device(tc_stream s);
 
 /* Buffers (clCreateBuffer / clReleaseMemObject / clEnqueueFillBuffer /
  * clEnqueueCopyBufferRect / clEnqueueMapBuffer) */
 tc_status tc_buf_alloc(tc_device dev, size_t bytes, tc_buf *out);
 tc_status tc_buf_free(tc_buf b);
 void *tc_buf_device_ptr(tc_buf b);
+tc_device tc_buf_device(tc_buf b);
 size_t tc_buf_size(tc_buf b);
 tc_status tc_buf_fill32(tc_stream s, tc_buf b, size_t byte_off, uint32_t value, size_t count);
 tc_status tc_buf_copy_h2d(tc_stream s, tc_buf dst, size_t dst_off, const void *src, size_t bytes);
@@ -136,8 +138,9 @@ tc_status tc_launch_idwt_final(tc_stream s, int irreversible, int ncomp, int out
 
 /* Standalone inverse colour transform / DC shift over int32 planes in place, for the
  * Preprocessor drop-in (color_decoder_lossless/lossy, idc_level_shifting,
- * TC/Preprocessor.cpp:102-157).  kind: 0 DC shift only, 1 inverse RCT, 2 inverse ICT.
- * For kind 2 the planes hold float samples. */
+ * TC/Preprocessor.cpp:102-157).  kind: 0 DC shift only, 1 inverse RCT, 2 inverse ICT,
+ * 3 DC shift of float samples.  For kinds 2 and 3 the planes hold float samples on
+ * entry and rounded int32 samples on return. */
 tc_status tc_launch_colour(tc_stream s, int kind, void *c0, void *c1, void *c2, size_t count, int32_t shift,
                            int32_t minv, int32_t maxv);
 

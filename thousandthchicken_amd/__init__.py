@@ -85,6 +85,7 @@ def lib():
         "tc_decoder_launch_graph": (i32, [vp]),
         "tc_decoder_get_stats": (i32, [vp, P(DecoderStats)]),
         "tc_decoder_stream": (vp, [vp]),
+        "tc_decode_staged": (i32, [vp, ctypes.c_char_p, sz, vp, sz]),
     }
     for name, (res, args) in sigs.items():
         f = getattr(L, name)
@@ -165,6 +166,16 @@ def default_device():
 def _shape(info: ImageInfo):
     dt = np.uint8 if info.bytes_per_sample == 1 else np.uint16
     return (info.height, info.width, info.ncomp), dt
+
+
+def decode_staged(cs: bytes, device: "Device" = None) -> np.ndarray:
+    """Decode through the drop-in stage classes (reference stage order)."""
+    dev = device or default_device()
+    info = probe(cs)
+    shape, dt = _shape(info)
+    out = np.empty(shape, dt)
+    check(lib().tc_decode_staged(dev.handle, cs, len(cs), out.ctypes.data, out.nbytes), "tc_decode_staged")
+    return out
 
 
 class Decoder:

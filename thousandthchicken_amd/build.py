@@ -22,6 +22,7 @@ SOURCES = [
     "kernels/colour.hip",
     "host/codestream.cpp",
     "host/pipeline.cpp",
+    "host/plan_util.cpp",
     "host/decoder_api.cpp",
     "host/stages.cpp",
     "host/server.cpp",

@@ -179,6 +179,8 @@ tc_status tc_buf_free(tc_buf b) {
 }
 
 void* tc_buf_device_ptr(tc_buf b) { return b ? b->p : nullptr; }
+tc_device tc_buf_device(tc_buf b) { return b ? b->dev : nullptr; }
+tc_device tc_stream_device(tc_stream s) { return s ? s->dev : nullptr; }
 size_t tc_buf_size(tc_buf b) { return b ? b->n : 0; }
 
 static bool in_range(tc_buf b, size_t off, size_t n) { return b && off <= b->n && n <= b->n - off; }
@@ -353,7 +355,8 @@ tc_status tc_launch_idwt_final(tc_stream s, int irr, int nc, int ob, const void*
 
 tc_status tc_launch_colour(tc_stream s, int kind, void* c0, void* c1, void* c2, size_t n, int32_t shift,
                            int32_t minv, int32_t maxv) {
-    if (!s || kind < 0 || kind > 2 || !c0 || (kind > 0 && (!c1 || !c2))) return fail(TC_ERR_INVALID_ARG, "tc_launch_colour");
+    if (!s || kind < 0 || kind > 3 || !c0 || ((kind == 1 || kind == 2) && (!c1 || !c2)))
+        return fail(TC_ERR_INVALID_ARG, "tc_launch_colour");
     return hip_fail(launch_colour(s->s, kind, c0, c1, c2, n, shift, minv, maxv), "colour launch");
 }
 

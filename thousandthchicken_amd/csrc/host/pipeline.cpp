@@ -7,6 +7,7 @@
 #include <cstring>
 
 #include "../tc_internal.hpp"
+#include "plan_util.hpp"
 
 namespace tc {
 
@@ -124,13 +125,8 @@ Status Pipeline::prepare(const std::vector<const Image*>& imgs, std::string& err
         const Image& im = *imgs[i];
         for (size_t t = 0; t < im.tiles.size(); t++)
             for (int c = 0; c < im.ncomp; c++) {
-                const TileComp& tc = im.tiles[t].comps[c];
-                uint64_t sz[2] = {0, 0};
-                for (int n = 2; n <= tc.nlev; n++) {
-                    const Resolution& R = tc.res[tc.nlev - n + 1];
-                    uint64_t e = (uint64_t)R.width() * R.height();
-                    sz[(tc.nlev - n) % 2] = std::max(sz[(tc.nlev - n) % 2], e);
-                }
+                uint64_t sz[2];
+                work_sizes(im.tiles[t].comps[c], sz);
                 for (int k = 0; k < 2; k++) {
                     pos[i][t][c].work[k] = work;
                     work += align_up(sz[k], 64);
@@ -145,50 +141,11 @@ Status Pipeline::prepare(const std::vector<const Image*>& imgs, std::string& err
     std::vector<uint32_t> cls_idx[2];
     for (size_t i = 0; i < imgs.size(); i++) {
         const Image& im = *imgs[i];
-        // segments grouped per code-block, in codestream order
-        std::vector<uint32_t> seg_start(im.cblks.size() + 1, 0), seg_list(im.segs.size());
-        for (const Segment& s : im.segs) seg_start[s.cblk + 1]++;
-        for (size_t k = 0; k < im.cblks.size(); k++) seg_start[k + 1] += seg_start[k];
-        {
-            std::vector<uint32_t> fillp(seg_start.begin(), seg_start.end() - 1);
-            for (uint32_t s = 0; s < im.segs.size(); s++) seg_list[fillp[im.segs[s].cblk]++] = s;
-        }
-        for (size_t k = 0; k < im.cblks.size(); k++) {
-            const CodeBlock& cb = im.cblks[k];
-            int w = cb.x1 - cb.x0, h = cb.y1 - cb.y0;
-            if (!cb.included || cb.numbps <= 0 || cb.npasses <= 0 || w <= 0 || h <= 0) continue;
-            if (w > 64 || h > 64) {
-                err = "code-blocks larger than 64x64 are not supported";
-                return Status::Unsupported;
-            }
-            const TileComp& tc = im.tiles[cb.tile].comps[cb.comp];
-            const Band& B = tc.res[cb.res].band[cb.band];
-            int ox = 0, oy = 0;
-            if (cb.res > 0) {
-                if (B.orient == 1 || B.orient == 3) ox = tc.res[cb.res - 1].width();
-                if (B.orient == 2 || B.orient == 3) oy = tc.res[cb.res - 1].height();
-            }
-            tc_cblk_desc d;
-            std::memset(&d, 0, sizeof d);
-            cw.resize((cw.size() + 3) & ~(size_t)3, 0);  // codewords start 4-byte aligned
-            d.data_off = (uint32_t)cw.size();
-            d.data_len = cb.nbytes;
-            for (uint32_t s = seg_start[k]; s < seg_start[k + 1]; s++) {
-                const Segment& sg = im.segs[seg_list[s]];
-                cw.insert(cw.end(), im.cs + sg.off, im.cs + sg.off + sg.len);
-            }
-            d.out_off = pos[i][cb.tile][cb.comp].plane + (uint64_t)(oy + cb.y0 - B.y0) * tc.width() +
-                        (uint64_t)(ox + cb.x0 - B.x0);
-            d.out_pitch = (uint32_t)tc.width();
-            d.w = (uint16_t)w;
-            d.h = (uint16_t)h;
-            d.numbps = (uint8_t)cb.numbps;
-            d.npasses = (uint8_t)std::min(cb.npasses, 255);
-            d.cls = (uint8_t)(B.orient == 1 ? 1 : (B.orient == 3 ? 2 : 0));
-            d.mode = tc.reversible ? 0 : 1;
-            d.delta = B.delta;
-            cls_idx[(w <= 32 && h <= 32) ? 0 : 1].push_back((uint32_t)descs.size());
-            descs.push_back(d);
+        for (size_t t = 0; t < im.tiles.size(); t++) {
+            std::vector<uint64_t> planes(im.ncomp);
+            for (int c = 0; c < im.ncomp; c++) planes[c] = pos[i][t][c].plane;
+            Status st = append_tile_cblks(im, (int)t, planes.data(), cw, descs, cls_idx, err);
+            if (st != Status::Ok) return st;
         }
     }
     cw_bytes_ = (int64_t)cw.size();
@@ -204,13 +161,7 @@ Status Pipeline::prepare(const std::vector<const Image*>& imgs, std::string& err
         auto& v = cls_idx[k];
         bpw_[k] = 0;
         if (v.empty()) continue;
-        // packing: blocks with similar pass counts and shapes share a wave
-        std::stable_sort(v.begin(), v.end(), [&](uint32_t a, uint32_t b) {
-            const tc_cblk_desc &x = descs[a], &y = descs[b];
-            if (x.npasses != y.npasses) return x.npasses > y.npasses;
-            if (x.h != y.h) return x.h > y.h;
-            return x.w > y.w;
-        });
+        sort_for_packing(v, descs);
         Group g;
         g.small = k == 0;
         g.bpw = pick_blocks_per_wave((int)v.size(), g.small, cus, tuning_);
@@ -235,32 +186,11 @@ Status Pipeline::prepare(const std::vector<const Image*>& imgs, std::string& err
                     const TileComp& tc = im.tiles[t].comps[c];
                     int n = tc.nlev - s;
                     if (n < 2) continue;
-                    int r = tc.nlev - n + 1;
-                    const Resolution &R = tc.res[r], &Rl = tc.res[r - 1];
+                    const Resolution& R = tc.res[tc.nlev - n + 1];
                     if (R.width() <= 0 || R.height() <= 0) continue;
                     const TCPos& P = pos[i][t][c];
-                    int lw = Rl.width(), lh = Rl.height();
-                    tc_dwt_job j;
-                    std::memset(&j, 0, sizeof j);
-                    if (n == tc.nlev) {
-                        j.ll_off = P.plane;
-                        j.ll_pitch = (uint32_t)tc.width();
-                    } else {
-                        j.ll_off = P.work[(tc.nlev - n - 1) % 2];
-                        j.ll_pitch = (uint32_t)lw;
-                    }
-                    j.hl_off = P.plane + lw;
-                    j.lh_off = P.plane + (uint64_t)lh * tc.width();
-                    j.hh_off = P.plane + (uint64_t)lh * tc.width() + lw;
-                    j.band_pitch = (uint32_t)tc.width();
-                    j.out_off = P.work[(tc.nlev - n) % 2];
-                    j.out_pitch = (uint32_t)R.width();
-                    j.u0 = R.x0;
-                    j.u1 = R.x1;
-                    j.v0 = R.y0;
-                    j.v1 = R.y1;
+                    tc_dwt_job j = level_job(tc, n, P.plane, P.work, false, 0, 0, TX, TY);
                     int irr = tc.reversible ? 0 : 1;
-                    j.tiles_x = (R.width() + TX - 1) / TX;
                     int tiles_y = (R.height() + TY - 1) / TY;
                     j.wg_base = (int32_t)maps[irr].size();
                     uint32_t jid = (uint32_t)jobs[irr].size();

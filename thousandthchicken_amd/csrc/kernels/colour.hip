@@ -24,6 +24,8 @@ __global__ void colour_kernel(int kind, int32_t* __restrict__ c0, int32_t* __res
             c0[i] = clampi(y2 + g + shift, lo, hi);
             c1[i] = clampi(g + shift, lo, hi);
             c2[i] = clampi(y1 + g + shift, lo, hi);
+        } else if (kind == 3) {
+            c0[i] = clampi((int)rintf(__int_as_float(c0[i])) + shift, lo, hi);
         } else {
             float y0 = __int_as_float(c0[i]), y1 = __int_as_float(c1[i]), y2 = __int_as_float(c2[i]);
             float r = fmaf(1.402f, y2, y0);
