@@ -1,0 +1,87 @@
+"""Both tier-1 kernels (one wave per block, one lane per block) through their
+typed C-ABI launches on random code-blocks of every shape, orientation, bit
+depth and truncation point, against the oracle's tier-1 decode."""
+import ctypes
+
+import numpy as np
+import pytest
+
+import thousandthchicken_amd as tca
+from oracle import oracle
+from thousandthchicken_amd import synth
+
+pytestmark = pytest.mark.gpu
+
+
+def _blocks(seed, n, maxw=64, maxh=64):
+    rng = np.random.default_rng(seed)
+    L = synth._enc_lib()
+    out = []
+    for _ in range(n):
+        w, h = int(rng.integers(1, maxw + 1)), int(rng.integers(1, maxh + 1))
+        orient = int(rng.integers(0, 4))
+        bits = int(rng.integers(1, 18))
+        a = rng.integers(-(1 << bits) + 1, 1 << bits, (h, w)).astype(np.int32)
+        a[rng.uniform(size=(h, w)) < rng.uniform(0, 0.95)] = 0
+        buf = np.zeros(1 << 18, np.uint8)
+        nb = ctypes.c_int(0)
+        keep = int(rng.integers(1, 60)) if rng.uniform() < 0.5 else 999
+        n_ = L.tce_t1_encode_truncated(a.ctypes.data, w, h, orient, keep, buf.ctypes.data, buf.size, ctypes.byref(nb))
+        if nb.value == 0:
+            continue
+        npass = min(keep, 3 * nb.value - 2)
+        out.append(dict(w=w, h=h, orient=orient, numbps=nb.value, npasses=npass, data=bytes(buf[:n_]), src=a))
+    return out
+
+
+def _expected(b):
+    mag, sign, low, _ = oracle.t1_decode(b["data"], b["w"], b["h"], b["orient"], b["numbps"], b["npasses"])
+    half = np.where(low >= 1, np.left_shift(1, np.maximum(low.astype(np.int32) - 1, 0)), 0)
+    v = np.where(low >= 0, mag + half, 0)
+    return np.where(sign.astype(bool), -v, v).astype(np.int32)
+
+
+@pytest.mark.parametrize("kernel", ["wave", 8, 16, 32, 64])
+@pytest.mark.parametrize("small", [True, False])
+def test_t1_kernels_on_random_blocks(kernel, small):
+    lim = 32 if small else 64
+    blocks = _blocks(7 if small else 8, 300, lim, lim)
+    print(f"t1 kernel={kernel} small={small} blocks={len(blocks)}", flush=True)
+    dev = tca.default_device()
+    L = tca.lib()
+    stream = ctypes.c_void_p()
+    tca.check(L.tc_stream_create(dev.handle, ctypes.byref(stream)), "stream")
+    arena = bytearray()
+    descs = np.zeros(len(blocks), tca.CBLK_DESC)
+    pitch = 64
+    for i, b in enumerate(blocks):
+        arena += bytes((-len(arena)) % 4)
+        descs[i] = (len(arena), len(b["data"]), i * pitch * 64, pitch, b["w"], b["h"], b["numbps"], b["npasses"],
+                    1 if b["orient"] == 1 else (2 if b["orient"] == 3 else 0), 0, 1.0)
+        arena += b["data"]
+    arena += bytes((-len(arena)) % 4) + b"\xff" * 16
+    arena = np.frombuffer(bytes(arena), np.uint8)
+    n_out = len(blocks) * pitch * 64
+    bufs = [tca.DeviceBuffer(dev, max(x, 256)) for x in (arena.nbytes, descs.nbytes, n_out * 4)]
+    try:
+        bufs[0].upload(stream, arena)
+        bufs[1].upload(stream, descs)
+        tca.check(L.tc_buf_fill32(stream, bufs[2].handle, 0, 0, n_out), "fill")
+        if kernel == "wave":
+            st = L.tc_launch_t1_decode_wave(stream, 1 if small else 0, 512, bufs[1].ptr, None, len(blocks),
+                                            bufs[0].ptr, arena.nbytes, bufs[2].ptr)
+        else:
+            st = L.tc_launch_t1_decode(stream, 1 if small else 0, kernel, bufs[1].ptr, None, len(blocks),
+                                       bufs[0].ptr, bufs[2].ptr)
+        tca.check(st, "launch")
+        out = bufs[2].download(stream, np.int32, n_out).reshape(len(blocks), 64, pitch)
+        for i, b in enumerate(blocks):
+            got = out[i, :b["h"], :b["w"]]
+            exp = _expected(b)
+            assert np.array_equal(got, exp), (i, b["w"], b["h"], b["orient"], b["numbps"], b["npasses"])
+            if b["npasses"] == 3 * b["numbps"] - 2:
+                assert np.array_equal(got, b["src"])
+    finally:
+        for x in bufs:
+            x.free()
+        L.tc_stream_destroy(stream)

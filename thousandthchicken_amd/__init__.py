@@ -86,6 +86,17 @@ def lib():
         "tc_decoder_get_stats": (i32, [vp, P(DecoderStats)]),
         "tc_decoder_stream": (vp, [vp]),
         "tc_decode_staged": (i32, [vp, ctypes.c_char_p, sz, vp, sz]),
+        "tc_stream_create": (i32, [vp, P(vp)]),
+        "tc_stream_destroy": (i32, [vp]),
+        "tc_stream_sync": (i32, [vp]),
+        "tc_buf_alloc": (i32, [vp, sz, P(vp)]),
+        "tc_buf_free": (i32, [vp]),
+        "tc_buf_device_ptr": (vp, [vp]),
+        "tc_buf_copy_h2d": (i32, [vp, vp, sz, vp, sz]),
+        "tc_buf_copy_d2h": (i32, [vp, vp, vp, sz, sz]),
+        "tc_buf_fill32": (i32, [vp, vp, sz, ctypes.c_uint32, sz]),
+        "tc_launch_t1_decode": (i32, [vp, i32, i32, vp, vp, i32, vp, vp]),
+        "tc_launch_t1_decode_wave": (i32, [vp, i32, i32, vp, vp, i32, vp, sz, vp]),
     }
     for name, (res, args) in sigs.items():
         f = getattr(L, name)
@@ -176,6 +187,41 @@ def decode_staged(cs: bytes, device: "Device" = None) -> np.ndarray:
     out = np.empty(shape, dt)
     check(lib().tc_decode_staged(dev.handle, cs, len(cs), out.ctypes.data, out.nbytes), "tc_decode_staged")
     return out
+
+
+CBLK_DESC = np.dtype([("data_off", "<u4"), ("data_len", "<u4"), ("out_off", "<u8"), ("out_pitch", "<u4"),
+                      ("w", "<u2"), ("h", "<u2"), ("numbps", "u1"), ("npasses", "u1"), ("cls", "u1"),
+                      ("mode", "u1"), ("delta", "<f4")])
+assert CBLK_DESC.itemsize == 32
+
+
+class DeviceBuffer:
+    """A device allocation through the C ABI (for tests of the typed launches)."""
+
+    def __init__(self, device, nbytes):
+        h = ctypes.c_void_p()
+        check(lib().tc_buf_alloc(device.handle, nbytes, ctypes.byref(h)), "tc_buf_alloc")
+        self.handle, self.nbytes = h, nbytes
+
+    @property
+    def ptr(self):
+        return lib().tc_buf_device_ptr(self.handle)
+
+    def upload(self, stream, arr):
+        a = np.ascontiguousarray(arr)
+        check(lib().tc_buf_copy_h2d(stream, self.handle, 0, a.ctypes.data, a.nbytes), "h2d")
+        check(lib().tc_stream_sync(stream), "sync")
+
+    def download(self, stream, dtype, count):
+        out = np.empty(count, dtype)
+        check(lib().tc_buf_copy_d2h(stream, out.ctypes.data, self.handle, 0, out.nbytes), "d2h")
+        check(lib().tc_stream_sync(stream), "sync")
+        return out
+
+    def free(self):
+        if self.handle:
+            lib().tc_buf_free(self.handle)
+            self.handle = None
 
 
 class Decoder:

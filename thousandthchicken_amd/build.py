@@ -47,8 +47,24 @@ def _compile(src):
     return obj, r.stderr
 
 
+ENC_SRC = os.path.join(CSRC, "encoder", "encoder.cpp")
+ENC_OUT = os.path.join(HERE, "libtc_encode.so")
+
+
+def build_encoder():
+    """Host-only encoder library (synthetic workloads for tests and benchmarks)."""
+    tmp = ENC_OUT + ".tmp"
+    cmd = ["g++", "-O2", "-std=c++17", "-fPIC", "-shared", "-pthread", "-Wall", "-o", tmp, ENC_SRC]
+    r = subprocess.run(cmd, capture_output=True, text=True)
+    if r.returncode != 0:
+        raise RuntimeError(f"encoder build failed\n{r.stdout}\n{r.stderr}")
+    os.replace(tmp, ENC_OUT)
+    return ENC_OUT
+
+
 def build(verbose=False):
-    """Compiles every source and links libtc.so.  Always rebuilds."""
+    """Compiles every source and links libtc.so (and the encoder).  Always rebuilds."""
+    build_encoder()
     os.makedirs(OBJ, exist_ok=True)
     srcs = [s for s in SOURCES if os.path.exists(os.path.join(CSRC, s))]
     with cf.ThreadPoolExecutor(max_workers=min(8, len(srcs))) as ex:

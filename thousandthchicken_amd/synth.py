@@ -1,6 +1,7 @@
 """Seeded synthetic images for the benchmark workloads and tests.
 
-Recipe (per component, every value drawn from a SplitMix64-seeded PCG32):
+Recipe (per component; shape parameters drawn from a SplitMix64-seeded PCG32,
+per-pixel noise from SplitMix64 applied to the pixel index):
   * 8 sinusoids a*sin(2*pi*(fx*x + fy*y) + phi), a ~ U[10,40],
     fx ~ U[0.5,16]/width, fy ~ U[0.5,16]/height, phi ~ U[0, 2*pi);
   * Gaussian noise with standard deviation 12 (the "sigma 12"), blurred with a
@@ -69,9 +70,14 @@ def _gauss_kernel(sigma=1.0, radius=3):
 def _blur(a, k):
     r = len(k) // 2
     p = np.pad(a, ((0, 0), (r, r)), mode="edge")
-    a = sum(k[i] * p[:, i:i + a.shape[1]] for i in range(len(k)))
-    p = np.pad(a, ((r, r), (0, 0)), mode="edge")
-    return sum(k[i] * p[i:i + a.shape[0], :] for i in range(len(k)))
+    out = k[0] * p[:, 0:a.shape[1]]
+    for i in range(1, len(k)):
+        out += k[i] * p[:, i:i + a.shape[1]]
+    p = np.pad(out, ((r, r), (0, 0)), mode="edge")
+    out = k[0] * p[0:a.shape[0], :]
+    for i in range(1, len(k)):
+        out += k[i] * p[i:i + a.shape[0], :]
+    return out
 
 
 def component(height, width, seed, comp):
@@ -85,7 +91,10 @@ def component(height, width, seed, comp):
         fx = (0.5 + 15.5 * g.uniform()[0]) / width
         fy = (0.5 + 15.5 * g.uniform()[0]) / height
         ph = 2 * math.pi * g.uniform()[0]
-        img += a * np.sin(2 * math.pi * (fx * x + fy * y) + ph)
+        # sin(u + v) = sin u cos v + cos u sin v, with u along x and v along y
+        u = 2 * math.pi * fx * x + ph
+        v = 2 * math.pi * fy * y
+        img += a * (np.sin(u) * np.cos(v) + np.cos(u) * np.sin(v))
     for _ in range(20):
         rw = max(1, int(round(width * (0.04 + 0.36 * g.uniform()[0]))))
         rh = max(1, int(round(height * (0.04 + 0.36 * g.uniform()[0]))))
@@ -93,17 +102,14 @@ def component(height, width, seed, comp):
         ry = int(g.uniform()[0] * max(1, height - rh + 1))
         val = 255 * g.uniform()[0]
         img[ry:ry + rh, rx:rx + rw] = 0.5 * img[ry:ry + rh, rx:rx + rw] + 0.5 * val
-    # per-row noise streams, Box-Muller
-    rows = PCG32(_splitmix64(np.arange(height, dtype=np.uint64) + np.uint64(seed * 7919 + comp * 104729)),
-                 np.arange(height, dtype=np.uint64) + np.uint64(comp))
-    noise = np.empty((height, width))
-    for j in range(0, width, 2):
-        u1 = (rows.next_u32().astype(np.float64) + 1.0) / 4294967297.0
-        u2 = rows.uniform()
-        r = np.sqrt(-2.0 * np.log(u1))
-        noise[:, j] = r * np.cos(2 * math.pi * u2)
-        if j + 1 < width:
-            noise[:, j + 1] = r * np.sin(2 * math.pi * u2)
+    # per-pixel noise from SplitMix64 over the pixel index (counter based), Box-Muller
+    idx = np.arange(height * width, dtype=np.uint64)
+    base = _splitmix64(np.uint64(seed) * np.uint64(0x100000001B3) + np.uint64(comp + 1))
+    with np.errstate(over="ignore"):
+        z = _splitmix64(idx + base)
+    u1 = ((z >> np.uint64(32)).astype(np.float64) + 1.0) / 4294967297.0
+    u2 = (z & np.uint64(0xFFFFFFFF)).astype(np.float64) / 4294967296.0
+    noise = (np.sqrt(-2.0 * np.log(u1)) * np.cos(2 * math.pi * u2)).reshape(height, width)
     img += _blur(12.0 * noise, _gauss_kernel(1.0))
     return img
 
@@ -115,3 +121,62 @@ def image(height, width, ncomp, seed, bits=8):
     if bits == 8:
         return a.astype(np.uint8)
     return np.rint(a * ((1 << bits) - 1) / 255.0).astype(np.uint16)
+
+
+# ---------------------------------------------------------------- encoder
+
+_ENC = None
+
+
+def _enc_lib():
+    global _ENC
+    if _ENC is None:
+        import ctypes
+        import os
+        path = os.path.join(os.path.dirname(os.path.abspath(__file__)), "libtc_encode.so")
+        if not os.path.exists(path):
+            from . import build as _b
+            _b.build_encoder()
+        L = ctypes.CDLL(path)
+        vp, i32 = ctypes.c_void_p, ctypes.c_int
+        L.tce_encode.argtypes = [vp] + [i32] * 10 + [i32, ctypes.c_double, ctypes.c_double, i32,
+                                                      ctypes.POINTER(ctypes.c_void_p),
+                                                      ctypes.POINTER(ctypes.c_size_t)]
+        L.tce_free.argtypes = [vp]
+        L.tce_t1_encode_block.argtypes = [vp, i32, i32, i32, vp, i32, ctypes.POINTER(i32)]
+        L.tce_t1_encode_truncated.argtypes = [vp, i32, i32, i32, i32, vp, i32, ctypes.POINTER(i32)]
+        _ENC = L
+    return _ENC
+
+
+def encode(img, bits=None, levels=5, cb=64, tile=None, mct=None, lossy=False, bpp=None, base_step=0.5,
+           threads=0):
+    """Encodes img [H, W] or [H, W, C] (uint8 / uint16) into a raw codestream.
+
+    Lossless: RCT (when mct) + 5/3, every pass kept.  Lossy: ICT + 9/7 with
+    expounded quantization, truncated to about `bpp` bits per pixel."""
+    import ctypes
+    a = np.ascontiguousarray(img)
+    if a.ndim == 2:
+        a = a[..., None]
+    h, w, c = a.shape
+    if bits is None:
+        bits = 8 if a.dtype == np.uint8 else 16
+    if bits > 8:
+        a = np.ascontiguousarray(a.astype(np.uint16))
+    else:
+        a = np.ascontiguousarray(a.astype(np.uint8))
+    if mct is None:
+        mct = c >= 3
+    tw, th = (tile if tile else (0, 0))
+    target = (bpp * w * h / 8.0) if (lossy and bpp) else 0.0
+    out = ctypes.c_void_p()
+    n = ctypes.c_size_t()
+    cbe = int(cb).bit_length() - 1
+    rc = _enc_lib().tce_encode(a.ctypes.data, w, h, c, bits, levels, cbe, cbe, tw, th, 1 if mct else 0,
+                               1 if lossy else 0, target, base_step, threads, ctypes.byref(out), ctypes.byref(n))
+    if rc:
+        raise RuntimeError(f"tce_encode failed ({rc})")
+    data = ctypes.string_at(out.value, n.value)
+    _enc_lib().tce_free(out)
+    return data
