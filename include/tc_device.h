@@ -99,6 +99,8 @@ tc_status tc_event_create(tc_event *out);
 tc_status tc_event_destroy(tc_event e);
 tc_status tc_event_record(tc_event e, tc_stream s);
 tc_status tc_event_sync(tc_event e);
+/* TC_SUCCESS when the work before the event has finished, TC_ERR_BUSY if not */
+tc_status tc_event_query(tc_event e);
 tc_status tc_event_elapsed_ms(tc_event start, tc_event stop, float *ms);
 
 /* Graph capture of the launches issued on a stream (batched form). */

@@ -1,0 +1,67 @@
+/*
+ * tc_server.h -- C ABI of the decode server: continuous batching of decode
+ * requests on one MI355X.
+ *
+ * No counterpart in the reference (its Decoder decodes one file per call,
+ * TC/Decoder.cpp:94-216).  Requests are parsed when submitted; a scheduler
+ * thread admits queued requests into the next batch of a free slot while the
+ * batch's device-memory footprint fits the slot's share of the budget, and
+ * launches it (one launch per stage for the whole batch) on the slot's own
+ * stream, so batches of different slots overlap on the device.  Device buffers
+ * of every slot stay resident between batches.  Results are copied back with
+ * the batch and kept until read.  Every request is decoded independently, so
+ * its result does not depend on what it was batched with.
+ */
+#ifndef TC_SERVER_H
+#define TC_SERVER_H
+
+#include "tc_device.h"
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+typedef struct tc_server_s *tc_server;
+
+typedef struct {
+    int32_t slots;            /* concurrent batches (streams), default 3 */
+    int32_t max_batch;        /* most requests per batch, default 256 */
+    uint64_t memory_budget;   /* device bytes for all slots together, default 32 GiB */
+    int32_t bpw_small, bpw_large; /* tier-1 packing (0: automatic) */
+} tc_server_config;
+
+typedef struct {
+    tc_status status;
+    int32_t width, height, ncomp, bytes_per_sample;
+    double submit_s, start_s, done_s; /* seconds since the server started */
+    uint64_t batch;                   /* batch sequence number */
+    int32_t batch_size;
+    uint64_t footprint;               /* device bytes accounted for the request */
+} tc_request_info;
+
+typedef struct {
+    uint64_t submitted, completed, failed, batches;
+    uint64_t pixels;          /* decoded pixels */
+    double busy_s;            /* time with at least one batch on the device */
+    uint64_t peak_bytes;      /* largest admitted footprint of all slots */
+    int32_t queued, in_flight;
+} tc_server_stats;
+
+tc_status tc_server_create(tc_device dev, const tc_server_config *cfg, tc_server *out);
+tc_status tc_server_destroy(tc_server srv);
+/* Copies the codestream; *id identifies the request. */
+tc_status tc_server_submit(tc_server srv, const uint8_t *cs, size_t len, uint64_t *id);
+/* *done = 1 once the request finished (successfully or not). */
+tc_status tc_server_poll(tc_server srv, uint64_t id, int *done);
+/* Blocks until the request finished or timeout_s passed (TC_ERR_BUSY). */
+tc_status tc_server_wait(tc_server srv, uint64_t id, double timeout_s);
+/* Copies the decoded image (interleaved, as tc_decoder_decode) and releases the
+ * request.  out may be NULL to read only info (the request is then kept). */
+tc_status tc_server_result(tc_server srv, uint64_t id, void *out, size_t cap, tc_request_info *info);
+tc_status tc_server_get_stats(tc_server srv, tc_server_stats *stats);
+double tc_server_now(tc_server srv);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

@@ -87,6 +87,14 @@ def lib():
         "tc_decoder_stream": (vp, [vp]),
         "tc_decode_staged": (i32, [vp, ctypes.c_char_p, sz, vp, sz]),
         "tc_stream_create": (i32, [vp, P(vp)]),
+        "tc_server_create": (i32, [vp, vp, P(vp)]),
+        "tc_server_destroy": (i32, [vp]),
+        "tc_server_submit": (i32, [vp, ctypes.c_char_p, sz, P(ctypes.c_uint64)]),
+        "tc_server_poll": (i32, [vp, ctypes.c_uint64, P(i32)]),
+        "tc_server_wait": (i32, [vp, ctypes.c_uint64, ctypes.c_double]),
+        "tc_server_result": (i32, [vp, ctypes.c_uint64, vp, sz, vp]),
+        "tc_server_get_stats": (i32, [vp, vp]),
+        "tc_server_now": (ctypes.c_double, [vp]),
         "tc_stream_destroy": (i32, [vp]),
         "tc_stream_sync": (i32, [vp]),
         "tc_buf_alloc": (i32, [vp, sz, P(vp)]),
@@ -193,6 +201,87 @@ CBLK_DESC = np.dtype([("data_off", "<u4"), ("data_len", "<u4"), ("out_off", "<u8
                       ("w", "<u2"), ("h", "<u2"), ("numbps", "u1"), ("npasses", "u1"), ("cls", "u1"),
                       ("mode", "u1"), ("delta", "<f4")])
 assert CBLK_DESC.itemsize == 32
+
+
+class ServerConfig(ctypes.Structure):
+    _fields_ = [("slots", ctypes.c_int32), ("max_batch", ctypes.c_int32), ("memory_budget", ctypes.c_uint64),
+                ("bpw_small", ctypes.c_int32), ("bpw_large", ctypes.c_int32)]
+
+
+class RequestInfo(ctypes.Structure):
+    _fields_ = [("status", ctypes.c_int32), ("width", ctypes.c_int32), ("height", ctypes.c_int32),
+                ("ncomp", ctypes.c_int32), ("bytes_per_sample", ctypes.c_int32), ("submit_s", ctypes.c_double),
+                ("start_s", ctypes.c_double), ("done_s", ctypes.c_double), ("batch", ctypes.c_uint64),
+                ("batch_size", ctypes.c_int32), ("footprint", ctypes.c_uint64)]
+
+
+class ServerStats(ctypes.Structure):
+    _fields_ = [(n, ctypes.c_uint64) for n in ("submitted", "completed", "failed", "batches", "pixels")] + [
+        ("busy_s", ctypes.c_double), ("peak_bytes", ctypes.c_uint64), ("queued", ctypes.c_int32),
+        ("in_flight", ctypes.c_int32)]
+
+
+class Server:
+    """Continuous-batching decode server over the C ABI (include/tc_server.h)."""
+
+    def __init__(self, device: "Device" = None, slots=3, max_batch=256, memory_budget=32 << 30, bpw_small=0,
+                 bpw_large=0):
+        self.device = device or default_device()
+        cfg = ServerConfig(slots, max_batch, memory_budget, bpw_small, bpw_large)
+        h = ctypes.c_void_p()
+        check(lib().tc_server_create(self.device.handle, ctypes.byref(cfg), ctypes.byref(h)), "tc_server_create")
+        self.handle = h
+
+    def submit(self, cs: bytes) -> int:
+        rid = ctypes.c_uint64()
+        check(lib().tc_server_submit(self.handle, cs, len(cs), ctypes.byref(rid)), "submit")
+        return rid.value
+
+    def done(self, rid) -> bool:
+        d = ctypes.c_int(0)
+        check(lib().tc_server_poll(self.handle, rid, ctypes.byref(d)), "poll")
+        return bool(d.value)
+
+    def wait(self, rid, timeout=60.0) -> bool:
+        st = lib().tc_server_wait(self.handle, rid, timeout)
+        if st == 9:
+            return False
+        check(st, "wait")
+        return True
+
+    def info(self, rid) -> RequestInfo:
+        i = RequestInfo()
+        check(lib().tc_server_result(self.handle, rid, None, 0, ctypes.byref(i)), "result info")
+        return i
+
+    def result(self, rid):
+        """(image array, RequestInfo); releases the request."""
+        i = self.info(rid)
+        if i.status != 0:
+            check(lib().tc_server_result(self.handle, rid, ctypes.c_void_p(1), 0, None), "request")
+        dt = np.uint8 if i.bytes_per_sample == 1 else np.uint16
+        out = np.empty((i.height, i.width, i.ncomp), dt)
+        check(lib().tc_server_result(self.handle, rid, out.ctypes.data, out.nbytes, None), "result")
+        return out, i
+
+    def now(self) -> float:
+        return lib().tc_server_now(self.handle)
+
+    def stats(self) -> ServerStats:
+        s = ServerStats()
+        check(lib().tc_server_get_stats(self.handle, ctypes.byref(s)), "stats")
+        return s
+
+    def close(self):
+        if getattr(self, "handle", None):
+            lib().tc_server_destroy(self.handle)
+            self.handle = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
 
 
 class DeviceBuffer:

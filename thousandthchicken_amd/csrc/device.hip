@@ -280,6 +280,13 @@ tc_status tc_event_sync(tc_event e) {
     return hip_fail(hipEventSynchronize(e->e), "hipEventSynchronize");
 }
 
+tc_status tc_event_query(tc_event e) {
+    if (!e) return fail(TC_ERR_INVALID_ARG, "tc_event_query: null");
+    hipError_t r = hipEventQuery(e->e);
+    if (r == hipErrorNotReady) return TC_ERR_BUSY;
+    return hip_fail(r, "hipEventQuery");
+}
+
 tc_status tc_event_elapsed_ms(tc_event a, tc_event b, float* ms) {
     if (!a || !b || !ms) return fail(TC_ERR_INVALID_ARG, "tc_event_elapsed_ms: null");
     return hip_fail(hipEventElapsedTime(ms, a->e, b->e), "hipEventElapsedTime");
