@@ -58,6 +58,8 @@ tc_status tc_server_wait(tc_server srv, uint64_t id, double timeout_s);
 /* Copies the decoded image (interleaved, as tc_decoder_decode) and releases the
  * request.  out may be NULL to read only info (the request is then kept). */
 tc_status tc_server_result(tc_server srv, uint64_t id, void *out, size_t cap, tc_request_info *info);
+/* Drops a finished request and its result without copying it. */
+tc_status tc_server_release(tc_server srv, uint64_t id);
 tc_status tc_server_get_stats(tc_server srv, tc_server_stats *stats);
 double tc_server_now(tc_server srv);
 

@@ -1,0 +1,106 @@
+"""Decode server (continuous batching): every result equals the same request
+decoded alone, admission respects the per-slot memory share, a broken request
+fails alone, and an open-loop Poisson run completes every request."""
+import numpy as np
+import pytest
+
+import thousandthchicken_amd as tca
+from oracle import oracle
+from thousandthchicken_amd import serve, synth, workloads
+
+pytestmark = pytest.mark.gpu
+
+
+@pytest.fixture(scope="module")
+def streams():
+    s = workloads.cfg4(n=20, scale=0.12)
+    big = synth.encode(synth.image(300, 520, 1, 77, 16), bits=16, levels=6, cb=64)
+    return s + [big]
+
+
+def _alone(streams):
+    srv = tca.Server(slots=1, max_batch=1)
+    out = []
+    for cs in streams:
+        rid = srv.submit(cs)
+        assert srv.wait(rid, 60.0)
+        img, info = srv.result(rid)
+        assert info.batch_size == 1
+        out.append(img)
+    srv.close()
+    return out
+
+
+def test_alone_equals_mix(streams):
+    print(f"server alone vs mix, {len(streams)} requests", flush=True)
+    alone = _alone(streams)
+    srv = tca.Server(slots=3, max_batch=64)
+    ids = [srv.submit(cs) for cs in streams]
+    infos = []
+    for i, rid in enumerate(ids):
+        assert srv.wait(rid, 60.0)
+        img, info = srv.result(rid)
+        infos.append(info)
+        assert img.dtype == alone[i].dtype and img.shape == alone[i].shape
+        assert np.array_equal(img, alone[i]), i
+    assert max(x.batch_size for x in infos) > 1, "requests were never batched together"
+    st = srv.stats()
+    assert st.completed == len(streams) and st.failed == 0
+    srv.close()
+    # and both agree with the oracle (lossless exact, lossy within 1 LSB)
+    lay = workloads.cfg4_layout(20)
+    for i, cs in enumerate(streams):
+        ref = oracle.decode(cs).transpose(1, 2, 0)
+        lim = 0 if (i >= 20 or lay[i]["lossless"]) else 1
+        assert np.abs(alone[i].astype(np.int32) - ref).max() <= lim, i
+
+
+def test_admission_by_footprint(streams):
+    small = streams[:12]
+    fp = []
+    for cs in small:
+        srv = tca.Server(slots=1, max_batch=1)
+        rid = srv.submit(cs)
+        assert srv.wait(rid, 60.0)
+        fp.append(srv.info(rid).footprint)
+        srv.release(rid)
+        srv.close()
+    share = 2 * max(fp)
+    srv = tca.Server(slots=2, max_batch=256, memory_budget=2 * share)
+    ids = [srv.submit(cs) for cs in small]
+    batches = {}
+    for rid in ids:
+        assert srv.wait(rid, 60.0)
+        info = srv.info(rid)
+        batches.setdefault(info.batch, []).append(info.footprint)
+        srv.release(rid)
+    for b, f in batches.items():
+        assert len(f) == 1 or sum(f) <= share, (b, f)
+    assert srv.stats().peak_bytes <= 2 * share
+    srv.close()
+
+
+def test_bad_request_fails_alone(streams):
+    srv = tca.Server(slots=2)
+    good = srv.submit(streams[0])
+    bad = srv.submit(b"\xff\x4f\xff\x51" + b"\x00" * 40)
+    assert srv.wait(good, 60.0) and srv.wait(bad, 60.0)
+    assert srv.info(bad).status != 0
+    with pytest.raises(tca.TCError):
+        srv.result(bad)
+    img, info = srv.result(good)
+    assert info.status == 0 and img.size > 0
+    srv.close()
+
+
+def test_poisson_run(streams):
+    srv = tca.Server(slots=3)
+    order = serve.mix(streams[:20], streams[20:], 60, 0.1, seed=9)
+    seq = [streams[i] if i >= 0 else streams[20 + (-1 - i)] for i in order]
+    r = serve.run(srv, seq, rate=400.0, seed=9, keep={0, 5, 17})
+    assert r["failed"] == 0 and r["requests"] == 60
+    assert r["p99_ms"] >= r["p50_ms"] > 0
+    for i, img in r["images"].items():
+        ref = oracle.decode(seq[i]).transpose(1, 2, 0)
+        assert np.abs(img.astype(np.int32) - ref).max() <= 1
+    srv.close()

@@ -94,6 +94,7 @@ def lib():
         "tc_server_wait": (i32, [vp, ctypes.c_uint64, ctypes.c_double]),
         "tc_server_result": (i32, [vp, ctypes.c_uint64, vp, sz, vp]),
         "tc_server_get_stats": (i32, [vp, vp]),
+        "tc_server_release": (i32, [vp, ctypes.c_uint64]),
         "tc_server_now": (ctypes.c_double, [vp]),
         "tc_stream_destroy": (i32, [vp]),
         "tc_stream_sync": (i32, [vp]),
@@ -258,11 +259,15 @@ class Server:
         """(image array, RequestInfo); releases the request."""
         i = self.info(rid)
         if i.status != 0:
-            check(lib().tc_server_result(self.handle, rid, ctypes.c_void_p(1), 0, None), "request")
+            self.release(rid)
+            check(i.status, f"request {rid}")
         dt = np.uint8 if i.bytes_per_sample == 1 else np.uint16
         out = np.empty((i.height, i.width, i.ncomp), dt)
         check(lib().tc_server_result(self.handle, rid, out.ctypes.data, out.nbytes, None), "result")
         return out, i
+
+    def release(self, rid):
+        check(lib().tc_server_release(self.handle, rid), "release")
 
     def now(self) -> float:
         return lib().tc_server_now(self.handle)

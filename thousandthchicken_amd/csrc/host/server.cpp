@@ -328,6 +328,16 @@ tc_status tc_server_result(tc_server s, uint64_t id, void* out, size_t cap, tc_r
     return TC_SUCCESS;
 }
 
+tc_status tc_server_release(tc_server s, uint64_t id) {
+    if (!s) return fail(TC_ERR_INVALID_ARG, "tc_server_release: null");
+    std::lock_guard<std::mutex> lk(s->mu);
+    auto it = s->reqs.find(id);
+    if (it == s->reqs.end()) return fail(TC_ERR_NOT_FOUND, "tc_server_release: unknown request");
+    if (!it->second->done) return fail(TC_ERR_BUSY, "tc_server_release: not finished");
+    s->reqs.erase(it);
+    return TC_SUCCESS;
+}
+
 tc_status tc_server_get_stats(tc_server s, tc_server_stats* out) {
     if (!s || !out) return fail(TC_ERR_INVALID_ARG, "tc_server_get_stats: null");
     std::lock_guard<std::mutex> lk(s->mu);
