@@ -1,14 +1,26 @@
 """Decode throughput benchmark (decoded Mpixel/s on one MI355X).
 
-Default workload: one 2048x2048 3-component 8-bit image from the seeded
+Headline workload: one 2048x2048 3-component 8-bit image from the seeded
 recipe (seed 2), coded lossy with ICT + 5-level 9/7, expounded quantization,
-64x64 code-blocks, one tile, one layer truncated to about 1.0 bpp.  A step
-decodes it from the device-resident codestream to the interleaved 8-bit
-image in HBM (fill, tier-1, inverse DWT levels, fused final level).
+64x64 code-blocks, one tile, one layer truncated to about 1.0 bpp (the
+package's encoder).  A step decodes it from the device-resident codestream to
+the interleaved 8-bit image in HBM: fill, tier-1, inverse DWT levels, fused
+final level (dequantization, ICT, DC shift, rounding, packing).
 
-Prints one JSON line (rank 0).
+Beside the headline line the same JSON object carries (unless --quick):
+  per_image  the same image through the per-image entry (host codestream in,
+             host image out, blocking), one call after the other;
+  batched    the mixed batch of 256 images (and the 3840x2160 12-bit tiled
+             image) decoded with one launch per stage, eager and from a
+             captured graph, against the same images decoded one by one;
+  served     Poisson arrivals (90% mixed-batch images, 10% 7680x4320 16-bit)
+             at several offered loads through the decode server, with p50/p99
+             latency, against serial per-image decode of the same requests.
+
+Prints one JSON line.
 """
 import argparse
+import ctypes
 import json
 import os
 import sys
@@ -20,27 +32,22 @@ sys.path.insert(0, ROOT)
 import numpy as np  # noqa: E402
 
 HBM_PEAK_GBS = 8000.0
-
-WORKLOADS = {
-    # name: (height, width, ncomp, bits, seed, lossless, levels, cb, bpp)
-    "cfg1_lossless_512": (512, 512, 3, 8, 1, True, 5, 64, None),
-    "cfg2_lossy_2048": (2048, 2048, 3, 8, 2, False, 5, 64, 1.0),
-}
+TRAFFIC_FILE = os.path.join(ROOT, "profiles", "traffic_t1_wave_kernel_cfg2_lossy_2048.json")
 
 
-def make_codestream(name):
-    from thousandthchicken_amd import synth
-    from tests import imaging
-    h, w, c, bits, seed, lossless, lev, cb, bpp = WORKLOADS[name]
-    src = synth.image(h, w, c, seed, bits)
-    kw = dict(num_resolutions=lev + 1, codeblock_size=(cb, cb), mct=1 if c == 3 else 0)
-    if lossless:
-        cs = imaging.encode_openjpeg(src if c > 1 else src[..., 0], **kw)
-    else:
-        rate = (c * bits) / bpp
-        cs = imaging.encode_openjpeg(src if c > 1 else src[..., 0], irreversible=True, quality_mode="rates",
-                                     quality_layers=[rate], **kw)
-    return src, cs
+def log(msg):
+    print(msg, file=sys.stderr, flush=True)
+
+
+def workload(name):
+    from thousandthchicken_amd import workloads
+    if name == "cfg1_lossless_512":
+        return workloads.cfg1(), workloads.source_image("cfg1")
+    if name == "cfg2_lossy_2048":
+        return workloads.cfg2(), None
+    if name == "cfg3_lossless_4k_tiled":
+        return workloads.cfg3(), workloads.source_image("cfg3")
+    raise SystemExit(f"unknown workload {name}")
 
 
 def cpu_baseline(cs, pixels, budget_s=20.0):
@@ -54,8 +61,175 @@ def cpu_baseline(cs, pixels, budget_s=20.0):
         el = time.perf_counter() - t0
         if el > budget_s / 4 or n >= 3:
             break
-    return {"value": pixels * n / el / 1e6, "unit": "Mpixel/s", "cores": 1, "kind": "port",
+    return {"value": round(pixels * n / el / 1e6, 3), "unit": "Mpixel/s", "cores": 1, "kind": "port",
             "sample": f"{n} full decode(s) of the benchmark codestream by the C oracle, {el:.1f} s"}
+
+
+def check_output(out, cs, src, what):
+    if src is not None:
+        if not np.array_equal(out.reshape(src.shape), src):
+            raise SystemExit(f"{what}: decode is not bit-exact")
+        return
+    from oracle import oracle
+    ref = oracle.decode(cs).transpose(1, 2, 0)
+    dmax = int(np.abs(out.astype(np.int32) - ref).max())
+    if dmax > 1:
+        raise SystemExit(f"{what}: decode differs from the oracle by {dmax} LSB")
+
+
+def stage_times(dec, reps=10):
+    acc = np.zeros(5)
+    for _ in range(reps):
+        dec.run(timed=True)
+        dec.sync()
+        s = dec.stats()
+        acc += [s.fill_ms, s.t1_ms, s.dwt_ms, s.final_ms, s.total_ms]
+    return acc / reps
+
+
+def timed_runs(dec, steps, graph=False):
+    dec.sync()
+    t0 = time.perf_counter()
+    for _ in range(steps):
+        dec.launch_graph() if graph else dec.run()
+    dec.sync()
+    return (time.perf_counter() - t0) / steps
+
+
+def traffic_record():
+    """HBM bytes per tier-1 launch measured with FETCH_SIZE / WRITE_SIZE."""
+    try:
+        with open(TRAFFIC_FILE) as f:
+            return json.load(f)
+    except (OSError, ValueError):
+        return None
+
+
+def bench_batched(tca, steps):
+    from thousandthchicken_amd import workloads
+    res = {}
+    t = time.perf_counter()
+    streams = workloads.cfg4(threads=16)
+    log(f"mixed batch generated in {time.perf_counter() - t:.1f} s")
+    lay = workloads.cfg4_layout()
+    px = 0
+    serial_dec = tca.Decoder()
+    outs = []
+    for cs in streams[:4]:
+        serial_dec.decode(cs)
+    t0 = time.perf_counter()
+    for cs in streams:
+        o = serial_dec.decode(cs)
+        outs.append(o)
+        px += o.shape[0] * o.shape[1]
+    serial_s = time.perf_counter() - t0
+    serial_dec.close()
+    dec = tca.Decoder()
+    dec.prepare(streams)
+    dec.run()
+    dec.sync()
+    lossy_diff = 0
+    for i in range(len(streams)):
+        d = int(np.abs(dec.fetch(i).astype(np.int32) - outs[i]).max())
+        if d > (0 if lay[i]["lossless"] else 1):
+            raise SystemExit(f"mixed batch image {i}: batched differs from per-image by {d}")
+        lossy_diff = max(lossy_diff, d)
+    st = stage_times(dec, 3)
+    eager = timed_runs(dec, steps)
+    dec.capture_graph()
+    dec.launch_graph()
+    graph = timed_runs(dec, steps, graph=True)
+    s = dec.stats()
+    res["cfg4_mixed_256"] = {
+        "images": len(streams), "mpixel": round(px / 1e6, 2), "cblks": s.cblks,
+        "mpixel_s_eager": round(px / eager / 1e6, 1), "mpixel_s_graph": round(px / graph / 1e6, 1),
+        "ms_per_batch": round(graph * 1e3, 3),
+        "stages_ms": {"fill": round(st[0], 3), "tier1": round(st[1], 3), "idwt_levels": round(st[2], 3),
+                      "final_level": round(st[3], 3)},
+        "per_image_serial_mpixel_s": round(px / serial_s / 1e6, 1),
+        "speedup_vs_per_image": round(serial_s / graph, 2),
+        "blocks_per_wave": {"small": s.bpw_small, "large": s.bpw_large},
+        "outputs": "lossless identical to per-image decode",
+        "lossy_max_diff_vs_per_image": lossy_diff}
+    dec.close()
+    # one large tiled image (12-bit, 1024x1024 tiles, 32x32 blocks)
+    cs, src = workload("cfg3_lossless_4k_tiled")
+    dec = tca.Decoder()
+    dec.prepare([cs])
+    dec.run()
+    dec.sync()
+    check_output(dec.fetch(0), cs, src, "cfg3")
+    dec.capture_graph()
+    dec.launch_graph()
+    graph = timed_runs(dec, steps, graph=True)
+    one = tca.Decoder()
+    one.decode(cs)
+    t0 = time.perf_counter()
+    for _ in range(3):
+        one.decode(cs)
+    per = (time.perf_counter() - t0) / 3
+    s = dec.stats()
+    res["cfg3_lossless_4k_tiled"] = {
+        "mpixel_s_graph": round(3840 * 2160 / graph / 1e6, 1), "ms": round(graph * 1e3, 3), "cblks": s.cblks,
+        "per_image_mpixel_s": round(3840 * 2160 / per / 1e6, 1), "bit_exact": True}
+    dec.close()
+    one.close()
+    return res
+
+
+def bench_served(tca, loads, n_req):
+    from thousandthchicken_amd import serve, workloads
+    t = time.perf_counter()
+    small = workloads.cfg4(threads=16)
+    large = [workloads.cfg5_large(5), workloads.cfg5_large(6)]
+    log(f"serving pool generated in {time.perf_counter() - t:.1f} s")
+    order = serve.mix(small, large, n_req, 0.1, seed=11)
+    seq = [small[i] if i >= 0 else large[-1 - i] for i in order]
+    keep = set(range(0, n_req, max(1, n_req // 16)))
+    dec = tca.Decoder()
+    for cs in seq[:3]:
+        dec.decode(cs)
+    ser = serve.serial(dec, seq)
+    ref = {i: dec.decode(seq[i]) for i in keep}
+    dec.close()
+    out = {"requests": n_req, "large_fraction": 0.1, "serial_per_image": ser, "loads": []}
+    for rate in loads:
+        srv = tca.Server(slots=3, max_batch=256)
+        r = serve.run(srv, seq, rate, seed=int(rate), keep=keep)
+        for i, img in r.pop("images").items():
+            d = int(np.abs(img.astype(np.int32) - ref[i]).max())
+            if d > (0 if tca.probe(seq[i]).reversible else 1):
+                raise SystemExit(f"served request {i} differs from its per-image decode by {d}")
+        r["speedup_vs_serial"] = round(r["mpixel_s"] / ser["mpixel_s"], 2)
+        srv.close()
+        out["loads"].append(r)
+        log(f"served {rate} req/s: {r}")
+    out["outputs"] = (f"{len(keep)} sampled requests checked against per-image decode "
+                      "(lossless identical, lossy within 1 LSB)")
+    return out
+
+
+def profile_mode(tca, args):
+    """A few steps of the workload plus a device copy of known size, for counters."""
+    cs, _ = workload(args.workload)
+    dec = tca.Decoder()
+    dec.prepare([cs])
+    for _ in range(args.steps):
+        dec.run()
+    dec.sync()
+    n = 256 << 20
+    L = tca.lib()
+    a, b = ctypes.c_void_p(), ctypes.c_void_p()
+    tca.check(L.tc_buf_alloc(dec.device.handle, n, ctypes.byref(a)))
+    tca.check(L.tc_buf_alloc(dec.device.handle, n, ctypes.byref(b)))
+    st = L.tc_decoder_stream(dec.handle)
+    tca.check(L.tc_buf_fill32(st, a, 0, 1, n // 4))
+    for _ in range(3):
+        tca.check(L.tc_buf_copy_d2d(st, b, 0, a, 0, n))
+    tca.check(L.tc_stream_sync(st))
+    L.tc_buf_free(a)
+    L.tc_buf_free(b)
+    print(json.dumps({"profile": args.workload, "steps": args.steps, "copy_bytes": n}), flush=True)
 
 
 def main():
@@ -63,10 +237,15 @@ def main():
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=50)
     ap.add_argument("--warmup", type=int, default=5)
-    ap.add_argument("--workload", default="cfg2_lossy_2048", choices=sorted(WORKLOADS))
+    ap.add_argument("--workload", default="cfg2_lossy_2048",
+                    choices=["cfg1_lossless_512", "cfg2_lossy_2048", "cfg3_lossless_4k_tiled"])
+    ap.add_argument("--quick", action="store_true", help="headline workload only (no batched/served)")
     ap.add_argument("--no-cpu-baseline", action="store_true")
-    ap.add_argument("--bpw-large", type=int, default=0, help="blocks per wave, blocks up to 64x64 (0 auto, 1 wave/block)")
-    ap.add_argument("--bpw-small", type=int, default=0, help="blocks per wave, blocks up to 32x32 (0 auto, 1 wave/block)")
+    ap.add_argument("--profile", action="store_true", help="workload steps and a calibration copy only")
+    ap.add_argument("--loads", default="100,1000,5000", help="offered loads (req/s) of the serving run")
+    ap.add_argument("--requests", type=int, default=300, help="requests per serving run")
+    ap.add_argument("--bpw-large", type=int, default=0, help="blocks per wave, blocks up to 64x64 (0 auto)")
+    ap.add_argument("--bpw-small", type=int, default=0, help="blocks per wave, blocks up to 32x32 (0 auto)")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py measures one GPU: this build has no multi-GPU path (asked for {args.gpus})",
@@ -74,8 +253,10 @@ def main():
         sys.exit(2)
 
     import thousandthchicken_amd as tca
+    if args.profile:
+        return profile_mode(tca, args)
 
-    src, cs = make_codestream(args.workload)
+    cs, src = workload(args.workload)
     info = tca.probe(cs)
     pixels = info.width * info.height
     dec = tca.Decoder()
@@ -85,43 +266,33 @@ def main():
     for _ in range(args.warmup):
         dec.run()
     dec.sync()
-    # correctness guard on the benchmarked decode
-    out = dec.fetch(0)
-    h, w, c, bits, seed, lossless, lev, cb, bpp = WORKLOADS[args.workload]
-    if lossless and not np.array_equal(out.reshape(src.shape), src):
-        raise SystemExit("benchmark decode is not bit-exact")
-    if not lossless:
-        from oracle import oracle
-        ref = oracle.decode(cs)
-        dmax = int(np.abs(out.transpose(2, 0, 1).astype(np.int32) - ref).max())
-        if dmax > 1:
-            raise SystemExit(f"benchmark decode differs from the oracle by {dmax} LSB")
+    check_output(dec.fetch(0), cs, src, args.workload)
 
-    dec.sync()
-    t0 = time.perf_counter()
-    for _ in range(args.steps):
-        dec.run()
-    dec.sync()
-    el = time.perf_counter() - t0
-    ms = el * 1e3 / args.steps
-
-    # per-stage times from HIP events on the decoder's stream
-    reps = 10
-    acc = np.zeros(5)
-    for _ in range(reps):
-        dec.run(timed=True)
-        dec.sync()
-        s = dec.stats()
-        acc += [s.fill_ms, s.t1_ms, s.dwt_ms, s.final_ms, s.total_ms]
-    acc /= reps
+    step = timed_runs(dec, args.steps)
+    ms = step * 1e3
+    st = stage_times(dec)
     s = dec.stats()
-    t1_bytes = s.codeword_bytes + 4 * s.samples
-    t1_gbs = t1_bytes / (acc[1] * 1e-3) / 1e9
-    dwt_gbs = (s.dwt_bytes + s.final_bytes) / ((acc[2] + acc[3]) * 1e-3) / 1e9 if acc[2] + acc[3] > 0 else 0.0
+    # tier-1 algorithmic bytes per launch: codeword bytes read + 4 B per coefficient written
+    # (the coefficients of the blocks that have coding passes; the rest stay as filled)
+    t1_bytes = s.codeword_bytes + 4 * s.t1_samples
+    t1_gbs = t1_bytes / (st[1] * 1e-3) / 1e9
+    dwt_gbs = (s.dwt_bytes + s.final_bytes) / ((st[2] + st[3]) * 1e-3) / 1e9 if st[2] + st[3] > 0 else 0.0
+    tr = traffic_record()
+    traffic = tr["hbm_bytes_per_launch"] if tr and tr.get("workload") == args.workload else None
+
+    # plain per-image entry (host in, host out), one call after the other
+    per = tca.Decoder()
+    per.decode(cs)
+    t0 = time.perf_counter()
+    n_per = max(5, args.steps // 5)
+    for _ in range(n_per):
+        per.decode(cs)
+    per_s = (time.perf_counter() - t0) / n_per
+    per.close()
 
     line = {
         "metric": "decoded Mpixel/s on 1 MI355X",
-        "value": round(pixels / (ms * 1e-3) / 1e6, 2),
+        "value": round(pixels / step / 1e6, 2),
         "unit": "Mpixel/s",
         "n_gpus": 1,
         "steps": args.steps,
@@ -131,21 +302,30 @@ def main():
         "scaling": "weak",
         "vs_baseline": None,
         "dtype": "int32+fp32",
-        "data": "synthetic (seeded recipe, coded by OpenJPEG 2.5.4)",
+        "data": "synthetic (seeded recipe, coded by the package encoder)",
         "config": {"workload": args.workload, "width": info.width, "height": info.height,
                    "components": info.ncomp, "reversible": bool(info.reversible), "levels": info.nlevels,
                    "codeblock": info.cbw, "codestream_bytes": len(cs),
-                   "bpp": round(len(cs) * 8 / pixels, 3)},
-        "stages_ms": {"fill": round(acc[0], 4), "tier1": round(acc[1], 4), "idwt_levels": round(acc[2], 4),
-                      "final_level": round(acc[3], 4), "total": round(acc[4], 4)},
-        "roofline": {"bound": "hbm", "kernel": "t1_decode_kernel", "achieved": round(t1_gbs, 2),
+                   "bpp": round(len(cs) * 8 / pixels, 3), "inputs": "codestream resident in HBM"},
+        "stages_ms": {"fill": round(st[0], 4), "tier1": round(st[1], 4), "idwt_levels": round(st[2], 4),
+                      "final_level": round(st[3], 4), "total": round(st[4], 4)},
+        "roofline": {"bound": "hbm", "kernel": "t1_wave_kernel", "achieved": round(t1_gbs, 2),
                      "peak": HBM_PEAK_GBS, "unit": "GB/s", "frac": round(t1_gbs / HBM_PEAK_GBS, 5),
-                     "traffic": None, "note": "tier-1 is issue/latency bound; bytes = codewords + 4 B/sample"},
+                     "traffic": traffic, "algorithmic_bytes": t1_bytes,
+                     "note": "tier-1 is bound by the serial MQ symbol chain of the heaviest code-block, "
+                             "not by HBM; bytes = codewords + 4 B per coefficient"},
         "roofline_dwt": {"bound": "hbm", "achieved": round(dwt_gbs, 2), "peak": HBM_PEAK_GBS, "unit": "GB/s",
                          "frac": round(dwt_gbs / HBM_PEAK_GBS, 5)},
         "blocks_per_wave": {"small": s.bpw_small, "large": s.bpw_large},
         "cblks": s.cblks,
+        "per_image": {"mpixel_s": round(pixels / per_s / 1e6, 2), "ms": round(per_s * 1e3, 3),
+                      "note": "host codestream in, host image out, PCIe included"},
     }
+    dec.close()
+    if not args.quick:
+        line["batched"] = bench_batched(tca, max(3, args.steps // 5))
+        loads = [float(x) for x in args.loads.split(",") if x]
+        line["served"] = bench_served(tca, loads, args.requests)
     if not args.no_cpu_baseline:
         line["cpu_baseline"] = cpu_baseline(cs, pixels)
     print(json.dumps(line), flush=True)

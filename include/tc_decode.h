@@ -51,6 +51,7 @@ typedef struct {
     int32_t bpw_small, bpw_large;   /* blocks per wave used (0: class absent) */
     int32_t images;
     int64_t device_bytes;
+    int64_t t1_samples; /* coefficients of the code-blocks tier-1 decodes (blocks with coding passes) */
 } tc_decoder_stats;
 
 /* Main-header information (SIZ/COD/QCD) of a raw codestream or JP2 file. */

@@ -41,7 +41,7 @@ class DecoderStats(ctypes.Structure):
         (n, ctypes.c_int64) for n in ("cblks", "codeword_bytes", "samples", "pixels", "dwt_bytes",
                                       "final_bytes")] + [
         ("bpw_small", ctypes.c_int32), ("bpw_large", ctypes.c_int32), ("images", ctypes.c_int32),
-        ("device_bytes", ctypes.c_int64)]
+        ("device_bytes", ctypes.c_int64), ("t1_samples", ctypes.c_int64)]
 
 
 class DeviceProps(ctypes.Structure):
@@ -104,6 +104,7 @@ def lib():
         "tc_buf_copy_h2d": (i32, [vp, vp, sz, vp, sz]),
         "tc_buf_copy_d2h": (i32, [vp, vp, vp, sz, sz]),
         "tc_buf_fill32": (i32, [vp, vp, sz, ctypes.c_uint32, sz]),
+        "tc_buf_copy_d2d": (i32, [vp, vp, sz, vp, sz, sz]),
         "tc_launch_t1_decode": (i32, [vp, i32, i32, vp, vp, i32, vp, vp]),
         "tc_launch_t1_decode_wave": (i32, [vp, i32, i32, vp, vp, i32, vp, sz, vp]),
     }

@@ -228,6 +228,7 @@ tc_status tc_decoder_get_stats(tc_decoder d, tc_decoder_stats* st) {
     st->cblks = d->pipe->num_cblks();
     st->codeword_bytes = d->pipe->codeword_bytes();
     st->samples = d->pipe->sample_count();
+    st->t1_samples = d->pipe->t1_sample_count();
     st->pixels = d->pipe->pixel_count();
     st->dwt_bytes = d->pipe->dwt_bytes();
     st->final_bytes = d->pipe->final_bytes();

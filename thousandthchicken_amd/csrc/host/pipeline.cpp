@@ -152,6 +152,8 @@ Status Pipeline::prepare(const std::vector<const Image*>& imgs, std::string& err
     cw.resize(((cw.size() + 3) & ~(size_t)3) + 16, 0xFF);  // slack for look-ahead reads
     cw_size_ = cw.size();
     ncblk_ = (int64_t)descs.size();
+    t1_samples_ = 0;
+    for (const auto& d : descs) t1_samples_ += (int64_t)d.w * d.h;
 
     std::vector<uint8_t> blob;
     uint64_t cw_off = put(blob, cw);

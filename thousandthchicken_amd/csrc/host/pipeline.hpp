@@ -64,6 +64,7 @@ class Pipeline {
     int64_t num_cblks() const { return ncblk_; }
     int64_t codeword_bytes() const { return cw_bytes_; }
     int64_t sample_count() const { return nsamples_; }
+    int64_t t1_sample_count() const { return t1_samples_; }
     int64_t pixel_count() const { return npixels_; }
     int blocks_per_wave(int cls) const { return bpw_[cls]; }
     // bytes moved by the HBM-bound stages (algorithmic): dwt levels and final
@@ -112,6 +113,7 @@ class Pipeline {
     std::vector<FinalLaunch> finals_;
     std::vector<ImageOut> images_;
     uint64_t descs_off_ = 0, coef_elems_ = 0, cw_size_ = 0;
+    int64_t t1_samples_ = 0;
     int64_t ncblk_ = 0, cw_bytes_ = 0, nsamples_ = 0, npixels_ = 0, dwt_bytes_ = 0, final_bytes_ = 0;
     int bpw_[2] = {0, 0};
     Tuning tuning_;
