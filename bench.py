@@ -105,7 +105,15 @@ def traffic_record():
         return None
 
 
-def bench_batched(tca, steps):
+def tuned_decoder(tca, args):
+    dec = tca.Decoder()
+    dec.set_blocks_per_wave(False, args.bpw_large)
+    dec.set_blocks_per_wave(True, args.bpw_small)
+    dec.set_t1_kernel(args.t1_kernel)
+    return dec
+
+
+def bench_batched(tca, steps, args):
     from thousandthchicken_amd import workloads
     res = {}
     t = time.perf_counter()
@@ -124,7 +132,7 @@ def bench_batched(tca, steps):
         px += o.shape[0] * o.shape[1]
     serial_s = time.perf_counter() - t0
     serial_dec.close()
-    dec = tca.Decoder()
+    dec = tuned_decoder(tca, args)
     dec.prepare(streams)
     dec.run()
     dec.sync()
@@ -154,7 +162,7 @@ def bench_batched(tca, steps):
     dec.close()
     # one large tiled image (12-bit, 1024x1024 tiles, 32x32 blocks)
     cs, src = workload("cfg3_lossless_4k_tiled")
-    dec = tca.Decoder()
+    dec = tuned_decoder(tca, args)
     dec.prepare([cs])
     dec.run()
     dec.sync()
@@ -177,7 +185,7 @@ def bench_batched(tca, steps):
     return res
 
 
-def bench_served(tca, loads, n_req):
+def bench_served(tca, loads, n_req, args):
     from thousandthchicken_amd import serve, workloads
     t = time.perf_counter()
     small = workloads.cfg4(threads=16)
@@ -194,7 +202,7 @@ def bench_served(tca, loads, n_req):
     dec.close()
     out = {"requests": n_req, "large_fraction": 0.1, "serial_per_image": ser, "loads": []}
     for rate in loads:
-        srv = tca.Server(slots=3, max_batch=256)
+        srv = tca.Server(slots=3, max_batch=256, bpw_small=args.bpw_small, bpw_large=args.bpw_large)
         r = serve.run(srv, seq, rate, seed=int(rate), keep=keep)
         for i, img in r.pop("images").items():
             d = int(np.abs(img.astype(np.int32) - ref[i]).max())
@@ -211,9 +219,13 @@ def bench_served(tca, loads, n_req):
 
 def profile_mode(tca, args):
     """A few steps of the workload plus a device copy of known size, for counters."""
-    cs, _ = workload(args.workload)
-    dec = tca.Decoder()
-    dec.prepare([cs])
+    if args.workload == "cfg4_mixed_256":
+        from thousandthchicken_amd import workloads
+        streams = workloads.cfg4(threads=16)
+    else:
+        streams = [workload(args.workload)[0]]
+    dec = tuned_decoder(tca, args)
+    dec.prepare(streams)
     for _ in range(args.steps):
         dec.run()
     dec.sync()
@@ -238,14 +250,17 @@ def main():
     ap.add_argument("--steps", type=int, default=50)
     ap.add_argument("--warmup", type=int, default=5)
     ap.add_argument("--workload", default="cfg2_lossy_2048",
-                    choices=["cfg1_lossless_512", "cfg2_lossy_2048", "cfg3_lossless_4k_tiled"])
+                    choices=["cfg1_lossless_512", "cfg2_lossy_2048", "cfg3_lossless_4k_tiled", "cfg4_mixed_256"],
+                    help="headline workload (cfg4_mixed_256: --profile only)")
     ap.add_argument("--quick", action="store_true", help="headline workload only (no batched/served)")
+    ap.add_argument("--no-served", action="store_true", help="skip the serving runs")
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--profile", action="store_true", help="workload steps and a calibration copy only")
     ap.add_argument("--loads", default="100,1000,5000", help="offered loads (req/s) of the serving run")
     ap.add_argument("--requests", type=int, default=300, help="requests per serving run")
     ap.add_argument("--bpw-large", type=int, default=0, help="blocks per wave, blocks up to 64x64 (0 auto)")
     ap.add_argument("--bpw-small", type=int, default=0, help="blocks per wave, blocks up to 32x32 (0 auto)")
+    ap.add_argument("--t1-kernel", type=int, default=0, help="one-wave-per-block kernel: 0 stripe-column, 1 row-mask")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py measures one GPU: this build has no multi-GPU path (asked for {args.gpus})",
@@ -259,9 +274,7 @@ def main():
     cs, src = workload(args.workload)
     info = tca.probe(cs)
     pixels = info.width * info.height
-    dec = tca.Decoder()
-    dec.set_blocks_per_wave(False, args.bpw_large)
-    dec.set_blocks_per_wave(True, args.bpw_small)
+    dec = tuned_decoder(tca, args)
     dec.prepare([cs])
     for _ in range(args.warmup):
         dec.run()
@@ -309,7 +322,7 @@ def main():
                    "bpp": round(len(cs) * 8 / pixels, 3), "inputs": "codestream resident in HBM"},
         "stages_ms": {"fill": round(st[0], 4), "tier1": round(st[1], 4), "idwt_levels": round(st[2], 4),
                       "final_level": round(st[3], 4), "total": round(st[4], 4)},
-        "roofline": {"bound": "hbm", "kernel": "t1_wave_kernel", "achieved": round(t1_gbs, 2),
+        "roofline": {"bound": "hbm", "kernel": "t1_stripe_kernel" if args.t1_kernel == 0 else "t1_wave_kernel", "achieved": round(t1_gbs, 2),
                      "peak": HBM_PEAK_GBS, "unit": "GB/s", "frac": round(t1_gbs / HBM_PEAK_GBS, 5),
                      "traffic": traffic, "algorithmic_bytes": t1_bytes,
                      "note": "tier-1 is bound by the serial MQ symbol chain of the heaviest code-block, "
@@ -323,9 +336,10 @@ def main():
     }
     dec.close()
     if not args.quick:
-        line["batched"] = bench_batched(tca, max(3, args.steps // 5))
+        line["batched"] = bench_batched(tca, max(3, args.steps // 5), args)
         loads = [float(x) for x in args.loads.split(",") if x]
-        line["served"] = bench_served(tca, loads, args.requests)
+        if not args.no_served:
+            line["served"] = bench_served(tca, loads, args.requests, args)
     if not args.no_cpu_baseline:
         line["cpu_baseline"] = cpu_baseline(cs, pixels)
     print(json.dumps(line), flush=True)

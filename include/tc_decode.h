@@ -62,12 +62,21 @@ tc_status tc_image_probe(const uint8_t *cs, size_t len, tc_image_info *info);
  * written to out (which may be NULL). */
 tc_status tc_image_list_cblks(const uint8_t *cs, size_t len, tc_cblk_info *out, int max, int *count);
 
+/* Codeword bytes of code-block `index` (order of tc_image_list_cblks), its
+ * segments concatenated; *nbytes receives the size (out may be NULL). */
+tc_status tc_image_cblk_data(const uint8_t *cs, size_t len, int index, uint8_t *out, size_t cap, size_t *nbytes);
+
 tc_status tc_decoder_create(tc_device dev, tc_decoder *out);
 tc_status tc_decoder_destroy(tc_decoder dec);
 
 /* Code-block-to-wave packing: blocks per wave (1, 8, 16, 32, 64; 0 = automatic)
  * for blocks up to 32x32 (small_blocks = 1) or up to 64x64 (small_blocks = 0). */
 tc_status tc_decoder_set_blocks_per_wave(tc_decoder dec, int small_blocks, int bpw);
+
+/* Kernel for blocks decoded one per wavefront: 0 = stripe-column form
+ * (tc_launch_t1_decode_stripe, default), 1 = row-mask form
+ * (tc_launch_t1_decode_wave). */
+tc_status tc_decoder_set_t1_kernel(tc_decoder dec, int kind);
 
 /* Per-image decode: parse, upload, decode, download (blocking). */
 tc_status tc_decoder_decode(tc_decoder dec, const uint8_t *cs, size_t len, void *out, size_t out_cap,

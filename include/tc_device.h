@@ -127,6 +127,12 @@ tc_status tc_launch_t1_decode(tc_stream s, int small_blocks, int blocks_per_wave
 tc_status tc_launch_t1_decode_wave(tc_stream s, int small_blocks, int grid, const void *descs, const uint32_t *order,
                                    int n, const uint8_t *arena, size_t arena_bytes, void *coef);
 
+/* Same contract as tc_launch_t1_decode_wave (one wavefront per code-block,
+ * blocks up to 64x64 of any mix), with block state held column-major across
+ * the lanes and the scan restricted to the columns holding candidates. */
+tc_status tc_launch_t1_decode_stripe(tc_stream s, int grid, const void *descs, const uint32_t *order, int n,
+                                     const uint8_t *arena, size_t arena_bytes, void *coef);
+
 /* One inverse DWT level over a job list (run, TC/dwt_r53.cl:183 / TC/dwt_r97.cl:229).
  * jobs: device array of tc_dwt_job; wg_job: work-group -> job map of nwg entries. */
 tc_status tc_launch_idwt_level(tc_stream s, int irreversible, const void *jobs, const uint32_t *wg_job, int nwg,

@@ -41,7 +41,7 @@ def _expected(b):
     return np.where(sign.astype(bool), -v, v).astype(np.int32)
 
 
-@pytest.mark.parametrize("kernel", ["wave", 8, 16, 32, 64])
+@pytest.mark.parametrize("kernel", ["stripe", "wave", 8, 16, 32, 64])
 @pytest.mark.parametrize("small", [True, False])
 def test_t1_kernels_on_random_blocks(kernel, small):
     lim = 32 if small else 64
@@ -67,7 +67,10 @@ def test_t1_kernels_on_random_blocks(kernel, small):
         bufs[0].upload(stream, arena)
         bufs[1].upload(stream, descs)
         tca.check(L.tc_buf_fill32(stream, bufs[2].handle, 0, 0, n_out), "fill")
-        if kernel == "wave":
+        if kernel == "stripe":
+            st = L.tc_launch_t1_decode_stripe(stream, 512, bufs[1].ptr, None, len(blocks), bufs[0].ptr, arena.nbytes,
+                                              bufs[2].ptr)
+        elif kernel == "wave":
             st = L.tc_launch_t1_decode_wave(stream, 1 if small else 0, 512, bufs[1].ptr, None, len(blocks),
                                             bufs[0].ptr, arena.nbytes, bufs[2].ptr)
         else:

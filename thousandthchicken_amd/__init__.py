@@ -75,6 +75,8 @@ def lib():
         "tc_decoder_create": (i32, [vp, P(vp)]),
         "tc_decoder_destroy": (i32, [vp]),
         "tc_decoder_set_blocks_per_wave": (i32, [vp, i32, i32]),
+        "tc_decoder_set_t1_kernel": (i32, [vp, i32]),
+        "tc_image_cblk_data": (i32, [ctypes.c_char_p, sz, i32, vp, sz, P(sz)]),
         "tc_decoder_decode": (i32, [vp, ctypes.c_char_p, sz, vp, sz, P(ImageInfo)]),
         "tc_decoder_prepare": (i32, [vp, i32, vp, vp]),
         "tc_decoder_run": (i32, [vp, i32]),
@@ -98,6 +100,12 @@ def lib():
         "tc_server_now": (ctypes.c_double, [vp]),
         "tc_stream_destroy": (i32, [vp]),
         "tc_stream_sync": (i32, [vp]),
+        "tc_event_create": (i32, [P(vp)]),
+        "tc_event_destroy": (i32, [vp]),
+        "tc_event_record": (i32, [vp, vp]),
+        "tc_event_sync": (i32, [vp]),
+        "tc_event_query": (i32, [vp]),
+        "tc_event_elapsed_ms": (i32, [vp, vp, P(ctypes.c_float)]),
         "tc_buf_alloc": (i32, [vp, sz, P(vp)]),
         "tc_buf_free": (i32, [vp]),
         "tc_buf_device_ptr": (vp, [vp]),
@@ -107,6 +115,7 @@ def lib():
         "tc_buf_copy_d2d": (i32, [vp, vp, sz, vp, sz, sz]),
         "tc_launch_t1_decode": (i32, [vp, i32, i32, vp, vp, i32, vp, vp]),
         "tc_launch_t1_decode_wave": (i32, [vp, i32, i32, vp, vp, i32, vp, sz, vp]),
+        "tc_launch_t1_decode_stripe": (i32, [vp, i32, vp, vp, i32, vp, sz, vp]),
     }
     for name, (res, args) in sigs.items():
         f = getattr(L, name)
@@ -149,6 +158,15 @@ def list_cblks(cs: bytes):
     arr = (CblkInfo * max(n.value, 1))()
     check(lib().tc_image_list_cblks(cs, len(cs), arr, n.value, ctypes.byref(n)), "tc_image_list_cblks")
     return [arr[i] for i in range(n.value)]
+
+
+def cblk_data(cs: bytes, index: int) -> bytes:
+    """Codeword bytes of one code-block (order of list_cblks)."""
+    n = ctypes.c_size_t(0)
+    check(lib().tc_image_cblk_data(cs, len(cs), index, None, 0, ctypes.byref(n)), "tc_image_cblk_data")
+    buf = ctypes.create_string_buffer(max(n.value, 1))
+    check(lib().tc_image_cblk_data(cs, len(cs), index, buf, n.value, ctypes.byref(n)), "tc_image_cblk_data")
+    return buf.raw[:n.value]
 
 
 class Device:
@@ -340,6 +358,9 @@ class Decoder:
             self.close()
         except Exception:
             pass
+
+    def set_t1_kernel(self, kind):
+        check(lib().tc_decoder_set_t1_kernel(self.handle, kind), "set_t1_kernel")
 
     def set_blocks_per_wave(self, small, bpw):
         check(lib().tc_decoder_set_blocks_per_wave(self.handle, 1 if small else 0, bpw), "set_blocks_per_wave")

@@ -18,6 +18,7 @@ SOURCES = [
     "device.hip",
     "kernels/t1_decode.hip",
     "kernels/t1_wave.hip",
+    "kernels/t1_stripe.hip",
     "kernels/idwt.hip",
     "kernels/colour.hip",
     "host/codestream.cpp",

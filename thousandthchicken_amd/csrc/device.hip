@@ -14,6 +14,8 @@ hipError_t launch_t1(hipStream_t s, bool cls32, int g, const tc_cblk_desc* descs
                      const uint8_t* arena, uint32_t* coef);
 hipError_t launch_t1_wave(hipStream_t s, bool narrow, int grid, const tc_cblk_desc* descs, const uint32_t* order,
                           int n, const uint8_t* arena, uint32_t arena_bytes, uint32_t* coef);
+hipError_t launch_t1_stripe(hipStream_t s, int grid, const tc_cblk_desc* descs, const uint32_t* order, int n,
+                            const uint8_t* arena, uint32_t arena_bytes, uint32_t* coef);
 hipError_t launch_idwt_level(hipStream_t s, bool is97, const tc_dwt_job* jobs, const uint32_t* wg_job, int nwg,
                              void* base);
 hipError_t launch_idwt_final(hipStream_t s, bool is97, int nc, int out_bytes, const tc_final_job* jobs,
@@ -344,6 +346,16 @@ tc_status tc_launch_t1_decode_wave(tc_stream s, int small_blocks, int grid, cons
                                    (uint32_t)arena_bytes,
                                    (uint32_t*)coef),
                     "t1_decode_wave launch");
+}
+
+tc_status tc_launch_t1_decode_stripe(tc_stream s, int grid, const void* descs, const uint32_t* order, int n,
+                                     const uint8_t* arena, size_t arena_bytes, void* coef) {
+    if (!s || n < 0 || grid <= 0 || (n > 0 && (!descs || !arena || !coef)) || (arena_bytes & 3) ||
+        arena_bytes > 0xFFFFFFFFull || ((uintptr_t)arena & 3))
+        return fail(TC_ERR_INVALID_ARG, "tc_launch_t1_decode_stripe");
+    return hip_fail(launch_t1_stripe(s->s, grid, (const tc_cblk_desc*)descs, order, n, arena, (uint32_t)arena_bytes,
+                                     (uint32_t*)coef),
+                    "t1_decode_stripe launch");
 }
 
 tc_status tc_launch_idwt_level(tc_stream s, int irr, const void* jobs, const uint32_t* wg_job, int nwg, void* base) {

@@ -122,6 +122,24 @@ tc_status tc_image_list_cblks(const uint8_t* cs, size_t len, tc_cblk_info* out, 
     return TC_SUCCESS;
 }
 
+tc_status tc_image_cblk_data(const uint8_t* cs, size_t len, int index, uint8_t* out, size_t cap, size_t* nbytes) {
+    if (!cs || !nbytes || index < 0) return report(Status::InvalidArgument, "tc_image_cblk_data: args");
+    Image im;
+    std::string err;
+    Status s = parse_codestream(cs, len, im, err);
+    if (s != Status::Ok) return report(s, err);
+    if ((size_t)index >= im.cblks.size()) return report(Status::InvalidArgument, "tc_image_cblk_data: index");
+    size_t n = 0;
+    for (const Segment& g : im.segs)
+        if (g.cblk == (uint32_t)index) {
+            if (out && n + g.len <= cap) std::memcpy(out + n, im.cs + g.off, g.len);
+            n += g.len;
+        }
+    *nbytes = n;
+    if (out && n > cap) return report(Status::InvalidArgument, "tc_image_cblk_data: buffer too small");
+    return TC_SUCCESS;
+}
+
 tc_status tc_decoder_create(tc_device dev, tc_decoder* out) {
     if (!dev || !out) return report(Status::InvalidArgument, "tc_decoder_create: null");
     auto d = new tc_decoder_s();
@@ -145,6 +163,14 @@ tc_status tc_decoder_set_blocks_per_wave(tc_decoder d, int small, int bpw) {
         return report(Status::InvalidArgument, "blocks per wave must be 0 (auto), 1 (one wave per block), 8, 16, 32 or 64");
     Tuning t = d->pipe->tuning();
     (small ? t.bpw32 : t.bpw64) = bpw;
+    d->pipe->set_tuning(t);
+    return TC_SUCCESS;
+}
+
+tc_status tc_decoder_set_t1_kernel(tc_decoder d, int kind) {
+    if (!d || kind < 0 || kind > 1) return report(Status::InvalidArgument, "t1 kernel must be 0 (stripe) or 1 (row)");
+    Tuning t = d->pipe->tuning();
+    t.t1_kernel = kind;
     d->pipe->set_tuning(t);
     return TC_SUCCESS;
 }

@@ -347,7 +347,10 @@ Status Pipeline::run(bool timed) {
     if (timed) tc_event_record(ev_[1], stream_);
     for (const Group& g : groups_) {
         if (s != TC_SUCCESS) break;
-        if (g.bpw == 1)
+        if (g.bpw == 1 && tuning_.t1_kernel == 0)
+            s = tc_launch_t1_decode_stripe(stream_, std::min(g.n, device_cu_count(dev_) * 16), up + descs_off_,
+                                           (const uint32_t*)(up + g.order_off), g.n, up, cw_size_, base);
+        else if (g.bpw == 1)
             s = tc_launch_t1_decode_wave(stream_, g.small ? 1 : 0, std::min(g.n, device_cu_count(dev_) * 16), up + descs_off_,
                                          (const uint32_t*)(up + g.order_off), g.n, up, cw_size_, base);
         else

@@ -26,6 +26,7 @@ namespace tc {
 struct Tuning {
     int bpw64 = 0;  // blocks per wave for blocks up to 64x64 (0 = pick automatically)
     int bpw32 = 0;  // blocks per wave for blocks up to 32x32 (0 = pick automatically)
+    int t1_kernel = 0;  // one-wave-per-block kernel: 0 stripe-column form, 1 row-mask form
 };
 
 // Chooses blocks-per-wave from the block count and the machine size; the

@@ -1,0 +1,627 @@
+// Tier-1 decoding, one wavefront per code-block, stripe-column form.
+//
+// Reference behaviour: g_decode (TC/coefficient_coder.cl:778-819) with its
+// passes (:255-700) as T.800 Annex D defines them, the pass count honoured and
+// the MQ decoder of Annex C (C.3).
+//
+// Mapping.  The MQ decoder and the scan run on the scalar unit: registers,
+// context states of the refinement, run-length and uniform contexts, and the
+// loop state are wave-uniform.  Block state is column-major in vector
+// registers: lane x holds 64-bit masks of column x (bit y = row y) for
+// significance, sign, visited-in-this-plane, refined-before and the current
+// plane's 1-bits.  For each stripe and pass the 64 lanes build, in parallel,
+// a per-column word (the 3-column x 6-row significance window, candidate rows,
+// neighbour signs, refinement labels); the scalar loop then visits only the
+// columns that hold candidates, reading each column word with one v_readlane,
+// carries new significance from column x to x+1 in scalar registers, and
+// leaves its per-column results in one lane write.  After the stripe the lanes
+// fold the results back into the masks.  The zero-coding and sign-coding
+// tables, the 17 zero/sign/refinement context states and the MQ successor
+// table are 64-entry lane arrays.  The MQ decoder's renormalising path is one
+// scalar block that shifts by min(needed, CT) per step (T.800 C.3.3/C.3.4
+// semantics, bit-exact), so the register state never moves between paths.
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+#include "../tc_kernels.h"
+#include "t1_common.hpp"
+
+namespace tc {
+namespace {
+
+__device__ int lane_write_i32(int val, int lane, int old) __asm("llvm.amdgcn.writelane.i32");
+
+__device__ __forceinline__ uint32_t rl(uint32_t v, uint32_t i) {
+    return (uint32_t)__builtin_amdgcn_readlane((int)v, (int)i);
+}
+__device__ __forceinline__ uint32_t wl(uint32_t v, uint32_t i, uint32_t val) {
+    return (uint32_t)lane_write_i32((int)val, (int)i, (int)v);
+}
+__device__ __forceinline__ uint32_t uni(uint32_t v) { return (uint32_t)__builtin_amdgcn_readfirstlane((int)v); }
+
+// context state: Qe << 16 | I << 1 | MPS
+__device__ __forceinline__ uint32_t st_pack(uint32_t I, uint32_t mps) {
+    return ((uint32_t)kQe[I] << 16) | (I << 1) | mps;
+}
+
+struct Mq {
+    uint32_t C;      // the C register of C.3
+    uint32_t A;      // the A register << 16
+    uint32_t CT;     // shifts left before BYTEIN
+    uint32_t cur;    // the byte at BP
+    uint32_t np;     // BP + 1
+    uint32_t wc;     // cached codeword dword
+    uint32_t wci;    // its dword index (0xFFFFFFFF: none)
+    uint32_t len;    // codeword bytes; later bytes read as 0xFF
+    const uint32_t* words;
+};
+
+__device__ __forceinline__ uint32_t mq_byte(Mq& m, uint32_t i) {
+    if (i >= m.len) return 0xFFu;
+    const uint32_t wi = i >> 2;
+    if (wi != m.wci) {
+        m.wc = m.words[wi];
+        m.wci = wi;
+    }
+    return (m.wc >> ((i & 3u) * 8u)) & 0xFFu;
+}
+
+// BYTEIN (C.3.4)
+__device__ __forceinline__ void mq_bytein(Mq& m) {
+    const uint32_t b1 = mq_byte(m, m.np);
+    if (m.cur == 0xFFu) {
+        if (b1 > 0x8Fu) {
+            m.C += 0xFF00u;
+            m.CT = 8;
+        } else {
+            m.np++;
+            m.cur = b1;
+            m.C += b1 << 9;
+            m.CT = 7;
+        }
+    } else {
+        m.np++;
+        m.cur = b1;
+        m.C += b1 << 8;
+        m.CT = 8;
+    }
+}
+
+// INITDEC (C.3.5)
+__device__ __forceinline__ void mq_init(Mq& m, const uint32_t* words, uint32_t len) {
+    m.words = words;
+    m.len = len;
+    m.wci = 0xFFFFFFFFu;
+    m.cur = mq_byte(m, 0);
+    m.np = 1;
+    m.C = m.cur << 16;
+    mq_bytein(m);
+    m.C <<= 7;
+    m.CT -= 7;
+    m.A = 0x80000000u;
+}
+
+// DECODE (C.3.2) with the conditional exchanges (C.3.3), the successor state
+// and RENORMD/BYTEIN (C.3.3/C.3.4), RENORMD shifting min(needed, CT) bits per
+// step (the same bits as one at a time).  One scalar block, so the decoder
+// state keeps its registers on every path; both successor states are read
+// first so their latency overlaps the interval arithmetic (the fast path reads
+// them back before leaving, so no lane read is still in flight).  st (Qe << 16 | I << 1 | MPS) is
+// updated in place; trm/trl: lane I holds the packed successor state of I
+// after an MPS / LPS (the LPS one with SWITCH in bit 0).  The codeword dword
+// load waits for itself inside the block.
+__device__ __forceinline__ uint32_t mq_dec(Mq& m, uint32_t& st, uint32_t trm, uint32_t trl) {
+    uint32_t d, q, t0, t1, t2, t3, n;
+    asm volatile(
+        "s_bfe_u32 %[t0], %[st], 0x60001\n\t"
+        "s_and_b32 %[q], %[st], 0xffff0000\n\t"
+        "s_and_b32 %[d], %[st], 1\n\t"
+        "s_sub_u32 %[A], %[A], %[q]\n\t"
+        "s_nop 1\n\t"
+        "v_readlane_b32 %[t2], %[trm], %[t0]\n\t"
+        "v_readlane_b32 %[t3], %[trl], %[t0]\n\t"
+        "s_cmp_lt_u32 %[C], %[q]\n\t"
+        "s_cbranch_scc1 L%=_lower\n\t"
+        "s_sub_u32 %[C], %[C], %[q]\n\t"
+        "s_cmp_lt_i32 %[A], 0\n\t"
+        "s_cbranch_scc1 L%=_fast\n\t"
+        // upper sub-interval, renormalising: LPS exchange when A < Qe
+        "s_cmp_lt_u32 %[A], %[q]\n\t"
+        "s_cselect_b32 %[t2], %[t3], %[t2]\n\t"
+        "s_cselect_b32 %[t1], 1, 0\n"
+        "L%=_renorm:\n\t"
+        "s_xor_b32 %[st], %[t2], %[d]\n\t"
+        "s_xor_b32 %[d], %[d], %[t1]\n\t"
+        "s_flbit_i32_b32 %[n], %[A]\n\t"
+        "s_cmp_lt_u32 %[CT], %[n]\n\t"
+        "s_cbranch_scc1 L%=_loop\n\t"
+        "s_lshl_b32 %[C], %[C], %[n]\n\t"
+        "s_lshl_b32 %[A], %[A], %[n]\n\t"
+        "s_sub_u32 %[CT], %[CT], %[n]\n\t"
+        "s_branch L%=_end\n"
+        // lower sub-interval: MPS exchange when A < Qe, then A = Qe
+        "L%=_lower:\n\t"
+        "s_cmp_lt_u32 %[A], %[q]\n\t"
+        "s_cselect_b32 %[t2], %[t2], %[t3]\n\t"
+        "s_cselect_b32 %[t1], 0, 1\n\t"
+        "s_mov_b32 %[A], %[q]\n\t"
+        "s_branch L%=_renorm\n"
+        "L%=_fast:\n\t"
+        "s_or_b32 %[t2], %[t2], %[t3]\n\t"
+        "s_branch L%=_end\n"
+        // RENORMD with BYTEIN when CT runs out
+        "L%=_loop:\n\t"
+        "s_cmp_eq_u32 %[CT], 0\n\t"
+        "s_cbranch_scc0 L%=_shift\n\t"
+        "s_cmp_ge_u32 %[np], %[len]\n\t"
+        "s_mov_b32 %[t0], 0xff\n\t"
+        "s_cbranch_scc1 L%=_got\n\t"
+        "s_lshr_b32 %[t1], %[np], 2\n\t"
+        "s_cmp_eq_u32 %[t1], %[wci]\n\t"
+        "s_cbranch_scc1 L%=_cached\n\t"
+        "s_mov_b32 %[wci], %[t1]\n\t"
+        "s_lshl_b32 %[t1], %[t1], 2\n\t"
+        "s_load_dword %[wc], %[wp], %[t1]\n\t"
+        "s_waitcnt lgkmcnt(0)\n"
+        "L%=_cached:\n\t"
+        "s_and_b32 %[t1], %[np], 3\n\t"
+        "s_lshl_b32 %[t1], %[t1], 3\n\t"
+        "s_lshr_b32 %[t0], %[wc], %[t1]\n\t"
+        "s_and_b32 %[t0], %[t0], 0xff\n"
+        "L%=_got:\n\t"
+        "s_cmp_eq_u32 %[cur], 0xff\n\t"
+        "s_cbranch_scc0 L%=_plain\n\t"
+        "s_cmp_gt_u32 %[t0], 0x8f\n\t"
+        "s_cbranch_scc0 L%=_stuff\n\t"
+        "s_add_u32 %[C], %[C], 0xff00\n\t"
+        "s_mov_b32 %[CT], 8\n\t"
+        "s_branch L%=_shift\n"
+        "L%=_stuff:\n\t"
+        "s_add_u32 %[np], %[np], 1\n\t"
+        "s_mov_b32 %[cur], %[t0]\n\t"
+        "s_lshl_b32 %[t0], %[t0], 9\n\t"
+        "s_add_u32 %[C], %[C], %[t0]\n\t"
+        "s_mov_b32 %[CT], 7\n\t"
+        "s_branch L%=_shift\n"
+        "L%=_plain:\n\t"
+        "s_add_u32 %[np], %[np], 1\n\t"
+        "s_mov_b32 %[cur], %[t0]\n\t"
+        "s_lshl_b32 %[t0], %[t0], 8\n\t"
+        "s_add_u32 %[C], %[C], %[t0]\n\t"
+        "s_mov_b32 %[CT], 8\n"
+        "L%=_shift:\n\t"
+        "s_min_u32 %[t1], %[n], %[CT]\n\t"
+        "s_lshl_b32 %[C], %[C], %[t1]\n\t"
+        "s_lshl_b32 %[A], %[A], %[t1]\n\t"
+        "s_sub_u32 %[CT], %[CT], %[t1]\n\t"
+        "s_sub_u32 %[n], %[n], %[t1]\n\t"
+        "s_cmp_eq_u32 %[n], 0\n\t"
+        "s_cbranch_scc0 L%=_loop\n"
+        "L%=_end:"
+        : [A] "+s"(m.A), [C] "+s"(m.C), [CT] "+s"(m.CT), [st] "+s"(st), [cur] "+s"(m.cur), [np] "+s"(m.np),
+          [wc] "+s"(m.wc), [wci] "+s"(m.wci), [d] "=&s"(d), [q] "=&s"(q), [t0] "=&s"(t0), [t1] "=&s"(t1),
+          [t2] "=&s"(t2), [t3] "=&s"(t3), [n] "=&s"(n)
+        : [len] "s"(m.len), [wp] "s"(m.words), [trm] "v"(trm), [trl] "v"(trl)
+        : "scc");
+    return d;
+}
+
+// Zero-coding label (T.800 Table D.1) of a 9-bit window (centre ignored),
+// computed per lane from the neighbour counts.
+__device__ __forceinline__ uint32_t zc_lab_lane(uint32_t nb, int cls) {
+    uint32_t h = (uint32_t)__popc(nb & 0x28u), v = (uint32_t)__popc(nb & 0x82u);
+    const uint32_t dg = (uint32_t)__popc(nb & 0x145u);
+    if (cls == 1) {
+        const uint32_t t = h;
+        h = v;
+        v = t;
+    }
+    if (cls < 2) {
+        if (h == 2) return 8;
+        if (h == 1) return v ? 7 : (dg ? 6 : 5);
+        if (v == 2) return 4;
+        if (v == 1) return 3;
+        return dg >= 2 ? 2 : dg;
+    }
+    const uint32_t hv = h + v;
+    if (dg >= 3) return 8;
+    if (dg == 2) return hv ? 7 : 6;
+    if (dg == 1) return hv >= 2 ? 5 : (hv ? 4 : 3);
+    return hv >= 2 ? 2 : hv;
+}
+
+struct BlockArgs {
+    uint32_t data_off, data_len;
+    uint64_t out_off;
+    uint32_t out_pitch;
+    int w, h, numbps, npasses, mode, cls;
+    float delta;
+};
+
+__device__ __forceinline__ BlockArgs load_args(const tc_cblk_desc* __restrict__ dp) {
+    const uint32_t* q = reinterpret_cast<const uint32_t*>(dp);  // 8 dwords, layout of tc_cblk_desc
+    BlockArgs a;
+    a.data_off = uni(q[0]);
+    a.data_len = uni(q[1]);
+    a.out_off = (uint64_t)uni(q[2]) | ((uint64_t)uni(q[3]) << 32);
+    a.out_pitch = uni(q[4]);
+    const uint32_t wh = uni(q[5]), pk = uni(q[6]);
+    a.w = (int)(wh & 0xFFFF);
+    a.h = (int)(wh >> 16);
+    a.numbps = (int)(pk & 0xFF);
+    a.npasses = (int)((pk >> 8) & 0xFF);
+    a.cls = (int)((pk >> 16) & 0xFF);
+    a.mode = (int)((pk >> 24) & 0xFF);
+    a.delta = __uint_as_float(uni(q[7]));
+    return a;
+}
+
+// bits 0..5 of v to bits 0,3,...,15
+__device__ __forceinline__ uint32_t spread3(uint32_t v) {
+    return (v & 1u) | ((v & 2u) << 2) | ((v & 4u) << 4) | ((v & 8u) << 6) | ((v & 16u) << 8) | ((v & 32u) << 10);
+}
+
+// rows y0-1 .. y0+4 of a column mask as a 6-bit value (row y0-1 in bit 0)
+__device__ __forceinline__ uint32_t win6(uint64_t m, int y0) {
+    return y0 == 0 ? (uint32_t)(m << 1) & 63u : (uint32_t)(m >> (y0 - 1)) & 63u;
+}
+
+__device__ __forceinline__ uint32_t from_left(uint32_t v) {
+    const uint32_t u = (uint32_t)__shfl_up((int)v, 1, 64);
+    return threadIdx.x == 0 ? 0u : u;
+}
+__device__ __forceinline__ uint32_t from_right(uint32_t v) {
+    const uint32_t u = (uint32_t)__shfl_down((int)v, 1, 64);
+    return threadIdx.x == 63 ? 0u : u;
+}
+
+// Decoder state of one block.  Scalar members are wave-uniform; the lane
+// arrays hold one entry per lane.
+struct Blk {
+    Mq m;
+    uint32_t ctxv;            // lane l < 17: state of zero/sign/refinement context l
+    uint32_t cl, cs;          // context used last and its current state
+    uint32_t st_rl, st_uni;
+    uint32_t zcv, scv, trm, trl;
+    uint64_t sig, sgn, pip, ref, ones;  // lane x: column x
+    int w, h, cls;
+
+    // The state of the context used last stays in a scalar register (cl, cs)
+    // and goes back to its lane when another context is used.
+    __device__ __forceinline__ uint32_t dec_ctx(uint32_t lab) {
+        if (lab != cl) {
+            ctxv = wl(ctxv, cl, cs);
+            cs = rl(ctxv, lab);
+            cl = lab;
+        }
+        return mq_dec(m, cs, trm, trl);
+    }
+    __device__ __forceinline__ uint32_t zc(uint32_t nb9) const {
+        return (rl(zcv, nb9 >> 3) >> ((nb9 & 7u) << 2)) & 15u;
+    }
+    // sign of the sample at stripe row r that just became significant.
+    // sl: significance window (3 bits per row, left/own/right), ls: left
+    // column signs rows 0..3, os: own column signs rows -1..4, rs: right
+    // column signs rows 0..3.
+    __device__ __forceinline__ uint32_t dec_sign(uint32_t sl, int r, uint32_t ls, uint32_t os, uint32_t rs) {
+        const uint32_t s = sl >> (3 * r);
+        const uint32_t idx = ((s >> 3) & 1u) | (((ls >> r) & 1u) << 1) | (((s >> 5) & 1u) << 2) |
+                             (((rs >> r) & 1u) << 3) | (((s >> 1) & 1u) << 4) | (((os >> r) & 1u) << 5) |
+                             (((s >> 7) & 1u) << 6) | (((os >> (r + 2)) & 1u) << 7);
+        const uint32_t e = (rl(scv, idx >> 2) >> ((idx & 3u) << 3)) & 0xFFu;
+        return dec_ctx(e & 15u) ^ (e >> 4);
+    }
+
+    // Per-column words of a stripe for the significance and cleanup passes.
+    // w0: window (18 bits) | candidate rows << 18; w1: signs (own rows -1..4,
+    // left rows 0..3 << 6, right rows 0..3 << 10) | zero-coding labels of rows
+    // 0..3 from the window at the start of the stripe << 14 (4 bits each).
+    __device__ __forceinline__ void column_words(int y0, uint32_t rowm, bool cleanup, uint32_t& w0, uint32_t& w1,
+                                                 uint64_t& cols) const {
+        const uint32_t so = win6(sig, y0);
+        const uint32_t sl = from_left(so), sr = from_right(so);
+        const uint32_t slice = spread3(sl) | (spread3(so) << 1) | (spread3(sr) << 2);
+        const uint32_t s4 = (so >> 1) & 15u;
+        const uint32_t p4 = (uint32_t)(pip >> y0) & 15u;
+        uint32_t cand;
+        if (cleanup) {
+            cand = ~s4 & ~p4 & rowm;
+        } else {
+            uint32_t nz = 0;
+#pragma unroll
+            for (int r = 0; r < 4; r++) nz |= (((slice >> (3 * r)) & 0x1EFu) != 0u ? 1u : 0u) << r;
+            cand = ~s4 & nz & rowm;
+        }
+        if ((int)threadIdx.x >= w) cand = 0;  // columns outside the block
+        w0 = slice | (cand << 18);
+        const uint32_t go = win6(sgn, y0);
+        const uint32_t g4 = (go >> 1) & 15u;
+        w1 = go | (from_left(g4) << 6) | (from_right(g4) << 10);
+        cols = __ballot(cand != 0u);
+        if (cols) {  // zero-coding labels of the four rows from the window as it is now
+            uint32_t labs = 0;
+#pragma unroll
+            for (int r = 0; r < 4; r++) labs |= zc_lab_lane((slice >> (3 * r)) & 0x1FFu, cls) << (4 * r);
+            w1 |= labs << 14;
+        }
+    }
+
+    // significance propagation over one stripe (T.800 D.3.1)
+    __device__ __forceinline__ void spp(int y0, int nr) {
+        const uint32_t rowm = (1u << nr) - 1u;
+        uint32_t vw0, vw1;
+        uint64_t cols;
+        column_words(y0, rowm, false, vw0, vw1, cols);
+        uint32_t res = 0;
+        int x = -1;
+        uint32_t carry = 0, csgn = 0;  // new significance of column x: window bits / signs by row
+        for (;;) {
+            int nx;
+            if (carry != 0u && x + 1 < w) {
+                nx = x + 1;
+            } else {
+                const uint64_t rem = x >= 63 ? 0ull : cols & (~0ull << (x + 1));
+                if (!rem) break;
+                nx = (int)__builtin_ctzll(rem);
+            }
+            const bool adj = nx == x + 1;
+            const uint32_t lc = adj ? carry : 0u, ls = adj ? csgn : 0u;
+            x = nx;
+            const uint32_t w0 = rl(vw0, x), w1 = rl(vw1, x);
+            uint32_t sl = (w0 & 0x3FFFFu) | lc;
+            const uint32_t lsg = ((w1 >> 6) & 15u) | ls, rsg = (w1 >> 10) & 15u;
+            uint32_t osg = w1 & 63u;
+            uint32_t vis = 0, nsig = 0;
+            carry = 0;
+            csgn = 0;
+            for (int r = 0; r < nr; r++) {
+                const uint32_t nb = (sl >> (3 * r)) & 0x1FFu;
+                if ((nb & 0x10u) || !(nb & 0x1EFu)) continue;
+                vis |= 1u << r;
+                const uint32_t lab = ((sl ^ w0) >> (3 * r)) & 0x1FFu ? zc(nb) : (w1 >> (14 + 4 * r)) & 15u;
+                if (dec_ctx(lab)) {
+                    const uint32_t s = dec_sign(sl, r, lsg, osg, rsg);
+                    sl |= 1u << (3 * r + 4);
+                    osg |= s << (r + 1);
+                    nsig |= 1u << r;
+                    carry |= 1u << (3 * r + 3);
+                    csgn |= s << r;
+                }
+            }
+            if (vis) res = wl(res, (uint32_t)x, vis | (nsig << 4) | (csgn << 8));
+        }
+        const uint64_t nv = (uint64_t)(res & 15u) << y0, ns = (uint64_t)((res >> 4) & 15u) << y0;
+        pip |= nv;
+        sig |= ns;
+        ones |= ns;
+        sgn |= (uint64_t)((res >> 8) & 15u) << y0;
+    }
+
+    // magnitude refinement over one stripe (T.800 D.3.3)
+    __device__ __forceinline__ void mrp(int y0, int nr) {
+        const uint32_t rowm = (1u << nr) - 1u;
+        const uint32_t so = win6(sig, y0);
+        const uint32_t sl = from_left(so), sr = from_right(so);
+        const uint32_t slice = spread3(sl) | (spread3(so) << 1) | (spread3(sr) << 2);
+        const uint32_t s4 = (so >> 1) & 15u;
+        const uint32_t p4 = (uint32_t)(pip >> y0) & 15u;
+        const uint32_t r4 = (uint32_t)(ref >> y0) & 15u;
+        uint32_t nz = 0;
+#pragma unroll
+        for (int r = 0; r < 4; r++) nz |= (((slice >> (3 * r)) & 0x1EFu) != 0u ? 1u : 0u) << r;
+        const uint32_t cand = (int)threadIdx.x < w ? s4 & ~p4 & rowm : 0u;
+        uint32_t labs = 0;
+#pragma unroll
+        for (int r = 0; r < 4; r++) labs |= (((r4 >> r) & 1u) ? 2u : ((nz >> r) & 1u)) << (4 * r);
+        const uint32_t vinfo = cand | (labs << 4);
+        uint64_t cols = __ballot(cand != 0u);
+        uint32_t res = 0;
+        while (cols) {
+            const int x = (int)__builtin_ctzll(cols);
+            cols &= cols - 1;
+            const uint32_t iw = rl(vinfo, (uint32_t)x);
+            uint32_t c = iw & 15u, one = 0;
+            while (c) {
+                const int r = (int)__builtin_ctz(c);
+                c &= c - 1u;
+                // label 16 if refined before, else 15 with a significant neighbour, else 14
+                one |= dec_ctx(14u + ((iw >> (4 + 4 * r)) & 15u)) << r;
+            }
+            if (one) res = wl(res, (uint32_t)x, one);
+        }
+        ref |= (uint64_t)cand << y0;
+        ones |= (uint64_t)(res & 15u) << y0;
+    }
+
+    // cleanup with run-length mode over one stripe (T.800 D.3.4)
+    __device__ __forceinline__ void cup(int y0, int nr) {
+        const uint32_t rowm = (1u << nr) - 1u;
+        uint32_t vw0, vw1;
+        uint64_t cols;
+        column_words(y0, rowm, true, vw0, vw1, cols);
+        uint32_t res = 0;
+        int lastx = -2;
+        uint32_t carry = 0, csgn = 0;
+        while (cols) {
+            const int x = (int)__builtin_ctzll(cols);
+            cols &= cols - 1;
+            const bool adj = x == lastx + 1;
+            const uint32_t lc = adj ? carry : 0u, ls = adj ? csgn : 0u;
+            lastx = x;
+            carry = 0;
+            csgn = 0;
+            const uint32_t w0 = rl(vw0, (uint32_t)x), w1 = rl(vw1, (uint32_t)x);
+            uint32_t sl = (w0 & 0x3FFFFu) | lc;
+            const uint32_t cand = w0 >> 18;
+            const uint32_t lsg = ((w1 >> 6) & 15u) | ls, rsg = (w1 >> 10) & 15u;
+            uint32_t osg = w1 & 63u;
+            uint32_t nsig = 0;
+            int r0 = 0;
+            if (cand == 15u && sl == 0u) {
+                if (!mq_dec(m, st_rl, trm, trl)) continue;
+                int r = (int)mq_dec(m, st_uni, trm, trl) << 1;
+                r |= (int)mq_dec(m, st_uni, trm, trl);
+                const uint32_t s = dec_sign(sl, r, lsg, osg, rsg);
+                sl |= 1u << (3 * r + 4);
+                osg |= s << (r + 1);
+                nsig |= 1u << r;
+                carry |= 1u << (3 * r + 3);
+                csgn |= s << r;
+                r0 = r + 1;
+            }
+            for (int r = r0; r < nr; r++) {
+                if (!((cand >> r) & 1u)) continue;
+                const uint32_t nb = (sl >> (3 * r)) & 0x1FFu;
+                const uint32_t lab = ((sl ^ w0) >> (3 * r)) & 0x1FFu ? zc(nb) : (w1 >> (14 + 4 * r)) & 15u;
+                if (dec_ctx(lab)) {
+                    const uint32_t s = dec_sign(sl, r, lsg, osg, rsg);
+                    sl |= 1u << (3 * r + 4);
+                    osg |= s << (r + 1);
+                    nsig |= 1u << r;
+                    carry |= 1u << (3 * r + 3);
+                    csgn |= s << r;
+                }
+            }
+            if (nsig) res = wl(res, (uint32_t)x, nsig | (csgn << 8));
+        }
+        const uint64_t ns = (uint64_t)(res & 15u) << y0;
+        sig |= ns;
+        ones |= ns;
+        sgn |= (uint64_t)((res >> 8) & 15u) << y0;
+    }
+};
+
+__device__ __forceinline__ void decode_block(const BlockArgs d, const uint32_t* __restrict__ arena_w,
+                                             uint32_t arena_words, uint32_t* __restrict__ coef,
+                                             uint64_t (*planes)[64], uint64_t* sgncols, uint32_t zcv,
+                                             uint32_t scv, uint32_t trm, uint32_t trl, uint32_t ctx0) {
+    const int lane = threadIdx.x;
+    const int w = d.w, h = d.h;
+    const int numbps = d.numbps, npasses = d.npasses;
+
+    Blk B;
+    B.w = w;
+    B.h = h;
+    B.cls = d.cls;
+    B.ctxv = ctx0;
+    B.cl = 0;
+    B.cs = rl(ctx0, 0);
+    B.st_rl = st_pack(3, 0);
+    B.st_uni = st_pack(46, 0);
+    B.zcv = zcv;
+    B.scv = scv;
+    B.trm = trm;
+    B.trl = trl;
+    B.sig = B.sgn = B.pip = B.ref = B.ones = 0;
+    const uint32_t w0 = d.data_off >> 2;
+    // bytes past the arena read as 0xFF, like bytes past the codeword
+    const uint32_t avail = arena_words > w0 ? (arena_words - w0) * 4u : 0u;
+    mq_init(B.m, arena_w + w0, d.data_len < avail ? d.data_len : avail);
+
+    for (int k = 0; k < npasses; k++) {
+        const int p = numbps - 1 - (k + 2) / 3;
+        if (p < 0) break;
+        const int type = k == 0 ? 2 : (k - 1) % 3;  // 0 sigprop, 1 magref, 2 cleanup
+        for (int y0 = 0; y0 < h; y0 += 4) {
+            const int nr = h - y0 < 4 ? h - y0 : 4;
+            if (type == 0)
+                B.spp(y0, nr);
+            else if (type == 1)
+                B.mrp(y0, nr);
+            else
+                B.cup(y0, nr);
+        }
+        // a plane ends after its cleanup pass, or when the passes run out
+        if (type == 2 || k == npasses - 1) {
+            planes[p][lane] = B.ones;
+            B.ones = 0;
+            B.pip = 0;
+        }
+    }
+    sgncols[lane] = B.sgn;
+    __syncthreads();
+
+    // rebuild, dequantize and store the block's rows
+    const int kl = npasses - 1;
+    int plast = numbps - 1 - (kl + 2) / 3;
+    if (plast < 0) plast = 0;
+    const int tlast = kl == 0 ? 2 : (kl - 1) % 3;
+    const uint32_t sigonly = 1u << plast;
+    int span = 1;
+    while (span < w) span <<= 1;
+    const int rows_per = 64 / span;
+    const int lx = lane % span, ly = lane / span;
+    uint32_t* out = coef + d.out_off;
+    if (lx < w) {
+        const uint64_t sc = sgncols[lx];
+        for (int y = ly; y < h; y += rows_per) {
+            uint32_t mag = 0;
+            for (int pp = plast; pp < numbps; pp++) mag |= (uint32_t)((planes[pp][lx] >> y) & 1u) << pp;
+            uint32_t v = 0;
+            if (mag) {
+                const bool neg = ((sc >> y) & 1u) != 0;
+                const int q = (tlast == 0 && mag != sigonly) ? plast + 1 : plast;
+                if (d.mode == 0) {
+                    const int32_t iv = (int32_t)(mag + (q > 0 ? (1u << (q - 1)) : 0u));
+                    v = (uint32_t)(neg ? -iv : iv);
+                } else {
+                    const float f = ((float)mag + ldexpf(0.5f, q)) * d.delta;
+                    v = __float_as_uint(neg ? -f : f);
+                }
+            }
+            out[(uint32_t)y * d.out_pitch + lx] = v;
+        }
+    }
+    __syncthreads();
+}
+
+__global__ __launch_bounds__(64) void t1_stripe_kernel(const tc_cblk_desc* __restrict__ descs,
+                                                       const uint32_t* __restrict__ order, int n,
+                                                       const uint32_t* __restrict__ arena_w, uint32_t arena_words,
+                                                       uint32_t* __restrict__ coef) {
+    __shared__ uint64_t planes[32][64];
+    __shared__ uint64_t sgncols[64];
+    const int lane = threadIdx.x;
+    // lane tables: zero-coding labels for the 9-bit windows lane*8 .. lane*8+7
+    // (4 bits each, one word per band class), sign entries for the sign
+    // indices lane*4 .. lane*4+3 (8 bits each), MQ successor states
+    uint32_t zc3[3];
+#pragma unroll
+    for (int c = 0; c < 3; c++) {
+        uint32_t v = 0;
+        for (int b = 0; b < 8; b++) {
+            const uint32_t i9 = (uint32_t)(lane * 8 + b);
+            const uint32_t n8 = (i9 & 0xFu) | ((i9 >> 5) << 4);  // drop the centre bit
+            v |= zc_label(c, n8) << (4 * b);
+        }
+        zc3[c] = v;
+    }
+    uint32_t scv = 0;
+    for (int b = 0; b < 4; b++) scv |= sc_entry((uint32_t)(lane * 4 + b)) << (8 * b);
+    const uint32_t trm = lane < 47 ? st_pack(kNmps[lane], 0) : 0u;
+    const uint32_t trl =
+        lane < 47 ? (st_pack(kNlps[lane], 0) | ((lane == 0 || lane == 6 || lane == 14) ? 1u : 0u)) : 0u;
+    const uint32_t ctx0 = st_pack(lane == 0 ? 4 : 0, 0);
+    for (int b = blockIdx.x; b < n; b += gridDim.x) {
+        const uint32_t id = order ? uni(order[b]) : (uint32_t)b;
+        const BlockArgs d = load_args(descs + id);
+        if (d.numbps <= 0 || d.npasses <= 0 || d.numbps > 32 || d.w <= 0 || d.h <= 0 || d.w > 64 || d.h > 64)
+            continue;
+        const uint32_t zcv = d.cls == 0 ? zc3[0] : (d.cls == 1 ? zc3[1] : zc3[2]);
+        decode_block(d, arena_w, arena_words, coef, planes, sgncols, zcv, scv, trm, trl, ctx0);
+    }
+}
+
+}  // namespace
+
+hipError_t launch_t1_stripe(hipStream_t s, int grid, const tc_cblk_desc* descs, const uint32_t* order, int n,
+                            const uint8_t* arena, uint32_t arena_bytes, uint32_t* coef) {
+    if (n <= 0) return hipSuccess;
+    if (grid > n) grid = n;
+    if (grid < 1) grid = 1;
+    hipLaunchKernelGGL(t1_stripe_kernel, dim3(grid), dim3(64), 0, s, descs, order, n, (const uint32_t*)arena,
+                       arena_bytes / 4, coef);
+    return hipGetLastError();
+}
+
+}  // namespace tc
