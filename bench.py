@@ -96,6 +96,13 @@ def timed_runs(dec, steps, graph=False):
     return (time.perf_counter() - t0) / steps
 
 
+def t1_kernel_name(stats, args):
+    bpw = stats.bpw_large or stats.bpw_small
+    if bpw > 1:
+        return f"t1_decode_kernel<{bpw}>"
+    return "t1_stripe_kernel" if args.t1_kernel == 0 else "t1_wave_kernel"
+
+
 def traffic_record():
     """HBM bytes per tier-1 launch measured with FETCH_SIZE / WRITE_SIZE."""
     try:
@@ -134,6 +141,7 @@ def bench_batched(tca, steps, args):
     serial_dec.close()
     dec = tuned_decoder(tca, args)
     dec.prepare(streams)
+    tuning4 = None if args.no_autotune else dec.autotune(reps=1).as_dict()
     dec.run()
     dec.sync()
     lossy_diff = 0
@@ -157,6 +165,7 @@ def bench_batched(tca, steps, args):
         "per_image_serial_mpixel_s": round(px / serial_s / 1e6, 1),
         "speedup_vs_per_image": round(serial_s / graph, 2),
         "blocks_per_wave": {"small": s.bpw_small, "large": s.bpw_large},
+        "autotuned": tuning4,
         "outputs": "lossless identical to per-image decode",
         "lossy_max_diff_vs_per_image": lossy_diff}
     dec.close()
@@ -164,6 +173,7 @@ def bench_batched(tca, steps, args):
     cs, src = workload("cfg3_lossless_4k_tiled")
     dec = tuned_decoder(tca, args)
     dec.prepare([cs])
+    tuning3 = None if args.no_autotune else dec.autotune(reps=2).as_dict()
     dec.run()
     dec.sync()
     check_output(dec.fetch(0), cs, src, "cfg3")
@@ -179,7 +189,8 @@ def bench_batched(tca, steps, args):
     s = dec.stats()
     res["cfg3_lossless_4k_tiled"] = {
         "mpixel_s_graph": round(3840 * 2160 / graph / 1e6, 1), "ms": round(graph * 1e3, 3), "cblks": s.cblks,
-        "per_image_mpixel_s": round(3840 * 2160 / per / 1e6, 1), "bit_exact": True}
+        "per_image_mpixel_s": round(3840 * 2160 / per / 1e6, 1), "bit_exact": True,
+        "autotuned": tuning3}
     dec.close()
     one.close()
     return res
@@ -256,11 +267,12 @@ def main():
     ap.add_argument("--no-served", action="store_true", help="skip the serving runs")
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--profile", action="store_true", help="workload steps and a calibration copy only")
-    ap.add_argument("--loads", default="100,1000,5000", help="offered loads (req/s) of the serving run")
+    ap.add_argument("--loads", default="50,100,1000,5000", help="offered loads (req/s) of the serving run")
     ap.add_argument("--requests", type=int, default=300, help="requests per serving run")
     ap.add_argument("--bpw-large", type=int, default=0, help="blocks per wave, blocks up to 64x64 (0 auto)")
     ap.add_argument("--bpw-small", type=int, default=0, help="blocks per wave, blocks up to 32x32 (0 auto)")
     ap.add_argument("--t1-kernel", type=int, default=0, help="one-wave-per-block kernel: 0 stripe-column, 1 row-mask")
+    ap.add_argument("--no-autotune", action="store_true", help="keep the default packing and DWT tile shape")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py measures one GPU: this build has no multi-GPU path (asked for {args.gpus})",
@@ -276,6 +288,7 @@ def main():
     pixels = info.width * info.height
     dec = tuned_decoder(tca, args)
     dec.prepare([cs])
+    tuning = None if args.no_autotune else dec.autotune(reps=2).as_dict()
     for _ in range(args.warmup):
         dec.run()
     dec.sync()
@@ -291,7 +304,9 @@ def main():
     t1_gbs = t1_bytes / (st[1] * 1e-3) / 1e9
     dwt_gbs = (s.dwt_bytes + s.final_bytes) / ((st[2] + st[3]) * 1e-3) / 1e9 if st[2] + st[3] > 0 else 0.0
     tr = traffic_record()
-    traffic = tr["hbm_bytes_per_launch"] if tr and tr.get("workload") == args.workload else None
+    traffic = (tr["hbm_bytes_per_launch"]
+               if tr and tr.get("workload") == args.workload and tr.get("kernel", "").startswith(t1_kernel_name(s, args))
+               else None)
 
     # plain per-image entry (host in, host out), one call after the other
     per = tca.Decoder()
@@ -322,7 +337,7 @@ def main():
                    "bpp": round(len(cs) * 8 / pixels, 3), "inputs": "codestream resident in HBM"},
         "stages_ms": {"fill": round(st[0], 4), "tier1": round(st[1], 4), "idwt_levels": round(st[2], 4),
                       "final_level": round(st[3], 4), "total": round(st[4], 4)},
-        "roofline": {"bound": "hbm", "kernel": "t1_stripe_kernel" if args.t1_kernel == 0 else "t1_wave_kernel", "achieved": round(t1_gbs, 2),
+        "roofline": {"bound": "hbm", "kernel": t1_kernel_name(s, args), "achieved": round(t1_gbs, 2),
                      "peak": HBM_PEAK_GBS, "unit": "GB/s", "frac": round(t1_gbs / HBM_PEAK_GBS, 5),
                      "traffic": traffic, "algorithmic_bytes": t1_bytes,
                      "note": "tier-1 is bound by the serial MQ symbol chain of the heaviest code-block, "
@@ -330,6 +345,7 @@ def main():
         "roofline_dwt": {"bound": "hbm", "achieved": round(dwt_gbs, 2), "peak": HBM_PEAK_GBS, "unit": "GB/s",
                          "frac": round(dwt_gbs / HBM_PEAK_GBS, 5)},
         "blocks_per_wave": {"small": s.bpw_small, "large": s.bpw_large},
+        "autotuned": tuning,
         "cblks": s.cblks,
         "per_image": {"mpixel_s": round(pixels / per_s / 1e6, 2), "ms": round(per_s * 1e3, 3),
                       "note": "host codestream in, host image out, PCIe included"},

@@ -78,6 +78,22 @@ tc_status tc_decoder_set_blocks_per_wave(tc_decoder dec, int small_blocks, int b
  * (tc_launch_t1_decode_wave). */
 tc_status tc_decoder_set_t1_kernel(tc_decoder dec, int kind);
 
+/* Launch configuration of a decoder (applies from the next prepare). */
+typedef struct {
+    int32_t bpw_small, bpw_large; /* blocks per wave per block class: 0 auto, 1 one wave per block, 8..64 */
+    int32_t t1_kernel;            /* one-wave-per-block kernel: 0 stripe, 1 row-mask */
+    int32_t dwt_shape;            /* inverse-DWT tile shape, see tc_idwt_tile_shape */
+    float t1_ms, dwt_ms;          /* stage times the autotuner measured for this choice */
+} tc_tuning;
+
+tc_status tc_decoder_get_tuning(tc_decoder dec, tc_tuning *out);
+tc_status tc_decoder_set_tuning(tc_decoder dec, const tc_tuning *in);
+
+/* Autotuner: times the prepared batch under candidate code-block-to-wave
+ * packings (per block class) and inverse-DWT tile shapes, keeps the fastest,
+ * and re-prepares with it.  *best (may be NULL) receives the choice. */
+tc_status tc_decoder_autotune(tc_decoder dec, int reps, tc_tuning *best);
+
 /* Per-image decode: parse, upload, decode, download (blocking). */
 tc_status tc_decoder_decode(tc_decoder dec, const uint8_t *cs, size_t len, void *out, size_t out_cap,
                             tc_image_info *info);

@@ -133,15 +133,21 @@ tc_status tc_launch_t1_decode_wave(tc_stream s, int small_blocks, int grid, cons
 tc_status tc_launch_t1_decode_stripe(tc_stream s, int grid, const void *descs, const uint32_t *order, int n,
                                      const uint8_t *arena, size_t arena_bytes, void *coef);
 
+/* Output tile shapes of the inverse DWT launches: shape 0..*count-1 covers
+ * *w x *h output samples per work-group (the workgroup-window choice of
+ * DWTKernel::enqueue, TC/DWTKernel.cpp:89-125, made by the autotuner here). */
+tc_status tc_idwt_tile_shape(int shape, int *w, int *h, int *count);
+
 /* One inverse DWT level over a job list (run, TC/dwt_r53.cl:183 / TC/dwt_r97.cl:229).
- * jobs: device array of tc_dwt_job; wg_job: work-group -> job map of nwg entries. */
-tc_status tc_launch_idwt_level(tc_stream s, int irreversible, const void *jobs, const uint32_t *wg_job, int nwg,
-                               void *base);
+ * jobs: device array of tc_dwt_job built for tile shape `shape`; wg_job:
+ * work-group -> job map of nwg entries. */
+tc_status tc_launch_idwt_level(tc_stream s, int irreversible, int shape, const void *jobs, const uint32_t *wg_job,
+                               int nwg, void *base);
 
 /* Final level fused with inverse RCT/ICT, DC shift, rounding, clamping and packing
  * (tcr_kernel / tci_kernel / idc_level_shift_kernel, TC/preprocess_*_inverse.cl).
  * ncomp: 1 or 3; out_bytes: 1 (8-bit) or 2 (>8-bit). */
-tc_status tc_launch_idwt_final(tc_stream s, int irreversible, int ncomp, int out_bytes, const void *jobs,
+tc_status tc_launch_idwt_final(tc_stream s, int irreversible, int shape, int ncomp, int out_bytes, const void *jobs,
                                const uint32_t *wg_job, int nwg, const void *base, void *out);
 
 /* Standalone inverse colour transform / DC shift over int32 planes in place, for the

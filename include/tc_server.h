@@ -28,6 +28,7 @@ typedef struct {
     int32_t max_batch;        /* most requests per batch, default 256 */
     uint64_t memory_budget;   /* device bytes for all slots together, default 32 GiB */
     int32_t bpw_small, bpw_large; /* tier-1 packing (0: automatic) */
+    int32_t dwt_shape;            /* inverse-DWT tile shape (tc_idwt_tile_shape) */
 } tc_server_config;
 
 typedef struct {

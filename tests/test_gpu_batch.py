@@ -53,3 +53,48 @@ def test_batch_graph_replay_matches_eager(gpu_decoder):
         dec.run()  # eager work between replays
         dec.sync()
     dec.close()
+
+
+def _shape_count():
+    import ctypes
+    n = ctypes.c_int(0)
+    tca.check(tca.lib().tc_idwt_tile_shape(0, None, None, ctypes.byref(n)), "shapes")
+    return n.value
+
+
+@pytest.mark.parametrize("shape", range(4))
+def test_dwt_tile_shapes(gpu_decoder, shape):
+    assert _shape_count() == 4
+    streams = _streams()
+    print(f"dwt tile shape {shape}", flush=True)
+    dec = tca.Decoder()
+    t = dec.tuning()
+    t.dwt_shape = shape
+    dec.set_tuning(t)
+    dec.prepare(streams)
+    dec.run()
+    dec.sync()
+    for i, cs in enumerate(streams):
+        ref = oracle.decode(cs).transpose(1, 2, 0)
+        lim = 0 if tca.probe(cs).reversible else 1
+        assert np.abs(dec.fetch(i).astype(np.int32) - ref).max() <= lim, (shape, i)
+    dec.close()
+
+
+def test_autotune_keeps_outputs(gpu_decoder):
+    streams = _streams()
+    dec = tca.Decoder()
+    dec.prepare(streams)
+    dec.run()
+    dec.sync()
+    before = [dec.fetch(i) for i in range(len(streams))]
+    best = dec.autotune(reps=1)
+    print("autotune:", best.as_dict(), flush=True)
+    assert best.bpw_small in (1, 8, 16, 32, 64) and best.bpw_large in (1, 8, 16, 32, 64)
+    assert 0 <= best.dwt_shape < 4 and best.t1_ms > 0
+    dec.run()
+    dec.sync()
+    for i, cs in enumerate(streams):
+        lim = 0 if tca.probe(cs).reversible else 1
+        assert np.abs(dec.fetch(i).astype(np.int32) - before[i].astype(np.int32)).max() <= lim
+    dec.close()

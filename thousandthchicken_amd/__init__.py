@@ -76,6 +76,9 @@ def lib():
         "tc_decoder_destroy": (i32, [vp]),
         "tc_decoder_set_blocks_per_wave": (i32, [vp, i32, i32]),
         "tc_decoder_set_t1_kernel": (i32, [vp, i32]),
+        "tc_decoder_autotune": (i32, [vp, i32, vp]),
+        "tc_decoder_get_tuning": (i32, [vp, vp]),
+        "tc_decoder_set_tuning": (i32, [vp, vp]),
         "tc_image_cblk_data": (i32, [ctypes.c_char_p, sz, i32, vp, sz, P(sz)]),
         "tc_decoder_decode": (i32, [vp, ctypes.c_char_p, sz, vp, sz, P(ImageInfo)]),
         "tc_decoder_prepare": (i32, [vp, i32, vp, vp]),
@@ -116,6 +119,7 @@ def lib():
         "tc_launch_t1_decode": (i32, [vp, i32, i32, vp, vp, i32, vp, vp]),
         "tc_launch_t1_decode_wave": (i32, [vp, i32, i32, vp, vp, i32, vp, sz, vp]),
         "tc_launch_t1_decode_stripe": (i32, [vp, i32, vp, vp, i32, vp, sz, vp]),
+        "tc_idwt_tile_shape": (i32, [i32, P(i32), P(i32), P(i32)]),
     }
     for name, (res, args) in sigs.items():
         f = getattr(L, name)
@@ -223,9 +227,17 @@ CBLK_DESC = np.dtype([("data_off", "<u4"), ("data_len", "<u4"), ("out_off", "<u8
 assert CBLK_DESC.itemsize == 32
 
 
+class Tuning(ctypes.Structure):
+    _fields_ = [("bpw_small", ctypes.c_int32), ("bpw_large", ctypes.c_int32), ("t1_kernel", ctypes.c_int32),
+                ("dwt_shape", ctypes.c_int32), ("t1_ms", ctypes.c_float), ("dwt_ms", ctypes.c_float)]
+
+    def as_dict(self):
+        return {k: (round(getattr(self, k), 4) if k.endswith("_ms") else getattr(self, k)) for k, _ in self._fields_}
+
+
 class ServerConfig(ctypes.Structure):
     _fields_ = [("slots", ctypes.c_int32), ("max_batch", ctypes.c_int32), ("memory_budget", ctypes.c_uint64),
-                ("bpw_small", ctypes.c_int32), ("bpw_large", ctypes.c_int32)]
+                ("bpw_small", ctypes.c_int32), ("bpw_large", ctypes.c_int32), ("dwt_shape", ctypes.c_int32)]
 
 
 class RequestInfo(ctypes.Structure):
@@ -245,9 +257,9 @@ class Server:
     """Continuous-batching decode server over the C ABI (include/tc_server.h)."""
 
     def __init__(self, device: "Device" = None, slots=3, max_batch=256, memory_budget=32 << 30, bpw_small=0,
-                 bpw_large=0):
+                 bpw_large=0, dwt_shape=0):
         self.device = device or default_device()
-        cfg = ServerConfig(slots, max_batch, memory_budget, bpw_small, bpw_large)
+        cfg = ServerConfig(slots, max_batch, memory_budget, bpw_small, bpw_large, dwt_shape)
         h = ctypes.c_void_p()
         check(lib().tc_server_create(self.device.handle, ctypes.byref(cfg), ctypes.byref(h)), "tc_server_create")
         self.handle = h
@@ -358,6 +370,20 @@ class Decoder:
             self.close()
         except Exception:
             pass
+
+    def autotune(self, reps=3) -> "Tuning":
+        """Times the prepared batch under candidate packings and DWT tile shapes; keeps the fastest."""
+        t = Tuning()
+        check(lib().tc_decoder_autotune(self.handle, reps, ctypes.byref(t)), "autotune")
+        return t
+
+    def tuning(self) -> "Tuning":
+        t = Tuning()
+        check(lib().tc_decoder_get_tuning(self.handle, ctypes.byref(t)), "get_tuning")
+        return t
+
+    def set_tuning(self, t: "Tuning"):
+        check(lib().tc_decoder_set_tuning(self.handle, ctypes.byref(t)), "set_tuning")
 
     def set_t1_kernel(self, kind):
         check(lib().tc_decoder_set_t1_kernel(self.handle, kind), "set_t1_kernel")

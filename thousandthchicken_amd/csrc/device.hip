@@ -16,9 +16,12 @@ hipError_t launch_t1_wave(hipStream_t s, bool narrow, int grid, const tc_cblk_de
                           int n, const uint8_t* arena, uint32_t arena_bytes, uint32_t* coef);
 hipError_t launch_t1_stripe(hipStream_t s, int grid, const tc_cblk_desc* descs, const uint32_t* order, int n,
                             const uint8_t* arena, uint32_t arena_bytes, uint32_t* coef);
-hipError_t launch_idwt_level(hipStream_t s, bool is97, const tc_dwt_job* jobs, const uint32_t* wg_job, int nwg,
+int idwt_shape_count();
+int idwt_tile_w(int shape);
+int idwt_tile_h(int shape);
+hipError_t launch_idwt_level(hipStream_t s, int shape, bool is97, const tc_dwt_job* jobs, const uint32_t* wg_job, int nwg,
                              void* base);
-hipError_t launch_idwt_final(hipStream_t s, bool is97, int nc, int out_bytes, const tc_final_job* jobs,
+hipError_t launch_idwt_final(hipStream_t s, int shape, bool is97, int nc, int out_bytes, const tc_final_job* jobs,
                              const uint32_t* wg_job, int nwg, const void* base, void* out);
 hipError_t launch_fill32(hipStream_t s, uint32_t* p, uint32_t v, uint64_t n);
 hipError_t launch_colour(hipStream_t s, int kind, void* c0, void* c1, void* c2, size_t n, int shift, int minv,
@@ -358,17 +361,29 @@ tc_status tc_launch_t1_decode_stripe(tc_stream s, int grid, const void* descs, c
                     "t1_decode_stripe launch");
 }
 
-tc_status tc_launch_idwt_level(tc_stream s, int irr, const void* jobs, const uint32_t* wg_job, int nwg, void* base) {
-    if (!s || nwg < 0 || (nwg > 0 && (!jobs || !wg_job || !base))) return fail(TC_ERR_INVALID_ARG, "tc_launch_idwt_level");
-    return hip_fail(launch_idwt_level(s->s, irr != 0, (const tc_dwt_job*)jobs, wg_job, nwg, base), "idwt_level launch");
+tc_status tc_idwt_tile_shape(int shape, int* w, int* h, int* count) {
+    if (count) *count = idwt_shape_count();
+    if (shape < 0 || shape >= idwt_shape_count()) return fail(TC_ERR_INVALID_ARG, "tc_idwt_tile_shape: shape");
+    if (w) *w = idwt_tile_w(shape);
+    if (h) *h = idwt_tile_h(shape);
+    return TC_SUCCESS;
 }
 
-tc_status tc_launch_idwt_final(tc_stream s, int irr, int nc, int ob, const void* jobs, const uint32_t* wg_job,
+tc_status tc_launch_idwt_level(tc_stream s, int irr, int shape, const void* jobs, const uint32_t* wg_job, int nwg,
+                               void* base) {
+    if (!s || nwg < 0 || (nwg > 0 && (!jobs || !wg_job || !base)) || shape < 0 || shape >= idwt_shape_count())
+        return fail(TC_ERR_INVALID_ARG, "tc_launch_idwt_level");
+    return hip_fail(launch_idwt_level(s->s, shape, irr != 0, (const tc_dwt_job*)jobs, wg_job, nwg, base),
+                    "idwt_level launch");
+}
+
+tc_status tc_launch_idwt_final(tc_stream s, int irr, int shape, int nc, int ob, const void* jobs, const uint32_t* wg_job,
                                int nwg, const void* base, void* out) {
     if (!s || nwg < 0 || (nwg > 0 && (!jobs || !wg_job || !base || !out)) || (nc != 1 && nc != 3) ||
-        (ob != 1 && ob != 2))
+        (ob != 1 && ob != 2) || shape < 0 || shape >= idwt_shape_count())
         return fail(TC_ERR_INVALID_ARG, "tc_launch_idwt_final");
-    return hip_fail(launch_idwt_final(s->s, irr != 0, nc, ob, (const tc_final_job*)jobs, wg_job, nwg, base, out),
+    return hip_fail(launch_idwt_final(s->s, shape, irr != 0, nc, ob, (const tc_final_job*)jobs, wg_job, nwg, base,
+                                      out),
                     "idwt_final launch");
 }
 

@@ -1,4 +1,5 @@
 // C ABI of the decode path (include/tc_decode.h).
+#include <algorithm>
 #include <cstring>
 #include <memory>
 #include <string>
@@ -172,6 +173,119 @@ tc_status tc_decoder_set_t1_kernel(tc_decoder d, int kind) {
     Tuning t = d->pipe->tuning();
     t.t1_kernel = kind;
     d->pipe->set_tuning(t);
+    return TC_SUCCESS;
+}
+
+namespace {
+
+Status reprepare(tc_decoder d, std::string& err) {
+    std::vector<const Image*> v;
+    for (auto& im : d->imgs) v.push_back(im.get());
+    return d->pipe->prepare(v, err);
+}
+
+void to_tuning(const tc_tuning& in, Tuning& t) {
+    t.bpw32 = in.bpw_small;
+    t.bpw64 = in.bpw_large;
+    t.t1_kernel = in.t1_kernel;
+    t.dwt_shape = in.dwt_shape;
+}
+
+}  // namespace
+
+tc_status tc_decoder_get_tuning(tc_decoder d, tc_tuning* out) {
+    if (!d || !out) return report(Status::InvalidArgument, "tc_decoder_get_tuning: null");
+    const Tuning& t = d->pipe->tuning();
+    *out = tc_tuning{};
+    out->bpw_small = t.bpw32;
+    out->bpw_large = t.bpw64;
+    out->t1_kernel = t.t1_kernel;
+    out->dwt_shape = t.dwt_shape;
+    return TC_SUCCESS;
+}
+
+tc_status tc_decoder_set_tuning(tc_decoder d, const tc_tuning* in) {
+    int nshape = 0;
+    tc_idwt_tile_shape(0, nullptr, nullptr, &nshape);
+    auto ok_bpw = [](int b) { return b == 0 || b == 1 || b == 8 || b == 16 || b == 32 || b == 64; };
+    if (!d || !in || !ok_bpw(in->bpw_small) || !ok_bpw(in->bpw_large) || in->t1_kernel < 0 || in->t1_kernel > 1 ||
+        in->dwt_shape < 0 || in->dwt_shape >= nshape)
+        return report(Status::InvalidArgument, "tc_decoder_set_tuning: value out of range");
+    Tuning t = d->pipe->tuning();
+    to_tuning(*in, t);
+    d->pipe->set_tuning(t);
+    return TC_SUCCESS;
+}
+
+// Coordinate search over the prepared batch: blocks-per-wave of each block
+// class (the other class held), then the inverse-DWT tile shape, each
+// candidate timed with the pipeline's stage events (median of reps runs).
+tc_status tc_decoder_autotune(tc_decoder d, int reps, tc_tuning* best) {
+    if (!d || reps <= 0) return report(Status::InvalidArgument, "tc_decoder_autotune: args");
+    if (d->imgs.empty()) return report(Status::InvalidArgument, "tc_decoder_autotune: nothing prepared");
+    std::string err;
+    auto measure = [&](const Tuning& t, float* t1, float* dwt) -> Status {
+        d->pipe->set_tuning(t);
+        Status s = reprepare(d, err);
+        if (s != Status::Ok) return s;
+        std::vector<float> a, b;
+        for (int r = 0; r <= reps; r++) {
+            if (d->pipe->run(true) != Status::Ok || d->pipe->sync() != Status::Ok) return Status::Device;
+            StageTimes st = d->pipe->times();
+            if (r == 0) continue;  // warm-up
+            a.push_back(st.t1_ms);
+            b.push_back(st.dwt_ms + st.final_ms);
+        }
+        std::nth_element(a.begin(), a.begin() + a.size() / 2, a.end());
+        std::nth_element(b.begin(), b.begin() + b.size() / 2, b.end());
+        *t1 = a[a.size() / 2];
+        *dwt = b[b.size() / 2];
+        return Status::Ok;
+    };
+    // start from what the prepared plan uses; a class without blocks is not tuned
+    Tuning cur = d->pipe->tuning();
+    const bool has[2] = {d->pipe->blocks_per_wave(0) != 0, d->pipe->blocks_per_wave(1) != 0};
+    cur.bpw32 = has[0] ? d->pipe->blocks_per_wave(0) : 1;
+    cur.bpw64 = has[1] ? d->pipe->blocks_per_wave(1) : 1;
+    float t1 = 0, dw = 0, best_t1, best_dwt;
+    Status s = measure(cur, &best_t1, &best_dwt);
+    if (s != Status::Ok) return report(s, err.empty() ? "autotune run failed" : err);
+    const int cand[] = {1, 8, 16, 32, 64};
+    for (int cls = 0; cls < 2; cls++) {
+        if (!has[cls]) continue;
+        for (int b : cand) {
+            Tuning t = cur;
+            (cls == 0 ? t.bpw32 : t.bpw64) = b;
+            if ((cls == 0 ? cur.bpw32 : cur.bpw64) == b) continue;
+            s = measure(t, &t1, &dw);
+            if (s != Status::Ok) return report(s, err.empty() ? "autotune run failed" : err);
+            if (t1 < best_t1) {
+                best_t1 = t1;
+                cur = t;
+            }
+        }
+    }
+    int nshape = 0;
+    tc_idwt_tile_shape(0, nullptr, nullptr, &nshape);
+    for (int sh = 0; sh < nshape; sh++) {
+        if (sh == cur.dwt_shape) continue;
+        Tuning t = cur;
+        t.dwt_shape = sh;
+        s = measure(t, &t1, &dw);
+        if (s != Status::Ok) return report(s, err.empty() ? "autotune run failed" : err);
+        if (dw < best_dwt) {
+            best_dwt = dw;
+            cur = t;
+        }
+    }
+    d->pipe->set_tuning(cur);
+    s = reprepare(d, err);
+    if (s != Status::Ok) return report(s, err);
+    if (best) {
+        tc_decoder_get_tuning(d, best);
+        best->t1_ms = best_t1;
+        best->dwt_ms = best_dwt;
+    }
     return TC_SUCCESS;
 }
 

@@ -11,8 +11,8 @@
 
 namespace tc {
 
-int idwt_tile_w();
-int idwt_tile_h();
+int idwt_tile_w(int shape);
+int idwt_tile_h(int shape);
 
 namespace {
 
@@ -30,14 +30,13 @@ Status from_tc(tc_status s) { return s == TC_SUCCESS ? Status::Ok : (s == TC_ERR
 
 }  // namespace
 
+// Default packing when none is set, from the autotuner's choices on MI355X:
+// 8 blocks per wave (lane kernel) once that still gives every CU two waves,
+// one wave per block (stripe kernel) below, where latency dominates.
 int pick_blocks_per_wave(int n, bool small, int cus, const Tuning& t) {
     int fixed = small ? t.bpw32 : t.bpw64;
     if (fixed) return fixed;
-    int per = n / std::max(cus, 1);
-    int g = 1;
-    for (int c : {8, 16, 32, 64})
-        if (per >= c) g = c;
-    return g;
+    return n >= 8 * 2 * std::max(cus, 1) ? 8 : 1;
 }
 
 Pipeline::Pipeline(tc_device dev) : dev_(dev) {
@@ -95,7 +94,8 @@ Status Pipeline::prepare(const std::vector<const Image*>& imgs, std::string& err
     }
     drop_graph();
     tc_stream_sync(stream_);
-    const int TX = idwt_tile_w(), TY = idwt_tile_h();
+    shape_ = tuning_.dwt_shape;
+    const int TX = idwt_tile_w(shape_), TY = idwt_tile_h(shape_);
     groups_.clear();
     levels_.clear();
     finals_.clear();
@@ -360,12 +360,12 @@ Status Pipeline::run(bool timed) {
     if (timed) tc_event_record(ev_[2], stream_);
     for (const LevelLaunch& L : levels_) {
         if (s != TC_SUCCESS) break;
-        s = tc_launch_idwt_level(stream_, L.irr ? 1 : 0, up + L.jobs_off, (const uint32_t*)(up + L.map_off), L.nwg, base);
+        s = tc_launch_idwt_level(stream_, L.irr ? 1 : 0, shape_, up + L.jobs_off, (const uint32_t*)(up + L.map_off), L.nwg, base);
     }
     if (timed) tc_event_record(ev_[3], stream_);
     for (const FinalLaunch& F : finals_) {
         if (s != TC_SUCCESS) break;
-        s = tc_launch_idwt_final(stream_, F.irr ? 1 : 0, F.nc, F.ob, up + F.jobs_off, (const uint32_t*)(up + F.map_off),
+        s = tc_launch_idwt_final(stream_, F.irr ? 1 : 0, shape_, F.nc, F.ob, up + F.jobs_off, (const uint32_t*)(up + F.map_off),
                                  F.nwg, base, out);
     }
     if (timed) tc_event_record(ev_[4], stream_);

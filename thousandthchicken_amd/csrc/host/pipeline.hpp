@@ -27,6 +27,7 @@ struct Tuning {
     int bpw64 = 0;  // blocks per wave for blocks up to 64x64 (0 = pick automatically)
     int bpw32 = 0;  // blocks per wave for blocks up to 32x32 (0 = pick automatically)
     int t1_kernel = 0;  // one-wave-per-block kernel: 0 stripe-column form, 1 row-mask form
+    int dwt_shape = 0;  // inverse-DWT output tile shape (tc_idwt_tile_shape)
 };
 
 // Chooses blocks-per-wave from the block count and the machine size; the
@@ -118,6 +119,7 @@ class Pipeline {
     int64_t ncblk_ = 0, cw_bytes_ = 0, nsamples_ = 0, npixels_ = 0, dwt_bytes_ = 0, final_bytes_ = 0;
     int bpw_[2] = {0, 0};
     Tuning tuning_;
+    int shape_ = 0;  // tile shape the prepared job lists were built for
     tc_event ev_[5] = {nullptr, nullptr, nullptr, nullptr, nullptr};
     bool timed_ = false;
     tc_graph graph_ = nullptr;

@@ -213,6 +213,7 @@ tc_status tc_server_create(tc_device dev, const tc_server_config* cfg, tc_server
     s->cfg.memory_budget = cfg && cfg->memory_budget ? cfg->memory_budget : (32ull << 30);
     s->cfg.bpw_small = cfg ? cfg->bpw_small : 0;
     s->cfg.bpw_large = cfg ? cfg->bpw_large : 0;
+    s->cfg.dwt_shape = cfg ? cfg->dwt_shape : 0;
     std::memset(&s->st, 0, sizeof s->st);
     s->t0 = std::chrono::steady_clock::now();
     s->slots.resize(s->cfg.slots);
@@ -225,6 +226,7 @@ tc_status tc_server_create(tc_device dev, const tc_server_config* cfg, tc_server
         Tuning t;
         t.bpw32 = s->cfg.bpw_small;
         t.bpw64 = s->cfg.bpw_large;
+        t.dwt_shape = s->cfg.dwt_shape;
         sl.pipe->set_tuning(t);
     }
     s->worker = std::thread([s]() { s->loop(); });

@@ -19,8 +19,8 @@
 #include "plan_util.hpp"
 
 namespace tc {
-int idwt_tile_w();
-int idwt_tile_h();
+int idwt_tile_w(int shape);
+int idwt_tile_h(int shape);
 }  // namespace tc
 
 using namespace tc;
@@ -207,7 +207,7 @@ tDeviceRC DWTKernel<T>::run_at(tDeviceMem in, tDeviceMem out, int x0, int y0, in
     tc_device dev = nullptr;
     (void)dev;
     // work buffers and job arrays live with this object
-    const int TX = idwt_tile_w(), TY = idwt_tile_h();
+    const int TX = idwt_tile_w(0), TY = idwt_tile_h(0);
     std::vector<std::vector<tc_dwt_job>> jobs(levels);
     std::vector<std::vector<uint32_t>> maps(levels);
     // addresses: everything relative to the lowest of in/out/work
@@ -257,7 +257,7 @@ tDeviceRC DWTKernel<T>::run_at(tDeviceMem in, tDeviceMem out, int x0, int y0, in
     uint8_t* jp = (uint8_t*)tc_buf_device_ptr(jobs_);
     for (int k = 0; k < levels && !s; k++) {
         if (jobs[k].empty()) continue;
-        s = tc_launch_idwt_level(queue_, irreversible() ? 1 : 0, jp + joff[k], (const uint32_t*)(jp + moff[k]),
+        s = tc_launch_idwt_level(queue_, irreversible() ? 1 : 0, 0, jp + joff[k], (const uint32_t*)(jp + moff[k]),
                                  (int)maps[k].size(), base);
     }
     return s;

@@ -27,9 +27,13 @@
 namespace tc {
 namespace {
 
-constexpr int TX = 64;
-constexpr int TY = 32;
 constexpr int NT = 256;
+
+// Output tile shapes (TX x TY) a launch can use; the planner's job lists and
+// the launch must agree on the shape.
+constexpr int kShapes = 4;
+__host__ __device__ constexpr int shape_w(int s) { return s == 1 ? 128 : (s == 2 ? 32 : 64); }
+__host__ __device__ constexpr int shape_h(int s) { return s == 1 ? 16 : (s == 3 ? 64 : 32); }
 
 // T.800 Table F.4 lifting parameters (the reference's TC/dwt_r97.cl:5-8 and
 // TC/dwt_common.cl:10-11 hold the same values, applied as x += c*(l+r)).
@@ -132,7 +136,7 @@ struct Lift<true> {
 
 // Loads the extended tile, lifts rows then columns.  Returns with the
 // reconstructed samples at tile[H+ty][H+tx].
-template <bool IS97>
+template <bool IS97, int TX, int TY>
 __device__ void tile_idwt(typename Lift<IS97>::T* tile, int pitch, const typename Lift<IS97>::T* __restrict__ src,
                           const Src& s, int u0, int u1, int v0, int v1, int ox, int oy, bool copy_only) {
     typedef typename Lift<IS97>::T T;
@@ -171,7 +175,7 @@ __device__ void tile_idwt(typename Lift<IS97>::T* tile, int pitch, const typenam
     if (!v1x) Lift<IS97>::steps(tile, pitch, 1, W, HH, (oy - H) & 1 ? 1 : 0, tid);
 }
 
-template <bool IS97>
+template <bool IS97, int TX, int TY>
 __global__ __launch_bounds__(NT) void idwt_level_kernel(const tc_dwt_job* __restrict__ jobs,
                                                        const uint32_t* __restrict__ wg_job,
                                                        typename Lift<IS97>::T* __restrict__ base) {
@@ -183,7 +187,7 @@ __global__ __launch_bounds__(NT) void idwt_level_kernel(const tc_dwt_job* __rest
     int t = (int)blockIdx.x - J.wg_base;
     int ox = J.u0 + (t % J.tiles_x) * TX, oy = J.v0 + (t / J.tiles_x) * TY;
     Src s{J.ll_off, J.hl_off, J.lh_off, J.hh_off, J.ll_pitch, J.band_pitch};
-    tile_idwt<IS97>(tile, PITCH, base, s, J.u0, J.u1, J.v0, J.v1, ox, oy, false);
+    tile_idwt<IS97, TX, TY>(tile, PITCH, base, s, J.u0, J.u1, J.v0, J.v1, ox, oy, false);
     for (int i = threadIdx.x; i < TX * TY; i += NT) {
         int ly = i / TX, lx = i % TX;
         int u = ox + lx, v = oy + ly;
@@ -197,7 +201,7 @@ __device__ __forceinline__ O clamp_out(int v, int lo, int hi) {
     return (O)(v < lo ? lo : (v > hi ? hi : v));
 }
 
-template <bool IS97, int NC, typename O>
+template <bool IS97, int NC, typename O, int TX, int TY>
 __global__ __launch_bounds__(NT) void idwt_final_kernel(const tc_final_job* __restrict__ jobs,
                                                        const uint32_t* __restrict__ wg_job,
                                                        const typename Lift<IS97>::T* __restrict__ base,
@@ -213,7 +217,7 @@ __global__ __launch_bounds__(NT) void idwt_final_kernel(const tc_final_job* __re
 #pragma unroll
     for (int c = 0; c < NC; c++) {
         Src s{J.ll_off[c], J.hl_off[c], J.lh_off[c], J.hh_off[c], J.ll_pitch[c], J.band_pitch[c]};
-        tile_idwt<IS97>(tile + c * TS, PITCH, base, s, J.u0, J.u1, J.v0, J.v1, ox, oy, J.levels == 0);
+        tile_idwt<IS97, TX, TY>(tile + c * TS, PITCH, base, s, J.u0, J.u1, J.v0, J.v1, ox, oy, J.levels == 0);
     }
     __syncthreads();
     for (int i = threadIdx.x; i < TX * TY; i += NT) {
@@ -252,21 +256,36 @@ __global__ __launch_bounds__(NT) void idwt_final_kernel(const tc_final_job* __re
 
 }  // namespace
 
-hipError_t launch_idwt_level(hipStream_t s, bool is97, const tc_dwt_job* jobs, const uint32_t* wg_job, int nwg,
-                             void* base) {
-    if (nwg <= 0) return hipSuccess;
+template <int S>
+static void level_launch(hipStream_t s, bool is97, const tc_dwt_job* jobs, const uint32_t* wg_job, int nwg,
+                         void* base) {
+    constexpr int TX = shape_w(S), TY = shape_h(S);
     if (is97)
-        hipLaunchKernelGGL(idwt_level_kernel<true>, dim3(nwg), dim3(NT), 0, s, jobs, wg_job, (float*)base);
+        hipLaunchKernelGGL((idwt_level_kernel<true, TX, TY>), dim3(nwg), dim3(NT), 0, s, jobs, wg_job, (float*)base);
     else
-        hipLaunchKernelGGL(idwt_level_kernel<false>, dim3(nwg), dim3(NT), 0, s, jobs, wg_job, (int32_t*)base);
+        hipLaunchKernelGGL((idwt_level_kernel<false, TX, TY>), dim3(nwg), dim3(NT), 0, s, jobs, wg_job,
+                           (int32_t*)base);
+}
+
+hipError_t launch_idwt_level(hipStream_t s, int shape, bool is97, const tc_dwt_job* jobs, const uint32_t* wg_job,
+                             int nwg, void* base) {
+    if (nwg <= 0) return hipSuccess;
+    switch (shape) {
+        case 0: level_launch<0>(s, is97, jobs, wg_job, nwg, base); break;
+        case 1: level_launch<1>(s, is97, jobs, wg_job, nwg, base); break;
+        case 2: level_launch<2>(s, is97, jobs, wg_job, nwg, base); break;
+        case 3: level_launch<3>(s, is97, jobs, wg_job, nwg, base); break;
+        default: return hipErrorInvalidValue;
+    }
     return hipGetLastError();
 }
 
-hipError_t launch_idwt_final(hipStream_t s, bool is97, int nc, int out_bytes, const tc_final_job* jobs,
-                             const uint32_t* wg_job, int nwg, const void* base, void* out) {
-    if (nwg <= 0) return hipSuccess;
-#define TC_FINAL(I97, NCC, OT)                                                                      \
-    hipLaunchKernelGGL((idwt_final_kernel<I97, NCC, OT>), dim3(nwg), dim3(NT), 0, s, jobs, wg_job, \
+template <int S>
+static hipError_t final_launch(hipStream_t s, bool is97, int nc, int out_bytes, const tc_final_job* jobs,
+                               const uint32_t* wg_job, int nwg, const void* base, void* out) {
+    constexpr int TX = shape_w(S), TY = shape_h(S);
+#define TC_FINAL(I97, NCC, OT)                                                                            \
+    hipLaunchKernelGGL((idwt_final_kernel<I97, NCC, OT, TX, TY>), dim3(nwg), dim3(NT), 0, s, jobs, wg_job, \
                        (const typename Lift<I97>::T*)base, (OT*)out)
     if (nc == 3) {
         if (is97) {
@@ -284,10 +303,25 @@ hipError_t launch_idwt_final(hipStream_t s, bool is97, int nc, int out_bytes, co
         return hipErrorInvalidValue;
     }
 #undef TC_FINAL
-    return hipGetLastError();
+    return hipSuccess;
 }
 
-int idwt_tile_w() { return TX; }
-int idwt_tile_h() { return TY; }
+hipError_t launch_idwt_final(hipStream_t s, int shape, bool is97, int nc, int out_bytes, const tc_final_job* jobs,
+                             const uint32_t* wg_job, int nwg, const void* base, void* out) {
+    if (nwg <= 0) return hipSuccess;
+    hipError_t e;
+    switch (shape) {
+        case 0: e = final_launch<0>(s, is97, nc, out_bytes, jobs, wg_job, nwg, base, out); break;
+        case 1: e = final_launch<1>(s, is97, nc, out_bytes, jobs, wg_job, nwg, base, out); break;
+        case 2: e = final_launch<2>(s, is97, nc, out_bytes, jobs, wg_job, nwg, base, out); break;
+        case 3: e = final_launch<3>(s, is97, nc, out_bytes, jobs, wg_job, nwg, base, out); break;
+        default: return hipErrorInvalidValue;
+    }
+    return e != hipSuccess ? e : hipGetLastError();
+}
+
+int idwt_shape_count() { return kShapes; }
+int idwt_tile_w(int shape) { return shape_w(shape); }
+int idwt_tile_h(int shape) { return shape_h(shape); }
 
 }  // namespace tc
