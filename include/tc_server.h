@@ -46,12 +46,20 @@ typedef struct {
     double busy_s;            /* time with at least one batch on the device */
     uint64_t peak_bytes;      /* largest admitted footprint of all slots */
     int32_t queued, in_flight;
+    double plan_s;            /* summed over batches: planning and staging (host) */
+    double device_s;          /* launch to results in host memory */
+    double copy_s;            /* publishing results (they stay in pinned memory until read) */
 } tc_server_stats;
 
 tc_status tc_server_create(tc_device dev, const tc_server_config *cfg, tc_server *out);
 tc_status tc_server_destroy(tc_server srv);
-/* Copies the codestream; *id identifies the request. */
+/* Copies the codestream; *id identifies the request.  The main header is read
+ * here (for the request's size and device footprint); the rest is parsed by
+ * the worker that decodes it. */
 tc_status tc_server_submit(tc_server srv, const uint8_t *cs, size_t len, uint64_t *id);
+/* Same without the copy: cs must stay valid and unchanged until the request
+ * has finished (tc_server_poll reports done). */
+tc_status tc_server_submit_ref(tc_server srv, const uint8_t *cs, size_t len, uint64_t *id);
 /* *done = 1 once the request finished (successfully or not). */
 tc_status tc_server_poll(tc_server srv, uint64_t id, int *done);
 /* Blocks until the request finished or timeout_s passed (TC_ERR_BUSY). */
@@ -62,6 +70,8 @@ tc_status tc_server_result(tc_server srv, uint64_t id, void *out, size_t cap, tc
 /* Drops a finished request and its result without copying it. */
 tc_status tc_server_release(tc_server srv, uint64_t id);
 tc_status tc_server_get_stats(tc_server srv, tc_server_stats *stats);
+/* Zeroes the counters and times of the stats (queued/in-flight stay live). */
+tc_status tc_server_reset_stats(tc_server srv);
 double tc_server_now(tc_server srv);
 
 #ifdef __cplusplus

@@ -93,6 +93,26 @@ def test_bad_request_fails_alone(streams):
     srv.close()
 
 
+def test_broken_packets_fail_alone_in_batch(streams):
+    # main header intact (accepted at submit), tile-part marker broken (fails in the worker's parse)
+    bad = bytearray(streams[1])
+    k = bytes(bad).find(b"\xff\x90")
+    assert k > 0
+    bad[k + 1] = 0x00
+    srv = tca.Server(slots=1, max_batch=64)
+    ids = [srv.submit(streams[0]), srv.submit(bytes(bad)), srv.submit(streams[2])]
+    for rid in ids:
+        assert srv.wait(rid, 60.0)
+    assert srv.info(ids[1]).status != 0
+    for rid, cs in ((ids[0], streams[0]), (ids[2], streams[2])):
+        img, info = srv.result(rid)
+        ref = oracle.decode(cs).transpose(1, 2, 0)
+        assert np.abs(img.astype(np.int32) - ref).max() <= 1
+    srv.release(ids[1])
+    assert srv.stats().failed == 1
+    srv.close()
+
+
 def test_poisson_run(streams):
     srv = tca.Server(slots=3)
     order = serve.mix(streams[:20], streams[20:], 60, 0.1, seed=9)

@@ -95,11 +95,13 @@ def lib():
         "tc_server_create": (i32, [vp, vp, P(vp)]),
         "tc_server_destroy": (i32, [vp]),
         "tc_server_submit": (i32, [vp, ctypes.c_char_p, sz, P(ctypes.c_uint64)]),
+        "tc_server_submit_ref": (i32, [vp, ctypes.c_char_p, sz, P(ctypes.c_uint64)]),
         "tc_server_poll": (i32, [vp, ctypes.c_uint64, P(i32)]),
         "tc_server_wait": (i32, [vp, ctypes.c_uint64, ctypes.c_double]),
         "tc_server_result": (i32, [vp, ctypes.c_uint64, vp, sz, vp]),
         "tc_server_get_stats": (i32, [vp, vp]),
         "tc_server_release": (i32, [vp, ctypes.c_uint64]),
+        "tc_server_reset_stats": (i32, [vp]),
         "tc_server_now": (ctypes.c_double, [vp]),
         "tc_stream_destroy": (i32, [vp]),
         "tc_stream_sync": (i32, [vp]),
@@ -250,7 +252,8 @@ class RequestInfo(ctypes.Structure):
 class ServerStats(ctypes.Structure):
     _fields_ = [(n, ctypes.c_uint64) for n in ("submitted", "completed", "failed", "batches", "pixels")] + [
         ("busy_s", ctypes.c_double), ("peak_bytes", ctypes.c_uint64), ("queued", ctypes.c_int32),
-        ("in_flight", ctypes.c_int32)]
+        ("in_flight", ctypes.c_int32), ("plan_s", ctypes.c_double), ("device_s", ctypes.c_double),
+        ("copy_s", ctypes.c_double)]
 
 
 class Server:
@@ -263,10 +266,18 @@ class Server:
         h = ctypes.c_void_p()
         check(lib().tc_server_create(self.device.handle, ctypes.byref(cfg), ctypes.byref(h)), "tc_server_create")
         self.handle = h
+        self._hold = {}
 
-    def submit(self, cs: bytes) -> int:
+    def submit(self, cs: bytes, copy=False) -> int:
+        """Queues a codestream.  Without copy the server reads the bytes object
+        itself, which this wrapper keeps alive until the request is read or released."""
         rid = ctypes.c_uint64()
-        check(lib().tc_server_submit(self.handle, cs, len(cs), ctypes.byref(rid)), "submit")
+        if copy:
+            check(lib().tc_server_submit(self.handle, cs, len(cs), ctypes.byref(rid)), "submit")
+        else:
+            cs = bytes(cs)
+            check(lib().tc_server_submit_ref(self.handle, cs, len(cs), ctypes.byref(rid)), "submit")
+            self._hold[rid.value] = cs
         return rid.value
 
     def done(self, rid) -> bool:
@@ -295,10 +306,15 @@ class Server:
         dt = np.uint8 if i.bytes_per_sample == 1 else np.uint16
         out = np.empty((i.height, i.width, i.ncomp), dt)
         check(lib().tc_server_result(self.handle, rid, out.ctypes.data, out.nbytes, None), "result")
+        self._hold.pop(rid, None)
         return out, i
+
+    def reset_stats(self):
+        check(lib().tc_server_reset_stats(self.handle), "reset_stats")
 
     def release(self, rid):
         check(lib().tc_server_release(self.handle, rid), "release")
+        self._hold.pop(rid, None)
 
     def now(self) -> float:
         return lib().tc_server_now(self.handle)
@@ -310,8 +326,9 @@ class Server:
 
     def close(self):
         if getattr(self, "handle", None):
-            lib().tc_server_destroy(self.handle)
+            lib().tc_server_destroy(self.handle)  # drains the queue first
             self.handle = None
+            self._hold.clear()
 
     def __del__(self):
         try:

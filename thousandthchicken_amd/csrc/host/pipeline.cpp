@@ -5,6 +5,7 @@
 
 #include <algorithm>
 #include <cstring>
+#include <thread>
 
 #include "../tc_internal.hpp"
 #include "plan_util.hpp"
@@ -18,12 +19,37 @@ namespace {
 
 uint64_t align_up(uint64_t v, uint64_t a) { return (v + a - 1) / a * a; }
 
+// Upload blob: the codeword arena [0, base) is gathered straight into the
+// staging buffer; v holds everything after it.
+struct Blob {
+    uint64_t base = 0;
+    std::vector<uint8_t> v;
+};
+
 template <typename T>
-uint64_t put(std::vector<uint8_t>& blob, const std::vector<T>& v) {
-    uint64_t off = align_up(blob.size(), 256);
-    blob.resize(off + v.size() * sizeof(T));
-    if (!v.empty()) std::memcpy(blob.data() + off, v.data(), v.size() * sizeof(T));
-    return off;
+uint64_t put(Blob& blob, const std::vector<T>& v) {
+    uint64_t off = align_up(blob.v.size(), 256);
+    blob.v.resize(off + v.size() * sizeof(T));
+    if (!v.empty()) std::memcpy(blob.v.data() + off, v.data(), v.size() * sizeof(T));
+    return blob.base + off;
+}
+
+// Gathers the codeword segments into the staging buffer, on a few threads
+// when there is much to copy.
+void gather(uint8_t* dst, const std::vector<CwCopy>& copies, uint64_t bytes) {
+    const int nt = bytes > (64u << 20) ? 4 : 1;
+    auto part = [&](int k) {
+        const size_t a = copies.size() * k / nt, b = copies.size() * (k + 1) / nt;
+        for (size_t i = a; i < b; i++) std::memcpy(dst + copies[i].dst, copies[i].src, copies[i].len);
+    };
+    if (nt == 1) {
+        part(0);
+        return;
+    }
+    std::vector<std::thread> th;
+    for (int k = 1; k < nt; k++) th.emplace_back(part, k);
+    part(0);
+    for (auto& t : th) t.join();
 }
 
 Status from_tc(tc_status s) { return s == TC_SUCCESS ? Status::Ok : (s == TC_ERR_OUT_OF_MEMORY ? Status::OutOfMemory : Status::Device); }
@@ -62,7 +88,7 @@ Status Pipeline::ensure(tc_buf& b, size_t bytes, const char* what, std::string& 
         tc_buf_free(b);
         b = nullptr;
     }
-    size_t want = std::max<size_t>(bytes + bytes / 8, 1 << 20);
+    size_t want = std::max<size_t>(2 * bytes, 1 << 20);  // grow geometrically: reallocation syncs the device
     tc_status s = tc_buf_alloc(dev_, want, &b);
     if (s != TC_SUCCESS) {
         err = std::string("device allocation of ") + what + " failed: " + tc_last_error();
@@ -76,7 +102,7 @@ Status Pipeline::ensure_pinned(size_t bytes, std::string& err) {
     if (pinned_ && pinned_cap_ >= bytes) return Status::Ok;
     if (pinned_) tc_host_free_pinned(pinned_);
     pinned_ = nullptr;
-    size_t want = std::max<size_t>(bytes + bytes / 8, 1 << 20);
+    size_t want = std::max<size_t>(2 * bytes, 1 << 20);  // grow geometrically: reallocation syncs the device
     tc_status s = tc_host_alloc_pinned(want, &pinned_);
     if (s != TC_SUCCESS) {
         err = std::string("pinned allocation failed: ") + tc_last_error();
@@ -136,7 +162,8 @@ Status Pipeline::prepare(const std::vector<const Image*>& imgs, std::string& err
     coef_elems_ = coef;
 
     // ---- codewords and tier-1 descriptors ----
-    std::vector<uint8_t> cw;
+    uint64_t cw_end = 0;
+    std::vector<CwCopy> copies;
     std::vector<tc_cblk_desc> descs;
     std::vector<uint32_t> cls_idx[2];
     for (size_t i = 0; i < imgs.size(); i++) {
@@ -144,19 +171,18 @@ Status Pipeline::prepare(const std::vector<const Image*>& imgs, std::string& err
         for (size_t t = 0; t < im.tiles.size(); t++) {
             std::vector<uint64_t> planes(im.ncomp);
             for (int c = 0; c < im.ncomp; c++) planes[c] = pos[i][t][c].plane;
-            Status st = append_tile_cblks(im, (int)t, planes.data(), cw, descs, cls_idx, err);
+            Status st = plan_tile_cblks(im, (int)t, planes.data(), cw_end, copies, descs, cls_idx, err);
             if (st != Status::Ok) return st;
         }
     }
-    cw_bytes_ = (int64_t)cw.size();
-    cw.resize(((cw.size() + 3) & ~(size_t)3) + 16, 0xFF);  // slack for look-ahead reads
-    cw_size_ = cw.size();
+    cw_bytes_ = (int64_t)cw_end;
+    cw_size_ = ((cw_end + 3) & ~(uint64_t)3) + 16;  // slack of 0xFF for look-ahead reads
     ncblk_ = (int64_t)descs.size();
     t1_samples_ = 0;
     for (const auto& d : descs) t1_samples_ += (int64_t)d.w * d.h;
 
-    std::vector<uint8_t> blob;
-    uint64_t cw_off = put(blob, cw);
+    Blob blob;
+    blob.base = align_up(cw_size_, 256);
     descs_off_ = put(blob, descs);
     int cus = device_cu_count(dev_);
     for (int k = 0; k < 2; k++) {
@@ -320,18 +346,23 @@ Status Pipeline::prepare(const std::vector<const Image*>& imgs, std::string& err
             }
 
     // ---- device buffers and upload ----
-    Status st = ensure(upload_, blob.size(), "descriptors", err);
+    const uint64_t total = blob.base + blob.v.size();
+    Status st = ensure(upload_, total, "descriptors", err);
     if (st == Status::Ok) st = ensure(samples_, std::max<uint64_t>(work, 1) * 4, "sample planes", err);
     if (st == Status::Ok) st = ensure(out_, std::max<uint64_t>(out_bytes_total, 1), "output images", err);
-    if (st == Status::Ok) st = ensure_pinned(blob.size(), err);
+    if (st == Status::Ok) st = ensure_pinned(total, err);
     if (st != Status::Ok) return st;
-    std::memcpy(pinned_, blob.data(), blob.size());
-    tc_status ts = tc_buf_copy_h2d(stream_, upload_, 0, pinned_, blob.size());
+    uint8_t* stage = (uint8_t*)pinned_;
+    // alignment gaps between codewords are never read as data; the slack after
+    // the last one reads as 0xFF
+    std::memset(stage + cw_end, 0xFF, blob.base - cw_end);
+    gather(stage, copies, cw_end);
+    if (!blob.v.empty()) std::memcpy(stage + blob.base, blob.v.data(), blob.v.size());
+    tc_status ts = tc_buf_copy_h2d(stream_, upload_, 0, pinned_, total);
     if (ts != TC_SUCCESS) {
         err = tc_last_error();
         return from_tc(ts);
     }
-    (void)cw_off;
     return Status::Ok;
 }
 

@@ -30,8 +30,8 @@ struct Tuning {
     int dwt_shape = 0;  // inverse-DWT output tile shape (tc_idwt_tile_shape)
 };
 
-// Chooses blocks-per-wave from the block count and the machine size; the
-// autotuned table, when loaded, overrides it.
+// Chooses blocks-per-wave from the block count and the machine size, by a rule
+// taken from the autotuner's choices; a fixed value in Tuning overrides it.
 int pick_blocks_per_wave(int nblocks, bool small_blocks, int cu_count, const Tuning& t);
 
 struct StageTimes {

@@ -6,8 +6,8 @@
 
 namespace tc {
 
-Status append_tile_cblks(const Image& im, int t, const uint64_t* plane, std::vector<uint8_t>& cw,
-                         std::vector<tc_cblk_desc>& descs, std::vector<uint32_t> (&cls)[2], std::string& err) {
+Status plan_tile_cblks(const Image& im, int t, const uint64_t* plane, uint64_t& cw_size, std::vector<CwCopy>& copies,
+                       std::vector<tc_cblk_desc>& descs, std::vector<uint32_t> (&cls)[2], std::string& err) {
     // segment lists of this tile's blocks (segments are in codestream order)
     size_t first = im.cblks.size(), last = 0;
     for (size_t k = 0; k < im.cblks.size(); k++)
@@ -16,11 +16,17 @@ Status append_tile_cblks(const Image& im, int t, const uint64_t* plane, std::vec
             last = k + 1;
         }
     if (first >= last) return Status::Ok;
-    std::vector<std::vector<uint32_t>> segs(last - first);
-    bool multi = false;
+    // first segment of each block, and the next segment of the same block
+    std::vector<uint32_t> head(last - first, UINT32_MAX), next(im.segs.size(), UINT32_MAX);
+    std::vector<uint32_t> tail(last - first, UINT32_MAX);
     for (uint32_t s = 0; s < im.segs.size(); s++) {
-        uint32_t c = im.segs[s].cblk;
-        if (c >= first && c < last) segs[c - first].push_back(s);
+        const uint32_t c = im.segs[s].cblk;
+        if (c < first || c >= last) continue;
+        if (head[c - first] == UINT32_MAX)
+            head[c - first] = s;
+        else
+            next[tail[c - first]] = s;
+        tail[c - first] = s;
     }
     for (size_t k = first; k < last; k++) {
         const CodeBlock& cb = im.cblks[k];
@@ -39,14 +45,18 @@ Status append_tile_cblks(const Image& im, int t, const uint64_t* plane, std::vec
         }
         tc_cblk_desc d;
         std::memset(&d, 0, sizeof d);
-        cw.resize((cw.size() + 3) & ~(size_t)3, 0);
-        d.data_off = (uint32_t)cw.size();
-        d.data_len = cb.nbytes;
-        for (uint32_t s : segs[k - first]) {
-            const Segment& sg = im.segs[s];
-            cw.insert(cw.end(), im.cs + sg.off, im.cs + sg.off + sg.len);
+        cw_size = (cw_size + 3) & ~(uint64_t)3;
+        if (cw_size + cb.nbytes > 0xFFFFFFF0ull) {
+            err = "codeword arena beyond 4 GiB";
+            return Status::Unsupported;
         }
-        multi |= segs[k - first].size() > 1;
+        d.data_off = (uint32_t)cw_size;
+        d.data_len = cb.nbytes;
+        for (uint32_t s = head[k - first]; s != UINT32_MAX; s = next[s]) {
+            const Segment& sg = im.segs[s];
+            copies.push_back(CwCopy{im.cs + sg.off, sg.len, cw_size});
+            cw_size += sg.len;
+        }
         d.out_off = plane[cb.comp] + (uint64_t)(oy + cb.y0 - B.y0) * tc.width() + (uint64_t)(ox + cb.x0 - B.x0);
         d.out_pitch = (uint32_t)tc.width();
         d.w = (uint16_t)w;
@@ -59,7 +69,17 @@ Status append_tile_cblks(const Image& im, int t, const uint64_t* plane, std::vec
         cls[(w <= 32 && h <= 32) ? 0 : 1].push_back((uint32_t)descs.size());
         descs.push_back(d);
     }
-    (void)multi;
+    return Status::Ok;
+}
+
+Status append_tile_cblks(const Image& im, int t, const uint64_t* plane, std::vector<uint8_t>& cw,
+                         std::vector<tc_cblk_desc>& descs, std::vector<uint32_t> (&cls)[2], std::string& err) {
+    uint64_t size = cw.size();
+    std::vector<CwCopy> copies;
+    Status st = plan_tile_cblks(im, t, plane, size, copies, descs, cls, err);
+    if (st != Status::Ok) return st;
+    cw.resize(size, 0);
+    for (const CwCopy& c : copies) std::memcpy(cw.data() + c.dst, c.src, c.len);
     return Status::Ok;
 }
 

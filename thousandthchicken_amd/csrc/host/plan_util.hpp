@@ -17,6 +17,19 @@ namespace tc {
 Status append_tile_cblks(const Image& im, int t, const uint64_t* plane, std::vector<uint8_t>& cw,
                          std::vector<tc_cblk_desc>& descs, std::vector<uint32_t> (&cls)[2], std::string& err);
 
+// One codeword segment to gather: len bytes from src to offset dst of the arena.
+struct CwCopy {
+    const uint8_t* src;
+    uint32_t len;
+    uint64_t dst;
+};
+
+// Same work list as append_tile_cblks, but the codeword bytes are not moved:
+// the arena grows by the block sizes (cw_size, 4-byte aligned per block) and
+// the segment copies are listed for the caller to perform.
+Status plan_tile_cblks(const Image& im, int t, const uint64_t* plane, uint64_t& cw_size, std::vector<CwCopy>& copies,
+                       std::vector<tc_cblk_desc>& descs, std::vector<uint32_t> (&cls)[2], std::string& err);
+
 // Orders a class of blocks for packing into waves: similar pass counts and
 // shapes side by side, heaviest first.
 void sort_for_packing(std::vector<uint32_t>& idx, const std::vector<tc_cblk_desc>& descs);

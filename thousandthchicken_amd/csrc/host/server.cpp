@@ -24,42 +24,58 @@ namespace {
 
 struct Request {
     uint64_t id = 0;
-    std::vector<uint8_t> data;
-    Image img;
+    std::vector<uint8_t> data;     // owned copy of the codestream (tc_server_submit)
+    const uint8_t* cs = nullptr;   // the codestream (the copy, or the caller's bytes)
+    size_t cs_len = 0;
+    Image img;                     // main header at submit; fully parsed by the worker
     uint64_t footprint = 0, out_bytes = 0;
     tc_request_info info;
-    std::vector<uint8_t> out;
+    std::vector<uint8_t> out;     // the result, once moved out of pinned memory
+    const uint8_t* pin = nullptr;  // the result in a slot's pinned buffer
     bool done = false;
 };
 
 struct Slot {
     std::unique_ptr<Pipeline> pipe;
     std::vector<Request*> batch;
-    bool busy = false;
     tc_event ev = nullptr;
-    void* pinned = nullptr;
-    size_t pinned_cap = 0;
+    // Results land in one of two pinned buffers and stay there until read; a
+    // buffer is reused every other batch, after moving out what is unread.
+    void* pinned[2] = {nullptr, nullptr};
+    size_t pinned_cap[2] = {0, 0};
+    std::vector<uint64_t> holders[2];  // request ids with results in each buffer
+    int cur = 0;
     std::vector<size_t> offs;
+    std::vector<Request*> rejected;  // of the current batch: failed to parse
     uint64_t footprint = 0;
+    std::thread worker;
 };
 
 // Device bytes a request needs in a pipeline: coefficient planes and work
 // buffers (4 B per element), the packed output, codewords and descriptors.
+// From the main header: tile 0's geometry stands for every tile (tile 0 is a
+// full-size tile), and one descriptor per 16 codeword bytes bounds the blocks.
 uint64_t footprint_of(const Image& im) {
     uint64_t f = 0;
-    for (const auto& t : im.tiles)
-        for (const auto& c : t.comps) {
+    if (!im.tiles.empty())
+        for (const auto& c : im.tiles[0].comps) {
             uint64_t sz[2];
             work_sizes(c, sz);
             f += ((uint64_t)c.width() * c.height() + sz[0] + sz[1] + 192) * 4;
         }
+    f *= (uint64_t)std::max<size_t>(im.tiles.size(), 1);
     f += (uint64_t)im.width() * im.height() * im.ncomp * (im.max_prec() > 8 ? 2 : 1);
-    f += im.cs_len + im.cblks.size() * 40 + 4096;
+    f += im.cs_len + im.cs_len / 16 * 40 + 4096;
     return f;
 }
 
 }  // namespace
 
+// One worker thread per slot: it takes the next batch from the queue (FIFO,
+// within the slot's share of the memory budget), prepares and launches it on
+// the slot's stream, waits for it, and copies the results out.  Host work of
+// one slot (planning, staging, result copies) overlaps the device work of the
+// others, and batches of different slots run concurrently on the device.
 struct tc_server_s {
     tc_device dev;
     tc_server_config cfg;
@@ -68,136 +84,162 @@ struct tc_server_s {
     std::deque<Request*> queue;
     std::unordered_map<uint64_t, std::unique_ptr<Request>> reqs;
     std::vector<Slot> slots;
-    std::thread worker;
     bool stop = false;
     uint64_t next_id = 1, batch_seq = 0;
     std::chrono::steady_clock::time_point t0;
     tc_server_stats st;
     double busy_since = -1;
     int busy_slots = 0;
+    uint64_t inflight_bytes = 0;
+    int idle_workers = 0;
 
     double now() const {
         return std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count();
     }
 
-    void finish_slot(Slot& s) {
-        double t = now();
-        for (size_t i = 0; i < s.batch.size(); i++) {
-            Request* r = s.batch[i];
-            r->out.assign((uint8_t*)s.pinned + s.offs[i], (uint8_t*)s.pinned + s.offs[i] + r->out_bytes);
-            r->info.done_s = t;
-            r->info.status = TC_SUCCESS;
-            r->done = true;
-            st.completed++;
-            st.pixels += (uint64_t)r->img.width() * r->img.height();
-        }
-        s.batch.clear();
-        s.busy = false;
+    // caller holds mu
+    void slot_idle(Slot& s) {
+        inflight_bytes -= s.footprint;
         s.footprint = 0;
+        s.batch.clear();
         if (--busy_slots == 0 && busy_since >= 0) {
-            st.busy_s += t - busy_since;
+            st.busy_s += now() - busy_since;
             busy_since = -1;
         }
     }
 
-    void fail_batch(Slot& s, tc_status code) {
-        double t = now();
-        for (Request* r : s.batch) {
-            r->info.status = code;
-            r->info.done_s = t;
-            r->done = true;
-            st.failed++;
-        }
-        s.batch.clear();
-    }
-
-    // Launches a batch on slot s (called without the lock held).
-    tc_status launch(Slot& s) {
+    // Runs a batch on slot s without the lock: plan, upload, launch, copy the
+    // results to pinned memory, wait.
+    tc_status execute(Slot& s) {
         std::vector<const Image*> imgs;
         size_t total = 0;
         s.offs.clear();
+        // full parse; a request whose packets do not parse fails alone
+        std::string perr;
+        std::vector<Request*> ok;
+        for (Request* r : s.batch) {
+            if (parse_codestream(r->cs, r->cs_len, r->img, perr) == Status::Ok)
+                ok.push_back(r);
+            else
+                s.rejected.push_back(r);
+        }
+        s.batch.swap(ok);
+        if (s.batch.empty()) return TC_SUCCESS;
         for (Request* r : s.batch) {
             imgs.push_back(&r->img);
             s.offs.push_back(total);
             total += (r->out_bytes + 255) & ~(size_t)255;
         }
         std::string err;
+        const double tp = now();
         if (s.pipe->prepare(imgs, err) != Status::Ok) {
             set_error(err);
             return TC_ERR_DEVICE;
         }
-        if (!s.pinned || s.pinned_cap < total) {
-            if (s.pinned) tc_host_free_pinned(s.pinned);
-            s.pinned = nullptr;
-            s.pinned_cap = 0;
-            tc_status e = tc_host_alloc_pinned(std::max<size_t>(total, 1 << 20), &s.pinned);
+        void*& pin = s.pinned[s.cur];
+        if (!pin || s.pinned_cap[s.cur] < total) {
+            if (pin) tc_host_free_pinned(pin);
+            pin = nullptr;
+            s.pinned_cap[s.cur] = 0;
+            const size_t want = std::max<size_t>(2 * total, 16 << 20);
+            tc_status e = tc_host_alloc_pinned(want, &pin);
             if (e) return e;
-            s.pinned_cap = std::max<size_t>(total, 1 << 20);
+            s.pinned_cap[s.cur] = want;
         }
+        const double tr = now();
         if (s.pipe->run(false) != Status::Ok) return TC_ERR_DEVICE;
         hipStream_t hs = stream_handle(s.pipe->stream());
         for (size_t i = 0; i < s.batch.size(); i++) {
-            hipError_t e = hipMemcpyAsync((uint8_t*)s.pinned + s.offs[i], s.pipe->output_ptr((int)i),
+            hipError_t e = hipMemcpyAsync((uint8_t*)pin + s.offs[i], s.pipe->output_ptr((int)i),
                                           s.batch[i]->out_bytes, hipMemcpyDeviceToHost, hs);
             if (e != hipSuccess) return hip_fail(e, "result copy");
         }
-        return tc_event_record(s.ev, s.pipe->stream());
+        tc_status e = tc_event_record(s.ev, s.pipe->stream());
+        if (!e) e = tc_event_sync(s.ev);
+        const double td = now();
+        std::lock_guard<std::mutex> lk(mu);
+        st.plan_s += tr - tp;
+        st.device_s += td - tr;
+        return e;
     }
 
-    void loop() {
-        std::unique_lock<std::mutex> lk(mu);
+    void work(Slot& s) {
         const uint64_t slot_budget = cfg.memory_budget / (uint64_t)std::max(1, cfg.slots);
+        std::unique_lock<std::mutex> lk(mu);
         for (;;) {
-            bool progress = false;
-            for (Slot& s : slots)
-                if (s.busy && tc_event_query(s.ev) == TC_SUCCESS) {
-                    finish_slot(s);
-                    progress = true;
-                }
-            if (progress) cv_done.notify_all();
-            if (stop && queue.empty() && busy_slots == 0) break;
-            for (Slot& s : slots) {
-                if (s.busy || queue.empty()) continue;
-                // admission: FIFO while the batch fits the slot's memory share
-                uint64_t fp = 0;
-                while (!queue.empty() && (int)s.batch.size() < cfg.max_batch) {
-                    Request* r = queue.front();
-                    if (!s.batch.empty() && fp + r->footprint > slot_budget) break;
-                    fp += r->footprint;
-                    s.batch.push_back(r);
-                    queue.pop_front();
-                }
-                uint64_t seq = ++batch_seq;
-                double t = now();
-                for (Request* r : s.batch) {
-                    r->info.start_s = t;
-                    r->info.batch = seq;
-                    r->info.batch_size = (int32_t)s.batch.size();
-                }
-                s.footprint = fp;
-                s.busy = true;
-                if (busy_slots++ == 0) busy_since = t;
-                st.batches++;
-                uint64_t inflight = 0;
-                for (Slot& o : slots) inflight += o.footprint;
-                st.peak_bytes = std::max(st.peak_bytes, inflight);
-                lk.unlock();
-                tc_status e = launch(s);
-                lk.lock();
-                if (e != TC_SUCCESS) {
-                    tc_stream_sync(s.pipe->stream());
-                    fail_batch(s, e);
-                    s.busy = false;
-                    s.footprint = 0;
-                    if (--busy_slots == 0 && busy_since >= 0) {
-                        st.busy_s += now() - busy_since;
-                        busy_since = -1;
-                    }
-                    cv_done.notify_all();
-                }
-                progress = true;
+            idle_workers++;
+            cv_work.wait(lk, [&] { return stop || !queue.empty(); });
+            idle_workers--;
+            if (queue.empty()) break;  // stopping and drained
+            // admission: FIFO while the batch fits the slot's memory share, and
+            // at most this worker's share of the queue when other slots are idle
+            const size_t share = (queue.size() + idle_workers) / (size_t)(idle_workers + 1);
+            const int cap = std::min<int>(cfg.max_batch, (int)std::max<size_t>(share, 1));
+            uint64_t fp = 0;
+            while (!queue.empty() && (int)s.batch.size() < cap) {
+                Request* r = queue.front();
+                if (!s.batch.empty() && fp + r->footprint > slot_budget) break;
+                fp += r->footprint;
+                s.batch.push_back(r);
+                queue.pop_front();
             }
-            if (!progress) cv_work.wait_for(lk, std::chrono::microseconds(busy_slots ? 50 : 2000));
+            const uint64_t seq = ++batch_seq;
+            const double t = now();
+            for (Request* r : s.batch) {
+                r->info.start_s = t;
+                r->info.batch = seq;
+                r->info.batch_size = (int32_t)s.batch.size();
+            }
+            s.footprint = fp;
+            inflight_bytes += fp;
+            st.peak_bytes = std::max(st.peak_bytes, inflight_bytes);
+            if (busy_slots++ == 0) busy_since = t;
+            st.batches++;
+            if (!queue.empty()) cv_work.notify_one();  // another slot can start on the rest
+            s.cur ^= 1;
+            std::vector<Request*> unread;
+            for (uint64_t id : s.holders[s.cur]) {
+                auto it = reqs.find(id);
+                if (it != reqs.end() && it->second->pin) unread.push_back(it->second.get());
+            }
+            s.holders[s.cur].clear();
+            for (Request* r : unread) {  // results nobody read since the batch before last
+                r->out.assign(r->pin, r->pin + r->out_bytes);
+                r->pin = nullptr;
+            }
+            lk.unlock();
+            const tc_status e = execute(s);
+            const double tc0 = now();
+            if (e != TC_SUCCESS) tc_stream_sync(s.pipe->stream());
+            lk.lock();
+            if (e == TC_SUCCESS)
+                for (size_t i = 0; i < s.batch.size(); i++) {
+                    Request* r = s.batch[i];
+                    r->pin = (const uint8_t*)s.pinned[s.cur] + s.offs[i];
+                    s.holders[s.cur].push_back(r->id);
+                }
+            st.copy_s += now() - tc0;
+            const double td = now();
+            for (Request* r : s.rejected) {
+                r->info.done_s = td;
+                r->info.status = TC_ERR_UNSUPPORTED;
+                r->done = true;
+                st.failed++;
+            }
+            s.rejected.clear();
+            for (Request* r : s.batch) {
+                r->info.done_s = td;
+                r->info.status = e;
+                r->done = true;
+                if (e == TC_SUCCESS) {
+                    st.completed++;
+                    st.pixels += (uint64_t)r->img.width() * r->img.height();
+                } else {
+                    st.failed++;
+                }
+            }
+            slot_idle(s);
+            cv_done.notify_all();
         }
     }
 };
@@ -229,7 +271,7 @@ tc_status tc_server_create(tc_device dev, const tc_server_config* cfg, tc_server
         t.dwt_shape = s->cfg.dwt_shape;
         sl.pipe->set_tuning(t);
     }
-    s->worker = std::thread([s]() { s->loop(); });
+    for (Slot& sl : s->slots) sl.worker = std::thread([s, &sl]() { s->work(sl); });
     *out = s;
     return TC_SUCCESS;
 }
@@ -241,23 +283,31 @@ tc_status tc_server_destroy(tc_server s) {
         s->stop = true;
     }
     s->cv_work.notify_all();
-    if (s->worker.joinable()) s->worker.join();
+    for (Slot& sl : s->slots)
+        if (sl.worker.joinable()) sl.worker.join();
     for (Slot& sl : s->slots) {
         if (sl.pipe) tc_stream_sync(sl.pipe->stream());
         tc_event_destroy(sl.ev);
-        if (sl.pinned) tc_host_free_pinned(sl.pinned);
+        for (void* p : sl.pinned)
+            if (p) tc_host_free_pinned(p);
     }
     delete s;
     return TC_SUCCESS;
 }
 
-tc_status tc_server_submit(tc_server s, const uint8_t* cs, size_t len, uint64_t* id) {
+static tc_status submit(tc_server s, const uint8_t* cs, size_t len, uint64_t* id, bool copy) {
     if (!s || !cs || !id) return fail(TC_ERR_INVALID_ARG, "tc_server_submit: null");
     std::unique_ptr<Request> r(new Request());
     std::memset(&r->info, 0, sizeof r->info);
-    r->data.assign(cs, cs + len);
+    if (copy) {
+        r->data.assign(cs, cs + len);
+        r->cs = r->data.data();
+    } else {
+        r->cs = cs;
+    }
+    r->cs_len = len;
     std::string err;
-    Status st = parse_codestream(r->data.data(), r->data.size(), r->img, err);
+    Status st = probe_codestream(r->cs, r->cs_len, r->img, err);
     std::lock_guard<std::mutex> lk(s->mu);
     r->id = s->next_id++;
     r->info.submit_s = s->now();
@@ -283,6 +333,14 @@ tc_status tc_server_submit(tc_server s, const uint8_t* cs, size_t len, uint64_t*
     s->reqs[r->id] = std::move(r);
     s->cv_work.notify_one();
     return TC_SUCCESS;
+}
+
+tc_status tc_server_submit(tc_server s, const uint8_t* cs, size_t len, uint64_t* id) {
+    return submit(s, cs, len, id, true);
+}
+
+tc_status tc_server_submit_ref(tc_server s, const uint8_t* cs, size_t len, uint64_t* id) {
+    return submit(s, cs, len, id, false);
 }
 
 tc_status tc_server_poll(tc_server s, uint64_t id, int* done) {
@@ -324,8 +382,8 @@ tc_status tc_server_result(tc_server s, uint64_t id, void* out, size_t cap, tc_r
         s->reqs.erase(it);
         return fail(st, "request failed");
     }
-    if (cap < r->out.size()) return fail(TC_ERR_CAPACITY, "tc_server_result: output buffer too small");
-    std::memcpy(out, r->out.data(), r->out.size());
+    if (cap < r->out_bytes) return fail(TC_ERR_CAPACITY, "tc_server_result: output buffer too small");
+    std::memcpy(out, r->pin ? r->pin : r->out.data(), r->out_bytes);
     s->reqs.erase(it);
     return TC_SUCCESS;
 }
@@ -347,6 +405,15 @@ tc_status tc_server_get_stats(tc_server s, tc_server_stats* out) {
     if (s->busy_since >= 0) out->busy_s += s->now() - s->busy_since;
     out->queued = (int32_t)s->queue.size();
     out->in_flight = s->busy_slots;
+    return TC_SUCCESS;
+}
+
+tc_status tc_server_reset_stats(tc_server s) {
+    if (!s) return fail(TC_ERR_INVALID_ARG, "tc_server_reset_stats: null");
+    std::lock_guard<std::mutex> lk(s->mu);
+    std::memset(&s->st, 0, sizeof s->st);
+    s->st.peak_bytes = s->inflight_bytes;
+    if (s->busy_since >= 0) s->busy_since = s->now();
     return TC_SUCCESS;
 }
 

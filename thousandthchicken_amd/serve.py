@@ -64,11 +64,14 @@ def run(server, streams, rate, seed=0, keep=None):
     th = threading.Thread(target=collect, daemon=True)
     th.start()
     t0 = time.perf_counter()
+    t_sub = 0.0
     for i, cs in enumerate(streams):
         delay = t_arr[i] - (time.perf_counter() - t0)
         if delay > 0:
             time.sleep(delay)
+        ts = time.perf_counter()
         ids[i] = server.submit(cs)
+        t_sub += time.perf_counter() - ts
         ready.release()
     th.join()
     if err:
@@ -79,7 +82,10 @@ def run(server, streams, rate, seed=0, keep=None):
     span = max(x.done_s for x in infos) - min(x.submit_s for x in infos)
     bs = np.array([x.batch_size for x in infos])
     failed = sum(1 for x in infos if x.status != 0)
+    stt = server.stats()
     return {
+        "server_s": {"busy": round(stt.busy_s, 3), "plan": round(stt.plan_s, 3), "device": round(stt.device_s, 3),
+                     "copy": round(stt.copy_s, 3), "batches": int(stt.batches), "submit": round(t_sub, 3)},
         "offered_rps": rate,
         "requests": n,
         "failed": failed,
