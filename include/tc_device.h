@@ -127,11 +127,15 @@ tc_status tc_launch_t1_decode(tc_stream s, int small_blocks, int blocks_per_wave
 tc_status tc_launch_t1_decode_wave(tc_stream s, int small_blocks, int grid, const void *descs, const uint32_t *order,
                                    int n, const uint8_t *arena, size_t arena_bytes, void *coef);
 
-/* Same contract as tc_launch_t1_decode_wave (one wavefront per code-block,
- * blocks up to 64x64 of any mix), with block state held column-major across
- * the lanes and the scan restricted to the columns holding candidates. */
-tc_status tc_launch_t1_decode_stripe(tc_stream s, int grid, const void *descs, const uint32_t *order, int n,
-                                     const uint8_t *arena, size_t arena_bytes, void *coef);
+/* Same contract as tc_launch_t1_decode_wave (one wavefront per code-block),
+ * with block state held column-major across the lanes and the scan restricted
+ * to the columns holding candidates.  small_blocks: every block is at most
+ * 32x32.  Blocks with more bit-planes than TC_T1_STRIPE_MAX_PLANES(small) are
+ * skipped (their coefficients keep what the plane held); decode them with
+ * tc_launch_t1_decode_wave. */
+#define TC_T1_STRIPE_MAX_PLANES(small) ((small) ? 32 : 20)
+tc_status tc_launch_t1_decode_stripe(tc_stream s, int small_blocks, int grid, const void *descs, const uint32_t *order,
+                                     int n, const uint8_t *arena, size_t arena_bytes, void *coef);
 
 /* Output tile shapes of the inverse DWT launches: shape 0..*count-1 covers
  * *w x *h output samples per work-group (the workgroup-window choice of

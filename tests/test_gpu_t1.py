@@ -68,8 +68,8 @@ def test_t1_kernels_on_random_blocks(kernel, small):
         bufs[1].upload(stream, descs)
         tca.check(L.tc_buf_fill32(stream, bufs[2].handle, 0, 0, n_out), "fill")
         if kernel == "stripe":
-            st = L.tc_launch_t1_decode_stripe(stream, 512, bufs[1].ptr, None, len(blocks), bufs[0].ptr, arena.nbytes,
-                                              bufs[2].ptr)
+            st = L.tc_launch_t1_decode_stripe(stream, 1 if small else 0, 512, bufs[1].ptr, None, len(blocks),
+                                              bufs[0].ptr, arena.nbytes, bufs[2].ptr)
         elif kernel == "wave":
             st = L.tc_launch_t1_decode_wave(stream, 1 if small else 0, 512, bufs[1].ptr, None, len(blocks),
                                             bufs[0].ptr, arena.nbytes, bufs[2].ptr)

@@ -14,8 +14,8 @@ hipError_t launch_t1(hipStream_t s, bool cls32, int g, const tc_cblk_desc* descs
                      const uint8_t* arena, uint32_t* coef);
 hipError_t launch_t1_wave(hipStream_t s, bool narrow, int grid, const tc_cblk_desc* descs, const uint32_t* order,
                           int n, const uint8_t* arena, uint32_t arena_bytes, uint32_t* coef);
-hipError_t launch_t1_stripe(hipStream_t s, int grid, const tc_cblk_desc* descs, const uint32_t* order, int n,
-                            const uint8_t* arena, uint32_t arena_bytes, uint32_t* coef);
+hipError_t launch_t1_stripe(hipStream_t s, bool small, int grid, const tc_cblk_desc* descs, const uint32_t* order,
+                            int n, const uint8_t* arena, uint32_t arena_bytes, uint32_t* coef);
 int idwt_shape_count();
 int idwt_tile_w(int shape);
 int idwt_tile_h(int shape);
@@ -351,13 +351,13 @@ tc_status tc_launch_t1_decode_wave(tc_stream s, int small_blocks, int grid, cons
                     "t1_decode_wave launch");
 }
 
-tc_status tc_launch_t1_decode_stripe(tc_stream s, int grid, const void* descs, const uint32_t* order, int n,
-                                     const uint8_t* arena, size_t arena_bytes, void* coef) {
+tc_status tc_launch_t1_decode_stripe(tc_stream s, int small_blocks, int grid, const void* descs, const uint32_t* order,
+                                     int n, const uint8_t* arena, size_t arena_bytes, void* coef) {
     if (!s || n < 0 || grid <= 0 || (n > 0 && (!descs || !arena || !coef)) || (arena_bytes & 3) ||
         arena_bytes > 0xFFFFFFFFull || ((uintptr_t)arena & 3))
         return fail(TC_ERR_INVALID_ARG, "tc_launch_t1_decode_stripe");
-    return hip_fail(launch_t1_stripe(s->s, grid, (const tc_cblk_desc*)descs, order, n, arena, (uint32_t)arena_bytes,
-                                     (uint32_t*)coef),
+    return hip_fail(launch_t1_stripe(s->s, small_blocks != 0, grid, (const tc_cblk_desc*)descs, order, n, arena,
+                                     (uint32_t)arena_bytes, (uint32_t*)coef),
                     "t1_decode_stripe launch");
 }
 

@@ -194,6 +194,7 @@ Status Pipeline::prepare(const std::vector<const Image*>& imgs, std::string& err
         g.small = k == 0;
         g.bpw = pick_blocks_per_wave((int)v.size(), g.small, cus, tuning_);
         g.n = (int)v.size();
+        for (uint32_t i : v) g.max_planes = std::max<int>(g.max_planes, descs[i].numbps);
         g.order_off = put(blob, v);
         bpw_[k] = g.bpw;
         groups_.push_back(g);
@@ -378,9 +379,10 @@ Status Pipeline::run(bool timed) {
     if (timed) tc_event_record(ev_[1], stream_);
     for (const Group& g : groups_) {
         if (s != TC_SUCCESS) break;
-        if (g.bpw == 1 && tuning_.t1_kernel == 0)
-            s = tc_launch_t1_decode_stripe(stream_, std::min(g.n, device_cu_count(dev_) * 16), up + descs_off_,
-                                           (const uint32_t*)(up + g.order_off), g.n, up, cw_size_, base);
+        if (g.bpw == 1 && tuning_.t1_kernel == 0 && g.max_planes <= TC_T1_STRIPE_MAX_PLANES(g.small))
+            s = tc_launch_t1_decode_stripe(stream_, g.small ? 1 : 0, std::min(g.n, device_cu_count(dev_) * 32),
+                                           up + descs_off_, (const uint32_t*)(up + g.order_off), g.n, up, cw_size_,
+                                           base);
         else if (g.bpw == 1)
             s = tc_launch_t1_decode_wave(stream_, g.small ? 1 : 0, std::min(g.n, device_cu_count(dev_) * 16), up + descs_off_,
                                          (const uint32_t*)(up + g.order_off), g.n, up, cw_size_, base);

@@ -82,6 +82,7 @@ class Pipeline {
     struct Group {  // one t1 launch
         bool small = false;
         int bpw = 1;
+        int max_planes = 0;  // most bit-planes of a block in the class
         uint64_t order_off = 0;
         int n = 0;
     };

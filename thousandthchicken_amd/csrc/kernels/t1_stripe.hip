@@ -283,7 +283,7 @@ struct Blk {
     uint32_t cl, cs;          // context used last and its current state
     uint32_t st_rl, st_uni;
     uint32_t zcv, scv, trm, trl;
-    uint64_t sig, sgn, pip, ref, ones;  // lane x: column x
+    uint64_t sig, sgn, pip, ref, ones;  // lane x: column x (bit y = row y)
     int w, h, cls;
 
     // The state of the context used last stays in a scalar register (cl, cs)
@@ -491,9 +491,10 @@ struct Blk {
     }
 };
 
+template <typename M>
 __device__ __forceinline__ void decode_block(const BlockArgs d, const uint32_t* __restrict__ arena_w,
                                              uint32_t arena_words, uint32_t* __restrict__ coef,
-                                             uint64_t (*planes)[64], uint64_t* sgncols, uint32_t zcv,
+                                             M (*planes)[64], M* sgncols, uint32_t zcv,
                                              uint32_t scv, uint32_t trm, uint32_t trl, uint32_t ctx0) {
     const int lane = threadIdx.x;
     const int w = d.w, h = d.h;
@@ -533,12 +534,12 @@ __device__ __forceinline__ void decode_block(const BlockArgs d, const uint32_t* 
         }
         // a plane ends after its cleanup pass, or when the passes run out
         if (type == 2 || k == npasses - 1) {
-            planes[p][lane] = B.ones;
+            planes[p][lane] = (M)B.ones;
             B.ones = 0;
             B.pip = 0;
         }
     }
-    sgncols[lane] = B.sgn;
+    sgncols[lane] = (M)B.sgn;
     __syncthreads();
 
     // rebuild, dequantize and store the block's rows
@@ -553,7 +554,7 @@ __device__ __forceinline__ void decode_block(const BlockArgs d, const uint32_t* 
     const int lx = lane % span, ly = lane / span;
     uint32_t* out = coef + d.out_off;
     if (lx < w) {
-        const uint64_t sc = sgncols[lx];
+        const M sc = sgncols[lx];
         for (int y = ly; y < h; y += rows_per) {
             uint32_t mag = 0;
             for (int pp = plast; pp < numbps; pp++) mag |= (uint32_t)((planes[pp][lx] >> y) & 1u) << pp;
@@ -575,12 +576,18 @@ __device__ __forceinline__ void decode_block(const BlockArgs d, const uint32_t* 
     __syncthreads();
 }
 
+// M: type of the per-plane column words kept in LDS (32 bits for blocks up to
+// 32x32); NPL: bit-planes a block may carry (deeper blocks are left to the
+// row-mask kernel); NL: the widest block.  Every lane writes its plane
+// word (lanes past the block hold zeros), so no store is divergent.  LDS per
+// work-group is (NPL+1)*64*sizeof(uint64_t): 8.3 KiB small, 10.5 KiB large.
+template <typename M, int NPL, int NL>
 __global__ __launch_bounds__(64) void t1_stripe_kernel(const tc_cblk_desc* __restrict__ descs,
                                                        const uint32_t* __restrict__ order, int n,
                                                        const uint32_t* __restrict__ arena_w, uint32_t arena_words,
                                                        uint32_t* __restrict__ coef) {
-    __shared__ uint64_t planes[32][64];
-    __shared__ uint64_t sgncols[64];
+    __shared__ M planes[NPL][64];
+    __shared__ M sgncols[64];
     const int lane = threadIdx.x;
     // lane tables: zero-coding labels for the 9-bit windows lane*8 .. lane*8+7
     // (4 bits each, one word per band class), sign entries for the sign
@@ -605,22 +612,29 @@ __global__ __launch_bounds__(64) void t1_stripe_kernel(const tc_cblk_desc* __res
     for (int b = blockIdx.x; b < n; b += gridDim.x) {
         const uint32_t id = order ? uni(order[b]) : (uint32_t)b;
         const BlockArgs d = load_args(descs + id);
-        if (d.numbps <= 0 || d.npasses <= 0 || d.numbps > 32 || d.w <= 0 || d.h <= 0 || d.w > 64 || d.h > 64)
+        if (d.numbps <= 0 || d.npasses <= 0 || d.numbps > NPL || d.w <= 0 || d.h <= 0 || d.w > NL ||
+            d.h > (int)sizeof(uint64_t) * 8)
             continue;
         const uint32_t zcv = d.cls == 0 ? zc3[0] : (d.cls == 1 ? zc3[1] : zc3[2]);
-        decode_block(d, arena_w, arena_words, coef, planes, sgncols, zcv, scv, trm, trl, ctx0);
+        decode_block<M>(d, arena_w, arena_words, coef, planes, sgncols, zcv, scv, trm, trl, ctx0);
     }
 }
 
 }  // namespace
 
-hipError_t launch_t1_stripe(hipStream_t s, int grid, const tc_cblk_desc* descs, const uint32_t* order, int n,
-                            const uint8_t* arena, uint32_t arena_bytes, uint32_t* coef) {
+int t1_stripe_max_planes(bool small) { return small ? 32 : 20; }
+
+hipError_t launch_t1_stripe(hipStream_t s, bool small, int grid, const tc_cblk_desc* descs, const uint32_t* order,
+                            int n, const uint8_t* arena, uint32_t arena_bytes, uint32_t* coef) {
     if (n <= 0) return hipSuccess;
     if (grid > n) grid = n;
     if (grid < 1) grid = 1;
-    hipLaunchKernelGGL(t1_stripe_kernel, dim3(grid), dim3(64), 0, s, descs, order, n, (const uint32_t*)arena,
-                       arena_bytes / 4, coef);
+    if (small)
+        hipLaunchKernelGGL((t1_stripe_kernel<uint32_t, 32, 32>), dim3(grid), dim3(64), 0, s, descs, order, n,
+                           (const uint32_t*)arena, arena_bytes / 4, coef);
+    else
+        hipLaunchKernelGGL((t1_stripe_kernel<uint64_t, 20, 64>), dim3(grid), dim3(64), 0, s, descs, order, n,
+                           (const uint32_t*)arena, arena_bytes / 4, coef);
     return hipGetLastError();
 }
 
