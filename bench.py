@@ -32,7 +32,6 @@ sys.path.insert(0, ROOT)
 import numpy as np  # noqa: E402
 
 HBM_PEAK_GBS = 8000.0
-TRAFFIC_FILE = os.path.join(ROOT, "profiles", "traffic_t1_wave_kernel_cfg2_lossy_2048.json")
 
 
 def log(msg):
@@ -103,10 +102,12 @@ def t1_kernel_name(stats, args):
     return "t1_stripe_kernel" if args.t1_kernel == 0 else "t1_wave_kernel"
 
 
-def traffic_record():
-    """HBM bytes per tier-1 launch measured with FETCH_SIZE / WRITE_SIZE."""
+def traffic_record(kernel, workload):
+    """HBM bytes per tier-1 launch measured with FETCH_SIZE / WRITE_SIZE
+    (profiles/traffic_<kernel>_<workload>.json)."""
+    path = os.path.join(ROOT, "profiles", f"traffic_{kernel.split('<')[0]}_{workload}.json")
     try:
-        with open(TRAFFIC_FILE) as f:
+        with open(path) as f:
             return json.load(f)
     except (OSError, ValueError):
         return None
@@ -308,10 +309,8 @@ def main():
     t1_bytes = s.codeword_bytes + 4 * s.t1_samples
     t1_gbs = t1_bytes / (st[1] * 1e-3) / 1e9
     dwt_gbs = (s.dwt_bytes + s.final_bytes) / ((st[2] + st[3]) * 1e-3) / 1e9 if st[2] + st[3] > 0 else 0.0
-    tr = traffic_record()
-    traffic = (tr["hbm_bytes_per_launch"]
-               if tr and tr.get("workload") == args.workload and tr.get("kernel", "").startswith(t1_kernel_name(s, args))
-               else None)
+    tr = traffic_record(t1_kernel_name(s, args), args.workload)
+    traffic = tr["hbm_bytes_per_launch"] if tr else None
 
     # plain per-image entry (host in, host out), one call after the other
     per = tca.Decoder()
