@@ -99,6 +99,9 @@ tc_status tc_event_create(tc_event *out);
 tc_status tc_event_destroy(tc_event e);
 tc_status tc_event_record(tc_event e, tc_stream s);
 tc_status tc_event_sync(tc_event e);
+/* Work queued on s after this call waits for the work before e (cross-stream
+ * dependency; also recorded as a graph edge while s is being captured). */
+tc_status tc_stream_wait_event(tc_stream s, tc_event e);
 /* TC_SUCCESS when the work before the event has finished, TC_ERR_BUSY if not */
 tc_status tc_event_query(tc_event e);
 tc_status tc_event_elapsed_ms(tc_event start, tc_event stop, float *ms);

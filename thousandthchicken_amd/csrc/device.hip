@@ -280,6 +280,11 @@ tc_status tc_event_record(tc_event e, tc_stream s) {
     return hip_fail(hipEventRecord(e->e, s->s), "hipEventRecord");
 }
 
+tc_status tc_stream_wait_event(tc_stream s, tc_event e) {
+    if (!s || !e) return fail(TC_ERR_INVALID_ARG, "tc_stream_wait_event: null");
+    return hip_fail(hipStreamWaitEvent(s->s, e->e, 0), "hipStreamWaitEvent");
+}
+
 tc_status tc_event_sync(tc_event e) {
     if (!e) return fail(TC_ERR_INVALID_ARG, "tc_event_sync: null");
     return hip_fail(hipEventSynchronize(e->e), "hipEventSynchronize");

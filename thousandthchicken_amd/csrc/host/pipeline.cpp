@@ -67,13 +67,20 @@ int pick_blocks_per_wave(int n, bool small, int cus, const Tuning& t) {
 
 Pipeline::Pipeline(tc_device dev) : dev_(dev) {
     if (tc_stream_create(dev, &stream_) != TC_SUCCESS) stream_ = nullptr;
+    if (tc_stream_create(dev, &side_) != TC_SUCCESS) side_ = nullptr;
     for (auto& e : ev_) tc_event_create(&e);
+    tc_event_create(&fork_);
+    tc_event_create(&join_);
 }
 
 Pipeline::~Pipeline() {
     if (stream_) tc_stream_sync(stream_);
     drop_graph();
+    if (side_) tc_stream_sync(side_);
     for (auto& e : ev_) tc_event_destroy(e);
+    tc_event_destroy(fork_);
+    tc_event_destroy(join_);
+    tc_stream_destroy(side_);
     tc_buf_free(upload_);
     tc_buf_free(samples_);
     tc_buf_free(out_);
@@ -377,18 +384,32 @@ Status Pipeline::run(bool timed) {
     if (timed) tc_event_record(ev_[0], stream_);
     s = tc_launch_fill32(stream_, base, 0u, coef_elems_);
     if (timed) tc_event_record(ev_[1], stream_);
-    for (const Group& g : groups_) {
+    // Tier-1: one launch per block class.  With two classes the second runs on
+    // the side stream, so each launch's tail overlaps the other's work.
+    const bool fork = groups_.size() == 2 && side_ && fork_ && join_;
+    if (fork && s == TC_SUCCESS) {
+        s = tc_event_record(fork_, stream_);
+        if (s == TC_SUCCESS) s = tc_stream_wait_event(side_, fork_);
+    }
+    for (size_t gi = 0; gi < groups_.size(); gi++) {
         if (s != TC_SUCCESS) break;
+        const Group& g = groups_[gi];
+        tc_stream st = (fork && gi == 1) ? side_ : stream_;
         if (g.bpw == 1 && tuning_.t1_kernel == 0 && g.max_planes <= TC_T1_STRIPE_MAX_PLANES(g.small))
-            s = tc_launch_t1_decode_stripe(stream_, g.small ? 1 : 0, std::min(g.n, device_cu_count(dev_) * 32),
+            s = tc_launch_t1_decode_stripe(st, g.small ? 1 : 0, std::min(g.n, device_cu_count(dev_) * 32),
                                            up + descs_off_, (const uint32_t*)(up + g.order_off), g.n, up, cw_size_,
                                            base);
         else if (g.bpw == 1)
-            s = tc_launch_t1_decode_wave(stream_, g.small ? 1 : 0, std::min(g.n, device_cu_count(dev_) * 16), up + descs_off_,
-                                         (const uint32_t*)(up + g.order_off), g.n, up, cw_size_, base);
+            s = tc_launch_t1_decode_wave(st, g.small ? 1 : 0, std::min(g.n, device_cu_count(dev_) * 16),
+                                         up + descs_off_, (const uint32_t*)(up + g.order_off), g.n, up, cw_size_,
+                                         base);
         else
-            s = tc_launch_t1_decode(stream_, g.small ? 1 : 0, g.bpw, up + descs_off_,
+            s = tc_launch_t1_decode(st, g.small ? 1 : 0, g.bpw, up + descs_off_,
                                     (const uint32_t*)(up + g.order_off), g.n, up, base);
+    }
+    if (fork && s == TC_SUCCESS) {
+        s = tc_event_record(join_, side_);
+        if (s == TC_SUCCESS) s = tc_stream_wait_event(stream_, join_);
     }
     if (timed) tc_event_record(ev_[2], stream_);
     for (const LevelLaunch& L : levels_) {

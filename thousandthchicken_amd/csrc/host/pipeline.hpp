@@ -122,6 +122,8 @@ class Pipeline {
     Tuning tuning_;
     int shape_ = 0;  // tile shape the prepared job lists were built for
     tc_event ev_[5] = {nullptr, nullptr, nullptr, nullptr, nullptr};
+    tc_stream side_ = nullptr;           // second tier-1 class runs here, beside the first
+    tc_event fork_ = nullptr, join_ = nullptr;
     bool timed_ = false;
     tc_graph graph_ = nullptr;
 };
