@@ -144,6 +144,47 @@ __device__ void tile_idwt(typename Lift<IS97>::T* tile, int pitch, const typenam
     constexpr int W = TX + 2 * H, HH = TY + 2 * H;
     const int tid = threadIdx.x;
     const bool h1 = (u1 - u0) == 1, v1x = (v1 - v0) == 1;
+    // interior tile: the extended window needs no mirroring, and every load of
+    // a thread is issued before the first is used
+    if (!copy_only && ox - H >= u0 && ox + TX + H <= u1 && oy - H >= v0 && oy + TY + H <= v1) {
+        constexpr int NIT = (W * HH + NT - 1) / NT;
+        const int cu0 = ceil_half(u0), fu0 = floor_half(u0), cv0 = ceil_half(v0), fv0 = floor_half(v0);
+        T vals[NIT];
+#pragma unroll
+        for (int k = 0; k < NIT; k++) {
+            const int i = tid + k * NT;
+            if (i < W * HH) {
+                const int ly = i / W, lx = i % W;
+                const int u = ox - H + lx, v = oy - H + ly;
+                const int ou = u & 1, ov = v & 1;
+                const int cx = ou ? floor_half(u) - fu0 : (u >> 1) - cu0;
+                const int cy = ov ? floor_half(v) - fv0 : (v >> 1) - cv0;
+                const uint64_t b = ov ? (ou ? s.hh : s.lh) : (ou ? s.hl : s.ll);
+                const uint32_t bpitch = (ou | ov) ? s.bp : s.llp;
+                vals[k] = src[b + (uint64_t)cy * bpitch + cx];
+            }
+        }
+#pragma unroll
+        for (int k = 0; k < NIT; k++) {
+            const int i = tid + k * NT;
+            if (i < W * HH) {
+                const int ly = i / W, lx = i % W;
+                T val = vals[k];
+                if (IS97) {
+                    const int ou = (ox - H + lx) & 1, ov = (oy - H + ly) & 1;
+                    float f = (float)val;
+                    f *= ou ? kInvK : kK;
+                    f *= ov ? kInvK : kK;
+                    val = (T)f;
+                }
+                tile[ly * pitch + lx] = val;
+            }
+        }
+        __syncthreads();
+        Lift<IS97>::steps(tile, 1, pitch, HH, W, (ox - H) & 1 ? 1 : 0, tid);
+        Lift<IS97>::steps(tile, pitch, 1, W, HH, (oy - H) & 1 ? 1 : 0, tid);
+        return;
+    }
     for (int i = tid; i < W * HH; i += NT) {
         int ly = i / W, lx = i % W;
         int u = ox - H + lx, v = oy - H + ly;
