@@ -313,7 +313,8 @@ struct Blk {
     }
 
     // Per-column words of a stripe for the significance and cleanup passes.
-    // w0: window (18 bits) | candidate rows << 18; w1: signs (own rows -1..4,
+    // w0: window (18 bits) | candidate rows << 18 | significant rows << 22;
+    // w1: signs (own rows -1..4,
     // left rows 0..3 << 6, right rows 0..3 << 10) | zero-coding labels of rows
     // 0..3 from the window at the start of the stripe << 14 (4 bits each).
     __device__ __forceinline__ void column_words(int y0, uint32_t rowm, bool cleanup, uint32_t& w0, uint32_t& w1,
@@ -333,7 +334,7 @@ struct Blk {
             cand = ~s4 & nz & rowm;
         }
         if ((int)threadIdx.x >= w) cand = 0;  // columns outside the block
-        w0 = slice | (cand << 18);
+        w0 = slice | (cand << 18) | (s4 << 22);
         const uint32_t go = win6(sgn, y0);
         const uint32_t g4 = (go >> 1) & 15u;
         w1 = go | (from_left(g4) << 6) | (from_right(g4) << 10);
@@ -354,10 +355,11 @@ struct Blk {
         column_words(y0, rowm, false, vw0, vw1, cols);
         uint32_t res = 0;
         int x = -1;
-        uint32_t carry = 0, csgn = 0;  // new significance of column x: window bits / signs by row
+        // new significance of column x: window bits, rows, signs by row
+        uint32_t carry = 0, crows = 0, csgn = 0;
         for (;;) {
             int nx;
-            if (carry != 0u && x + 1 < w) {
+            if (crows != 0u && x + 1 < w) {
                 nx = x + 1;
             } else {
                 const uint64_t rem = x >= 63 ? 0ull : cols & (~0ull << (x + 1));
@@ -365,27 +367,32 @@ struct Blk {
                 nx = (int)__builtin_ctzll(rem);
             }
             const bool adj = nx == x + 1;
-            const uint32_t lc = adj ? carry : 0u, ls = adj ? csgn : 0u;
+            const uint32_t lc = adj ? carry : 0u, lr = adj ? crows : 0u, ls = adj ? csgn : 0u;
             x = nx;
             const uint32_t w0 = rl(vw0, x), w1 = rl(vw1, x);
             uint32_t sl = (w0 & 0x3FFFFu) | lc;
-            const uint32_t lsg = ((w1 >> 6) & 15u) | ls, rsg = (w1 >> 10) & 15u;
+            const uint32_t s4 = (w0 >> 22) & 15u;
+            // rows to code: the static candidates, and insignificant rows next to
+            // the left column's new significance
+            uint32_t rows = ((w0 >> 18) & 15u) | ((lr | (lr << 1) | (lr >> 1)) & ~s4 & rowm);
             uint32_t osg = w1 & 63u;
             uint32_t vis = 0, nsig = 0;
-            carry = 0;
-            csgn = 0;
-            for (int r = 0; r < nr; r++) {
-                const uint32_t nb = (sl >> (3 * r)) & 0x1FFu;
-                if ((nb & 0x10u) || !(nb & 0x1EFu)) continue;
+            carry = crows = csgn = 0;
+            while (rows) {
+                const int r = (int)__builtin_ctz(rows);
+                rows &= rows - 1u;
                 vis |= 1u << r;
+                const uint32_t nb = (sl >> (3 * r)) & 0x1FFu;
                 const uint32_t lab = ((sl ^ w0) >> (3 * r)) & 0x1FFu ? zc(nb) : (w1 >> (14 + 4 * r)) & 15u;
                 if (dec_ctx(lab)) {
-                    const uint32_t s = dec_sign(sl, r, lsg, osg, rsg);
+                    const uint32_t sg = dec_sign(sl, r, ((w1 >> 6) & 15u) | ls, osg, (w1 >> 10) & 15u);
                     sl |= 1u << (3 * r + 4);
-                    osg |= s << (r + 1);
+                    osg |= sg << (r + 1);
                     nsig |= 1u << r;
                     carry |= 1u << (3 * r + 3);
-                    csgn |= s << r;
+                    crows |= 1u << r;
+                    csgn |= sg << r;
+                    rows |= (2u << r) & ~s4 & rowm;  // the row below now has a significant neighbour
                 }
             }
             if (vis) res = wl(res, (uint32_t)x, vis | (nsig << 4) | (csgn << 8));
@@ -410,27 +417,44 @@ struct Blk {
 #pragma unroll
         for (int r = 0; r < 4; r++) nz |= (((slice >> (3 * r)) & 0x1EFu) != 0u ? 1u : 0u) << r;
         const uint32_t cand = (int)threadIdx.x < w ? s4 & ~p4 & rowm : 0u;
-        uint32_t labs = 0;
+        // the candidates' labels in scan order, 4 bits each (label - 14: 2 if
+        // refined before, else 1 with a significant neighbour, else 0), 15 ends
+        uint32_t stream = 0;
+        int k = 0;
 #pragma unroll
-        for (int r = 0; r < 4; r++) labs |= (((r4 >> r) & 1u) ? 2u : ((nz >> r) & 1u)) << (4 * r);
-        const uint32_t vinfo = cand | (labs << 4);
+        for (int r = 0; r < 4; r++)
+            if ((cand >> r) & 1u) {
+                stream |= (((r4 >> r) & 1u) ? 2u : ((nz >> r) & 1u)) << (4 * k);
+                k++;
+            }
+        stream |= 15u << (4 * k);
         uint64_t cols = __ballot(cand != 0u);
         uint32_t res = 0;
         while (cols) {
             const int x = (int)__builtin_ctzll(cols);
             cols &= cols - 1;
-            const uint32_t iw = rl(vinfo, (uint32_t)x);
-            uint32_t c = iw & 15u, one = 0;
-            while (c) {
-                const int r = (int)__builtin_ctz(c);
-                c &= c - 1u;
-                // label 16 if refined before, else 15 with a significant neighbour, else 14
-                one |= dec_ctx(14u + ((iw >> (4 + 4 * r)) & 15u)) << r;
+            uint32_t iw = rl(stream, (uint32_t)x);
+            uint32_t bits = 0;  // decoded bits, first candidate highest
+            for (;;) {
+                const uint32_t code = iw & 15u;
+                if (code == 15u) break;
+                iw >>= 4;
+                bits = (bits << 1) | dec_ctx(14u + code);
             }
-            if (one) res = wl(res, (uint32_t)x, one);
+            res = wl(res, (uint32_t)x, bits);
         }
+        // back to rows: candidate j of n (scan order) holds bit n-1-j
+        const int n = __popc(cand);
+        uint32_t one = 0;
+        int j = 0;
+#pragma unroll
+        for (int r = 0; r < 4; r++)
+            if ((cand >> r) & 1u) {
+                one |= ((res >> (n - 1 - j)) & 1u) << r;
+                j++;
+            }
         ref |= (uint64_t)cand << y0;
-        ones |= (uint64_t)(res & 15u) << y0;
+        ones |= (uint64_t)one << y0;
     }
 
     // cleanup with run-length mode over one stripe (T.800 D.3.4)
@@ -452,34 +476,33 @@ struct Blk {
             csgn = 0;
             const uint32_t w0 = rl(vw0, (uint32_t)x), w1 = rl(vw1, (uint32_t)x);
             uint32_t sl = (w0 & 0x3FFFFu) | lc;
-            const uint32_t cand = w0 >> 18;
-            const uint32_t lsg = ((w1 >> 6) & 15u) | ls, rsg = (w1 >> 10) & 15u;
+            uint32_t rows = (w0 >> 18) & 15u;
             uint32_t osg = w1 & 63u;
             uint32_t nsig = 0;
-            int r0 = 0;
-            if (cand == 15u && sl == 0u) {
+            if (rows == 15u && sl == 0u) {  // run-length mode
                 if (!mq_dec(m, st_rl, trm, trl)) continue;
                 int r = (int)mq_dec(m, st_uni, trm, trl) << 1;
                 r |= (int)mq_dec(m, st_uni, trm, trl);
-                const uint32_t s = dec_sign(sl, r, lsg, osg, rsg);
+                const uint32_t sg = dec_sign(sl, r, ((w1 >> 6) & 15u) | ls, osg, (w1 >> 10) & 15u);
                 sl |= 1u << (3 * r + 4);
-                osg |= s << (r + 1);
+                osg |= sg << (r + 1);
                 nsig |= 1u << r;
                 carry |= 1u << (3 * r + 3);
-                csgn |= s << r;
-                r0 = r + 1;
+                csgn |= sg << r;
+                rows &= ~((2u << r) - 1u);  // rows after r
             }
-            for (int r = r0; r < nr; r++) {
-                if (!((cand >> r) & 1u)) continue;
+            while (rows) {
+                const int r = (int)__builtin_ctz(rows);
+                rows &= rows - 1u;
                 const uint32_t nb = (sl >> (3 * r)) & 0x1FFu;
                 const uint32_t lab = ((sl ^ w0) >> (3 * r)) & 0x1FFu ? zc(nb) : (w1 >> (14 + 4 * r)) & 15u;
                 if (dec_ctx(lab)) {
-                    const uint32_t s = dec_sign(sl, r, lsg, osg, rsg);
+                    const uint32_t sg = dec_sign(sl, r, ((w1 >> 6) & 15u) | ls, osg, (w1 >> 10) & 15u);
                     sl |= 1u << (3 * r + 4);
-                    osg |= s << (r + 1);
+                    osg |= sg << (r + 1);
                     nsig |= 1u << r;
                     carry |= 1u << (3 * r + 3);
-                    csgn |= s << r;
+                    csgn |= sg << r;
                 }
             }
             if (nsig) res = wl(res, (uint32_t)x, nsig | (csgn << 8));
