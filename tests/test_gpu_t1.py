@@ -1,5 +1,5 @@
-"""Both tier-1 kernels (one wave per block, one lane per block) through their
-typed C-ABI launches on random code-blocks of every shape, orientation, bit
+"""Every tier-1 kernel (one wave per block, one lane per block, 64 blocks per
+wave in lockstep) through its typed C-ABI launch on random code-blocks of every shape, orientation, bit
 depth and truncation point, against the oracle's tier-1 decode."""
 import ctypes
 
@@ -41,7 +41,7 @@ def _expected(b):
     return np.where(sign.astype(bool), -v, v).astype(np.int32)
 
 
-@pytest.mark.parametrize("kernel", ["stripe", "wave", 8, 16, 32, 64])
+@pytest.mark.parametrize("kernel", ["stripe", "wave", "lockstep", "lockstep_unaligned", 8, 16, 32, 64])
 @pytest.mark.parametrize("small", [True, False])
 def test_t1_kernels_on_random_blocks(kernel, small):
     lim = 32 if small else 64
@@ -55,14 +55,16 @@ def test_t1_kernels_on_random_blocks(kernel, small):
     descs = np.zeros(len(blocks), tca.CBLK_DESC)
     pitch = 64
     for i, b in enumerate(blocks):
-        arena += bytes((-len(arena)) % 4)
+        # the lockstep kernel takes codewords at any byte offset
+        arena += bytes(i % 4) if kernel == "lockstep_unaligned" else bytes((-len(arena)) % 4)
         descs[i] = (len(arena), len(b["data"]), i * pitch * 64, pitch, b["w"], b["h"], b["numbps"], b["npasses"],
                     1 if b["orient"] == 1 else (2 if b["orient"] == 3 else 0), 0, 1.0)
         arena += b["data"]
-    arena += bytes((-len(arena)) % 4) + b"\xff" * 16
+    arena += bytes((-len(arena)) % 16) + b"\xff" * 64
     arena = np.frombuffer(bytes(arena), np.uint8)
     n_out = len(blocks) * pitch * 64
-    bufs = [tca.DeviceBuffer(dev, max(x, 256)) for x in (arena.nbytes, descs.nbytes, n_out * 4)]
+    scratch = L.tc_t1_lockstep_scratch_bytes(1 if small else 0, len(blocks))
+    bufs = [tca.DeviceBuffer(dev, max(x, 256)) for x in (arena.nbytes, descs.nbytes, n_out * 4, scratch)]
     try:
         bufs[0].upload(stream, arena)
         bufs[1].upload(stream, descs)
@@ -70,6 +72,11 @@ def test_t1_kernels_on_random_blocks(kernel, small):
         if kernel == "stripe":
             st = L.tc_launch_t1_decode_stripe(stream, 1 if small else 0, 512, bufs[1].ptr, None, len(blocks),
                                               bufs[0].ptr, arena.nbytes, bufs[2].ptr)
+        elif kernel in ("lockstep", "lockstep_unaligned"):
+            # the scratch state needs no initialisation: start it dirty
+            tca.check(L.tc_buf_fill32(stream, bufs[3].handle, 0, 0xFFFFFFFF, scratch // 4), "fill")
+            st = L.tc_launch_t1_decode_lockstep(stream, 1 if small else 0, bufs[1].ptr, None, len(blocks),
+                                                bufs[0].ptr, arena.nbytes, bufs[3].ptr, scratch, bufs[2].ptr)
         elif kernel == "wave":
             st = L.tc_launch_t1_decode_wave(stream, 1 if small else 0, 512, bufs[1].ptr, None, len(blocks),
                                             bufs[0].ptr, arena.nbytes, bufs[2].ptr)

@@ -19,6 +19,7 @@ SOURCES = [
     "kernels/t1_decode.hip",
     "kernels/t1_wave.hip",
     "kernels/t1_stripe.hip",
+    "kernels/t1_lockstep.hip",
     "kernels/idwt.hip",
     "kernels/colour.hip",
     "host/codestream.cpp",

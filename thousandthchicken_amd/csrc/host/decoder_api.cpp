@@ -159,9 +159,14 @@ tc_status tc_decoder_destroy(tc_decoder d) {
     return TC_SUCCESS;
 }
 
+static bool valid_bpw(int b) {
+    return b == 0 || b == 1 || b == 8 || b == 16 || b == 32 || b == 64 || b == TC_BPW_LOCKSTEP;
+}
+
 tc_status tc_decoder_set_blocks_per_wave(tc_decoder d, int small, int bpw) {
-    if (!d || (bpw != 0 && bpw != 1 && bpw != 8 && bpw != 16 && bpw != 32 && bpw != 64))
-        return report(Status::InvalidArgument, "blocks per wave must be 0 (auto), 1 (one wave per block), 8, 16, 32 or 64");
+    if (!d || !valid_bpw(bpw))
+        return report(Status::InvalidArgument,
+                      "blocks per wave must be 0 (auto), 1 (one wave per block), 8, 16, 32, 64 or TC_BPW_LOCKSTEP");
     Tuning t = d->pipe->tuning();
     (small ? t.bpw32 : t.bpw64) = bpw;
     d->pipe->set_tuning(t);
@@ -207,8 +212,7 @@ tc_status tc_decoder_get_tuning(tc_decoder d, tc_tuning* out) {
 tc_status tc_decoder_set_tuning(tc_decoder d, const tc_tuning* in) {
     int nshape = 0;
     tc_idwt_tile_shape(0, nullptr, nullptr, &nshape);
-    auto ok_bpw = [](int b) { return b == 0 || b == 1 || b == 8 || b == 16 || b == 32 || b == 64; };
-    if (!d || !in || !ok_bpw(in->bpw_small) || !ok_bpw(in->bpw_large) || in->t1_kernel < 0 || in->t1_kernel > 1 ||
+    if (!d || !in || !valid_bpw(in->bpw_small) || !valid_bpw(in->bpw_large) || in->t1_kernel < 0 || in->t1_kernel > 1 ||
         in->dwt_shape < 0 || in->dwt_shape >= nshape)
         return report(Status::InvalidArgument, "tc_decoder_set_tuning: value out of range");
     Tuning t = d->pipe->tuning();
@@ -250,7 +254,7 @@ tc_status tc_decoder_autotune(tc_decoder d, int reps, tc_tuning* best) {
     float t1 = 0, dw = 0, best_t1, best_dwt;
     Status s = measure(cur, &best_t1, &best_dwt);
     if (s != Status::Ok) return report(s, err.empty() ? "autotune run failed" : err);
-    const int cand[] = {1, 8, 16, 32, 64};
+    const int cand[] = {1, 8, 16, 32, 64, TC_BPW_LOCKSTEP};
     for (int cls = 0; cls < 2; cls++) {
         if (!has[cls]) continue;
         for (int b : cand) {

@@ -56,13 +56,16 @@ Status from_tc(tc_status s) { return s == TC_SUCCESS ? Status::Ok : (s == TC_ERR
 
 }  // namespace
 
-// Default packing when none is set, from the autotuner's choices on MI355X:
-// 8 blocks per wave (lane kernel) once that still gives every CU two waves,
-// one wave per block (stripe kernel) below, where latency dominates.
+// Default packing when none is set, from timings on MI355X: 8 blocks per wave
+// (lane kernel) once that gives every SIMD about two waves, one wave per block
+// (stripe kernel) below.  One 7680x4320 16-bit lossless image (8227 blocks of
+// 64x64) decodes its tier-1 in 33 ms one wave per block against 72 ms at 8 per
+// wave, where each SIMD holds a single latency-bound wave; four such images
+// (32908 blocks) take 134 ms against 90 ms.
 int pick_blocks_per_wave(int n, bool small, int cus, const Tuning& t) {
     int fixed = small ? t.bpw32 : t.bpw64;
     if (fixed) return fixed;
-    return n >= 8 * 2 * std::max(cus, 1) ? 8 : 1;
+    return n >= 8 * 8 * std::max(cus, 1) ? 8 : 1;
 }
 
 Pipeline::Pipeline(tc_device dev) : dev_(dev) {
@@ -84,6 +87,7 @@ Pipeline::~Pipeline() {
     tc_buf_free(upload_);
     tc_buf_free(samples_);
     tc_buf_free(out_);
+    tc_buf_free(scratch_);
     if (pinned_) tc_host_free_pinned(pinned_);
     tc_stream_destroy(stream_);
 }
@@ -183,7 +187,7 @@ Status Pipeline::prepare(const std::vector<const Image*>& imgs, std::string& err
         }
     }
     cw_bytes_ = (int64_t)cw_end;
-    cw_size_ = ((cw_end + 3) & ~(uint64_t)3) + 16;  // slack of 0xFF for look-ahead reads
+    cw_size_ = ((cw_end + 15) & ~(uint64_t)15) + 64;  // slack of 0xFF for look-ahead reads
     ncblk_ = (int64_t)descs.size();
     t1_samples_ = 0;
     for (const auto& d : descs) t1_samples_ += (int64_t)d.w * d.h;
@@ -192,6 +196,7 @@ Status Pipeline::prepare(const std::vector<const Image*>& imgs, std::string& err
     blob.base = align_up(cw_size_, 256);
     descs_off_ = put(blob, descs);
     int cus = device_cu_count(dev_);
+    uint64_t scratch_total = 0;
     for (int k = 0; k < 2; k++) {
         auto& v = cls_idx[k];
         bpw_[k] = 0;
@@ -203,6 +208,10 @@ Status Pipeline::prepare(const std::vector<const Image*>& imgs, std::string& err
         g.n = (int)v.size();
         for (uint32_t i : v) g.max_planes = std::max<int>(g.max_planes, descs[i].numbps);
         g.order_off = put(blob, v);
+        if (g.bpw == TC_BPW_LOCKSTEP) {
+            g.scratch_off = scratch_total;
+            scratch_total += align_up(tc_t1_lockstep_scratch_bytes(g.small ? 1 : 0, g.n), 256);
+        }
         bpw_[k] = g.bpw;
         groups_.push_back(g);
     }
@@ -358,6 +367,7 @@ Status Pipeline::prepare(const std::vector<const Image*>& imgs, std::string& err
     Status st = ensure(upload_, total, "descriptors", err);
     if (st == Status::Ok) st = ensure(samples_, std::max<uint64_t>(work, 1) * 4, "sample planes", err);
     if (st == Status::Ok) st = ensure(out_, std::max<uint64_t>(out_bytes_total, 1), "output images", err);
+    if (st == Status::Ok && scratch_total) st = ensure(scratch_, scratch_total, "tier-1 state", err);
     if (st == Status::Ok) st = ensure_pinned(total, err);
     if (st != Status::Ok) return st;
     uint8_t* stage = (uint8_t*)pinned_;
@@ -395,7 +405,11 @@ Status Pipeline::run(bool timed) {
         if (s != TC_SUCCESS) break;
         const Group& g = groups_[gi];
         tc_stream st = (fork && gi == 1) ? side_ : stream_;
-        if (g.bpw == 1 && tuning_.t1_kernel == 0 && g.max_planes <= TC_T1_STRIPE_MAX_PLANES(g.small))
+        if (g.bpw == TC_BPW_LOCKSTEP)
+            s = tc_launch_t1_decode_lockstep(st, g.small ? 1 : 0, up + descs_off_, (const uint32_t*)(up + g.order_off),
+                                             g.n, up, cw_size_, (uint8_t*)tc_buf_device_ptr(scratch_) + g.scratch_off,
+                                             tc_buf_size(scratch_) - g.scratch_off, base);
+        else if (g.bpw == 1 && tuning_.t1_kernel == 0 && g.max_planes <= TC_T1_STRIPE_MAX_PLANES(g.small))
             s = tc_launch_t1_decode_stripe(st, g.small ? 1 : 0, std::min(g.n, device_cu_count(dev_) * 32),
                                            up + descs_off_, (const uint32_t*)(up + g.order_off), g.n, up, cw_size_,
                                            base);
@@ -492,6 +506,8 @@ void Pipeline::drop_graph() {
     graph_ = nullptr;
 }
 
-size_t Pipeline::device_bytes() const { return tc_buf_size(upload_) + tc_buf_size(samples_) + tc_buf_size(out_); }
+size_t Pipeline::device_bytes() const {
+    return tc_buf_size(upload_) + tc_buf_size(samples_) + tc_buf_size(out_) + tc_buf_size(scratch_);
+}
 
 }  // namespace tc

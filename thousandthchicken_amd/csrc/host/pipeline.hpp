@@ -17,6 +17,7 @@
 #include <string>
 #include <vector>
 
+#include "../../../include/tc_decode.h"
 #include "../../../include/tc_device.h"
 #include "../tc_kernels.h"
 #include "codestream.hpp"
@@ -84,6 +85,7 @@ class Pipeline {
         int bpw = 1;
         int max_planes = 0;  // most bit-planes of a block in the class
         uint64_t order_off = 0;
+        uint64_t scratch_off = 0;  // lockstep packing: its state area in scratch_
         int n = 0;
     };
     struct LevelLaunch {
@@ -108,7 +110,7 @@ class Pipeline {
 
     tc_device dev_;
     tc_stream stream_ = nullptr;
-    tc_buf upload_ = nullptr, samples_ = nullptr, out_ = nullptr;
+    tc_buf upload_ = nullptr, samples_ = nullptr, out_ = nullptr, scratch_ = nullptr;
     void* pinned_ = nullptr;
     size_t pinned_cap_ = 0;
     std::vector<Group> groups_;

@@ -110,5 +110,45 @@ __device__ __forceinline__ uint64_t trans_entry(int i) {
     return lo | (hi << 32);
 }
 
+// Converts one decoded block from sign-magnitude to its coefficient values.
+// Midpoint reconstruction follows the usual decoder convention: a significant
+// coefficient is rebuilt at mag + 2^(q-1) (reversible, integer) or
+// (mag + 0.5*2^q)*Delta_b (irreversible), q being the lowest bit-plane coded
+// for it.  q is the plane of the last pass, one higher for coefficients that
+// were significant before a final significance-propagation pass and so were
+// not refined in it.
+__device__ void convert_block(int lane, const tc_cblk_desc& d, uint32_t* __restrict__ coef) {
+    int numbps = d.numbps, np = d.npasses;
+    if (numbps <= 0 || np <= 0) return;
+    int kl = np - 1;
+    int plast = numbps - 1 - (kl + 2) / 3;
+    if (plast < 0) plast = 0;
+    int tlast = kl == 0 ? 2 : (kl - 1) % 3;
+    uint32_t* out = coef + d.out_off;
+    const int w = d.w, h = d.h;
+    const uint32_t sigonly = 1u << plast;
+    // the 64 lanes cover 64/span rows per step, span = w rounded up to a power of two
+    int span = 1;
+    while (span < w) span <<= 1;
+    int rows_per = 64 / span;
+    int lx = lane % span, ly = lane / span;
+    if (lx >= w) return;
+    for (int y = ly; y < h; y += rows_per) {
+        uint32_t* p = out + (uint32_t)y * d.out_pitch + lx;
+        uint32_t v = *p;
+        if (!v) continue;
+        uint32_t mag = v & 0x7FFFFFFFu;
+        int q = (tlast == 0 && mag != sigonly) ? plast + 1 : plast;
+        bool neg = (v >> 31) != 0;
+        if (d.mode == 0) {
+            int32_t iv = (int32_t)(mag + (q > 0 ? (1u << (q - 1)) : 0u));
+            *p = (uint32_t)(neg ? -iv : iv);
+        } else {
+            float f = ((float)mag + ldexpf(0.5f, q)) * d.delta;
+            *p = __float_as_uint(neg ? -f : f);
+        }
+    }
+}
+
 }  // namespace
 }  // namespace tc

@@ -1,0 +1,487 @@
+// Tier-1 EBCOT decoding with 64 code-blocks per wavefront advanced in
+// lockstep: every step, each lane decodes exactly one MQ symbol of its own
+// block, so one VALU instruction serves up to 64 blocks.
+//
+// Reference behaviour: g_decode, TC/coefficient_coder.cl:778-819 (pass order),
+// the MQ decoder TC/coefficient_coder.cl:105-252, context formation
+// :330-469 and the three pass functors :471-586, with the by-value no-op
+// setters repaired and the pass count honoured (ITU-T T.800 Annex C, D).
+//
+// Why lockstep.  A lane that runs a block's nested pass / stripe / column /
+// row loops on its own diverges from its neighbours at every branch, and a
+// wave then issues the union of all lanes' paths.  Here the pass and stripe
+// loops are wave-uniform (the host packs blocks with equal pass counts into a
+// wave) and the work inside one stripe of one pass is flattened into a
+// sequence of single-symbol steps.  A step is: pick the next coefficient
+// (find-first-set over the stripe's candidate row masks) or continue a
+// pending multi-symbol sequence (ZC 1 -> sign; run-length 1 -> two uniform
+// symbols -> sign), form the context, decode one symbol, apply it.  All lanes
+// execute the same instruction stream; only the lane masks differ.
+//
+// State.  A 64x64 block needs 4 bits per coefficient (significance, sign,
+// coded-in-this-plane's-significance-pass, refined-once): 2 KiB, 128 KiB for
+// 64 blocks, which LDS cannot hold for more than one wave per CU.  The full
+// state therefore lives in a global scratch area laid out
+// [wave][mask][row][lane] (each row load or store of the 64 lanes is one
+// contiguous 512-byte access) and only the rows of the current stripe and
+// its two halo rows are staged into LDS ([row][lane], conflict-free) while
+// the stripe is decoded.  Context states (19 per lane) and the MQ / zero-coding
+// / sign-coding tables also live in LDS.  Magnitude and sign bits are posted to
+// the zero-filled coefficient plane with atomic ORs (no return), and after the
+// decode the wave converts its blocks in place (convert_block).
+//
+// Codewords are staged through a per-lane ring of 8 dwords in LDS.  The ring
+// is topped up in a wave-uniform step whenever some lane has fewer than four
+// bytes left in it (one symbol consumes at most two), so no global load is in
+// flight across symbol steps and the MQ byte-in reads LDS only.
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+#include "../tc_kernels.h"
+#include "t1_common.hpp"
+
+namespace tc {
+namespace {
+
+constexpr int kLockLabels = 19;  // 0-16 ZC/SC/MR, 17 run-length, 18 uniform
+constexpr int kLaneCount = 64;
+
+// Pending symbol of a multi-symbol sequence.
+enum : uint32_t { PEND_NONE = 0, PEND_SC = 1, PEND_UNI1 = 2, PEND_UNI2 = 3 };
+
+// Staged stripe rows, index into LdsLock::rows: significance rows y0-1..y0+4,
+// then sign rows y0-1..y0+4 (in a refinement pass, which reads no signs, the
+// refined-once rows y0..y0+3 take their place).  The coded-in-this-pass rows
+// are never read inside a pass and stay in registers.
+constexpr int RS = 0, RX = 6, NROWS = 12;
+// Masks of the global scratch state.
+constexpr int GS = 0, GX = 1, GP = 2, GR = 3;
+
+template <typename M>
+struct LdsLock {
+    uint64_t trans[48];
+    uint32_t ctx[kLockLabels][kLaneCount];
+    M rows[NROWS][kLaneCount];
+    uint32_t ring[8][kLaneCount];  // codeword dwords, slot = dword index mod 8
+    uint8_t zc[3][512];  // label by 3x3 significance window (centre ignored)
+    uint8_t sc[256];     // sign label | xor<<4 by (sig,sign) of up,left,right,down
+    int kmax, hmax;
+};
+
+// Per-lane MQ decoder reading its codeword from the LDS ring.
+struct LaneMq {
+    uint32_t A, C;
+    int CT;
+    uint32_t bp, len, cur;
+    uint32_t base;   // arena byte offset of the codeword
+    uint32_t rend;   // arena byte offset one past the ring's last byte
+    const uint32_t* ring;  // this lane's column of the ring (stride kLaneCount)
+};
+
+// Codeword byte i (i <= bp + 3 while the ring holds four bytes past bp);
+// past the end the decoder sees 0xFF.
+__device__ __forceinline__ uint32_t lane_byte(const LaneMq& m, uint32_t i) {
+    const uint32_t a = m.base + i;
+    const uint32_t b = (m.ring[((a >> 2) & 7) * kLaneCount] >> ((a & 3) * 8)) & 0xFFu;
+    return i < m.len ? b : 0xFFu;
+}
+
+// Loads the 16-byte-aligned 32 bytes holding codeword byte bp (clamped to the
+// arena's last 32 bytes; nwords is a multiple of 4) into the ring; at least
+// 16 bytes past bp are then held.
+__device__ __forceinline__ void lane_refill(LaneMq& m, uint32_t* ring_col, const uint32_t* __restrict__ words,
+                                           uint32_t nwords) {
+    uint32_t q0 = ((m.base + m.bp) >> 2) & ~3u;
+    if (q0 > nwords - 8) q0 = nwords - 8;
+    typedef uint32_t u32x4 __attribute__((ext_vector_type(4)));
+    const u32x4* src = reinterpret_cast<const u32x4*>(words + q0);
+    u32x4 v[2];
+#pragma unroll
+    for (int j = 0; j < 2; j++) v[j] = __builtin_nontemporal_load(src + j);
+#pragma unroll
+    for (int j = 0; j < 2; j++) {
+        ring_col[((q0 + 4 * j + 0) & 7) * kLaneCount] = v[j].x;
+        ring_col[((q0 + 4 * j + 1) & 7) * kLaneCount] = v[j].y;
+        ring_col[((q0 + 4 * j + 2) & 7) * kLaneCount] = v[j].z;
+        ring_col[((q0 + 4 * j + 3) & 7) * kLaneCount] = v[j].w;
+    }
+    m.rend = (q0 + 8) * 4;
+}
+
+// BYTEIN, T.800 Figure C.19.
+__device__ __forceinline__ void lane_bytein(LaneMq& m) {
+    const uint32_t b1 = lane_byte(m, m.bp + 1);
+    if (m.cur == 0xFF) {
+        if (b1 > 0x8F) {
+            m.C += 0xFF00;
+            m.CT = 8;
+        } else {
+            m.bp++;
+            m.cur = b1;
+            m.C += b1 << 9;
+            m.CT = 7;
+        }
+    } else {
+        m.bp++;
+        m.cur = b1;
+        m.C += b1 << 8;
+        m.CT = 8;
+    }
+}
+
+// INITDEC, T.800 Figure C.20 (the ring must hold the first bytes).
+__device__ __forceinline__ void lane_init(LaneMq& m) {
+    m.bp = 0;
+    m.cur = lane_byte(m, 0);
+    m.C = m.cur << 16;
+    lane_bytein(m);
+    m.C <<= 7;
+    m.CT -= 7;
+    m.A = 0x8000;
+}
+
+// DECODE, T.800 Figures C.15-C.18, on a packed context state
+// (Qe | I << 16 | MPS << 22); RENORMD shifts by whole runs up to the next
+// BYTEIN.
+__device__ __forceinline__ uint32_t lane_decode(LaneMq& m, uint32_t& st, const uint64_t* trans) {
+    const uint32_t qe = st & 0xFFFFu;
+    const uint32_t mps = (st >> 22) & 1u;
+    uint32_t A = m.A - qe;
+    const bool lps = (m.C >> 16) < qe;
+    if (!lps) m.C -= qe << 16;
+    uint32_t d = mps;
+    if (lps || A < 0x8000u) {
+        const uint64_t e = trans[(st >> 16) & 63];
+        const bool take_mps = lps ? (A < qe) : !(A < qe);
+        const uint32_t e2 = (uint32_t)(e >> 32);
+        const uint32_t nm = mps ^ ((e2 >> 22) & 1u);
+        st = take_mps ? (((uint32_t)e & 0x3FFFFFu) | (mps << 22)) : ((e2 & 0x3FFFFFu) | (nm << 22));
+        d = take_mps ? mps : (mps ^ 1u);
+        if (lps) A = qe;
+        int n = __clz(A) - 16;
+        do {
+            if (m.CT == 0) lane_bytein(m);
+            int s = n < m.CT ? n : m.CT;
+            A <<= s;
+            m.C <<= s;
+            m.CT -= s;
+            n -= s;
+        } while (n > 0);
+    }
+    m.A = A;
+    return d;
+}
+
+// Bits x-1, x, x+1 of m (x-1 in bit 0; columns outside the row read 0).
+template <typename M>
+__device__ __forceinline__ uint32_t col3(M m, int x) {
+    return x > 0 ? (uint32_t)(m >> (x - 1)) & 7u : (uint32_t)(m << 1) & 7u;
+}
+
+// Decodes one stripe of one pass for every lane with work, one symbol per
+// lane per step.  TYPE: 0 significance propagation (T.800 D.3.1), 1 magnitude
+// refinement (D.3.3), 2 cleanup with run-length mode (D.3.4).  c0..c3 hold
+// the candidate columns of the stripe's rows (visited ones are cleared, scan
+// order is column-major); TYPE 0: e0..e3 the rows that may still join and
+// p0..p3 collect every position that became a candidate (= coded in the
+// pass); TYPE 2: sany is the OR of the six staged significance rows.  out
+// points at row y0.
+template <typename M, int TYPE>
+__device__ __forceinline__ void stripe_symbols(LdsLock<M>& L, int lane, LaneMq& m, uint32_t* ring_col,
+                                               const uint32_t* __restrict__ words, uint32_t nwords, bool work, M& c0,
+                                               M& c1, M& c2, M& c3, M e0, M e1, M e2, M e3, M& p0, M& p1, M& p2,
+                                               M& p3, M& sany, uint32_t cls, uint32_t* __restrict__ out,
+                                               uint32_t pitch, uint32_t pbit) {
+    auto lr = [&](int i) -> M& { return L.rows[i][lane]; };
+    uint32_t pend = PEND_NONE;
+    int x = 0, r = 0;
+    while (__ballot(work)) {
+        // top up the rings of lanes running short before anyone reads past them
+        if (__ballot(work && m.rend - (m.base + m.bp) < 4u)) {
+            if (work && m.rend - (m.base + m.bp) < 16u) lane_refill(m, ring_col, words, nwords);
+        }
+        if (work) {
+            bool rl = false;
+            if (pend == PEND_NONE) {
+                // next coefficient in stripe scan order: lowest column, then lowest row
+                const M any = c0 | c1 | c2 | c3;
+                x = (int)__builtin_ctzll((unsigned long long)any);
+                const M bx = (M)1 << x;
+                const bool b0 = (c0 & bx) != 0, b1 = (c1 & bx) != 0, b2 = (c2 & bx) != 0, b3 = (c3 & bx) != 0;
+                r = b0 ? 0 : (b1 ? 1 : (b2 ? 2 : 3));
+                c0 &= ~(b0 ? bx : (M)0);
+                c1 &= ~((!b0 && b1) ? bx : (M)0);
+                c2 &= ~((!b0 && !b1 && b2) ? bx : (M)0);
+                c3 &= ~((!b0 && !b1 && !b2) ? bx : (M)0);
+                if (TYPE == 2) rl = b0 && b1 && b2 && b3 && col3<M>(sany, x) == 0;
+            }
+            const M bx = (M)1 << x;
+            uint32_t lab, sxor = 0;
+            if (TYPE == 1) {
+                // any significant neighbour (the centre itself is significant)
+                const uint32_t nb =
+                    col3<M>(lr(RS + r), x) | (col3<M>(lr(RS + r + 1), x) & 5u) | col3<M>(lr(RS + r + 2), x);
+                const bool refined = (lr(RX + r) & bx) != 0;
+                lab = refined ? 16u : (nb ? 15u : 14u);
+            } else if (pend == PEND_NONE && !rl) {
+                const uint32_t w9 = col3<M>(lr(RS + r), x) | (col3<M>(lr(RS + r + 1), x) << 3) |
+                                    (col3<M>(lr(RS + r + 2), x) << 6);
+                lab = L.zc[cls][w9];
+            } else if (pend == PEND_SC) {
+                const uint32_t w9 = col3<M>(lr(RS + r), x) | (col3<M>(lr(RS + r + 1), x) << 3) |
+                                    (col3<M>(lr(RS + r + 2), x) << 6);
+                const uint32_t xw = col3<M>(lr(RX + r), x) | (col3<M>(lr(RX + r + 1), x) << 3) |
+                                    (col3<M>(lr(RX + r + 2), x) << 6);
+                const uint32_t e = L.sc[(w9 & 0xAAu) | ((xw & 0xAAu) >> 1)];
+                lab = e & 15u;
+                sxor = e >> 4;
+            } else {
+                lab = rl ? 17u : 18u;  // run-length or uniform
+            }
+            uint32_t st = L.ctx[lab][lane];
+            const uint32_t bit = lane_decode(m, st, L.trans);
+            L.ctx[lab][lane] = st;
+            uint32_t* const at = out + (uint32_t)r * pitch + x;
+            if (TYPE == 1) {
+                if (bit) __hip_atomic_fetch_or(at, pbit, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+            } else if (pend == PEND_NONE && !rl) {
+                // zero coding
+                if (bit) {
+                    lr(RS + r + 1) |= bx;
+                    pend = PEND_SC;
+                    if (TYPE == 2) sany |= bx;
+                    if (TYPE == 0) {
+                        // neighbours later in scan order may now be coded in this pass
+                        const M nx = bx << 1;
+                        const M a0 = ((r <= 1) ? nx : (M)0) & e0;
+                        const M a1 = (((r <= 2) ? nx : (M)0) | ((r == 0) ? bx : (M)0)) & e1;
+                        const M a2 = (((r >= 1) ? nx : (M)0) | ((r == 1) ? bx : (M)0)) & e2;
+                        const M a3 = (((r >= 2) ? nx : (M)0) | ((r == 2) ? bx : (M)0)) & e3;
+                        c0 |= a0, c1 |= a1, c2 |= a2, c3 |= a3;
+                        p0 |= a0, p1 |= a1, p2 |= a2, p3 |= a3;
+                    }
+                }
+            } else if (pend == PEND_SC) {
+                const uint32_t sg = bit ^ sxor;
+                if (sg) lr(RX + r + 1) |= bx;
+                __hip_atomic_fetch_or(at, pbit | (sg << 31), __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+                pend = PEND_NONE;
+            } else if (TYPE == 2) {
+                if (rl) {
+                    if (bit) {
+                        pend = PEND_UNI1;
+                    } else {
+                        c1 &= ~bx;
+                        c2 &= ~bx;
+                        c3 &= ~bx;
+                    }
+                } else if (pend == PEND_UNI1) {
+                    r = (int)(bit << 1);
+                    pend = PEND_UNI2;
+                } else {
+                    r |= (int)bit;
+                    c1 &= ~((r >= 1) ? bx : (M)0);
+                    c2 &= ~((r >= 2) ? bx : (M)0);
+                    c3 &= ~((r >= 3) ? bx : (M)0);
+                    lr(RS + r + 1) |= bx;
+                    sany |= bx;
+                    pend = PEND_SC;
+                }
+            }
+            work = pend != PEND_NONE || ((c0 | c1 | c2 | c3) != 0);
+        }
+    }
+}
+
+template <typename M, int MAXH>
+__device__ void decode_lockstep(LdsLock<M>& L, const tc_cblk_desc& d, bool active, int lane,
+                                const uint32_t* __restrict__ words, uint32_t nwords, uint32_t* __restrict__ coef,
+                                M* __restrict__ g) {
+    constexpr int NB = (int)sizeof(M) * 8;
+    const int w = active ? d.w : 0, h = active ? d.h : 0;
+    const int numbps = active ? d.numbps : 0, npasses = active ? d.npasses : 0;
+    const uint32_t cls = active ? d.cls : 0;
+    const M VM = (w >= NB) ? ~(M)0 : (((M)1 << w) - 1);
+    uint32_t* const out = coef + (active ? d.out_off : 0);
+    const uint32_t pitch = active ? d.out_pitch : 0;
+
+    for (int c = 0; c < kLockLabels; c++)
+        L.ctx[c][lane] = pack_state(c == 0 ? 4 : (c == 17 ? 3 : (c == 18 ? 46 : 0)), 0);
+    LaneMq m;
+    uint32_t* const ring_col = &L.ring[0][lane];
+    m.ring = ring_col;
+    m.base = active ? d.data_off : 0;
+    m.len = active ? d.data_len : 0;
+    m.bp = 0;
+    if (active) {
+        lane_refill(m, ring_col, words, nwords);
+        lane_init(m);
+    }
+
+    auto gat = [&](int mask, int row) -> M& { return g[(mask * MAXH + row) * kLaneCount + lane]; };
+    auto lr = [&](int i) -> M& { return L.rows[i][lane]; };
+
+    M carry_s = 0, carry_x = 0;  // rows y0-1 of the previous stripe
+    const int kmax = L.kmax, hmax = L.hmax;
+    for (int k = 0; k < kmax; k++) {
+        const int type = k == 0 ? 2 : (k - 1) % 3;  // 0 sigprop, 1 magref, 2 cleanup (wave-uniform)
+        const int p = numbps - 1 - (k + 2) / 3;
+        const bool pass_on = active && k < npasses && p >= 0;
+        const uint32_t pbit = pass_on ? (1u << p) : 0u;
+        for (int y0 = 0; y0 < hmax; y0 += 4) {
+            const bool on = pass_on && y0 < h;
+            M c0 = 0, c1 = 0, c2 = 0, c3 = 0;  // candidate columns per stripe row
+            M e0 = 0, e1 = 0, e2 = 0, e3 = 0;  // sigprop: rows that may still join; magref: refined rows after
+            M p0 = 0, p1 = 0, p2 = 0, p3 = 0;  // sigprop: positions coded in the pass
+            M sany = 0;                        // OR of the six significance rows
+            if (on) {
+                M S[6], X[6], P[4], R[4];
+                S[0] = y0 > 0 ? carry_s : (M)0;
+                X[0] = (y0 > 0 && type != 1) ? carry_x : (M)0;
+#pragma unroll
+                for (int r = 0; r < 5; r++) {
+                    const bool ok = k > 0 && y0 + r < h;
+                    S[r + 1] = ok ? gat(GS, y0 + r) : (M)0;
+                    X[r + 1] = (ok && type != 1) ? gat(GX, y0 + r) : (M)0;
+                }
+#pragma unroll
+                for (int r = 0; r < 4; r++) {
+                    const bool ok = k > 0 && y0 + r < h;
+                    P[r] = (ok && type != 0) ? gat(GP, y0 + r) : (M)0;
+                    R[r] = (ok && type == 1) ? gat(GR, y0 + r) : (M)0;
+                }
+#pragma unroll
+                for (int i = 0; i < 6; i++) {
+                    lr(RS + i) = S[i];
+                    sany |= S[i];
+                }
+                if (type == 1) {
+#pragma unroll
+                    for (int r = 0; r < 4; r++) lr(RX + r) = R[r];
+                } else {
+#pragma unroll
+                    for (int i = 0; i < 6; i++) lr(RX + i) = X[i];
+                }
+                M cr[4], er[4];
+#pragma unroll
+                for (int r = 0; r < 4; r++) {
+                    const M vr = (y0 + r < h) ? VM : (M)0;
+                    if (type == 0) {
+                        er[r] = ~S[r + 1] & vr;
+                        cr[r] = er[r] & spread<M>(S[r] | S[r + 1] | S[r + 2]);
+                    } else if (type == 1) {
+                        cr[r] = S[r + 1] & ~P[r];
+                        er[r] = R[r] | cr[r];  // every candidate is refined in the pass
+                    } else {
+                        er[r] = 0;
+                        cr[r] = ~S[r + 1] & ~P[r] & vr;
+                    }
+                }
+                c0 = cr[0], c1 = cr[1], c2 = cr[2], c3 = cr[3];
+                e0 = er[0], e1 = er[1], e2 = er[2], e3 = er[3];
+                p0 = c0, p1 = c1, p2 = c2, p3 = c3;
+            }
+
+            const bool work = on && ((c0 | c1 | c2 | c3) != 0);
+            if (type == 0)
+                stripe_symbols<M, 0>(L, lane, m, ring_col, words, nwords, work, c0, c1, c2, c3, e0, e1, e2, e3, p0,
+                                     p1, p2, p3, sany, cls, out + (uint32_t)y0 * pitch, pitch, pbit);
+            else if (type == 1)
+                stripe_symbols<M, 1>(L, lane, m, ring_col, words, nwords, work, c0, c1, c2, c3, e0, e1, e2, e3, p0,
+                                     p1, p2, p3, sany, cls, out + (uint32_t)y0 * pitch, pitch, pbit);
+            else
+                stripe_symbols<M, 2>(L, lane, m, ring_col, words, nwords, work, c0, c1, c2, c3, e0, e1, e2, e3, p0,
+                                     p1, p2, p3, sany, cls, out + (uint32_t)y0 * pitch, pitch, pbit);
+
+            if (on) {
+                const M pr[4] = {p0, p1, p2, p3}, rr[4] = {e0, e1, e2, e3};
+#pragma unroll
+                for (int i = 0; i < 4; i++) {
+                    if (type != 1) {
+                        gat(GS, y0 + i) = lr(RS + i + 1);
+                        gat(GX, y0 + i) = lr(RX + i + 1);
+                        gat(GP, y0 + i) = type == 0 ? pr[i] : (M)0;
+                    }
+                    // the first pass also clears the refined rows the scratch holds
+                    if (type == 1 || k == 0) gat(GR, y0 + i) = type == 1 ? rr[i] : (M)0;
+                }
+                carry_s = lr(RS + 4);
+                carry_x = lr(RX + 4);
+            }
+        }
+    }
+}
+
+template <typename M, int MAXH>
+__global__ __launch_bounds__(64) void t1_lockstep_kernel(const tc_cblk_desc* __restrict__ descs,
+                                                         const uint32_t* __restrict__ order, int n,
+                                                         const uint8_t* __restrict__ arena, uint32_t arena_words,
+                                                         uint32_t* __restrict__ coef, M* __restrict__ scratch) {
+    __shared__ LdsLock<M> L;
+    const int lane = threadIdx.x;
+    for (int i = lane; i < 47; i += 64) L.trans[i] = trans_entry(i);
+    for (int i = lane; i < 3 * 512; i += 64) {
+        const uint32_t w9 = (uint32_t)(i & 511);
+        L.zc[i >> 9][w9] = (uint8_t)zc_label(i >> 9, (w9 & 0xFu) | ((w9 >> 1) & 0xF0u));
+    }
+    for (int i = lane; i < 256; i += 64) {
+        // index bits: sgn_u, sig_u, sgn_l, sig_l, sgn_r, sig_r, sgn_d, sig_d (bit 0 up)
+        const uint32_t u = (uint32_t)i;
+        const uint32_t e = ((u >> 3) & 1u) | (((u >> 2) & 1u) << 1) | (((u >> 5) & 1u) << 2) | (((u >> 4) & 1u) << 3) |
+                           (((u >> 1) & 1u) << 4) | ((u & 1u) << 5) | (((u >> 7) & 1u) << 6) | (((u >> 6) & 1u) << 7);
+        L.sc[i] = (uint8_t)sc_entry(e);
+    }
+    if (lane == 0) {
+        L.kmax = 0;
+        L.hmax = 0;
+    }
+    __syncthreads();
+
+    const int base = blockIdx.x * kLaneCount;
+    const bool have = base + lane < n;
+    tc_cblk_desc d;
+    if (have) d = descs[order ? order[base + lane] : (uint32_t)(base + lane)];
+    const bool active = have && d.numbps > 0 && d.npasses > 0;
+    if (active) {
+        atomicMax(&L.kmax, (int)d.npasses);
+        atomicMax(&L.hmax, (int)d.h);
+    }
+    __syncthreads();
+    decode_lockstep<M, MAXH>(L, d, active, lane, reinterpret_cast<const uint32_t*>(arena), arena_words, coef,
+                             scratch + (size_t)blockIdx.x * 4 * MAXH * kLaneCount);
+
+    // make every lane's posted ORs visible, and drop stale L1 lines, before the
+    // cooperative read-back
+    __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "agent");
+    __syncthreads();
+    for (int j = 0; j < kLaneCount && base + j < n; j++) {
+        const tc_cblk_desc dj = descs[order ? order[base + j] : (uint32_t)(base + j)];
+        convert_block(lane, dj, coef);
+    }
+}
+
+template <typename M, int MAXH>
+static hipError_t launch_lockstep_variant(hipStream_t s, const tc_cblk_desc* descs, const uint32_t* order, int n,
+                                          const uint8_t* arena, uint32_t arena_bytes, uint32_t* coef, void* scratch) {
+    if (n <= 0) return hipSuccess;
+    const int grid = (n + kLaneCount - 1) / kLaneCount;
+    hipLaunchKernelGGL((t1_lockstep_kernel<M, MAXH>), dim3(grid), dim3(kLaneCount), 0, s, descs, order, n, arena,
+                       arena_bytes / 4, coef, static_cast<M*>(scratch));
+    return hipGetLastError();
+}
+
+}  // namespace
+
+// Scratch bytes the lockstep kernel needs for n blocks of one size class.
+size_t t1_lockstep_scratch_bytes(bool cls32, int n) {
+    const size_t waves = (size_t)((n + kLaneCount - 1) / kLaneCount);
+    return cls32 ? waves * 4 * 32 * kLaneCount * sizeof(uint32_t) : waves * 4 * 64 * kLaneCount * sizeof(uint64_t);
+}
+
+hipError_t launch_t1_lockstep(hipStream_t s, bool cls32, const tc_cblk_desc* descs, const uint32_t* order, int n,
+                              const uint8_t* arena, uint32_t arena_bytes, uint32_t* coef, void* scratch) {
+    if (cls32) return launch_lockstep_variant<uint32_t, 32>(s, descs, order, n, arena, arena_bytes, coef, scratch);
+    return launch_lockstep_variant<uint64_t, 64>(s, descs, order, n, arena, arena_bytes, coef, scratch);
+}
+
+}  // namespace tc
