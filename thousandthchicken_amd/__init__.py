@@ -13,6 +13,8 @@ import numpy as np
 
 _HERE = os.path.dirname(os.path.abspath(__file__))
 LIB_PATH = os.path.join(_HERE, "libtc.so")
+# Packing value: 64 code-blocks per wavefront decoded in lockstep (TC_BPW_LOCKSTEP).
+BPW_LOCKSTEP = -64
 
 _lib = None
 

@@ -56,16 +56,16 @@ Status from_tc(tc_status s) { return s == TC_SUCCESS ? Status::Ok : (s == TC_ERR
 
 }  // namespace
 
-// Default packing when none is set, from timings on MI355X: 8 blocks per wave
-// (lane kernel) once that gives every SIMD about two waves, one wave per block
-// (stripe kernel) below.  One 7680x4320 16-bit lossless image (8227 blocks of
-// 64x64) decodes its tier-1 in 33 ms one wave per block against 72 ms at 8 per
-// wave, where each SIMD holds a single latency-bound wave; four such images
-// (32908 blocks) take 134 ms against 90 ms.
+// Default packing when none is set, from timings on MI355X: 64 blocks per wave
+// in lockstep from 64 blocks per CU up, one wave per block (stripe kernel)
+// below, where a few long lockstep waves would leave most SIMDs idle.  One
+// 7680x4320 16-bit lossless image (8227 blocks of 64x64, 129 lockstep waves)
+// decodes its tier-1 in 33 ms one wave per block against 96 ms in lockstep;
+// four such images take 134 ms against 85 ms, sixteen 530 ms against 84 ms.
 int pick_blocks_per_wave(int n, bool small, int cus, const Tuning& t) {
     int fixed = small ? t.bpw32 : t.bpw64;
     if (fixed) return fixed;
-    return n >= 8 * 8 * std::max(cus, 1) ? 8 : 1;
+    return n >= 64 * std::max(cus, 1) ? TC_BPW_LOCKSTEP : 1;
 }
 
 Pipeline::Pipeline(tc_device dev) : dev_(dev) {
