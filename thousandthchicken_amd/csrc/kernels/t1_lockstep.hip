@@ -50,9 +50,10 @@ constexpr int kLaneCount = 64;
 enum : uint32_t { PEND_NONE = 0, PEND_SC = 1, PEND_UNI1 = 2, PEND_UNI2 = 3 };
 
 // Staged stripe rows, index into LdsLock::rows: significance rows y0-1..y0+4,
-// then sign rows y0-1..y0+4 (in a refinement pass, which reads no signs, the
-// refined-once rows y0..y0+3 take their place).  The coded-in-this-pass rows
-// are never read inside a pass and stay in registers.
+// then sign rows y0-1..y0+4.  A refinement pass stages nothing: significance
+// is fixed during it, so its context labels are formed as masks at stripe
+// entry.  The coded-in-this-pass rows are never read inside a pass and stay
+// in registers.
 constexpr int RS = 0, RX = 6, NROWS = 12;
 // Masks of the global scratch state.
 constexpr int GS = 0, GX = 1, GP = 2, GR = 3;
@@ -144,38 +145,41 @@ __device__ __forceinline__ void lane_init(LaneMq& m) {
 // (Qe | I << 16 | MPS << 22); RENORMD shifts by whole runs up to the next
 // BYTEIN.
 __device__ __forceinline__ uint32_t lane_decode(LaneMq& m, uint32_t& st, const uint64_t* trans) {
+    // Nearly every step has some lane that renormalises, so the exchange is
+    // computed for all lanes with selects rather than behind a branch.
     const uint32_t qe = st & 0xFFFFu;
     const uint32_t mps = (st >> 22) & 1u;
+    const uint64_t e = trans[(st >> 16) & 63];
     uint32_t A = m.A - qe;
     const bool lps = (m.C >> 16) < qe;
-    if (!lps) m.C -= qe << 16;
-    uint32_t d = mps;
-    if (lps || A < 0x8000u) {
-        const uint64_t e = trans[(st >> 16) & 63];
-        const bool take_mps = lps ? (A < qe) : !(A < qe);
-        const uint32_t e2 = (uint32_t)(e >> 32);
-        const uint32_t nm = mps ^ ((e2 >> 22) & 1u);
-        st = take_mps ? (((uint32_t)e & 0x3FFFFFu) | (mps << 22)) : ((e2 & 0x3FFFFFu) | (nm << 22));
-        d = take_mps ? mps : (mps ^ 1u);
-        if (lps) A = qe;
-        int n = __clz(A) - 16;
-        do {
-            if (m.CT == 0) lane_bytein(m);
-            int s = n < m.CT ? n : m.CT;
-            A <<= s;
-            m.C <<= s;
-            m.CT -= s;
-            n -= s;
-        } while (n > 0);
+    m.C = lps ? m.C : m.C - (qe << 16);
+    const bool renorm = lps || A < 0x8000u;
+    const bool take_mps = lps == (A < qe);  // LPS exchange: A < Qe gives MPS; MPS exchange: the reverse
+    const uint32_t e2 = (uint32_t)(e >> 32);
+    const uint32_t nm = mps ^ ((e2 >> 22) & 1u);
+    const uint32_t st2 = take_mps ? (((uint32_t)e & 0x3FFFFFu) | (mps << 22)) : ((e2 & 0x3FFFFFu) | (nm << 22));
+    const uint32_t d = (renorm && !take_mps) ? (mps ^ 1u) : mps;
+    st = renorm ? st2 : st;
+    A = lps ? qe : A;
+    int n = __clz(A) - 16;  // 0 unless renormalising
+    while (n > 0) {
+        if (m.CT == 0) lane_bytein(m);
+        const int s = n < m.CT ? n : m.CT;
+        A <<= s;
+        m.C <<= s;
+        m.CT -= s;
+        n -= s;
     }
     m.A = A;
     return d;
 }
 
-// Bits x-1, x, x+1 of m (x-1 in bit 0; columns outside the row read 0).
+// Bits x-1, x, x+1 of m (x-1 in bit 0; columns outside the row read 0),
+// without a branch, so that the row loads feeding it issue together.
 template <typename M>
 __device__ __forceinline__ uint32_t col3(M m, int x) {
-    return x > 0 ? (uint32_t)(m >> (x - 1)) & 7u : (uint32_t)(m << 1) & 7u;
+    const int sh = x > 0 ? x - 1 : 0;
+    return ((uint32_t)(m >> sh) << (x == 0 ? 1 : 0)) & 7u;
 }
 
 // Decodes one stripe of one pass for every lane with work, one symbol per
@@ -184,8 +188,10 @@ __device__ __forceinline__ uint32_t col3(M m, int x) {
 // the candidate columns of the stripe's rows (visited ones are cleared, scan
 // order is column-major); TYPE 0: e0..e3 the rows that may still join and
 // p0..p3 collect every position that became a candidate (= coded in the
-// pass); TYPE 2: sany is the OR of the six staged significance rows.  out
-// points at row y0.
+// pass); TYPE 1: e0..e3 mark candidates refined before (label 16), p0..p3
+// the others with a significant neighbour (label 15; the rest take 14);
+// TYPE 2: sany is the OR of the six staged significance rows.  out points at
+// row y0.
 template <typename M, int TYPE>
 __device__ __forceinline__ void stripe_symbols(LdsLock<M>& L, int lane, LaneMq& m, uint32_t* ring_col,
                                                const uint32_t* __restrict__ words, uint32_t nwords, bool work, M& c0,
@@ -218,11 +224,10 @@ __device__ __forceinline__ void stripe_symbols(LdsLock<M>& L, int lane, LaneMq& 
             const M bx = (M)1 << x;
             uint32_t lab, sxor = 0;
             if (TYPE == 1) {
-                // any significant neighbour (the centre itself is significant)
-                const uint32_t nb =
-                    col3<M>(lr(RS + r), x) | (col3<M>(lr(RS + r + 1), x) & 5u) | col3<M>(lr(RS + r + 2), x);
-                const bool refined = (lr(RX + r) & bx) != 0;
-                lab = refined ? 16u : (nb ? 15u : 14u);
+                // e: refined before, p: not refined but with a significant neighbour
+                const M er = r == 0 ? e0 : (r == 1 ? e1 : (r == 2 ? e2 : e3));
+                const M pr = r == 0 ? p0 : (r == 1 ? p1 : (r == 2 ? p2 : p3));
+                lab = (er & bx) ? 16u : ((pr & bx) ? 15u : 14u);
             } else if (pend == PEND_NONE && !rl) {
                 const uint32_t w9 = col3<M>(lr(RS + r), x) | (col3<M>(lr(RS + r + 1), x) << 3) |
                                     (col3<M>(lr(RS + r + 2), x) << 6);
@@ -350,17 +355,13 @@ __device__ void decode_lockstep(LdsLock<M>& L, const tc_cblk_desc& d, bool activ
                     P[r] = (ok && type != 0) ? gat(GP, y0 + r) : (M)0;
                     R[r] = (ok && type == 1) ? gat(GR, y0 + r) : (M)0;
                 }
+                if (type != 1) {
 #pragma unroll
-                for (int i = 0; i < 6; i++) {
-                    lr(RS + i) = S[i];
-                    sany |= S[i];
-                }
-                if (type == 1) {
-#pragma unroll
-                    for (int r = 0; r < 4; r++) lr(RX + r) = R[r];
-                } else {
-#pragma unroll
-                    for (int i = 0; i < 6; i++) lr(RX + i) = X[i];
+                    for (int i = 0; i < 6; i++) {
+                        lr(RS + i) = S[i];
+                        lr(RX + i) = X[i];
+                        sany |= S[i];
+                    }
                 }
                 M cr[4], er[4];
 #pragma unroll
@@ -371,7 +372,7 @@ __device__ void decode_lockstep(LdsLock<M>& L, const tc_cblk_desc& d, bool activ
                         cr[r] = er[r] & spread<M>(S[r] | S[r + 1] | S[r + 2]);
                     } else if (type == 1) {
                         cr[r] = S[r + 1] & ~P[r];
-                        er[r] = R[r] | cr[r];  // every candidate is refined in the pass
+                        er[r] = cr[r] & R[r];
                     } else {
                         er[r] = 0;
                         cr[r] = ~S[r + 1] & ~P[r] & vr;
@@ -380,6 +381,19 @@ __device__ void decode_lockstep(LdsLock<M>& L, const tc_cblk_desc& d, bool activ
                 c0 = cr[0], c1 = cr[1], c2 = cr[2], c3 = cr[3];
                 e0 = er[0], e1 = er[1], e2 = er[2], e3 = er[3];
                 p0 = c0, p1 = c1, p2 = c2, p3 = c3;
+                if (type == 1) {
+                    // significance does not change in a refinement pass: every label is known now
+                    M pr[4];
+#pragma unroll
+                    for (int r = 0; r < 4; r++) {
+                        const M nb = spread<M>(S[r] | S[r + 2]) | (S[r + 1] << 1) | (S[r + 1] >> 1);
+                        pr[r] = cr[r] & ~R[r] & nb;
+                        // after the pass every coefficient significant before it has been refined
+                        gat(GR, y0 + r) = cr[r];
+                    }
+                    p0 = pr[0], p1 = pr[1], p2 = pr[2], p3 = pr[3];
+                    carry_s = S[4];
+                }
             }
 
             const bool work = on && ((c0 | c1 | c2 | c3) != 0);
@@ -394,19 +408,19 @@ __device__ void decode_lockstep(LdsLock<M>& L, const tc_cblk_desc& d, bool activ
                                      p1, p2, p3, sany, cls, out + (uint32_t)y0 * pitch, pitch, pbit);
 
             if (on) {
-                const M pr[4] = {p0, p1, p2, p3}, rr[4] = {e0, e1, e2, e3};
+                if (type != 1) {
+                    const M pr[4] = {p0, p1, p2, p3};
 #pragma unroll
-                for (int i = 0; i < 4; i++) {
-                    if (type != 1) {
+                    for (int i = 0; i < 4; i++) {
                         gat(GS, y0 + i) = lr(RS + i + 1);
                         gat(GX, y0 + i) = lr(RX + i + 1);
                         gat(GP, y0 + i) = type == 0 ? pr[i] : (M)0;
+                        // the first pass also clears the refined rows the scratch holds
+                        if (k == 0) gat(GR, y0 + i) = (M)0;
                     }
-                    // the first pass also clears the refined rows the scratch holds
-                    if (type == 1 || k == 0) gat(GR, y0 + i) = type == 1 ? rr[i] : (M)0;
+                    carry_s = lr(RS + 4);
+                    carry_x = lr(RX + 4);
                 }
-                carry_s = lr(RS + 4);
-                carry_x = lr(RX + 4);
             }
         }
     }
