@@ -182,24 +182,37 @@ __device__ __forceinline__ uint32_t col3(M m, int x) {
     return ((uint32_t)(m >> sh) << (x == 0 ? 1 : 0)) & 7u;
 }
 
+// Bit x of four row masks as a 4-bit column nibble (row 0 in bit 0).
+template <typename M>
+__device__ __forceinline__ uint32_t nibble(M a0, M a1, M a2, M a3, int x) {
+    return ((uint32_t)(a0 >> x) & 1u) | (((uint32_t)(a1 >> x) & 1u) << 1) | (((uint32_t)(a2 >> x) & 1u) << 2) |
+           (((uint32_t)(a3 >> x) & 1u) << 3);
+}
+
 // Decodes one stripe of one pass for every lane with work, one symbol per
 // lane per step.  TYPE: 0 significance propagation (T.800 D.3.1), 1 magnitude
-// refinement (D.3.3), 2 cleanup with run-length mode (D.3.4).  c0..c3 hold
-// the candidate columns of the stripe's rows (visited ones are cleared, scan
-// order is column-major); TYPE 0: e0..e3 the rows that may still join and
-// p0..p3 collect every position that became a candidate (= coded in the
-// pass); TYPE 1: e0..e3 mark candidates refined before (label 16), p0..p3
-// the others with a significant neighbour (label 15; the rest take 14);
-// TYPE 2: sany is the OR of the six staged significance rows.  out points at
-// row y0.
+// refinement (D.3.3), 2 cleanup with run-length mode (D.3.4).
+//
+// c0..c3 hold the candidate columns of the stripe's rows.  A lane walks them
+// in scan order with a column cursor: `cany` holds the columns still to visit;
+// taking the lowest one loads its rows as a nibble `cur`, whose set bits are
+// then popped one per step.  TYPE 0: e0..e3 are the rows that may still join;
+// a coefficient turning significant adds its later neighbours (the row below
+// in this column through `cur`, three rows of the next column through c0..c3
+// and `cany`), so on return c0..c3 hold every position coded in the pass.
+// TYPE 1: e0..e3 mark candidates refined before (label 16), p0..p3 the others
+// with a significant neighbour (label 15; the rest take 14).  TYPE 2: sany is
+// the OR of the six staged significance rows (run-length test).  out points
+// at row y0.
 template <typename M, int TYPE>
 __device__ __forceinline__ void stripe_symbols(LdsLock<M>& L, int lane, LaneMq& m, uint32_t* ring_col,
                                                const uint32_t* __restrict__ words, uint32_t nwords, bool work, M& c0,
-                                               M& c1, M& c2, M& c3, M e0, M e1, M e2, M e3, M& p0, M& p1, M& p2,
-                                               M& p3, M& sany, uint32_t cls, uint32_t* __restrict__ out,
-                                               uint32_t pitch, uint32_t pbit) {
+                                               M& c1, M& c2, M& c3, M e0, M e1, M e2, M e3, M p0, M p1, M p2, M p3,
+                                               M& sany, uint32_t cls, uint32_t* __restrict__ out, uint32_t pitch,
+                                               uint32_t pbit) {
     auto lr = [&](int i) -> M& { return L.rows[i][lane]; };
-    uint32_t pend = PEND_NONE;
+    M cany = c0 | c1 | c2 | c3;
+    uint32_t pend = PEND_NONE, cur = 0, enib = 0, pnib = 0;
     int x = 0, r = 0;
     while (__ballot(work)) {
         // top up the rings of lanes running short before anyone reads past them
@@ -209,25 +222,22 @@ __device__ __forceinline__ void stripe_symbols(LdsLock<M>& L, int lane, LaneMq& 
         if (work) {
             bool rl = false;
             if (pend == PEND_NONE) {
-                // next coefficient in stripe scan order: lowest column, then lowest row
-                const M any = c0 | c1 | c2 | c3;
-                x = (int)__builtin_ctzll((unsigned long long)any);
-                const M bx = (M)1 << x;
-                const bool b0 = (c0 & bx) != 0, b1 = (c1 & bx) != 0, b2 = (c2 & bx) != 0, b3 = (c3 & bx) != 0;
-                r = b0 ? 0 : (b1 ? 1 : (b2 ? 2 : 3));
-                c0 &= ~(b0 ? bx : (M)0);
-                c1 &= ~((!b0 && b1) ? bx : (M)0);
-                c2 &= ~((!b0 && !b1 && b2) ? bx : (M)0);
-                c3 &= ~((!b0 && !b1 && !b2) ? bx : (M)0);
-                if (TYPE == 2) rl = b0 && b1 && b2 && b3 && col3<M>(sany, x) == 0;
+                if (cur == 0) {
+                    // next candidate column
+                    x = (int)__builtin_ctzll((unsigned long long)cany);
+                    cany &= cany - 1;
+                    cur = nibble<M>(c0, c1, c2, c3, x);
+                    if (TYPE != 2) enib = nibble<M>(e0, e1, e2, e3, x);
+                    if (TYPE == 1) pnib = nibble<M>(p0, p1, p2, p3, x);
+                    if (TYPE == 2) rl = cur == 15u && col3<M>(sany, x) == 0;
+                }
+                r = __builtin_ctz(cur);
+                cur &= cur - 1;
             }
             const M bx = (M)1 << x;
             uint32_t lab, sxor = 0;
             if (TYPE == 1) {
-                // e: refined before, p: not refined but with a significant neighbour
-                const M er = r == 0 ? e0 : (r == 1 ? e1 : (r == 2 ? e2 : e3));
-                const M pr = r == 0 ? p0 : (r == 1 ? p1 : (r == 2 ? p2 : p3));
-                lab = (er & bx) ? 16u : ((pr & bx) ? 15u : 14u);
+                lab = ((enib >> r) & 1u) ? 16u : 14u + ((pnib >> r) & 1u);
             } else if (pend == PEND_NONE && !rl) {
                 const uint32_t w9 = col3<M>(lr(RS + r), x) | (col3<M>(lr(RS + r + 1), x) << 3) |
                                     (col3<M>(lr(RS + r + 2), x) << 6);
@@ -257,13 +267,15 @@ __device__ __forceinline__ void stripe_symbols(LdsLock<M>& L, int lane, LaneMq& 
                     if (TYPE == 2) sany |= bx;
                     if (TYPE == 0) {
                         // neighbours later in scan order may now be coded in this pass
+                        // (the row below joins this column's nibble too; c0..c3 record all)
+                        cur |= enib & (2u << r) & 15u;
                         const M nx = bx << 1;
                         const M a0 = ((r <= 1) ? nx : (M)0) & e0;
                         const M a1 = (((r <= 2) ? nx : (M)0) | ((r == 0) ? bx : (M)0)) & e1;
                         const M a2 = (((r >= 1) ? nx : (M)0) | ((r == 1) ? bx : (M)0)) & e2;
                         const M a3 = (((r >= 2) ? nx : (M)0) | ((r == 2) ? bx : (M)0)) & e3;
                         c0 |= a0, c1 |= a1, c2 |= a2, c3 |= a3;
-                        p0 |= a0, p1 |= a1, p2 |= a2, p3 |= a3;
+                        cany |= (a0 | a1 | a2 | a3) & nx;
                     }
                 }
             } else if (pend == PEND_SC) {
@@ -273,27 +285,22 @@ __device__ __forceinline__ void stripe_symbols(LdsLock<M>& L, int lane, LaneMq& 
                 pend = PEND_NONE;
             } else if (TYPE == 2) {
                 if (rl) {
-                    if (bit) {
+                    if (bit)
                         pend = PEND_UNI1;
-                    } else {
-                        c1 &= ~bx;
-                        c2 &= ~bx;
-                        c3 &= ~bx;
-                    }
+                    else
+                        cur = 0;  // the whole column stays insignificant
                 } else if (pend == PEND_UNI1) {
                     r = (int)(bit << 1);
                     pend = PEND_UNI2;
                 } else {
                     r |= (int)bit;
-                    c1 &= ~((r >= 1) ? bx : (M)0);
-                    c2 &= ~((r >= 2) ? bx : (M)0);
-                    c3 &= ~((r >= 3) ? bx : (M)0);
+                    cur &= ~((2u << r) - 1u);  // rows above r were coded as insignificant
                     lr(RS + r + 1) |= bx;
                     sany |= bx;
                     pend = PEND_SC;
                 }
             }
-            work = pend != PEND_NONE || ((c0 | c1 | c2 | c3) != 0);
+            work = pend != PEND_NONE || cur != 0 || cany != 0;
         }
     }
 }
@@ -336,8 +343,8 @@ __device__ void decode_lockstep(LdsLock<M>& L, const tc_cblk_desc& d, bool activ
         for (int y0 = 0; y0 < hmax; y0 += 4) {
             const bool on = pass_on && y0 < h;
             M c0 = 0, c1 = 0, c2 = 0, c3 = 0;  // candidate columns per stripe row
-            M e0 = 0, e1 = 0, e2 = 0, e3 = 0;  // sigprop: rows that may still join; magref: refined rows after
-            M p0 = 0, p1 = 0, p2 = 0, p3 = 0;  // sigprop: positions coded in the pass
+            M e0 = 0, e1 = 0, e2 = 0, e3 = 0;  // sigprop: rows that may still join; magref: label-16 candidates
+            M p0 = 0, p1 = 0, p2 = 0, p3 = 0;  // magref: label-15 candidates
             M sany = 0;                        // OR of the six significance rows
             if (on) {
                 M S[6], X[6], P[4], R[4];
@@ -380,7 +387,6 @@ __device__ void decode_lockstep(LdsLock<M>& L, const tc_cblk_desc& d, bool activ
                 }
                 c0 = cr[0], c1 = cr[1], c2 = cr[2], c3 = cr[3];
                 e0 = er[0], e1 = er[1], e2 = er[2], e3 = er[3];
-                p0 = c0, p1 = c1, p2 = c2, p3 = c3;
                 if (type == 1) {
                     // significance does not change in a refinement pass: every label is known now
                     M pr[4];
@@ -409,7 +415,7 @@ __device__ void decode_lockstep(LdsLock<M>& L, const tc_cblk_desc& d, bool activ
 
             if (on) {
                 if (type != 1) {
-                    const M pr[4] = {p0, p1, p2, p3};
+                    const M pr[4] = {c0, c1, c2, c3};  // sigprop: every position coded in the pass
 #pragma unroll
                     for (int i = 0; i < 4; i++) {
                         gat(GS, y0 + i) = lr(RS + i + 1);
