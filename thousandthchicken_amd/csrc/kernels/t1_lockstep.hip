@@ -109,25 +109,19 @@ __device__ __forceinline__ void lane_refill(LaneMq& m, uint32_t* ring_col, const
     m.rend = (q0 + 8) * 4;
 }
 
-// BYTEIN, T.800 Figure C.19.
-__device__ __forceinline__ void lane_bytein(LaneMq& m) {
+// BYTEIN, T.800 Figure C.19, with selects: after 0xFF a byte above 0x8F is a
+// marker (feed 1-bits, stay), any other byte enters 7 bits; otherwise 8 bits.
+// Applied only where `when` holds.
+__device__ __forceinline__ void lane_bytein(LaneMq& m, bool when = true) {
     const uint32_t b1 = lane_byte(m, m.bp + 1);
-    if (m.cur == 0xFF) {
-        if (b1 > 0x8F) {
-            m.C += 0xFF00;
-            m.CT = 8;
-        } else {
-            m.bp++;
-            m.cur = b1;
-            m.C += b1 << 9;
-            m.CT = 7;
-        }
-    } else {
-        m.bp++;
-        m.cur = b1;
-        m.C += b1 << 8;
-        m.CT = 8;
-    }
+    const bool ff = m.cur == 0xFFu;
+    const bool stay = ff && b1 > 0x8Fu;
+    const uint32_t add = stay ? 0xFF00u : (b1 << (ff ? 9 : 8));
+    m.C += when ? add : 0u;
+    m.CT = when ? ((ff && !stay) ? 7 : 8) : m.CT;
+    const bool adv = when && !stay;
+    m.bp += adv ? 1u : 0u;
+    m.cur = adv ? b1 : m.cur;
 }
 
 // INITDEC, T.800 Figure C.20 (the ring must hold the first bytes).
@@ -157,13 +151,16 @@ __device__ __forceinline__ uint32_t lane_decode(LaneMq& m, uint32_t& st, const u
     const bool take_mps = lps == (A < qe);  // LPS exchange: A < Qe gives MPS; MPS exchange: the reverse
     const uint32_t e2 = (uint32_t)(e >> 32);
     const uint32_t nm = mps ^ ((e2 >> 22) & 1u);
-    const uint32_t st2 = take_mps ? (((uint32_t)e & 0x3FFFFFu) | (mps << 22)) : ((e2 & 0x3FFFFFu) | (nm << 22));
+    const uint32_t smps = ((uint32_t)e & 0x3FFFFFu) | (mps << 22);
+    const uint32_t slps = (e2 & 0x3FFFFFu) | (nm << 22);
+    const uint32_t pick = take_mps ? ~0u : 0u;  // bitwise select keeps this off the branch path
+    const uint32_t st2 = (smps & pick) | (slps & ~pick);
     const uint32_t d = (renorm && !take_mps) ? (mps ^ 1u) : mps;
     st = renorm ? st2 : st;
     A = lps ? qe : A;
     int n = __clz(A) - 16;  // 0 unless renormalising
     while (n > 0) {
-        if (m.CT == 0) lane_bytein(m);
+        lane_bytein(m, m.CT == 0);
         const int s = n < m.CT ? n : m.CT;
         A <<= s;
         m.C <<= s;
