@@ -63,7 +63,8 @@ def test_t1_kernels_on_random_blocks(kernel, small):
     arena += bytes((-len(arena)) % 16) + b"\xff" * 64
     arena = np.frombuffer(bytes(arena), np.uint8)
     n_out = len(blocks) * pitch * 64
-    scratch = L.tc_t1_lockstep_scratch_bytes(1 if small else 0, len(blocks))
+    max_passes = max(b["npasses"] for b in blocks)
+    scratch = L.tc_t1_lockstep_scratch_bytes(1 if small else 0, len(blocks), max_passes)
     bufs = [tca.DeviceBuffer(dev, max(x, 256)) for x in (arena.nbytes, descs.nbytes, n_out * 4, scratch)]
     try:
         bufs[0].upload(stream, arena)
@@ -76,7 +77,8 @@ def test_t1_kernels_on_random_blocks(kernel, small):
             # the scratch state needs no initialisation: start it dirty
             tca.check(L.tc_buf_fill32(stream, bufs[3].handle, 0, 0xFFFFFFFF, scratch // 4), "fill")
             st = L.tc_launch_t1_decode_lockstep(stream, 1 if small else 0, bufs[1].ptr, None, len(blocks),
-                                                bufs[0].ptr, arena.nbytes, bufs[3].ptr, scratch, bufs[2].ptr)
+                                                max_passes, bufs[0].ptr, arena.nbytes, bufs[3].ptr, scratch,
+                                                bufs[2].ptr)
         elif kernel == "wave":
             st = L.tc_launch_t1_decode_wave(stream, 1 if small else 0, 512, bufs[1].ptr, None, len(blocks),
                                             bufs[0].ptr, arena.nbytes, bufs[2].ptr)

@@ -206,11 +206,14 @@ Status Pipeline::prepare(const std::vector<const Image*>& imgs, std::string& err
         g.small = k == 0;
         g.bpw = pick_blocks_per_wave((int)v.size(), g.small, cus, tuning_);
         g.n = (int)v.size();
-        for (uint32_t i : v) g.max_planes = std::max<int>(g.max_planes, descs[i].numbps);
+        for (uint32_t i : v) {
+            g.max_planes = std::max<int>(g.max_planes, descs[i].numbps);
+            g.max_passes = std::max<int>(g.max_passes, descs[i].npasses);
+        }
         g.order_off = put(blob, v);
         if (g.bpw == TC_BPW_LOCKSTEP) {
             g.scratch_off = scratch_total;
-            scratch_total += align_up(tc_t1_lockstep_scratch_bytes(g.small ? 1 : 0, g.n), 256);
+            scratch_total += align_up(tc_t1_lockstep_scratch_bytes(g.small ? 1 : 0, g.n, g.max_passes), 256);
         }
         bpw_[k] = g.bpw;
         groups_.push_back(g);
@@ -407,7 +410,7 @@ Status Pipeline::run(bool timed) {
         tc_stream st = (fork && gi == 1) ? side_ : stream_;
         if (g.bpw == TC_BPW_LOCKSTEP)
             s = tc_launch_t1_decode_lockstep(st, g.small ? 1 : 0, up + descs_off_, (const uint32_t*)(up + g.order_off),
-                                             g.n, up, cw_size_, (uint8_t*)tc_buf_device_ptr(scratch_) + g.scratch_off,
+                                             g.n, g.max_passes, up, cw_size_, (uint8_t*)tc_buf_device_ptr(scratch_) + g.scratch_off,
                                              tc_buf_size(scratch_) - g.scratch_off, base);
         else if (g.bpw == 1 && tuning_.t1_kernel == 0 && g.max_planes <= TC_T1_STRIPE_MAX_PLANES(g.small))
             s = tc_launch_t1_decode_stripe(st, g.small ? 1 : 0, std::min(g.n, device_cu_count(dev_) * 32),

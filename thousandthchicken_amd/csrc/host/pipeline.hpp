@@ -84,6 +84,7 @@ class Pipeline {
         bool small = false;
         int bpw = 1;
         int max_planes = 0;  // most bit-planes of a block in the class
+        int max_passes = 0;  // most coding passes of a block in the class
         uint64_t order_off = 0;
         uint64_t scratch_off = 0;  // lockstep packing: its state area in scratch_
         int n = 0;

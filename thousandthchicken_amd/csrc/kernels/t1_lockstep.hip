@@ -26,9 +26,16 @@
 // contiguous 512-byte access) and only the rows of the current stripe and
 // its two halo rows are staged into LDS ([row][lane], conflict-free) while
 // the stripe is decoded.  Context states (19 per lane) and the MQ / zero-coding
-// / sign-coding tables also live in LDS.  Magnitude and sign bits are posted to
-// the zero-filled coefficient plane with atomic ORs (no return), and after the
-// decode the wave converts its blocks in place (convert_block).
+// / sign-coding tables also live in LDS.
+//
+// Output.  Scattered read-modify-writes of single bits would saturate the
+// memory system once a few waves share a SIMD, so a coefficient's word in the
+// zero-filled plane is written once, by a plain store, when it turns
+// significant (its top magnitude bit and sign).  The 1-bits of each
+// refinement pass are collected per stripe as row masks and stored to a
+// per-pass log in the scratch area ([wave][pass][row][lane], coalesced); after
+// the decode the wave merges the logs into the words and dequantizes them in
+// place (convert_lockstep_block).
 //
 // Codewords are staged through a per-lane ring of 8 dwords in LDS.  The ring
 // is topped up in a wave-uniform step whenever some lane has fewer than four
@@ -198,18 +205,28 @@ __device__ __forceinline__ uint32_t nibble(M a0, M a1, M a2, M a3, int x) {
 // in this column through `cur`, three rows of the next column through c0..c3
 // and `cany`), so on return c0..c3 hold every position coded in the pass.
 // TYPE 1: e0..e3 mark candidates refined before (label 16), p0..p3 the others
-// with a significant neighbour (label 15; the rest take 14).  TYPE 2: sany is
-// the OR of the six staged significance rows (run-length test).  out points
-// at row y0.
+// with a significant neighbour (label 15; the rest take 14); the decoded
+// 1-bits are returned as row masks in o0..o3.  TYPE 2: sany is the OR of the
+// six staged significance rows (run-length test).  out points at row y0.
 template <typename M, int TYPE>
 __device__ __forceinline__ void stripe_symbols(LdsLock<M>& L, int lane, LaneMq& m, uint32_t* ring_col,
                                                const uint32_t* __restrict__ words, uint32_t nwords, bool work, M& c0,
                                                M& c1, M& c2, M& c3, M e0, M e1, M e2, M e3, M p0, M p1, M p2, M p3,
-                                               M& sany, uint32_t cls, uint32_t* __restrict__ out, uint32_t pitch,
-                                               uint32_t pbit) {
+                                               M& o0, M& o1, M& o2, M& o3, M& sany, uint32_t cls,
+                                               uint32_t* __restrict__ out, uint32_t pitch, uint32_t pbit) {
     auto lr = [&](int i) -> M& { return L.rows[i][lane]; };
     M cany = c0 | c1 | c2 | c3;
     uint32_t pend = PEND_NONE, cur = 0, enib = 0, pnib = 0;
+    uint32_t onib = 0;  // TYPE 1: refinement 1-bits of the current column
+    // moves the current column's refinement 1-bits into the row masks
+    auto flush = [&](int col) {
+        const M bc = (M)1 << col;
+        o0 |= (onib & 1u) ? bc : (M)0;
+        o1 |= (onib & 2u) ? bc : (M)0;
+        o2 |= (onib & 4u) ? bc : (M)0;
+        o3 |= (onib & 8u) ? bc : (M)0;
+        onib = 0;
+    };
     int x = 0, r = 0;
     while (__ballot(work)) {
         // top up the rings of lanes running short before anyone reads past them
@@ -221,6 +238,7 @@ __device__ __forceinline__ void stripe_symbols(LdsLock<M>& L, int lane, LaneMq& 
             if (pend == PEND_NONE) {
                 if (cur == 0) {
                     // next candidate column
+                    if (TYPE == 1) flush(x);
                     x = (int)__builtin_ctzll((unsigned long long)cany);
                     cany &= cany - 1;
                     cur = nibble<M>(c0, c1, c2, c3, x);
@@ -255,7 +273,7 @@ __device__ __forceinline__ void stripe_symbols(LdsLock<M>& L, int lane, LaneMq& 
             L.ctx[lab][lane] = st;
             uint32_t* const at = out + (uint32_t)r * pitch + x;
             if (TYPE == 1) {
-                if (bit) __hip_atomic_fetch_or(at, pbit, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+                onib |= bit << r;
             } else if (pend == PEND_NONE && !rl) {
                 // zero coding
                 if (bit) {
@@ -278,7 +296,7 @@ __device__ __forceinline__ void stripe_symbols(LdsLock<M>& L, int lane, LaneMq& 
             } else if (pend == PEND_SC) {
                 const uint32_t sg = bit ^ sxor;
                 if (sg) lr(RX + r + 1) |= bx;
-                __hip_atomic_fetch_or(at, pbit | (sg << 31), __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+                *at = pbit | (sg << 31);  // the word's first bits: the plane was zero here
                 pend = PEND_NONE;
             } else if (TYPE == 2) {
                 if (rl) {
@@ -300,12 +318,15 @@ __device__ __forceinline__ void stripe_symbols(LdsLock<M>& L, int lane, LaneMq& 
             work = pend != PEND_NONE || cur != 0 || cany != 0;
         }
     }
+    if (TYPE == 1) flush(x);
 }
 
+// g: the wave's state rows [mask][row][lane]; lg: its refinement logs
+// [pass][row][lane], one per refinement pass (pass k = 3j + 2 writes log j).
 template <typename M, int MAXH>
 __device__ void decode_lockstep(LdsLock<M>& L, const tc_cblk_desc& d, bool active, int lane,
                                 const uint32_t* __restrict__ words, uint32_t nwords, uint32_t* __restrict__ coef,
-                                M* __restrict__ g) {
+                                M* __restrict__ g, M* __restrict__ lg) {
     constexpr int NB = (int)sizeof(M) * 8;
     const int w = active ? d.w : 0, h = active ? d.h : 0;
     const int numbps = active ? d.numbps : 0, npasses = active ? d.npasses : 0;
@@ -343,6 +364,7 @@ __device__ void decode_lockstep(LdsLock<M>& L, const tc_cblk_desc& d, bool activ
             M e0 = 0, e1 = 0, e2 = 0, e3 = 0;  // sigprop: rows that may still join; magref: label-16 candidates
             M p0 = 0, p1 = 0, p2 = 0, p3 = 0;  // magref: label-15 candidates
             M sany = 0;                        // OR of the six significance rows
+            M o0 = 0, o1 = 0, o2 = 0, o3 = 0;  // magref: 1-bits decoded in the stripe
             if (on) {
                 M S[6], X[6], P[4], R[4];
                 S[0] = y0 > 0 ? carry_s : (M)0;
@@ -402,13 +424,16 @@ __device__ void decode_lockstep(LdsLock<M>& L, const tc_cblk_desc& d, bool activ
             const bool work = on && ((c0 | c1 | c2 | c3) != 0);
             if (type == 0)
                 stripe_symbols<M, 0>(L, lane, m, ring_col, words, nwords, work, c0, c1, c2, c3, e0, e1, e2, e3, p0,
-                                     p1, p2, p3, sany, cls, out + (uint32_t)y0 * pitch, pitch, pbit);
+                                     p1, p2, p3, o0, o1, o2, o3, sany, cls, out + (uint32_t)y0 * pitch, pitch,
+                                     pbit);
             else if (type == 1)
                 stripe_symbols<M, 1>(L, lane, m, ring_col, words, nwords, work, c0, c1, c2, c3, e0, e1, e2, e3, p0,
-                                     p1, p2, p3, sany, cls, out + (uint32_t)y0 * pitch, pitch, pbit);
+                                     p1, p2, p3, o0, o1, o2, o3, sany, cls, out + (uint32_t)y0 * pitch, pitch,
+                                     pbit);
             else
                 stripe_symbols<M, 2>(L, lane, m, ring_col, words, nwords, work, c0, c1, c2, c3, e0, e1, e2, e3, p0,
-                                     p1, p2, p3, sany, cls, out + (uint32_t)y0 * pitch, pitch, pbit);
+                                     p1, p2, p3, o0, o1, o2, o3, sany, cls, out + (uint32_t)y0 * pitch, pitch,
+                                     pbit);
 
             if (on) {
                 if (type != 1) {
@@ -423,8 +448,85 @@ __device__ void decode_lockstep(LdsLock<M>& L, const tc_cblk_desc& d, bool activ
                     }
                     carry_s = lr(RS + 4);
                     carry_x = lr(RX + 4);
+                } else {
+                    M* const lj = lg + (size_t)((k - 2) / 3) * MAXH * kLaneCount;
+                    lj[(y0 + 0) * kLaneCount + lane] = o0;
+                    lj[(y0 + 1) * kLaneCount + lane] = o1;
+                    lj[(y0 + 2) * kLaneCount + lane] = o2;
+                    lj[(y0 + 3) * kLaneCount + lane] = o3;
                 }
             }
+        }
+    }
+}
+
+// Finishes block j of the wave: merges its refinement logs (log entry
+// [j'][row][j] holds plane numbps-2-j') into the words its significance
+// stores left, then rebuilds and dequantizes each coefficient as
+// convert_block does (midpoint of the last coded plane).  The 64 lanes cover
+// 64/span rows per step, span = w rounded up to a power of two; each lane
+// takes four rows at a time and loads eight logs of each together, so about
+// 32 loads are in flight per lane.
+template <typename M, int MAXH>
+__device__ void convert_lockstep_block(int lane, int j, const tc_cblk_desc& d, uint32_t* __restrict__ coef,
+                                       const M* __restrict__ lg) {
+    constexpr int RU = 4, TU = 8;
+    const int numbps = d.numbps, np = d.npasses;
+    if (numbps <= 0 || np <= 0) return;
+    const int kl = np - 1;
+    int plast = numbps - 1 - (kl + 2) / 3;
+    if (plast < 0) plast = 0;
+    const int tlast = kl == 0 ? 2 : (kl - 1) % 3;
+    const int nlog = np / 3;  // refinement passes the block went through
+    uint32_t* const out = coef + d.out_off;
+    const int w = d.w, h = d.h;
+    const uint32_t sigonly = 1u << plast;
+    int span = 1;
+    while (span < w) span <<= 1;
+    const int rows_per = 64 / span;
+    const int lx = lane % span, ly = lane / span;
+    if (lx >= w) return;
+    for (int y0 = ly; y0 < h; y0 += RU * rows_per) {
+        uint32_t v[RU], mag[RU];
+        int yy[RU];
+#pragma unroll
+        for (int q = 0; q < RU; q++) {
+            yy[q] = y0 + q * rows_per;
+            const int yc = yy[q] < h ? yy[q] : y0;  // rows past the block re-read row y0 (in bounds, unused)
+            v[q] = out[(uint32_t)yc * d.out_pitch + lx];
+            mag[q] = v[q] & 0x7FFFFFFFu;
+        }
+        for (int t0 = 0; t0 < nlog; t0 += TU) {
+            M r[RU][TU];
+#pragma unroll
+            for (int q = 0; q < RU; q++) {
+                const int yc = yy[q] < h ? yy[q] : y0;
+#pragma unroll
+                for (int u = 0; u < TU; u++) {
+                    const int t = t0 + u < nlog ? t0 + u : nlog - 1;  // clamped: duplicate loads are harmless
+                    r[q][u] = lg[((size_t)t * MAXH + yc) * kLaneCount + j];
+                }
+            }
+#pragma unroll
+            for (int q = 0; q < RU; q++)
+#pragma unroll
+                for (int u = 0; u < TU; u++)
+                    if (t0 + u < nlog) mag[q] |= (uint32_t)((r[q][u] >> lx) & 1u) << (numbps - 2 - t0 - u);
+        }
+#pragma unroll
+        for (int q = 0; q < RU; q++) {
+            if (yy[q] >= h || v[q] == 0) continue;
+            const int qp = (tlast == 0 && mag[q] != sigonly) ? plast + 1 : plast;
+            const bool neg = (v[q] >> 31) != 0;
+            uint32_t res;
+            if (d.mode == 0) {
+                const int32_t iv = (int32_t)(mag[q] + (qp > 0 ? (1u << (qp - 1)) : 0u));
+                res = (uint32_t)(neg ? -iv : iv);
+            } else {
+                const float f = ((float)mag[q] + ldexpf(0.5f, qp)) * d.delta;
+                res = __float_as_uint(neg ? -f : f);
+            }
+            out[(uint32_t)yy[q] * d.out_pitch + lx] = res;
         }
     }
 }
@@ -433,7 +535,8 @@ template <typename M, int MAXH>
 __global__ __launch_bounds__(64) void t1_lockstep_kernel(const tc_cblk_desc* __restrict__ descs,
                                                          const uint32_t* __restrict__ order, int n,
                                                          const uint8_t* __restrict__ arena, uint32_t arena_words,
-                                                         uint32_t* __restrict__ coef, M* __restrict__ scratch) {
+                                                         uint32_t* __restrict__ coef, M* __restrict__ scratch,
+                                                         int nlog_max) {
     __shared__ LdsLock<M> L;
     const int lane = threadIdx.x;
     for (int i = lane; i < 47; i += 64) L.trans[i] = trans_entry(i);
@@ -464,41 +567,48 @@ __global__ __launch_bounds__(64) void t1_lockstep_kernel(const tc_cblk_desc* __r
         atomicMax(&L.hmax, (int)d.h);
     }
     __syncthreads();
-    decode_lockstep<M, MAXH>(L, d, active, lane, reinterpret_cast<const uint32_t*>(arena), arena_words, coef,
-                             scratch + (size_t)blockIdx.x * 4 * MAXH * kLaneCount);
+    M* const g = scratch + (size_t)blockIdx.x * (4 + nlog_max) * MAXH * kLaneCount;
+    M* const lg = g + 4 * MAXH * kLaneCount;
+    decode_lockstep<M, MAXH>(L, d, active, lane, reinterpret_cast<const uint32_t*>(arena), arena_words, coef, g, lg);
 
-    // make every lane's posted ORs visible, and drop stale L1 lines, before the
+    // make every lane's stores visible, and drop stale L1 lines, before the
     // cooperative read-back
     __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "agent");
     __syncthreads();
     for (int j = 0; j < kLaneCount && base + j < n; j++) {
         const tc_cblk_desc dj = descs[order ? order[base + j] : (uint32_t)(base + j)];
-        convert_block(lane, dj, coef);
+        convert_lockstep_block<M, MAXH>(lane, j, dj, coef, lg);
     }
 }
 
 template <typename M, int MAXH>
 static hipError_t launch_lockstep_variant(hipStream_t s, const tc_cblk_desc* descs, const uint32_t* order, int n,
-                                          const uint8_t* arena, uint32_t arena_bytes, uint32_t* coef, void* scratch) {
+                                          const uint8_t* arena, uint32_t arena_bytes, uint32_t* coef, void* scratch,
+                                          int max_passes) {
     if (n <= 0) return hipSuccess;
     const int grid = (n + kLaneCount - 1) / kLaneCount;
     hipLaunchKernelGGL((t1_lockstep_kernel<M, MAXH>), dim3(grid), dim3(kLaneCount), 0, s, descs, order, n, arena,
-                       arena_bytes / 4, coef, static_cast<M*>(scratch));
+                       arena_bytes / 4, coef, static_cast<M*>(scratch), max_passes / 3);
     return hipGetLastError();
 }
 
 }  // namespace
 
-// Scratch bytes the lockstep kernel needs for n blocks of one size class.
-size_t t1_lockstep_scratch_bytes(bool cls32, int n) {
+// Scratch bytes the lockstep kernel needs for n blocks of one size class with
+// at most max_passes coding passes each: per wave, four state masks and one
+// refinement log per refinement pass, each MAXH rows of 64 lanes.
+size_t t1_lockstep_scratch_bytes(bool cls32, int n, int max_passes) {
     const size_t waves = (size_t)((n + kLaneCount - 1) / kLaneCount);
-    return cls32 ? waves * 4 * 32 * kLaneCount * sizeof(uint32_t) : waves * 4 * 64 * kLaneCount * sizeof(uint64_t);
+    const size_t rows = (size_t)(4 + max_passes / 3);
+    return cls32 ? waves * rows * 32 * kLaneCount * sizeof(uint32_t) : waves * rows * 64 * kLaneCount * sizeof(uint64_t);
 }
 
 hipError_t launch_t1_lockstep(hipStream_t s, bool cls32, const tc_cblk_desc* descs, const uint32_t* order, int n,
-                              const uint8_t* arena, uint32_t arena_bytes, uint32_t* coef, void* scratch) {
-    if (cls32) return launch_lockstep_variant<uint32_t, 32>(s, descs, order, n, arena, arena_bytes, coef, scratch);
-    return launch_lockstep_variant<uint64_t, 64>(s, descs, order, n, arena, arena_bytes, coef, scratch);
+                              const uint8_t* arena, uint32_t arena_bytes, uint32_t* coef, void* scratch,
+                              int max_passes) {
+    if (cls32)
+        return launch_lockstep_variant<uint32_t, 32>(s, descs, order, n, arena, arena_bytes, coef, scratch, max_passes);
+    return launch_lockstep_variant<uint64_t, 64>(s, descs, order, n, arena, arena_bytes, coef, scratch, max_passes);
 }
 
 }  // namespace tc
