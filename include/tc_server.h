@@ -46,7 +46,7 @@ typedef struct {
     double busy_s;            /* time with at least one batch on the device */
     uint64_t peak_bytes;      /* largest admitted footprint of all slots */
     int32_t queued, in_flight;
-    double plan_s;            /* summed over batches: planning and staging (host) */
+    double plan_s;            /* summed over batches: parsing, planning and staging (host) */
     double device_s;          /* launch to results in host memory */
     double copy_s;            /* publishing results (they stay in pinned memory until read) */
 } tc_server_stats;

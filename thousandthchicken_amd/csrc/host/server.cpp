@@ -76,6 +76,9 @@ uint64_t footprint_of(const Image& im) {
 // the slot's stream, waits for it, and copies the results out.  Host work of
 // one slot (planning, staging, result copies) overlaps the device work of the
 // others, and batches of different slots run concurrently on the device.
+// Threads one slot uses to parse a large batch's codestreams.
+constexpr int kParseThreads = 4;
+
 struct tc_server_s {
     tc_device dev;
     tc_server_config cfg;
@@ -114,15 +117,28 @@ struct tc_server_s {
         std::vector<const Image*> imgs;
         size_t total = 0;
         s.offs.clear();
-        // full parse; a request whose packets do not parse fails alone
-        std::string perr;
-        std::vector<Request*> ok;
-        for (Request* r : s.batch) {
-            if (parse_codestream(r->cs, r->cs_len, r->img, perr) == Status::Ok)
-                ok.push_back(r);
-            else
-                s.rejected.push_back(r);
+        const double tp = now();
+        // full parse, on a few threads when the batch is large; a request whose
+        // packets do not parse fails alone
+        std::vector<char> good(s.batch.size(), 0);
+        size_t bytes = 0;
+        for (Request* r : s.batch) bytes += r->cs_len;
+        const int nt = (int)std::min<size_t>({(size_t)kParseThreads, s.batch.size(), 1 + bytes / (4u << 20)});
+        auto parse_part = [&](int k) {
+            std::string perr;
+            for (size_t i = (size_t)k; i < s.batch.size(); i += (size_t)nt)
+                good[i] = parse_codestream(s.batch[i]->cs, s.batch[i]->cs_len, s.batch[i]->img, perr) == Status::Ok;
+        };
+        if (nt <= 1) {
+            parse_part(0);
+        } else {
+            std::vector<std::thread> th;
+            for (int k = 1; k < nt; k++) th.emplace_back(parse_part, k);
+            parse_part(0);
+            for (auto& t : th) t.join();
         }
+        std::vector<Request*> ok;
+        for (size_t i = 0; i < s.batch.size(); i++) (good[i] ? ok : s.rejected).push_back(s.batch[i]);
         s.batch.swap(ok);
         if (s.batch.empty()) return TC_SUCCESS;
         for (Request* r : s.batch) {
@@ -131,7 +147,6 @@ struct tc_server_s {
             total += (r->out_bytes + 255) & ~(size_t)255;
         }
         std::string err;
-        const double tp = now();
         if (s.pipe->prepare(imgs, err) != Status::Ok) {
             set_error(err);
             return TC_ERR_DEVICE;
@@ -172,14 +187,18 @@ struct tc_server_s {
             idle_workers--;
             if (queue.empty()) break;  // stopping and drained
             // admission: FIFO while the batch fits the slot's memory share, and
-            // at most this worker's share of the queue when other slots are idle
-            const size_t share = (queue.size() + idle_workers) / (size_t)(idle_workers + 1);
-            const int cap = std::min<int>(cfg.max_batch, (int)std::max<size_t>(share, 1));
-            uint64_t fp = 0;
-            while (!queue.empty() && (int)s.batch.size() < cap) {
+            // at most this worker's share of the queued codestream bytes when
+            // other slots are idle (so that large requests spread over slots and
+            // one slot's host work overlaps another's device work)
+            uint64_t queued = 0;
+            for (const Request* r : queue) queued += r->cs_len;
+            const uint64_t share = queued / (uint64_t)(idle_workers + 1);
+            uint64_t fp = 0, taken = 0;
+            while (!queue.empty() && (int)s.batch.size() < cfg.max_batch) {
                 Request* r = queue.front();
-                if (!s.batch.empty() && fp + r->footprint > slot_budget) break;
+                if (!s.batch.empty() && (fp + r->footprint > slot_budget || taken >= share)) break;
                 fp += r->footprint;
+                taken += r->cs_len;
                 s.batch.push_back(r);
                 queue.pop_front();
             }
