@@ -29,6 +29,8 @@ typedef struct {
     uint64_t memory_budget;   /* device bytes for all slots together, default 32 GiB */
     int32_t bpw_small, bpw_large; /* tier-1 packing (0: automatic) */
     int32_t dwt_shape;            /* inverse-DWT tile shape (tc_idwt_tile_shape) */
+    uint64_t max_batch_bytes;     /* most codestream bytes per batch (a larger request
+                                     runs alone), default 256 MiB */
 } tc_server_config;
 
 typedef struct {

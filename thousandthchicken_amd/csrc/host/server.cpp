@@ -186,17 +186,21 @@ struct tc_server_s {
             cv_work.wait(lk, [&] { return stop || !queue.empty(); });
             idle_workers--;
             if (queue.empty()) break;  // stopping and drained
-            // admission: FIFO while the batch fits the slot's memory share, and
-            // at most this worker's share of the queued codestream bytes when
-            // other slots are idle (so that large requests spread over slots and
-            // one slot's host work overlaps another's device work)
+            // admission: FIFO while the batch fits the slot's memory share and
+            // the per-batch byte cap (which also bounds the staging buffers, so
+            // that they stop growing once warm), and at most this worker's share
+            // of the queued codestream bytes when other slots are idle (so that
+            // large requests spread over slots and one slot's host work overlaps
+            // another's device work)
             uint64_t queued = 0;
             for (const Request* r : queue) queued += r->cs_len;
-            const uint64_t share = queued / (uint64_t)(idle_workers + 1);
+            const uint64_t share = std::min<uint64_t>(queued / (uint64_t)(idle_workers + 1), cfg.max_batch_bytes);
             uint64_t fp = 0, taken = 0;
             while (!queue.empty() && (int)s.batch.size() < cfg.max_batch) {
                 Request* r = queue.front();
-                if (!s.batch.empty() && (fp + r->footprint > slot_budget || taken >= share)) break;
+                if (!s.batch.empty() &&
+                    (fp + r->footprint > slot_budget || taken >= share || taken + r->cs_len > cfg.max_batch_bytes))
+                    break;
                 fp += r->footprint;
                 taken += r->cs_len;
                 s.batch.push_back(r);
@@ -272,6 +276,7 @@ tc_status tc_server_create(tc_device dev, const tc_server_config* cfg, tc_server
     s->cfg.slots = cfg && cfg->slots > 0 ? cfg->slots : 3;
     s->cfg.max_batch = cfg && cfg->max_batch > 0 ? cfg->max_batch : 256;
     s->cfg.memory_budget = cfg && cfg->memory_budget ? cfg->memory_budget : (32ull << 30);
+    s->cfg.max_batch_bytes = cfg && cfg->max_batch_bytes ? cfg->max_batch_bytes : (256ull << 20);
     s->cfg.bpw_small = cfg ? cfg->bpw_small : 0;
     s->cfg.bpw_large = cfg ? cfg->bpw_large : 0;
     s->cfg.dwt_shape = cfg ? cfg->dwt_shape : 0;
