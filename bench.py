@@ -97,7 +97,7 @@ def timed_runs(dec, steps, graph=False):
 
 def t1_kernel_name(stats, args):
     bpw = stats.bpw_large or stats.bpw_small
-    if bpw == -64:
+    if bpw < 0:
         return "t1_lockstep_kernel"
     if bpw > 1:
         return f"t1_decode_kernel<{bpw}>"
@@ -277,8 +277,8 @@ def main():
     ap.add_argument("--profile", action="store_true", help="workload steps and a calibration copy only")
     ap.add_argument("--loads", default="50,100,1000,5000", help="offered loads (req/s) of the serving run")
     ap.add_argument("--requests", type=int, default=300, help="requests per serving run")
-    ap.add_argument("--bpw-large", type=int, default=0, help="blocks per wave, blocks up to 64x64 (0 auto, -64 lockstep)")
-    ap.add_argument("--bpw-small", type=int, default=0, help="blocks per wave, blocks up to 32x32 (0 auto, -64 lockstep)")
+    ap.add_argument("--bpw-large", type=int, default=0, help="blocks per wave, blocks up to 64x64 (0 auto, -64/-32 lockstep)")
+    ap.add_argument("--bpw-small", type=int, default=0, help="blocks per wave, blocks up to 32x32 (0 auto, -64/-32 lockstep)")
     ap.add_argument("--t1-kernel", type=int, default=0, help="one-wave-per-block kernel: 0 stripe-column, 1 row-mask")
     ap.add_argument("--no-autotune", action="store_true", help="keep the default packing and DWT tile shape")
     args = ap.parse_args()

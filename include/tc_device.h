@@ -140,20 +140,22 @@ tc_status tc_launch_t1_decode_wave(tc_stream s, int small_blocks, int grid, cons
 tc_status tc_launch_t1_decode_stripe(tc_stream s, int small_blocks, int grid, const void *descs, const uint32_t *order,
                                      int n, const uint8_t *arena, size_t arena_bytes, void *coef);
 
-/* Same work as tc_launch_t1_decode with 64 code-blocks per wavefront decoded
- * in lockstep, one MQ symbol per lane per step (throughput form for large
- * batches).  Blocks that share a wavefront should have equal pass counts (the
- * packing order sorts them so); any mix decodes correctly.  max_passes: the
- * largest npasses of the n blocks (at most 255).  scratch: device memory of
- * at least tc_t1_lockstep_scratch_bytes(small_blocks, n, max_passes) bytes,
+/* Same work as tc_launch_t1_decode with blocks_per_wave (64, 32 or 16)
+ * code-blocks per wavefront decoded in lockstep, one MQ symbol per lane per
+ * step (throughput form for large batches; fewer than 64 per wave makes more
+ * waves for launches with few blocks per CU).  Blocks that share a wavefront
+ * should have equal pass counts (the packing order sorts them so); any mix
+ * decodes correctly.  max_passes: the largest npasses of the n blocks (at
+ * most 255).  scratch: device memory of at least
+ * tc_t1_lockstep_scratch_bytes(small_blocks, blocks_per_wave, n, max_passes) bytes,
  * 8-byte aligned, holding per-block significance / sign / pass state and the
  * refinement 1-bits of each pass (no initialisation needed).  Codewords may
  * start at any byte offset; arena is 16-byte aligned and arena_bytes (a
  * multiple of 16, at least 64) readable bytes cover every codeword. */
-size_t tc_t1_lockstep_scratch_bytes(int small_blocks, int n, int max_passes);
-tc_status tc_launch_t1_decode_lockstep(tc_stream s, int small_blocks, const void *descs, const uint32_t *order, int n,
-                                       int max_passes, const uint8_t *arena, size_t arena_bytes, void *scratch,
-                                       size_t scratch_bytes, void *coef);
+size_t tc_t1_lockstep_scratch_bytes(int small_blocks, int blocks_per_wave, int n, int max_passes);
+tc_status tc_launch_t1_decode_lockstep(tc_stream s, int small_blocks, int blocks_per_wave, const void *descs,
+                                       const uint32_t *order, int n, int max_passes, const uint8_t *arena,
+                                       size_t arena_bytes, void *scratch, size_t scratch_bytes, void *coef);
 
 /* Output tile shapes of the inverse DWT launches: shape 0..*count-1 covers
  * *w x *h output samples per work-group (the workgroup-window choice of

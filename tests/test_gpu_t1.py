@@ -41,7 +41,7 @@ def _expected(b):
     return np.where(sign.astype(bool), -v, v).astype(np.int32)
 
 
-@pytest.mark.parametrize("kernel", ["stripe", "wave", "lockstep", "lockstep_unaligned", 8, 16, 32, 64])
+@pytest.mark.parametrize("kernel", ["stripe", "wave", "lockstep", "lockstep_unaligned", "lockstep16", 8, 16, 32, 64])
 @pytest.mark.parametrize("small", [True, False])
 def test_t1_kernels_on_random_blocks(kernel, small):
     lim = 32 if small else 64
@@ -64,7 +64,8 @@ def test_t1_kernels_on_random_blocks(kernel, small):
     arena = np.frombuffer(bytes(arena), np.uint8)
     n_out = len(blocks) * pitch * 64
     max_passes = max(b["npasses"] for b in blocks)
-    scratch = L.tc_t1_lockstep_scratch_bytes(1 if small else 0, len(blocks), max_passes)
+    lbpw = 16 if kernel == "lockstep16" else 64
+    scratch = L.tc_t1_lockstep_scratch_bytes(1 if small else 0, lbpw, len(blocks), max_passes)
     bufs = [tca.DeviceBuffer(dev, max(x, 256)) for x in (arena.nbytes, descs.nbytes, n_out * 4, scratch)]
     try:
         bufs[0].upload(stream, arena)
@@ -73,10 +74,10 @@ def test_t1_kernels_on_random_blocks(kernel, small):
         if kernel == "stripe":
             st = L.tc_launch_t1_decode_stripe(stream, 1 if small else 0, 512, bufs[1].ptr, None, len(blocks),
                                               bufs[0].ptr, arena.nbytes, bufs[2].ptr)
-        elif kernel in ("lockstep", "lockstep_unaligned"):
+        elif kernel in ("lockstep", "lockstep_unaligned", "lockstep16"):
             # the scratch state needs no initialisation: start it dirty
             tca.check(L.tc_buf_fill32(stream, bufs[3].handle, 0, 0xFFFFFFFF, scratch // 4), "fill")
-            st = L.tc_launch_t1_decode_lockstep(stream, 1 if small else 0, bufs[1].ptr, None, len(blocks),
+            st = L.tc_launch_t1_decode_lockstep(stream, 1 if small else 0, lbpw, bufs[1].ptr, None, len(blocks),
                                                 max_passes, bufs[0].ptr, arena.nbytes, bufs[3].ptr, scratch,
                                                 bufs[2].ptr)
         elif kernel == "wave":

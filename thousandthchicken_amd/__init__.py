@@ -13,8 +13,10 @@ import numpy as np
 
 _HERE = os.path.dirname(os.path.abspath(__file__))
 LIB_PATH = os.path.join(_HERE, "libtc.so")
-# Packing value: 64 code-blocks per wavefront decoded in lockstep (TC_BPW_LOCKSTEP).
+# Packing values: 64 / 32 code-blocks per wavefront decoded in lockstep
+# (TC_BPW_LOCKSTEP, TC_BPW_LOCKSTEP32).
 BPW_LOCKSTEP = -64
+BPW_LOCKSTEP32 = -32
 
 _lib = None
 
@@ -123,8 +125,8 @@ def lib():
         "tc_launch_t1_decode": (i32, [vp, i32, i32, vp, vp, i32, vp, vp]),
         "tc_launch_t1_decode_wave": (i32, [vp, i32, i32, vp, vp, i32, vp, sz, vp]),
         "tc_launch_t1_decode_stripe": (i32, [vp, i32, i32, vp, vp, i32, vp, sz, vp]),
-        "tc_t1_lockstep_scratch_bytes": (sz, [i32, i32, i32]),
-        "tc_launch_t1_decode_lockstep": (i32, [vp, i32, vp, vp, i32, i32, vp, sz, vp, sz, vp]),
+        "tc_t1_lockstep_scratch_bytes": (sz, [i32, i32, i32, i32]),
+        "tc_launch_t1_decode_lockstep": (i32, [vp, i32, i32, vp, vp, i32, i32, vp, sz, vp, sz, vp]),
         "tc_idwt_tile_shape": (i32, [i32, P(i32), P(i32), P(i32)]),
     }
     for name, (res, args) in sigs.items():

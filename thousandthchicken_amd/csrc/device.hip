@@ -19,9 +19,9 @@ hipError_t launch_t1_wave(hipStream_t s, bool narrow, int grid, const tc_cblk_de
                           int n, const uint8_t* arena, uint32_t arena_bytes, uint32_t* coef);
 hipError_t launch_t1_stripe(hipStream_t s, bool small, int grid, const tc_cblk_desc* descs, const uint32_t* order,
                             int n, const uint8_t* arena, uint32_t arena_bytes, uint32_t* coef);
-size_t t1_lockstep_scratch_bytes(bool cls32, int n, int max_passes);
-hipError_t launch_t1_lockstep(hipStream_t s, bool cls32, const tc_cblk_desc* descs, const uint32_t* order, int n,
-                              const uint8_t* arena, uint32_t arena_bytes, uint32_t* coef, void* scratch,
+size_t t1_lockstep_scratch_bytes(bool cls32, int n, int max_passes, int bpw);
+hipError_t launch_t1_lockstep(hipStream_t s, bool cls32, int bpw, const tc_cblk_desc* descs, const uint32_t* order,
+                              int n, const uint8_t* arena, uint32_t arena_bytes, uint32_t* coef, void* scratch,
                               int max_passes);
 int idwt_shape_count();
 int idwt_tile_w(int shape);
@@ -392,21 +392,25 @@ tc_status tc_launch_t1_decode_stripe(tc_stream s, int small_blocks, int grid, co
                     "t1_decode_stripe launch");
 }
 
-size_t tc_t1_lockstep_scratch_bytes(int small_blocks, int n, int max_passes) {
-    return n > 0 && max_passes >= 0 ? t1_lockstep_scratch_bytes(small_blocks != 0, n, max_passes) : 0;
+static bool lockstep_bpw_ok(int b) { return b == 64 || b == 32 || b == 16; }
+
+size_t tc_t1_lockstep_scratch_bytes(int small_blocks, int blocks_per_wave, int n, int max_passes) {
+    return n > 0 && max_passes >= 0 && lockstep_bpw_ok(blocks_per_wave)
+               ? t1_lockstep_scratch_bytes(small_blocks != 0, n, max_passes, blocks_per_wave)
+               : 0;
 }
 
-tc_status tc_launch_t1_decode_lockstep(tc_stream s, int small_blocks, const void* descs, const uint32_t* order, int n,
-                                       int max_passes, const uint8_t* arena, size_t arena_bytes, void* scratch,
-                                       size_t scratch_bytes, void* coef) {
+tc_status tc_launch_t1_decode_lockstep(tc_stream s, int small_blocks, int blocks_per_wave, const void* descs,
+                                       const uint32_t* order, int n, int max_passes, const uint8_t* arena,
+                                       size_t arena_bytes, void* scratch, size_t scratch_bytes, void* coef) {
     if (!s || n < 0 || (n > 0 && (!descs || !arena || !coef || !scratch)) || ((uintptr_t)arena & 15) ||
         ((uintptr_t)scratch & 7) || (arena_bytes & 15) || arena_bytes < 64 || arena_bytes > 0xFFFFFFF0ull ||
-        max_passes < 0 || max_passes > 255)
+        max_passes < 0 || max_passes > 255 || !lockstep_bpw_ok(blocks_per_wave))
         return fail(TC_ERR_INVALID_ARG, "tc_launch_t1_decode_lockstep");
-    if (n > 0 && scratch_bytes < t1_lockstep_scratch_bytes(small_blocks != 0, n, max_passes))
+    if (n > 0 && scratch_bytes < t1_lockstep_scratch_bytes(small_blocks != 0, n, max_passes, blocks_per_wave))
         return fail(TC_ERR_INVALID_ARG, "tc_launch_t1_decode_lockstep: scratch smaller than tc_t1_lockstep_scratch_bytes");
-    return hip_fail(launch_t1_lockstep(s->s, small_blocks != 0, (const tc_cblk_desc*)descs, order, n, arena,
-                                       (uint32_t)arena_bytes, (uint32_t*)coef, scratch, max_passes),
+    return hip_fail(launch_t1_lockstep(s->s, small_blocks != 0, blocks_per_wave, (const tc_cblk_desc*)descs, order, n,
+                                       arena, (uint32_t)arena_bytes, (uint32_t*)coef, scratch, max_passes),
                     "t1_decode_lockstep launch");
 }
 

@@ -56,16 +56,20 @@ Status from_tc(tc_status s) { return s == TC_SUCCESS ? Status::Ok : (s == TC_ERR
 
 }  // namespace
 
-// Default packing when none is set, from timings on MI355X: 64 blocks per wave
-// in lockstep from 64 blocks per CU up, one wave per block (stripe kernel)
-// below, where a few long lockstep waves would leave most SIMDs idle.  One
-// 7680x4320 16-bit lossless image (8227 blocks of 64x64, 129 lockstep waves)
-// decodes its tier-1 in 33 ms one wave per block against 96 ms in lockstep;
-// four such images take 134 ms against 85 ms, sixteen 530 ms against 84 ms.
+// Default packing when none is set, from timings on MI355X: lockstep from 64
+// blocks per CU up, one wave per block (stripe kernel) below, where a few long
+// lockstep waves would leave most SIMDs idle.  Lockstep waves hold 64 blocks
+// from 8 such waves per CU up, else 32 (more, half-full waves run side by
+// side).  Tier-1 of 7680x4320 16-bit lossless images (8227 blocks of 64x64
+// each): one image 33 ms one wave per block, 74 ms in lockstep; four 134 ms
+// one per wave, 67 ms lockstep-64, 58 ms lockstep-32; eight 52 ms
+// lockstep-32 against 59 ms lockstep-64.
 int pick_blocks_per_wave(int n, bool small, int cus, const Tuning& t) {
     int fixed = small ? t.bpw32 : t.bpw64;
     if (fixed) return fixed;
-    return n >= 64 * std::max(cus, 1) ? TC_BPW_LOCKSTEP : 1;
+    const int c = std::max(cus, 1);
+    if (n < 64 * c) return 1;
+    return n >= 8 * 64 * c ? TC_BPW_LOCKSTEP : TC_BPW_LOCKSTEP32;
 }
 
 Pipeline::Pipeline(tc_device dev) : dev_(dev) {
@@ -211,9 +215,9 @@ Status Pipeline::prepare(const std::vector<const Image*>& imgs, std::string& err
             g.max_passes = std::max<int>(g.max_passes, descs[i].npasses);
         }
         g.order_off = put(blob, v);
-        if (g.bpw == TC_BPW_LOCKSTEP) {
+        if (g.bpw < 0) {  // lockstep, -bpw blocks per wave
             g.scratch_off = scratch_total;
-            scratch_total += align_up(tc_t1_lockstep_scratch_bytes(g.small ? 1 : 0, g.n, g.max_passes), 256);
+            scratch_total += align_up(tc_t1_lockstep_scratch_bytes(g.small ? 1 : 0, -g.bpw, g.n, g.max_passes), 256);
         }
         bpw_[k] = g.bpw;
         groups_.push_back(g);
@@ -408,8 +412,9 @@ Status Pipeline::run(bool timed) {
         if (s != TC_SUCCESS) break;
         const Group& g = groups_[gi];
         tc_stream st = (fork && gi == 1) ? side_ : stream_;
-        if (g.bpw == TC_BPW_LOCKSTEP)
-            s = tc_launch_t1_decode_lockstep(st, g.small ? 1 : 0, up + descs_off_, (const uint32_t*)(up + g.order_off),
+        if (g.bpw < 0)
+            s = tc_launch_t1_decode_lockstep(st, g.small ? 1 : 0, -g.bpw, up + descs_off_,
+                                             (const uint32_t*)(up + g.order_off),
                                              g.n, g.max_passes, up, cw_size_, (uint8_t*)tc_buf_device_ptr(scratch_) + g.scratch_off,
                                              tc_buf_size(scratch_) - g.scratch_off, base);
         else if (g.bpw == 1 && tuning_.t1_kernel == 0 && g.max_planes <= TC_T1_STRIPE_MAX_PLANES(g.small))

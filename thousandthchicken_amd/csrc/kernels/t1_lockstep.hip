@@ -536,7 +536,7 @@ __global__ __launch_bounds__(64) void t1_lockstep_kernel(const tc_cblk_desc* __r
                                                          const uint32_t* __restrict__ order, int n,
                                                          const uint8_t* __restrict__ arena, uint32_t arena_words,
                                                          uint32_t* __restrict__ coef, M* __restrict__ scratch,
-                                                         int nlog_max) {
+                                                         int nlog_max, int bpw) {
     __shared__ LdsLock<M> L;
     const int lane = threadIdx.x;
     for (int i = lane; i < 47; i += 64) L.trans[i] = trans_entry(i);
@@ -557,8 +557,10 @@ __global__ __launch_bounds__(64) void t1_lockstep_kernel(const tc_cblk_desc* __r
     }
     __syncthreads();
 
-    const int base = blockIdx.x * kLaneCount;
-    const bool have = base + lane < n;
+    // bpw (64, 32 or 16) blocks per wave: fewer than 64 leaves lanes idle but
+    // makes more waves when a launch has few per CU
+    const int base = blockIdx.x * bpw;
+    const bool have = lane < bpw && base + lane < n;
     tc_cblk_desc d;
     if (have) d = descs[order ? order[base + lane] : (uint32_t)(base + lane)];
     const bool active = have && d.numbps > 0 && d.npasses > 0;
@@ -575,7 +577,7 @@ __global__ __launch_bounds__(64) void t1_lockstep_kernel(const tc_cblk_desc* __r
     // cooperative read-back
     __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "agent");
     __syncthreads();
-    for (int j = 0; j < kLaneCount && base + j < n; j++) {
+    for (int j = 0; j < bpw && base + j < n; j++) {
         const tc_cblk_desc dj = descs[order ? order[base + j] : (uint32_t)(base + j)];
         convert_lockstep_block<M, MAXH>(lane, j, dj, coef, lg);
     }
@@ -584,31 +586,34 @@ __global__ __launch_bounds__(64) void t1_lockstep_kernel(const tc_cblk_desc* __r
 template <typename M, int MAXH>
 static hipError_t launch_lockstep_variant(hipStream_t s, const tc_cblk_desc* descs, const uint32_t* order, int n,
                                           const uint8_t* arena, uint32_t arena_bytes, uint32_t* coef, void* scratch,
-                                          int max_passes) {
+                                          int max_passes, int bpw) {
     if (n <= 0) return hipSuccess;
-    const int grid = (n + kLaneCount - 1) / kLaneCount;
+    const int grid = (n + bpw - 1) / bpw;
     hipLaunchKernelGGL((t1_lockstep_kernel<M, MAXH>), dim3(grid), dim3(kLaneCount), 0, s, descs, order, n, arena,
-                       arena_bytes / 4, coef, static_cast<M*>(scratch), max_passes / 3);
+                       arena_bytes / 4, coef, static_cast<M*>(scratch), max_passes / 3, bpw);
     return hipGetLastError();
 }
 
 }  // namespace
 
 // Scratch bytes the lockstep kernel needs for n blocks of one size class with
-// at most max_passes coding passes each: per wave, four state masks and one
-// refinement log per refinement pass, each MAXH rows of 64 lanes.
-size_t t1_lockstep_scratch_bytes(bool cls32, int n, int max_passes) {
-    const size_t waves = (size_t)((n + kLaneCount - 1) / kLaneCount);
+// at most max_passes coding passes each, bpw blocks per wave: per wave, four
+// state masks and one refinement log per refinement pass, each MAXH rows of
+// 64 lanes.
+size_t t1_lockstep_scratch_bytes(bool cls32, int n, int max_passes, int bpw) {
+    const size_t waves = (size_t)((n + bpw - 1) / bpw);
     const size_t rows = (size_t)(4 + max_passes / 3);
     return cls32 ? waves * rows * 32 * kLaneCount * sizeof(uint32_t) : waves * rows * 64 * kLaneCount * sizeof(uint64_t);
 }
 
-hipError_t launch_t1_lockstep(hipStream_t s, bool cls32, const tc_cblk_desc* descs, const uint32_t* order, int n,
-                              const uint8_t* arena, uint32_t arena_bytes, uint32_t* coef, void* scratch,
+hipError_t launch_t1_lockstep(hipStream_t s, bool cls32, int bpw, const tc_cblk_desc* descs, const uint32_t* order,
+                              int n, const uint8_t* arena, uint32_t arena_bytes, uint32_t* coef, void* scratch,
                               int max_passes) {
     if (cls32)
-        return launch_lockstep_variant<uint32_t, 32>(s, descs, order, n, arena, arena_bytes, coef, scratch, max_passes);
-    return launch_lockstep_variant<uint64_t, 64>(s, descs, order, n, arena, arena_bytes, coef, scratch, max_passes);
+        return launch_lockstep_variant<uint32_t, 32>(s, descs, order, n, arena, arena_bytes, coef, scratch, max_passes,
+                                                     bpw);
+    return launch_lockstep_variant<uint64_t, 64>(s, descs, order, n, arena, arena_bytes, coef, scratch, max_passes,
+                                                 bpw);
 }
 
 }  // namespace tc
