@@ -57,6 +57,12 @@ typedef struct {
 /* Main-header information (SIZ/COD/QCD) of a raw codestream or JP2 file. */
 tc_status tc_image_probe(const uint8_t *cs, size_t len, tc_image_info *info);
 
+/* Same for the image decoded with its `reduce` finest DWT levels discarded
+ * (tc_decoder_set_reduce): x0, y0, width, height and nlevels are those of the
+ * resolution `reduce` levels below full, ceil(x / 2^reduce).  Fails when the
+ * first tile-component has fewer than `reduce` levels. */
+tc_status tc_image_probe_reduced(const uint8_t *cs, size_t len, int reduce, tc_image_info *info);
+
 /* Parses the whole codestream and lists its code-blocks (tile, component,
  * resolution, band, raster order).  *count receives the total; at most max are
  * written to out (which may be NULL). */
@@ -77,6 +83,14 @@ tc_status tc_decoder_destroy(tc_decoder dec);
 #define TC_BPW_LOCKSTEP (-64)
 #define TC_BPW_LOCKSTEP32 (-32)
 tc_status tc_decoder_set_blocks_per_wave(tc_decoder dec, int small_blocks, int bpw);
+
+/* Resolution-reduced decode (applies from the next prepare or decode): the
+ * `reduce` finest DWT levels of every tile-component are not reconstructed and
+ * their code-blocks are not decoded; the output is the image at that
+ * resolution (size from tc_image_probe_reduced), after the colour transform
+ * and level shift.  0 decodes at full resolution.  Decoding fails for an image
+ * with a tile-component of fewer levels. */
+tc_status tc_decoder_set_reduce(tc_decoder dec, int reduce);
 
 /* Kernel for blocks decoded one per wavefront: 0 = stripe-column form
  * (tc_launch_t1_decode_stripe, default), 1 = row-mask form

@@ -44,6 +44,8 @@ def lib():
         L.or_list_cblks.argtypes = [ctypes.c_char_p, ctypes.c_size_t, P(Cblk), ctypes.c_int]
         L.or_t1_symbol_counts.argtypes = [ctypes.c_char_p, ctypes.c_size_t, ctypes.c_void_p, ctypes.c_int]
         L.or_decode.argtypes = [ctypes.c_char_p, ctypes.c_size_t, ctypes.c_void_p, ctypes.c_void_p]
+        L.or_decode_reduced.argtypes = [ctypes.c_char_p, ctypes.c_size_t, ctypes.c_int, ctypes.c_void_p,
+                                        ctypes.c_void_p]
         L.or_t1_decode.argtypes = [ctypes.c_char_p, ctypes.c_int, ctypes.c_int, ctypes.c_int,
                                    ctypes.c_int, ctypes.c_int, ctypes.c_int,
                                    ctypes.c_void_p, ctypes.c_void_p, ctypes.c_void_p]
@@ -69,16 +71,21 @@ def info(cs: bytes) -> Info:
     return i
 
 
-def decode(cs: bytes, want_float=False):
-    """Full decode -> int32 array [ncomp, H, W] (and float64 pre-rounding values)."""
+def decode(cs: bytes, want_float=False, reduce=0):
+    """Full decode -> int32 array [ncomp, H, W] (and float64 pre-rounding values).
+    reduce > 0 discards that many finest DWT levels: the image at the
+    resolution ceil(X / 2^reduce)."""
     i = info(cs)
-    shape = (i.ncomp, i.y1 - i.y0, i.x1 - i.x0)
+
+    def c(v):
+        return -((-v) >> reduce)
+    shape = (i.ncomp, c(i.y1) - c(i.y0), c(i.x1) - c(i.x0))
     out = np.zeros(shape, np.int32)
     outf = np.zeros(shape, np.float64) if want_float else None
-    rc = lib().or_decode(cs, len(cs), out.ctypes.data,
-                         outf.ctypes.data if want_float else None)
+    rc = lib().or_decode_reduced(cs, len(cs), reduce, out.ctypes.data,
+                                 outf.ctypes.data if want_float else None)
     if rc:
-        raise OracleError(f"or_decode failed ({rc})")
+        raise OracleError(f"or_decode_reduced failed ({rc})")
     return (out, outf) if want_float else out
 
 

@@ -1410,6 +1410,7 @@ typedef struct {
     int32_t *out;
     double *out_f;
     int W, H, X0, Y0;
+    int reduce; /* finest DWT levels not reconstructed */
 } dec_ctx;
 
 /* Mallat position of a band of resolution r inside the tile-component array. */
@@ -1422,10 +1423,18 @@ static void band_origin(const tc_t *tc, int r, int orient, int *ox, int *oy) {
     if (orient == 2 || orient == 3) *oy = lo->y1 - lo->y0;
 }
 
+/* Decodes one tile into D->out.  With D->reduce = k > 0 only resolutions
+ * 0..nres-1-k are decoded and inverse-transformed: the tile-component becomes
+ * its resolution nres-1-k rectangle (coordinates ceil(x / 2^k), B.5), and the
+ * inverse DWT runs nlev - k levels from that rectangle's origin. */
 static int decode_tile(const hdr_t *h, int t, tc_t *tcs, void *user) {
     dec_ctx *D = (dec_ctx *)user;
     int C = h->C;
-    int tw = tcs[0].x1 - tcs[0].x0, th = tcs[0].y1 - tcs[0].y0;
+    if (C <= 0) return OR_ERR_UNSUPPORTED;
+    for (int c = 0; c < C; c++)
+        if (tcs[c].nres - 1 < D->reduce) return OR_ERR_UNSUPPORTED;
+    const res_t *top0 = &tcs[0].r[tcs[0].nres - 1 - D->reduce];
+    int tw = top0->x1 - top0->x0, th = top0->y1 - top0->y0;
     long n = (long)tw * th;
     double **planes = (double **)calloc((size_t)C, sizeof(double *));
     if (!planes) return OR_ERR_NOMEM;
@@ -1441,7 +1450,7 @@ static int decode_tile(const hdr_t *h, int t, tc_t *tcs, void *user) {
             rc = OR_ERR_NOMEM;
             break;
         }
-        for (int r = 0; r < tc->nres && !rc; r++)
+        for (int r = 0; r < tc->nres - D->reduce && !rc; r++)
             for (int k = 0; k < tc->r[r].nb && !rc; k++) {
                 band_t *B = &tc->r[r].b[k];
                 int ox, oy;
@@ -1486,11 +1495,12 @@ static int decode_tile(const hdr_t *h, int t, tc_t *tcs, void *user) {
                 }
             }
         if (!rc) {
+            const res_t *top = &tc->r[tc->nres - 1 - D->reduce];
             if (rev) {
-                rc = or_idwt53(ia, tw, th, tc->x0, tc->y0, tc->cc.nlev);
+                rc = or_idwt53(ia, tw, th, top->x0, top->y0, tc->cc.nlev - D->reduce);
                 for (long i = 0; i < n; i++) da[i] = ia[i];
             } else
-                rc = or_idwt97(da, tw, th, tc->x0, tc->y0, tc->cc.nlev);
+                rc = or_idwt97(da, tw, th, top->x0, top->y0, tc->cc.nlev - D->reduce);
         }
         free(ia);
     }
@@ -1526,7 +1536,8 @@ static int decode_tile(const hdr_t *h, int t, tc_t *tcs, void *user) {
                     long iv = rev ? (long)v : (long)rint(v);
                     if (iv < lo) iv = lo;
                     if (iv > hi) iv = hi;
-                    long o = (long)c * D->W * D->H + (long)(tcs[c].y0 - D->Y0 + y) * D->W + (tcs[c].x0 - D->X0 + x);
+                    const res_t *top = &tcs[c].r[tcs[c].nres - 1 - D->reduce];
+                    long o = (long)c * D->W * D->H + (long)(top->y0 - D->Y0 + y) * D->W + (top->x0 - D->X0 + x);
                     D->out[o] = (int32_t)iv;
                     if (D->out_f) D->out_f[o] = v;
                 }
@@ -1539,10 +1550,19 @@ static int decode_tile(const hdr_t *h, int t, tc_t *tcs, void *user) {
 }
 
 int or_decode(const uint8_t *cs, size_t len, int32_t *out, double *out_f) {
+    return or_decode_reduced(cs, len, 0, out, out_f);
+}
+
+static int ceil_shift(int v, int k) { return (int)(((long)v + (1L << k) - 1) >> k); }
+
+int or_decode_reduced(const uint8_t *cs, size_t len, int reduce, int32_t *out, double *out_f) {
     or_info info;
     int rc = or_read_info(cs, len, &info);
     if (rc) return rc;
-    dec_ctx D = {out, out_f, info.x1 - info.x0, info.y1 - info.y0, info.x0, info.y0};
+    if (reduce < 0 || reduce > 32) return -OR_ERR_UNSUPPORTED;
+    int X0 = ceil_shift(info.x0, reduce), Y0 = ceil_shift(info.y0, reduce);
+    int X1 = ceil_shift(info.x1, reduce), Y1 = ceil_shift(info.y1, reduce);
+    dec_ctx D = {out, out_f, X1 - X0, Y1 - Y0, X0, Y0, reduce};
     rc = walk_tiles(cs, len, decode_tile, &D);
     return rc ? -rc : OR_OK;
 }

@@ -78,6 +78,10 @@ int or_t1_symbol_counts(const uint8_t *cs, size_t len, int32_t *nsym, int max);
  * DC shift, rounding and clamping.  out_f (optional): the same values before
  * rounding/clamping (FP64 for the 9/7 path, exact integers for 5/3). */
 int or_decode(const uint8_t *cs, size_t len, int32_t *out, double *out_f);
+/* Same with the `reduce` finest DWT levels not reconstructed: out holds the
+ * image at that resolution, ceil(X / 2^reduce) wide (T.800 B.5), after the
+ * colour transform and level shift. */
+int or_decode_reduced(const uint8_t *cs, size_t len, int reduce, int32_t *out, double *out_f);
 
 /* Tier-1 decode of one code-block (code-block style 0).  orient: 0 LL,
  * 1 HL, 2 LH, 3 HH.  numbps = M_b - zero_bp bit-planes.  Outputs per

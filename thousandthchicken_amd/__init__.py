@@ -75,6 +75,8 @@ def lib():
         "tc_device_close": (i32, [vp]),
         "tc_device_get_props": (i32, [vp, P(DeviceProps)]),
         "tc_image_probe": (i32, [ctypes.c_char_p, sz, P(ImageInfo)]),
+        "tc_image_probe_reduced": (i32, [ctypes.c_char_p, sz, i32, P(ImageInfo)]),
+        "tc_decoder_set_reduce": (i32, [vp, i32]),
         "tc_image_list_cblks": (i32, [ctypes.c_char_p, sz, vp, i32, P(i32)]),
         "tc_decoder_create": (i32, [vp, P(vp)]),
         "tc_decoder_destroy": (i32, [vp]),
@@ -158,9 +160,14 @@ def exported_symbols():
     return sorted(names)
 
 
-def probe(cs: bytes) -> ImageInfo:
+def probe(cs: bytes, reduce=0) -> ImageInfo:
+    """Main-header information; with reduce > 0, of the image decoded with its
+    `reduce` finest DWT levels discarded."""
     info = ImageInfo()
-    check(lib().tc_image_probe(cs, len(cs), ctypes.byref(info)), "tc_image_probe")
+    if reduce:
+        check(lib().tc_image_probe_reduced(cs, len(cs), reduce, ctypes.byref(info)), "tc_image_probe_reduced")
+    else:
+        check(lib().tc_image_probe(cs, len(cs), ctypes.byref(info)), "tc_image_probe")
     return info
 
 
@@ -383,6 +390,7 @@ class Decoder:
         self.handle = h
         self._infos = []
         self._keep = None
+        self._reduce = 0
 
     def close(self):
         if getattr(self, "handle", None):
@@ -412,11 +420,16 @@ class Decoder:
     def set_t1_kernel(self, kind):
         check(lib().tc_decoder_set_t1_kernel(self.handle, kind), "set_t1_kernel")
 
+    def set_reduce(self, reduce):
+        """Decode with the `reduce` finest DWT levels discarded (0: full resolution)."""
+        check(lib().tc_decoder_set_reduce(self.handle, reduce), "set_reduce")
+        self._reduce = reduce
+
     def set_blocks_per_wave(self, small, bpw):
         check(lib().tc_decoder_set_blocks_per_wave(self.handle, 1 if small else 0, bpw), "set_blocks_per_wave")
 
     def decode(self, cs: bytes) -> np.ndarray:
-        info = probe(cs)
+        info = probe(cs, self._reduce)
         shape, dt = _shape(info)
         out = np.empty(shape, dt)
         check(lib().tc_decoder_decode(self.handle, cs, len(cs), out.ctypes.data, out.nbytes, None), "decode")
@@ -424,7 +437,7 @@ class Decoder:
 
     def prepare(self, streams):
         streams = [bytes(s) for s in streams]
-        self._infos = [probe(s) for s in streams]
+        self._infos = [probe(s, self._reduce) for s in streams]
         n = len(streams)
         ptrs = (ctypes.c_char_p * n)(*streams)
         lens = (ctypes.c_size_t * n)(*[len(s) for s in streams])

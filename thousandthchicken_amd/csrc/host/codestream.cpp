@@ -773,6 +773,45 @@ Status probe_codestream(const uint8_t* data, size_t len, Image& img, std::string
     return parse_impl(data, len, img, err, false);
 }
 
+Status reduce_image(Image& img, int k, std::string& err) {
+    if (k == 0) return Status::Ok;
+    if (k < 0 || k > 32) {
+        err = "reduce out of range";
+        return Status::InvalidArgument;
+    }
+    for (const Tile& T : img.tiles)
+        for (const TileComp& tc : T.comps)
+            if (tc.nlev < k) {
+                err = "reduce " + std::to_string(k) + " exceeds the " + std::to_string(tc.nlev) +
+                      " decomposition levels of a tile-component";
+                return Status::Unsupported;
+            }
+    // resolution coordinates k levels down: ceil(x / 2^k) (B.5)
+    auto cdiv = [k](int v) { return (int)(((int64_t)v + ((int64_t)1 << k) - 1) >> k); };
+    for (Tile& T : img.tiles) {
+        for (TileComp& tc : T.comps) {
+            tc.nlev -= k;
+            tc.res.resize((size_t)tc.nlev + 1);
+            const Resolution& R = tc.res[tc.nlev];
+            tc.x0 = R.x0;
+            tc.y0 = R.y0;
+            tc.x1 = R.x1;
+            tc.y1 = R.y1;
+        }
+        T.x0 = cdiv(T.x0);
+        T.y0 = cdiv(T.y0);
+        T.x1 = cdiv(T.x1);
+        T.y1 = cdiv(T.y1);
+    }
+    for (CodeBlock& cb : img.cblks)
+        if (cb.res > img.tiles[cb.tile].comps[cb.comp].nlev) cb.included = false;
+    img.x0 = cdiv(img.x0);
+    img.y0 = cdiv(img.y0);
+    img.x1 = cdiv(img.x1);
+    img.y1 = cdiv(img.y1);
+    return Status::Ok;
+}
+
 const char* status_name(Status s) {
     switch (s) {
     case Status::Ok: return "ok";

@@ -111,6 +111,14 @@ Status parse_codestream(const uint8_t* data, size_t len, Image& img, std::string
 // Main header only (SIZ/COD/QCD), no packets.
 Status probe_codestream(const uint8_t* data, size_t len, Image& img, std::string& err);
 
+// Resolution-reduced decode: turns a parsed image into the image k DWT levels
+// down (its LL band k levels below full resolution, then the usual colour
+// transform and level shift).  Each tile-component loses its k finest
+// levels, code-blocks of those levels are dropped, and every rectangle moves
+// to that resolution's coordinates, ceil(x / 2^k) (T.800 B.5).  Fails when a
+// tile-component has fewer than k levels.
+Status reduce_image(Image& img, int k, std::string& err);
+
 const char* status_name(Status s);
 
 }  // namespace tc

@@ -17,6 +17,7 @@ struct tc_decoder_s {
     std::unique_ptr<Pipeline> pipe;
     std::vector<std::vector<uint8_t>> data;  // owned copies of the prepared codestreams
     std::vector<std::unique_ptr<Image>> imgs;
+    int reduce = 0;  // finest DWT levels not reconstructed
 };
 
 namespace {
@@ -87,6 +88,17 @@ tc_status tc_image_probe(const uint8_t* cs, size_t len, tc_image_info* info) {
     Image im;
     std::string err;
     Status s = probe_codestream(cs, len, im, err);
+    if (s != Status::Ok) return report(s, err);
+    fill_info(im, info);
+    return TC_SUCCESS;
+}
+
+tc_status tc_image_probe_reduced(const uint8_t* cs, size_t len, int reduce, tc_image_info* info) {
+    if (!cs || !info) return report(Status::InvalidArgument, "tc_image_probe_reduced: null");
+    Image im;
+    std::string err;
+    Status s = probe_codestream(cs, len, im, err);
+    if (s == Status::Ok) s = reduce_image(im, reduce, err);
     if (s != Status::Ok) return report(s, err);
     fill_info(im, info);
     return TC_SUCCESS;
@@ -172,6 +184,12 @@ tc_status tc_decoder_set_blocks_per_wave(tc_decoder d, int small, int bpw) {
     Tuning t = d->pipe->tuning();
     (small ? t.bpw32 : t.bpw64) = bpw;
     d->pipe->set_tuning(t);
+    return TC_SUCCESS;
+}
+
+tc_status tc_decoder_set_reduce(tc_decoder d, int reduce) {
+    if (!d || reduce < 0 || reduce > 32) return report(Status::InvalidArgument, "tc_decoder_set_reduce: 0..32");
+    d->reduce = reduce;
     return TC_SUCCESS;
 }
 
@@ -307,6 +325,7 @@ tc_status tc_decoder_prepare(tc_decoder d, int n, const uint8_t* const* cs, cons
         d->imgs.emplace_back(new Image());
         std::string err;
         Status s = parse_codestream(d->data.back().data(), d->data.back().size(), *d->imgs.back(), err);
+        if (s == Status::Ok) s = reduce_image(*d->imgs.back(), d->reduce, err);
         if (s != Status::Ok) return report(s, "image " + std::to_string(i) + ": " + err);
         v.push_back(d->imgs.back().get());
     }
