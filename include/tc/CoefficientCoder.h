@@ -23,6 +23,7 @@ class CoefficientCoder : public DeviceKernel {
   private:
     tc_device device() const;
     tc_buf d_upload_ = nullptr;
+    tc_buf d_scratch_ = nullptr;  // lockstep tier-1 state (code-block styles)
     size_t upload_cap_ = 0;
     int bpw32_ = 0, bpw64_ = 0;
     float last_ms_ = 0.f;

@@ -151,7 +151,10 @@ tc_status tc_launch_t1_decode_stripe(tc_stream s, int small_blocks, int grid, co
  * 8-byte aligned, holding per-block significance / sign / pass state and the
  * refinement 1-bits of each pass (no initialisation needed).  Codewords may
  * start at any byte offset; arena is 16-byte aligned and arena_bytes (a
- * multiple of 16, at least 64) readable bytes cover every codeword. */
+ * multiple of 16, at least 64) readable bytes cover every codeword.  This is
+ * the only tier-1 launch that decodes the code-block style bits of each
+ * descriptor's mode (RESET, CAUSAL, SEGSYM; T.800 Table A.19); the others
+ * ignore them. */
 size_t tc_t1_lockstep_scratch_bytes(int small_blocks, int blocks_per_wave, int n, int max_passes);
 tc_status tc_launch_t1_decode_lockstep(tc_stream s, int small_blocks, int blocks_per_wave, const void *descs,
                                        const uint32_t *order, int n, int max_passes, const uint8_t *arena,

@@ -49,6 +49,9 @@ def lib():
         L.or_t1_decode.argtypes = [ctypes.c_char_p, ctypes.c_int, ctypes.c_int, ctypes.c_int,
                                    ctypes.c_int, ctypes.c_int, ctypes.c_int,
                                    ctypes.c_void_p, ctypes.c_void_p, ctypes.c_void_p]
+        L.or_t1_decode_style.argtypes = [ctypes.c_char_p, ctypes.c_int, ctypes.c_int, ctypes.c_int,
+                                         ctypes.c_int, ctypes.c_int, ctypes.c_int, ctypes.c_int,
+                                         ctypes.c_void_p, ctypes.c_void_p, ctypes.c_void_p]
         for f in ("or_idwt53", "or_fdwt53", "or_idwt97", "or_fdwt97"):
             getattr(L, f).argtypes = [ctypes.c_void_p] + [ctypes.c_int] * 5
         L.or_mq_decode_symbols.argtypes = [ctypes.c_char_p, ctypes.c_int, ctypes.c_char_p,
@@ -108,12 +111,16 @@ def symbol_counts(cs: bytes):
     return out[:n]
 
 
-def t1_decode(data: bytes, w, h, orient, numbps, npasses):
+# Code-block style bits (T.800 Table A.19) the oracle decodes.
+STYLE_RESET, STYLE_CAUSAL, STYLE_PTERM, STYLE_SEGSYM = 2, 8, 16, 32
+
+
+def t1_decode(data: bytes, w, h, orient, numbps, npasses, style=0):
     mag = np.zeros((h, w), np.int32)
     sign = np.zeros((h, w), np.int8)
     low = np.zeros((h, w), np.int8)
-    rc = lib().or_t1_decode(data, len(data), w, h, orient, numbps, npasses,
-                            mag.ctypes.data, sign.ctypes.data, low.ctypes.data)
+    rc = lib().or_t1_decode_style(data, len(data), w, h, orient, numbps, npasses, style,
+                                  mag.ctypes.data, sign.ctypes.data, low.ctypes.data)
     if rc < 0:
         raise OracleError(f"or_t1_decode failed ({rc})")
     return mag, sign, low, rc

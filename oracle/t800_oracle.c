@@ -225,28 +225,37 @@ void or_tables(uint8_t *zc, uint32_t *mq) {
 }
 
 /* ------------------------------------------------------------------------ */
-/* Tier-1 decoding of one code-block, style 0 (T.800 D.3-D.5).  Follows the
- * pass order of g_decode, TC/coefficient_coder.cl:778-819: a cleanup pass on
- * the top plane, then significance propagation, magnitude refinement and
- * cleanup on every lower plane, stripes of 4 rows, column-major inside a
- * stripe (TC/coefficient_coder.cl:681-776).  npasses bounds the passes decoded
- * (truncated code-blocks), which the reference ignored. */
+/* Tier-1 decoding of one code-block (T.800 D.3-D.5).  Follows the pass order
+ * of g_decode, TC/coefficient_coder.cl:778-819: a cleanup pass on the top
+ * plane, then significance propagation, magnitude refinement and cleanup on
+ * every lower plane, stripes of 4 rows, column-major inside a stripe
+ * (TC/coefficient_coder.cl:681-776).  npasses bounds the passes decoded
+ * (truncated code-blocks), which the reference ignored.  Code-block style
+ * (COD/COC SPcod, Table A.19) adds, where set: RESET, every context back to
+ * its initial state after each pass (D.6); CAUSAL, the row below a stripe
+ * taken as insignificant when forming any context of the stripe's last row
+ * (D.7); SEGSYM, four uniform-context symbols after each cleanup pass (D.5).
+ * The reference decodes style 0 only. */
 
 typedef struct {
     int w, h, W; /* W = w + 2 (bordered pitch) */
     uint8_t *sig, *vis, *ref, *sgn;
     int32_t *mag;
     int8_t *low;
+    int causal;
 } t1_t;
 
 #define AT(t, x, y) (((y) + 1) * (t)->W + (x) + 1)
 
+/* 1 when the row below (x, y) counts as a neighbour */
+static int below(const t1_t *t, int y) { return !(t->causal && (y & 3) == 3); }
+
 static int nb_counts(const t1_t *t, int x, int y, int *h, int *v, int *d) {
     const uint8_t *s = t->sig;
-    int i = AT(t, x, y), W = t->W;
+    int i = AT(t, x, y), W = t->W, b = below(t, y);
     *h = s[i - 1] + s[i + 1];
-    *v = s[i - W] + s[i + W];
-    *d = s[i - W - 1] + s[i - W + 1] + s[i + W - 1] + s[i + W + 1];
+    *v = s[i - W] + (b ? s[i + W] : 0);
+    *d = s[i - W - 1] + s[i - W + 1] + (b ? s[i + W - 1] + s[i + W + 1] : 0);
     return *h + *v + *d;
 }
 
@@ -255,7 +264,7 @@ static int contrib(const t1_t *t, int i) { return t->sig[i] ? (t->sgn[i] ? -1 : 
 static void decode_sign(t1_t *t, mqd_t *m, int x, int y) {
     int i = AT(t, x, y), W = t->W, xr;
     int h = contrib(t, i - 1) + contrib(t, i + 1);
-    int v = contrib(t, i - W) + contrib(t, i + W);
+    int v = contrib(t, i - W) + (below(t, y) ? contrib(t, i + W) : 0);
     h = h > 0 ? 1 : (h < 0 ? -1 : 0);
     v = v > 0 ? 1 : (v < 0 ? -1 : 0);
     int lab = or_sign_ctx(h, v, &xr);
@@ -332,7 +341,13 @@ static void pass_cleanup(t1_t *t, mqd_t *m, int cls, int p) {
 
 int or_t1_decode(const uint8_t *data, int len, int w, int h, int orient, int numbps, int npasses,
                  int32_t *mag, int8_t *sign, int8_t *lowplane) {
+    return or_t1_decode_style(data, len, w, h, orient, numbps, npasses, 0, mag, sign, lowplane);
+}
+
+int or_t1_decode_style(const uint8_t *data, int len, int w, int h, int orient, int numbps, int npasses, int style,
+                       int32_t *mag, int8_t *sign, int8_t *lowplane) {
     if (w <= 0 || h <= 0 || w > 1024 || h > 1024 || orient < 0 || orient > 3) return -OR_ERR_ARG;
+    if (style & ~OR_STYLE_DECODABLE) return -OR_ERR_UNSUPPORTED;
     int cls = orient == 1 ? 1 : (orient == 3 ? 2 : 0);
     t1_t t;
     int n = (w + 2) * (h + 2);
@@ -347,6 +362,7 @@ int or_t1_decode(const uint8_t *data, int len, int w, int h, int orient, int num
     t.sgn = flags + 3 * n;
     t.mag = mag;
     t.low = lowplane;
+    t.causal = (style & OR_STYLE_CAUSAL) != 0;
     for (int i = 0; i < w * h; i++) {
         mag[i] = 0;
         lowplane[i] = -1;
@@ -361,8 +377,20 @@ int or_t1_decode(const uint8_t *data, int len, int w, int h, int orient, int num
             pass_sigprop(&t, &m, cls, p);
         else if (type == 1)
             pass_magref(&t, &m, p);
-        else
+        else {
             pass_cleanup(&t, &m, cls, p);
+            if (style & OR_STYLE_SEGSYM) {
+                /* the segmentation symbol 1010 (D.5): anything else means the
+                 * codeword (or the style signalled for it) is wrong */
+                int seg = 0;
+                for (int s = 0; s < 4; s++) seg = (seg << 1) | mq_decode(&m, CX_UNI);
+                if (seg != 0xA) {
+                    free(flags);
+                    return -OR_ERR_CORRUPT;
+                }
+            }
+        }
+        if (style & OR_STYLE_RESET) mq_reset_ctx(&m);
     }
     for (int y = 0; y < h; y++)
         for (int x = 0; x < w; x++) sign[y * w + x] = (int8_t)t.sgn[AT(&t, x, y)];
@@ -971,7 +999,7 @@ static int build_tc(const hdr_t *h, int t, int c, tc_t *tc) {
     tc->qc = h->qc[c];
     int NL = tc->cc.nlev;
     tc->nres = NL + 1;
-    if (tc->cc.cbsty != 0) return OR_ERR_UNSUPPORTED;
+    if (tc->cc.cbsty & ~OR_STYLE_DECODABLE) return OR_ERR_UNSUPPORTED;
     if (tc->qc.style == 0 && tc->qc.n < 3 * NL + 1) return OR_ERR_MARKER;
     if (tc->qc.style == 2 && tc->qc.n < 3 * NL + 1) return OR_ERR_MARKER;
     if (tc->qc.style == 1 && tc->qc.n < 1) return OR_ERR_MARKER;
@@ -1364,7 +1392,8 @@ static int list_tile(const hdr_t *h, int t, tc_t *tcs, void *user) {
                             int32_t *mag = (int32_t *)malloc(sizeof(int32_t) * (size_t)w * hh);
                             int8_t *sg = (int8_t *)malloc((size_t)w * hh), *lp = (int8_t *)malloc((size_t)w * hh);
                             if (mag && sg && lp)
-                                k = or_t1_decode(C->data, C->len, w, hh, B->orient, B->mb - C->zbp, C->npasses, mag, sg, lp);
+                                k = or_t1_decode_style(C->data, C->len, w, hh, B->orient, B->mb - C->zbp, C->npasses,
+                                                       tcs[c].cc.cbsty, mag, sg, lp);
                             free(mag);
                             free(sg);
                             free(lp);
@@ -1473,7 +1502,8 @@ static int decode_tile(const hdr_t *h, int t, tc_t *tcs, void *user) {
                         break;
                     }
                     int numbps = B->mb - CB->zbp;
-                    int nr = or_t1_decode(CB->data, CB->len, w, hh, B->orient, numbps, CB->npasses, mag, sg, lp);
+                    int nr = or_t1_decode_style(CB->data, CB->len, w, hh, B->orient, numbps, CB->npasses, tc->cc.cbsty,
+                                                mag, sg, lp);
                     if (nr < 0) rc = -nr;
                     for (int y = 0; y < hh && !rc; y++)
                         for (int x = 0; x < w; x++) {

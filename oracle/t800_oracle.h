@@ -35,6 +35,7 @@ enum {
     OR_ERR_GEOMETRY = 4,
     OR_ERR_NOMEM = 5,
     OR_ERR_ARG = 6,
+    OR_ERR_CORRUPT = 7, /* a codeword check failed (segmentation symbol) */
 };
 
 typedef struct {
@@ -88,6 +89,16 @@ int or_decode_reduced(const uint8_t *cs, size_t len, int reduce, int32_t *out, d
  * coefficient (row-major w*h): magnitude bits, sign (0/1) and the lowest
  * bit-plane coded for it (-1 when it never became significant).  Returns the
  * number of MQ symbols decoded, or -error. */
+/* Code-block style bits (T.800 Table A.19) the oracle decodes; BYPASS (1) and
+ * TERMALL (4) are not among them. */
+#define OR_STYLE_RESET 2
+#define OR_STYLE_CAUSAL 8
+#define OR_STYLE_PTERM 16
+#define OR_STYLE_SEGSYM 32
+#define OR_STYLE_DECODABLE (OR_STYLE_RESET | OR_STYLE_CAUSAL | OR_STYLE_PTERM | OR_STYLE_SEGSYM)
+/* or_t1_decode with code-block style bits */
+int or_t1_decode_style(const uint8_t *data, int len, int w, int h, int orient, int numbps, int npasses, int style,
+                       int32_t *mag, int8_t *sign, int8_t *lowplane);
 int or_t1_decode(const uint8_t *data, int len, int w, int h, int orient,
                  int numbps, int npasses, int32_t *mag, int8_t *sign, int8_t *lowplane);
 

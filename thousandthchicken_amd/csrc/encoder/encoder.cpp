@@ -61,6 +61,10 @@ struct MQEncoder {
         s.C = 0;
         s.CT = 12;
         buf.assign(1, 0);
+        reset_contexts();
+    }
+    // initial context states, T.800 Table D.7 (also the RESET style after each pass)
+    void reset_contexts() {
         std::memset(s.I, 0, sizeof s.I);
         std::memset(s.MPS, 0, sizeof s.MPS);
         s.I[0] = 4;
@@ -139,6 +143,10 @@ struct MQEncoder {
     size_t bytes_so_far() const { return buf.size() - 1; }
 };
 
+// Recorded in place of a context label where the RESET style reset every
+// context; never a real label.
+constexpr uint8_t kResetMark = 255;
+
 // MQ decoder used only to validate truncation points.
 struct MQCheck {
     const uint8_t* d;
@@ -183,6 +191,14 @@ struct MQCheck {
         I[CTX_UNI] = 46;
     }
     int decode(int cx) {
+        if (cx == kResetMark) {  // RESET style: contexts back to their initial states
+            std::memset(I, 0, sizeof I);
+            std::memset(MPS, 0, sizeof MPS);
+            I[0] = 4;
+            I[CTX_RUN] = 3;
+            I[CTX_UNI] = 46;
+            return 0;
+        }
         int i = I[cx];
         uint32_t qe = QE[i];
         int dd;
@@ -264,8 +280,12 @@ struct BlockResult {
 
 // Encodes w x h signed coefficients (integer quantization indices) of a band
 // with orientation orient.  weight scales the distortion estimate (square of
-// the synthesis norm times the step size squared).
-void t1_encode(const int32_t* coef, int w, int h, int pitch, int orient, double weight, bool record, BlockResult& R) {
+// the synthesis norm times the step size squared).  style: code-block style
+// bits RESET (2), CAUSAL (8) and SEGSYM (32) are honoured (T.800 D.5-D.7);
+// PTERM (16) needs nothing beyond this encoder's standard flush.
+void t1_encode(const int32_t* coef, int w, int h, int pitch, int orient, double weight, bool record, BlockResult& R,
+               int style = 0) {
+    const bool causal = (style & 8) != 0;
     int cls = orient == 1 ? 1 : (orient == 3 ? 2 : 0);
     int W = w + 2;
     std::vector<uint8_t> sig((size_t)W * (h + 2)), vis((size_t)W * (h + 2)), ref((size_t)W * (h + 2)),
@@ -301,18 +321,21 @@ void t1_encode(const int32_t* coef, int w, int h, int pitch, int orient, double 
         }
     };
     auto AT = [&](int x, int y) { return (size_t)(y + 1) * W + x + 1; };
+    // CAUSAL: the row below a stripe is never a neighbour of the stripe's last row
+    auto below = [&](int y) { return !(causal && (y & 3) == 3); };
     auto counts = [&](int x, int y, int& hh, int& vv, int& dd) {
         size_t i = AT(x, y);
+        const bool b = below(y);
         hh = sig[i - 1] + sig[i + 1];
-        vv = sig[i - W] + sig[i + W];
-        dd = sig[i - W - 1] + sig[i - W + 1] + sig[i + W - 1] + sig[i + W + 1];
+        vv = sig[i - W] + (b ? sig[i + W] : 0);
+        dd = sig[i - W - 1] + sig[i - W + 1] + (b ? sig[i + W - 1] + sig[i + W + 1] : 0);
         return hh + vv + dd;
     };
     auto contrib = [&](size_t i) { return sig[i] ? (sgn[i] ? -1 : 1) : 0; };
     double dist = 0;
     auto code_sign = [&](int x, int y) {
         size_t i = AT(x, y);
-        int H = contrib(i - 1) + contrib(i + 1), V = contrib(i - W) + contrib(i + W);
+        int H = contrib(i - 1) + contrib(i + 1), V = contrib(i - W) + (below(y) ? contrib(i + W) : 0);
         H = H > 0 ? 1 : (H < 0 ? -1 : 0);
         V = V > 0 ? 1 : (V < 0 ? -1 : 0);
         int xr;
@@ -417,6 +440,19 @@ void t1_encode(const int32_t* coef, int w, int h, int pitch, int orient, double 
                     }
                 }
             std::fill(vis.begin(), vis.end(), 0);
+            if (style & 32) {  // SEGSYM: 1010 in the uniform context
+                enc(1, CTX_UNI);
+                enc(0, CTX_UNI);
+                enc(1, CTX_UNI);
+                enc(0, CTX_UNI);
+            }
+        }
+        if (style & 2) {
+            mq.reset_contexts();
+            if (record) {
+                R.sym_ctx.push_back(kResetMark);
+                R.sym_bit.push_back(0);
+            }
         }
         R.passes.push_back(Pass{mq.bytes_so_far(), dist, R.sym_ctx.size()});
     }
@@ -663,6 +699,7 @@ struct Params {
     int w, h, ncomp, prec, levels, xcb, ycb, tw, th, mct, irreversible, threads;
     double target_bytes;  // irreversible: codestream size target (0: keep every pass)
     double base_step;     // irreversible: step for a unit-gain band
+    int cbsty = 0;        // code-block style (RESET 2, CAUSAL 8, PTERM 16, SEGSYM 32)
 };
 
 // L2 norm of the synthesis basis function of a band at level nb (orientation o),
@@ -910,7 +947,7 @@ int encode(const void* pixels, const Params& P, std::vector<uint8_t>& out) {
         const BandG& B = g.res[cb.band_idx / 3][cb.band_idx % 3];
         int w = g.x1 - g.x0;
         const int32_t* src = T.coef[all[i].c].data() + (size_t)(B.oy + cb.y0 - B.y0) * w + (B.ox + cb.x0 - B.x0);
-        t1_encode(src, cb.x1 - cb.x0, cb.y1 - cb.y0, w, B.orient, cb.weight, rate, cb.r);
+        t1_encode(src, cb.x1 - cb.x0, cb.y1 - cb.y0, w, B.orient, cb.weight, rate, cb.r, P.cbsty);
         cb.keep = cb.r.npasses;
         cb.keep_len = (uint32_t)cb.r.data.size();
     });
@@ -1009,7 +1046,7 @@ int encode(const void* pixels, const Params& P, std::vector<uint8_t>& out) {
     out.push_back((uint8_t)P.levels);
     out.push_back((uint8_t)(P.xcb - 2));
     out.push_back((uint8_t)(P.ycb - 2));
-    out.push_back(0);                  // code-block style
+    out.push_back((uint8_t)P.cbsty);   // code-block style
     out.push_back(irr ? 0 : 1);        // 9/7 or 5/3
     put16(out, 0xFF5C);  // QCD
     if (!irr) {
@@ -1123,11 +1160,11 @@ extern "C" {
 // Returns 0 and a malloc'ed codestream in *out (free with tce_free).
 int tce_encode(const void* pixels, int w, int h, int ncomp, int prec, int levels, int cbw_exp, int cbh_exp,
                int tile_w, int tile_h, int mct, int irreversible, double target_bytes, double base_step, int threads,
-               uint8_t** out, size_t* out_len) {
-    if (!pixels || !out || !out_len || w <= 0 || h <= 0) return -1;
+               int cbsty, uint8_t** out, size_t* out_len) {
+    if (!pixels || !out || !out_len || w <= 0 || h <= 0 || (cbsty & ~(2 | 8 | 16 | 32))) return -1;
     tce::Params P{w, h, ncomp, prec, levels, cbw_exp, cbh_exp, tile_w, tile_h, mct, irreversible,
                   threads > 0 ? threads : (int)std::max(1u, std::thread::hardware_concurrency()), target_bytes,
-                  base_step > 0 ? base_step : 0.5};
+                  base_step > 0 ? base_step : 0.5, cbsty};
     std::vector<uint8_t> cs;
     int rc = tce::encode(pixels, P, cs);
     if (rc) return rc;
@@ -1151,11 +1188,20 @@ int tce_t1_encode_block(const int32_t* coef, int w, int h, int orient, uint8_t* 
     return R.numbps;
 }
 
+int tce_t1_encode_truncated_style(const int32_t* coef, int w, int h, int orient, int keep, int style, uint8_t* out,
+                                  int cap, int* numbps);
+
 // Tier-1 encode and truncate after `keep` passes at the shortest decodable
 // length (test hook).  Returns the length.
 int tce_t1_encode_truncated(const int32_t* coef, int w, int h, int orient, int keep, uint8_t* out, int cap, int* numbps) {
+    return tce_t1_encode_truncated_style(coef, w, h, orient, keep, 0, out, cap, numbps);
+}
+
+// Same with code-block style bits (RESET 2, CAUSAL 8, PTERM 16, SEGSYM 32).
+int tce_t1_encode_truncated_style(const int32_t* coef, int w, int h, int orient, int keep, int style, uint8_t* out,
+                                  int cap, int* numbps) {
     tce::BlockResult R;
-    tce::t1_encode(coef, w, h, w, orient, 1.0, true, R);
+    tce::t1_encode(coef, w, h, w, orient, 1.0, true, R, style);
     *numbps = R.numbps;
     if (!R.npasses) return 0;
     if (keep > R.npasses) keep = R.npasses;

@@ -419,8 +419,11 @@ Status build_tile(Image& img, int t, const Params& P, std::vector<std::vector<Ba
         tc.nlev = C.nlev;
         tc.reversible = C.xform;
         tc.cbsty = C.cbsty;
-        if (C.cbsty != 0) {
-            err = "code-block style flags are not supported";
+        // decodable styles: RESET (2), CAUSAL (8), PTERM (16, nothing to do for a
+        // decoder) and SEGSYM (32); BYPASS (1) and TERMALL (4) split a codeword
+        // into per-pass segments, which the tier-1 kernels do not take
+        if (C.cbsty & ~(2 | 8 | 16 | 32)) {
+            err = "code-block styles BYPASS and TERMALL are not supported";
             return Status::Unsupported;
         }
         int nq = Q.style == 1 ? 1 : 3 * C.nlev + 1;

@@ -210,10 +210,14 @@ Status Pipeline::prepare(const std::vector<const Image*>& imgs, std::string& err
         g.small = k == 0;
         g.bpw = pick_blocks_per_wave((int)v.size(), g.small, cus, tuning_);
         g.n = (int)v.size();
+        int style = 0;
         for (uint32_t i : v) {
             g.max_planes = std::max<int>(g.max_planes, descs[i].numbps);
             g.max_passes = std::max<int>(g.max_passes, descs[i].npasses);
+            style |= descs[i].mode & TC_MODE_STYLE;
         }
+        // only the lockstep kernel decodes the code-block styles
+        if (style && g.bpw >= 0) g.bpw = g.n >= 8 * 64 * std::max(cus, 1) ? TC_BPW_LOCKSTEP : TC_BPW_LOCKSTEP32;
         g.order_off = put(blob, v);
         if (g.bpw < 0) {  // lockstep, -bpw blocks per wave
             g.scratch_off = scratch_total;

@@ -64,7 +64,8 @@ Status plan_tile_cblks(const Image& im, int t, const uint64_t* plane, uint64_t& 
         d.numbps = (uint8_t)cb.numbps;
         d.npasses = (uint8_t)std::min(cb.npasses, 255);
         d.cls = (uint8_t)(B.orient == 1 ? 1 : (B.orient == 3 ? 2 : 0));
-        d.mode = tc.reversible ? 0 : 1;
+        d.mode = (uint8_t)((tc.reversible ? 0 : TC_MODE_IRREVERSIBLE) | ((tc.cbsty & 2) ? TC_MODE_RESET : 0) |
+                           ((tc.cbsty & 8) ? TC_MODE_CAUSAL : 0) | ((tc.cbsty & 32) ? TC_MODE_SEGSYM : 0));
         d.delta = B.delta;
         cls[(w <= 32 && h <= 32) ? 0 : 1].push_back((uint32_t)descs.size());
         descs.push_back(d);

@@ -65,7 +65,10 @@ struct ImageState {
 CoefficientCoder::CoefficientCoder(KernelInitInfoBase initInfo)
     : DeviceKernel(KernelInitInfo(initInfo, "t1_decode.hip", "t1_decode")) {}
 
-CoefficientCoder::~CoefficientCoder(void) { tc_buf_free(d_upload_); }
+CoefficientCoder::~CoefficientCoder(void) {
+    tc_buf_free(d_upload_);
+    tc_buf_free(d_scratch_);
+}
 
 void CoefficientCoder::decode_tile(type_tile* tile) {
     if (!tile || !tile->parent_img || !tile->parent_img->impl) throw Error("decode_tile: tile without parsed image");
@@ -101,7 +104,7 @@ void CoefficientCoder::decode_tile(type_tile* tile) {
         for (int c = 0; c < nc; c++) tile->tile_comp[c].stage = 1;
         return;
     }
-    cw.resize(((cw.size() + 3) & ~(size_t)3) + 16, 0xFF);
+    cw.resize(((cw.size() + 15) & ~(size_t)15) + 64, 0xFF);
     size_t cw_bytes = cw.size();
     size_t off_desc = align_up(cw_bytes, 256);
     size_t off_o0 = align_up(off_desc + descs.size() * sizeof(tc_cblk_desc), 256);
@@ -132,7 +135,19 @@ void CoefficientCoder::decode_tile(type_tile* tile) {
             if (per >= c) g = c;
         if (fixed) g = fixed;
         const uint32_t* order = (const uint32_t*)(up + (k == 0 ? off_o0 : off_o1));
-        if (g == 1)
+        int style = 0, max_passes = 0;
+        for (uint32_t i : cls[k]) {
+            style |= descs[i].mode & TC_MODE_STYLE;
+            max_passes = std::max<int>(max_passes, descs[i].npasses);
+        }
+        if (style) {
+            // only the lockstep kernel decodes the code-block styles
+            const size_t sb = tc_t1_lockstep_scratch_bytes(k == 0, 32, n, max_passes);
+            ensure(queue_, S->dev, d_scratch_, sb);
+            check(tc_launch_t1_decode_lockstep(queue_, k == 0, 32, up + off_desc, order, n, max_passes, up, cw_bytes,
+                                               tc_buf_device_ptr(d_scratch_), tc_buf_size(d_scratch_), base),
+                  "t1 launch");
+        } else if (g == 1)
             check(tc_launch_t1_decode_wave(queue_, k == 0, std::min(n, cus * 16), up + off_desc, order, n, up, cw_bytes,
                                            base),
                   "t1 launch");

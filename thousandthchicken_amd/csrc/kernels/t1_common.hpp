@@ -140,7 +140,7 @@ __device__ void convert_block(int lane, const tc_cblk_desc& d, uint32_t* __restr
         uint32_t mag = v & 0x7FFFFFFFu;
         int q = (tlast == 0 && mag != sigonly) ? plast + 1 : plast;
         bool neg = (v >> 31) != 0;
-        if (d.mode == 0) {
+        if ((d.mode & TC_MODE_IRREVERSIBLE) == 0) {
             int32_t iv = (int32_t)(mag + (q > 0 ? (1u << (q - 1)) : 0u));
             *p = (uint32_t)(neg ? -iv : iv);
         } else {

@@ -37,6 +37,11 @@
 // the decode the wave merges the logs into the words and dequantizes them in
 // place (convert_lockstep_block).
 //
+// Code-block styles (descriptor mode bits): RESET re-initialises every context
+// after each pass, CAUSAL drops the row below the stripe from the staged rows
+// (so no context of the stripe sees it), SEGSYM decodes the four uniform
+// symbols that follow each cleanup pass.
+//
 // Codewords are staged through a per-lane ring of 8 dwords in LDS.  The ring
 // is topped up in a wave-uniform step whenever some lane has fewer than four
 // bytes left in it (one symbol consumes at most two), so no global load is in
@@ -331,6 +336,7 @@ __device__ void decode_lockstep(LdsLock<M>& L, const tc_cblk_desc& d, bool activ
     const int w = active ? d.w : 0, h = active ? d.h : 0;
     const int numbps = active ? d.numbps : 0, npasses = active ? d.npasses : 0;
     const uint32_t cls = active ? d.cls : 0;
+    const uint32_t style = active ? (d.mode & TC_MODE_STYLE) : 0u;
     const M VM = (w >= NB) ? ~(M)0 : (((M)1 << w) - 1);
     uint32_t* const out = coef + (active ? d.out_off : 0);
     const uint32_t pitch = active ? d.out_pitch : 0;
@@ -371,7 +377,8 @@ __device__ void decode_lockstep(LdsLock<M>& L, const tc_cblk_desc& d, bool activ
                 X[0] = (y0 > 0 && type != 1) ? carry_x : (M)0;
 #pragma unroll
                 for (int r = 0; r < 5; r++) {
-                    const bool ok = k > 0 && y0 + r < h;
+                    // stripe-causal contexts take the row below the stripe as insignificant
+                    const bool ok = k > 0 && y0 + r < h && !(r == 4 && (style & TC_MODE_CAUSAL));
                     S[r + 1] = ok ? gat(GS, y0 + r) : (M)0;
                     X[r + 1] = (ok && type != 1) ? gat(GX, y0 + r) : (M)0;
                 }
@@ -457,6 +464,19 @@ __device__ void decode_lockstep(LdsLock<M>& L, const tc_cblk_desc& d, bool activ
                 }
             }
         }
+        if (pass_on && type == 2 && (style & TC_MODE_SEGSYM)) {
+            // segmentation symbol: four uniform-context decisions after each
+            // cleanup pass (decoded to keep the MQ state in step, not checked)
+            if (m.rend - (m.base + m.bp) < 16u) lane_refill(m, ring_col, words, nwords);
+            for (int s4 = 0; s4 < 4; s4++) {
+                uint32_t st = L.ctx[18][lane];
+                (void)lane_decode(m, st, L.trans);
+                L.ctx[18][lane] = st;
+            }
+        }
+        if (pass_on && (style & TC_MODE_RESET))
+            for (int c = 0; c < kLockLabels; c++)
+                L.ctx[c][lane] = pack_state(c == 0 ? 4 : (c == 17 ? 3 : (c == 18 ? 46 : 0)), 0);
     }
 }
 
@@ -519,7 +539,7 @@ __device__ void convert_lockstep_block(int lane, int j, const tc_cblk_desc& d, u
             const int qp = (tlast == 0 && mag[q] != sigonly) ? plast + 1 : plast;
             const bool neg = (v[q] >> 31) != 0;
             uint32_t res;
-            if (d.mode == 0) {
+            if ((d.mode & TC_MODE_IRREVERSIBLE) == 0) {
                 const int32_t iv = (int32_t)(mag[q] + (qp > 0 ? (1u << (qp - 1)) : 0u));
                 res = (uint32_t)(neg ? -iv : iv);
             } else {

@@ -585,7 +585,7 @@ __device__ __forceinline__ void decode_block(const BlockArgs d, const uint32_t* 
             if (mag) {
                 const bool neg = ((sc >> y) & 1u) != 0;
                 const int q = (tlast == 0 && mag != sigonly) ? plast + 1 : plast;
-                if (d.mode == 0) {
+                if ((d.mode & TC_MODE_IRREVERSIBLE) == 0) {
                     const int32_t iv = (int32_t)(mag + (q > 0 ? (1u << (q - 1)) : 0u));
                     v = (uint32_t)(neg ? -iv : iv);
                 } else {

@@ -20,9 +20,18 @@ typedef struct tc_cblk_desc {
     uint8_t numbps;      // bit-planes to decode: M_b minus missing MSBs
     uint8_t npasses;     // coding passes present (<= 3*numbps-2)
     uint8_t cls;         // context class: 0 = LL/LH, 1 = HL, 2 = HH
-    uint8_t mode;        // 0 = reversible (int32 out), 1 = irreversible (float out)
+    uint8_t mode;        // TC_MODE_* bits: irreversible (float out, else int32) and the code-block style
     float delta;         // quantization step Delta_b (irreversible)
 } tc_cblk_desc;
+
+// tc_cblk_desc::mode bits.  The style bits (T.800 Table A.19: RESET, CAUSAL,
+// SEGSYM) are decoded by the lockstep tier-1 kernel only; the planner sends
+// every class holding such a block there.
+#define TC_MODE_IRREVERSIBLE 1
+#define TC_MODE_RESET 2   // every context reset after each pass (D.6)
+#define TC_MODE_CAUSAL 4  // stripe-causal contexts (D.7)
+#define TC_MODE_SEGSYM 8  // segmentation symbol after each cleanup pass (D.5)
+#define TC_MODE_STYLE (TC_MODE_RESET | TC_MODE_CAUSAL | TC_MODE_SEGSYM)
 
 // One level of a 2-D inverse DWT over one tile-component: rebuilds the
 // resolution rectangle [u0,u1) x [v0,v1) from its LL (previous level's output

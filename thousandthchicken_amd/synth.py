@@ -139,22 +139,25 @@ def _enc_lib():
             _b.build_encoder()
         L = ctypes.CDLL(path)
         vp, i32 = ctypes.c_void_p, ctypes.c_int
-        L.tce_encode.argtypes = [vp] + [i32] * 10 + [i32, ctypes.c_double, ctypes.c_double, i32,
+        L.tce_encode.argtypes = [vp] + [i32] * 10 + [i32, ctypes.c_double, ctypes.c_double, i32, i32,
                                                       ctypes.POINTER(ctypes.c_void_p),
                                                       ctypes.POINTER(ctypes.c_size_t)]
         L.tce_free.argtypes = [vp]
         L.tce_t1_encode_block.argtypes = [vp, i32, i32, i32, vp, i32, ctypes.POINTER(i32)]
         L.tce_t1_encode_truncated.argtypes = [vp, i32, i32, i32, i32, vp, i32, ctypes.POINTER(i32)]
+        L.tce_t1_encode_truncated_style.argtypes = [vp, i32, i32, i32, i32, i32, vp, i32, ctypes.POINTER(i32)]
         _ENC = L
     return _ENC
 
 
 def encode(img, bits=None, levels=5, cb=64, tile=None, mct=None, lossy=False, bpp=None, base_step=0.5,
-           threads=0):
+           threads=0, cbsty=0):
     """Encodes img [H, W] or [H, W, C] (uint8 / uint16) into a raw codestream.
 
     Lossless: RCT (when mct) + 5/3, every pass kept.  Lossy: ICT + 9/7 with
-    expounded quantization, truncated to about `bpp` bits per pixel."""
+    expounded quantization, truncated to about `bpp` bits per pixel.  cbsty:
+    code-block style bits written to COD and used by tier-1 (RESET 2, CAUSAL 8,
+    PTERM 16, SEGSYM 32)."""
     import ctypes
     a = np.ascontiguousarray(img)
     if a.ndim == 2:
@@ -174,7 +177,8 @@ def encode(img, bits=None, levels=5, cb=64, tile=None, mct=None, lossy=False, bp
     n = ctypes.c_size_t()
     cbe = int(cb).bit_length() - 1
     rc = _enc_lib().tce_encode(a.ctypes.data, w, h, c, bits, levels, cbe, cbe, tw, th, 1 if mct else 0,
-                               1 if lossy else 0, target, base_step, threads, ctypes.byref(out), ctypes.byref(n))
+                               1 if lossy else 0, target, base_step, threads, cbsty, ctypes.byref(out),
+                               ctypes.byref(n))
     if rc:
         raise RuntimeError(f"tce_encode failed ({rc})")
     data = ctypes.string_at(out.value, n.value)
