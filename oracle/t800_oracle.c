@@ -613,8 +613,8 @@ int or_fdwt97(double *a, int w, int h, int x0, int y0, int levels) {
  * TC/codestream.c:17-258 (main header), :260-298 (tile header), :336-470
  * (packets) and TC/codestream_image.c:14-277 (geometry), extended with
  * expounded quantization, COC/QCC, tile-part headers, COM/TLM/PLT skipping,
- * several layers and tile-parts, the five progression orders (one precinct per
- * resolution) and empty packets. */
+ * several layers and tile-parts, precincts (B.6), the five progression orders
+ * (B.12.1) and empty packets. */
 
 typedef struct {
     int nlev, xcb, ycb, cbsty, xform;
@@ -829,16 +829,26 @@ typedef struct {
     int nleaf, nnode;
 } tt_t;
 
+/* the code-blocks of one band inside one precinct: columns i0..i0+w-1 and
+ * rows j0..j0+h-1 of the band's code-block grid, with their tag trees (B.10.2) */
+typedef struct {
+    int i0, j0, w, h;
+    tt_t incl, zbp;
+} prc_t;
+
 typedef struct {
     int orient, x0, y0, x1, y1, xcb, ycb;
     int ncx, ncy;
     cb_t *cb;
-    tt_t incl, zbp;
+    prc_t *prc; /* one per precinct of the resolution */
     int eps, mu, mb;
 } band_t;
 
 typedef struct {
     int x0, y0, x1, y1, nb;
+    int ppx, ppy;   /* precinct exponents PPx, PPy of the resolution */
+    long px0, py0;  /* precinct index of the resolution's origin */
+    int pw, ph;     /* precincts across and down (0 for an empty resolution) */
     band_t b[3];
 } res_t;
 
@@ -1006,11 +1016,18 @@ static int build_tc(const hdr_t *h, int t, int c, tc_t *tc) {
     for (int r = 0; r <= NL; r++) {
         res_t *R = &tc->r[r];
         res_rect(tx0, ty0, tx1, ty1, NL - r, &R->x0, &R->y0, &R->x1, &R->y1);
-        /* one precinct per resolution is supported */
+        /* precinct partition of the resolution, anchored at 0 (B.6, B-16) */
+        R->ppx = tc->cc.ppx[r];
+        R->ppy = tc->cc.ppy[r];
+        if (r > 0 && (R->ppx == 0 || R->ppy == 0)) return OR_ERR_MARKER;
         if (R->x1 > R->x0 && R->y1 > R->y0) {
-            long npx = ceildiv(R->x1, 1L << tc->cc.ppx[r]) - floordiv(R->x0, 1L << tc->cc.ppx[r]);
-            long npy = ceildiv(R->y1, 1L << tc->cc.ppy[r]) - floordiv(R->y0, 1L << tc->cc.ppy[r]);
-            if (npx != 1 || npy != 1) return OR_ERR_UNSUPPORTED;
+            R->px0 = floordiv(R->x0, 1L << R->ppx);
+            R->py0 = floordiv(R->y0, 1L << R->ppy);
+            long npx = ceildiv(R->x1, 1L << R->ppx) - R->px0;
+            long npy = ceildiv(R->y1, 1L << R->ppy) - R->py0;
+            if (npx * npy > (1L << 22)) return OR_ERR_UNSUPPORTED;
+            R->pw = (int)npx;
+            R->ph = (int)npy;
         }
         R->nb = r == 0 ? 1 : 3;
         int nb = r == 0 ? NL : NL - r + 1;
@@ -1055,9 +1072,32 @@ static int build_tc(const hdr_t *h, int t, int c, tc_t *tc) {
                     C->y1 = (cy0 + (1 << ycb)) < B->y1 ? cy0 + (1 << ycb) : B->y1;
                     C->lblock = 3;
                 }
-            int rc = tt_init(&B->incl, B->ncx, B->ncy);
-            if (!rc) rc = tt_init(&B->zbp, B->ncx, B->ncy);
-            if (rc) return rc;
+            /* the band's share of each precinct: its precinct partition is the
+             * resolution's halved (r > 0), and code-blocks never straddle it */
+            int np = R->pw * R->ph;
+            B->prc = (prc_t *)calloc((size_t)(np ? np : 1), sizeof(prc_t));
+            if (!B->prc) return OR_ERR_NOMEM;
+            for (int q = 0; q < np; q++) {
+                prc_t *P = &B->prc[q];
+                if (B->ncx > 0 && B->ncy > 0) {
+                    long X = R->px0 + q % R->pw, Y = R->py0 + q / R->pw;
+                    long i0 = (X << (px - xcb)) - gx0, i1 = ((X + 1) << (px - xcb)) - gx0;
+                    long j0 = (Y << (py - ycb)) - gy0, j1 = ((Y + 1) << (py - ycb)) - gy0;
+                    if (i0 < 0) i0 = 0;
+                    if (j0 < 0) j0 = 0;
+                    if (i1 > B->ncx) i1 = B->ncx;
+                    if (j1 > B->ncy) j1 = B->ncy;
+                    if (i1 > i0 && j1 > j0) {
+                        P->i0 = (int)i0;
+                        P->j0 = (int)j0;
+                        P->w = (int)(i1 - i0);
+                        P->h = (int)(j1 - j0);
+                    }
+                }
+                int rc = tt_init(&P->incl, P->w, P->h);
+                if (!rc) rc = tt_init(&P->zbp, P->w, P->h);
+                if (rc) return rc;
+            }
             /* quantization exponent / mantissa, E.1 (derived: E-5) */
             int idx = r == 0 ? 0 : 3 * (r - 1) + k + 1;
             if (tc->qc.style == 1) {
@@ -1081,15 +1121,20 @@ static void free_tc(tc_t *tc) {
             if (B->cb)
                 for (int i = 0; i < B->ncx * B->ncy; i++) free(B->cb[i].data);
             free(B->cb);
-            tt_free(&B->incl);
-            tt_free(&B->zbp);
+            if (B->prc)
+                for (int q = 0; q < tc->r[r].pw * tc->r[r].ph; q++) {
+                    tt_free(&B->prc[q].incl);
+                    tt_free(&B->prc[q].zbp);
+                }
+            free(B->prc);
         }
 }
 
-/* One packet (B.9, B.10): header then body. */
-static int read_packet(const hdr_t *h, tc_t *tc, int r, int layer, const uint8_t *d, size_t len, size_t *pos) {
+/* One packet of precinct q (B.9, B.10): header then body. */
+static int read_packet(const hdr_t *h, tc_t *tc, int r, int q, int layer, const uint8_t *d, size_t len,
+                       size_t *pos) {
     res_t *R = &tc->r[r];
-    if (R->x1 <= R->x0 || R->y1 <= R->y0) return OR_OK; /* no precinct, no packet */
+    if (q >= R->pw * R->ph) return OR_OK; /* no precinct, no packet */
     if (h->g.sop && *pos + 6 <= len && d[*pos] == 0xFF && d[*pos + 1] == 0x91) *pos += 6;
     bio_t b;
     memset(&b, 0, sizeof b);
@@ -1104,17 +1149,18 @@ static int read_packet(const hdr_t *h, tc_t *tc, int r, int layer, const uint8_t
     if (present) {
         for (int k = 0; k < R->nb; k++) {
             band_t *B = &R->b[k];
-            for (int i = 0; i < B->ncx * B->ncy; i++) {
-                cb_t *C = &B->cb[i];
+            prc_t *P = &B->prc[q];
+            for (int i = 0; i < P->w * P->h; i++) {
+                cb_t *C = &B->cb[(P->j0 + i / P->w) * B->ncx + P->i0 + i % P->w];
                 int inc;
                 if (!C->included)
-                    inc = tt_decode(&B->incl, &b, i, layer + 1);
+                    inc = tt_decode(&P->incl, &b, i, layer + 1);
                 else
                     inc = bio_bit(&b);
                 if (!inc) continue;
                 if (!C->included) {
                     int z = 0;
-                    while (!tt_decode(&B->zbp, &b, i, z + 1)) z++;
+                    while (!tt_decode(&P->zbp, &b, i, z + 1)) z++;
                     C->zbp = z;
                     C->included = 1;
                 }
@@ -1166,47 +1212,98 @@ static int read_packet(const hdr_t *h, tc_t *tc, int r, int layer, const uint8_t
     return OR_OK;
 }
 
-/* Runs every packet of a tile in its progression order (B.12, one precinct). */
+/* Precinct of tile-component tc, resolution r at reference-grid position
+ * (x, y) of a position-driven progression, or -1 when no precinct of that
+ * resolution starts there (B.12.1.3-B.12.1.5: a precinct starts where the
+ * position is a multiple of its size projected onto the reference grid, or
+ * at the tile origin when the resolution's origin is not). */
+static int prc_at(const tc_t *tc, int r, long x, long y) {
+    const res_t *R = &tc->r[r];
+    if (R->pw == 0 || R->ph == 0) return -1;
+    int lv = tc->nres - 1 - r;
+    long sx = 1L << (R->ppx + lv), sy = 1L << (R->ppy + lv);
+    int okx = x % sx == 0 || (x == tc->x0 && (((long)R->x0 << lv) % sx) != 0);
+    int oky = y % sy == 0 || (y == tc->y0 && (((long)R->y0 << lv) % sy) != 0);
+    if (!okx || !oky) return -1;
+    long i = floordiv(ceildiv(x, 1L << lv), 1L << R->ppx) - R->px0;
+    long j = floordiv(ceildiv(y, 1L << lv), 1L << R->ppy) - R->py0;
+    if (i < 0 || j < 0 || i >= R->pw || j >= R->ph) return -1;
+    return (int)(j * R->pw + i);
+}
+
+/* Runs every packet of a tile in its progression order (B.12.1). */
 static int read_tile_packets(const hdr_t *h, tc_t *tcs, const uint8_t *d, size_t len) {
     size_t pos = 0;
     int L = h->g.nlayers, C = h->C, maxr = 0;
     for (int c = 0; c < C; c++)
         if (tcs[c].nres > maxr) maxr = tcs[c].nres;
+    /* position steps: the smallest precinct projected onto the reference grid */
+    long xs = 1L << 40, ys = 1L << 40;
+    for (int c = 0; c < C; c++)
+        for (int r = 0; r < tcs[c].nres; r++) {
+            int lv = tcs[c].nres - 1 - r;
+            long a = 1L << (tcs[c].r[r].ppx + lv), b = 1L << (tcs[c].r[r].ppy + lv);
+            if (a < xs) xs = a;
+            if (b < ys) ys = b;
+        }
+    const long tx0 = tcs[0].x0, ty0 = tcs[0].y0, tx1 = tcs[0].x1, ty1 = tcs[0].y1;
     int rc = OR_OK;
-#define PK(c_, r_, l_)                                                               \
+#define PK(c_, r_, q_, l_)                                                           \
     do {                                                                             \
-        if ((r_) < tcs[c_].nres) {                                                   \
-            if (pos >= len) return OR_OK; /* truncated after whole packets */        \
-            rc = read_packet(h, &tcs[c_], r_, l_, d, len, &pos);                     \
-            if (rc) return rc;                                                       \
-        }                                                                            \
+        if (pos >= len) return OR_OK; /* truncated after whole packets */            \
+        rc = read_packet(h, &tcs[c_], r_, q_, l_, d, len, &pos);                     \
+        if (rc) return rc;                                                           \
     } while (0)
+#define NPRC(c_, r_) ((r_) < tcs[c_].nres ? tcs[c_].r[r_].pw * tcs[c_].r[r_].ph : 0)
     switch (h->g.prog) {
     case 0: /* LRCP */
         for (int l = 0; l < L; l++)
             for (int r = 0; r < maxr; r++)
-                for (int c = 0; c < C; c++) PK(c, r, l);
+                for (int c = 0; c < C; c++)
+                    for (int q = 0; q < NPRC(c, r); q++) PK(c, r, q, l);
         break;
     case 1: /* RLCP */
         for (int r = 0; r < maxr; r++)
             for (int l = 0; l < L; l++)
-                for (int c = 0; c < C; c++) PK(c, r, l);
+                for (int c = 0; c < C; c++)
+                    for (int q = 0; q < NPRC(c, r); q++) PK(c, r, q, l);
         break;
     case 2: /* RPCL */
         for (int r = 0; r < maxr; r++)
-            for (int c = 0; c < C; c++)
-                for (int l = 0; l < L; l++) PK(c, r, l);
+            for (long y = ty0; y < ty1; y += ys - y % ys)
+                for (long x = tx0; x < tx1; x += xs - x % xs)
+                    for (int c = 0; c < C; c++) {
+                        if (r >= tcs[c].nres) continue;
+                        int q = prc_at(&tcs[c], r, x, y);
+                        if (q >= 0)
+                            for (int l = 0; l < L; l++) PK(c, r, q, l);
+                    }
         break;
     case 3: /* PCRL */
+        for (long y = ty0; y < ty1; y += ys - y % ys)
+            for (long x = tx0; x < tx1; x += xs - x % xs)
+                for (int c = 0; c < C; c++)
+                    for (int r = 0; r < tcs[c].nres; r++) {
+                        int q = prc_at(&tcs[c], r, x, y);
+                        if (q >= 0)
+                            for (int l = 0; l < L; l++) PK(c, r, q, l);
+                    }
+        break;
     case 4: /* CPRL */
         for (int c = 0; c < C; c++)
-            for (int r = 0; r < maxr; r++)
-                for (int l = 0; l < L; l++) PK(c, r, l);
+            for (long y = ty0; y < ty1; y += ys - y % ys)
+                for (long x = tx0; x < tx1; x += xs - x % xs)
+                    for (int r = 0; r < tcs[c].nres; r++) {
+                        int q = prc_at(&tcs[c], r, x, y);
+                        if (q >= 0)
+                            for (int l = 0; l < L; l++) PK(c, r, q, l);
+                    }
         break;
     default:
         return OR_ERR_UNSUPPORTED;
     }
 #undef PK
+#undef NPRC
     return rc;
 }
 

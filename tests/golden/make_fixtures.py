@@ -65,6 +65,16 @@ CASES = [
                                                    quality_layers=[40, 20, 8], num_resolutions=4, mct=1)),
     ("ly_gray", (96, 100, 1, 8), 29, dict(irreversible=True, quality_mode="rates", quality_layers=[6],
                                           num_resolutions=4)),
+    # precincts (OpenJPEG halves the given size at each lower resolution)
+    ("ll_prec_pcrl_tiles", (90, 120, 3, 8), 30, dict(num_resolutions=4, codeblock_size=(16, 16),
+                                                      precinct_size=(64, 64), progression="PCRL",
+                                                      tile_size=(64, 64), offset=(3, 5), mct=1)),
+    ("ll_prec_cprl", (70, 90, 3, 8), 31, dict(num_resolutions=4, codeblock_size=(16, 16), precinct_size=(32, 32),
+                                               progression="CPRL", mct=1)),
+    ("ly_prec_rpcl_layers", (100, 100, 3, 8), 32, dict(irreversible=True, quality_mode="rates",
+                                                        quality_layers=[30, 10], num_resolutions=4,
+                                                        codeblock_size=(16, 16), precinct_size=(64, 64),
+                                                        progression="RPCL", mct=1)),
 ]
 
 

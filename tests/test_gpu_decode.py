@@ -20,6 +20,10 @@ LOSSLESS = [
     ("gray_nomct", (130, 257, 1), dict(num_resolutions=5)),
     ("rgb_nomct", (96, 80, 3), dict(num_resolutions=3, mct=0)),
     ("rgb_l0", (40, 30, 3), dict(num_resolutions=1, mct=1)),
+    ("rgb_prec_pcrl", (300, 200, 3), dict(num_resolutions=5, codeblock_size=(32, 32), precinct_size=(128, 128),
+                                          progression="PCRL", mct=1)),
+    ("rgb_prec_cprl_tiles", (257, 300, 3), dict(num_resolutions=4, codeblock_size=(16, 16), precinct_size=(64, 64),
+                                                progression="CPRL", tile_size=(128, 128), offset=(5, 3), mct=1)),
 ]
 
 LOSSY = [
@@ -27,6 +31,8 @@ LOSSY = [
     ("lossy_odd", (255, 301, 3), dict(quality_layers=[12], num_resolutions=5, mct=1)),
     ("lossy_layers", (200, 220, 3), dict(quality_layers=[40, 20, 10], num_resolutions=5, mct=1)),
     ("lossy_gray", (190, 120, 1), dict(quality_layers=[8], num_resolutions=4)),
+    ("lossy_prec_rlcp", (240, 260, 3), dict(quality_layers=[20, 8], num_resolutions=5, codeblock_size=(32, 32),
+                                            precinct_size=(128, 128), progression="RLCP", mct=1)),
 ]
 
 

@@ -2,6 +2,7 @@
 #include "codestream.hpp"
 
 #include <algorithm>
+#include <tuple>
 #include <cmath>
 #include <cstring>
 
@@ -178,9 +179,22 @@ class TagTree {
     std::vector<uint8_t> known_;
 };
 
-struct BandState {
+// The code-blocks of one band inside one precinct: columns i0..i0+w-1 and rows
+// j0..j0+h-1 of the band's code-block grid, with their tag trees (B.10.2).
+struct PrecBand {
+    int i0 = 0, j0 = 0, w = 0, h = 0;
     TagTree incl, zbp;
 };
+
+// Precinct partition of one resolution, anchored at 0 (B.6), and each band's
+// share of every precinct.
+struct ResState {
+    int ppx = 15, ppy = 15;
+    int64_t px0 = 0, py0 = 0;  // precinct index of the resolution's origin
+    int pw = 0, ph = 0;        // precincts across and down (0: empty resolution)
+    std::vector<PrecBand> prc[3];
+};
+using TileState = std::vector<std::vector<ResState>>;  // [component][resolution]
 
 Status read_cs_params(Reader& r, CodingStyle& c, bool user_prec) {
     c.nlev = (int)r.u8();
@@ -398,7 +412,7 @@ Status parse_main_header(Reader& r, Image& img, Params& P, std::string& err) {
 }
 
 // Geometry of one tile and its code-blocks (Annex B.3-B.7).
-Status build_tile(Image& img, int t, const Params& P, std::vector<std::vector<BandState>>& bstate, std::string& err) {
+Status build_tile(Image& img, int t, const Params& P, TileState& bstate, std::string& err) {
     Tile& T = img.tiles[t];
     int p = t % img.ntx, q = t / img.ntx;
     T.x0 = std::max(img.tx0 + p * img.tw, img.x0);
@@ -432,7 +446,7 @@ Status build_tile(Image& img, int t, const Params& P, std::vector<std::vector<Ba
             return Status::BadMarker;
         }
         tc.res.assign(C.nlev + 1, Resolution());
-        bstate[c].assign((C.nlev + 1) * 3, BandState());
+        bstate[c].assign(C.nlev + 1, ResState());
         for (int r = 0; r <= C.nlev; r++) {
             Resolution& R = tc.res[r];
             int64_t s = (int64_t)1 << (C.nlev - r);
@@ -440,12 +454,24 @@ Status build_tile(Image& img, int t, const Params& P, std::vector<std::vector<Ba
             R.y0 = (int)ceil_div(tc.y0, s);
             R.x1 = (int)ceil_div(tc.x1, s);
             R.y1 = (int)ceil_div(tc.y1, s);
+            ResState& RS = bstate[c][r];
+            RS.ppx = C.ppx[r];
+            RS.ppy = C.ppy[r];
+            if (r > 0 && (RS.ppx == 0 || RS.ppy == 0)) {
+                err = "precinct exponent 0 above the lowest resolution";
+                return Status::BadMarker;
+            }
             if (R.x1 > R.x0 && R.y1 > R.y0) {
-                int64_t ppw = (int64_t)1 << C.ppx[r], pph = (int64_t)1 << C.ppy[r];
-                if (ceil_div(R.x1, ppw) - floor_div(R.x0, ppw) != 1 || ceil_div(R.y1, pph) - floor_div(R.y0, pph) != 1) {
-                    err = "more than one precinct per resolution is not supported";
-                    return Status::Unsupported;
+                RS.px0 = floor_div(R.x0, (int64_t)1 << RS.ppx);
+                RS.py0 = floor_div(R.y0, (int64_t)1 << RS.ppy);
+                int64_t npx = ceil_div(R.x1, (int64_t)1 << RS.ppx) - RS.px0;
+                int64_t npy = ceil_div(R.y1, (int64_t)1 << RS.ppy) - RS.py0;
+                if (npx * npy > ((int64_t)1 << 22)) {
+                    err = "more than 2^22 precincts in a resolution";
+                    return Status::Capacity;
                 }
+                RS.pw = (int)npx;
+                RS.ph = (int)npy;
             }
             R.nbands = r == 0 ? 1 : 3;
             int nb = r == 0 ? C.nlev : C.nlev - r + 1;
@@ -499,8 +525,27 @@ Status build_tile(Image& img, int t, const Params& P, std::vector<std::vector<Ba
                             cb.y1 = std::min<int64_t>(B.y1, (gy0 + j + 1) * ch);
                             img.cblks.push_back(cb);
                         }
-                    bstate[c][r * 3 + k].incl.init(B.ncx, B.ncy);
-                    bstate[c][r * 3 + k].zbp.init(B.ncx, B.ncy);
+                    // each precinct's code-blocks: the band's precinct partition is
+                    // the resolution's halved above r = 0, and no code-block
+                    // straddles it (B.7: xcb' <= the band's precinct exponent)
+                    const int sx = (r == 0 ? RS.ppx : RS.ppx - 1) - B.xcb;
+                    const int sy = (r == 0 ? RS.ppy : RS.ppy - 1) - B.ycb;
+                    RS.prc[k].resize((size_t)RS.pw * RS.ph);
+                    for (int q = 0; q < RS.pw * RS.ph; q++) {
+                        int64_t X = RS.px0 + q % RS.pw, Y = RS.py0 + q / RS.pw;
+                        int64_t i0 = std::max<int64_t>(0, (X << sx) - gx0);
+                        int64_t i1 = std::min<int64_t>(B.ncx, ((X + 1) << sx) - gx0);
+                        int64_t j0 = std::max<int64_t>(0, (Y << sy) - gy0);
+                        int64_t j1 = std::min<int64_t>(B.ncy, ((Y + 1) << sy) - gy0);
+                        if (i1 <= i0 || j1 <= j0) continue;
+                        PrecBand& PB = RS.prc[k][q];
+                        PB.i0 = (int)i0;
+                        PB.j0 = (int)j0;
+                        PB.w = (int)(i1 - i0);
+                        PB.h = (int)(j1 - j0);
+                        PB.incl.init(PB.w, PB.h);
+                        PB.zbp.init(PB.w, PB.h);
+                    }
                 } else {
                     B.ncx = B.ncy = 0;
                 }
@@ -529,14 +574,15 @@ int ilog2(int v) {
     return l;
 }
 
-// Reads one packet of (component c, resolution r, layer l) at *pos within the
-// tile data ranges.
-Status read_packet(Image& img, int t, int c, int r, int layer, const Params& P,
-                   std::vector<std::vector<BandState>>& bstate, const std::vector<Range>& ranges, size_t& ri,
+// Reads the packet of (component c, resolution r, precinct q, layer l) at
+// *pos within the tile data ranges.
+Status read_packet(Image& img, int t, int c, int r, int q, int layer, const Params& P,
+                   TileState& bstate, const std::vector<Range>& ranges, size_t& ri,
                    size_t& pos, std::string& err) {
     const TileComp& tc = img.tiles[t].comps[c];
     const Resolution& R = tc.res[r];
-    if (R.x1 <= R.x0 || R.y1 <= R.y0) return Status::Ok;  // empty resolution: no precinct
+    ResState& RS = bstate[c][r];
+    if (q >= RS.pw * RS.ph) return Status::Ok;  // empty resolution: no precinct
     while (ri < ranges.size() && pos >= ranges[ri].off + ranges[ri].len) {
         ++ri;
         if (ri < ranges.size()) pos = ranges[ri].off;
@@ -556,10 +602,11 @@ Status read_packet(Image& img, int t, int c, int r, int layer, const Params& P,
     if (br.bit()) {
         for (int k = 0; k < R.nbands; k++) {
             const Band& B = R.band[k];
-            BandState& S = bstate[c][r * 3 + k];
-            for (int j = 0; j < B.ncy; j++)
-                for (int i = 0; i < B.ncx; i++) {
-                    uint32_t id = (uint32_t)(B.first_cblk + j * B.ncx + i);
+            if (RS.prc[k].empty()) continue;
+            PrecBand& S = RS.prc[k][q];
+            for (int j = 0; j < S.h; j++)
+                for (int i = 0; i < S.w; i++) {
+                    uint32_t id = (uint32_t)(B.first_cblk + (S.j0 + j) * B.ncx + S.i0 + i);
                     CodeBlock& cb = img.cblks[id];
                     bool inc = cb.included ? br.bit() != 0 : S.incl.decode(br, i, j, layer + 1);
                     if (br.bad()) break;
@@ -621,43 +668,79 @@ Status read_packet(Image& img, int t, int c, int r, int layer, const Params& P,
     return Status::Ok;
 }
 
-Status read_packets(Image& img, int t, const Params& P, std::vector<std::vector<BandState>>& bstate,
-                    const std::vector<Range>& ranges, std::string& err) {
+Status read_packets(Image& img, int t, const Params& P, TileState& bstate, const std::vector<Range>& ranges,
+                    std::string& err) {
     if (ranges.empty()) return Status::Ok;
     size_t ri = 0, pos = ranges[0].off;
+    const Tile& T = img.tiles[t];
     int C = img.ncomp, L = P.nlayers, maxr = 0;
-    for (int c = 0; c < C; c++) maxr = std::max(maxr, img.tiles[t].comps[c].nlev + 1);
-    auto pk = [&](int c, int r, int l) -> Status {
-        if (r > img.tiles[t].comps[c].nlev) return Status::Ok;
-        return read_packet(img, t, c, r, l, P, bstate, ranges, ri, pos, err);
+    for (int c = 0; c < C; c++) maxr = std::max(maxr, T.comps[c].nlev + 1);
+    auto pk = [&](int c, int r, int q) -> Status {
+        for (int l = 0; l < L; l++) {
+            Status s = read_packet(img, t, c, r, q, l, P, bstate, ranges, ri, pos, err);
+            if (s != Status::Ok) return s;
+        }
+        return Status::Ok;
+    };
+    auto pk_layer = [&](int c, int r, int l) -> Status {
+        if (r > T.comps[c].nlev) return Status::Ok;
+        const ResState& RS = bstate[c][r];
+        for (int q = 0; q < RS.pw * RS.ph; q++) {
+            Status s = read_packet(img, t, c, r, q, l, P, bstate, ranges, ri, pos, err);
+            if (s != Status::Ok) return s;
+        }
+        return Status::Ok;
     };
     Status s = Status::Ok;
-    switch (P.prog) {
-    case 0:  // LRCP
+    if (P.prog == 0) {  // LRCP
         for (int l = 0; l < L; l++)
             for (int r = 0; r < maxr; r++)
                 for (int c = 0; c < C; c++)
-                    if ((s = pk(c, r, l)) != Status::Ok) return s;
-        break;
-    case 1:  // RLCP
+                    if ((s = pk_layer(c, r, l)) != Status::Ok) return s;
+        return Status::Ok;
+    }
+    if (P.prog == 1) {  // RLCP
         for (int r = 0; r < maxr; r++)
             for (int l = 0; l < L; l++)
                 for (int c = 0; c < C; c++)
-                    if ((s = pk(c, r, l)) != Status::Ok) return s;
-        break;
-    case 2:  // RPCL (one precinct per resolution)
-        for (int r = 0; r < maxr; r++)
-            for (int c = 0; c < C; c++)
-                for (int l = 0; l < L; l++)
-                    if ((s = pk(c, r, l)) != Status::Ok) return s;
-        break;
-    default:  // PCRL, CPRL (one precinct per resolution)
-        for (int c = 0; c < C; c++)
-            for (int r = 0; r < maxr; r++)
-                for (int l = 0; l < L; l++)
-                    if ((s = pk(c, r, l)) != Status::Ok) return s;
-        break;
+                    if ((s = pk_layer(c, r, l)) != Status::Ok) return s;
+        return Status::Ok;
     }
+    // RPCL, PCRL, CPRL (B.12.1.3-B.12.1.5) step through reference-grid
+    // positions and emit a precinct where it starts: at a multiple of its size
+    // projected onto the grid, or at the tile origin for the first precinct of
+    // a resolution whose origin is not such a multiple.  Every precinct starts
+    // at exactly one position, so sorting the precincts by that position
+    // gives the same sequence.
+    struct Start {
+        int64_t y, x;
+        int c, r, q;
+    };
+    std::vector<Start> seq;
+    for (int c = 0; c < C; c++) {
+        const TileComp& tc = T.comps[c];
+        for (int r = 0; r <= tc.nlev; r++) {
+            const ResState& RS = bstate[c][r];
+            const Resolution& R = tc.res[r];
+            const int lv = tc.nlev - r;
+            const int64_t sx = (int64_t)1 << (RS.ppx + lv), sy = (int64_t)1 << (RS.ppy + lv);
+            const bool ux = (((int64_t)R.x0 << lv) % sx) != 0, uy = (((int64_t)R.y0 << lv) % sy) != 0;
+            for (int q = 0; q < RS.pw * RS.ph; q++) {
+                int i = q % RS.pw, j = q / RS.pw;
+                int64_t x = (i == 0 && ux) ? tc.x0 : (RS.px0 + i) * sx;
+                int64_t y = (j == 0 && uy) ? tc.y0 : (RS.py0 + j) * sy;
+                seq.push_back(Start{y, x, c, r, q});
+            }
+        }
+    }
+    auto key = [&](const Start& a) {
+        if (P.prog == 2) return std::make_tuple((int64_t)a.r, a.y, a.x, (int64_t)a.c);  // RPCL
+        if (P.prog == 3) return std::make_tuple(a.y, a.x, (int64_t)a.c, (int64_t)a.r);  // PCRL
+        return std::make_tuple((int64_t)a.c, a.y, a.x, (int64_t)a.r);                   // CPRL
+    };
+    std::sort(seq.begin(), seq.end(), [&](const Start& a, const Start& b) { return key(a) < key(b); });
+    for (const Start& e : seq)
+        if ((s = pk(e.c, e.r, e.q)) != Status::Ok) return s;
     return Status::Ok;
 }
 
@@ -678,7 +761,7 @@ Status parse_impl(const uint8_t* data, size_t len, Image& img, std::string& err,
         if (s == Status::Ok) {
             // geometry of tile 0 only, for probing
             img.tiles.resize((size_t)img.ntx * img.nty);
-            std::vector<std::vector<BandState>> bs;
+            TileState bs;
             s = build_tile(img, 0, main, bs, err);
             img.cblks.clear();
         }
@@ -735,7 +818,7 @@ Status parse_impl(const uint8_t* data, size_t len, Image& img, std::string& err,
     }
     for (int t = 0; t < nt; t++) {
         const Params& P = seen[t] ? tp[t] : main;
-        std::vector<std::vector<BandState>> bs;
+        TileState bs;
         s = build_tile(img, t, P, bs, err);
         if (s != Status::Ok) return s;
         s = read_packets(img, t, P, bs, ranges[t], err);
