@@ -630,6 +630,13 @@ typedef struct {
     int prog, nlayers, mct, sop, eph;
 } gcs_t; /* COD global part */
 
+/* One progression order change (POC, A.6.6): resolutions rs..re-1,
+ * components cs..ce-1, layers below le, in order prog. */
+typedef struct {
+    int rs, cs, le, re, ce, prog;
+} poc_t;
+#define MAXPOC 64
+
 typedef struct {
     int X0, Y0, X1, Y1, XT0, YT0, XT, YT, C;
     int prec[MAXC], sgnd[MAXC];
@@ -637,6 +644,8 @@ typedef struct {
     ccs_t cc[MAXC];
     qcs_t qc[MAXC];
     int ntx, nty;
+    int npoc, poc_tile; /* POC entries; poc_tile: taken from the tile's own headers */
+    poc_t poc[MAXPOC];
 } hdr_t;
 
 typedef struct {
@@ -749,6 +758,27 @@ static int header_segment(rd_t *r, int mk, hdr_t *h, int *coc_set, int *qcc_set,
         qcc_set[ci] = 1;
         break;
     }
+    case 0xFF5F: { /* POC: a tile's own entries replace the main header's */
+        int wide = h->C >= 257, n = (L - 2) / (wide ? 9 : 7);
+        if (n < 1 || (L - 2) % (wide ? 9 : 7)) return OR_ERR_MARKER;
+        if (!main_hdr && !h->poc_tile) {
+            h->npoc = 0;
+            h->poc_tile = 1;
+        }
+        for (int i = 0; i < n; i++) {
+            if (h->npoc >= MAXPOC) return OR_ERR_UNSUPPORTED;
+            poc_t *e = &h->poc[h->npoc++];
+            e->rs = rd8(r);
+            e->cs = wide ? rd16(r) : rd8(r);
+            e->le = rd16(r);
+            e->re = rd8(r);
+            e->ce = wide ? rd16(r) : rd8(r);
+            e->prog = rd8(r);
+            if (e->ce == 0) e->ce = wide ? 16384 : 256;
+            if (e->prog > 4) return OR_ERR_UNSUPPORTED;
+        }
+        break;
+    }
     case 0xFF64: /* COM */
     case 0xFF55: /* TLM */
     case 0xFF57: /* PLM */
@@ -756,8 +786,7 @@ static int header_segment(rd_t *r, int mk, hdr_t *h, int *coc_set, int *qcc_set,
     case 0xFF63: /* CRG */
         break;
     default:
-        (void)main_hdr;
-        return OR_ERR_UNSUPPORTED; /* RGN, POC, PPM, PPT, unknown */
+        return OR_ERR_UNSUPPORTED; /* RGN, PPM, PPT, unknown */
     }
     if (rc) return rc;
     r->pos = end;
@@ -849,6 +878,7 @@ typedef struct {
     int ppx, ppy;   /* precinct exponents PPx, PPy of the resolution */
     long px0, py0;  /* precinct index of the resolution's origin */
     int pw, ph;     /* precincts across and down (0 for an empty resolution) */
+    int *nextl;     /* per precinct: the layer of its next packet */
     band_t b[3];
 } res_t;
 
@@ -1029,6 +1059,8 @@ static int build_tc(const hdr_t *h, int t, int c, tc_t *tc) {
             R->pw = (int)npx;
             R->ph = (int)npy;
         }
+        R->nextl = (int *)calloc((size_t)(R->pw * R->ph > 0 ? R->pw * R->ph : 1), sizeof(int));
+        if (!R->nextl) return OR_ERR_NOMEM;
         R->nb = r == 0 ? 1 : 3;
         int nb = r == 0 ? NL : NL - r + 1;
         for (int k = 0; k < R->nb; k++) {
@@ -1115,6 +1147,7 @@ static int build_tc(const hdr_t *h, int t, int c, tc_t *tc) {
 }
 
 static void free_tc(tc_t *tc) {
+    for (int r = 0; r < tc->nres; r++) free(tc->r[r].nextl);
     for (int r = 0; r < tc->nres; r++)
         for (int k = 0; k < tc->r[r].nb; k++) {
             band_t *B = &tc->r[r].b[k];
@@ -1231,80 +1264,110 @@ static int prc_at(const tc_t *tc, int r, long x, long y) {
     return (int)(j * R->pw + i);
 }
 
-/* Runs every packet of a tile in its progression order (B.12.1). */
+#define OR_END 1000 /* internal: the tile's data ended at a packet boundary */
+
+/* Reads packet (c, r, q, l) unless that precinct has no packets or its
+ * packet of layer l came earlier (B.12.2: a packet appears once, and the
+ * packets of a precinct come in layer order). */
+static int emit(const hdr_t *h, tc_t *tc, int r, int q, int l, const uint8_t *d, size_t len, size_t *pos) {
+    if (r >= tc->nres) return OR_OK;
+    res_t *R = &tc->r[r];
+    if (q < 0 || q >= R->pw * R->ph || R->nextl[q] != l) return OR_OK;
+    if (*pos >= len) return OR_END;
+    R->nextl[q]++;
+    return read_packet(h, tc, r, q, l, d, len, pos);
+}
+
+/* The packets of one progression volume (B.12.1) in order v->prog.  xs, ys:
+ * position steps of the position-driven orders, no larger than any precinct
+ * projected onto the reference grid. */
+static int read_volume(const hdr_t *h, tc_t *tcs, const poc_t *v, long xs, long ys, const uint8_t *d, size_t len,
+                       size_t *pos) {
+    const long tx0 = tcs[0].x0, ty0 = tcs[0].y0, tx1 = tcs[0].x1, ty1 = tcs[0].y1;
+    const int cs = v->cs, ce = v->ce < h->C ? v->ce : h->C, rs = v->rs, re = v->re;
+    const int le = v->le < h->g.nlayers ? v->le : h->g.nlayers;
+    int rc;
+#define EM(c_, r_, q_, l_)                                          \
+    do {                                                            \
+        rc = emit(h, &tcs[c_], r_, q_, l_, d, len, pos);            \
+        if (rc) return rc;                                          \
+    } while (0)
+#define NPRC(c_, r_) ((r_) < tcs[c_].nres ? tcs[c_].r[r_].pw * tcs[c_].r[r_].ph : 0)
+    switch (v->prog) {
+    case 0: /* LRCP */
+        for (int l = 0; l < le; l++)
+            for (int r = rs; r < re; r++)
+                for (int c = cs; c < ce; c++)
+                    for (int q = 0; q < NPRC(c, r); q++) EM(c, r, q, l);
+        break;
+    case 1: /* RLCP */
+        for (int r = rs; r < re; r++)
+            for (int l = 0; l < le; l++)
+                for (int c = cs; c < ce; c++)
+                    for (int q = 0; q < NPRC(c, r); q++) EM(c, r, q, l);
+        break;
+    case 2: /* RPCL */
+        for (int r = rs; r < re; r++)
+            for (long y = ty0; y < ty1; y += ys - y % ys)
+                for (long x = tx0; x < tx1; x += xs - x % xs)
+                    for (int c = cs; c < ce; c++) {
+                        if (r >= tcs[c].nres) continue;
+                        int q = prc_at(&tcs[c], r, x, y);
+                        if (q >= 0)
+                            for (int l = 0; l < le; l++) EM(c, r, q, l);
+                    }
+        break;
+    case 3: /* PCRL */
+        for (long y = ty0; y < ty1; y += ys - y % ys)
+            for (long x = tx0; x < tx1; x += xs - x % xs)
+                for (int c = cs; c < ce; c++)
+                    for (int r = rs; r < re && r < tcs[c].nres; r++) {
+                        int q = prc_at(&tcs[c], r, x, y);
+                        if (q >= 0)
+                            for (int l = 0; l < le; l++) EM(c, r, q, l);
+                    }
+        break;
+    case 4: /* CPRL */
+        for (int c = cs; c < ce; c++)
+            for (long y = ty0; y < ty1; y += ys - y % ys)
+                for (long x = tx0; x < tx1; x += xs - x % xs)
+                    for (int r = rs; r < re && r < tcs[c].nres; r++) {
+                        int q = prc_at(&tcs[c], r, x, y);
+                        if (q >= 0)
+                            for (int l = 0; l < le; l++) EM(c, r, q, l);
+                    }
+        break;
+    default:
+        return OR_ERR_UNSUPPORTED;
+    }
+#undef EM
+#undef NPRC
+    return OR_OK;
+}
+
+/* Runs every packet of a tile: the POC volumes in order, then the COD
+ * progression over whatever they left (B.12). */
 static int read_tile_packets(const hdr_t *h, tc_t *tcs, const uint8_t *d, size_t len) {
     size_t pos = 0;
-    int L = h->g.nlayers, C = h->C, maxr = 0;
-    for (int c = 0; c < C; c++)
+    int maxr = 0;
+    for (int c = 0; c < h->C; c++)
         if (tcs[c].nres > maxr) maxr = tcs[c].nres;
     /* position steps: the smallest precinct projected onto the reference grid */
     long xs = 1L << 40, ys = 1L << 40;
-    for (int c = 0; c < C; c++)
+    for (int c = 0; c < h->C; c++)
         for (int r = 0; r < tcs[c].nres; r++) {
             int lv = tcs[c].nres - 1 - r;
             long a = 1L << (tcs[c].r[r].ppx + lv), b = 1L << (tcs[c].r[r].ppy + lv);
             if (a < xs) xs = a;
             if (b < ys) ys = b;
         }
-    const long tx0 = tcs[0].x0, ty0 = tcs[0].y0, tx1 = tcs[0].x1, ty1 = tcs[0].y1;
     int rc = OR_OK;
-#define PK(c_, r_, q_, l_)                                                           \
-    do {                                                                             \
-        if (pos >= len) return OR_OK; /* truncated after whole packets */            \
-        rc = read_packet(h, &tcs[c_], r_, q_, l_, d, len, &pos);                     \
-        if (rc) return rc;                                                           \
-    } while (0)
-#define NPRC(c_, r_) ((r_) < tcs[c_].nres ? tcs[c_].r[r_].pw * tcs[c_].r[r_].ph : 0)
-    switch (h->g.prog) {
-    case 0: /* LRCP */
-        for (int l = 0; l < L; l++)
-            for (int r = 0; r < maxr; r++)
-                for (int c = 0; c < C; c++)
-                    for (int q = 0; q < NPRC(c, r); q++) PK(c, r, q, l);
-        break;
-    case 1: /* RLCP */
-        for (int r = 0; r < maxr; r++)
-            for (int l = 0; l < L; l++)
-                for (int c = 0; c < C; c++)
-                    for (int q = 0; q < NPRC(c, r); q++) PK(c, r, q, l);
-        break;
-    case 2: /* RPCL */
-        for (int r = 0; r < maxr; r++)
-            for (long y = ty0; y < ty1; y += ys - y % ys)
-                for (long x = tx0; x < tx1; x += xs - x % xs)
-                    for (int c = 0; c < C; c++) {
-                        if (r >= tcs[c].nres) continue;
-                        int q = prc_at(&tcs[c], r, x, y);
-                        if (q >= 0)
-                            for (int l = 0; l < L; l++) PK(c, r, q, l);
-                    }
-        break;
-    case 3: /* PCRL */
-        for (long y = ty0; y < ty1; y += ys - y % ys)
-            for (long x = tx0; x < tx1; x += xs - x % xs)
-                for (int c = 0; c < C; c++)
-                    for (int r = 0; r < tcs[c].nres; r++) {
-                        int q = prc_at(&tcs[c], r, x, y);
-                        if (q >= 0)
-                            for (int l = 0; l < L; l++) PK(c, r, q, l);
-                    }
-        break;
-    case 4: /* CPRL */
-        for (int c = 0; c < C; c++)
-            for (long y = ty0; y < ty1; y += ys - y % ys)
-                for (long x = tx0; x < tx1; x += xs - x % xs)
-                    for (int r = 0; r < tcs[c].nres; r++) {
-                        int q = prc_at(&tcs[c], r, x, y);
-                        if (q >= 0)
-                            for (int l = 0; l < L; l++) PK(c, r, q, l);
-                    }
-        break;
-    default:
-        return OR_ERR_UNSUPPORTED;
+    for (int i = 0; i < h->npoc && !rc; i++) rc = read_volume(h, tcs, &h->poc[i], xs, ys, d, len, &pos);
+    if (!rc) {
+        poc_t all = {0, 0, h->g.nlayers, maxr, h->C, h->g.prog};
+        rc = read_volume(h, tcs, &all, xs, ys, d, len, &pos);
     }
-#undef PK
-#undef NPRC
-    return rc;
+    return rc == OR_END ? OR_OK : rc;
 }
 
 /* Collects the data of every tile-part of tile t into one buffer and applies

@@ -75,6 +75,10 @@ CASES = [
                                                         quality_layers=[30, 10], num_resolutions=4,
                                                         codeblock_size=(16, 16), precinct_size=(64, 64),
                                                         progression="RPCL", mct=1)),
+    # Digital Cinema profiles: CPRL with precincts, tile-parts per resolution
+    # range and TLM; the 4K profile writes a POC marker in the tile header
+    ("ly_cinema4k_poc", (200, 256, 3, 8), 33, dict(cinema_mode="cinema4k-24")),
+    ("ly_cinema2k", (200, 256, 3, 8), 34, dict(cinema_mode="cinema2k-24")),
 ]
 
 

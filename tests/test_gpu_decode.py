@@ -67,3 +67,19 @@ def test_lossy_within_one(gpu_decoder, name, shape, kw):
     d = np.abs(out - ref)
     assert d.max() <= 1, f"max |gpu - oracle| = {d.max()}"
     assert (d > 0).mean() < 0.01
+
+
+@pytest.mark.parametrize("mode,shape", [("cinema2k-24", (1080, 2048, 3)), ("cinema4k-24", (2160, 4096, 3))])
+def test_cinema_profiles_within_one(gpu_decoder, mode, shape):
+    """OpenJPEG's Digital Cinema profiles at full frame size: CPRL with
+    precincts, tile-parts, TLM and (4K) a tile-header POC."""
+    if imaging.pillow_j2k() is None:
+        pytest.skip("OpenJPEG not available to make the codestream")
+    src = _img(shape, 13)
+    cs = imaging.encode_openjpeg(src, cinema_mode=mode)
+    print(f"case {mode} {shape} {len(cs)} bytes", flush=True)
+    out = gpu_decoder.decode(cs).transpose(2, 0, 1).astype(np.int32)
+    ref = oracle.decode(cs)
+    d = np.abs(out - ref)
+    assert d.max() <= 1, f"max |gpu - oracle| = {d.max()}"
+    assert (d > 0).mean() < 0.01

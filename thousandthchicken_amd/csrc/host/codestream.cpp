@@ -26,11 +26,19 @@ struct Quant {
     std::vector<uint16_t> v;  // style 0: exponent only (eps); 1/2: eps<<11|mu
 };
 
+// One progression order change (POC, A.6.6): resolutions rs..re-1,
+// components cs..ce-1, layers below le, in order prog.
+struct Poc {
+    int rs, cs, le, re, ce, prog;
+};
+
 struct Params {
     int prog = 0, nlayers = 1, mct = 0;
     bool sop = false, eph = false;
     std::vector<CodingStyle> cs;
     std::vector<Quant> q;
+    std::vector<Poc> poc;
+    bool poc_tile = false;  // poc came from the tile's own headers
 };
 
 struct Range {
@@ -193,6 +201,7 @@ struct ResState {
     int64_t px0 = 0, py0 = 0;  // precinct index of the resolution's origin
     int pw = 0, ph = 0;        // precincts across and down (0: empty resolution)
     std::vector<PrecBand> prc[3];
+    std::vector<int> nextl;    // per precinct: the layer of its next packet
 };
 using TileState = std::vector<std::vector<ResState>>;  // [component][resolution]
 
@@ -234,7 +243,7 @@ Status read_quant(Reader& r, size_t end, Quant& q) {
 
 // One parameter marker segment (COD/COC/QCD/QCC) or a skippable one.
 Status read_segment(Reader& r, uint32_t marker, int ncomp, Params& P, std::vector<uint8_t>& coc_set,
-                    std::vector<uint8_t>& qcc_set, std::string& err) {
+                    std::vector<uint8_t>& qcc_set, std::string& err, bool tile_hdr = false) {
     size_t lpos = r.pos();
     uint32_t L = r.u16();
     if (!r.ok() || L < 2 || lpos + L > r.size()) return Status::Truncated;
@@ -279,6 +288,35 @@ Status read_segment(Reader& r, uint32_t marker, int ncomp, Params& P, std::vecto
         qcc_set[ci] = 1;
         break;
     }
+    case 0xFF5F: {  // POC: a tile's own entries replace the main header's
+        const bool wide = ncomp >= 257;
+        const uint32_t esz = wide ? 9 : 7;
+        if (L - 2 < esz || (L - 2) % esz) return Status::BadMarker;
+        if (tile_hdr && !P.poc_tile) {
+            P.poc.clear();
+            P.poc_tile = true;
+        }
+        for (uint32_t i = 0; i < (L - 2) / esz; i++) {
+            Poc e;
+            e.rs = (int)r.u8();
+            e.cs = (int)(wide ? r.u16() : r.u8());
+            e.le = (int)r.u16();
+            e.re = (int)r.u8();
+            e.ce = (int)(wide ? r.u16() : r.u8());
+            e.prog = (int)r.u8();
+            if (e.ce == 0) e.ce = wide ? 16384 : 256;
+            if (e.prog > 4) {
+                err = "bad progression order in POC";
+                return Status::Unsupported;
+            }
+            if (P.poc.size() >= 256) {
+                err = "more than 256 progression order changes";
+                return Status::Unsupported;
+            }
+            P.poc.push_back(e);
+        }
+        break;
+    }
     case 0xFF64:  // COM
     case 0xFF55:  // TLM
     case 0xFF57:  // PLM
@@ -289,7 +327,7 @@ Status read_segment(Reader& r, uint32_t marker, int ncomp, Params& P, std::vecto
         char b[64];
         snprintf(b, sizeof b, "unsupported marker 0x%04X", marker);
         err = b;
-        return Status::Unsupported;  // RGN, POC, PPM, PPT and unknown markers
+        return Status::Unsupported;  // RGN, PPM, PPT and unknown markers
     }
     }
     if (st != Status::Ok) return st;
@@ -472,6 +510,7 @@ Status build_tile(Image& img, int t, const Params& P, TileState& bstate, std::st
                 }
                 RS.pw = (int)npx;
                 RS.ph = (int)npy;
+                RS.nextl.assign((size_t)RS.pw * RS.ph, 0);
             }
             R.nbands = r == 0 ? 1 : 3;
             int nb = r == 0 ? C.nlev : C.nlev - r + 1;
@@ -673,75 +712,86 @@ Status read_packets(Image& img, int t, const Params& P, TileState& bstate, const
     if (ranges.empty()) return Status::Ok;
     size_t ri = 0, pos = ranges[0].off;
     const Tile& T = img.tiles[t];
-    int C = img.ncomp, L = P.nlayers, maxr = 0;
+    int C = img.ncomp, maxr = 0;
     for (int c = 0; c < C; c++) maxr = std::max(maxr, T.comps[c].nlev + 1);
-    auto pk = [&](int c, int r, int q) -> Status {
-        for (int l = 0; l < L; l++) {
-            Status s = read_packet(img, t, c, r, q, l, P, bstate, ranges, ri, pos, err);
-            if (s != Status::Ok) return s;
-        }
-        return Status::Ok;
-    };
-    auto pk_layer = [&](int c, int r, int l) -> Status {
+    // packet (c, r, q, l) unless the precinct has none or its layer-l packet
+    // came earlier (B.12.2: a packet appears once, in layer order per precinct)
+    auto emit = [&](int c, int r, int q, int l) -> Status {
         if (r > T.comps[c].nlev) return Status::Ok;
-        const ResState& RS = bstate[c][r];
-        for (int q = 0; q < RS.pw * RS.ph; q++) {
-            Status s = read_packet(img, t, c, r, q, l, P, bstate, ranges, ri, pos, err);
-            if (s != Status::Ok) return s;
-        }
-        return Status::Ok;
+        ResState& RS = bstate[c][r];
+        if (q >= RS.pw * RS.ph || RS.nextl[q] != l) return Status::Ok;
+        RS.nextl[q]++;
+        return read_packet(img, t, c, r, q, l, P, bstate, ranges, ri, pos, err);
     };
-    Status s = Status::Ok;
-    if (P.prog == 0) {  // LRCP
-        for (int l = 0; l < L; l++)
-            for (int r = 0; r < maxr; r++)
-                for (int c = 0; c < C; c++)
-                    if ((s = pk_layer(c, r, l)) != Status::Ok) return s;
-        return Status::Ok;
-    }
-    if (P.prog == 1) {  // RLCP
-        for (int r = 0; r < maxr; r++)
-            for (int l = 0; l < L; l++)
-                for (int c = 0; c < C; c++)
-                    if ((s = pk_layer(c, r, l)) != Status::Ok) return s;
-        return Status::Ok;
-    }
-    // RPCL, PCRL, CPRL (B.12.1.3-B.12.1.5) step through reference-grid
+    // RPCL, PCRL and CPRL (B.12.1.3-B.12.1.5) step through reference-grid
     // positions and emit a precinct where it starts: at a multiple of its size
     // projected onto the grid, or at the tile origin for the first precinct of
     // a resolution whose origin is not such a multiple.  Every precinct starts
-    // at exactly one position, so sorting the precincts by that position
-    // gives the same sequence.
+    // at exactly one position, so sorting the precincts by that position gives
+    // the same sequence.
     struct Start {
         int64_t y, x;
         int c, r, q;
     };
-    std::vector<Start> seq;
-    for (int c = 0; c < C; c++) {
-        const TileComp& tc = T.comps[c];
-        for (int r = 0; r <= tc.nlev; r++) {
-            const ResState& RS = bstate[c][r];
-            const Resolution& R = tc.res[r];
-            const int lv = tc.nlev - r;
-            const int64_t sx = (int64_t)1 << (RS.ppx + lv), sy = (int64_t)1 << (RS.ppy + lv);
-            const bool ux = (((int64_t)R.x0 << lv) % sx) != 0, uy = (((int64_t)R.y0 << lv) % sy) != 0;
-            for (int q = 0; q < RS.pw * RS.ph; q++) {
-                int i = q % RS.pw, j = q / RS.pw;
-                int64_t x = (i == 0 && ux) ? tc.x0 : (RS.px0 + i) * sx;
-                int64_t y = (j == 0 && uy) ? tc.y0 : (RS.py0 + j) * sy;
-                seq.push_back(Start{y, x, c, r, q});
+    std::vector<Start> starts;
+    bool positional = P.prog >= 2;
+    for (const Poc& v : P.poc) positional |= v.prog >= 2;
+    if (positional)
+        for (int c = 0; c < C; c++) {
+            const TileComp& tc = T.comps[c];
+            for (int r = 0; r <= tc.nlev; r++) {
+                const ResState& RS = bstate[c][r];
+                const Resolution& R = tc.res[r];
+                const int lv = tc.nlev - r;
+                const int64_t sx = (int64_t)1 << (RS.ppx + lv), sy = (int64_t)1 << (RS.ppy + lv);
+                const bool ux = (((int64_t)R.x0 << lv) % sx) != 0, uy = (((int64_t)R.y0 << lv) % sy) != 0;
+                for (int q = 0; q < RS.pw * RS.ph; q++) {
+                    int i = q % RS.pw, j = q / RS.pw;
+                    int64_t x = (i == 0 && ux) ? tc.x0 : (RS.px0 + i) * sx;
+                    int64_t y = (j == 0 && uy) ? tc.y0 : (RS.py0 + j) * sy;
+                    starts.push_back(Start{y, x, c, r, q});
+                }
             }
         }
-    }
-    auto key = [&](const Start& a) {
-        if (P.prog == 2) return std::make_tuple((int64_t)a.r, a.y, a.x, (int64_t)a.c);  // RPCL
-        if (P.prog == 3) return std::make_tuple(a.y, a.x, (int64_t)a.c, (int64_t)a.r);  // PCRL
-        return std::make_tuple((int64_t)a.c, a.y, a.x, (int64_t)a.r);                   // CPRL
+    // one progression volume (B.12.1) in order v.prog
+    auto volume = [&](const Poc& v) -> Status {
+        const int ce = std::min(v.ce, C), le = std::min(v.le, P.nlayers);
+        Status s = Status::Ok;
+        if (v.prog <= 1) {
+            const bool lrcp = v.prog == 0;
+            const int n0 = lrcp ? le : v.re, n1 = lrcp ? v.re : le;
+            for (int a = lrcp ? 0 : v.rs; a < n0; a++)
+                for (int b = lrcp ? v.rs : 0; b < n1; b++) {
+                    const int l = lrcp ? a : b, r = lrcp ? b : a;
+                    if (r >= maxr) continue;
+                    for (int c = v.cs; c < ce; c++) {
+                        if (r > T.comps[c].nlev) continue;
+                        const ResState& RS = bstate[c][r];
+                        for (int q = 0; q < RS.pw * RS.ph; q++)
+                            if ((s = emit(c, r, q, l)) != Status::Ok) return s;
+                    }
+                }
+            return Status::Ok;
+        }
+        std::vector<Start> seq;
+        for (const Start& e : starts)
+            if (e.c >= v.cs && e.c < ce && e.r >= v.rs && e.r < v.re) seq.push_back(e);
+        auto key = [&](const Start& a) {
+            if (v.prog == 2) return std::make_tuple((int64_t)a.r, a.y, a.x, (int64_t)a.c);  // RPCL
+            if (v.prog == 3) return std::make_tuple(a.y, a.x, (int64_t)a.c, (int64_t)a.r);  // PCRL
+            return std::make_tuple((int64_t)a.c, a.y, a.x, (int64_t)a.r);                   // CPRL
+        };
+        std::sort(seq.begin(), seq.end(), [&](const Start& a, const Start& b) { return key(a) < key(b); });
+        for (const Start& e : seq)
+            for (int l = 0; l < le; l++)
+                if ((s = emit(e.c, e.r, e.q, l)) != Status::Ok) return s;
+        return Status::Ok;
     };
-    std::sort(seq.begin(), seq.end(), [&](const Start& a, const Start& b) { return key(a) < key(b); });
-    for (const Start& e : seq)
-        if ((s = pk(e.c, e.r, e.q)) != Status::Ok) return s;
-    return Status::Ok;
+    // the POC volumes in order, then the COD progression over what they left (B.12)
+    Status s = Status::Ok;
+    for (const Poc& v : P.poc)
+        if ((s = volume(v)) != Status::Ok) return s;
+    return volume(Poc{0, 0, P.nlayers, maxr, C, P.prog});
 }
 
 Status parse_impl(const uint8_t* data, size_t len, Image& img, std::string& err, bool packets) {
@@ -810,7 +860,7 @@ Status parse_impl(const uint8_t* data, size_t len, Image& img, std::string& err,
                 err = "coding parameters in a later tile-part";
                 return Status::BadMarker;
             }
-            s = read_segment(r, mk, img.ncomp, tp[I], coc, qcc, err);
+            s = read_segment(r, mk, img.ncomp, tp[I], coc, qcc, err, true);
             if (s != Status::Ok) return s;
         }
         if (end > r.pos()) ranges[I].push_back(Range{r.pos(), end - r.pos()});
