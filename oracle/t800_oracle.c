@@ -646,6 +646,9 @@ typedef struct {
     int ntx, nty;
     int npoc, poc_tile; /* POC entries; poc_tile: taken from the tile's own headers */
     poc_t poc[MAXPOC];
+    int nppm; /* PPM segments of the main header (A.7.4): index, offset, length */
+    int ppm_z[256];
+    size_t ppm_off[256], ppm_len[256];
 } hdr_t;
 
 typedef struct {
@@ -779,6 +782,13 @@ static int header_segment(rd_t *r, int mk, hdr_t *h, int *coc_set, int *qcc_set,
         }
         break;
     }
+    case 0xFF60: /* PPM: packed packet headers of every tile-part */
+        if (!main_hdr || h->nppm >= 256 || L < 3) return OR_ERR_MARKER;
+        h->ppm_z[h->nppm] = rd8(r);
+        h->ppm_off[h->nppm] = r->pos;
+        h->ppm_len[h->nppm] = end - r->pos;
+        h->nppm++;
+        break;
     case 0xFF64: /* COM */
     case 0xFF55: /* TLM */
     case 0xFF57: /* PLM */
@@ -786,7 +796,7 @@ static int header_segment(rd_t *r, int mk, hdr_t *h, int *coc_set, int *qcc_set,
     case 0xFF63: /* CRG */
         break;
     default:
-        return OR_ERR_UNSUPPORTED; /* RGN, PPM, PPT, unknown */
+        return OR_ERR_UNSUPPORTED; /* RGN, PPT outside a tile-part header, unknown */
     }
     if (rc) return rc;
     r->pos = end;
@@ -1163,17 +1173,18 @@ static void free_tc(tc_t *tc) {
         }
 }
 
-/* One packet of precinct q (B.9, B.10): header then body. */
+/* One packet of precinct q (B.9, B.10): header then body.  The header comes
+ * from d at *pos, or from the packed headers hd at *hpos when hd is set. */
 static int read_packet(const hdr_t *h, tc_t *tc, int r, int q, int layer, const uint8_t *d, size_t len,
-                       size_t *pos) {
+                       size_t *pos, const uint8_t *hd, size_t hlen, size_t *hpos) {
     res_t *R = &tc->r[r];
     if (q >= R->pw * R->ph) return OR_OK; /* no precinct, no packet */
     if (h->g.sop && *pos + 6 <= len && d[*pos] == 0xFF && d[*pos + 1] == 0x91) *pos += 6;
     bio_t b;
     memset(&b, 0, sizeof b);
-    b.p = d;
-    b.len = len;
-    b.pos = *pos;
+    b.p = hd ? hd : d;
+    b.len = hd ? hlen : len;
+    b.pos = hd ? *hpos : *pos;
     /* included segment lengths in header order */
     int present = bio_bit(&b);
     int nseg = 0, cap = 0;
@@ -1220,10 +1231,16 @@ static int read_packet(const hdr_t *h, tc_t *tc, int r, int q, int layer, const 
         free(segcb);
         return b.err;
     }
-    *pos = b.pos;
+    if (hd)
+        *hpos = b.pos;
+    else
+        *pos = b.pos;
     if (h->g.eph) {
-        if (*pos + 2 <= len && d[*pos] == 0xFF && d[*pos + 1] == 0x92)
-            *pos += 2;
+        /* EPH ends the header, wherever the header is */
+        const uint8_t *e = hd ? hd : d;
+        size_t elen = hd ? hlen : len, *ep = hd ? hpos : pos;
+        if (*ep + 2 <= elen && e[*ep] == 0xFF && e[*ep + 1] == 0x92)
+            *ep += 2;
         else {
             free(seglen);
             free(segcb);
@@ -1269,27 +1286,31 @@ static int prc_at(const tc_t *tc, int r, long x, long y) {
 /* Reads packet (c, r, q, l) unless that precinct has no packets or its
  * packet of layer l came earlier (B.12.2: a packet appears once, and the
  * packets of a precinct come in layer order). */
-static int emit(const hdr_t *h, tc_t *tc, int r, int q, int l, const uint8_t *d, size_t len, size_t *pos) {
+typedef struct {
+    const uint8_t *d, *hd; /* tile data; packed headers or NULL */
+    size_t len, hlen, pos, hpos;
+} pkt_src_t;
+
+static int emit(const hdr_t *h, tc_t *tc, int r, int q, int l, pkt_src_t *S) {
     if (r >= tc->nres) return OR_OK;
     res_t *R = &tc->r[r];
     if (q < 0 || q >= R->pw * R->ph || R->nextl[q] != l) return OR_OK;
-    if (*pos >= len) return OR_END;
+    if (S->hd ? S->hpos >= S->hlen : S->pos >= S->len) return OR_END;
     R->nextl[q]++;
-    return read_packet(h, tc, r, q, l, d, len, pos);
+    return read_packet(h, tc, r, q, l, S->d, S->len, &S->pos, S->hd, S->hlen, &S->hpos);
 }
 
 /* The packets of one progression volume (B.12.1) in order v->prog.  xs, ys:
  * position steps of the position-driven orders, no larger than any precinct
  * projected onto the reference grid. */
-static int read_volume(const hdr_t *h, tc_t *tcs, const poc_t *v, long xs, long ys, const uint8_t *d, size_t len,
-                       size_t *pos) {
+static int read_volume(const hdr_t *h, tc_t *tcs, const poc_t *v, long xs, long ys, pkt_src_t *S) {
     const long tx0 = tcs[0].x0, ty0 = tcs[0].y0, tx1 = tcs[0].x1, ty1 = tcs[0].y1;
     const int cs = v->cs, ce = v->ce < h->C ? v->ce : h->C, rs = v->rs, re = v->re;
     const int le = v->le < h->g.nlayers ? v->le : h->g.nlayers;
     int rc;
 #define EM(c_, r_, q_, l_)                                          \
     do {                                                            \
-        rc = emit(h, &tcs[c_], r_, q_, l_, d, len, pos);            \
+        rc = emit(h, &tcs[c_], r_, q_, l_, S);                      \
         if (rc) return rc;                                          \
     } while (0)
 #define NPRC(c_, r_) ((r_) < tcs[c_].nres ? tcs[c_].r[r_].pw * tcs[c_].r[r_].ph : 0)
@@ -1347,8 +1368,9 @@ static int read_volume(const hdr_t *h, tc_t *tcs, const poc_t *v, long xs, long 
 
 /* Runs every packet of a tile: the POC volumes in order, then the COD
  * progression over whatever they left (B.12). */
-static int read_tile_packets(const hdr_t *h, tc_t *tcs, const uint8_t *d, size_t len) {
-    size_t pos = 0;
+static int read_tile_packets(const hdr_t *h, tc_t *tcs, const uint8_t *d, size_t len, const uint8_t *hd,
+                             size_t hlen) {
+    pkt_src_t S = {d, hd, len, hlen, 0, 0};
     int maxr = 0;
     for (int c = 0; c < h->C; c++)
         if (tcs[c].nres > maxr) maxr = tcs[c].nres;
@@ -1362,67 +1384,143 @@ static int read_tile_packets(const hdr_t *h, tc_t *tcs, const uint8_t *d, size_t
             if (b < ys) ys = b;
         }
     int rc = OR_OK;
-    for (int i = 0; i < h->npoc && !rc; i++) rc = read_volume(h, tcs, &h->poc[i], xs, ys, d, len, &pos);
+    for (int i = 0; i < h->npoc && !rc; i++) rc = read_volume(h, tcs, &h->poc[i], xs, ys, &S);
     if (!rc) {
         poc_t all = {0, 0, h->g.nlayers, maxr, h->C, h->g.prog};
-        rc = read_volume(h, tcs, &all, xs, ys, d, len, &pos);
+        rc = read_volume(h, tcs, &all, xs, ys, &S);
     }
     return rc == OR_END ? OR_OK : rc;
 }
 
 /* Collects the data of every tile-part of tile t into one buffer and applies
- * tile-part header overrides (A.4.2). */
+ * tile-part header overrides (A.4.2).  Packed packet headers (PPM of the main
+ * header, PPT of the tile-part headers; A.7.4, A.7.5) are gathered per tile
+ * in the same tile-part order. */
 typedef struct {
-    uint8_t *data;
-    size_t len, cap;
+    uint8_t *data, *hdr;
+    size_t len, cap, hlen, hcap;
     hdr_t h; /* effective parameters for the tile */
-    int seen;
+    int seen, packed;
 } tiledata_t;
+
+static int buf_append(uint8_t **p, size_t *len, size_t *cap, const uint8_t *src, size_t n) {
+    if (*len + n > *cap) {
+        size_t nc = (*len + n) * 2 + 64;
+        uint8_t *q = (uint8_t *)realloc(*p, nc);
+        if (!q) return OR_ERR_NOMEM;
+        *p = q;
+        *cap = nc;
+    }
+    if (n) memcpy(*p + *len, src, n);
+    *len += n;
+    return OR_OK;
+}
+
+/* The payloads of n marker segments in order of their index byte z. */
+static int concat_by_index(const uint8_t *cs, int n, const int *z, const size_t *off, const size_t *slen,
+                           uint8_t **out, size_t *olen, size_t *ocap) {
+    for (int want = 0, done = 0; done < n; want++) {
+        if (want > 255) return OR_ERR_MARKER;
+        for (int i = 0; i < n; i++)
+            if (z[i] == want) {
+                int rc = buf_append(out, olen, ocap, cs + off[i], slen[i]);
+                if (rc) return rc;
+                done++;
+            }
+    }
+    return OR_OK;
+}
 
 static int gather_tiles(const uint8_t *cs, size_t len, const hdr_t *mh, tiledata_t *td, size_t start) {
     rd_t r = {cs, len, start, 0};
     int nt = mh->ntx * mh->nty;
-    for (;;) {
-        if (r.pos + 2 > len) return OR_OK; /* tolerate missing EOC */
+    uint8_t *ppm = NULL;
+    size_t ppm_len = 0, ppm_cap = 0, ppm_pos = 0;
+    int rc = concat_by_index(cs, mh->nppm, mh->ppm_z, mh->ppm_off, mh->ppm_len, &ppm, &ppm_len, &ppm_cap);
+    for (; !rc;) {
+        if (r.pos + 2 > len) break; /* tolerate missing EOC */
         int mk = rd16(&r);
-        if (mk == 0xFFD9) return OR_OK;
-        if (mk != 0xFF90) return OR_ERR_MARKER;
+        if (mk == 0xFFD9) break;
+        if (mk != 0xFF90) {
+            rc = OR_ERR_MARKER;
+            break;
+        }
         size_t sot = r.pos - 2;
         int Lsot = rd16(&r);
         int I = rd16(&r);
         uint32_t Psot = rd32(&r);
         int TP = rd8(&r);
         rd8(&r);
-        if (r.err) return r.err;
-        if (Lsot != 10 || I >= nt) return OR_ERR_MARKER;
+        if (r.err) {
+            rc = r.err;
+            break;
+        }
         size_t end = Psot ? sot + Psot : len - 2;
-        if (end > len || end < r.pos) return OR_ERR_TRUNCATED;
+        if (Lsot != 10 || I >= nt) rc = OR_ERR_MARKER;
+        else if (end > len || end < r.pos) rc = OR_ERR_TRUNCATED;
+        if (rc) break;
         tiledata_t *T = &td[I];
         if (!T->seen) {
             T->h = *mh;
             T->seen = 1;
         }
         int coc_set[MAXC] = {0}, qcc_set[MAXC] = {0};
-        for (;;) {
+        int nppt = 0, ppt_z[256];
+        size_t ppt_off[256], ppt_len[256];
+        for (; !rc;) {
             int m = rd16(&r);
-            if (r.err) return r.err;
+            if (r.err) {
+                rc = r.err;
+                break;
+            }
             if (m == 0xFF93) break;
-            if (TP != 0 && (m == 0xFF52 || m == 0xFF53 || m == 0xFF5C || m == 0xFF5D)) return OR_ERR_MARKER;
-            int rc = header_segment(&r, m, &T->h, coc_set, qcc_set, 0);
-            if (rc) return rc;
+            if (TP != 0 && (m == 0xFF52 || m == 0xFF53 || m == 0xFF5C || m == 0xFF5D)) {
+                rc = OR_ERR_MARKER;
+                break;
+            }
+            if (m == 0xFF61) { /* PPT */
+                int L = rd16(&r);
+                size_t e = r.pos + (size_t)L - 2;
+                if (L < 3 || e > len || nppt >= 256) {
+                    rc = OR_ERR_MARKER;
+                    break;
+                }
+                ppt_z[nppt] = rd8(&r);
+                ppt_off[nppt] = r.pos;
+                ppt_len[nppt] = e - r.pos;
+                nppt++;
+                r.pos = e;
+                continue;
+            }
+            rc = header_segment(&r, m, &T->h, coc_set, qcc_set, 0);
         }
-        size_t n = end - r.pos;
-        if (T->len + n > T->cap) {
-            size_t nc = (T->len + n) * 2 + 64;
-            uint8_t *p = (uint8_t *)realloc(T->data, nc);
-            if (!p) return OR_ERR_NOMEM;
-            T->data = p;
-            T->cap = nc;
+        if (rc) break;
+        if (nppt && mh->nppm) rc = OR_ERR_MARKER; /* PPM and PPT exclude each other */
+        if (!rc && nppt) {
+            rc = concat_by_index(cs, nppt, ppt_z, ppt_off, ppt_len, &T->hdr, &T->hlen, &T->hcap);
+            T->packed = 1;
         }
-        memcpy(T->data + T->len, cs + r.pos, n);
-        T->len += n;
+        if (!rc && mh->nppm) { /* Nppm, then Ippm, per tile-part in codestream order */
+            if (ppm_pos + 4 > ppm_len) {
+                rc = OR_ERR_TRUNCATED;
+                break;
+            }
+            size_t n = ((size_t)ppm[ppm_pos] << 24) | ((size_t)ppm[ppm_pos + 1] << 16) |
+                       ((size_t)ppm[ppm_pos + 2] << 8) | ppm[ppm_pos + 3];
+            ppm_pos += 4;
+            if (ppm_pos + n > ppm_len) {
+                rc = OR_ERR_TRUNCATED;
+                break;
+            }
+            rc = buf_append(&T->hdr, &T->hlen, &T->hcap, ppm + ppm_pos, n);
+            ppm_pos += n;
+            T->packed = 1;
+        }
+        if (!rc) rc = buf_append(&T->data, &T->len, &T->cap, cs + r.pos, end - r.pos);
         r.pos = end;
     }
+    free(ppm);
+    return rc;
 }
 
 static size_t find_codestream(const uint8_t *cs, size_t len, size_t *clen) {
@@ -1522,12 +1620,16 @@ static int walk_tiles(const uint8_t *cs0, size_t len0, tile_fn fn, void *user) {
             break;
         }
         for (int c = 0; c < h.C && !rc; c++) rc = build_tc(th, t, c, &tcs[c]);
-        if (!rc && td[t].seen) rc = read_tile_packets(th, tcs, td[t].data, td[t].len);
+        if (!rc && td[t].seen)
+            rc = read_tile_packets(th, tcs, td[t].data, td[t].len, td[t].packed ? td[t].hdr : NULL, td[t].hlen);
         if (!rc) rc = fn(th, t, tcs, user);
         for (int c = 0; c < h.C; c++) free_tc(&tcs[c]);
         free(tcs);
     }
-    for (int t = 0; t < nt; t++) free(td[t].data);
+    for (int t = 0; t < nt; t++) {
+        free(td[t].data);
+        free(td[t].hdr);
+    }
     free(td);
     return rc;
 }

@@ -83,3 +83,18 @@ def test_cinema_profiles_within_one(gpu_decoder, mode, shape):
     d = np.abs(out - ref)
     assert d.max() <= 1, f"max |gpu - oracle| = {d.max()}"
     assert (d > 0).mean() < 0.01
+
+
+@pytest.mark.parametrize("packed", ["ppt", "ppm"])
+def test_packed_headers_equal_inline(gpu_decoder, packed):
+    """PPT / PPM packed packet headers (package encoder, several tiles and
+    small segments): the GPU image equals the in-line stream's and the source."""
+    import thousandthchicken_amd as tca
+    src = _img((300, 260, 3), 14)
+    inline = synth.encode(src, levels=4, cb=32, tile=(128, 128))
+    cs = synth.encode(src, levels=4, cb=32, tile=(128, 128), packed=packed, packed_max=64)
+    print(f"case {packed} {len(cs)} bytes", flush=True)
+    out = gpu_decoder.decode(cs)
+    assert np.array_equal(out, gpu_decoder.decode(inline))
+    assert np.array_equal(out.reshape(src.shape), src)
+    assert np.array_equal(tca.decode_staged(cs).reshape(src.shape), src)

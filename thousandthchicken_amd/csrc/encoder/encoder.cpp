@@ -700,6 +700,8 @@ struct Params {
     double target_bytes;  // irreversible: codestream size target (0: keep every pass)
     double base_step;     // irreversible: step for a unit-gain band
     int cbsty = 0;        // code-block style (RESET 2, CAUSAL 8, PTERM 16, SEGSYM 32)
+    int packed = 0;       // packet headers: 0 in the packets, 1 PPT, 2 PPM (A.7.4, A.7.5)
+    int packed_max = 0;   // largest payload of one PPT/PPM segment (0: 65532)
 };
 
 // L2 norm of the synthesis basis function of a band at level nb (orientation o),
@@ -1058,10 +1060,16 @@ int encode(const void* pixels, const Params& P, std::vector<uint8_t>& out) {
         out.push_back((uint8_t)((G << 5) | 2));
         for (auto& q : qp) put16(out, (uint32_t)((q.eps << 11) | q.mu));
     }
-    // tiles: one tile-part each, packets in LRCP order with one layer
+    // tiles: one tile-part each, packets in LRCP order with one layer; with
+    // packed headers the packet headers go to PPT (tile-part header) or PPM
+    // (main header) marker segments and the tile data holds the bodies only
+    std::vector<uint8_t> parts;
+    std::vector<std::vector<uint8_t>> packed_hdrs(tiles.size());
+    const size_t seg_max = P.packed_max > 0 ? (size_t)std::min(P.packed_max, 65532) : 65532;
     for (int t = 0; t < (int)tiles.size(); t++) {
         TileWork& T = tiles[t];
         std::vector<uint8_t> body;
+        std::vector<uint8_t>& phdr = packed_hdrs[t];
         std::vector<std::vector<TagTree>> incl(C), zbp(C);
         std::vector<std::vector<int>> lblock(C);
         for (int c = 0; c < C; c++) {
@@ -1137,18 +1145,50 @@ int encode(const void* pixels, const Params& P, std::vector<uint8_t>& out) {
                     }
                 }
                 bw.align();
-                body.insert(body.end(), hdr.begin(), hdr.end());
+                std::vector<uint8_t>& dst = P.packed ? phdr : body;
+                dst.insert(dst.end(), hdr.begin(), hdr.end());
                 for (const CB* cb : bodies) body.insert(body.end(), cb->r.data.begin(), cb->r.data.begin() + cb->keep_len);
             }
-        put16(out, 0xFF90);  // SOT
-        put16(out, 10);
-        put16(out, (uint32_t)t);
-        put32(out, (uint32_t)(12 + 2 + body.size()));
-        out.push_back(0);
-        out.push_back(1);
-        put16(out, 0xFF93);  // SOD
-        out.insert(out.end(), body.begin(), body.end());
+        std::vector<uint8_t> tph;  // tile-part header segments
+        if (P.packed == 1)
+            for (size_t off = 0, z = 0; off < phdr.size() || z == 0; z++) {
+                if (z > 255) return -4;  // Zppt is one byte
+                size_t n = std::min(seg_max, phdr.size() - off);
+                put16(tph, 0xFF61);  // PPT
+                put16(tph, (uint32_t)(3 + n));
+                tph.push_back((uint8_t)z);
+                tph.insert(tph.end(), phdr.begin() + off, phdr.begin() + off + n);
+                off += n;
+            }
+        put16(parts, 0xFF90);  // SOT
+        put16(parts, 10);
+        put16(parts, (uint32_t)t);
+        put32(parts, (uint32_t)(12 + tph.size() + 2 + body.size()));
+        parts.push_back(0);
+        parts.push_back(1);
+        parts.insert(parts.end(), tph.begin(), tph.end());
+        put16(parts, 0xFF93);  // SOD
+        parts.insert(parts.end(), body.begin(), body.end());
     }
+    if (P.packed == 2) {
+        // Nppm (4 bytes) + Ippm per tile-part in codestream order, split over
+        // PPM segments wherever the payload limit falls
+        std::vector<uint8_t> stream;
+        for (auto& h : packed_hdrs) {
+            put32(stream, (uint32_t)h.size());
+            stream.insert(stream.end(), h.begin(), h.end());
+        }
+        for (size_t off = 0, z = 0; off < stream.size(); z++) {
+            if (z > 255) return -4;  // Zppm is one byte
+            size_t n = std::min(seg_max, stream.size() - off);
+            put16(out, 0xFF60);  // PPM
+            put16(out, (uint32_t)(3 + n));
+            out.push_back((uint8_t)z);
+            out.insert(out.end(), stream.begin() + off, stream.begin() + off + n);
+            off += n;
+        }
+    }
+    out.insert(out.end(), parts.begin(), parts.end());
     put16(out, 0xFFD9);  // EOC
     return 0;
 }
@@ -1160,11 +1200,12 @@ extern "C" {
 // Returns 0 and a malloc'ed codestream in *out (free with tce_free).
 int tce_encode(const void* pixels, int w, int h, int ncomp, int prec, int levels, int cbw_exp, int cbh_exp,
                int tile_w, int tile_h, int mct, int irreversible, double target_bytes, double base_step, int threads,
-               int cbsty, uint8_t** out, size_t* out_len) {
-    if (!pixels || !out || !out_len || w <= 0 || h <= 0 || (cbsty & ~(2 | 8 | 16 | 32))) return -1;
+               int cbsty, int packed, int packed_max, uint8_t** out, size_t* out_len) {
+    if (!pixels || !out || !out_len || w <= 0 || h <= 0 || (cbsty & ~(2 | 8 | 16 | 32)) || packed < 0 || packed > 2)
+        return -1;
     tce::Params P{w, h, ncomp, prec, levels, cbw_exp, cbh_exp, tile_w, tile_h, mct, irreversible,
                   threads > 0 ? threads : (int)std::max(1u, std::thread::hardware_concurrency()), target_bytes,
-                  base_step > 0 ? base_step : 0.5, cbsty};
+                  base_step > 0 ? base_step : 0.5, cbsty, packed, packed_max};
     std::vector<uint8_t> cs;
     int rc = tce::encode(pixels, P, cs);
     if (rc) return rc;

@@ -32,6 +32,12 @@ struct Poc {
     int rs, cs, le, re, ce, prog;
 };
 
+// A PPM or PPT marker segment: index byte and payload range in the codestream.
+struct PackedSeg {
+    int z;
+    size_t off, len;
+};
+
 struct Params {
     int prog = 0, nlayers = 1, mct = 0;
     bool sop = false, eph = false;
@@ -39,6 +45,7 @@ struct Params {
     std::vector<Quant> q;
     std::vector<Poc> poc;
     bool poc_tile = false;  // poc came from the tile's own headers
+    std::vector<PackedSeg> ppm;  // main header only (A.7.4)
 };
 
 struct Range {
@@ -317,6 +324,19 @@ Status read_segment(Reader& r, uint32_t marker, int ncomp, Params& P, std::vecto
         }
         break;
     }
+    case 0xFF60:  // PPM: packed packet headers of every tile-part (A.7.4)
+        if (tile_hdr || L < 3) {
+            err = "PPM outside the main header";
+            return Status::BadMarker;
+        }
+        {
+            int z = (int)r.u8();
+            P.ppm.push_back(PackedSeg{z, r.pos(), end - r.pos()});
+        }
+        break;
+    case 0xFF61:  // PPT: the tile-part loop takes these
+        err = "PPT outside a tile-part header";
+        return Status::BadMarker;
     case 0xFF64:  // COM
     case 0xFF55:  // TLM
     case 0xFF57:  // PLM
@@ -613,11 +633,12 @@ int ilog2(int v) {
     return l;
 }
 
-// Reads the packet of (component c, resolution r, precinct q, layer l) at
-// *pos within the tile data ranges.
+// Reads the packet of (component c, resolution r, precinct q, layer l): its
+// body at pos within the tile data ranges, its header there too or, for
+// packed headers (PPM/PPT), in *ph at hpos.
 Status read_packet(Image& img, int t, int c, int r, int q, int layer, const Params& P,
                    TileState& bstate, const std::vector<Range>& ranges, size_t& ri,
-                   size_t& pos, std::string& err) {
+                   size_t& pos, const std::vector<uint8_t>* ph, size_t& hpos, std::string& err) {
     const TileComp& tc = img.tiles[t].comps[c];
     const Resolution& R = tc.res[r];
     ResState& RS = bstate[c][r];
@@ -626,11 +647,11 @@ Status read_packet(Image& img, int t, int c, int r, int q, int layer, const Para
         ++ri;
         if (ri < ranges.size()) pos = ranges[ri].off;
     }
-    if (ri >= ranges.size()) return Status::Ok;  // stream ends at a packet boundary
+    if (ph ? hpos >= ph->size() : ri >= ranges.size()) return Status::Ok;  // stream ends at a packet boundary
     const uint8_t* d = img.cs;
-    size_t end = ranges[ri].off + ranges[ri].len;
+    size_t end = ri < ranges.size() ? ranges[ri].off + ranges[ri].len : pos;
     if (P.sop && pos + 6 <= end && d[pos] == 0xFF && d[pos + 1] == 0x91) pos += 6;
-    BitReader br(d, end, pos);
+    BitReader br = ph ? BitReader(ph->data(), ph->size(), hpos) : BitReader(d, end, pos);
     struct Inc {
         uint32_t cb;
         uint32_t len;
@@ -680,10 +701,14 @@ Status read_packet(Image& img, int t, int c, int r, int q, int layer, const Para
         err = "truncated packet header";
         return Status::Truncated;
     }
-    pos = br.pos();
+    // the header ends here, with EPH when signalled, in whichever stream holds it
+    const uint8_t* hd = ph ? ph->data() : d;
+    const size_t hend = ph ? ph->size() : end;
+    size_t& hp = ph ? hpos : pos;
+    hp = br.pos();
     if (P.eph) {
-        if (pos + 2 <= end && d[pos] == 0xFF && d[pos + 1] == 0x92)
-            pos += 2;
+        if (hp + 2 <= hend && hd[hp] == 0xFF && hd[hp + 1] == 0x92)
+            hp += 2;
         else {
             err = "missing EPH";
             return Status::BadMarker;
@@ -708,9 +733,9 @@ Status read_packet(Image& img, int t, int c, int r, int q, int layer, const Para
 }
 
 Status read_packets(Image& img, int t, const Params& P, TileState& bstate, const std::vector<Range>& ranges,
-                    std::string& err) {
-    if (ranges.empty()) return Status::Ok;
-    size_t ri = 0, pos = ranges[0].off;
+                    const std::vector<uint8_t>* ph, std::string& err) {
+    if (ranges.empty() && !ph) return Status::Ok;
+    size_t ri = 0, pos = ranges.empty() ? 0 : ranges[0].off, hpos = 0;
     const Tile& T = img.tiles[t];
     int C = img.ncomp, maxr = 0;
     for (int c = 0; c < C; c++) maxr = std::max(maxr, T.comps[c].nlev + 1);
@@ -721,7 +746,7 @@ Status read_packets(Image& img, int t, const Params& P, TileState& bstate, const
         ResState& RS = bstate[c][r];
         if (q >= RS.pw * RS.ph || RS.nextl[q] != l) return Status::Ok;
         RS.nextl[q]++;
-        return read_packet(img, t, c, r, q, l, P, bstate, ranges, ri, pos, err);
+        return read_packet(img, t, c, r, q, l, P, bstate, ranges, ri, pos, ph, hpos, err);
     };
     // RPCL, PCRL and CPRL (B.12.1.3-B.12.1.5) step through reference-grid
     // positions and emit a precinct where it starts: at a multiple of its size
@@ -820,8 +845,17 @@ Status parse_impl(const uint8_t* data, size_t len, Image& img, std::string& err,
     int nt = img.ntx * img.nty;
     img.tiles.resize(nt);
     std::vector<Params> tp(nt);
-    std::vector<uint8_t> seen(nt, 0);
+    std::vector<uint8_t> seen(nt, 0), packed(nt, 0);
     std::vector<std::vector<Range>> ranges(nt);
+    std::vector<std::vector<uint8_t>> hdrs(nt);  // packed packet headers per tile
+    // payloads of marker segments in the order of their index bytes
+    auto concat = [&](std::vector<PackedSeg> segs, std::vector<uint8_t>& dst) {
+        std::stable_sort(segs.begin(), segs.end(), [](const PackedSeg& a, const PackedSeg& b) { return a.z < b.z; });
+        for (const PackedSeg& g : segs) dst.insert(dst.end(), cs + g.off, cs + g.off + g.len);
+    };
+    std::vector<uint8_t> ppm;  // Nppm + Ippm per tile-part, in codestream order
+    size_t ppm_pos = 0;
+    concat(main.ppm, ppm);
     // tile-parts (A.4.2)
     for (;;) {
         if (r.pos() + 2 > cl) break;  // tolerate a missing EOC
@@ -852,16 +886,53 @@ Status parse_impl(const uint8_t* data, size_t len, Image& img, std::string& err,
             seen[I] = 1;
         }
         std::vector<uint8_t> coc(img.ncomp, 0), qcc(img.ncomp, 0);
+        std::vector<PackedSeg> ppt;
         for (;;) {
             uint32_t mk = r.u16();
             if (!r.ok()) return Status::Truncated;
             if (mk == 0xFF93) break;
+            if (mk == 0xFF61) {  // PPT (A.7.5)
+                size_t lpos = r.pos();
+                uint32_t L = r.u16();
+                if (!r.ok() || L < 3 || lpos + L > end) {
+                    err = "bad PPT";
+                    return Status::BadMarker;
+                }
+                int z = (int)r.u8();
+                ppt.push_back(PackedSeg{z, r.pos(), lpos + L - r.pos()});
+                r.seek(lpos + L);
+                continue;
+            }
             if (TP != 0 && (mk == 0xFF52 || mk == 0xFF53 || mk == 0xFF5C || mk == 0xFF5D)) {
                 err = "coding parameters in a later tile-part";
                 return Status::BadMarker;
             }
             s = read_segment(r, mk, img.ncomp, tp[I], coc, qcc, err, true);
             if (s != Status::Ok) return s;
+        }
+        if (!ppt.empty() && !main.ppm.empty()) {
+            err = "PPM and PPT together";
+            return Status::BadMarker;
+        }
+        if (!ppt.empty()) {
+            concat(ppt, hdrs[I]);
+            packed[I] = 1;
+        }
+        if (!main.ppm.empty()) {
+            if (ppm_pos + 4 > ppm.size()) {
+                err = "PPM holds fewer tile-parts than the codestream";
+                return Status::Truncated;
+            }
+            size_t n = ((size_t)ppm[ppm_pos] << 24) | ((size_t)ppm[ppm_pos + 1] << 16) |
+                       ((size_t)ppm[ppm_pos + 2] << 8) | ppm[ppm_pos + 3];
+            ppm_pos += 4;
+            if (ppm_pos + n > ppm.size()) {
+                err = "truncated PPM";
+                return Status::Truncated;
+            }
+            hdrs[I].insert(hdrs[I].end(), ppm.begin() + ppm_pos, ppm.begin() + ppm_pos + n);
+            ppm_pos += n;
+            packed[I] = 1;
         }
         if (end > r.pos()) ranges[I].push_back(Range{r.pos(), end - r.pos()});
         r.seek(end);
@@ -871,7 +942,7 @@ Status parse_impl(const uint8_t* data, size_t len, Image& img, std::string& err,
         TileState bs;
         s = build_tile(img, t, P, bs, err);
         if (s != Status::Ok) return s;
-        s = read_packets(img, t, P, bs, ranges[t], err);
+        s = read_packets(img, t, P, bs, ranges[t], packed[t] ? &hdrs[t] : nullptr, err);
         if (s != Status::Ok) return s;
     }
     for (auto& cb : img.cblks) {

@@ -139,7 +139,7 @@ def _enc_lib():
             _b.build_encoder()
         L = ctypes.CDLL(path)
         vp, i32 = ctypes.c_void_p, ctypes.c_int
-        L.tce_encode.argtypes = [vp] + [i32] * 10 + [i32, ctypes.c_double, ctypes.c_double, i32, i32,
+        L.tce_encode.argtypes = [vp] + [i32] * 10 + [i32, ctypes.c_double, ctypes.c_double, i32, i32, i32, i32,
                                                       ctypes.POINTER(ctypes.c_void_p),
                                                       ctypes.POINTER(ctypes.c_size_t)]
         L.tce_free.argtypes = [vp]
@@ -151,13 +151,15 @@ def _enc_lib():
 
 
 def encode(img, bits=None, levels=5, cb=64, tile=None, mct=None, lossy=False, bpp=None, base_step=0.5,
-           threads=0, cbsty=0):
+           threads=0, cbsty=0, packed=None, packed_max=0):
     """Encodes img [H, W] or [H, W, C] (uint8 / uint16) into a raw codestream.
 
     Lossless: RCT (when mct) + 5/3, every pass kept.  Lossy: ICT + 9/7 with
     expounded quantization, truncated to about `bpp` bits per pixel.  cbsty:
     code-block style bits written to COD and used by tier-1 (RESET 2, CAUSAL 8,
-    PTERM 16, SEGSYM 32)."""
+    PTERM 16, SEGSYM 32).  packed: None, "ppt" or "ppm" -- packet headers in
+    the packets, in PPT segments of the tile-part headers or in PPM segments of
+    the main header; packed_max caps each segment's payload (0: 65532)."""
     import ctypes
     a = np.ascontiguousarray(img)
     if a.ndim == 2:
@@ -177,7 +179,8 @@ def encode(img, bits=None, levels=5, cb=64, tile=None, mct=None, lossy=False, bp
     n = ctypes.c_size_t()
     cbe = int(cb).bit_length() - 1
     rc = _enc_lib().tce_encode(a.ctypes.data, w, h, c, bits, levels, cbe, cbe, tw, th, 1 if mct else 0,
-                               1 if lossy else 0, target, base_step, threads, cbsty, ctypes.byref(out),
+                               1 if lossy else 0, target, base_step, threads, cbsty,
+                               {None: 0, "ppt": 1, "ppm": 2}[packed], packed_max, ctypes.byref(out),
                                ctypes.byref(n))
     if rc:
         raise RuntimeError(f"tce_encode failed ({rc})")
