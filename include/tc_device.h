@@ -177,6 +177,22 @@ tc_status tc_launch_idwt_level(tc_stream s, int irreversible, int shape, const v
 tc_status tc_launch_idwt_final(tc_stream s, int irreversible, int shape, int ncomp, int out_bytes, const void *jobs,
                                const uint32_t *wg_job, int nwg, const void *base, void *out);
 
+/* One forward DWT level over a job list (encoder side; run, TC/dwt_f53.cl:184 /
+ * TC/dwt_f97.cl:220): each tc_fdwt_job (thousandthchicken_amd/csrc/tc_kernels.h)
+ * splits a resolution raster into its four subbands, int32 5/3 or float 9/7.
+ * Jobs are built for tile shape `shape` (tc_idwt_tile_shape); wg_job maps each of
+ * the nwg work-groups to its job. */
+tc_status tc_launch_fdwt_level(tc_stream s, int irreversible, int shape, const void *jobs, const uint32_t *wg_job,
+                               int nwg, void *base);
+
+/* First forward level fused with the DC level shift and the forward RCT (5/3) or
+ * ICT (9/7) (fdc_level_shift_kernel TC/preprocess_dc_level_shift.cl:16,
+ * rct_kernel TC/preprocess_rct.cl:18, ict_kernel TC/preprocess_ict.cl:20): reads
+ * the interleaved image (in_bytes 1: 8-bit, 2: 16-bit; nc 1 or 3) and writes the
+ * level-1 subbands of every component per tc_fdwt_first_job. */
+tc_status tc_launch_fdwt_first(tc_stream s, int irreversible, int shape, int nc, int in_bytes, const void *jobs,
+                               const uint32_t *wg_job, int nwg, const void *img, void *base);
+
 /* Standalone inverse colour transform / DC shift over int32 planes in place, for the
  * Preprocessor drop-in (color_decoder_lossless/lossy, idc_level_shifting,
  * TC/Preprocessor.cpp:102-157).  kind: 0 DC shift only, 1 inverse RCT, 2 inverse ICT,

@@ -130,6 +130,21 @@ def lib():
         "tc_t1_lockstep_scratch_bytes": (sz, [i32, i32, i32, i32]),
         "tc_launch_t1_decode_lockstep": (i32, [vp, i32, i32, vp, vp, i32, i32, vp, sz, vp, sz, vp]),
         "tc_idwt_tile_shape": (i32, [i32, P(i32), P(i32), P(i32)]),
+        "tc_launch_fdwt_level": (i32, [vp, i32, i32, vp, vp, i32, vp]),
+        "tc_launch_fdwt_first": (i32, [vp, i32, i32, i32, i32, vp, vp, i32, vp, vp]),
+        "tc_forward_create": (i32, [vp, P(vp)]),
+        "tc_forward_destroy": (i32, [vp]),
+        "tc_forward_prepare": (i32, [vp, i32, i32, i32, i32, i32, i32, i32]),
+        "tc_forward_set_shape": (i32, [vp, i32]),
+        "tc_forward_upload": (i32, [vp, vp]),
+        "tc_forward_run": (i32, [vp]),
+        "tc_forward_run_timed": (i32, [vp, P(ctypes.c_float), P(ctypes.c_float)]),
+        "tc_forward_sync": (i32, [vp]),
+        "tc_forward_download": (i32, [vp, vp]),
+        "tc_forward_stream": (vp, [vp]),
+        "tc_forward_image_ptr": (vp, [vp]),
+        "tc_forward_planes_ptr": (vp, [vp]),
+        "tc_dwt_plane": (i32, [vp, i32, i32, vp, vp, i32, i32, i32]),
     }
     for name, (res, args) in sigs.items():
         f = getattr(L, name)
@@ -470,3 +485,101 @@ class Decoder:
 
     def stream_handle(self):
         return lib().tc_decoder_stream(self.handle)
+
+
+class Forward:
+    """GPU forward transform (encoder front end) over the C ABI: DC level shift,
+    forward RCT/ICT and an L-level forward 5/3 or 9/7 DWT of one image, into one
+    Mallat-layout plane per component (include/tc_forward.h)."""
+
+    def __init__(self, device: "Device" = None):
+        self.device = device or default_device()
+        h = ctypes.c_void_p()
+        check(lib().tc_forward_create(self.device.handle, ctypes.byref(h)), "tc_forward_create")
+        self.handle = h
+        self._geom = None
+
+    def close(self):
+        if getattr(self, "handle", None):
+            lib().tc_forward_destroy(self.handle)
+            self.handle = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def set_shape(self, shape):
+        check(lib().tc_forward_set_shape(self.handle, shape), "set_shape")
+
+    def prepare(self, w, h, ncomp, bits, levels, irreversible, mct):
+        check(lib().tc_forward_prepare(self.handle, w, h, ncomp, bits, levels, 1 if irreversible else 0,
+                                       1 if mct else 0), "forward prepare")
+        self._geom = (w, h, ncomp, bits, irreversible)
+
+    def upload(self, img: np.ndarray):
+        w, h, nc, bits, _ = self._geom
+        a = np.ascontiguousarray(img, np.uint16 if bits > 8 else np.uint8)
+        assert a.size == w * h * nc
+        check(lib().tc_forward_upload(self.handle, a.ctypes.data), "forward upload")
+
+    def run(self):
+        check(lib().tc_forward_run(self.handle), "forward run")
+
+    def run_timed(self):
+        """Runs once and returns (first-level ms, total ms) from stream events."""
+        a, b = ctypes.c_float(), ctypes.c_float()
+        check(lib().tc_forward_run_timed(self.handle, ctypes.byref(a), ctypes.byref(b)), "forward run_timed")
+        return a.value, b.value
+
+    def sync(self):
+        check(lib().tc_forward_sync(self.handle), "forward sync")
+
+    def download(self) -> np.ndarray:
+        w, h, nc, _, irr = self._geom
+        out = np.empty((nc, h, w), np.float32 if irr else np.int32)
+        check(lib().tc_forward_download(self.handle, out.ctypes.data), "forward download")
+        return out
+
+    def stream_handle(self):
+        return lib().tc_forward_stream(self.handle)
+
+
+def forward_transform(img: np.ndarray, levels=5, irreversible=False, mct=None, bits=None,
+                      device: "Device" = None) -> np.ndarray:
+    """Encoder front end on the GPU: img [H, W] or [H, W, C] (uint8/uint16) ->
+    [C, H, W] planes (int32 for 5/3 + RCT, float32 for 9/7 + ICT), each in the
+    Mallat layout of an L-level forward DWT."""
+    a = np.asarray(img)
+    if a.ndim == 2:
+        a = a[..., None]
+    h, w, c = a.shape
+    if bits is None:
+        bits = 8 if a.dtype == np.uint8 else 16
+    if mct is None:
+        mct = c == 3
+    f = Forward(device)
+    try:
+        f.prepare(w, h, c, bits, levels, irreversible, mct)
+        f.upload(a)
+        f.run()
+        f.sync()
+        return f.download()
+    finally:
+        f.close()
+
+
+def dwt_plane(a: np.ndarray, levels, forward=True, device: "Device" = None) -> np.ndarray:
+    """One plane through the DWT drop-in classes (DWTForward53/97 or
+    DWTReverse53/97 by direction and dtype: int32 5/3, float32 9/7)."""
+    a = np.ascontiguousarray(a)
+    irr = a.dtype == np.float32
+    if not irr:
+        a = np.ascontiguousarray(a, np.int32)
+    out = np.empty_like(a)
+    dev = device or default_device()
+    h, w = a.shape
+    check(lib().tc_dwt_plane(dev.handle, 1 if forward else 0, 1 if irr else 0, a.ctypes.data, out.ctypes.data, w, h,
+                             levels), "tc_dwt_plane")
+    return out

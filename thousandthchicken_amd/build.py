@@ -22,12 +22,14 @@ SOURCES = [
     "kernels/t1_lockstep.hip",
     "kernels/idwt.hip",
     "kernels/colour.hip",
+    "kernels/fdwt.hip",
     "host/codestream.cpp",
     "host/pipeline.cpp",
     "host/plan_util.cpp",
     "host/decoder_api.cpp",
     "host/stages.cpp",
     "host/server.cpp",
+    "host/forward.cpp",
 ]
 
 HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")

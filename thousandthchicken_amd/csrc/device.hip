@@ -30,6 +30,11 @@ hipError_t launch_idwt_level(hipStream_t s, int shape, bool is97, const tc_dwt_j
                              void* base);
 hipError_t launch_idwt_final(hipStream_t s, int shape, bool is97, int nc, int out_bytes, const tc_final_job* jobs,
                              const uint32_t* wg_job, int nwg, const void* base, void* out);
+hipError_t launch_fdwt_level(hipStream_t s, int shape, bool is97, const tc_fdwt_job* jobs, const uint32_t* wg_job,
+                             int nwg, void* base);
+hipError_t launch_fdwt_first(hipStream_t s, int shape, bool is97, int nc, int in_bytes,
+                             const tc_fdwt_first_job* jobs, const uint32_t* wg_job, int nwg, const void* img,
+                             void* base);
 hipError_t launch_fill32(hipStream_t s, uint32_t* p, uint32_t v, uint64_t n);
 hipError_t launch_colour(hipStream_t s, int kind, void* c0, void* c1, void* c2, size_t n, int shift, int minv,
                          int maxv);
@@ -438,6 +443,24 @@ tc_status tc_launch_idwt_final(tc_stream s, int irr, int shape, int nc, int ob, 
     return hip_fail(launch_idwt_final(s->s, shape, irr != 0, nc, ob, (const tc_final_job*)jobs, wg_job, nwg, base,
                                       out),
                     "idwt_final launch");
+}
+
+tc_status tc_launch_fdwt_level(tc_stream s, int irr, int shape, const void* jobs, const uint32_t* wg_job, int nwg,
+                               void* base) {
+    if (!s || nwg < 0 || (nwg > 0 && (!jobs || !wg_job || !base)) || shape < 0 || shape >= idwt_shape_count())
+        return fail(TC_ERR_INVALID_ARG, "tc_launch_fdwt_level");
+    return hip_fail(launch_fdwt_level(s->s, shape, irr != 0, (const tc_fdwt_job*)jobs, wg_job, nwg, base),
+                    "fdwt_level launch");
+}
+
+tc_status tc_launch_fdwt_first(tc_stream s, int irr, int shape, int nc, int in_bytes, const void* jobs,
+                               const uint32_t* wg_job, int nwg, const void* img, void* base) {
+    if (!s || nwg < 0 || (nwg > 0 && (!jobs || !wg_job || !base || !img)) || (nc != 1 && nc != 3) ||
+        (in_bytes != 1 && in_bytes != 2) || shape < 0 || shape >= idwt_shape_count())
+        return fail(TC_ERR_INVALID_ARG, "tc_launch_fdwt_first");
+    return hip_fail(launch_fdwt_first(s->s, shape, irr != 0, nc, in_bytes, (const tc_fdwt_first_job*)jobs, wg_job,
+                                      nwg, img, base),
+                    "fdwt_first launch");
 }
 
 tc_status tc_launch_colour(tc_stream s, int kind, void* c0, void* c1, void* c2, size_t n, int32_t shift,

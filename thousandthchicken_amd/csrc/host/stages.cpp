@@ -15,7 +15,10 @@
 #include "../../../include/tc/Decoder.h"
 #include "../../../include/tc_decode.h"
 #include "../tc_internal.hpp"
+#include "../../../include/tc/DWTForward53.h"
+#include "../../../include/tc/DWTForward97.h"
 #include "codestream.hpp"
+#include "forward.hpp"
 #include "plan_util.hpp"
 
 namespace tc {
@@ -191,9 +194,65 @@ tDeviceRC DWTKernel<T>::run(tDeviceMem in, tDeviceMem out, int sizeX, int sizeY,
 }
 
 template <typename T>
+tDeviceRC DWTKernel<T>::run_forward(tDeviceMem in, tDeviceMem out, int sizeX, int sizeY, int levels) {
+    dimX = sizeX;
+    dimY = sizeY;
+    srcMem = in;
+    dstMem = out;
+    if (levels == 0) return tc_buf_copy_d2d(queue_, out, 0, in, 0, (size_t)sizeX * sizeY * 4);
+    const size_t half = (size_t)((sizeX + 1) / 2) * ((sizeY + 1) / 2);
+    const size_t wbytes = (align_up(half, 64) * 2) * 4 + 256;
+    if (!work_ || tc_buf_size(work_) < wbytes) {
+        tc_stream_sync(queue_);
+        tc_buf_free(work_);
+        work_ = nullptr;
+        tc_status s = tc_buf_alloc(tc_buf_device(in), wbytes, &work_);
+        if (s) return s;
+    }
+    void* base = std::min(std::min(tc_buf_device_ptr(in), tc_buf_device_ptr(out)), tc_buf_device_ptr(work_));
+    const uint64_t w0 = eoff(base, tc_buf_device_ptr(work_));
+    const uint64_t wk[2] = {w0, w0 + align_up(half, 64)};
+    const int TX = idwt_tile_w(0), TY = idwt_tile_h(0);
+    std::vector<std::vector<tc_fdwt_job>> lv;
+    plan_forward_levels(sizeX, sizeY, levels, 1, eoff(base, tc_buf_device_ptr(in)), (uint32_t)sizeX,
+                        eoff(base, tc_buf_device_ptr(out)), wk, TX, TY, lv);
+    std::vector<uint8_t> blob;
+    std::vector<size_t> joff(levels + 1), moff(levels + 1);
+    std::vector<int> nwg(levels + 1, 0);
+    for (int n = 1; n <= levels; n++) {
+        tc_fdwt_job& j = lv[n][0];
+        j.wg_base = 0;
+        nwg[n] = j.tiles_x * ((j.v1 - j.v0 + TY - 1) / TY);
+        joff[n] = align_up(blob.size(), 256);
+        blob.resize(joff[n] + sizeof(tc_fdwt_job));
+        std::memcpy(blob.data() + joff[n], &j, sizeof j);
+        moff[n] = align_up(blob.size(), 256);
+        blob.resize(moff[n] + (size_t)nwg[n] * 4, 0);
+    }
+    if (!jobs_ || tc_buf_size(jobs_) < blob.size()) {
+        tc_stream_sync(queue_);
+        tc_buf_free(jobs_);
+        jobs_ = nullptr;
+        tc_status s = tc_buf_alloc(tc_buf_device(in), blob.size() + 256, &jobs_);
+        if (s) return s;
+    }
+    tc_status s = tc_buf_copy_h2d(queue_, jobs_, 0, blob.data(), blob.size());
+    if (!s) s = tc_stream_sync(queue_);
+    uint8_t* jp = (uint8_t*)tc_buf_device_ptr(jobs_);
+    for (int n = 1; n <= levels && !s; n++)
+        s = tc_launch_fdwt_level(queue_, irreversible() ? 1 : 0, 0, jp + joff[n], (const uint32_t*)(jp + moff[n]),
+                                 nwg[n], base);
+    return s;
+}
+
+template <typename T>
 tDeviceRC DWTKernel<T>::run_at(tDeviceMem in, tDeviceMem out, int x0, int y0, int sizeX, int sizeY, int levels) {
     if (!in || !out || sizeX <= 0 || sizeY <= 0 || levels < 0 || levels > 32) return TC_ERR_INVALID_ARG;
     if (tc_buf_size(in) < (size_t)sizeX * sizeY * 4 || tc_buf_size(out) < (size_t)sizeX * sizeY * 4) return TC_ERR_INVALID_ARG;
+    if (forward()) {
+        if (x0 || y0) return TC_ERR_UNSUPPORTED;  // forward classes take origin (0, 0)
+        return run_forward(in, out, sizeX, sizeY, levels);
+    }
     dimX = sizeX;
     dimY = sizeY;
     srcMem = in;
@@ -309,6 +368,8 @@ T* DWTKernel<T>::mapOutputBufferToHost() {
 template class DWTKernel<int>;
 template class DWTKernel<float>;
 
+DWTForward53::DWTForward53(KernelInitInfoBase initInfo) : DWTKernel<int>(3, KernelInitInfo(initInfo, "fdwt.hip", "fdwt_level_kernel<false>")) {}
+DWTForward97::DWTForward97(KernelInitInfoBase initInfo) : DWTKernel<float>(9, KernelInitInfo(initInfo, "fdwt.hip", "fdwt_level_kernel<true>")) {}
 DWTReverse53::DWTReverse53(KernelInitInfoBase initInfo) : DWTKernel<int>(5, KernelInitInfo(initInfo, "idwt.hip", "idwt_level_kernel<false>")) {}
 DWTReverse97::DWTReverse97(KernelInitInfoBase initInfo) : DWTKernel<float>(9, KernelInitInfo(initInfo, "idwt.hip", "idwt_level_kernel<true>")) {}
 
@@ -588,4 +649,45 @@ extern "C" tc_status tc_decode_staged(tc_device dev, const uint8_t* cs, size_t l
     tc_stream_destroy(q);
     if (rc == -2) return fail(TC_ERR_CAPACITY, "tc_decode_staged: output buffer too small");
     return rc == 0 ? TC_SUCCESS : TC_ERR_DEVICE;
+}
+
+// Runs one DWT drop-in class over a host plane (DWTKernel::run(T*, ...) then
+// mapOutputBufferToHost, TC/DWTKernel.cpp:45-78,180-191).
+extern "C" tc_status tc_dwt_plane(tc_device dev, int fwd, int irr, const void* in, void* out, int w, int h,
+                                  int levels) {
+    if (!dev || !in || !out || w <= 0 || h <= 0 || levels < 0 || levels > 32)
+        return fail(TC_ERR_INVALID_ARG, "tc_dwt_plane");
+    tc_stream q = nullptr;
+    tc_status s = tc_stream_create(dev, &q);
+    if (s) return s;
+    KernelInitInfoBase info(q, "");
+    const size_t bytes = (size_t)w * h * 4;
+    auto go = [&](auto& k, auto* src) -> tc_status {
+        tc_status r = k.run(src, w, h, levels);
+        if (r) return r;
+        const void* res = k.mapOutputBufferToHost();
+        if (!res) return fail(TC_ERR_DEVICE, "tc_dwt_plane: map");
+        std::memcpy(out, res, bytes);
+        return TC_SUCCESS;
+    };
+    std::vector<uint8_t> copy((const uint8_t*)in, (const uint8_t*)in + bytes);
+    try {
+        if (fwd && !irr) {
+            DWTForward53 k(info);
+            s = go(k, (int*)copy.data());
+        } else if (fwd) {
+            DWTForward97 k(info);
+            s = go(k, (float*)copy.data());
+        } else if (!irr) {
+            DWTReverse53 k(info);
+            s = go(k, (int*)copy.data());
+        } else {
+            DWTReverse97 k(info);
+            s = go(k, (float*)copy.data());
+        }
+    } catch (const Error& e) {
+        s = fail(TC_ERR_DEVICE, e.what());
+    }
+    tc_stream_destroy(q);
+    return s;
 }

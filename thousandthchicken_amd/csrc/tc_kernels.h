@@ -63,6 +63,36 @@ typedef struct tc_final_job {
     int32_t mct;            // 1: apply RCT (5/3) / ICT (9/7) to components 0..2
 } tc_final_job;
 
+// One level of a 2-D forward DWT over one tile-component (encoder side): splits
+// the resolution rectangle [u0,u1) x [v0,v1), a raster at in_off with in_pitch
+// elements per row, into its LL/HL/LH/HH subbands.  Offsets are in elements
+// from the kernel's base pointer.
+typedef struct tc_fdwt_job {
+    uint64_t in_off, ll_off, hl_off, lh_off, hh_off;
+    uint32_t in_pitch, ll_pitch, band_pitch;
+    int32_t u0, u1, v0, v1;
+    int32_t tiles_x;     // work-group tiles across
+    int32_t wg_base;     // first work-group index of this job in the launch
+    int32_t pad;
+} tc_fdwt_job;
+
+// First forward level fused with the DC level shift and the forward RCT (5/3)
+// or ICT (9/7): reads the interleaved 8/16-bit image and writes the level-1
+// subbands of components 0..nc-1.  levels == 0 writes the shifted (and
+// colour-transformed) samples to ll_off instead.
+typedef struct tc_fdwt_first_job {
+    uint64_t ll_off[3], hl_off[3], lh_off[3], hh_off[3];
+    uint32_t ll_pitch[3], band_pitch[3];
+    int32_t shift[3];
+    int32_t u0, u1, v0, v1;
+    uint64_t in_off;        // image element index of (u0,v0), first component
+    uint32_t in_pitch;      // image elements per row
+    uint32_t in_cstride;    // image elements between adjacent pixels
+    int32_t tiles_x, wg_base;
+    int32_t levels;         // 0: no DWT, the samples go to ll_off
+    int32_t mct;            // 1: forward RCT (5/3) / ICT (9/7) on components 0..2
+} tc_fdwt_first_job;
+
 #ifdef __cplusplus
 }
 #endif
