@@ -64,7 +64,7 @@ struct FLift<false> {
         const int half = (n + 1) / 2;
         // STEP1: odd -= floor((even_l + even_r) / 2)
         for (int i = tid; i < nlines * half; i += NT) {
-            const int line = i / half, j = i % half;
+            const int line = i % nlines, j = i / nlines;
             const int pos = 2 * j + 1 - (par0 & 1);
             if (pos >= 1 && pos < n - 1) {
                 int32_t* p = base + line * sl + pos * sp;
@@ -74,7 +74,7 @@ struct FLift<false> {
         __syncthreads();
         // STEP2: even += floor((odd_l + odd_r + 2) / 4)
         for (int i = tid; i < nlines * half; i += NT) {
-            const int line = i / half, j = i % half;
+            const int line = i % nlines, j = i / nlines;
             const int pos = 2 * j + (par0 & 1);
             if (pos >= 2 && pos < n - 2) {
                 int32_t* p = base + line * sl + pos * sp;
@@ -94,7 +94,7 @@ struct FLift<true> {
     __device__ static void step(float* base, int sp, int sl, int nlines, int n, int par, int lo, float c, int tid) {
         const int half = (n + 1) / 2;
         for (int i = tid; i < nlines * half; i += NT) {
-            const int line = i / half, j = i % half;
+            const int line = i % nlines, j = i / nlines;
             const int pos = 2 * j + par;
             if (pos >= lo && pos < n - lo) {
                 float* p = base + line * sl + pos * sp;

@@ -85,7 +85,7 @@ struct Lift<false> {
         int half = (n + 1) / 2;
         // STEP1: even -= floor((odd_l + odd_r + 2) / 4)
         for (int i = tid; i < nlines * half; i += NT) {
-            int line = i / half, j = i % half;
+            int line = i % nlines, j = i / nlines;
             int pos = 2 * j + (par0 & 1);  // absolute-even positions
             if (pos >= 1 && pos < n - 1) {
                 int32_t* p = base + line * stride_line + pos * stride_pos;
@@ -95,7 +95,7 @@ struct Lift<false> {
         __syncthreads();
         // STEP2: odd += floor((even_l + even_r) / 2)
         for (int i = tid; i < nlines * half; i += NT) {
-            int line = i / half, j = i % half;
+            int line = i % nlines, j = i / nlines;
             int pos = 2 * j + 1 - (par0 & 1);
             if (pos >= 2 && pos < n - 2) {
                 int32_t* p = base + line * stride_line + pos * stride_pos;
@@ -114,7 +114,7 @@ struct Lift<true> {
                                 int lo, float c, int tid) {
         int half = (n + 1) / 2;
         for (int i = tid; i < nlines * half; i += NT) {
-            int line = i / half, j = i % half;
+            int line = i % nlines, j = i / nlines;
             int pos = 2 * j + par;
             if (pos >= lo && pos < n - lo) {
                 float* p = base + line * stride_line + pos * stride_pos;
