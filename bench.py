@@ -287,6 +287,41 @@ def bench_forward(tca, steps):
     return res
 
 
+def bench_region(tca, steps):
+    """Region decode of the cfg3 codestream (3840x2160 12-bit lossless, twelve
+    1024x1024 tiles): resident decode time of windows against the full image,
+    with the code-blocks decoded.  Each window is checked against the same
+    window of the full decode first (bit-exact)."""
+    from thousandthchicken_amd import workloads
+    cs = workloads.cfg3()
+    full_dec = tca.Decoder()
+    full = full_dec.decode(cs)
+    full_dec.prepare([cs])
+    full_dec.run()
+    full_dec.sync()
+    full_ms = timed_runs(full_dec, steps) * 1e3
+    full_blocks = full_dec.stats().cblks
+    full_dec.close()
+    res = {"full": {"ms": round(full_ms, 3), "cblks": full_blocks}}
+    for name, (x, y, w, h) in {"center_512": (1664, 824, 512, 512), "quarter_1920x1080": (0, 0, 1920, 1080),
+                               "strip_3840x64": (0, 1000, 3840, 64)}.items():
+        dec = tca.Decoder()
+        dec.set_region(x, y, w, h)
+        out = dec.decode(cs)
+        if not np.array_equal(out, full[y:y + h, x:x + w]):
+            raise SystemExit(f"region {name} differs from the full decode")
+        dec.prepare([cs])
+        dec.run()
+        dec.sync()
+        ms = timed_runs(dec, steps) * 1e3
+        res[name] = {"window": [x, y, w, h], "ms": round(ms, 3), "cblks": dec.stats().cblks,
+                     "speedup_vs_full": round(full_ms / ms, 2), "output_mpixel_s": round(w * h / ms / 1e3, 1)}
+        dec.close()
+    res["note"] = ("cfg3 codestream resident in HBM; only the code-blocks whose lifting support reaches the window "
+                   "are decoded, the inverse DWT still runs over every tile")
+    return res
+
+
 def bench_served(tca, loads, n_req, args):
     from thousandthchicken_amd import serve, workloads
     t = time.perf_counter()
@@ -365,6 +400,7 @@ def main():
     ap.add_argument("--profile", action="store_true", help="workload steps and a calibration copy only")
     ap.add_argument("--forward-only", action="store_true",
                     help="only the GPU forward transform (encoder front end) leg")
+    ap.add_argument("--region-only", action="store_true", help="only the region-decode leg")
     ap.add_argument("--loads", default="50,100,1000,5000", help="offered loads (req/s) of the serving run")
     ap.add_argument("--requests", type=int, default=300, help="requests per serving run")
     ap.add_argument("--bpw-large", type=int, default=0, help="blocks per wave, blocks up to 64x64 (0 auto, -64/-32 lockstep)")
@@ -382,6 +418,9 @@ def main():
         return profile_mode(tca, args)
     if args.forward_only:
         print(json.dumps({"forward": bench_forward(tca, args.steps)}), flush=True)
+        return
+    if args.region_only:
+        print(json.dumps({"region": bench_region(tca, args.steps)}), flush=True)
         return
 
     cs, src = workload(args.workload)
@@ -453,6 +492,7 @@ def main():
     if not args.quick:
         line["reduced"] = bench_reduced(tca, cs, max(5, args.steps // 2))
         line["forward"] = bench_forward(tca, max(5, args.steps // 2))
+        line["region"] = bench_region(tca, max(5, args.steps // 5))
         line["batched"] = bench_batched(tca, max(3, args.steps // 5), args)
         loads = [float(x) for x in args.loads.split(",") if x]
         if not args.no_served:

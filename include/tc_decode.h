@@ -92,6 +92,15 @@ tc_status tc_decoder_set_blocks_per_wave(tc_decoder dec, int small_blocks, int b
  * with a tile-component of fewer levels. */
 tc_status tc_decoder_set_reduce(tc_decoder dec, int reduce);
 
+/* Region decode (applies from the next prepare or decode; no counterpart in
+ * the reference, the "decode area" of JPEG 2000 decoders): only the w x h
+ * window at (x, y) from the image origin is output, clipped to the image (of
+ * the reduced image when a reduce is set), and only the code-blocks that can
+ * influence it are decoded.  The window's samples equal those of the full
+ * decode.  w or h 0 decodes the whole image.  The output of every image of a
+ * batch is its window, row-major and interleaved like the full image. */
+tc_status tc_decoder_set_region(tc_decoder dec, int x, int y, int w, int h);
+
 /* Kernel for blocks decoded one per wavefront: 0 = stripe-column form
  * (tc_launch_t1_decode_stripe, default), 1 = row-mask form
  * (tc_launch_t1_decode_wave). */

@@ -77,6 +77,7 @@ def lib():
         "tc_image_probe": (i32, [ctypes.c_char_p, sz, P(ImageInfo)]),
         "tc_image_probe_reduced": (i32, [ctypes.c_char_p, sz, i32, P(ImageInfo)]),
         "tc_decoder_set_reduce": (i32, [vp, i32]),
+        "tc_decoder_set_region": (i32, [vp, i32, i32, i32, i32]),
         "tc_image_list_cblks": (i32, [ctypes.c_char_p, sz, vp, i32, P(i32)]),
         "tc_decoder_create": (i32, [vp, P(vp)]),
         "tc_decoder_destroy": (i32, [vp]),
@@ -406,6 +407,7 @@ class Decoder:
         self._infos = []
         self._keep = None
         self._reduce = 0
+        self._region = None
 
     def close(self):
         if getattr(self, "handle", None):
@@ -440,11 +442,24 @@ class Decoder:
         check(lib().tc_decoder_set_reduce(self.handle, reduce), "set_reduce")
         self._reduce = reduce
 
+    def set_region(self, x=0, y=0, w=0, h=0):
+        """Decode only the w x h window at (x, y) (clipped to the image); 0 sizes: the whole image."""
+        check(lib().tc_decoder_set_region(self.handle, x, y, w, h), "set_region")
+        self._region = (x, y, w, h) if w > 0 and h > 0 else None
+
+    def _info(self, cs):
+        info = probe(cs, self._reduce)
+        if self._region:
+            x, y, w, h = self._region
+            info.width = max(0, min(w, info.width - x))
+            info.height = max(0, min(h, info.height - y))
+        return info
+
     def set_blocks_per_wave(self, small, bpw):
         check(lib().tc_decoder_set_blocks_per_wave(self.handle, 1 if small else 0, bpw), "set_blocks_per_wave")
 
     def decode(self, cs: bytes) -> np.ndarray:
-        info = probe(cs, self._reduce)
+        info = self._info(cs)
         shape, dt = _shape(info)
         out = np.empty(shape, dt)
         check(lib().tc_decoder_decode(self.handle, cs, len(cs), out.ctypes.data, out.nbytes, None), "decode")
@@ -452,7 +467,7 @@ class Decoder:
 
     def prepare(self, streams):
         streams = [bytes(s) for s in streams]
-        self._infos = [probe(s, self._reduce) for s in streams]
+        self._infos = [self._info(s) for s in streams]
         n = len(streams)
         ptrs = (ctypes.c_char_p * n)(*streams)
         lens = (ctypes.c_size_t * n)(*[len(s) for s in streams])

@@ -2,6 +2,7 @@
 #include "codestream.hpp"
 
 #include <algorithm>
+#include <array>
 #include <tuple>
 #include <cmath>
 #include <cstring>
@@ -832,6 +833,10 @@ Status parse_impl(const uint8_t* data, size_t len, Image& img, std::string& err,
     Reader r(cs, cl);
     Params main;
     Status s = parse_main_header(r, img, main, err);
+    img.wx0 = img.x0;
+    img.wy0 = img.y0;
+    img.wx1 = img.x1;
+    img.wy1 = img.y1;
     if (s != Status::Ok || !packets) {
         if (s == Status::Ok) {
             // geometry of tile 0 only, for probing
@@ -1016,6 +1021,53 @@ Status reduce_image(Image& img, int k, std::string& err) {
     img.y0 = cdiv(img.y0);
     img.x1 = cdiv(img.x1);
     img.y1 = cdiv(img.y1);
+    img.wx0 = img.x0;
+    img.wy0 = img.y0;
+    img.wx1 = img.x1;
+    img.wy1 = img.y1;
+    return Status::Ok;
+}
+
+Status window_image(Image& img, int x, int y, int w, int h, std::string& err) {
+    if (x < 0 || y < 0 || w <= 0 || h <= 0 || x >= img.width() || y >= img.height()) {
+        err = "region outside the image";
+        return Status::InvalidArgument;
+    }
+    img.wx0 = img.x0 + x;
+    img.wy0 = img.y0 + y;
+    img.wx1 = (int)std::min<int64_t>((int64_t)img.wx0 + w, img.x1);
+    img.wy1 = (int)std::min<int64_t>((int64_t)img.wy0 + h, img.y1);
+    // needed band ranges per tile-component and level: [a[n], b[n]) from
+    // [a[n-1] - 4, b[n-1] + 4) at the next finer resolution, halved
+    auto fdiv2 = [](int64_t v) { return v >= 0 ? v / 2 : -((-v + 1) / 2); };
+    std::vector<std::vector<std::array<int64_t, 4>>> need(img.tiles.size());
+    for (size_t t = 0; t < img.tiles.size(); t++) {
+        const Tile& T = img.tiles[t];
+        need[t].resize(T.comps.size() * 33);
+        for (size_t c = 0; c < T.comps.size(); c++) {
+            const TileComp& tc = T.comps[c];
+            int64_t ax = std::max(img.wx0, tc.x0), bx = std::min(img.wx1, tc.x1);
+            int64_t ay = std::max(img.wy0, tc.y0), by = std::min(img.wy1, tc.y1);
+            const bool empty = ax >= bx || ay >= by;
+            for (int n = 0; n <= tc.nlev; n++) {
+                if (n > 0) {
+                    ax = fdiv2(ax - 4);
+                    bx = -fdiv2(-(bx + 4));
+                    ay = fdiv2(ay - 4);
+                    by = -fdiv2(-(by + 4));
+                }
+                need[t][c * 33 + n] = empty ? std::array<int64_t, 4>{0, 0, 0, 0}
+                                            : std::array<int64_t, 4>{ax, bx, ay, by};
+            }
+        }
+    }
+    for (CodeBlock& cb : img.cblks) {
+        if (!cb.included) continue;
+        const TileComp& tc = img.tiles[cb.tile].comps[cb.comp];
+        const int n = cb.res == 0 ? tc.nlev : tc.nlev - cb.res + 1;
+        const auto& q = need[cb.tile][cb.comp * 33 + n];
+        if (!(cb.x0 < q[1] && cb.x1 > q[0] && cb.y0 < q[3] && cb.y1 > q[2])) cb.included = false;
+    }
     return Status::Ok;
 }
 

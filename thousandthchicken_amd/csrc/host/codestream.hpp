@@ -98,8 +98,13 @@ struct Image {
     std::vector<Segment> segs;  // in codestream order; contiguous per block when nseg == 1
     const uint8_t* cs = nullptr;  // codestream (inside a JP2 wrapper if any)
     size_t cs_len = 0;
+    // output window on the reference grid: the whole image unless
+    // window_image narrowed it
+    int wx0 = 0, wy0 = 0, wx1 = 0, wy1 = 0;
     int width() const { return x1 - x0; }
     int height() const { return y1 - y0; }
+    int out_width() const { return wx1 - wx0; }
+    int out_height() const { return wy1 - wy0; }
     int max_prec() const;
     bool all_reversible() const;
 };
@@ -118,6 +123,14 @@ Status probe_codestream(const uint8_t* data, size_t len, Image& img, std::string
 // to that resolution's coordinates, ceil(x / 2^k) (T.800 B.5).  Fails when a
 // tile-component has fewer than k levels.
 Status reduce_image(Image& img, int k, std::string& err);
+
+// Region decode: narrows the output window to the w x h rectangle at (x, y)
+// from the image origin (clipped to the image) and drops every code-block
+// that cannot influence it.  Per decomposition level the needed band range is
+// the next finer range mapped down with a margin of 4 samples on each side
+// (the longest lifting support, 9/7), so every sample inside the window is
+// reconstructed exactly as in the full decode.  Apply after reduce_image.
+Status window_image(Image& img, int x, int y, int w, int h, std::string& err);
 
 const char* status_name(Status s);
 

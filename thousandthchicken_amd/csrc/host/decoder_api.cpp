@@ -18,6 +18,7 @@ struct tc_decoder_s {
     std::vector<std::vector<uint8_t>> data;  // owned copies of the prepared codestreams
     std::vector<std::unique_ptr<Image>> imgs;
     int reduce = 0;  // finest DWT levels not reconstructed
+    int region[4] = {0, 0, 0, 0};  // x, y, w, h of the output window (w or h 0: whole image)
 };
 
 namespace {
@@ -45,10 +46,10 @@ tc_status report(Status s, const std::string& err) {
 
 void fill_info(const Image& im, tc_image_info* info) {
     std::memset(info, 0, sizeof *info);
-    info->x0 = im.x0;
-    info->y0 = im.y0;
-    info->width = im.width();
-    info->height = im.height();
+    info->x0 = im.wx0;
+    info->y0 = im.wy0;
+    info->width = im.out_width();
+    info->height = im.out_height();
     info->ncomp = im.ncomp;
     info->max_prec = im.max_prec();
     info->bytes_per_sample = info->max_prec > 8 ? 2 : 1;
@@ -193,6 +194,15 @@ tc_status tc_decoder_set_reduce(tc_decoder d, int reduce) {
     return TC_SUCCESS;
 }
 
+tc_status tc_decoder_set_region(tc_decoder d, int x, int y, int w, int h) {
+    if (!d || x < 0 || y < 0 || w < 0 || h < 0) return report(Status::InvalidArgument, "tc_decoder_set_region");
+    d->region[0] = x;
+    d->region[1] = y;
+    d->region[2] = w;
+    d->region[3] = h;
+    return TC_SUCCESS;
+}
+
 tc_status tc_decoder_set_t1_kernel(tc_decoder d, int kind) {
     if (!d || kind < 0 || kind > 1) return report(Status::InvalidArgument, "t1 kernel must be 0 (stripe) or 1 (row)");
     Tuning t = d->pipe->tuning();
@@ -326,6 +336,8 @@ tc_status tc_decoder_prepare(tc_decoder d, int n, const uint8_t* const* cs, cons
         std::string err;
         Status s = parse_codestream(d->data.back().data(), d->data.back().size(), *d->imgs.back(), err);
         if (s == Status::Ok) s = reduce_image(*d->imgs.back(), d->reduce, err);
+        if (s == Status::Ok && d->region[2] > 0 && d->region[3] > 0)
+            s = window_image(*d->imgs.back(), d->region[0], d->region[1], d->region[2], d->region[3], err);
         if (s != Status::Ok) return report(s, "image " + std::to_string(i) + ": " + err);
         v.push_back(d->imgs.back().get());
     }

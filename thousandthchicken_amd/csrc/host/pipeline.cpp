@@ -279,11 +279,12 @@ Status Pipeline::prepare(const std::vector<const Image*>& imgs, std::string& err
         ImageOut io;
         io.bps = im.max_prec() > 8 ? 2 : 1;
         io.off = align_up(out_bytes_total, 256);
-        io.bytes = (uint64_t)im.width() * im.height() * im.ncomp * io.bps;
+        const int ow = im.out_width(), oh = im.out_height();
+        io.bytes = (uint64_t)ow * oh * im.ncomp * io.bps;
         out_bytes_total = io.off + io.bytes;
         images_.push_back(io);
-        npixels_ += (int64_t)im.width() * im.height();
-        nsamples_ += (int64_t)im.width() * im.height() * im.ncomp;
+        npixels_ += (int64_t)ow * oh;
+        nsamples_ += (int64_t)ow * oh * im.ncomp;
         uint64_t out_elem_base = io.off / io.bps;
         for (size_t t = 0; t < im.tiles.size(); t++) {
             const Tile& T = im.tiles[t];
@@ -301,12 +302,20 @@ Status Pipeline::prepare(const std::vector<const Image*>& imgs, std::string& err
                 const TileComp& tc0 = T.comps[set[0]];
                 const Resolution& R = tc0.res[tc0.nlev];
                 if (R.width() <= 0 || R.height() <= 0) continue;
+                // the part of the output window this tile covers
+                const int jx0 = std::max(R.x0, im.wx0), jx1 = std::min(R.x1, im.wx1);
+                const int jy0 = std::max(R.y0, im.wy0), jy1 = std::min(R.y1, im.wy1);
+                if (jx1 <= jx0 || jy1 <= jy0) continue;
                 tc_final_job f;
                 std::memset(&f, 0, sizeof f);
                 f.u0 = R.x0;
                 f.u1 = R.x1;
                 f.v0 = R.y0;
                 f.v1 = R.y1;
+                f.wx0 = jx0;
+                f.wx1 = jx1;
+                f.wy0 = jy0;
+                f.wy1 = jy1;
                 f.levels = tc0.nlev;
                 f.mct = set.size() == 3 ? 1 : 0;
                 for (size_t k = 0; k < set.size(); k++) {
@@ -342,19 +351,19 @@ Status Pipeline::prepare(const std::vector<const Image*>& imgs, std::string& err
                         f.maxv[k] = (1 << prec) - 1;
                     }
                 }
-                f.out_off = out_elem_base + ((uint64_t)(R.y0 - im.y0) * im.width() + (uint64_t)(R.x0 - im.x0)) * im.ncomp +
+                f.out_off = out_elem_base + ((uint64_t)(jy0 - im.wy0) * ow + (uint64_t)(jx0 - im.wx0)) * im.ncomp +
                             (uint64_t)set[0];
-                f.out_pitch = (uint32_t)((uint64_t)im.width() * im.ncomp);
+                f.out_pitch = (uint32_t)((uint64_t)ow * im.ncomp);
                 f.out_cstride = (uint32_t)im.ncomp;
                 int irr = tc0.reversible ? 0 : 1, nc = (int)set.size();
-                f.tiles_x = (R.width() + TX - 1) / TX;
-                int tiles_y = (R.height() + TY - 1) / TY;
+                f.tiles_x = (jx1 - jx0 + TX - 1) / TX;
+                int tiles_y = (jy1 - jy0 + TY - 1) / TY;
                 int key = key_index(irr, nc, io.bps);
                 f.wg_base = (int32_t)fmaps[key].size();
                 uint32_t jid = (uint32_t)fjobs[key].size();
                 for (int q = 0; q < f.tiles_x * tiles_y; q++) fmaps[key].push_back(jid);
                 fjobs[key].push_back(f);
-                final_bytes_ += (int64_t)R.width() * R.height() * nc * (4 + io.bps);
+                final_bytes_ += (int64_t)(jx1 - jx0) * (jy1 - jy0) * nc * (4 + io.bps);
             }
         }
     }

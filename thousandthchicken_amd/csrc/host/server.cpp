@@ -64,7 +64,7 @@ uint64_t footprint_of(const Image& im) {
             f += ((uint64_t)c.width() * c.height() + sz[0] + sz[1] + 192) * 4;
         }
     f *= (uint64_t)std::max<size_t>(im.tiles.size(), 1);
-    f += (uint64_t)im.width() * im.height() * im.ncomp * (im.max_prec() > 8 ? 2 : 1);
+    f += (uint64_t)im.out_width() * im.out_height() * im.ncomp * (im.max_prec() > 8 ? 2 : 1);
     f += im.cs_len + im.cs_len / 16 * 40 + 4096;
     return f;
 }

@@ -254,7 +254,7 @@ __global__ __launch_bounds__(NT) void idwt_final_kernel(const tc_final_job* __re
     __shared__ T tile[NC * TS];
     const tc_final_job& J = jobs[wg_job[blockIdx.x]];
     int t = (int)blockIdx.x - J.wg_base;
-    int ox = J.u0 + (t % J.tiles_x) * TX, oy = J.v0 + (t / J.tiles_x) * TY;
+    int ox = J.wx0 + (t % J.tiles_x) * TX, oy = J.wy0 + (t / J.tiles_x) * TY;
 #pragma unroll
     for (int c = 0; c < NC; c++) {
         Src s{J.ll_off[c], J.hl_off[c], J.lh_off[c], J.hh_off[c], J.ll_pitch[c], J.band_pitch[c]};
@@ -264,9 +264,9 @@ __global__ __launch_bounds__(NT) void idwt_final_kernel(const tc_final_job* __re
     for (int i = threadIdx.x; i < TX * TY; i += NT) {
         int ly = i / TX, lx = i % TX;
         int u = ox + lx, v = oy + ly;
-        if (u >= J.u1 || v >= J.v1) continue;
+        if (u >= J.wx1 || v >= J.wy1) continue;
         int li = (ly + H) * PITCH + lx + H;
-        uint64_t o = J.out_off + (uint64_t)(v - J.v0) * J.out_pitch + (uint64_t)(u - J.u0) * J.out_cstride;
+        uint64_t o = J.out_off + (uint64_t)(v - J.wy0) * J.out_pitch + (uint64_t)(u - J.wx0) * J.out_cstride;
         if (NC == 3 && J.mct) {
             if (IS97) {
                 // inverse ICT, T.800 G.3 (not the YUV constants of TC/preprocess_ict_inverse.cl:31-35)

@@ -61,6 +61,10 @@ typedef struct tc_final_job {
     int32_t tiles_x, wg_base;
     int32_t levels;         // 0: copy through
     int32_t mct;            // 1: apply RCT (5/3) / ICT (9/7) to components 0..2
+    // output window inside [u0,u1) x [v0,v1): work-group tiles start at
+    // (wx0, wy0), only samples inside are written, and out_off addresses
+    // (wx0, wy0); the whole rectangle unless a region decode narrows it
+    int32_t wx0, wx1, wy0, wy1;
 } tc_final_job;
 
 // One level of a 2-D forward DWT over one tile-component (encoder side): splits
