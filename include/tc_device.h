@@ -153,8 +153,10 @@ tc_status tc_launch_t1_decode_stripe(tc_stream s, int small_blocks, int grid, co
  * start at any byte offset; arena is 16-byte aligned and arena_bytes (a
  * multiple of 16, at least 64) readable bytes cover every codeword.  This is
  * the only tier-1 launch that decodes the code-block style bits of each
- * descriptor's mode (RESET, CAUSAL, SEGSYM; T.800 Table A.19); the others
- * ignore them. */
+ * descriptor's mode (RESET, CAUSAL, SEGSYM, BYPASS, TERMALL; T.800 Table A.19);
+ * the others ignore them.  A BYPASS or TERMALL block's codeword starts 4-byte
+ * aligned, after its codeword-segment lengths and their count (one 4-byte word
+ * each, the count last). */
 size_t tc_t1_lockstep_scratch_bytes(int small_blocks, int blocks_per_wave, int n, int max_passes);
 tc_status tc_launch_t1_decode_lockstep(tc_stream s, int small_blocks, int blocks_per_wave, const void *descs,
                                        const uint32_t *order, int n, int max_passes, const uint8_t *arena,

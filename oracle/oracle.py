@@ -52,6 +52,8 @@ def lib():
         L.or_t1_decode_style.argtypes = [ctypes.c_char_p, ctypes.c_int, ctypes.c_int, ctypes.c_int,
                                          ctypes.c_int, ctypes.c_int, ctypes.c_int, ctypes.c_int,
                                          ctypes.c_void_p, ctypes.c_void_p, ctypes.c_void_p]
+        L.or_t1_decode_seg.argtypes = [ctypes.c_char_p, ctypes.c_int, ctypes.c_void_p, ctypes.c_int] + \
+            [ctypes.c_int] * 6 + [ctypes.c_void_p] * 3
         for f in ("or_idwt53", "or_fdwt53", "or_idwt97", "or_fdwt97"):
             getattr(L, f).argtypes = [ctypes.c_void_p] + [ctypes.c_int] * 5
         L.or_mq_decode_symbols.argtypes = [ctypes.c_char_p, ctypes.c_int, ctypes.c_char_p,
@@ -112,15 +114,17 @@ def symbol_counts(cs: bytes):
 
 
 # Code-block style bits (T.800 Table A.19) the oracle decodes.
-STYLE_RESET, STYLE_CAUSAL, STYLE_PTERM, STYLE_SEGSYM = 2, 8, 16, 32
+STYLE_BYPASS, STYLE_RESET, STYLE_TERMALL, STYLE_CAUSAL, STYLE_PTERM, STYLE_SEGSYM = 1, 2, 4, 8, 16, 32
 
 
-def t1_decode(data: bytes, w, h, orient, numbps, npasses, style=0):
+def t1_decode(data: bytes, w, h, orient, numbps, npasses, style=0, seglen=None):
+    """seglen: byte lengths of the codeword segments (BYPASS / TERMALL), in order."""
     mag = np.zeros((h, w), np.int32)
     sign = np.zeros((h, w), np.int8)
     low = np.zeros((h, w), np.int8)
-    rc = lib().or_t1_decode_style(data, len(data), w, h, orient, numbps, npasses, style,
-                                  mag.ctypes.data, sign.ctypes.data, low.ctypes.data)
+    segs = np.asarray(seglen if seglen is not None else [len(data)], np.int32)
+    rc = lib().or_t1_decode_seg(data, len(data), segs.ctypes.data, len(segs), w, h, orient, numbps, npasses, style,
+                                mag.ctypes.data, sign.ctypes.data, low.ctypes.data)
     if rc < 0:
         raise OracleError(f"or_t1_decode failed ({rc})")
     return mag, sign, low, rc

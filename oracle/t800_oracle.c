@@ -49,6 +49,7 @@ typedef struct {
     int CT;
     uint8_t I[NCTX], MPS[NCTX];
     long nsym;
+    int raw; /* 1: the current segment is raw (BYPASS style, T.800 D.6) */
 } mqd_t;
 
 static int mq_byte(const mqd_t *m, int i) { return i < m->len ? m->d[i] : 0xFF; }
@@ -82,18 +83,47 @@ static void mq_reset_ctx(mqd_t *m) {
     m->I[0] = 4;
 }
 
-/* INITDEC, T.800 Figure C.20 (TC/coefficient_coder.cl:202-221). */
-static void mq_init(mqd_t *m, const uint8_t *d, int len) {
+/* INITDEC, T.800 Figure C.20 (TC/coefficient_coder.cl:202-221), on a new
+ * codeword segment; the context states are left as they are. */
+static void mq_start(mqd_t *m, const uint8_t *d, int len) {
     m->d = d;
     m->len = len;
     m->bp = 0;
+    m->raw = 0;
     m->C = (uint32_t)mq_byte(m, 0) << 16;
     mq_bytein(m);
     m->C <<= 7;
     m->CT -= 7;
     m->A = 0x8000;
+}
+
+static void mq_init(mqd_t *m, const uint8_t *d, int len) {
+    mq_start(m, d, len);
     m->nsym = 0;
     mq_reset_ctx(m);
+}
+
+/* A raw segment (T.800 D.6): bits MSB first; a byte after 0xFF holds 7 bits
+ * below a stuffed 0; past the end the decoder sees 0xFF. */
+static void raw_start(mqd_t *m, const uint8_t *d, int len) {
+    m->d = d;
+    m->len = len;
+    m->bp = -1;
+    m->raw = 1;
+    m->C = 0;
+    m->CT = 0;
+}
+
+static int raw_bit(mqd_t *m) {
+    if (m->CT == 0) {
+        int prev = m->bp >= 0 ? mq_byte(m, m->bp) : 0;
+        m->bp++;
+        m->C = (uint32_t)mq_byte(m, m->bp);
+        m->CT = prev == 0xFF ? 7 : 8;
+    }
+    m->CT--;
+    m->nsym++;
+    return (int)((m->C >> m->CT) & 1);
 }
 
 /* RENORMD, T.800 Figure C.18 (TC/coefficient_coder.cl:138-150). */
@@ -109,6 +139,7 @@ static void mq_renorm(mqd_t *m) {
 /* DECODE with LPS/MPS exchange, T.800 Figures C.15-C.17
  * (TC/coefficient_coder.cl:152-252, with the MPS switch actually applied). */
 static int mq_decode(mqd_t *m, int cx) {
+    if (m->raw) return raw_bit(m);
     int I = m->I[cx];
     uint32_t qe = QE[I];
     int d;
@@ -346,8 +377,29 @@ int or_t1_decode(const uint8_t *data, int len, int w, int h, int orient, int num
 
 int or_t1_decode_style(const uint8_t *data, int len, int w, int h, int orient, int numbps, int npasses, int style,
                        int32_t *mag, int8_t *sign, int8_t *lowplane) {
-    if (w <= 0 || h <= 0 || w > 1024 || h > 1024 || orient < 0 || orient > 3) return -OR_ERR_ARG;
+    return or_t1_decode_seg(data, len, &len, 1, w, h, orient, numbps, npasses, style, mag, sign, lowplane);
+}
+
+/* BYPASS / TERMALL codeword segments (T.800 D.6, Table D.9): pass k (0 = the
+ * first cleanup) ends a segment; the SPP and MRP passes from the fifth
+ * bit-plane on are raw under BYPASS. */
+int or_seg_end(int style, int k) {
+    if (style & OR_STYLE_TERMALL) return 1;
+    if (style & OR_STYLE_BYPASS) return k == 9 || (k > 9 && (k - 1) % 3 != 0);
+    return 0;
+}
+
+static int raw_pass(int style, int k) { return (style & OR_STYLE_BYPASS) && k >= 10 && (k - 1) % 3 != 2; }
+
+int or_t1_decode_seg(const uint8_t *data, int len, const int *seglen, int nseg, int w, int h, int orient,
+                     int numbps, int npasses, int style, int32_t *mag, int8_t *sign, int8_t *lowplane) {
+    if (w <= 0 || h <= 0 || w > 1024 || h > 1024 || orient < 0 || orient > 3 || nseg < 1) return -OR_ERR_ARG;
     if (style & ~OR_STYLE_DECODABLE) return -OR_ERR_UNSUPPORTED;
+    {
+        long tot = 0;
+        for (int s = 0; s < nseg; s++) tot += seglen[s] < 0 ? 0 : seglen[s];
+        if (tot > len) return -OR_ERR_ARG;
+    }
     int cls = orient == 1 ? 1 : (orient == 3 ? 2 : 0);
     t1_t t;
     int n = (w + 2) * (h + 2);
@@ -368,11 +420,22 @@ int or_t1_decode_style(const uint8_t *data, int len, int w, int h, int orient, i
         lowplane[i] = -1;
     }
     mqd_t m;
-    mq_init(&m, data, len);
+    int seg = 0, off = 0;
+    mq_init(&m, data, seglen[0]);
     for (int k = 0; k < npasses; k++) {
         int p = numbps - 1 - (k + 2) / 3;
         if (p < 0) break;
         int type = k == 0 ? 2 : (k - 1) % 3; /* 0 sigprop, 1 magref, 2 cleanup */
+        if (k > 0 && or_seg_end(style, k - 1)) {
+            /* the next codeword segment: raw, or the MQ decoder re-initialised */
+            off += seg < nseg ? seglen[seg] : 0;
+            seg++;
+            int sl = seg < nseg ? seglen[seg] : 0;
+            if (raw_pass(style, k))
+                raw_start(&m, data + off, sl);
+            else
+                mq_start(&m, data + off, sl);
+        }
         if (type == 0)
             pass_sigprop(&t, &m, cls, p);
         else if (type == 1)
@@ -856,12 +919,22 @@ static int parse_main(rd_t *r, hdr_t *h) {
 
 /* ---- geometry of a tile ---- */
 
+#define MAXSEG 96 /* codeword segments of one code-block (one per pass at most) */
+
 typedef struct {
     int x0, y0, x1, y1;
     int included, zbp, npasses, lblock;
     uint8_t *data;
     int len, cap;
+    int nseg, seglen[MAXSEG]; /* BYPASS / TERMALL: bytes of each codeword segment */
 } cb_t;
+
+/* codeword segment of pass k under the style */
+static int seg_index(int style, int k) {
+    int n = 0;
+    for (int j = 0; j < k; j++) n += or_seg_end(style, j);
+    return n;
+}
 
 typedef struct {
     int *val, *low, *known, *parent;
@@ -1188,8 +1261,10 @@ static int read_packet(const hdr_t *h, tc_t *tc, int r, int q, int layer, const 
     /* included segment lengths in header order */
     int present = bio_bit(&b);
     int nseg = 0, cap = 0;
-    int *seglen = NULL;
+    int *seglen = NULL, *segidx = NULL;
     cb_t **segcb = NULL;
+    const int style = tc->cc.cbsty;
+    const int multi = (style & (OR_STYLE_BYPASS | OR_STYLE_TERMALL)) != 0;
     if (present) {
         for (int k = 0; k < R->nb; k++) {
             band_t *B = &R->b[k];
@@ -1210,17 +1285,29 @@ static int read_packet(const hdr_t *h, tc_t *tc, int r, int q, int layer, const 
                 }
                 int np = bits_npasses(&b);
                 while (bio_bit(&b)) C->lblock++;
-                int L = bio_bits(&b, C->lblock + floorlog2(np));
-                C->npasses += np;
-                if (nseg == cap) {
-                    cap = cap * 2 + 16;
-                    seglen = (int *)realloc(seglen, sizeof(int) * (size_t)cap);
-                    segcb = (cb_t **)realloc(segcb, sizeof(cb_t *) * (size_t)cap);
-                    if (!seglen || !segcb) return OR_ERR_NOMEM;
+                /* one length per codeword segment the passes touch (B.10.7.2) */
+                int k0 = C->npasses, k = k0;
+                while (k < k0 + np && !b.err) {
+                    int e = k;
+                    if (multi)
+                        while (e < k0 + np - 1 && !or_seg_end(style, e)) e++;
+                    else
+                        e = k0 + np - 1;
+                    int L = bio_bits(&b, C->lblock + floorlog2(e - k + 1));
+                    if (nseg == cap) {
+                        cap = cap * 2 + 16;
+                        seglen = (int *)realloc(seglen, sizeof(int) * (size_t)cap);
+                        segidx = (int *)realloc(segidx, sizeof(int) * (size_t)cap);
+                        segcb = (cb_t **)realloc(segcb, sizeof(cb_t *) * (size_t)cap);
+                        if (!seglen || !segcb || !segidx) return OR_ERR_NOMEM;
+                    }
+                    seglen[nseg] = L;
+                    segidx[nseg] = multi ? seg_index(style, k) : 0;
+                    segcb[nseg] = C;
+                    nseg++;
+                    k = e + 1;
                 }
-                seglen[nseg] = L;
-                segcb[nseg] = C;
-                nseg++;
+                C->npasses += np;
                 if (b.err) break;
             }
         }
@@ -1228,6 +1315,7 @@ static int read_packet(const hdr_t *h, tc_t *tc, int r, int q, int layer, const 
     bio_align(&b);
     if (b.err) {
         free(seglen);
+        free(segidx);
         free(segcb);
         return b.err;
     }
@@ -1243,6 +1331,7 @@ static int read_packet(const hdr_t *h, tc_t *tc, int r, int q, int layer, const 
             *ep += 2;
         else {
             free(seglen);
+            free(segidx);
             free(segcb);
             return OR_ERR_MARKER;
         }
@@ -1250,14 +1339,19 @@ static int read_packet(const hdr_t *h, tc_t *tc, int r, int q, int layer, const 
     for (int s = 0; s < nseg; s++) {
         if (*pos + (size_t)seglen[s] > len) {
             free(seglen);
+            free(segidx);
             free(segcb);
             return OR_ERR_TRUNCATED;
         }
         int rc = cb_append(segcb[s], d + *pos, seglen[s]);
         if (rc) return rc;
+        if (segidx[s] >= MAXSEG) return OR_ERR_UNSUPPORTED;
+        segcb[s]->seglen[segidx[s]] += seglen[s];
+        if (segidx[s] + 1 > segcb[s]->nseg) segcb[s]->nseg = segidx[s] + 1;
         *pos += (size_t)seglen[s];
     }
     free(seglen);
+    free(segidx);
     free(segcb);
     return OR_OK;
 }
@@ -1654,8 +1748,9 @@ static int list_tile(const hdr_t *h, int t, tc_t *tcs, void *user) {
                             int32_t *mag = (int32_t *)malloc(sizeof(int32_t) * (size_t)w * hh);
                             int8_t *sg = (int8_t *)malloc((size_t)w * hh), *lp = (int8_t *)malloc((size_t)w * hh);
                             if (mag && sg && lp)
-                                k = or_t1_decode_style(C->data, C->len, w, hh, B->orient, B->mb - C->zbp, C->npasses,
-                                                       tcs[c].cc.cbsty, mag, sg, lp);
+                                k = or_t1_decode_seg(C->data, C->len, C->nseg > 1 ? C->seglen : &C->len,
+                                                     C->nseg > 1 ? C->nseg : 1, w, hh, B->orient, B->mb - C->zbp,
+                                                     C->npasses, tcs[c].cc.cbsty, mag, sg, lp);
                             free(mag);
                             free(sg);
                             free(lp);
@@ -1764,8 +1859,9 @@ static int decode_tile(const hdr_t *h, int t, tc_t *tcs, void *user) {
                         break;
                     }
                     int numbps = B->mb - CB->zbp;
-                    int nr = or_t1_decode_style(CB->data, CB->len, w, hh, B->orient, numbps, CB->npasses, tc->cc.cbsty,
-                                                mag, sg, lp);
+                    int nr = or_t1_decode_seg(CB->data, CB->len, CB->nseg > 1 ? CB->seglen : &CB->len,
+                                              CB->nseg > 1 ? CB->nseg : 1, w, hh, B->orient, numbps, CB->npasses,
+                                              tc->cc.cbsty, mag, sg, lp);
                     if (nr < 0) rc = -nr;
                     for (int y = 0; y < hh && !rc; y++)
                         for (int x = 0; x < w; x++) {

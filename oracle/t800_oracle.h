@@ -89,18 +89,25 @@ int or_decode_reduced(const uint8_t *cs, size_t len, int reduce, int32_t *out, d
  * coefficient (row-major w*h): magnitude bits, sign (0/1) and the lowest
  * bit-plane coded for it (-1 when it never became significant).  Returns the
  * number of MQ symbols decoded, or -error. */
-/* Code-block style bits (T.800 Table A.19) the oracle decodes; BYPASS (1) and
- * TERMALL (4) are not among them. */
+/* Code-block style bits (T.800 Table A.19); the oracle decodes all six. */
+#define OR_STYLE_BYPASS 1
 #define OR_STYLE_RESET 2
+#define OR_STYLE_TERMALL 4
 #define OR_STYLE_CAUSAL 8
 #define OR_STYLE_PTERM 16
 #define OR_STYLE_SEGSYM 32
-#define OR_STYLE_DECODABLE (OR_STYLE_RESET | OR_STYLE_CAUSAL | OR_STYLE_PTERM | OR_STYLE_SEGSYM)
+#define OR_STYLE_DECODABLE 63
 /* or_t1_decode with code-block style bits */
 int or_t1_decode_style(const uint8_t *data, int len, int w, int h, int orient, int numbps, int npasses, int style,
                        int32_t *mag, int8_t *sign, int8_t *lowplane);
 int or_t1_decode(const uint8_t *data, int len, int w, int h, int orient,
                  int numbps, int npasses, int32_t *mag, int8_t *sign, int8_t *lowplane);
+/* With the codeword split into nseg codeword segments of seglen[] bytes (in
+ * order, summing to at most len): needed for BYPASS and TERMALL, whose passes
+ * start new segments (or_seg_end(style, k): pass k ends one). */
+int or_t1_decode_seg(const uint8_t *data, int len, const int *seglen, int nseg, int w, int h, int orient,
+                     int numbps, int npasses, int style, int32_t *mag, int8_t *sign, int8_t *lowplane);
+int or_seg_end(int style, int k);
 
 /* Inverse 2-D DWT of one tile-component held in Mallat layout (subbands in
  * quadrants, LL top-left), in place.  (x0,y0) is the tile-component origin

@@ -1,4 +1,4 @@
-"""Code-block styles (RESET, CAUSAL, PTERM, SEGSYM) on the GPU: the lockstep
+"""Code-block styles (BYPASS, RESET, TERMALL, CAUSAL, PTERM, SEGSYM) on the GPU: the lockstep
 tier-1 kernel through its C-ABI launch on random styled code-blocks against the
 oracle's styled tier-1, and whole styled codestreams through the fused, batched
 and staged paths against the oracle (lossless bit-exact, lossy within 1)."""
@@ -26,13 +26,18 @@ def test_lockstep_on_styled_blocks(style, small, bpw):
     L = tca.lib()
     stream = ctypes.c_void_p()
     tca.check(L.tc_stream_create(dev.handle, ctypes.byref(stream)), "stream")
-    mode = {2: 2, 8: 4, 32: 8}  # style bit -> descriptor mode bit
+    mode = {2: 2, 8: 4, 32: 8, 1: 16, 4: 32}  # style bit -> descriptor mode bit
     mbits = sum(v for k, v in mode.items() if style & k)
     arena = bytearray()
     descs = np.zeros(len(blocks), tca.CBLK_DESC)
     pitch = 64
     for i, b in enumerate(blocks):
-        arena += bytes(i % 3)
+        if style & 5:
+            # segment lengths and their count in the words before the codeword
+            arena += bytes((-len(arena)) % 4)
+            arena += np.asarray(b["seglen"] + [len(b["seglen"])], "<u4").tobytes()
+        else:
+            arena += bytes(i % 3)
         descs[i] = (len(arena), len(b["data"]), i * pitch * 64, pitch, b["w"], b["h"], b["numbps"], b["npasses"],
                     1 if b["orient"] == 1 else (2 if b["orient"] == 3 else 0), mbits, 1.0)
         arena += b["data"]
@@ -54,7 +59,7 @@ def test_lockstep_on_styled_blocks(style, small, bpw):
         for i, b in enumerate(blocks):
             got = out[i, :b["h"], :b["w"]]
             exp = reconstruct(*oracle.t1_decode(b["data"], b["w"], b["h"], b["orient"], b["numbps"], b["npasses"],
-                                                style)[:3])
+                                                style, b["seglen"])[:3])
             assert np.array_equal(got, exp), (i, b["w"], b["h"], b["orient"], b["numbps"], b["npasses"])
             if b["npasses"] == 3 * b["numbps"] - 2:
                 assert np.array_equal(got, b["src"])

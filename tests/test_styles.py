@@ -1,20 +1,25 @@
-"""Code-block styles (T.800 Table A.19) on the CPU: RESET (2), CAUSAL (8),
-PTERM (16) and SEGSYM (32) in the oracle and the host parser.
+"""Code-block styles (T.800 Table A.19) on the CPU: BYPASS (1), RESET (2),
+TERMALL (4), CAUSAL (8), PTERM (16) and SEGSYM (32) in the oracle and the host
+parser.
 
 No encoder available here writes these styles, so the oracle's styled tier-1 is
 pinned by properties that tie it to the style-free decode, which the reference's
 fixtures pin:
 - every full-pass styled block decodes to its source coefficients;
-- RESET, PTERM and SEGSYM change how passes are coded, never which bits a pass
-  carries, so at every truncation point a styled block decodes exactly as the
+- BYPASS, RESET, TERMALL, PTERM and SEGSYM change how passes are coded (raw
+  bits, context resets, terminated segments), never which bits a pass carries,
+  so at every truncation point a styled block decodes exactly as the
   style-free encoding of the same coefficients; CAUSAL changes which
   coefficients the significance pass visits, so for it the equality holds at
-  truncation points that end on a cleanup pass;
+  truncation points that end on a cleanup pass; BYPASS and TERMALL truncate
+  only where a codeword segment ends;
 - decoding a CAUSAL stream without the flag, or a style-free stream with
   SEGSYM, fails (wrong coefficients or a corrupt-segment error), so the flags
   are not ignored.
-Parity for RESET and CAUSAL is therefore partially pinned: the properties
-above, not a third-party vector."""
+- a BYPASS or TERMALL block decoded as one segment, without its segment
+  lengths, decodes wrongly.
+Parity for the styles is therefore partially pinned: the properties above, not
+a third-party vector."""
 import ctypes
 
 import numpy as np
@@ -24,14 +29,26 @@ import thousandthchicken_amd as tca
 from oracle import oracle
 from thousandthchicken_amd import synth
 
-STYLES = [2, 8, 16, 32, 2 | 8 | 32, 2 | 8 | 16 | 32]
+STYLES = [1, 2, 4, 8, 16, 32, 1 | 4, 2 | 8 | 32, 2 | 8 | 16 | 32, 63]
+
+
+def _encode(a, orient, keep, style):
+    """(codeword, segment lengths, bit-planes, passes kept) of one block."""
+    L = synth._enc_lib()
+    h, w = a.shape
+    buf = np.zeros(1 << 18, np.uint8)
+    segs = np.zeros(128, np.uint32)
+    nb, ns, npas = ctypes.c_int(0), ctypes.c_int(0), ctypes.c_int(0)
+    k = L.tce_t1_encode_segments(a.ctypes.data, w, h, orient, keep, style, buf.ctypes.data, buf.size,
+                                 ctypes.byref(nb), segs.ctypes.data, len(segs), ctypes.byref(ns), ctypes.byref(npas))
+    assert k >= 0
+    return bytes(buf[:k]), [int(v) for v in segs[:ns.value]], nb.value, npas.value
 
 
 def styled_blocks(seed, n, style, maxw=64, maxh=64, full=False):
     """Random code-blocks encoded with `style`, each with the style-free
     encoding of the same coefficients and truncation point."""
     rng = np.random.default_rng(seed)
-    L = synth._enc_lib()
     out = []
     for _ in range(n):
         w, h = int(rng.integers(1, maxw + 1)), int(rng.integers(1, maxh + 1))
@@ -40,19 +57,13 @@ def styled_blocks(seed, n, style, maxw=64, maxh=64, full=False):
         a = rng.integers(-(1 << bits) + 1, 1 << bits, (h, w)).astype(np.int32)
         a[rng.uniform(size=(h, w)) < rng.uniform(0, 0.95)] = 0
         keep = 999 if full or rng.uniform() < 0.4 else int(rng.integers(1, 50))
-        enc = []
-        for s in (style, 0):
-            buf = np.zeros(1 << 18, np.uint8)
-            nb = ctypes.c_int(0)
-            k = L.tce_t1_encode_truncated_style(a.ctypes.data, w, h, orient, keep, s, buf.ctypes.data, buf.size,
-                                                ctypes.byref(nb))
-            enc.append((bytes(buf[:k]), nb.value))
-        if enc[0][1] == 0:
+        data, seglen, numbps, npass = _encode(a, orient, keep, style)
+        if numbps == 0 or npass == 0:
             continue
-        assert enc[0][1] == enc[1][1]
-        npass = min(keep, 3 * enc[0][1] - 2)
-        out.append(dict(w=w, h=h, orient=orient, numbps=enc[0][1], npasses=npass, data=enc[0][0], plain=enc[1][0],
-                        src=a, style=style))
+        plain, _, nb0, np0 = _encode(a, orient, npass, 0)
+        assert nb0 == numbps and np0 == npass
+        out.append(dict(w=w, h=h, orient=orient, numbps=numbps, npasses=npass, data=data, seglen=seglen,
+                        plain=plain, src=a, style=style))
     return out
 
 
@@ -62,8 +73,9 @@ def reconstruct(mag, sign, low):
     return np.where(sign.astype(bool), -v, v).astype(np.int32)
 
 
-def _decode(b, data, style):
-    return reconstruct(*oracle.t1_decode(data, b["w"], b["h"], b["orient"], b["numbps"], b["npasses"], style)[:3])
+def _decode(b, data, style, seglen=None):
+    return reconstruct(*oracle.t1_decode(data, b["w"], b["h"], b["orient"], b["numbps"], b["npasses"], style,
+                                         seglen)[:3])
 
 
 @pytest.mark.parametrize("style", STYLES)
@@ -71,7 +83,7 @@ def test_oracle_styled_full_blocks_round_trip(style):
     blocks = styled_blocks(100 + style, 60, style, full=True)
     assert len(blocks) > 40
     for b in blocks:
-        assert np.array_equal(_decode(b, b["data"], style), b["src"]), (b["w"], b["h"], b["orient"])
+        assert np.array_equal(_decode(b, b["data"], style, b["seglen"]), b["src"]), (b["w"], b["h"], b["orient"])
 
 
 @pytest.mark.parametrize("style", STYLES)
@@ -81,7 +93,7 @@ def test_oracle_styled_truncated_equals_style_free(style):
     for b in blocks:
         if (style & oracle.STYLE_CAUSAL) and b["npasses"] % 3 != 1:
             continue
-        got = _decode(b, b["data"], style)
+        got = _decode(b, b["data"], style, b["seglen"])
         exp = _decode(b, b["plain"], 0)
         assert np.array_equal(got, exp), (b["w"], b["h"], b["orient"], b["numbps"], b["npasses"])
         checked += 1
@@ -111,6 +123,22 @@ def test_segsym_rejects_streams_without_the_marker():
     assert bad > 20
 
 
+@pytest.mark.parametrize("style", [1, 4, 1 | 4])
+def test_segments_are_needed_for_bypass_and_termall(style):
+    """Decoded as one segment (no segment lengths), a BYPASS / TERMALL block
+    goes wrong: its passes restart the decoder or are raw."""
+    blocks = styled_blocks(400 + style, 40, style, full=True)
+    wrong = 0
+    for b in [b for b in blocks if len(b["seglen"]) > 1]:
+        try:
+            wrong += not np.array_equal(_decode(b, b["data"], style), b["src"])
+        except oracle.OracleError:
+            wrong += 1
+        if style & 4:
+            assert len(b["seglen"]) == b["npasses"]
+    assert wrong >= 10
+
+
 @pytest.mark.parametrize("style", STYLES)
 @pytest.mark.parametrize("lossy", [False, True])
 def test_styled_codestreams_decode_in_oracle_and_parse_alike(style, lossy):
@@ -120,9 +148,10 @@ def test_styled_codestreams_decode_in_oracle_and_parse_alike(style, lossy):
     if not lossy:
         assert np.array_equal(got, src)
     else:
-        # rate control keeps different passes per style: check the fidelity only
+        # rate control keeps different passes per style (BYPASS / TERMALL only at
+        # segment ends): check the fidelity only
         mse = np.mean((got.astype(np.float64) - src) ** 2)
-        assert 10 * np.log10(255.0 ** 2 / mse) > 25
+        assert 10 * np.log10(255.0 ** 2 / mse) > (20 if style & 5 else 25)
     key = lambda c: (c.tile, c.comp, c.res, c.band, c.x0, c.y0, c.x1, c.y1, c.npasses, c.zero_bp, c.mb, c.nbytes,
                      c.crc)
     assert [key(c) for c in tca.list_cblks(cs)] == [key(c) for c in oracle.list_cblks(cs)]
@@ -134,8 +163,8 @@ def _cod_style_offset(cs):
     return i + 2 + 2 + 1 + 1 + 2 + 1 + 1 + 1 + 1
 
 
-@pytest.mark.parametrize("flag", [1, 4])
-def test_bypass_and_termall_are_reported_unsupported(flag):
+@pytest.mark.parametrize("flag", [64, 128])
+def test_reserved_style_bits_are_reported_unsupported(flag):
     cs = synth.encode(synth.image(40, 40, 1, 3), levels=2, cb=32, cbsty=2)
     off = _cod_style_offset(cs)
     assert cs[off] == 2

@@ -141,7 +141,58 @@ struct MQEncoder {
         return out;
     }
     size_t bytes_so_far() const { return buf.size() - 1; }
+    // registers back to INITENC after a termination; the contexts keep their
+    // states (T.800 D.4.1: only RESET resets them)
+    void restart() {
+        s.A = 0x8000;
+        s.C = 0;
+        s.CT = 12;
+        buf.assign(1, 0);
+    }
 };
+
+// Raw (bypass) coding of a segment, T.800 D.6: bits MSB first; after a 0xFF
+// byte the next byte carries 7 bits below a stuffed 0.
+struct RawEncoder {
+    std::vector<uint8_t> out;
+    uint32_t cur = 0;
+    int left = 8;  // bits still free in cur
+    bool any = false;
+    void init() {
+        out.clear();
+        cur = 0;
+        left = 8;
+        any = false;
+    }
+    void put(int bit) {
+        if (left == 0) {
+            out.push_back((uint8_t)cur);
+            left = cur == 0xFF ? 7 : 8;
+            cur = 0;
+        }
+        left--;
+        cur |= (uint32_t)(bit & 1) << left;
+        any = true;
+    }
+    // pads the last byte with 0s; a final 0xFF gets a 0x00 after it so the
+    // segment never ends on 0xFF
+    std::vector<uint8_t> flush() {
+        if (any) out.push_back((uint8_t)cur);
+        if (!out.empty() && out.back() == 0xFF) out.push_back(0);
+        return out;
+    }
+    size_t bytes_so_far() const { return out.size() + (any ? 1 : 0); }
+};
+
+// Codeword segments of the BYPASS (1) and TERMALL (4) styles (T.800 D.6,
+// Table D.9): pass k (0 = first cleanup) ends a segment, and raw_pass: pass k
+// is coded raw.
+inline bool seg_end(int style, int k) {
+    if (style & 4) return true;
+    if (style & 1) return k == 9 || (k > 9 && (k - 1) % 3 != 0);
+    return false;
+}
+inline bool raw_pass(int style, int k) { return (style & 1) && k >= 10 && (k - 1) % 3 != 2; }
 
 // Recorded in place of a context label where the RESET style reset every
 // context; never a real label.
@@ -276,6 +327,10 @@ struct BlockResult {
     std::vector<Pass> passes;
     std::vector<uint8_t> sym_ctx, sym_bit;  // every coded symbol (for truncation checks)
     std::vector<uint32_t> trunc_len;       // decodable length of each pass prefix
+    // BYPASS / TERMALL: length of each codeword segment and the pass count at
+    // its end; truncation only at segment ends
+    std::vector<uint32_t> seglen;
+    std::vector<int> seg_passes;
 };
 
 // Encodes w x h signed coefficients (integer quantization indices) of a band
@@ -313,7 +368,16 @@ void t1_encode(const int32_t* coef, int w, int h, int pitch, int orient, double 
     }
     MQEncoder mq;
     mq.init();
+    RawEncoder raw;
+    const bool term = (style & (1 | 4)) != 0;
+    bool raw_now = false;
+    R.seglen.clear();
+    R.seg_passes.clear();
     auto enc = [&](int bit, int cx) {
+        if (raw_now) {
+            raw.put(bit);
+            return;
+        }
         mq.encode(bit, cx);
         if (record) {
             R.sym_ctx.push_back((uint8_t)cx);
@@ -362,6 +426,7 @@ void t1_encode(const int32_t* coef, int w, int h, int pitch, int orient, double 
         int p = numbps - 1 - (k + 2) / 3;
         int type = k == 0 ? 2 : (k - 1) % 3;
         dist = 0;
+        raw_now = raw_pass(style, k);
         if (type == 0) {
             for (int y0 = 0; y0 < h; y0 += 4)
                 for (int x = 0; x < w; x++)
@@ -454,7 +519,29 @@ void t1_encode(const int32_t* coef, int w, int h, int pitch, int orient, double 
                 R.sym_bit.push_back(0);
             }
         }
+        if (term) {
+            size_t done = R.data.size() + (raw_now ? raw.bytes_so_far() : mq.bytes_so_far());
+            if (seg_end(style, k) || k == npass - 1) {
+                std::vector<uint8_t> seg = raw_now ? raw.flush() : mq.flush();
+                R.data.insert(R.data.end(), seg.begin(), seg.end());
+                R.seglen.push_back((uint32_t)seg.size());
+                R.seg_passes.push_back(k + 1);
+                done = R.data.size();
+                if (k + 1 < npass) {
+                    if (raw_pass(style, k + 1))
+                        raw.init();
+                    else
+                        mq.restart();
+                }
+            }
+            R.passes.push_back(Pass{done, dist, R.sym_ctx.size()});
+            continue;
+        }
         R.passes.push_back(Pass{mq.bytes_so_far(), dist, R.sym_ctx.size()});
+    }
+    if (term) {
+        R.npasses = npass;
+        return;
     }
     R.data = mq.flush();
     R.npasses = npass;
@@ -463,6 +550,11 @@ void t1_encode(const int32_t* coef, int w, int h, int pitch, int orient, double 
 // Shortest prefix of the codeword that decodes the first `npass` passes: start
 // from bytes emitted + 2 and grow until the recorded symbols decode.
 uint32_t truncation_length(const BlockResult& R, int npass) {
+    if (!R.seglen.empty()) {  // segment styles: whole segments up to the pass count
+        uint32_t L = 0;
+        for (size_t s = 0; s < R.seglen.size() && R.seg_passes[s] <= npass; s++) L += R.seglen[s];
+        return L;
+    }
     if (npass >= (int)R.passes.size()) return (uint32_t)R.data.size();
     size_t nsym = R.passes[npass - 1].nsym;
     size_t L = std::min(R.data.size(), R.passes[npass - 1].end_bytes + 2);
@@ -973,8 +1065,10 @@ int encode(const void* pixels, const Params& P, std::vector<uint8_t>& out) {
                 cumR[k + 1] = (double)R.passes[k].end_bytes + 1.0;
             }
             std::vector<int> pts{0};
+            std::vector<uint8_t> cut(R.passes.size() + 1, R.seglen.empty() ? 1 : 0);
+            for (int e : R.seg_passes) cut[e] = 1;
             for (int k = 1; k <= (int)R.passes.size(); k++) {
-                if (cumD[k] <= cumD[pts.back()]) continue;
+                if (!cut[k] || cumD[k] <= cumD[pts.back()]) continue;
                 while (pts.size() >= 2) {
                     int a = pts[pts.size() - 2], bb = pts.back();
                     double s1 = (cumD[bb] - cumD[a]) / std::max(1e-9, cumR[bb] - cumR[a]);
@@ -1129,17 +1223,37 @@ int encode(const void* pixels, const Params& P, std::vector<uint8_t>& out) {
                                     bw.bits((0xF << 5) | (np - 6), 9);
                                 else
                                     bw.bits((0x1FFu << 7) | (uint32_t)(np - 37), 16);
-                                int fl = 0;
-                                while ((1 << (fl + 1)) <= np) fl++;
-                                int need = 1;
-                                while (((uint32_t)1 << need) <= cb.keep_len) need++;
+                                // one length per codeword segment of the kept
+                                // passes, in Lblock + floor(log2(passes)) bits
+                                // each (B.10.7.1, B.10.7.2)
+                                std::vector<std::pair<uint32_t, int>> segs;
+                                if (cb.r.seglen.empty()) {
+                                    segs.push_back({cb.keep_len, np});
+                                } else {
+                                    int prev = 0;
+                                    for (size_t q = 0; q < cb.r.seglen.size() && cb.r.seg_passes[q] <= np; q++) {
+                                        segs.push_back({cb.r.seglen[q], cb.r.seg_passes[q] - prev});
+                                        prev = cb.r.seg_passes[q];
+                                    }
+                                }
+                                auto flog2 = [](int v) {
+                                    int l = 0;
+                                    while ((1 << (l + 1)) <= v) l++;
+                                    return l;
+                                };
                                 int& lb = lblock[c][idx];
-                                while (lb + fl < need) {
+                                int want = lb;
+                                for (auto& sg : segs) {
+                                    int need = 1;
+                                    while (((uint32_t)1 << need) <= sg.first) need++;
+                                    want = std::max(want, need - flog2(sg.second));
+                                }
+                                while (lb < want) {
                                     bw.bit(1);
                                     lb++;
                                 }
                                 bw.bit(0);
-                                bw.bits(cb.keep_len, lb + fl);
+                                for (auto& sg : segs) bw.bits(sg.first, lb + flog2(sg.second));
                                 bodies.push_back(&cb);
                             }
                     }
@@ -1201,7 +1315,7 @@ extern "C" {
 int tce_encode(const void* pixels, int w, int h, int ncomp, int prec, int levels, int cbw_exp, int cbh_exp,
                int tile_w, int tile_h, int mct, int irreversible, double target_bytes, double base_step, int threads,
                int cbsty, int packed, int packed_max, uint8_t** out, size_t* out_len) {
-    if (!pixels || !out || !out_len || w <= 0 || h <= 0 || (cbsty & ~(2 | 8 | 16 | 32)) || packed < 0 || packed > 2)
+    if (!pixels || !out || !out_len || w <= 0 || h <= 0 || (cbsty & ~63) || packed < 0 || packed > 2)
         return -1;
     tce::Params P{w, h, ncomp, prec, levels, cbw_exp, cbh_exp, tile_w, tile_h, mct, irreversible,
                   threads > 0 ? threads : (int)std::max(1u, std::thread::hardware_concurrency()), target_bytes,
@@ -1249,6 +1363,42 @@ int tce_t1_encode_truncated_style(const int32_t* coef, int w, int h, int orient,
     uint32_t L = tce::truncation_length(R, keep);
     if ((int)L > cap) return -1;
     std::memcpy(out, R.data.data(), L);
+    return (int)L;
+}
+
+// Same for every style, BYPASS (1) and TERMALL (4) included: truncates after
+// the last codeword segment that ends at or before `keep` passes, and returns
+// the segment lengths (seglen[0..*nseg)) and the pass count kept.
+int tce_t1_encode_segments(const int32_t* coef, int w, int h, int orient, int keep, int style, uint8_t* out, int cap,
+                           int* numbps, uint32_t* seglen, int seg_cap, int* nseg, int* npasses) {
+    tce::BlockResult R;
+    tce::t1_encode(coef, w, h, w, orient, 1.0, true, R, style);
+    *numbps = R.numbps;
+    *nseg = 0;
+    *npasses = 0;
+    if (!R.npasses) return 0;
+    if (keep > R.npasses) keep = R.npasses;
+    if (R.seglen.empty()) {
+        uint32_t L = tce::truncation_length(R, keep);
+        if ((int)L > cap || seg_cap < 1) return -1;
+        std::memcpy(out, R.data.data(), L);
+        seglen[0] = L;
+        *nseg = 1;
+        *npasses = keep;
+        return (int)L;
+    }
+    uint32_t L = 0;
+    int n = 0, kept = 0;
+    for (size_t s = 0; s < R.seglen.size() && R.seg_passes[s] <= keep; s++) {
+        if (n >= seg_cap) return -1;
+        seglen[n++] = R.seglen[s];
+        L += R.seglen[s];
+        kept = R.seg_passes[s];
+    }
+    if ((int)L > cap) return -1;
+    std::memcpy(out, R.data.data(), L);
+    *nseg = n;
+    *npasses = kept;
     return (int)L;
 }
 

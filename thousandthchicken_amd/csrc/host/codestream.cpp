@@ -495,8 +495,8 @@ Status build_tile(Image& img, int t, const Params& P, TileState& bstate, std::st
         // decodable styles: RESET (2), CAUSAL (8), PTERM (16, nothing to do for a
         // decoder) and SEGSYM (32); BYPASS (1) and TERMALL (4) split a codeword
         // into per-pass segments, which the tier-1 kernels do not take
-        if (C.cbsty & ~(2 | 8 | 16 | 32)) {
-            err = "code-block styles BYPASS and TERMALL are not supported";
+        if (C.cbsty & ~63) {
+            err = "reserved code-block style bits";
             return Status::Unsupported;
         }
         int nq = Q.style == 1 ? 1 : 3 * C.nlev + 1;
@@ -656,7 +656,10 @@ Status read_packet(Image& img, int t, int c, int r, int q, int layer, const Para
     struct Inc {
         uint32_t cb;
         uint32_t len;
+        int seg;  // codeword segment (BYPASS / TERMALL), else 0
     };
+    const int style = tc.cbsty;
+    const bool multi = (style & (1 | 4)) != 0;
     Inc incs[512];
     std::vector<Inc> more;
     int ninc = 0;
@@ -686,14 +689,21 @@ Status read_packet(Image& img, int t, int c, int r, int q, int layer, const Para
                     while (br.bit()) {
                         if (br.bad() || ++cb.lblock > 32) break;
                     }
-                    uint32_t L = br.bits(cb.lblock + ilog2(np));
+                    // one length per codeword segment the passes touch (B.10.7.2)
+                    for (int k = cb.npasses, k1 = cb.npasses + np; k < k1 && !br.bad();) {
+                        int e = k1 - 1;
+                        if (multi)
+                            for (e = k; e < k1 - 1 && !seg_end(style, e);) e++;
+                        uint32_t L = br.bits(cb.lblock + ilog2(e - k + 1));
+                        Inc inc{id, L, multi ? seg_index(style, k) : 0};
+                        if (ninc < 512)
+                            incs[ninc] = inc;
+                        else
+                            more.push_back(inc);
+                        ++ninc;
+                        k = e + 1;
+                    }
                     cb.npasses += np;
-                    Inc e{id, L};
-                    if (ninc < 512)
-                        incs[ninc] = e;
-                    else
-                        more.push_back(e);
-                    ++ninc;
                 }
         }
     }
@@ -722,6 +732,15 @@ Status read_packet(Image& img, int t, int c, int r, int q, int layer, const Para
             return Status::Truncated;
         }
         CodeBlock& cb = img.cblks[e.cb];
+        if (multi) {
+            if (cb.term < 0) {
+                cb.term = (int)img.termlens.size();
+                img.termlens.emplace_back();
+            }
+            std::vector<uint32_t>& tl = img.termlens[cb.term];
+            if ((int)tl.size() <= e.seg) tl.resize(e.seg + 1, 0);
+            tl[e.seg] += e.len;
+        }
         if (e.len) {
             if (cb.nseg == 0) cb.seg0_off = (uint32_t)pos;
             img.segs.push_back(Segment{e.cb, (uint32_t)pos, e.len});
@@ -950,6 +969,9 @@ Status parse_impl(const uint8_t* data, size_t len, Image& img, std::string& err,
         s = read_packets(img, t, P, bs, ranges[t], packed[t] ? &hdrs[t] : nullptr, err);
         if (s != Status::Ok) return s;
     }
+    // each BYPASS / TERMALL block's segment lengths end with their count (the
+    // table the tier-1 kernel reads before the codeword)
+    for (auto& tl : img.termlens) tl.push_back((uint32_t)tl.size());
     for (auto& cb : img.cblks) {
         if (!cb.included) continue;
         if (cb.numbps > 31 || cb.numbps < 0) {

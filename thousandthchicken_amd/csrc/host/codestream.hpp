@@ -77,6 +77,7 @@ struct CodeBlock {
     uint32_t nbytes = 0;  // codeword bytes over all segments
     int nseg = 0;
     uint32_t seg0_off = 0;  // first segment offset in the codestream
+    int term = -1;          // BYPASS / TERMALL: index into Image::termlens
 };
 
 struct Segment {
@@ -96,6 +97,9 @@ struct Image {
     std::vector<Tile> tiles;
     std::vector<CodeBlock> cblks;
     std::vector<Segment> segs;  // in codestream order; contiguous per block when nseg == 1
+    // BYPASS / TERMALL code-blocks: bytes of each codeword segment (the
+    // block's contributions concatenated in codestream order hold them in turn)
+    std::vector<std::vector<uint32_t>> termlens;
     const uint8_t* cs = nullptr;  // codestream (inside a JP2 wrapper if any)
     size_t cs_len = 0;
     // output window on the reference grid: the whole image unless
@@ -123,6 +127,19 @@ Status probe_codestream(const uint8_t* data, size_t len, Image& img, std::string
 // to that resolution's coordinates, ceil(x / 2^k) (T.800 B.5).  Fails when a
 // tile-component has fewer than k levels.
 Status reduce_image(Image& img, int k, std::string& err);
+
+// BYPASS (1) / TERMALL (4) codeword segments (T.800 D.6, Table D.9): pass k
+// (0 = the first cleanup) ends a segment.
+inline bool seg_end(int style, int k) {
+    if (style & 4) return true;
+    if (style & 1) return k == 9 || (k > 9 && (k - 1) % 3 != 0);
+    return false;
+}
+inline int seg_index(int style, int k) {
+    int n = 0;
+    for (int j = 0; j < k; j++) n += seg_end(style, j) ? 1 : 0;
+    return n;
+}
 
 // Region decode: narrows the output window to the w x h rectangle at (x, y)
 // from the image origin (clipped to the image) and drops every code-block

@@ -50,6 +50,12 @@ Status plan_tile_cblks(const Image& im, int t, const uint64_t* plane, uint64_t& 
             err = "codeword arena beyond 4 GiB";
             return Status::Unsupported;
         }
+        if (cb.term >= 0) {
+            // BYPASS / TERMALL: segment lengths, then their count, before the codeword
+            const std::vector<uint32_t>& tl = im.termlens[cb.term];
+            copies.push_back(CwCopy{reinterpret_cast<const uint8_t*>(tl.data()), (uint32_t)(tl.size() * 4), cw_size});
+            cw_size += tl.size() * 4;
+        }
         d.data_off = (uint32_t)cw_size;
         d.data_len = cb.nbytes;
         for (uint32_t s = head[k - first]; s != UINT32_MAX; s = next[s]) {
@@ -65,7 +71,8 @@ Status plan_tile_cblks(const Image& im, int t, const uint64_t* plane, uint64_t& 
         d.npasses = (uint8_t)std::min(cb.npasses, 255);
         d.cls = (uint8_t)(B.orient == 1 ? 1 : (B.orient == 3 ? 2 : 0));
         d.mode = (uint8_t)((tc.reversible ? 0 : TC_MODE_IRREVERSIBLE) | ((tc.cbsty & 2) ? TC_MODE_RESET : 0) |
-                           ((tc.cbsty & 8) ? TC_MODE_CAUSAL : 0) | ((tc.cbsty & 32) ? TC_MODE_SEGSYM : 0));
+                           ((tc.cbsty & 8) ? TC_MODE_CAUSAL : 0) | ((tc.cbsty & 32) ? TC_MODE_SEGSYM : 0) |
+                           ((tc.cbsty & 1) ? TC_MODE_BYPASS : 0) | ((tc.cbsty & 4) ? TC_MODE_TERMALL : 0));
         d.delta = B.delta;
         cls[(w <= 32 && h <= 32) ? 0 : 1].push_back((uint32_t)descs.size());
         descs.push_back(d);

@@ -86,6 +86,7 @@ struct LaneMq {
     uint32_t A, C;
     int CT;
     uint32_t bp, len, cur;
+    uint32_t raw;    // 1: the current codeword segment is raw (BYPASS), C/CT unused
     uint32_t base;   // arena byte offset of the codeword
     uint32_t rend;   // arena byte offset one past the ring's last byte
     const uint32_t* ring;  // this lane's column of the ring (stride kLaneCount)
@@ -145,6 +146,30 @@ __device__ __forceinline__ void lane_init(LaneMq& m) {
     m.C <<= 7;
     m.CT -= 7;
     m.A = 0x8000;
+}
+
+// One raw bit (BYPASS segments, T.800 D.6): MSB first; the byte after 0xFF
+// holds 7 bits below a stuffed 0; past the segment the bytes read 0xFF.  cur
+// is the byte at bp and CT the bits of it still unread.
+__device__ __forceinline__ uint32_t lane_raw(LaneMq& m) {
+    if (m.CT == 0) {
+        const uint32_t prev = m.cur;
+        m.bp += 1;
+        m.cur = lane_byte(m, m.bp);
+        m.CT = prev == 0xFFu ? 7 : 8;
+    }
+    m.CT -= 1;
+    return (m.cur >> m.CT) & 1u;
+}
+
+// Codeword segments of the BYPASS / TERMALL styles: pass k (0 = the first
+// cleanup) ends one, and SPP/MRP passes from the fifth bit-plane on are raw.
+__device__ __forceinline__ bool lock_seg_end(uint32_t style, int k) {
+    if (style & TC_MODE_TERMALL) return true;
+    return (style & TC_MODE_BYPASS) && (k == 9 || (k > 9 && (k - 1) % 3 != 0));
+}
+__device__ __forceinline__ bool lock_raw_pass(uint32_t style, int k) {
+    return (style & TC_MODE_BYPASS) && k >= 10 && (k - 1) % 3 != 2;
 }
 
 // DECODE, T.800 Figures C.15-C.18, on a packed context state
@@ -274,7 +299,7 @@ __device__ __forceinline__ void stripe_symbols(LdsLock<M>& L, int lane, LaneMq& 
                 lab = rl ? 17u : 18u;  // run-length or uniform
             }
             uint32_t st = L.ctx[lab][lane];
-            const uint32_t bit = lane_decode(m, st, L.trans);
+            const uint32_t bit = m.raw ? lane_raw(m) : lane_decode(m, st, L.trans);
             L.ctx[lab][lane] = st;
             uint32_t* const at = out + (uint32_t)r * pitch + x;
             if (TYPE == 1) {
@@ -349,6 +374,17 @@ __device__ void decode_lockstep(LdsLock<M>& L, const tc_cblk_desc& d, bool activ
     m.base = active ? d.data_off : 0;
     m.len = active ? d.data_len : 0;
     m.bp = 0;
+    m.raw = 0;
+    // BYPASS / TERMALL: the segment length table sits before the codeword
+    const bool multi = (style & (TC_MODE_BYPASS | TC_MODE_TERMALL)) != 0;
+    uint32_t seg = 0, nseg = 0, tab = 0;
+    if (multi) {
+        const uint32_t w0 = d.data_off >> 2;
+        nseg = w0 > 0 ? words[w0 - 1] : 0u;
+        if (nseg >= w0) nseg = 0;  // a table that cannot fit: no segments (bytes read as 0xFF)
+        tab = w0 - 1 - nseg;
+        m.len = nseg > 0 ? words[tab] : 0u;
+    }
     if (active) {
         lane_refill(m, ring_col, words, nwords);
         lane_init(m);
@@ -364,6 +400,22 @@ __device__ void decode_lockstep(LdsLock<M>& L, const tc_cblk_desc& d, bool activ
         const int p = numbps - 1 - (k + 2) / 3;
         const bool pass_on = active && k < npasses && p >= 0;
         const uint32_t pbit = pass_on ? (1u << p) : 0u;
+        if (pass_on && multi && k > 0 && lock_seg_end(style, k - 1)) {
+            // the next codeword segment: raw, or the MQ decoder re-initialised
+            m.base += m.len;
+            seg += 1;
+            m.len = seg < nseg ? words[tab + seg] : 0u;
+            m.bp = 0;
+            lane_refill(m, ring_col, words, nwords);
+            if (lock_raw_pass(style, k)) {
+                m.raw = 1;
+                m.cur = lane_byte(m, 0);
+                m.CT = 8;
+            } else {
+                m.raw = 0;
+                lane_init(m);
+            }
+        }
         for (int y0 = 0; y0 < hmax; y0 += 4) {
             const bool on = pass_on && y0 < h;
             M c0 = 0, c1 = 0, c2 = 0, c3 = 0;  // candidate columns per stripe row

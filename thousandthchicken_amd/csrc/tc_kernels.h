@@ -24,14 +24,21 @@ typedef struct tc_cblk_desc {
     float delta;         // quantization step Delta_b (irreversible)
 } tc_cblk_desc;
 
-// tc_cblk_desc::mode bits.  The style bits (T.800 Table A.19: RESET, CAUSAL,
-// SEGSYM) are decoded by the lockstep tier-1 kernel only; the planner sends
-// every class holding such a block there.
+// tc_cblk_desc::mode bits.  The style bits (T.800 Table A.19) are decoded by
+// the lockstep tier-1 kernel only; the planner sends every class holding such
+// a block there.
 #define TC_MODE_IRREVERSIBLE 1
-#define TC_MODE_RESET 2   // every context reset after each pass (D.6)
-#define TC_MODE_CAUSAL 4  // stripe-causal contexts (D.7)
-#define TC_MODE_SEGSYM 8  // segmentation symbol after each cleanup pass (D.5)
-#define TC_MODE_STYLE (TC_MODE_RESET | TC_MODE_CAUSAL | TC_MODE_SEGSYM)
+#define TC_MODE_RESET 2    // every context reset after each pass (D.6)
+#define TC_MODE_CAUSAL 4   // stripe-causal contexts (D.7)
+#define TC_MODE_SEGSYM 8   // segmentation symbol after each cleanup pass (D.5)
+// BYPASS: SPP/MRP passes from the fifth bit-plane on are raw; TERMALL: every
+// pass ends a codeword segment (D.6, Table D.9).  With either, the block's
+// codeword is its segments in order, and the 4-byte words before data_off
+// (which is then 4-byte aligned) hold the segment lengths and their count:
+// word data_off/4 - 1 = nseg, words data_off/4 - 1 - nseg + s = length of s.
+#define TC_MODE_BYPASS 16
+#define TC_MODE_TERMALL 32
+#define TC_MODE_STYLE (TC_MODE_RESET | TC_MODE_CAUSAL | TC_MODE_SEGSYM | TC_MODE_BYPASS | TC_MODE_TERMALL)
 
 // One level of a 2-D inverse DWT over one tile-component: rebuilds the
 // resolution rectangle [u0,u1) x [v0,v1) from its LL (previous level's output

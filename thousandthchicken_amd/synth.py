@@ -146,6 +146,8 @@ def _enc_lib():
         L.tce_t1_encode_block.argtypes = [vp, i32, i32, i32, vp, i32, ctypes.POINTER(i32)]
         L.tce_t1_encode_truncated.argtypes = [vp, i32, i32, i32, i32, vp, i32, ctypes.POINTER(i32)]
         L.tce_t1_encode_truncated_style.argtypes = [vp, i32, i32, i32, i32, i32, vp, i32, ctypes.POINTER(i32)]
+        L.tce_t1_encode_segments.argtypes = [vp, i32, i32, i32, i32, i32, vp, i32, ctypes.POINTER(i32), vp, i32,
+                                             ctypes.POINTER(i32), ctypes.POINTER(i32)]
         _ENC = L
     return _ENC
 
@@ -156,8 +158,8 @@ def encode(img, bits=None, levels=5, cb=64, tile=None, mct=None, lossy=False, bp
 
     Lossless: RCT (when mct) + 5/3, every pass kept.  Lossy: ICT + 9/7 with
     expounded quantization, truncated to about `bpp` bits per pixel.  cbsty:
-    code-block style bits written to COD and used by tier-1 (RESET 2, CAUSAL 8,
-    PTERM 16, SEGSYM 32).  packed: None, "ppt" or "ppm" -- packet headers in
+    code-block style bits written to COD and used by tier-1 (BYPASS 1, RESET 2,
+    TERMALL 4, CAUSAL 8, PTERM 16, SEGSYM 32).  packed: None, "ppt" or "ppm" -- packet headers in
     the packets, in PPT segments of the tile-part headers or in PPM segments of
     the main header; packed_max caps each segment's payload (0: 65532)."""
     import ctypes
