@@ -34,6 +34,12 @@ typedef struct {
     int32_t reversible;       /* component 0: 1 = 5/3, 0 = 9/7 */
     int32_t nlevels, ntx, nty, cbw, cbh, nlayers, mct, progression;
     int32_t prec[4], sgnd[4]; /* first four components */
+    /* 1 when a component is sub-sampled (XRsiz or YRsiz > 1): the output then
+     * holds each component as its own plane of comp_width x comp_height
+     * samples, one plane after the other (at most four components); else the
+     * components are interleaved per pixel */
+    int32_t planar;
+    int32_t comp_width[4], comp_height[4];
 } tc_image_info;
 
 /* tier-2 view of one code-block, for checking the parser */

@@ -18,7 +18,8 @@ class Info(ctypes.Structure):
                 ("x0", "y0", "x1", "y1", "tx0", "ty0", "tw", "th", "ntx", "nty", "ncomp")] + [
         ("prec", ctypes.c_int32 * 16), ("sgnd", ctypes.c_int32 * 16)] + [
         (n, ctypes.c_int32) for n in ("nlevels", "reversible", "mct", "cbw", "cbh", "nlayers",
-                                      "progression", "qstyle", "guard_bits")]
+                                      "progression", "qstyle", "guard_bits")] + [
+        ("dx", ctypes.c_int32 * 16), ("dy", ctypes.c_int32 * 16)]
 
 
 class Cblk(ctypes.Structure):
@@ -79,11 +80,16 @@ def info(cs: bytes) -> Info:
 def decode(cs: bytes, want_float=False, reduce=0):
     """Full decode -> int32 array [ncomp, H, W] (and float64 pre-rounding values).
     reduce > 0 discards that many finest DWT levels: the image at the
-    resolution ceil(X / 2^reduce)."""
+    resolution ceil(X / 2^reduce).  With sub-sampled components the result is a
+    list of per-component arrays, component c of size
+    ceil(ceil(H / YRsiz) / 2^reduce) x ceil(ceil(W / XRsiz) / 2^reduce)."""
     i = info(cs)
 
     def c(v):
         return -((-v) >> reduce)
+
+    def cd(v, d):
+        return c(-((-v) // d))
     shape = (i.ncomp, c(i.y1) - c(i.y0), c(i.x1) - c(i.x0))
     out = np.zeros(shape, np.int32)
     outf = np.zeros(shape, np.float64) if want_float else None
@@ -91,6 +97,15 @@ def decode(cs: bytes, want_float=False, reduce=0):
                                  outf.ctypes.data if want_float else None)
     if rc:
         raise OracleError(f"or_decode_reduced failed ({rc})")
+    if any(i.dx[k] != 1 or i.dy[k] != 1 for k in range(i.ncomp)):
+        def split(a):
+            res = []
+            for k in range(i.ncomp):
+                h = cd(i.y1, i.dy[k]) - cd(i.y0, i.dy[k])
+                w = cd(i.x1, i.dx[k]) - cd(i.x0, i.dx[k])
+                res.append(a[k].ravel()[:h * w].reshape(h, w).copy())
+            return res
+        return (split(out), split(outf)) if want_float else split(out)
     return (out, outf) if want_float else out
 
 

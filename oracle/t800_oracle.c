@@ -703,6 +703,7 @@ typedef struct {
 typedef struct {
     int X0, Y0, X1, Y1, XT0, YT0, XT, YT, C;
     int prec[MAXC], sgnd[MAXC];
+    int dx[MAXC], dy[MAXC]; /* component sub-sampling XRsiz, YRsiz */
     gcs_t g;
     ccs_t cc[MAXC];
     qcs_t qc[MAXC];
@@ -888,8 +889,9 @@ static int parse_main(rd_t *r, hdr_t *h) {
         int s = rd8(r);
         h->prec[i] = (s & 0x7F) + 1;
         h->sgnd[i] = s >> 7;
-        int dx = rd8(r), dy = rd8(r);
-        if (dx != 1 || dy != 1) return OR_ERR_UNSUPPORTED;
+        h->dx[i] = rd8(r);
+        h->dy[i] = rd8(r);
+        if (h->dx[i] < 1 || h->dy[i] < 1) return OR_ERR_MARKER;
         if (h->prec[i] > 16) return OR_ERR_UNSUPPORTED;
     }
     if (r->err) return r->err;
@@ -966,7 +968,9 @@ typedef struct {
 } res_t;
 
 typedef struct {
-    int x0, y0, x1, y1;
+    int x0, y0, x1, y1;   /* tile-component rectangle: ceil(tile / XRsiz) (B-12) */
+    int rx0, ry0, rx1, ry1; /* the tile on the reference grid */
+    int dx, dy;           /* XRsiz, YRsiz */
     int nres;
     res_t r[MAXLVL + 1];
     ccs_t cc;
@@ -1114,6 +1118,17 @@ static int build_tc(const hdr_t *h, int t, int c, tc_t *tc) {
     if (tx1 > h->X1) tx1 = h->X1;
     if (ty1 > h->Y1) ty1 = h->Y1;
     memset(tc, 0, sizeof *tc);
+    tc->rx0 = tx0;
+    tc->ry0 = ty0;
+    tc->rx1 = tx1;
+    tc->ry1 = ty1;
+    tc->dx = h->dx[c];
+    tc->dy = h->dy[c];
+    /* the tile-component rectangle (B-12): every later rectangle derives from it */
+    tx0 = (int)ceildiv(tx0, tc->dx);
+    ty0 = (int)ceildiv(ty0, tc->dy);
+    tx1 = (int)ceildiv(tx1, tc->dx);
+    ty1 = (int)ceildiv(ty1, tc->dy);
     tc->x0 = tx0;
     tc->y0 = ty0;
     tc->x1 = tx1;
@@ -1366,11 +1381,11 @@ static int prc_at(const tc_t *tc, int r, long x, long y) {
     if (R->pw == 0 || R->ph == 0) return -1;
     int lv = tc->nres - 1 - r;
     long sx = 1L << (R->ppx + lv), sy = 1L << (R->ppy + lv);
-    int okx = x % sx == 0 || (x == tc->x0 && (((long)R->x0 << lv) % sx) != 0);
-    int oky = y % sy == 0 || (y == tc->y0 && (((long)R->y0 << lv) % sy) != 0);
+    int okx = x % (tc->dx * sx) == 0 || (x == tc->rx0 && (((long)R->x0 << lv) % sx) != 0);
+    int oky = y % (tc->dy * sy) == 0 || (y == tc->ry0 && (((long)R->y0 << lv) % sy) != 0);
     if (!okx || !oky) return -1;
-    long i = floordiv(ceildiv(x, 1L << lv), 1L << R->ppx) - R->px0;
-    long j = floordiv(ceildiv(y, 1L << lv), 1L << R->ppy) - R->py0;
+    long i = floordiv(ceildiv(x, (long)tc->dx << lv), 1L << R->ppx) - R->px0;
+    long j = floordiv(ceildiv(y, (long)tc->dy << lv), 1L << R->ppy) - R->py0;
     if (i < 0 || j < 0 || i >= R->pw || j >= R->ph) return -1;
     return (int)(j * R->pw + i);
 }
@@ -1398,7 +1413,7 @@ static int emit(const hdr_t *h, tc_t *tc, int r, int q, int l, pkt_src_t *S) {
  * position steps of the position-driven orders, no larger than any precinct
  * projected onto the reference grid. */
 static int read_volume(const hdr_t *h, tc_t *tcs, const poc_t *v, long xs, long ys, pkt_src_t *S) {
-    const long tx0 = tcs[0].x0, ty0 = tcs[0].y0, tx1 = tcs[0].x1, ty1 = tcs[0].y1;
+    const long tx0 = tcs[0].rx0, ty0 = tcs[0].ry0, tx1 = tcs[0].rx1, ty1 = tcs[0].ry1;
     const int cs = v->cs, ce = v->ce < h->C ? v->ce : h->C, rs = v->rs, re = v->re;
     const int le = v->le < h->g.nlayers ? v->le : h->g.nlayers;
     int rc;
@@ -1473,7 +1488,7 @@ static int read_tile_packets(const hdr_t *h, tc_t *tcs, const uint8_t *d, size_t
     for (int c = 0; c < h->C; c++)
         for (int r = 0; r < tcs[c].nres; r++) {
             int lv = tcs[c].nres - 1 - r;
-            long a = 1L << (tcs[c].r[r].ppx + lv), b = 1L << (tcs[c].r[r].ppy + lv);
+            long a = (long)tcs[c].dx << (tcs[c].r[r].ppx + lv), b = (long)tcs[c].dy << (tcs[c].r[r].ppy + lv);
             if (a < xs) xs = a;
             if (b < ys) ys = b;
         }
@@ -1678,6 +1693,10 @@ int or_read_info(const uint8_t *cs0, size_t len0, or_info *info) {
     info->progression = h.g.prog;
     info->qstyle = h.qc[0].style;
     info->guard_bits = h.qc[0].guard;
+    for (int i = 0; i < h.C && i < 16; i++) {
+        info->dx[i] = h.dx[i];
+        info->dy[i] = h.dy[i];
+    }
     return OR_OK;
 }
 
@@ -1797,6 +1816,7 @@ typedef struct {
     double *out_f;
     int W, H, X0, Y0;
     int reduce; /* finest DWT levels not reconstructed */
+    int ix0, iy0, ix1, iy1; /* the image area at full resolution */
 } dec_ctx;
 
 /* Mallat position of a band of resolution r inside the tile-component array. */
@@ -1819,15 +1839,20 @@ static int decode_tile(const hdr_t *h, int t, tc_t *tcs, void *user) {
     if (C <= 0) return OR_ERR_UNSUPPORTED;
     for (int c = 0; c < C; c++)
         if (tcs[c].nres - 1 < D->reduce) return OR_ERR_UNSUPPORTED;
-    const res_t *top0 = &tcs[0].r[tcs[0].nres - 1 - D->reduce];
-    int tw = top0->x1 - top0->x0, th = top0->y1 - top0->y0;
-    long n = (long)tw * th;
+    int tws[MAXC], ths[MAXC]; /* each tile-component's (reduced) size */
+    for (int c = 0; c < C; c++) {
+        const res_t *top = &tcs[c].r[tcs[c].nres - 1 - D->reduce];
+        tws[c] = top->x1 - top->x0;
+        ths[c] = top->y1 - top->y0;
+    }
     double **planes = (double **)calloc((size_t)C, sizeof(double *));
     if (!planes) return OR_ERR_NOMEM;
     int rc = OR_OK;
     for (int c = 0; c < C && !rc; c++) {
         tc_t *tc = &tcs[c];
         int rev = tc->cc.xform == 1;
+        const int tw = tws[c], th = ths[c];
+        const long n = (long)tw * th;
         int32_t *ia = rev ? (int32_t *)calloc((size_t)n, sizeof(int32_t)) : NULL;
         double *da = (double *)calloc((size_t)n, sizeof(double));
         planes[c] = da;
@@ -1896,7 +1921,10 @@ static int decode_tile(const hdr_t *h, int t, tc_t *tcs, void *user) {
         /* inverse MCT (G.2 RCT / G.3 ICT), TC/Decoder.cpp:163-180 selection */
         if (h->g.mct && C >= 3) {
             int rev = tcs[0].cc.xform == 1;
+            for (int c = 0; c < 3; c++)
+                if (h->dx[c] != 1 || h->dy[c] != 1) rc = OR_ERR_UNSUPPORTED; /* MCT needs equal components */
             if (tcs[1].cc.xform != tcs[0].cc.xform || tcs[2].cc.xform != tcs[0].cc.xform) rc = OR_ERR_UNSUPPORTED;
+            const long n = (long)tws[0] * ths[0];
             for (long i = 0; i < n && !rc; i++) {
                 double y0 = planes[0][i], y1 = planes[1][i], y2 = planes[2][i];
                 if (rev) {
@@ -1918,6 +1946,7 @@ static int decode_tile(const hdr_t *h, int t, tc_t *tcs, void *user) {
             double shift = h->sgnd[c] ? 0.0 : ldexp(1.0, h->prec[c] - 1);
             long lo = h->sgnd[c] ? -(1L << (h->prec[c] - 1)) : 0;
             long hi = h->sgnd[c] ? (1L << (h->prec[c] - 1)) - 1 : (1L << h->prec[c]) - 1;
+            const int tw = tws[c], th = ths[c];
             for (int y = 0; y < th; y++)
                 for (int x = 0; x < tw; x++) {
                     double v = planes[c][(long)y * tw + x] + shift;
@@ -1925,7 +1954,12 @@ static int decode_tile(const hdr_t *h, int t, tc_t *tcs, void *user) {
                     if (iv < lo) iv = lo;
                     if (iv > hi) iv = hi;
                     const res_t *top = &tcs[c].r[tcs[c].nres - 1 - D->reduce];
-                    long o = (long)c * D->W * D->H + (long)(top->y0 - D->Y0 + y) * D->W + (top->x0 - D->X0 + x);
+                    /* component c's own area (ceil(area / XRsiz), B-12, then the
+                     * reduction), packed at pitch cw from c * W * H */
+                    long s2 = 1L << D->reduce;
+                    long cx0 = ceildiv(ceildiv(D->ix0, h->dx[c]), s2), cy0 = ceildiv(ceildiv(D->iy0, h->dy[c]), s2);
+                    long cw = ceildiv(ceildiv(D->ix1, h->dx[c]), s2) - cx0;
+                    long o = (long)c * D->W * D->H + (long)(top->y0 - cy0 + y) * cw + (top->x0 - cx0 + x);
                     D->out[o] = (int32_t)iv;
                     if (D->out_f) D->out_f[o] = v;
                 }
@@ -1950,7 +1984,7 @@ int or_decode_reduced(const uint8_t *cs, size_t len, int reduce, int32_t *out, d
     if (reduce < 0 || reduce > 32) return -OR_ERR_UNSUPPORTED;
     int X0 = ceil_shift(info.x0, reduce), Y0 = ceil_shift(info.y0, reduce);
     int X1 = ceil_shift(info.x1, reduce), Y1 = ceil_shift(info.y1, reduce);
-    dec_ctx D = {out, out_f, X1 - X0, Y1 - Y0, X0, Y0, reduce};
+    dec_ctx D = {out, out_f, X1 - X0, Y1 - Y0, X0, Y0, reduce, info.x0, info.y0, info.x1, info.y1};
     rc = walk_tiles(cs, len, decode_tile, &D);
     return rc ? -rc : OR_OK;
 }

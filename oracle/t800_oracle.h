@@ -53,6 +53,7 @@ typedef struct {
     int32_t progression;
     int32_t qstyle;              /* quantization style of component 0 */
     int32_t guard_bits;
+    int32_t dx[16], dy[16];      /* component sub-sampling XRsiz, YRsiz */
 } or_info;
 
 typedef struct {

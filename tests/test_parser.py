@@ -62,9 +62,9 @@ def test_damaged_codestreams_fail_cleanly():
 
 def test_unsupported_features_are_reported():
     cs = bytearray(open(os.path.join(GOLDEN, "openjpeg_ll_rgb64.j2k"), "rb").read())
-    # sub-sampled component: XRsiz of component 0 (SIZ at offset 2, Ssiz list at 2+2+2+36+2)
+    # bit depth 20 on component 0 (SIZ at offset 2, Ssiz list at 2+2+2+36+2)
     ssiz = 2 + 2 + 2 + 2 + 32 + 2  # SOC, SIZ, Lsiz, Rsiz, 8 x 32-bit sizes, Csiz
-    cs[ssiz + 1] = 2
+    cs[ssiz] = 19
     with pytest.raises(tca.TCError) as e:
         tca.probe(bytes(cs))
     assert e.value.status == 4

@@ -31,7 +31,8 @@ class ImageInfo(ctypes.Structure):
     _fields_ = [(n, ctypes.c_int32) for n in (
         "x0", "y0", "width", "height", "ncomp", "max_prec", "bytes_per_sample", "reversible",
         "nlevels", "ntx", "nty", "cbw", "cbh", "nlayers", "mct", "progression")] + [
-        ("prec", ctypes.c_int32 * 4), ("sgnd", ctypes.c_int32 * 4)]
+        ("prec", ctypes.c_int32 * 4), ("sgnd", ctypes.c_int32 * 4), ("planar", ctypes.c_int32),
+        ("comp_width", ctypes.c_int32 * 4), ("comp_height", ctypes.c_int32 * 4)]
 
 
 class CblkInfo(ctypes.Structure):
@@ -239,7 +240,21 @@ def default_device():
 
 def _shape(info: ImageInfo):
     dt = np.uint8 if info.bytes_per_sample == 1 else np.uint16
+    if info.planar:
+        return (sum(info.comp_width[c] * info.comp_height[c] for c in range(info.ncomp)),), dt
     return (info.height, info.width, info.ncomp), dt
+
+
+def _unpack(out, info: ImageInfo):
+    """Planar output (sub-sampled components) -> one 2-D array per component."""
+    if not info.planar:
+        return out
+    res, off = [], 0
+    for c in range(info.ncomp):
+        w, h = info.comp_width[c], info.comp_height[c]
+        res.append(out[off:off + w * h].reshape(h, w))
+        off += w * h
+    return res
 
 
 def decode_staged(cs: bytes, device: "Device" = None) -> np.ndarray:
@@ -463,7 +478,7 @@ class Decoder:
         shape, dt = _shape(info)
         out = np.empty(shape, dt)
         check(lib().tc_decoder_decode(self.handle, cs, len(cs), out.ctypes.data, out.nbytes, None), "decode")
-        return out
+        return _unpack(out, info)
 
     def prepare(self, streams):
         streams = [bytes(s) for s in streams]
@@ -491,7 +506,7 @@ class Decoder:
         shape, dt = _shape(self._infos[i])
         out = np.empty(shape, dt)
         check(lib().tc_decoder_fetch(self.handle, i, out.ctypes.data, out.nbytes), "fetch")
-        return out
+        return _unpack(out, self._infos[i])
 
     def stats(self) -> DecoderStats:
         s = DecoderStats()

@@ -794,6 +794,7 @@ struct Params {
     int cbsty = 0;        // code-block style (RESET 2, CAUSAL 8, PTERM 16, SEGSYM 32)
     int packed = 0;       // packet headers: 0 in the packets, 1 PPT, 2 PPM (A.7.4, A.7.5)
     int packed_max = 0;   // largest payload of one PPT/PPM segment (0: 65532)
+    int dx[4] = {1, 1, 1, 1}, dy[4] = {1, 1, 1, 1};  // component sub-sampling (XRsiz, YRsiz)
 };
 
 // L2 norm of the synthesis basis function of a band at level nb (orientation o),
@@ -853,6 +854,12 @@ int encode(const void* pixels, const Params& P, std::vector<uint8_t>& out) {
     const int C = P.ncomp, W = P.w, H = P.h;
     const int tw = P.tw > 0 ? P.tw : W, th = P.th > 0 ? P.th : H;
     const int ntx = (W + tw - 1) / tw, nty = (H + th - 1) / th;
+    bool subsampled = false;
+    for (int c = 0; c < C; c++) {
+        if (P.dx[c] < 1 || P.dy[c] < 1 || P.dx[c] > 255 || P.dy[c] > 255) return -1;
+        subsampled |= P.dx[c] != 1 || P.dy[c] != 1;
+    }
+    if (subsampled && P.mct) return -1;  // the colour transform needs equal components
     const bool mct = P.mct && C >= 3;
     const int G = 2;  // guard bits
     // quantization parameters per (level, orientation), shared by every tile
@@ -892,19 +899,24 @@ int encode(const void* pixels, const Params& P, std::vector<uint8_t>& out) {
         std::vector<uint8_t> data;
     };
     std::vector<TileWork> tiles(ntx * nty);
+    auto tx0_ref = [&](int px) { return px * tw; };
+    auto ty0_ref = [&](int py) { return py * th; };
+    auto tx1_ref = [&](int px) { return std::min(W, (px + 1) * tw); };
+    auto ty1_ref = [&](int py) { return std::min(H, (py + 1) * th); };
     for (int t = 0; t < ntx * nty; t++) {
         TileWork& T = tiles[t];
         int px = t % ntx, py = t / ntx;
-        int x0 = px * tw, y0 = py * th, x1 = std::min(W, x0 + tw), y1 = std::min(H, y0 + th);
         T.comps.resize(C);
         T.cbs.resize(C);
         T.coef.resize(C);
         for (int c = 0; c < C; c++) {
             TCG& g = T.comps[c];
-            g.x0 = x0;
-            g.y0 = y0;
-            g.x1 = x1;
-            g.y1 = y1;
+            // tile-component rectangle, T.800 B-12
+            g.x0 = (int)ceil_div(tx0_ref(px), P.dx[c]);
+            g.y0 = (int)ceil_div(ty0_ref(py), P.dy[c]);
+            g.x1 = (int)ceil_div(tx1_ref(px), P.dx[c]);
+            g.y1 = (int)ceil_div(ty1_ref(py), P.dy[c]);
+            const int x0 = g.x0, y0 = g.y0, x1 = g.x1, y1 = g.y1;
             g.res.resize(P.levels + 1);
             for (int r = 0; r <= P.levels; r++) {
                 int nb = r == 0 ? P.levels : P.levels - r + 1;
@@ -972,8 +984,10 @@ int encode(const void* pixels, const Params& P, std::vector<uint8_t>& out) {
     // colour transform + DWT + quantization per tile-component
     const uint8_t* p8 = (const uint8_t*)pixels;
     const uint16_t* p16 = (const uint16_t*)pixels;
+    // component c's sample at its own coordinates (x, y): the source pixel at
+    // (x * XRsiz, y * YRsiz) on the reference grid
     auto sample = [&](int x, int y, int c) -> int32_t {
-        size_t i = ((size_t)y * W + x) * C + c;
+        size_t i = ((size_t)y * P.dy[c] * W + (size_t)x * P.dx[c]) * C + c;
         return P.prec <= 8 ? (int32_t)p8[i] : (int32_t)p16[i];
     };
     const int32_t shift = 1 << (P.prec - 1);
@@ -1130,8 +1144,8 @@ int encode(const void* pixels, const Params& P, std::vector<uint8_t>& out) {
     put16(out, (uint32_t)C);
     for (int c = 0; c < C; c++) {
         out.push_back((uint8_t)(P.prec - 1));
-        out.push_back(1);
-        out.push_back(1);
+        out.push_back((uint8_t)P.dx[c]);
+        out.push_back((uint8_t)P.dy[c]);
     }
     put16(out, 0xFF52);  // COD
     put16(out, 12);
@@ -1314,12 +1328,17 @@ extern "C" {
 // Returns 0 and a malloc'ed codestream in *out (free with tce_free).
 int tce_encode(const void* pixels, int w, int h, int ncomp, int prec, int levels, int cbw_exp, int cbh_exp,
                int tile_w, int tile_h, int mct, int irreversible, double target_bytes, double base_step, int threads,
-               int cbsty, int packed, int packed_max, uint8_t** out, size_t* out_len) {
+               int cbsty, int packed, int packed_max, const int* sub, uint8_t** out, size_t* out_len) {
     if (!pixels || !out || !out_len || w <= 0 || h <= 0 || (cbsty & ~63) || packed < 0 || packed > 2)
         return -1;
     tce::Params P{w, h, ncomp, prec, levels, cbw_exp, cbh_exp, tile_w, tile_h, mct, irreversible,
                   threads > 0 ? threads : (int)std::max(1u, std::thread::hardware_concurrency()), target_bytes,
                   base_step > 0 ? base_step : 0.5, cbsty, packed, packed_max};
+    if (sub)  // sub[2c], sub[2c+1]: XRsiz, YRsiz of component c (NULL: none)
+        for (int c = 0; c < ncomp && c < 4; c++) {
+            P.dx[c] = sub[2 * c];
+            P.dy[c] = sub[2 * c + 1];
+        }
     std::vector<uint8_t> cs;
     int rc = tce::encode(pixels, P, cs);
     if (rc) return rc;

@@ -413,14 +413,17 @@ Status parse_main_header(Reader& r, Image& img, Params& P, std::string& err) {
     if (img.ncomp < 1 || img.ncomp > 16384) return Status::Unsupported;
     img.prec.resize(img.ncomp);
     img.sgnd.resize(img.ncomp);
+    img.dx.resize(img.ncomp);
+    img.dy.resize(img.ncomp);
     for (int i = 0; i < img.ncomp; i++) {
         uint32_t s = r.u8();
         img.prec[i] = (int)(s & 0x7F) + 1;
         img.sgnd[i] = (int)(s >> 7);
-        uint32_t dx = r.u8(), dy = r.u8();
-        if (dx != 1 || dy != 1) {
-            err = "component sub-sampling is not supported";
-            return Status::Unsupported;
+        img.dx[i] = (int)r.u8();
+        img.dy[i] = (int)r.u8();
+        if (r.ok() && (img.dx[i] < 1 || img.dy[i] < 1)) {
+            err = "component sub-sampling of 0";
+            return Status::BadMarker;
         }
         if (img.prec[i] > 16) {
             err = "bit depth above 16";
@@ -485,10 +488,11 @@ Status build_tile(Image& img, int t, const Params& P, TileState& bstate, std::st
         const CodingStyle& C = P.cs[c];
         const Quant& Q = P.q[c];
         TileComp& tc = T.comps[c];
-        tc.x0 = T.x0;
-        tc.y0 = T.y0;
-        tc.x1 = T.x1;
-        tc.y1 = T.y1;
+        // tile-component rectangle, T.800 B-12
+        tc.x0 = (int)ceil_div(T.x0, img.dx[c]);
+        tc.y0 = (int)ceil_div(T.y0, img.dy[c]);
+        tc.x1 = (int)ceil_div(T.x1, img.dx[c]);
+        tc.y1 = (int)ceil_div(T.y1, img.dy[c]);
         tc.nlev = C.nlev;
         tc.reversible = C.xform;
         tc.cbsty = C.cbsty;
@@ -790,10 +794,11 @@ Status read_packets(Image& img, int t, const Params& P, TileState& bstate, const
                 const int lv = tc.nlev - r;
                 const int64_t sx = (int64_t)1 << (RS.ppx + lv), sy = (int64_t)1 << (RS.ppy + lv);
                 const bool ux = (((int64_t)R.x0 << lv) % sx) != 0, uy = (((int64_t)R.y0 << lv) % sy) != 0;
+                // on the reference grid a precinct spans XRsiz * 2^(PPx + levels) samples
                 for (int q = 0; q < RS.pw * RS.ph; q++) {
                     int i = q % RS.pw, j = q / RS.pw;
-                    int64_t x = (i == 0 && ux) ? tc.x0 : (RS.px0 + i) * sx;
-                    int64_t y = (j == 0 && uy) ? tc.y0 : (RS.py0 + j) * sy;
+                    int64_t x = (i == 0 && ux) ? T.x0 : (RS.px0 + i) * sx * img.dx[c];
+                    int64_t y = (j == 0 && uy) ? T.y0 : (RS.py0 + j) * sy * img.dy[c];
                     starts.push_back(Start{y, x, c, r, q});
                 }
             }
@@ -856,6 +861,10 @@ Status parse_impl(const uint8_t* data, size_t len, Image& img, std::string& err,
     img.wy0 = img.y0;
     img.wx1 = img.x1;
     img.wy1 = img.y1;
+    img.fx0 = img.x0;
+    img.fy0 = img.y0;
+    img.fx1 = img.x1;
+    img.fy1 = img.y1;
     if (s != Status::Ok || !packets) {
         if (s == Status::Ok) {
             // geometry of tile 0 only, for probing
@@ -1047,10 +1056,15 @@ Status reduce_image(Image& img, int k, std::string& err) {
     img.wy0 = img.y0;
     img.wx1 = img.x1;
     img.wy1 = img.y1;
+    img.reduced += k;
     return Status::Ok;
 }
 
 Status window_image(Image& img, int x, int y, int w, int h, std::string& err) {
+    if (img.planar()) {
+        err = "region decode of sub-sampled components is not supported";
+        return Status::Unsupported;
+    }
     if (x < 0 || y < 0 || w <= 0 || h <= 0 || x >= img.width() || y >= img.height()) {
         err = "region outside the image";
         return Status::InvalidArgument;

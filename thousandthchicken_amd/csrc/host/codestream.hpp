@@ -15,6 +15,7 @@
 #include <stddef.h>
 #include <stdint.h>
 
+#include <algorithm>
 #include <string>
 #include <vector>
 
@@ -92,6 +93,7 @@ struct Image {
     int tx0 = 0, ty0 = 0, tw = 0, th = 0, ntx = 0, nty = 0;
     int ncomp = 0;
     std::vector<int> prec, sgnd;
+    std::vector<int> dx, dy;  // component sub-sampling XRsiz, YRsiz
     // COD (main header)
     int progression = 0, nlayers = 1, mct = 0;
     std::vector<Tile> tiles;
@@ -109,6 +111,37 @@ struct Image {
     int height() const { return y1 - y0; }
     int out_width() const { return wx1 - wx0; }
     int out_height() const { return wy1 - wy0; }
+    // some component sub-sampled: the output holds each component as its own
+    // plane (comp_rect), one after the other
+    bool planar() const {
+        for (size_t c = 0; c < dx.size(); c++)
+            if (dx[c] != 1 || dy[c] != 1) return true;
+        return false;
+    }
+    // full-resolution image area and the levels a reduce discarded
+    int fx0 = 0, fy0 = 0, fx1 = 0, fy1 = 0, reduced = 0;
+    // component c's area in its own coordinates: ceil(area / XRsiz) (B-12),
+    // then ceil(/ 2^reduced)
+    void comp_rect(int c, int& x0c, int& y0c, int& x1c, int& y1c) const {
+        auto cd = [](int64_t v, int64_t d) { return v >= 0 ? (v + d - 1) / d : -((-v) / d); };
+        const int64_t s = (int64_t)1 << reduced;
+        x0c = (int)cd(cd(fx0, dx[c]), s);
+        y0c = (int)cd(cd(fy0, dy[c]), s);
+        x1c = (int)cd(cd(fx1, dx[c]), s);
+        y1c = (int)cd(cd(fy1, dy[c]), s);
+    }
+    // bytes of the decoded output
+    uint64_t out_bytes() const {
+        const uint64_t bps = max_prec() > 8 ? 2 : 1;
+        if (!planar()) return (uint64_t)out_width() * out_height() * ncomp * bps;
+        uint64_t n = 0;
+        for (int c = 0; c < ncomp; c++) {
+            int a, b, e, f;
+            comp_rect(c, a, b, e, f);
+            n += (uint64_t)(e - a) * (f - b);
+        }
+        return n * bps;
+    }
     int max_prec() const;
     bool all_reversible() const;
 };

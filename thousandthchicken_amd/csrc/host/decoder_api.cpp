@@ -69,7 +69,16 @@ void fill_info(const Image& im, tc_image_info* info) {
     for (int c = 0; c < 4 && c < im.ncomp; c++) {
         info->prec[c] = im.prec[c];
         info->sgnd[c] = im.sgnd[c];
+        int x0, y0, x1, y1;
+        if (im.planar()) {
+            im.comp_rect(c, x0, y0, x1, y1);
+        } else {
+            x0 = im.wx0, y0 = im.wy0, x1 = im.wx1, y1 = im.wy1;
+        }
+        info->comp_width[c] = x1 - x0;
+        info->comp_height[c] = y1 - y0;
     }
+    info->planar = im.planar() ? 1 : 0;
 }
 
 uint32_t fnv1a(const uint8_t* p, size_t n, uint32_t h) {
@@ -338,6 +347,10 @@ tc_status tc_decoder_prepare(tc_decoder d, int n, const uint8_t* const* cs, cons
         if (s == Status::Ok) s = reduce_image(*d->imgs.back(), d->reduce, err);
         if (s == Status::Ok && d->region[2] > 0 && d->region[3] > 0)
             s = window_image(*d->imgs.back(), d->region[0], d->region[1], d->region[2], d->region[3], err);
+        if (s == Status::Ok && d->imgs.back()->planar() && d->imgs.back()->ncomp > 4) {
+            s = Status::Unsupported;
+            err = "sub-sampled images of more than four components are not supported";
+        }
         if (s != Status::Ok) return report(s, "image " + std::to_string(i) + ": " + err);
         v.push_back(d->imgs.back().get());
     }

@@ -4,6 +4,7 @@
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
+#include <array>
 #include <cstring>
 #include <thread>
 
@@ -280,16 +281,25 @@ Status Pipeline::prepare(const std::vector<const Image*>& imgs, std::string& err
         io.bps = im.max_prec() > 8 ? 2 : 1;
         io.off = align_up(out_bytes_total, 256);
         const int ow = im.out_width(), oh = im.out_height();
-        io.bytes = (uint64_t)ow * oh * im.ncomp * io.bps;
+        const bool planar = im.planar();
+        io.bytes = im.out_bytes();
         out_bytes_total = io.off + io.bytes;
         images_.push_back(io);
         npixels_ += (int64_t)ow * oh;
-        nsamples_ += (int64_t)ow * oh * im.ncomp;
+        nsamples_ += (int64_t)(io.bytes / io.bps);
         uint64_t out_elem_base = io.off / io.bps;
+        // planar output (sub-sampled components): each component's plane, in turn
+        std::vector<uint64_t> cbase(im.ncomp, 0);
+        std::vector<std::array<int, 4>> crect(im.ncomp);
+        for (int c = 0, acc = 0; planar && c < im.ncomp; c++) {
+            im.comp_rect(c, crect[c][0], crect[c][1], crect[c][2], crect[c][3]);
+            cbase[c] = out_elem_base + (uint64_t)acc;
+            acc += (crect[c][2] - crect[c][0]) * (crect[c][3] - crect[c][1]);
+        }
         for (size_t t = 0; t < im.tiles.size(); t++) {
             const Tile& T = im.tiles[t];
             std::vector<std::vector<int>> sets;
-            bool mct3 = T.mct && im.ncomp >= 3 && T.comps[0].reversible == T.comps[1].reversible &&
+            bool mct3 = !planar && T.mct && im.ncomp >= 3 && T.comps[0].reversible == T.comps[1].reversible &&
                         T.comps[0].reversible == T.comps[2].reversible && T.comps[0].nlev == T.comps[1].nlev &&
                         T.comps[0].nlev == T.comps[2].nlev;
             int c0 = 0;
@@ -302,9 +312,10 @@ Status Pipeline::prepare(const std::vector<const Image*>& imgs, std::string& err
                 const TileComp& tc0 = T.comps[set[0]];
                 const Resolution& R = tc0.res[tc0.nlev];
                 if (R.width() <= 0 || R.height() <= 0) continue;
-                // the part of the output window this tile covers
-                const int jx0 = std::max(R.x0, im.wx0), jx1 = std::min(R.x1, im.wx1);
-                const int jy0 = std::max(R.y0, im.wy0), jy1 = std::min(R.y1, im.wy1);
+                // the part of the output window this tile covers (planar: the
+                // whole tile-component)
+                const int jx0 = planar ? R.x0 : std::max(R.x0, im.wx0), jx1 = planar ? R.x1 : std::min(R.x1, im.wx1);
+                const int jy0 = planar ? R.y0 : std::max(R.y0, im.wy0), jy1 = planar ? R.y1 : std::min(R.y1, im.wy1);
                 if (jx1 <= jx0 || jy1 <= jy0) continue;
                 tc_final_job f;
                 std::memset(&f, 0, sizeof f);
@@ -351,10 +362,18 @@ Status Pipeline::prepare(const std::vector<const Image*>& imgs, std::string& err
                         f.maxv[k] = (1 << prec) - 1;
                     }
                 }
-                f.out_off = out_elem_base + ((uint64_t)(jy0 - im.wy0) * ow + (uint64_t)(jx0 - im.wx0)) * im.ncomp +
-                            (uint64_t)set[0];
-                f.out_pitch = (uint32_t)((uint64_t)ow * im.ncomp);
-                f.out_cstride = (uint32_t)im.ncomp;
+                if (planar) {
+                    const auto& cr = crect[set[0]];
+                    const int cw = cr[2] - cr[0];
+                    f.out_off = cbase[set[0]] + (uint64_t)(jy0 - cr[1]) * cw + (uint64_t)(jx0 - cr[0]);
+                    f.out_pitch = (uint32_t)cw;
+                    f.out_cstride = 1;
+                } else {
+                    f.out_off = out_elem_base + ((uint64_t)(jy0 - im.wy0) * ow + (uint64_t)(jx0 - im.wx0)) * im.ncomp +
+                                (uint64_t)set[0];
+                    f.out_pitch = (uint32_t)((uint64_t)ow * im.ncomp);
+                    f.out_cstride = (uint32_t)im.ncomp;
+                }
                 int irr = tc0.reversible ? 0 : 1, nc = (int)set.size();
                 f.tiles_x = (jx1 - jx0 + TX - 1) / TX;
                 int tiles_y = (jy1 - jy0 + TY - 1) / TY;

@@ -64,7 +64,7 @@ uint64_t footprint_of(const Image& im) {
             f += ((uint64_t)c.width() * c.height() + sz[0] + sz[1] + 192) * 4;
         }
     f *= (uint64_t)std::max<size_t>(im.tiles.size(), 1);
-    f += (uint64_t)im.out_width() * im.out_height() * im.ncomp * (im.max_prec() > 8 ? 2 : 1);
+    f += im.out_bytes();
     f += im.cs_len + im.cs_len / 16 * 40 + 4096;
     return f;
 }
@@ -127,7 +127,9 @@ struct tc_server_s {
         auto parse_part = [&](int k) {
             std::string perr;
             for (size_t i = (size_t)k; i < s.batch.size(); i += (size_t)nt)
-                good[i] = parse_codestream(s.batch[i]->cs, s.batch[i]->cs_len, s.batch[i]->img, perr) == Status::Ok;
+                // (sub-sampled images, whose output is planar, are not served)
+                good[i] = parse_codestream(s.batch[i]->cs, s.batch[i]->cs_len, s.batch[i]->img, perr) == Status::Ok &&
+                          !s.batch[i]->img.planar();
         };
         if (nt <= 1) {
             parse_part(0);

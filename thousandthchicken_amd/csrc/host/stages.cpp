@@ -517,6 +517,11 @@ int Decoder::decode_staged(const uint8_t* data, size_t len) {
         set_error(err);
         return -1;
     }
+    if (S->img.planar()) {
+        // the reference's type_image has one size for every component
+        set_error("staged decode of sub-sampled components is not supported (use tc_decoder_decode)");
+        return -1;
+    }
     const Image& im = S->img;
     type_image img;
     std::memset(&img, 0, sizeof img);

@@ -139,7 +139,7 @@ def _enc_lib():
             _b.build_encoder()
         L = ctypes.CDLL(path)
         vp, i32 = ctypes.c_void_p, ctypes.c_int
-        L.tce_encode.argtypes = [vp] + [i32] * 10 + [i32, ctypes.c_double, ctypes.c_double, i32, i32, i32, i32,
+        L.tce_encode.argtypes = [vp] + [i32] * 10 + [i32, ctypes.c_double, ctypes.c_double, i32, i32, i32, i32, vp,
                                                       ctypes.POINTER(ctypes.c_void_p),
                                                       ctypes.POINTER(ctypes.c_size_t)]
         L.tce_free.argtypes = [vp]
@@ -153,7 +153,7 @@ def _enc_lib():
 
 
 def encode(img, bits=None, levels=5, cb=64, tile=None, mct=None, lossy=False, bpp=None, base_step=0.5,
-           threads=0, cbsty=0, packed=None, packed_max=0):
+           threads=0, cbsty=0, packed=None, packed_max=0, subsampling=None):
     """Encodes img [H, W] or [H, W, C] (uint8 / uint16) into a raw codestream.
 
     Lossless: RCT (when mct) + 5/3, every pass kept.  Lossy: ICT + 9/7 with
@@ -161,7 +161,9 @@ def encode(img, bits=None, levels=5, cb=64, tile=None, mct=None, lossy=False, bp
     code-block style bits written to COD and used by tier-1 (BYPASS 1, RESET 2,
     TERMALL 4, CAUSAL 8, PTERM 16, SEGSYM 32).  packed: None, "ppt" or "ppm" -- packet headers in
     the packets, in PPT segments of the tile-part headers or in PPM segments of
-    the main header; packed_max caps each segment's payload (0: 65532)."""
+    the main header; packed_max caps each segment's payload (0: 65532).
+    subsampling: per component (XRsiz, YRsiz); component c is then coded from
+    the samples img[::YRsiz, ::XRsiz, c] (no colour transform)."""
     import ctypes
     a = np.ascontiguousarray(img)
     if a.ndim == 2:
@@ -174,7 +176,10 @@ def encode(img, bits=None, levels=5, cb=64, tile=None, mct=None, lossy=False, bp
     else:
         a = np.ascontiguousarray(a.astype(np.uint8))
     if mct is None:
-        mct = c >= 3
+        mct = c >= 3 and not (subsampling and any(d != (1, 1) for d in subsampling))
+    sub = None
+    if subsampling:
+        sub = np.asarray([v for d in subsampling for v in d], np.int32)
     tw, th = (tile if tile else (0, 0))
     target = (bpp * w * h / 8.0) if (lossy and bpp) else 0.0
     out = ctypes.c_void_p()
@@ -182,7 +187,8 @@ def encode(img, bits=None, levels=5, cb=64, tile=None, mct=None, lossy=False, bp
     cbe = int(cb).bit_length() - 1
     rc = _enc_lib().tce_encode(a.ctypes.data, w, h, c, bits, levels, cbe, cbe, tw, th, 1 if mct else 0,
                                1 if lossy else 0, target, base_step, threads, cbsty,
-                               {None: 0, "ppt": 1, "ppm": 2}[packed], packed_max, ctypes.byref(out),
+                               {None: 0, "ppt": 1, "ppm": 2}[packed], packed_max,
+                               None if sub is None else sub.ctypes.data, ctypes.byref(out),
                                ctypes.byref(n))
     if rc:
         raise RuntimeError(f"tce_encode failed ({rc})")
