@@ -107,6 +107,14 @@ tc_status tc_decoder_set_reduce(tc_decoder dec, int reduce);
  * batch is its window, row-major and interleaved like the full image. */
 tc_status tc_decoder_set_region(tc_decoder dec, int x, int y, int w, int h);
 
+/* Zero-copy host output (applies from the next prepare or decode; no
+ * counterpart in the reference, whose output is mapped back with
+ * clEnqueueMapBuffer, TC/Decoder.cpp:81-91): enable 1 makes the decoder keep
+ * its output images in pinned host memory that the final-level kernel writes
+ * directly over the bus.  tc_decoder_output then returns host addresses,
+ * readable once tc_decoder_sync returns; tc_decoder_fetch copies from them. */
+tc_status tc_decoder_set_host_output(tc_decoder dec, int enable);
+
 /* Kernel for blocks decoded one per wavefront: 0 = stripe-column form
  * (tc_launch_t1_decode_stripe, default), 1 = row-mask form
  * (tc_launch_t1_decode_wave). */

@@ -79,6 +79,7 @@ def lib():
         "tc_image_probe_reduced": (i32, [ctypes.c_char_p, sz, i32, P(ImageInfo)]),
         "tc_decoder_set_reduce": (i32, [vp, i32]),
         "tc_decoder_set_region": (i32, [vp, i32, i32, i32, i32]),
+        "tc_decoder_set_host_output": (i32, [vp, i32]),
         "tc_image_list_cblks": (i32, [ctypes.c_char_p, sz, vp, i32, P(i32)]),
         "tc_decoder_create": (i32, [vp, P(vp)]),
         "tc_decoder_destroy": (i32, [vp]),
@@ -469,6 +470,18 @@ class Decoder:
             info.width = max(0, min(w, info.width - x))
             info.height = max(0, min(h, info.height - y))
         return info
+
+    def set_host_output(self, enable=True):
+        """Output images in pinned host memory written directly by the final-level kernel."""
+        check(lib().tc_decoder_set_host_output(self.handle, 1 if enable else 0), "set_host_output")
+
+    def output_view(self, i) -> np.ndarray:
+        """Image i in place (host output: a view of the pinned buffer; valid until the next prepare)."""
+        p, n = ctypes.c_void_p(), ctypes.c_size_t()
+        check(lib().tc_decoder_output(self.handle, i, ctypes.byref(p), ctypes.byref(n)), "output")
+        shape, dt = _shape(self._infos[i])
+        buf = (ctypes.c_uint8 * n.value).from_address(p.value)
+        return _unpack(np.frombuffer(buf, dt).reshape(shape), self._infos[i])
 
     def set_blocks_per_wave(self, small, bpw):
         check(lib().tc_decoder_set_blocks_per_wave(self.handle, 1 if small else 0, bpw), "set_blocks_per_wave")

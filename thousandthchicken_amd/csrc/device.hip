@@ -41,6 +41,12 @@ hipError_t launch_colour(hipStream_t s, int kind, void* c0, void* c1, void* c2, 
 
 static thread_local std::string g_last_error;
 
+void* host_device_ptr(void* host) {
+    void* d = nullptr;
+    if (hipHostGetDevicePointer(&d, host, 0) != hipSuccess) return nullptr;
+    return d;
+}
+
 void set_error(const std::string& m) { g_last_error = m; }
 
 tc_status fail(tc_status code, const std::string& m) {

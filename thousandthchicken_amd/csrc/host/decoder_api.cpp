@@ -212,6 +212,12 @@ tc_status tc_decoder_set_region(tc_decoder d, int x, int y, int w, int h) {
     return TC_SUCCESS;
 }
 
+tc_status tc_decoder_set_host_output(tc_decoder d, int enable) {
+    if (!d) return report(Status::InvalidArgument, "tc_decoder_set_host_output: null");
+    d->pipe->set_host_output(enable != 0);
+    return TC_SUCCESS;
+}
+
 tc_status tc_decoder_set_t1_kernel(tc_decoder d, int kind) {
     if (!d || kind < 0 || kind > 1) return report(Status::InvalidArgument, "t1 kernel must be 0 (stripe) or 1 (row)");
     Tuning t = d->pipe->tuning();

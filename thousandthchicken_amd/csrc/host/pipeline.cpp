@@ -94,6 +94,7 @@ Pipeline::~Pipeline() {
     tc_buf_free(out_);
     tc_buf_free(scratch_);
     if (pinned_) tc_host_free_pinned(pinned_);
+    if (host_buf_) tc_host_free_pinned(host_buf_);
     tc_stream_destroy(stream_);
 }
 
@@ -405,7 +406,19 @@ Status Pipeline::prepare(const std::vector<const Image*>& imgs, std::string& err
     const uint64_t total = blob.base + blob.v.size();
     Status st = ensure(upload_, total, "descriptors", err);
     if (st == Status::Ok) st = ensure(samples_, std::max<uint64_t>(work, 1) * 4, "sample planes", err);
-    if (st == Status::Ok) st = ensure(out_, std::max<uint64_t>(out_bytes_total, 1), "output images", err);
+    if (st == Status::Ok && !host_out_) st = ensure(out_, std::max<uint64_t>(out_bytes_total, 1), "output images", err);
+    if (st == Status::Ok && host_out_ && host_cap_ < out_bytes_total) {
+        tc_stream_sync(stream_);
+        if (host_buf_) tc_host_free_pinned(host_buf_);
+        host_buf_ = host_dev_ = nullptr;
+        host_cap_ = 0;
+        const size_t want = std::max<size_t>(out_bytes_total, 4096);
+        if (tc_host_alloc_pinned(want, &host_buf_) != TC_SUCCESS || !(host_dev_ = host_device_ptr(host_buf_))) {
+            err = std::string("pinned output: ") + tc_last_error();
+            return Status::OutOfMemory;
+        }
+        host_cap_ = want;
+    }
     if (st == Status::Ok && scratch_total) st = ensure(scratch_, scratch_total, "tier-1 state", err);
     if (st == Status::Ok) st = ensure_pinned(total, err);
     if (st != Status::Ok) return st;
@@ -426,7 +439,7 @@ Status Pipeline::prepare(const std::vector<const Image*>& imgs, std::string& err
 Status Pipeline::run(bool timed) {
     uint8_t* up = (uint8_t*)tc_buf_device_ptr(upload_);
     void* base = tc_buf_device_ptr(samples_);
-    void* out = tc_buf_device_ptr(out_);
+    void* out = host_out_ ? host_dev_ : tc_buf_device_ptr(out_);
     if (!up || !base || !out) return Status::InvalidArgument;
     timed_ = timed;
     tc_status s = TC_SUCCESS;
@@ -503,8 +516,14 @@ Status Pipeline::fetch(int i, void* dst, size_t cap, std::string& err) {
         err = "output buffer too small";
         return Status::Capacity;
     }
-    tc_status s = tc_buf_copy_d2h(stream_, dst, out_, o.off, o.bytes);
-    if (s == TC_SUCCESS) s = tc_stream_sync(stream_);
+    tc_status s;
+    if (host_out_) {
+        s = tc_stream_sync(stream_);
+        if (s == TC_SUCCESS) std::memcpy(dst, (const uint8_t*)host_buf_ + o.off, o.bytes);
+    } else {
+        s = tc_buf_copy_d2h(stream_, dst, out_, o.off, o.bytes);
+        if (s == TC_SUCCESS) s = tc_stream_sync(stream_);
+    }
     if (s != TC_SUCCESS) {
         err = tc_last_error();
         return Status::Device;
@@ -513,6 +532,7 @@ Status Pipeline::fetch(int i, void* dst, size_t cap, std::string& err) {
 }
 
 const void* Pipeline::output_ptr(int i) const {
+    if (host_out_) return (const uint8_t*)host_buf_ + images_[i].off;
     return (const uint8_t*)tc_buf_device_ptr(out_) + images_[i].off;
 }
 

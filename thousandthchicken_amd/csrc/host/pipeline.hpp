@@ -62,6 +62,9 @@ class Pipeline {
     StageTimes times() const;
     tc_stream stream() const { return stream_; }
     void set_tuning(const Tuning& t) { tuning_ = t; }
+    // output images in pinned host memory, written by the final-level kernel
+    // directly (no device buffer, no copy); from the next prepare
+    void set_host_output(bool on) { host_out_ = on; }
     const Tuning& tuning() const { return tuning_; }
     // Statistics of the last prepare().
     int64_t num_cblks() const { return ncblk_; }
@@ -112,6 +115,10 @@ class Pipeline {
     tc_device dev_;
     tc_stream stream_ = nullptr;
     tc_buf upload_ = nullptr, samples_ = nullptr, out_ = nullptr, scratch_ = nullptr;
+    bool host_out_ = false;
+    void* host_buf_ = nullptr;  // pinned output (host_out_)
+    void* host_dev_ = nullptr;  // its device address
+    size_t host_cap_ = 0;
     void* pinned_ = nullptr;
     size_t pinned_cap_ = 0;
     std::vector<Group> groups_;

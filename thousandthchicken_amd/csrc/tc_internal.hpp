@@ -13,4 +13,6 @@ tc_status hip_fail(hipError_t e, const char* what);
 hipStream_t stream_handle(tc_stream s);
 int device_ordinal(tc_device d);
 int device_cu_count(tc_device d);
+// device address of pinned host memory from tc_host_alloc_pinned (kernels write it over the bus)
+void* host_device_ptr(void* host);
 }  // namespace tc
