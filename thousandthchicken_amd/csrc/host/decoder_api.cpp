@@ -3,6 +3,7 @@
 #include <cstring>
 #include <memory>
 #include <string>
+#include <thread>
 #include <vector>
 
 #include "../../../include/tc_decode.h"
@@ -343,22 +344,46 @@ tc_status tc_decoder_prepare(tc_decoder d, int n, const uint8_t* const* cs, cons
     d->pipe->sync();
     d->data.clear();
     d->imgs.clear();
+    for (int i = 0; i < n; i++)
+        if (!cs[i]) return report(Status::InvalidArgument, "tc_decoder_prepare: null codestream");
+    d->data.resize(n);
+    d->imgs.resize(n);
+    std::vector<Status> st(n, Status::Ok);
+    std::vector<std::string> errs(n);
+    // copy and parse the codestreams on up to eight threads (tier-2 of each
+    // image is independent); the resolution / region rewrites follow
+    auto work = [&](int k, int nt) {
+        for (int i = k; i < n; i += nt) {
+            d->data[i].assign(cs[i], cs[i] + lens[i]);
+            d->imgs[i].reset(new Image());
+            Image& im = *d->imgs[i];
+            std::string& err = errs[i];
+            Status s = parse_codestream(d->data[i].data(), d->data[i].size(), im, err);
+            if (s == Status::Ok) s = reduce_image(im, d->reduce, err);
+            if (s == Status::Ok && d->region[2] > 0 && d->region[3] > 0)
+                s = window_image(im, d->region[0], d->region[1], d->region[2], d->region[3], err);
+            if (s == Status::Ok && im.planar() && im.ncomp > 4) {
+                s = Status::Unsupported;
+                err = "sub-sampled images of more than four components are not supported";
+            }
+            st[i] = s;
+        }
+    };
+    size_t bytes = 0;
+    for (int i = 0; i < n; i++) bytes += lens[i];
+    const int nt = (int)std::min<size_t>({(size_t)8, (size_t)std::max(n, 1), 1 + bytes / (2u << 20)});
+    if (nt <= 1) {
+        work(0, 1);
+    } else {
+        std::vector<std::thread> th;
+        for (int k = 1; k < nt; k++) th.emplace_back(work, k, nt);
+        work(0, nt);
+        for (auto& t : th) t.join();
+    }
     std::vector<const Image*> v;
     for (int i = 0; i < n; i++) {
-        if (!cs[i]) return report(Status::InvalidArgument, "tc_decoder_prepare: null codestream");
-        d->data.emplace_back(cs[i], cs[i] + lens[i]);
-        d->imgs.emplace_back(new Image());
-        std::string err;
-        Status s = parse_codestream(d->data.back().data(), d->data.back().size(), *d->imgs.back(), err);
-        if (s == Status::Ok) s = reduce_image(*d->imgs.back(), d->reduce, err);
-        if (s == Status::Ok && d->region[2] > 0 && d->region[3] > 0)
-            s = window_image(*d->imgs.back(), d->region[0], d->region[1], d->region[2], d->region[3], err);
-        if (s == Status::Ok && d->imgs.back()->planar() && d->imgs.back()->ncomp > 4) {
-            s = Status::Unsupported;
-            err = "sub-sampled images of more than four components are not supported";
-        }
-        if (s != Status::Ok) return report(s, "image " + std::to_string(i) + ": " + err);
-        v.push_back(d->imgs.back().get());
+        if (st[i] != Status::Ok) return report(st[i], "image " + std::to_string(i) + ": " + errs[i]);
+        v.push_back(d->imgs[i].get());
     }
     std::string err;
     return report(d->pipe->prepare(v, err), err);
