@@ -56,28 +56,34 @@ template <bool IS97>
 struct FLift;
 
 // 1D_FILTD 5/3 (T.800 F.4.8.1) over LDS lines whose element 0 has parity par0.
+// The sweeps take NC component tiles at once (tile c at base + c * cs), so a
+// fused first level lifts every component between the same barriers.
 template <>
 struct FLift<false> {
     static constexpr int H = 2;
     typedef int32_t T;
-    __device__ static void steps(int32_t* base, int sp, int sl, int nlines, int n, int par0, int tid) {
+    template <int NC = 1>
+    __device__ static void steps(int32_t* base, int sp, int sl, int nlines, int n, int par0, int tid, int cs = 0) {
         const int half = (n + 1) / 2;
+        const int all = NC * nlines;
         // STEP1: odd -= floor((even_l + even_r) / 2)
-        for (int i = tid; i < nlines * half; i += NT) {
-            const int line = i % nlines, j = i / nlines;
+        for (int i = tid; i < all * half; i += NT) {
+            const int L = i % all, j = i / all;
+            const int c = NC == 1 ? 0 : L / nlines, line = NC == 1 ? L : L % nlines;
             const int pos = 2 * j + 1 - (par0 & 1);
             if (pos >= 1 && pos < n - 1) {
-                int32_t* p = base + line * sl + pos * sp;
+                int32_t* p = base + c * cs + line * sl + pos * sp;
                 p[0] -= (p[-sp] + p[sp]) >> 1;
             }
         }
         __syncthreads();
         // STEP2: even += floor((odd_l + odd_r + 2) / 4)
-        for (int i = tid; i < nlines * half; i += NT) {
-            const int line = i % nlines, j = i / nlines;
+        for (int i = tid; i < all * half; i += NT) {
+            const int L = i % all, j = i / all;
+            const int c = NC == 1 ? 0 : L / nlines, line = NC == 1 ? L : L % nlines;
             const int pos = 2 * j + (par0 & 1);
             if (pos >= 2 && pos < n - 2) {
-                int32_t* p = base + line * sl + pos * sp;
+                int32_t* p = base + c * cs + line * sl + pos * sp;
                 p[0] += (p[-sp] + p[sp] + 2) >> 2;
             }
         }
@@ -91,24 +97,29 @@ template <>
 struct FLift<true> {
     static constexpr int H = 4;
     typedef float T;
-    __device__ static void step(float* base, int sp, int sl, int nlines, int n, int par, int lo, float c, int tid) {
+    template <int NC>
+    __device__ static void step(float* base, int sp, int sl, int nlines, int n, int par, int lo, float k, int tid,
+                                int cs) {
         const int half = (n + 1) / 2;
-        for (int i = tid; i < nlines * half; i += NT) {
-            const int line = i % nlines, j = i / nlines;
+        const int all = NC * nlines;
+        for (int i = tid; i < all * half; i += NT) {
+            const int L = i % all, j = i / all;
+            const int c = NC == 1 ? 0 : L / nlines, line = NC == 1 ? L : L % nlines;
             const int pos = 2 * j + par;
             if (pos >= lo && pos < n - lo) {
-                float* p = base + line * sl + pos * sp;
-                p[0] = fmaf(c, p[-sp] + p[sp], p[0]);
+                float* p = base + c * cs + line * sl + pos * sp;
+                p[0] = fmaf(k, p[-sp] + p[sp], p[0]);
             }
         }
         __syncthreads();
     }
-    __device__ static void steps(float* base, int sp, int sl, int nlines, int n, int par0, int tid) {
+    template <int NC = 1>
+    __device__ static void steps(float* base, int sp, int sl, int nlines, int n, int par0, int tid, int cs = 0) {
         const int pe = par0 & 1, po = 1 - pe;
-        step(base, sp, sl, nlines, n, po, 1, kAlpha, tid);
-        step(base, sp, sl, nlines, n, pe, 2, kBeta, tid);
-        step(base, sp, sl, nlines, n, po, 3, kGamma, tid);
-        step(base, sp, sl, nlines, n, pe, 4, kDelta, tid);
+        step<NC>(base, sp, sl, nlines, n, po, 1, kAlpha, tid, cs);
+        step<NC>(base, sp, sl, nlines, n, pe, 2, kBeta, tid, cs);
+        step<NC>(base, sp, sl, nlines, n, po, 3, kGamma, tid, cs);
+        step<NC>(base, sp, sl, nlines, n, pe, 4, kDelta, tid, cs);
     }
 };
 
@@ -120,15 +131,66 @@ __device__ __forceinline__ float axis_gain(bool len1, int org, int p) {
     return IS97 ? (p ? kK : kInvK) : 1.0f;
 }
 
-// Lifts the loaded tile (columns, then rows) unless an axis has length 1.
-template <bool IS97, int TX, int TY>
+// One line of N samples lifted in registers (T.800 F.4.8 STEP1..4 for 9/7,
+// STEP1..2 for 5/3) over the same ranges as the LDS sweeps above: STEP s
+// updates positions [s, N - s) of its parity.  P: parity of position 0.  The
+// line is read from and written back to LDS at `stride`.
+template <bool IS97, int N, int P>
+__device__ __forceinline__ void lift_line(typename FLift<IS97>::T* p, int stride) {
+    typedef typename FLift<IS97>::T T;
+    T x[N];
+#pragma unroll
+    for (int i = 0; i < N; i++) x[i] = p[i * stride];
+    if (IS97) {
+        const float k[4] = {kAlpha, kBeta, kGamma, kDelta};
+#pragma unroll
+        for (int s = 1; s <= 4; s++)
+#pragma unroll
+            for (int i = s; i < N - s; i++)
+                if (((i + P) & 1) == (s & 1)) x[i] = fmaf(k[s - 1], x[i - 1] + x[i + 1], x[i]);
+    } else {
+#pragma unroll
+        for (int i = 1; i < N - 1; i++)
+            if ((i + P) & 1) x[i] -= ((int)x[i - 1] + (int)x[i + 1]) >> 1;
+#pragma unroll
+        for (int i = 2; i < N - 2; i++)
+            if (!((i + P) & 1)) x[i] += ((int)x[i - 1] + (int)x[i + 1] + 2) >> 2;
+    }
+#pragma unroll
+    for (int i = 0; i < N; i++) p[i * stride] = x[i];
+}
+
+// Lifts the loaded tiles of NC components (tile c at tile + c * cs): every
+// column, then every row, one line per thread in registers (no barrier inside
+// a sweep), unless an axis has length 1.
+template <bool IS97, int TX, int TY, int NC = 1>
 __device__ __forceinline__ void tile_fdwt(typename FLift<IS97>::T* tile, int pitch, int ox, int oy, bool h1,
-                                          bool v1x) {
+                                          bool v1x, int cs = 0) {
     constexpr int H = FLift<IS97>::H;
     constexpr int W = TX + 2 * H, HH = TY + 2 * H;
     const int tid = threadIdx.x;
-    if (!v1x) FLift<IS97>::steps(tile, pitch, 1, W, HH, (oy - H) & 1 ? 1 : 0, tid);
-    if (!h1) FLift<IS97>::steps(tile, 1, pitch, HH, W, (ox - H) & 1 ? 1 : 0, tid);
+    if (!v1x) {
+        const bool odd = ((oy - H) & 1) != 0;
+        for (int L = tid; L < NC * W; L += NT) {
+            typename FLift<IS97>::T* p = tile + (L / W) * cs + (L % W);
+            if (odd)
+                lift_line<IS97, HH, 1>(p, pitch);
+            else
+                lift_line<IS97, HH, 0>(p, pitch);
+        }
+        __syncthreads();
+    }
+    if (!h1) {
+        const bool odd = ((ox - H) & 1) != 0;
+        for (int L = tid; L < NC * HH; L += NT) {
+            typename FLift<IS97>::T* p = tile + (L / HH) * cs + (L % HH) * pitch;
+            if (odd)
+                lift_line<IS97, W, 1>(p, 1);
+            else
+                lift_line<IS97, W, 0>(p, 1);
+        }
+        __syncthreads();
+    }
 }
 
 // Writes the tile's four quarter-tiles to the subbands (consecutive threads on
@@ -177,12 +239,21 @@ __global__ __launch_bounds__(NT) void fdwt_level_kernel(const tc_fdwt_job* __res
     const int t = (int)blockIdx.x - J.wg_base;
     const int ox = J.u0 + (t % J.tiles_x) * TX, oy = J.v0 + (t / J.tiles_x) * TY;
     const bool h1 = (J.u1 - J.u0) == 1, v1x = (J.v1 - J.v0) == 1;
-    for (int i = threadIdx.x; i < W * HH; i += NT) {
-        const int ly = i / W, lx = i % W;
-        const int u = ox - H + lx, v = oy - H + ly;
-        const int mu = h1 ? J.u0 : pse(u, J.u0, J.u1);
-        const int mv = v1x ? J.v0 : pse(v, J.v0, J.v1);
-        tile[ly * PITCH + lx] = base[J.in_off + (uint64_t)(mv - J.v0) * J.in_pitch + (mu - J.u0)];
+    if (ox - H >= J.u0 && ox + TX + H <= J.u1 && oy - H >= J.v0 && oy + TY + H <= J.v1) {
+        // interior tile: the extended window needs no mirroring
+        const T* src = base + J.in_off + (uint64_t)(oy - H - J.v0) * J.in_pitch + (ox - H - J.u0);
+        for (int i = threadIdx.x; i < W * HH; i += NT) {
+            const int ly = i / W, lx = i % W;
+            tile[ly * PITCH + lx] = src[(uint64_t)ly * J.in_pitch + lx];
+        }
+    } else {
+        for (int i = threadIdx.x; i < W * HH; i += NT) {
+            const int ly = i / W, lx = i % W;
+            const int u = ox - H + lx, v = oy - H + ly;
+            const int mu = h1 ? J.u0 : pse(u, J.u0, J.u1);
+            const int mv = v1x ? J.v0 : pse(v, J.v0, J.v1);
+            tile[ly * PITCH + lx] = base[J.in_off + (uint64_t)(mv - J.v0) * J.in_pitch + (mu - J.u0)];
+        }
     }
     __syncthreads();
     tile_fdwt<IS97, TX, TY>(tile, PITCH, ox, oy, h1, v1x);
@@ -205,11 +276,13 @@ __global__ __launch_bounds__(NT) void fdwt_first_kernel(const tc_fdwt_first_job*
     const int t = (int)blockIdx.x - J.wg_base;
     const int ox = J.u0 + (t % J.tiles_x) * TX, oy = J.v0 + (t / J.tiles_x) * TY;
     const bool h1 = (J.u1 - J.u0) == 1, v1x = (J.v1 - J.v0) == 1;
+    const bool interior = ox - H >= J.u0 && ox + TX + H <= J.u1 && oy - H >= J.v0 && oy + TY + H <= J.v1;
     for (int i = threadIdx.x; i < W * HH; i += NT) {
         const int ly = i / W, lx = i % W;
         const int u = ox - H + lx, v = oy - H + ly;
-        const int mu = h1 ? J.u0 : pse(u, J.u0, J.u1);
-        const int mv = v1x ? J.v0 : pse(v, J.v0, J.v1);
+        // interior tiles need no mirroring
+        const int mu = interior ? u : (h1 ? J.u0 : pse(u, J.u0, J.u1));
+        const int mv = interior ? v : (v1x ? J.v0 : pse(v, J.v0, J.v1));
         const I* px = img + J.in_off + (uint64_t)(mv - J.v0) * J.in_pitch + (uint64_t)(mu - J.u0) * J.in_cstride;
         int s[NC];
 #pragma unroll
@@ -247,8 +320,7 @@ __global__ __launch_bounds__(NT) void fdwt_first_kernel(const tc_fdwt_first_job*
         }
         return;
     }
-#pragma unroll
-    for (int c = 0; c < NC; c++) tile_fdwt<IS97, TX, TY>(tile + c * TS, PITCH, ox, oy, h1, v1x);
+    tile_fdwt<IS97, TX, TY, NC>(tile, PITCH, ox, oy, h1, v1x, TS);
 #pragma unroll
     for (int c = 0; c < NC; c++)
         store_bands<IS97, TX, TY>(tile + c * TS, PITCH, base, J.ll_off[c], J.hl_off[c], J.lh_off[c], J.hh_off[c],
