@@ -134,11 +134,73 @@ struct Lift<true> {
     }
 };
 
-// Loads the extended tile, lifts rows then columns.  Returns with the
-// reconstructed samples at tile[H+ty][H+tx].
+// One line of N samples lifted in registers: 1D_SR STEP3..6 for 9/7 (the K
+// scaling is applied at load), STEP1..2 for 5/3, over the same ranges as the
+// LDS sweeps above (step s updates positions [s, N - s) of its parity).  P:
+// parity of position 0.  Read from and written back to LDS at `stride`.
+template <bool IS97, int N, int P>
+__device__ __forceinline__ void lift_line_inv(typename Lift<IS97>::T* p, int stride) {
+    typedef typename Lift<IS97>::T T;
+    T x[N];
+#pragma unroll
+    for (int i = 0; i < N; i++) x[i] = p[i * stride];
+    if (IS97) {
+        const float k[4] = {kDelta, kGamma, kBeta, kAlpha};
+#pragma unroll
+        for (int s = 1; s <= 4; s++)
+#pragma unroll
+            for (int i = s; i < N - s; i++)
+                if (((i + P) & 1) == ((s + 1) & 1)) x[i] = fmaf(-k[s - 1], x[i - 1] + x[i + 1], x[i]);
+    } else {
+#pragma unroll
+        for (int i = 1; i < N - 1; i++)
+            if (!((i + P) & 1)) x[i] -= ((int)x[i - 1] + (int)x[i + 1] + 2) >> 2;
+#pragma unroll
+        for (int i = 2; i < N - 2; i++)
+            if ((i + P) & 1) x[i] += ((int)x[i - 1] + (int)x[i + 1]) >> 1;
+    }
+#pragma unroll
+    for (int i = 0; i < N; i++) p[i * stride] = x[i];
+}
+
+// Lifts NC loaded tiles (tile c at tile + c * cs): every row, then every
+// column, one line per thread in registers, unless an axis has length 1.
+template <bool IS97, int TX, int TY, int NC>
+__device__ __forceinline__ void tile_lift_inv(typename Lift<IS97>::T* tile, int pitch, int ox, int oy, bool h1,
+                                              bool v1x, int cs) {
+    constexpr int H = Lift<IS97>::H;
+    constexpr int W = TX + 2 * H, HH = TY + 2 * H;
+    const int tid = threadIdx.x;
+    if (!h1) {
+        const bool odd = ((ox - H) & 1) != 0;
+        for (int L = tid; L < NC * HH; L += NT) {
+            typename Lift<IS97>::T* p = tile + (L / HH) * cs + (L % HH) * pitch;
+            if (odd)
+                lift_line_inv<IS97, W, 1>(p, 1);
+            else
+                lift_line_inv<IS97, W, 0>(p, 1);
+        }
+        __syncthreads();
+    }
+    if (!v1x) {
+        const bool odd = ((oy - H) & 1) != 0;
+        for (int L = tid; L < NC * W; L += NT) {
+            typename Lift<IS97>::T* p = tile + (L / W) * cs + (L % W);
+            if (odd)
+                lift_line_inv<IS97, HH, 1>(p, pitch);
+            else
+                lift_line_inv<IS97, HH, 0>(p, pitch);
+        }
+        __syncthreads();
+    }
+}
+
+// Loads the extended tile and, with `lift`, lifts rows then columns.  Returns
+// with the reconstructed samples at tile[H+ty][H+tx].
 template <bool IS97, int TX, int TY>
 __device__ void tile_idwt(typename Lift<IS97>::T* tile, int pitch, const typename Lift<IS97>::T* __restrict__ src,
-                          const Src& s, int u0, int u1, int v0, int v1, int ox, int oy, bool copy_only) {
+                          const Src& s, int u0, int u1, int v0, int v1, int ox, int oy, bool copy_only,
+                          bool lift = true) {
     typedef typename Lift<IS97>::T T;
     constexpr int H = Lift<IS97>::H;
     constexpr int W = TX + 2 * H, HH = TY + 2 * H;
@@ -181,8 +243,7 @@ __device__ void tile_idwt(typename Lift<IS97>::T* tile, int pitch, const typenam
             }
         }
         __syncthreads();
-        Lift<IS97>::steps(tile, 1, pitch, HH, W, (ox - H) & 1 ? 1 : 0, tid);
-        Lift<IS97>::steps(tile, pitch, 1, W, HH, (oy - H) & 1 ? 1 : 0, tid);
+        if (lift) tile_lift_inv<IS97, TX, TY, 1>(tile, pitch, ox, oy, false, false, 0);
         return;
     }
     for (int i = tid; i < W * HH; i += NT) {
@@ -210,10 +271,8 @@ __device__ void tile_idwt(typename Lift<IS97>::T* tile, int pitch, const typenam
         tile[ly * pitch + lx] = val;
     }
     __syncthreads();
-    if (copy_only) return;
-    // rows: positions along x, parity of lx=0 is parity of (ox - H) = parity of ox
-    if (!h1) Lift<IS97>::steps(tile, 1, pitch, HH, W, (ox - H) & 1 ? 1 : 0, tid);
-    if (!v1x) Lift<IS97>::steps(tile, pitch, 1, W, HH, (oy - H) & 1 ? 1 : 0, tid);
+    if (copy_only || !lift) return;
+    tile_lift_inv<IS97, TX, TY, 1>(tile, pitch, ox, oy, h1, v1x, 0);
 }
 
 template <bool IS97, int TX, int TY>
@@ -258,8 +317,11 @@ __global__ __launch_bounds__(NT) void idwt_final_kernel(const tc_final_job* __re
 #pragma unroll
     for (int c = 0; c < NC; c++) {
         Src s{J.ll_off[c], J.hl_off[c], J.lh_off[c], J.hh_off[c], J.ll_pitch[c], J.band_pitch[c]};
-        tile_idwt<IS97, TX, TY>(tile + c * TS, PITCH, base, s, J.u0, J.u1, J.v0, J.v1, ox, oy, J.levels == 0);
+        tile_idwt<IS97, TX, TY>(tile + c * TS, PITCH, base, s, J.u0, J.u1, J.v0, J.v1, ox, oy, J.levels == 0, false);
     }
+    // every component's rows, then every component's columns, between the same barriers
+    if (J.levels != 0)
+        tile_lift_inv<IS97, TX, TY, NC>(tile, PITCH, ox, oy, (J.u1 - J.u0) == 1, (J.v1 - J.v0) == 1, TS);
     __syncthreads();
     for (int i = threadIdx.x; i < TX * TY; i += NT) {
         int ly = i / TX, lx = i % TX;
