@@ -117,7 +117,6 @@ __device__ __forceinline__ uint32_t mq_dec(Mq& m, uint32_t& st, uint32_t trm, ui
         "s_and_b32 %[q], %[st], 0xffff0000\n\t"
         "s_and_b32 %[d], %[st], 1\n\t"
         "s_sub_u32 %[A], %[A], %[q]\n\t"
-        "s_nop 1\n\t"
         "v_readlane_b32 %[t2], %[trm], %[t0]\n\t"
         "v_readlane_b32 %[t3], %[trl], %[t0]\n\t"
         "s_cmp_lt_u32 %[C], %[q]\n\t"
@@ -204,6 +203,152 @@ __device__ __forceinline__ uint32_t mq_dec(Mq& m, uint32_t& st, uint32_t trm, ui
         : [len] "s"(m.len), [wp] "s"(m.words), [trm] "v"(trm), [trl] "v"(trl)
         : "scc");
     return d;
+}
+
+// The refinement decisions of one stripe (T.800 D.3.3) in one scalar loop.
+// cols: the columns holding candidates (not empty); lane x of stream: column
+// x's candidate labels in scan order, 4 bits each (label - 14), ended by 15.
+// Column x's decisions go to lane x of res, the first one highest.  Each
+// decision is DECODE as in mq_dec; a label other than the one used last swaps
+// the context states through the lane array ctxv (cl/cs as in Blk::dec_ctx;
+// cl is held as label - 14 inside).  The next column's label word is read
+// while the current column decodes.  The common path, a renormalising
+// decision with bits left in C, runs straight to the loop's one taken branch;
+// CT is reduced first and restored on the refill path.
+__device__ __forceinline__ void mq_mrp_stripe(Mq& m, uint32_t& cs, uint32_t& cl, uint32_t& ctxv, uint64_t cols,
+                                              uint32_t stream, uint32_t& res, uint32_t trm, uint32_t trl) {
+    uint32_t x, xn, iw, iwn, bits, code, d, q, t0, t1, t2, t3, n;
+    asm volatile(
+        "s_sub_u32 %[cl], %[cl], 14\n\t"
+        "s_ff1_i32_b64 %[x], %[cols]\n\t"
+        "v_readlane_b32 %[iw], %[stream], %[x]\n"
+        "L%=_col:\n\t"
+        "s_bitset0_b64 %[cols], %[x]\n\t"
+        "s_ff1_i32_b64 %[xn], %[cols]\n\t"
+        "s_max_i32 %[xn], %[xn], 0\n\t"
+        "s_mov_b32 %[bits], 0\n\t"
+        "v_readlane_b32 %[iwn], %[stream], %[xn]\n"
+        "L%=_next:\n\t"
+        "s_and_b32 %[code], %[iw], 15\n\t"
+        "s_lshr_b32 %[iw], %[iw], 4\n\t"
+        "s_cmp_eq_u32 %[code], %[cl]\n\t"
+        "s_cbranch_scc0 L%=_other\n"
+        "L%=_dec:\n\t"
+        "s_bfe_u32 %[t0], %[cs], 0x60001\n\t"
+        "s_and_b32 %[q], %[cs], 0xffff0000\n\t"
+        "s_and_b32 %[d], %[cs], 1\n\t"
+        "s_sub_u32 %[A], %[A], %[q]\n\t"
+        "v_readlane_b32 %[t2], %[trm], %[t0]\n\t"
+        "v_readlane_b32 %[t3], %[trl], %[t0]\n\t"
+        "s_cmp_lt_u32 %[C], %[q]\n\t"
+        "s_cbranch_scc1 L%=_lower\n\t"
+        "s_sub_u32 %[C], %[C], %[q]\n\t"
+        "s_cmp_lt_i32 %[A], 0\n\t"
+        "s_cbranch_scc1 L%=_fast\n\t"
+        "s_cmp_lt_u32 %[A], %[q]\n\t"
+        "s_cselect_b32 %[t2], %[t3], %[t2]\n\t"
+        "s_cselect_b32 %[t1], 1, 0\n"
+        "L%=_renorm:\n\t"
+        "s_xor_b32 %[cs], %[t2], %[d]\n\t"
+        "s_xor_b32 %[d], %[d], %[t1]\n\t"
+        "s_flbit_i32_b32 %[n], %[A]\n\t"
+        "s_lshl1_add_u32 %[bits], %[bits], %[d]\n\t"
+        "s_sub_u32 %[CT], %[CT], %[n]\n\t"
+        "s_cbranch_scc1 L%=_refill\n\t"
+        "s_lshl_b32 %[C], %[C], %[n]\n\t"
+        "s_lshl_b32 %[A], %[A], %[n]\n\t"
+        "s_branch L%=_next\n"
+        "L%=_fast:\n\t"
+        "s_lshl1_add_u32 %[bits], %[bits], %[d]\n\t"
+        "s_branch L%=_next\n"
+        "L%=_lower:\n\t"
+        "s_cmp_lt_u32 %[A], %[q]\n\t"
+        "s_cselect_b32 %[t2], %[t2], %[t3]\n\t"
+        "s_cselect_b32 %[t1], 0, 1\n\t"
+        "s_mov_b32 %[A], %[q]\n\t"
+        "s_branch L%=_renorm\n"
+        // another label, or the column's end
+        "L%=_other:\n\t"
+        "s_cmp_eq_u32 %[code], 15\n\t"
+        "s_cbranch_scc1 L%=_colend\n\t"
+        "s_add_u32 %[t0], %[cl], 14\n\t"
+        "s_mov_b32 m0, %[t0]\n\t"
+        "v_writelane_b32 %[ctxv], %[cs], m0\n\t"
+        "s_add_u32 %[t0], %[code], 14\n\t"
+        "s_mov_b32 %[cl], %[code]\n\t"
+        "v_readlane_b32 %[cs], %[ctxv], %[t0]\n\t"
+        "s_branch L%=_dec\n"
+        "L%=_colend:\n\t"
+        "s_mov_b32 m0, %[x]\n\t"
+        "v_writelane_b32 %[res], %[bits], m0\n\t"
+        "s_cmp_eq_u64 %[cols], 0\n\t"
+        "s_cbranch_scc1 L%=_end\n\t"
+        "s_mov_b32 %[x], %[xn]\n\t"
+        "s_mov_b32 %[iw], %[iwn]\n\t"
+        "s_branch L%=_col\n"
+        // CT < n: RENORMD with BYTEIN as in mq_dec
+        "L%=_refill:\n\t"
+        "s_add_u32 %[CT], %[CT], %[n]\n"
+        "L%=_loop:\n\t"
+        "s_cmp_eq_u32 %[CT], 0\n\t"
+        "s_cbranch_scc0 L%=_shift\n\t"
+        "s_cmp_ge_u32 %[np], %[len]\n\t"
+        "s_mov_b32 %[t0], 0xff\n\t"
+        "s_cbranch_scc1 L%=_got\n\t"
+        "s_lshr_b32 %[t1], %[np], 2\n\t"
+        "s_cmp_eq_u32 %[t1], %[wci]\n\t"
+        "s_cbranch_scc1 L%=_cached\n\t"
+        "s_mov_b32 %[wci], %[t1]\n\t"
+        "s_lshl_b32 %[t1], %[t1], 2\n\t"
+        "s_load_dword %[wc], %[wp], %[t1]\n\t"
+        "s_waitcnt lgkmcnt(0)\n"
+        "L%=_cached:\n\t"
+        "s_and_b32 %[t1], %[np], 3\n\t"
+        "s_lshl_b32 %[t1], %[t1], 3\n\t"
+        "s_lshr_b32 %[t0], %[wc], %[t1]\n\t"
+        "s_and_b32 %[t0], %[t0], 0xff\n"
+        "L%=_got:\n\t"
+        "s_cmp_eq_u32 %[cur], 0xff\n\t"
+        "s_cbranch_scc0 L%=_plain\n\t"
+        "s_cmp_gt_u32 %[t0], 0x8f\n\t"
+        "s_cbranch_scc0 L%=_stuff\n\t"
+        "s_add_u32 %[C], %[C], 0xff00\n\t"
+        "s_mov_b32 %[CT], 8\n\t"
+        "s_branch L%=_shift\n"
+        "L%=_stuff:\n\t"
+        "s_add_u32 %[np], %[np], 1\n\t"
+        "s_mov_b32 %[cur], %[t0]\n\t"
+        "s_lshl_b32 %[t0], %[t0], 9\n\t"
+        "s_add_u32 %[C], %[C], %[t0]\n\t"
+        "s_mov_b32 %[CT], 7\n\t"
+        "s_branch L%=_shift\n"
+        "L%=_plain:\n\t"
+        "s_add_u32 %[np], %[np], 1\n\t"
+        "s_mov_b32 %[cur], %[t0]\n\t"
+        "s_lshl_b32 %[t0], %[t0], 8\n\t"
+        "s_add_u32 %[C], %[C], %[t0]\n\t"
+        "s_mov_b32 %[CT], 8\n"
+        "L%=_shift:\n\t"
+        "s_min_u32 %[t1], %[n], %[CT]\n\t"
+        "s_lshl_b32 %[C], %[C], %[t1]\n\t"
+        "s_lshl_b32 %[A], %[A], %[t1]\n\t"
+        "s_sub_u32 %[CT], %[CT], %[t1]\n\t"
+        "s_sub_u32 %[n], %[n], %[t1]\n\t"
+        "s_cmp_eq_u32 %[n], 0\n\t"
+        "s_cbranch_scc0 L%=_loop\n\t"
+        "s_branch L%=_next\n"
+        // no lane read may still be in flight when the registers are handed back
+        "L%=_end:\n\t"
+        "s_or_b32 %[t2], %[t2], %[t3]\n\t"
+        "s_or_b32 %[t2], %[t2], %[iwn]\n\t"
+        "s_add_u32 %[cl], %[cl], 14"
+        : [A] "+s"(m.A), [C] "+s"(m.C), [CT] "+s"(m.CT), [cs] "+s"(cs), [cl] "+s"(cl), [cur] "+s"(m.cur),
+          [np] "+s"(m.np), [wc] "+s"(m.wc), [wci] "+s"(m.wci), [cols] "+s"(cols), [ctxv] "+v"(ctxv),
+          [res] "+v"(res), [x] "=&s"(x), [xn] "=&s"(xn), [iw] "=&s"(iw), [iwn] "=&s"(iwn), [bits] "=&s"(bits),
+          [code] "=&s"(code), [d] "=&s"(d), [q] "=&s"(q), [t0] "=&s"(t0), [t1] "=&s"(t1), [t2] "=&s"(t2),
+          [t3] "=&s"(t3), [n] "=&s"(n)
+        : [len] "s"(m.len), [wp] "s"(m.words), [stream] "v"(stream), [trm] "v"(trm), [trl] "v"(trl)
+        : "scc", "m0");
 }
 
 // Zero-coding label (T.800 Table D.1) of a 9-bit window (centre ignored),
@@ -429,20 +574,8 @@ struct Blk {
             }
         stream |= 15u << (4 * k);
         uint64_t cols = __ballot(cand != 0u);
-        uint32_t res = 0;
-        while (cols) {
-            const int x = (int)__builtin_ctzll(cols);
-            cols &= cols - 1;
-            uint32_t iw = rl(stream, (uint32_t)x);
-            uint32_t bits = 0;  // decoded bits, first candidate highest
-            for (;;) {
-                const uint32_t code = iw & 15u;
-                if (code == 15u) break;
-                iw >>= 4;
-                bits = (bits << 1) | dec_ctx(14u + code);
-            }
-            res = wl(res, (uint32_t)x, bits);
-        }
+        uint32_t res = 0;  // lane x: column x's decisions, first candidate highest
+        if (cols) mq_mrp_stripe(m, cs, cl, ctxv, cols, stream, res, trm, trl);
         // back to rows: candidate j of n (scan order) holds bit n-1-j
         const int n = __popc(cand);
         uint32_t one = 0;
