@@ -351,6 +351,212 @@ __device__ __forceinline__ void mq_mrp_stripe(Mq& m, uint32_t& cs, uint32_t& cl,
         : "scc", "m0");
 }
 
+// DECODE as in mq_dec over the operands of the asm block that uses it, with
+// label prefix P; every path ends with a branch to EXIT.
+#define TC_MQ_BODY(P, EXIT)                          \
+    "s_bfe_u32 %[t0], %[cs], 0x60001\n\t"             \
+    "s_and_b32 %[q], %[cs], 0xffff0000\n\t"           \
+    "s_and_b32 %[d], %[cs], 1\n\t"                    \
+    "s_sub_u32 %[A], %[A], %[q]\n\t"                  \
+    "v_readlane_b32 %[t2], %[trm], %[t0]\n\t"         \
+    "v_readlane_b32 %[t3], %[trl], %[t0]\n\t"         \
+    "s_cmp_lt_u32 %[C], %[q]\n\t"                     \
+    "s_cbranch_scc1 L%=" P "lower\n\t"                \
+    "s_sub_u32 %[C], %[C], %[q]\n\t"                  \
+    "s_cmp_lt_i32 %[A], 0\n\t"                        \
+    "s_cbranch_scc1 " EXIT "\n\t"                     \
+    "s_cmp_lt_u32 %[A], %[q]\n\t"                     \
+    "s_cselect_b32 %[t2], %[t3], %[t2]\n\t"           \
+    "s_cselect_b32 %[t1], 1, 0\n"                     \
+    "L%=" P "renorm:\n\t"                             \
+    "s_xor_b32 %[cs], %[t2], %[d]\n\t"                \
+    "s_xor_b32 %[d], %[d], %[t1]\n\t"                 \
+    "s_flbit_i32_b32 %[n], %[A]\n\t"                  \
+    "s_sub_u32 %[CT], %[CT], %[n]\n\t"                \
+    "s_cbranch_scc1 L%=" P "refill\n\t"               \
+    "s_lshl_b32 %[C], %[C], %[n]\n\t"                 \
+    "s_lshl_b32 %[A], %[A], %[n]\n\t"                 \
+    "s_branch " EXIT "\n"                             \
+    "L%=" P "lower:\n\t"                              \
+    "s_cmp_lt_u32 %[A], %[q]\n\t"                     \
+    "s_cselect_b32 %[t2], %[t2], %[t3]\n\t"           \
+    "s_cselect_b32 %[t1], 0, 1\n\t"                   \
+    "s_mov_b32 %[A], %[q]\n\t"                        \
+    "s_branch L%=" P "renorm\n"                       \
+    "L%=" P "refill:\n\t"                             \
+    "s_add_u32 %[CT], %[CT], %[n]\n"                  \
+    "L%=" P "loop:\n\t"                               \
+    "s_cmp_eq_u32 %[CT], 0\n\t"                       \
+    "s_cbranch_scc0 L%=" P "shift\n\t"                \
+    "s_cmp_ge_u32 %[np], %[len]\n\t"                  \
+    "s_mov_b32 %[t0], 0xff\n\t"                       \
+    "s_cbranch_scc1 L%=" P "got\n\t"                  \
+    "s_lshr_b32 %[t1], %[np], 2\n\t"                  \
+    "s_cmp_eq_u32 %[t1], %[wci]\n\t"                  \
+    "s_cbranch_scc1 L%=" P "cached\n\t"               \
+    "s_mov_b32 %[wci], %[t1]\n\t"                     \
+    "s_lshl_b32 %[t1], %[t1], 2\n\t"                  \
+    "s_load_dword %[wc], %[wp], %[t1]\n\t"            \
+    "s_waitcnt lgkmcnt(0)\n"                          \
+    "L%=" P "cached:\n\t"                             \
+    "s_and_b32 %[t1], %[np], 3\n\t"                   \
+    "s_lshl_b32 %[t1], %[t1], 3\n\t"                  \
+    "s_lshr_b32 %[t0], %[wc], %[t1]\n\t"              \
+    "s_and_b32 %[t0], %[t0], 0xff\n"                  \
+    "L%=" P "got:\n\t"                                \
+    "s_cmp_eq_u32 %[cur], 0xff\n\t"                   \
+    "s_cbranch_scc0 L%=" P "plain\n\t"                \
+    "s_cmp_gt_u32 %[t0], 0x8f\n\t"                    \
+    "s_cbranch_scc0 L%=" P "stuff\n\t"                \
+    "s_add_u32 %[C], %[C], 0xff00\n\t"                \
+    "s_mov_b32 %[CT], 8\n\t"                          \
+    "s_branch L%=" P "shift\n"                        \
+    "L%=" P "stuff:\n\t"                              \
+    "s_add_u32 %[np], %[np], 1\n\t"                   \
+    "s_mov_b32 %[cur], %[t0]\n\t"                     \
+    "s_lshl_b32 %[t0], %[t0], 9\n\t"                  \
+    "s_add_u32 %[C], %[C], %[t0]\n\t"                 \
+    "s_mov_b32 %[CT], 7\n\t"                          \
+    "s_branch L%=" P "shift\n"                        \
+    "L%=" P "plain:\n\t"                              \
+    "s_add_u32 %[np], %[np], 1\n\t"                   \
+    "s_mov_b32 %[cur], %[t0]\n\t"                     \
+    "s_lshl_b32 %[t0], %[t0], 8\n\t"                  \
+    "s_add_u32 %[C], %[C], %[t0]\n\t"                 \
+    "s_mov_b32 %[CT], 8\n"                            \
+    "L%=" P "shift:\n\t"                              \
+    "s_min_u32 %[t1], %[n], %[CT]\n\t"                \
+    "s_lshl_b32 %[C], %[C], %[t1]\n\t"                \
+    "s_lshl_b32 %[A], %[A], %[t1]\n\t"                \
+    "s_sub_u32 %[CT], %[CT], %[t1]\n\t"               \
+    "s_sub_u32 %[n], %[n], %[t1]\n\t"                 \
+    "s_cmp_eq_u32 %[n], 0\n\t"                        \
+    "s_cbranch_scc0 L%=" P "loop\n\t"                 \
+    "s_branch " EXIT "\n"
+
+// Per-column state of the significance and cleanup row loop.  sl: the
+// significance window (rows y0-1 .. y0+4 of the left, own and right columns,
+// 3 bits per row, left lowest); gl: the sign window in the same layout.
+struct ColRows {
+    uint32_t sl, gl;
+    uint32_t rows;    // rows still to code (bit r = stripe row r)
+    uint32_t vis;     // rows coded
+    uint32_t nsig;    // rows that became significant
+    uint32_t csgn;    // their signs (bit r)
+    uint32_t carry;   // their bits in the next column's windows (left position)
+    uint32_t gcarry;  // their signs, same positions
+};
+
+// The zero-coding decisions of one stripe column (T.800 D.3.1 / D.3.4) in
+// one scalar loop: each row of rows in order, its label from w1 (labels of
+// rows 0..3 as of the stripe's start, 4 bits each from bit 14) unless its
+// 3x3 window differs from the one in w0 (bits 0..17), then from the
+// zero-coding table zcv; a decision of 1 decodes the sign (T.800 D.3.2: the
+// sign-coding entry, label | XOR bit << 4, of the 8-bit key built from the
+// horizontal and vertical neighbours of sl and gl) and makes the row
+// significant.  below: rows that join the loop when the row above them
+// becomes significant (the significance pass), 0 for cleanup.  Context
+// states swap through ctxv as in Blk::dec_ctx.
+__device__ __forceinline__ void zc_rows(Mq& m, uint32_t& cs, uint32_t& cl, uint32_t& ctxv, ColRows& c, uint32_t w0,
+                                        uint32_t w1, uint32_t below, uint32_t zcv, uint32_t scv, uint32_t trm,
+                                        uint32_t trl) {
+    uint32_t r, r3, u0, u1, lab, xr, d, q, t0, t1, t2, t3, n;
+    asm volatile(
+        "s_cmp_eq_u32 %[rows], 0\n\t"
+        "s_cbranch_scc1 L%=end\n"
+        "L%=row:\n\t"
+        "s_ff1_i32_b32 %[r], %[rows]\n\t"
+        "s_bitset0_b32 %[rows], %[r]\n\t"
+        "s_bitset1_b32 %[vis], %[r]\n\t"
+        "s_lshl1_add_u32 %[r3], %[r], %[r]\n\t"
+        "s_xor_b32 %[u0], %[sl], %[w0]\n\t"
+        "s_add_u32 %[u1], %[r3], 0x90000\n\t"
+        "s_bfe_u32 %[u0], %[u0], %[u1]\n\t"
+        "s_cbranch_scc1 L%=zc\n\t"
+        "s_lshl2_add_u32 %[u1], %[r], 0x4000e\n\t"
+        "s_bfe_u32 %[lab], %[w1], %[u1]\n"
+        "L%=lab:\n\t"
+        "s_cmp_eq_u32 %[lab], %[cl]\n\t"
+        "s_cbranch_scc0 L%=zsw\n"
+        "L%=zdec:\n\t"
+        TC_MQ_BODY("z", "L%=zres")
+        "L%=zres:\n\t"
+        "s_cmp_eq_u32 %[d], 0\n\t"
+        "s_cbranch_scc0 L%=sig\n\t"
+        "s_cmp_lg_u32 %[rows], 0\n\t"
+        "s_cbranch_scc1 L%=row\n\t"
+        "s_branch L%=end\n"
+        // the window changed since the stripe began: label from the table
+        "L%=zc:\n\t"
+        "s_bfe_u32 %[u0], %[sl], %[u1]\n\t"
+        "s_lshr_b32 %[u1], %[u0], 3\n\t"
+        "v_readlane_b32 %[lab], %[zcv], %[u1]\n\t"
+        "s_and_b32 %[u0], %[u0], 7\n\t"
+        "s_lshl_b32 %[u0], %[u0], 2\n\t"
+        "s_lshr_b32 %[lab], %[lab], %[u0]\n\t"
+        "s_and_b32 %[lab], %[lab], 15\n\t"
+        "s_branch L%=lab\n"
+        "L%=zsw:\n\t"
+        "s_mov_b32 m0, %[cl]\n\t"
+        "v_writelane_b32 %[ctxv], %[cs], m0\n\t"
+        "s_mov_b32 %[cl], %[lab]\n\t"
+        "v_readlane_b32 %[cs], %[ctxv], %[lab]\n\t"
+        "s_branch L%=zdec\n"
+        // significant: the sign
+        "L%=sig:\n\t"
+        "s_lshr_b32 %[u0], %[sl], %[r3]\n\t"
+        "s_and_b32 %[u0], %[u0], 0xaa\n\t"
+        "s_lshr_b32 %[u1], %[gl], %[r3]\n\t"
+        "s_lshr_b32 %[u1], %[u1], 1\n\t"
+        "s_and_b32 %[u1], %[u1], 0x55\n\t"
+        "s_or_b32 %[u0], %[u0], %[u1]\n\t"
+        "s_lshr_b32 %[u1], %[u0], 2\n\t"
+        "v_readlane_b32 %[lab], %[scv], %[u1]\n\t"
+        "s_and_b32 %[u0], %[u0], 3\n\t"
+        "s_lshl_b32 %[u0], %[u0], 3\n\t"
+        "s_lshr_b32 %[lab], %[lab], %[u0]\n\t"
+        "s_bfe_u32 %[xr], %[lab], 0x10004\n\t"
+        "s_and_b32 %[lab], %[lab], 15\n\t"
+        "s_cmp_eq_u32 %[lab], %[cl]\n\t"
+        "s_cbranch_scc1 L%=sdec\n\t"
+        "s_mov_b32 m0, %[cl]\n\t"
+        "v_writelane_b32 %[ctxv], %[cs], m0\n\t"
+        "s_mov_b32 %[cl], %[lab]\n\t"
+        "v_readlane_b32 %[cs], %[ctxv], %[lab]\n"
+        "L%=sdec:\n\t"
+        TC_MQ_BODY("s", "L%=sres")
+        "L%=sres:\n\t"
+        "s_xor_b32 %[d], %[d], %[xr]\n\t"
+        "s_add_u32 %[u0], %[r3], 4\n\t"
+        "s_bitset1_b32 %[sl], %[u0]\n\t"
+        "s_lshl_b32 %[u1], %[d], %[u0]\n\t"
+        "s_or_b32 %[gl], %[gl], %[u1]\n\t"
+        "s_add_u32 %[u0], %[r3], 3\n\t"
+        "s_bitset1_b32 %[carry], %[u0]\n\t"
+        "s_lshl_b32 %[u1], %[d], %[u0]\n\t"
+        "s_or_b32 %[gcarry], %[gcarry], %[u1]\n\t"
+        "s_bitset1_b32 %[nsig], %[r]\n\t"
+        "s_lshl_b32 %[u1], %[d], %[r]\n\t"
+        "s_or_b32 %[csgn], %[csgn], %[u1]\n\t"
+        "s_lshl_b32 %[u1], 2, %[r]\n\t"
+        "s_and_b32 %[u1], %[u1], %[below]\n\t"
+        "s_or_b32 %[rows], %[rows], %[u1]\n\t"
+        "s_cmp_lg_u32 %[rows], 0\n\t"
+        "s_cbranch_scc1 L%=row\n"
+        // no lane read may still be in flight when the registers are handed back
+        "L%=end:\n\t"
+        "s_or_b32 %[u0], %[t2], %[t3]"
+        : [A] "+s"(m.A), [C] "+s"(m.C), [CT] "+s"(m.CT), [cs] "+s"(cs), [cl] "+s"(cl), [cur] "+s"(m.cur),
+          [np] "+s"(m.np), [wc] "+s"(m.wc), [wci] "+s"(m.wci), [ctxv] "+v"(ctxv), [sl] "+s"(c.sl), [gl] "+s"(c.gl),
+          [rows] "+s"(c.rows), [vis] "+s"(c.vis), [nsig] "+s"(c.nsig), [csgn] "+s"(c.csgn), [carry] "+s"(c.carry),
+          [gcarry] "+s"(c.gcarry), [r] "=&s"(r), [r3] "=&s"(r3), [u0] "=&s"(u0), [u1] "=&s"(u1), [lab] "=&s"(lab),
+          [xr] "=&s"(xr), [d] "=&s"(d), [q] "=&s"(q), [t0] "=&s"(t0), [t1] "=&s"(t1), [t2] "=&s"(t2), [t3] "=&s"(t3),
+          [n] "=&s"(n)
+        : [len] "s"(m.len), [wp] "s"(m.words), [w0] "s"(w0), [w1] "s"(w1), [below] "s"(below), [zcv] "v"(zcv),
+          [scv] "v"(scv), [trm] "v"(trm), [trl] "v"(trl)
+        : "scc", "m0");
+}
+
 // Zero-coding label (T.800 Table D.1) of a 9-bit window (centre ignored),
 // computed per lane from the neighbour counts.
 __device__ __forceinline__ uint32_t zc_lab_lane(uint32_t nb, int cls) {
@@ -441,29 +647,23 @@ struct Blk {
         }
         return mq_dec(m, cs, trm, trl);
     }
-    __device__ __forceinline__ uint32_t zc(uint32_t nb9) const {
-        return (rl(zcv, nb9 >> 3) >> ((nb9 & 7u) << 2)) & 15u;
-    }
-    // sign of the sample at stripe row r that just became significant.
-    // sl: significance window (3 bits per row, left/own/right), ls: left
-    // column signs rows 0..3, os: own column signs rows -1..4, rs: right
-    // column signs rows 0..3.
-    __device__ __forceinline__ uint32_t dec_sign(uint32_t sl, int r, uint32_t ls, uint32_t os, uint32_t rs) {
-        const uint32_t s = sl >> (3 * r);
-        const uint32_t idx = ((s >> 3) & 1u) | (((ls >> r) & 1u) << 1) | (((s >> 5) & 1u) << 2) |
-                             (((rs >> r) & 1u) << 3) | (((s >> 1) & 1u) << 4) | (((os >> r) & 1u) << 5) |
-                             (((s >> 7) & 1u) << 6) | (((os >> (r + 2)) & 1u) << 7);
+    // sign of the sample at stripe row r that just became significant (T.800
+    // D.3.2).  sl, gl: significance and sign windows (ColRows); the key holds
+    // (sign, significance) of the upper, left, right and lower neighbours, in
+    // bit pairs from bit 0.
+    __device__ __forceinline__ uint32_t dec_sign(uint32_t sl, uint32_t gl, int r) {
+        const uint32_t idx = ((sl >> (3 * r)) & 0xAAu) | ((gl >> (3 * r + 1)) & 0x55u);
         const uint32_t e = (rl(scv, idx >> 2) >> ((idx & 3u) << 3)) & 0xFFu;
         return dec_ctx(e & 15u) ^ (e >> 4);
     }
 
     // Per-column words of a stripe for the significance and cleanup passes.
-    // w0: window (18 bits) | candidate rows << 18 | significant rows << 22;
-    // w1: signs (own rows -1..4,
-    // left rows 0..3 << 6, right rows 0..3 << 10) | zero-coding labels of rows
-    // 0..3 from the window at the start of the stripe << 14 (4 bits each).
+    // w0: significance window (18 bits) | candidate rows << 18 | significant
+    // rows << 22; w1: zero-coding labels of rows 0..3 from the window at the
+    // start of the stripe << 14 (4 bits each); w2: sign window (18 bits, the
+    // layout of the significance window).
     __device__ __forceinline__ void column_words(int y0, uint32_t rowm, bool cleanup, uint32_t& w0, uint32_t& w1,
-                                                 uint64_t& cols) const {
+                                                 uint32_t& w2, uint64_t& cols) const {
         const uint32_t so = win6(sig, y0);
         const uint32_t sl = from_left(so), sr = from_right(so);
         const uint32_t slice = spread3(sl) | (spread3(so) << 1) | (spread3(sr) << 2);
@@ -481,8 +681,8 @@ struct Blk {
         if ((int)threadIdx.x >= w) cand = 0;  // columns outside the block
         w0 = slice | (cand << 18) | (s4 << 22);
         const uint32_t go = win6(sgn, y0);
-        const uint32_t g4 = (go >> 1) & 15u;
-        w1 = go | (from_left(g4) << 6) | (from_right(g4) << 10);
+        w2 = spread3(from_left(go)) | (spread3(go) << 1) | (spread3(from_right(go)) << 2);
+        w1 = 0;
         cols = __ballot(cand != 0u);
         if (cols) {  // zero-coding labels of the four rows from the window as it is now
             uint32_t labs = 0;
@@ -495,13 +695,13 @@ struct Blk {
     // significance propagation over one stripe (T.800 D.3.1)
     __device__ __forceinline__ void spp(int y0, int nr) {
         const uint32_t rowm = (1u << nr) - 1u;
-        uint32_t vw0, vw1;
+        uint32_t vw0, vw1, vw2;
         uint64_t cols;
-        column_words(y0, rowm, false, vw0, vw1, cols);
+        column_words(y0, rowm, false, vw0, vw1, vw2, cols);
         uint32_t res = 0;
         int x = -1;
-        // new significance of column x: window bits, rows, signs by row
-        uint32_t carry = 0, crows = 0, csgn = 0;
+        // new significance of column x: window bits, sign bits, rows
+        uint32_t carry = 0, gcarry = 0, crows = 0;
         for (;;) {
             int nx;
             if (crows != 0u && x + 1 < w) {
@@ -512,35 +712,23 @@ struct Blk {
                 nx = (int)__builtin_ctzll(rem);
             }
             const bool adj = nx == x + 1;
-            const uint32_t lc = adj ? carry : 0u, lr = adj ? crows : 0u, ls = adj ? csgn : 0u;
+            const uint32_t lc = adj ? carry : 0u, lr = adj ? crows : 0u, lg = adj ? gcarry : 0u;
             x = nx;
-            const uint32_t w0 = rl(vw0, x), w1 = rl(vw1, x);
-            uint32_t sl = (w0 & 0x3FFFFu) | lc;
+            const uint32_t w0 = rl(vw0, x), w1 = rl(vw1, x), w2 = rl(vw2, x);
             const uint32_t s4 = (w0 >> 22) & 15u;
+            ColRows c;
+            c.sl = (w0 & 0x3FFFFu) | lc;
+            c.gl = w2 | lg;
             // rows to code: the static candidates, and insignificant rows next to
-            // the left column's new significance
-            uint32_t rows = ((w0 >> 18) & 15u) | ((lr | (lr << 1) | (lr >> 1)) & ~s4 & rowm);
-            uint32_t osg = w1 & 63u;
-            uint32_t vis = 0, nsig = 0;
-            carry = crows = csgn = 0;
-            while (rows) {
-                const int r = (int)__builtin_ctz(rows);
-                rows &= rows - 1u;
-                vis |= 1u << r;
-                const uint32_t nb = (sl >> (3 * r)) & 0x1FFu;
-                const uint32_t lab = ((sl ^ w0) >> (3 * r)) & 0x1FFu ? zc(nb) : (w1 >> (14 + 4 * r)) & 15u;
-                if (dec_ctx(lab)) {
-                    const uint32_t sg = dec_sign(sl, r, ((w1 >> 6) & 15u) | ls, osg, (w1 >> 10) & 15u);
-                    sl |= 1u << (3 * r + 4);
-                    osg |= sg << (r + 1);
-                    nsig |= 1u << r;
-                    carry |= 1u << (3 * r + 3);
-                    crows |= 1u << r;
-                    csgn |= sg << r;
-                    rows |= (2u << r) & ~s4 & rowm;  // the row below now has a significant neighbour
-                }
-            }
-            if (vis) res = wl(res, (uint32_t)x, vis | (nsig << 4) | (csgn << 8));
+            // the left column's new significance; a row also joins when the row
+            // above it becomes significant
+            c.rows = ((w0 >> 18) & 15u) | ((lr | (lr << 1) | (lr >> 1)) & ~s4 & rowm);
+            c.vis = c.nsig = c.csgn = c.carry = c.gcarry = 0;
+            zc_rows(m, cs, cl, ctxv, c, w0, w1, ~s4 & rowm, zcv, scv, trm, trl);
+            carry = c.carry;
+            gcarry = c.gcarry;
+            crows = c.nsig;
+            if (c.vis) res = wl(res, (uint32_t)x, c.vis | (c.nsig << 4) | (c.csgn << 8));
         }
         const uint64_t nv = (uint64_t)(res & 15u) << y0, ns = (uint64_t)((res >> 4) & 15u) << y0;
         pip |= nv;
@@ -593,52 +781,43 @@ struct Blk {
     // cleanup with run-length mode over one stripe (T.800 D.3.4)
     __device__ __forceinline__ void cup(int y0, int nr) {
         const uint32_t rowm = (1u << nr) - 1u;
-        uint32_t vw0, vw1;
+        uint32_t vw0, vw1, vw2;
         uint64_t cols;
-        column_words(y0, rowm, true, vw0, vw1, cols);
+        column_words(y0, rowm, true, vw0, vw1, vw2, cols);
         uint32_t res = 0;
         int lastx = -2;
-        uint32_t carry = 0, csgn = 0;
+        uint32_t carry = 0, gcarry = 0;
         while (cols) {
             const int x = (int)__builtin_ctzll(cols);
             cols &= cols - 1;
             const bool adj = x == lastx + 1;
-            const uint32_t lc = adj ? carry : 0u, ls = adj ? csgn : 0u;
+            const uint32_t lc = adj ? carry : 0u, lg = adj ? gcarry : 0u;
             lastx = x;
             carry = 0;
-            csgn = 0;
-            const uint32_t w0 = rl(vw0, (uint32_t)x), w1 = rl(vw1, (uint32_t)x);
-            uint32_t sl = (w0 & 0x3FFFFu) | lc;
-            uint32_t rows = (w0 >> 18) & 15u;
-            uint32_t osg = w1 & 63u;
-            uint32_t nsig = 0;
-            if (rows == 15u && sl == 0u) {  // run-length mode
+            gcarry = 0;
+            const uint32_t w0 = rl(vw0, (uint32_t)x), w1 = rl(vw1, (uint32_t)x), w2 = rl(vw2, (uint32_t)x);
+            ColRows c;
+            c.sl = (w0 & 0x3FFFFu) | lc;
+            c.gl = w2 | lg;
+            c.rows = (w0 >> 18) & 15u;
+            c.vis = c.nsig = c.csgn = c.carry = c.gcarry = 0;
+            if (c.rows == 15u && c.sl == 0u) {  // run-length mode
                 if (!mq_dec(m, st_rl, trm, trl)) continue;
                 int r = (int)mq_dec(m, st_uni, trm, trl) << 1;
                 r |= (int)mq_dec(m, st_uni, trm, trl);
-                const uint32_t sg = dec_sign(sl, r, ((w1 >> 6) & 15u) | ls, osg, (w1 >> 10) & 15u);
-                sl |= 1u << (3 * r + 4);
-                osg |= sg << (r + 1);
-                nsig |= 1u << r;
-                carry |= 1u << (3 * r + 3);
-                csgn |= sg << r;
-                rows &= ~((2u << r) - 1u);  // rows after r
+                const uint32_t sg = dec_sign(c.sl, c.gl, r);
+                c.sl |= 1u << (3 * r + 4);
+                c.gl |= sg << (3 * r + 4);
+                c.nsig |= 1u << r;
+                c.csgn |= sg << r;
+                c.carry |= 1u << (3 * r + 3);
+                c.gcarry |= sg << (3 * r + 3);
+                c.rows &= ~((2u << r) - 1u);  // rows after r
             }
-            while (rows) {
-                const int r = (int)__builtin_ctz(rows);
-                rows &= rows - 1u;
-                const uint32_t nb = (sl >> (3 * r)) & 0x1FFu;
-                const uint32_t lab = ((sl ^ w0) >> (3 * r)) & 0x1FFu ? zc(nb) : (w1 >> (14 + 4 * r)) & 15u;
-                if (dec_ctx(lab)) {
-                    const uint32_t sg = dec_sign(sl, r, ((w1 >> 6) & 15u) | ls, osg, (w1 >> 10) & 15u);
-                    sl |= 1u << (3 * r + 4);
-                    osg |= sg << (r + 1);
-                    nsig |= 1u << r;
-                    carry |= 1u << (3 * r + 3);
-                    csgn |= sg << r;
-                }
-            }
-            if (nsig) res = wl(res, (uint32_t)x, nsig | (csgn << 8));
+            zc_rows(m, cs, cl, ctxv, c, w0, w1, 0u, zcv, scv, trm, trl);
+            carry = c.carry;
+            gcarry = c.gcarry;
+            if (c.nsig) res = wl(res, (uint32_t)x, c.nsig | (c.csgn << 8));
         }
         const uint64_t ns = (uint64_t)(res & 15u) << y0;
         sig |= ns;
@@ -760,7 +939,15 @@ __global__ __launch_bounds__(64) void t1_stripe_kernel(const tc_cblk_desc* __res
         zc3[c] = v;
     }
     uint32_t scv = 0;
-    for (int b = 0; b < 4; b++) scv |= sc_entry((uint32_t)(lane * 4 + b)) << (8 * b);
+    for (int b = 0; b < 4; b++) {
+        // key (Blk::dec_sign) -> the sign-coding table's index: significance
+        // and sign of left, right, upper, lower neighbour, bits 0..7
+        const uint32_t k = (uint32_t)(lane * 4 + b);
+        const uint32_t i = ((k >> 3) & 1u) | (((k >> 2) & 1u) << 1) | (((k >> 5) & 1u) << 2) |
+                           (((k >> 4) & 1u) << 3) | (((k >> 1) & 1u) << 4) | ((k & 1u) << 5) |
+                           (((k >> 7) & 1u) << 6) | (((k >> 6) & 1u) << 7);
+        scv |= sc_entry(i) << (8 * b);
+    }
     const uint32_t trm = lane < 47 ? st_pack(kNmps[lane], 0) : 0u;
     const uint32_t trl =
         lane < 47 ? (st_pack(kNlps[lane], 0) | ((lane == 0 || lane == 6 || lane == 14) ? 1u : 0u)) : 0u;
