@@ -101,116 +101,12 @@ __device__ __forceinline__ void mq_init(Mq& m, const uint32_t* words, uint32_t l
     m.A = 0x80000000u;
 }
 
-// DECODE (C.3.2) with the conditional exchanges (C.3.3), the successor state
-// and RENORMD/BYTEIN (C.3.3/C.3.4), RENORMD shifting min(needed, CT) bits per
-// step (the same bits as one at a time).  One scalar block, so the decoder
-// state keeps its registers on every path; both successor states are read
-// first so their latency overlaps the interval arithmetic (the fast path reads
-// them back before leaving, so no lane read is still in flight).  st (Qe << 16 | I << 1 | MPS) is
-// updated in place; trm/trl: lane I holds the packed successor state of I
-// after an MPS / LPS (the LPS one with SWITCH in bit 0).  The codeword dword
-// load waits for itself inside the block.
-__device__ __forceinline__ uint32_t mq_dec(Mq& m, uint32_t& st, uint32_t trm, uint32_t trl) {
-    uint32_t d, q, t0, t1, t2, t3, n;
-    asm volatile(
-        "s_bfe_u32 %[t0], %[st], 0x60001\n\t"
-        "s_and_b32 %[q], %[st], 0xffff0000\n\t"
-        "s_and_b32 %[d], %[st], 1\n\t"
-        "s_sub_u32 %[A], %[A], %[q]\n\t"
-        "v_readlane_b32 %[t2], %[trm], %[t0]\n\t"
-        "v_readlane_b32 %[t3], %[trl], %[t0]\n\t"
-        "s_cmp_lt_u32 %[C], %[q]\n\t"
-        "s_cbranch_scc1 L%=_lower\n\t"
-        "s_sub_u32 %[C], %[C], %[q]\n\t"
-        "s_cmp_lt_i32 %[A], 0\n\t"
-        "s_cbranch_scc1 L%=_fast\n\t"
-        // upper sub-interval, renormalising: LPS exchange when A < Qe
-        "s_cmp_lt_u32 %[A], %[q]\n\t"
-        "s_cselect_b32 %[t2], %[t3], %[t2]\n\t"
-        "s_cselect_b32 %[t1], 1, 0\n"
-        "L%=_renorm:\n\t"
-        "s_xor_b32 %[st], %[t2], %[d]\n\t"
-        "s_xor_b32 %[d], %[d], %[t1]\n\t"
-        "s_flbit_i32_b32 %[n], %[A]\n\t"
-        "s_cmp_lt_u32 %[CT], %[n]\n\t"
-        "s_cbranch_scc1 L%=_loop\n\t"
-        "s_lshl_b32 %[C], %[C], %[n]\n\t"
-        "s_lshl_b32 %[A], %[A], %[n]\n\t"
-        "s_sub_u32 %[CT], %[CT], %[n]\n\t"
-        "s_branch L%=_end\n"
-        // lower sub-interval: MPS exchange when A < Qe, then A = Qe
-        "L%=_lower:\n\t"
-        "s_cmp_lt_u32 %[A], %[q]\n\t"
-        "s_cselect_b32 %[t2], %[t2], %[t3]\n\t"
-        "s_cselect_b32 %[t1], 0, 1\n\t"
-        "s_mov_b32 %[A], %[q]\n\t"
-        "s_branch L%=_renorm\n"
-        "L%=_fast:\n\t"
-        "s_or_b32 %[t2], %[t2], %[t3]\n\t"
-        "s_branch L%=_end\n"
-        // RENORMD with BYTEIN when CT runs out
-        "L%=_loop:\n\t"
-        "s_cmp_eq_u32 %[CT], 0\n\t"
-        "s_cbranch_scc0 L%=_shift\n\t"
-        "s_cmp_ge_u32 %[np], %[len]\n\t"
-        "s_mov_b32 %[t0], 0xff\n\t"
-        "s_cbranch_scc1 L%=_got\n\t"
-        "s_lshr_b32 %[t1], %[np], 2\n\t"
-        "s_cmp_eq_u32 %[t1], %[wci]\n\t"
-        "s_cbranch_scc1 L%=_cached\n\t"
-        "s_mov_b32 %[wci], %[t1]\n\t"
-        "s_lshl_b32 %[t1], %[t1], 2\n\t"
-        "s_load_dword %[wc], %[wp], %[t1]\n\t"
-        "s_waitcnt lgkmcnt(0)\n"
-        "L%=_cached:\n\t"
-        "s_and_b32 %[t1], %[np], 3\n\t"
-        "s_lshl_b32 %[t1], %[t1], 3\n\t"
-        "s_lshr_b32 %[t0], %[wc], %[t1]\n\t"
-        "s_and_b32 %[t0], %[t0], 0xff\n"
-        "L%=_got:\n\t"
-        "s_cmp_eq_u32 %[cur], 0xff\n\t"
-        "s_cbranch_scc0 L%=_plain\n\t"
-        "s_cmp_gt_u32 %[t0], 0x8f\n\t"
-        "s_cbranch_scc0 L%=_stuff\n\t"
-        "s_add_u32 %[C], %[C], 0xff00\n\t"
-        "s_mov_b32 %[CT], 8\n\t"
-        "s_branch L%=_shift\n"
-        "L%=_stuff:\n\t"
-        "s_add_u32 %[np], %[np], 1\n\t"
-        "s_mov_b32 %[cur], %[t0]\n\t"
-        "s_lshl_b32 %[t0], %[t0], 9\n\t"
-        "s_add_u32 %[C], %[C], %[t0]\n\t"
-        "s_mov_b32 %[CT], 7\n\t"
-        "s_branch L%=_shift\n"
-        "L%=_plain:\n\t"
-        "s_add_u32 %[np], %[np], 1\n\t"
-        "s_mov_b32 %[cur], %[t0]\n\t"
-        "s_lshl_b32 %[t0], %[t0], 8\n\t"
-        "s_add_u32 %[C], %[C], %[t0]\n\t"
-        "s_mov_b32 %[CT], 8\n"
-        "L%=_shift:\n\t"
-        "s_min_u32 %[t1], %[n], %[CT]\n\t"
-        "s_lshl_b32 %[C], %[C], %[t1]\n\t"
-        "s_lshl_b32 %[A], %[A], %[t1]\n\t"
-        "s_sub_u32 %[CT], %[CT], %[t1]\n\t"
-        "s_sub_u32 %[n], %[n], %[t1]\n\t"
-        "s_cmp_eq_u32 %[n], 0\n\t"
-        "s_cbranch_scc0 L%=_loop\n"
-        "L%=_end:"
-        : [A] "+s"(m.A), [C] "+s"(m.C), [CT] "+s"(m.CT), [st] "+s"(st), [cur] "+s"(m.cur), [np] "+s"(m.np),
-          [wc] "+s"(m.wc), [wci] "+s"(m.wci), [d] "=&s"(d), [q] "=&s"(q), [t0] "=&s"(t0), [t1] "=&s"(t1),
-          [t2] "=&s"(t2), [t3] "=&s"(t3), [n] "=&s"(n)
-        : [len] "s"(m.len), [wp] "s"(m.words), [trm] "v"(trm), [trl] "v"(trl)
-        : "scc");
-    return d;
-}
-
 // The refinement decisions of one stripe (T.800 D.3.3) in one scalar loop.
 // cols: the columns holding candidates (not empty); lane x of stream: column
 // x's candidate labels in scan order, 4 bits each (label - 14), ended by 15.
 // Column x's decisions go to lane x of res, the first one highest.  Each
-// decision is DECODE as in mq_dec; a label other than the one used last swaps
-// the context states through the lane array ctxv (cl/cs as in Blk::dec_ctx;
+// decision is DECODE as in TC_MQ_BODY below; a label other than the one used
+// last swaps the context states through the lane array ctxv (cl/cs as in Blk;
 // cl is held as label - 14 inside).  The next column's label word is read
 // while the current column decodes.  The common path, a renormalising
 // decision with bits left in C, runs straight to the loop's one taken branch;
@@ -286,7 +182,7 @@ __device__ __forceinline__ void mq_mrp_stripe(Mq& m, uint32_t& cs, uint32_t& cl,
         "s_mov_b32 %[x], %[xn]\n\t"
         "s_mov_b32 %[iw], %[iwn]\n\t"
         "s_branch L%=_col\n"
-        // CT < n: RENORMD with BYTEIN as in mq_dec
+        // CT < n: RENORMD with BYTEIN as in TC_MQ_BODY
         "L%=_refill:\n\t"
         "s_add_u32 %[CT], %[CT], %[n]\n"
         "L%=_loop:\n\t"
@@ -351,12 +247,19 @@ __device__ __forceinline__ void mq_mrp_stripe(Mq& m, uint32_t& cs, uint32_t& cl,
         : "scc", "m0");
 }
 
-// DECODE as in mq_dec over the operands of the asm block that uses it, with
-// label prefix P; every path ends with a branch to EXIT.
-#define TC_MQ_BODY(P, EXIT)                          \
-    "s_bfe_u32 %[t0], %[cs], 0x60001\n\t"             \
-    "s_and_b32 %[q], %[cs], 0xffff0000\n\t"           \
-    "s_and_b32 %[d], %[cs], 1\n\t"                    \
+// DECODE (C.3.2) with the conditional exchanges (C.3.3), the successor state
+// and RENORMD/BYTEIN (C.3.3/C.3.4), RENORMD shifting min(needed, CT) bits per
+// step (the same bits as one at a time), over the operands of the asm block
+// that uses it: label prefix P, context state operand ST (Qe << 16 | I << 1 |
+// MPS, updated in place), the decision in d; every path ends with a branch to
+// EXIT.  trm/trl: lane I holds the packed successor state of I after an MPS /
+// LPS (the LPS one with SWITCH in bit 0); both are read first so their
+// latency overlaps the interval arithmetic.  CT is reduced first and restored
+// on the refill path; the codeword dword load waits for itself.
+#define TC_MQ_BODY(P, ST, EXIT)                      \
+    "s_bfe_u32 %[t0], %[" ST "], 0x60001\n\t"             \
+    "s_and_b32 %[q], %[" ST "], 0xffff0000\n\t"           \
+    "s_and_b32 %[d], %[" ST "], 1\n\t"                    \
     "s_sub_u32 %[A], %[A], %[q]\n\t"                  \
     "v_readlane_b32 %[t2], %[trm], %[t0]\n\t"         \
     "v_readlane_b32 %[t3], %[trl], %[t0]\n\t"         \
@@ -369,7 +272,7 @@ __device__ __forceinline__ void mq_mrp_stripe(Mq& m, uint32_t& cs, uint32_t& cl,
     "s_cselect_b32 %[t2], %[t3], %[t2]\n\t"           \
     "s_cselect_b32 %[t1], 1, 0\n"                     \
     "L%=" P "renorm:\n\t"                             \
-    "s_xor_b32 %[cs], %[t2], %[d]\n\t"                \
+    "s_xor_b32 %[" ST "], %[t2], %[d]\n\t"                \
     "s_xor_b32 %[d], %[d], %[t1]\n\t"                 \
     "s_flbit_i32_b32 %[n], %[A]\n\t"                  \
     "s_sub_u32 %[CT], %[CT], %[n]\n\t"                \
@@ -434,125 +337,291 @@ __device__ __forceinline__ void mq_mrp_stripe(Mq& m, uint32_t& cs, uint32_t& cl,
     "s_cbranch_scc0 L%=" P "loop\n\t"                 \
     "s_branch " EXIT "\n"
 
-// Per-column state of the significance and cleanup row loop.  sl: the
-// significance window (rows y0-1 .. y0+4 of the left, own and right columns,
-// 3 bits per row, left lowest); gl: the sign window in the same layout.
-struct ColRows {
-    uint32_t sl, gl;
-    uint32_t rows;    // rows still to code (bit r = stripe row r)
-    uint32_t vis;     // rows coded
-    uint32_t nsig;    // rows that became significant
-    uint32_t csgn;    // their signs (bit r)
-    uint32_t carry;   // their bits in the next column's windows (left position)
-    uint32_t gcarry;  // their signs, same positions
-};
+// The sign of stripe row r (r3 = 3r), which just became significant (T.800
+// D.3.2): the sign-coding entry (label | XOR bit << 4) of the 8-bit key that
+// pairs (sign, significance) of the upper, left, right and lower neighbours
+// from sl and gl, the context swapped in through ctxv as in Blk; then
+// row r joins sl/gl, nsig/csgn and the next column's carry/gcarry, and the
+// row below joins rows when it is in `below`.
+#define TC_SIGN_UPDATE(P, EXIT)                      \
+    "s_lshr_b32 %[u0], %[sl], %[r3]\n\t"              \
+    "s_and_b32 %[u0], %[u0], 0xaa\n\t"                \
+    "s_lshr_b32 %[u1], %[gl], %[r3]\n\t"              \
+    "s_lshr_b32 %[u1], %[u1], 1\n\t"                  \
+    "s_and_b32 %[u1], %[u1], 0x55\n\t"                \
+    "s_or_b32 %[u0], %[u0], %[u1]\n\t"                \
+    "s_lshr_b32 %[u1], %[u0], 2\n\t"                  \
+    "v_readlane_b32 %[lab], %[scv], %[u1]\n\t"        \
+    "s_and_b32 %[u0], %[u0], 3\n\t"                   \
+    "s_lshl_b32 %[u0], %[u0], 3\n\t"                  \
+    "s_lshr_b32 %[lab], %[lab], %[u0]\n\t"            \
+    "s_bfe_u32 %[xr], %[lab], 0x10004\n\t"            \
+    "s_and_b32 %[lab], %[lab], 15\n\t"                \
+    "s_cmp_eq_u32 %[lab], %[cl]\n\t"                  \
+    "s_cbranch_scc1 L%=" P "dec\n\t"                  \
+    "s_mov_b32 m0, %[cl]\n\t"                         \
+    "v_writelane_b32 %[ctxv], %[cs], m0\n\t"          \
+    "s_mov_b32 %[cl], %[lab]\n\t"                     \
+    "v_readlane_b32 %[cs], %[ctxv], %[lab]\n"         \
+    "L%=" P "dec:\n\t"                                \
+    TC_MQ_BODY(P "m", "cs", "L%=" P "res")            \
+    "L%=" P "res:\n\t"                                \
+    "s_xor_b32 %[d], %[d], %[xr]\n\t"                 \
+    "s_add_u32 %[u0], %[r3], 4\n\t"                   \
+    "s_bitset1_b32 %[sl], %[u0]\n\t"                  \
+    "s_lshl_b32 %[u1], %[d], %[u0]\n\t"               \
+    "s_or_b32 %[gl], %[gl], %[u1]\n\t"                \
+    "s_add_u32 %[u0], %[r3], 3\n\t"                   \
+    "s_bitset1_b32 %[carry], %[u0]\n\t"               \
+    "s_lshl_b32 %[u1], %[d], %[u0]\n\t"               \
+    "s_or_b32 %[gcarry], %[gcarry], %[u1]\n\t"        \
+    "s_bitset1_b32 %[nsig], %[r]\n\t"                 \
+    "s_lshl_b32 %[u1], %[d], %[r]\n\t"                \
+    "s_or_b32 %[csgn], %[csgn], %[u1]\n\t"            \
+    "s_lshl_b32 %[u1], 2, %[r]\n\t"                   \
+    "s_and_b32 %[u1], %[u1], %[below]\n\t"            \
+    "s_or_b32 %[rows], %[rows], %[u1]\n\t"            \
+    "s_branch " EXIT "\n"
 
-// The zero-coding decisions of one stripe column (T.800 D.3.1 / D.3.4) in
-// one scalar loop: each row of rows in order, its label from w1 (labels of
-// rows 0..3 as of the stripe's start, 4 bits each from bit 14) unless its
-// 3x3 window differs from the one in w0 (bits 0..17), then from the
-// zero-coding table zcv; a decision of 1 decodes the sign (T.800 D.3.2: the
-// sign-coding entry, label | XOR bit << 4, of the 8-bit key built from the
-// horizontal and vertical neighbours of sl and gl) and makes the row
-// significant.  below: rows that join the loop when the row above them
-// becomes significant (the significance pass), 0 for cleanup.  Context
-// states swap through ctxv as in Blk::dec_ctx.
-__device__ __forceinline__ void zc_rows(Mq& m, uint32_t& cs, uint32_t& cl, uint32_t& ctxv, ColRows& c, uint32_t w0,
-                                        uint32_t w1, uint32_t below, uint32_t zcv, uint32_t scv, uint32_t trm,
-                                        uint32_t trl) {
+// The zero-coding decisions of one stripe column (T.800 D.3.1 / D.3.4), rows
+// not empty: each row of rows in order (into vis), its label from w1 (labels
+// of rows 0..3 as of the stripe's start, 4 bits each from bit 14) unless its
+// 3x3 window in sl differs from the one in w0 (bits 0..17), then from the
+// zero-coding table zcv; a decision of 1 decodes the sign (TC_SIGN_UPDATE).
+// Ends with a branch to EXIT.
+#define TC_ZC_ROWS(P, EXIT)                          \
+    "L%=" P "row:\n\t"                                \
+    "s_ff1_i32_b32 %[r], %[rows]\n\t"                 \
+    "s_bitset0_b32 %[rows], %[r]\n\t"                 \
+    "s_bitset1_b32 %[vis], %[r]\n\t"                  \
+    "s_lshl1_add_u32 %[r3], %[r], %[r]\n\t"           \
+    "s_xor_b32 %[u0], %[sl], %[w0]\n\t"               \
+    "s_add_u32 %[u1], %[r3], 0x90000\n\t"             \
+    "s_bfe_u32 %[u0], %[u0], %[u1]\n\t"               \
+    "s_cbranch_scc1 L%=" P "zc\n\t"                   \
+    "s_lshl2_add_u32 %[u1], %[r], 0x4000e\n\t"        \
+    "s_bfe_u32 %[lab], %[w1], %[u1]\n"                \
+    "L%=" P "lab:\n\t"                                \
+    "s_cmp_eq_u32 %[lab], %[cl]\n\t"                  \
+    "s_cbranch_scc0 L%=" P "sw\n"                     \
+    "L%=" P "dec:\n\t"                                \
+    TC_MQ_BODY(P "z", "cs", "L%=" P "res")            \
+    "L%=" P "res:\n\t"                                \
+    "s_cmp_eq_u32 %[d], 0\n\t"                        \
+    "s_cbranch_scc0 L%=" P "sig\n"                    \
+    "L%=" P "next:\n\t"                               \
+    "s_cmp_lg_u32 %[rows], 0\n\t"                     \
+    "s_cbranch_scc1 L%=" P "row\n\t"                  \
+    "s_branch " EXIT "\n"                             \
+    "L%=" P "zc:\n\t"                                 \
+    "s_bfe_u32 %[u0], %[sl], %[u1]\n\t"               \
+    "s_lshr_b32 %[u1], %[u0], 3\n\t"                  \
+    "v_readlane_b32 %[lab], %[zcv], %[u1]\n\t"        \
+    "s_and_b32 %[u0], %[u0], 7\n\t"                   \
+    "s_lshl_b32 %[u0], %[u0], 2\n\t"                  \
+    "s_lshr_b32 %[lab], %[lab], %[u0]\n\t"            \
+    "s_and_b32 %[lab], %[lab], 15\n\t"                \
+    "s_branch L%=" P "lab\n"                          \
+    "L%=" P "sw:\n\t"                                 \
+    "s_mov_b32 m0, %[cl]\n\t"                         \
+    "v_writelane_b32 %[ctxv], %[cs], m0\n\t"          \
+    "s_mov_b32 %[cl], %[lab]\n\t"                     \
+    "v_readlane_b32 %[cs], %[ctxv], %[lab]\n\t"       \
+    "s_branch L%=" P "dec\n"                          \
+    "L%=" P "sig:\n\t"                                \
+    TC_SIGN_UPDATE(P "g", "L%=" P "next")
+
+// Operands shared by the two stripe loops below.
+#define TC_STRIPE_OPERANDS                                                                                          \
+    : [A] "+s"(m.A), [C] "+s"(m.C), [CT] "+s"(m.CT), [cs] "+s"(cs), [cl] "+s"(cl), [cur] "+s"(m.cur),               \
+      [np] "+s"(m.np), [wc] "+s"(m.wc), [wci] "+s"(m.wci), [cols] "+s"(cols), [ctxv] "+v"(ctxv), [res] "+v"(res),   \
+      [x] "=&s"(x), [xp] "=&s"(xp), [carry] "=&s"(carry), [gcarry] "=&s"(gcarry), [w0] "=&s"(w0), [w1] "=&s"(w1),  \
+      [pw0] "=&s"(pw0), [pw1] "=&s"(pw1), [pw2] "=&s"(pw2), [sl] "=&s"(sl), [gl] "=&s"(gl), [rows] "=&s"(rows),    \
+      [vis] "=&s"(vis), [nsig] "=&s"(nsig), [csgn] "=&s"(csgn), [below] "=&s"(below), [r] "=&s"(r),                 \
+      [r3] "=&s"(r3), [u0] "=&s"(u0), [u1] "=&s"(u1), [lab] "=&s"(lab), [xr] "=&s"(xr), [d] "=&s"(d), [q] "=&s"(q), \
+      [t0] "=&s"(t0), [t1] "=&s"(t1), [t2] "=&s"(t2), [t3] "=&s"(t3), [n] "=&s"(n)
+
+// Significance propagation over one stripe (T.800 D.3.1) in one scalar loop.
+// Columns are visited in order: the next candidate column of cols, or x + 1
+// (inside the block, w columns) after x gained significance.  Lane x of vw0,
+// vw1, vw2: column x's words (Blk::column_words); the next candidate column's
+// words are read while the current one decodes.  rowm: the stripe's rows.
+// Column x's result goes to lane x of res: coded rows | new significance << 4
+// | their signs << 8.  A column's window takes the rows that turned
+// significant in the column to its left (carry, gcarry); rows next to them
+// join its candidates.
+__device__ __forceinline__ void spp_stripe(Mq& m, uint32_t& cs, uint32_t& cl, uint32_t& ctxv, uint64_t cols,
+                                           uint32_t vw0, uint32_t vw1, uint32_t vw2, uint32_t& res, uint32_t rowm,
+                                           uint32_t w, uint32_t zcv, uint32_t scv, uint32_t trm, uint32_t trl) {
+    uint32_t x, xp, carry, gcarry, w0, w1, pw0, pw1, pw2, sl, gl, rows, vis, nsig, csgn, below;
     uint32_t r, r3, u0, u1, lab, xr, d, q, t0, t1, t2, t3, n;
     asm volatile(
-        "s_cmp_eq_u32 %[rows], 0\n\t"
-        "s_cbranch_scc1 L%=end\n"
-        "L%=row:\n\t"
-        "s_ff1_i32_b32 %[r], %[rows]\n\t"
-        "s_bitset0_b32 %[rows], %[r]\n\t"
-        "s_bitset1_b32 %[vis], %[r]\n\t"
-        "s_lshl1_add_u32 %[r3], %[r], %[r]\n\t"
-        "s_xor_b32 %[u0], %[sl], %[w0]\n\t"
-        "s_add_u32 %[u1], %[r3], 0x90000\n\t"
-        "s_bfe_u32 %[u0], %[u0], %[u1]\n\t"
-        "s_cbranch_scc1 L%=zc\n\t"
-        "s_lshl2_add_u32 %[u1], %[r], 0x4000e\n\t"
-        "s_bfe_u32 %[lab], %[w1], %[u1]\n"
-        "L%=lab:\n\t"
-        "s_cmp_eq_u32 %[lab], %[cl]\n\t"
-        "s_cbranch_scc0 L%=zsw\n"
-        "L%=zdec:\n\t"
-        TC_MQ_BODY("z", "L%=zres")
-        "L%=zres:\n\t"
-        "s_cmp_eq_u32 %[d], 0\n\t"
-        "s_cbranch_scc0 L%=sig\n\t"
-        "s_cmp_lg_u32 %[rows], 0\n\t"
-        "s_cbranch_scc1 L%=row\n\t"
-        "s_branch L%=end\n"
-        // the window changed since the stripe began: label from the table
-        "L%=zc:\n\t"
-        "s_bfe_u32 %[u0], %[sl], %[u1]\n\t"
-        "s_lshr_b32 %[u1], %[u0], 3\n\t"
-        "v_readlane_b32 %[lab], %[zcv], %[u1]\n\t"
-        "s_and_b32 %[u0], %[u0], 7\n\t"
-        "s_lshl_b32 %[u0], %[u0], 2\n\t"
-        "s_lshr_b32 %[lab], %[lab], %[u0]\n\t"
-        "s_and_b32 %[lab], %[lab], 15\n\t"
-        "s_branch L%=lab\n"
-        "L%=zsw:\n\t"
-        "s_mov_b32 m0, %[cl]\n\t"
-        "v_writelane_b32 %[ctxv], %[cs], m0\n\t"
-        "s_mov_b32 %[cl], %[lab]\n\t"
-        "v_readlane_b32 %[cs], %[ctxv], %[lab]\n\t"
-        "s_branch L%=zdec\n"
-        // significant: the sign
-        "L%=sig:\n\t"
-        "s_lshr_b32 %[u0], %[sl], %[r3]\n\t"
-        "s_and_b32 %[u0], %[u0], 0xaa\n\t"
-        "s_lshr_b32 %[u1], %[gl], %[r3]\n\t"
-        "s_lshr_b32 %[u1], %[u1], 1\n\t"
-        "s_and_b32 %[u1], %[u1], 0x55\n\t"
+        "s_mov_b32 %[x], -1\n\t"
+        "s_mov_b32 %[nsig], 0\n\t"
+        "s_mov_b32 %[carry], 0\n\t"
+        "s_mov_b32 %[gcarry], 0\n\t"
+        "s_ff1_i32_b64 %[xp], %[cols]\n\t"
+        "s_max_i32 %[u0], %[xp], 0\n\t"
+        "v_readlane_b32 %[pw0], %[vw0], %[u0]\n\t"
+        "v_readlane_b32 %[pw1], %[vw1], %[u0]\n\t"
+        "v_readlane_b32 %[pw2], %[vw2], %[u0]\n"
+        "L%=col:\n\t"
+        "s_add_u32 %[u0], %[x], 1\n\t"
+        "s_cmp_lt_u32 %[u0], %[w]\n\t"
+        "s_cselect_b32 %[u1], %[nsig], 0\n\t"
+        "s_cmp_lg_u32 %[u1], 0\n\t"
+        "s_cselect_b32 %[r], %[u0], %[xp]\n\t"
+        "s_cmp_lt_i32 %[r], 0\n\t"
+        "s_cbranch_scc1 L%=end\n\t"
+        "s_cmp_eq_u32 %[r], %[u0]\n\t"
+        "s_cselect_b32 %[lab], %[nsig], 0\n\t"
+        "s_cselect_b32 %[t0], %[carry], 0\n\t"
+        "s_cselect_b32 %[t1], %[gcarry], 0\n\t"
+        "s_mov_b32 %[x], %[r]\n\t"
+        "s_cmp_eq_u32 %[x], %[xp]\n\t"
+        "s_cbranch_scc1 L%=pf\n\t"
+        "v_readlane_b32 %[w0], %[vw0], %[x]\n\t"
+        "v_readlane_b32 %[w1], %[vw1], %[x]\n\t"
+        "v_readlane_b32 %[t2], %[vw2], %[x]\n\t"
+        "s_branch L%=got\n"
+        "L%=pf:\n\t"
+        "s_mov_b32 %[w0], %[pw0]\n\t"
+        "s_mov_b32 %[w1], %[pw1]\n\t"
+        "s_mov_b32 %[t2], %[pw2]\n"
+        "L%=got:\n\t"
+        "s_bitset0_b64 %[cols], %[x]\n\t"
+        "s_ff1_i32_b64 %[xp], %[cols]\n\t"
+        "s_max_i32 %[u0], %[xp], 0\n\t"
+        "v_readlane_b32 %[pw0], %[vw0], %[u0]\n\t"
+        "v_readlane_b32 %[pw1], %[vw1], %[u0]\n\t"
+        "v_readlane_b32 %[pw2], %[vw2], %[u0]\n\t"
+        "s_bfe_u32 %[t3], %[w0], 0x40016\n\t"
+        "s_andn2_b32 %[below], %[rowm], %[t3]\n\t"
+        "s_and_b32 %[sl], %[w0], 0x3ffff\n\t"
+        "s_or_b32 %[sl], %[sl], %[t0]\n\t"
+        "s_or_b32 %[gl], %[t2], %[t1]\n\t"
+        "s_lshl_b32 %[u0], %[lab], 1\n\t"
+        "s_lshr_b32 %[u1], %[lab], 1\n\t"
         "s_or_b32 %[u0], %[u0], %[u1]\n\t"
-        "s_lshr_b32 %[u1], %[u0], 2\n\t"
-        "v_readlane_b32 %[lab], %[scv], %[u1]\n\t"
-        "s_and_b32 %[u0], %[u0], 3\n\t"
-        "s_lshl_b32 %[u0], %[u0], 3\n\t"
-        "s_lshr_b32 %[lab], %[lab], %[u0]\n\t"
-        "s_bfe_u32 %[xr], %[lab], 0x10004\n\t"
-        "s_and_b32 %[lab], %[lab], 15\n\t"
-        "s_cmp_eq_u32 %[lab], %[cl]\n\t"
-        "s_cbranch_scc1 L%=sdec\n\t"
-        "s_mov_b32 m0, %[cl]\n\t"
-        "v_writelane_b32 %[ctxv], %[cs], m0\n\t"
-        "s_mov_b32 %[cl], %[lab]\n\t"
-        "v_readlane_b32 %[cs], %[ctxv], %[lab]\n"
-        "L%=sdec:\n\t"
-        TC_MQ_BODY("s", "L%=sres")
-        "L%=sres:\n\t"
-        "s_xor_b32 %[d], %[d], %[xr]\n\t"
-        "s_add_u32 %[u0], %[r3], 4\n\t"
-        "s_bitset1_b32 %[sl], %[u0]\n\t"
-        "s_lshl_b32 %[u1], %[d], %[u0]\n\t"
-        "s_or_b32 %[gl], %[gl], %[u1]\n\t"
-        "s_add_u32 %[u0], %[r3], 3\n\t"
-        "s_bitset1_b32 %[carry], %[u0]\n\t"
-        "s_lshl_b32 %[u1], %[d], %[u0]\n\t"
-        "s_or_b32 %[gcarry], %[gcarry], %[u1]\n\t"
-        "s_bitset1_b32 %[nsig], %[r]\n\t"
-        "s_lshl_b32 %[u1], %[d], %[r]\n\t"
-        "s_or_b32 %[csgn], %[csgn], %[u1]\n\t"
-        "s_lshl_b32 %[u1], 2, %[r]\n\t"
-        "s_and_b32 %[u1], %[u1], %[below]\n\t"
-        "s_or_b32 %[rows], %[rows], %[u1]\n\t"
-        "s_cmp_lg_u32 %[rows], 0\n\t"
-        "s_cbranch_scc1 L%=row\n"
+        "s_or_b32 %[u0], %[u0], %[lab]\n\t"
+        "s_and_b32 %[u0], %[u0], %[below]\n\t"
+        "s_bfe_u32 %[rows], %[w0], 0x40012\n\t"
+        "s_or_b32 %[rows], %[rows], %[u0]\n\t"
+        "s_mov_b32 %[vis], 0\n\t"
+        "s_mov_b32 %[nsig], 0\n\t"
+        "s_mov_b32 %[csgn], 0\n\t"
+        "s_mov_b32 %[carry], 0\n\t"
+        "s_mov_b32 %[gcarry], 0\n\t"
+        "s_cmp_eq_u32 %[rows], 0\n\t"
+        "s_cbranch_scc1 L%=cend\n"
+        TC_ZC_ROWS("p", "L%=cend")
+        "L%=cend:\n\t"
+        "s_cmp_eq_u32 %[vis], 0\n\t"
+        "s_cbranch_scc1 L%=col\n\t"
+        "s_lshl_b32 %[u0], %[nsig], 4\n\t"
+        "s_or_b32 %[u0], %[u0], %[vis]\n\t"
+        "s_lshl_b32 %[u1], %[csgn], 8\n\t"
+        "s_or_b32 %[u0], %[u0], %[u1]\n\t"
+        "s_mov_b32 m0, %[x]\n\t"
+        "v_writelane_b32 %[res], %[u0], m0\n\t"
+        "s_branch L%=col\n"
         // no lane read may still be in flight when the registers are handed back
         "L%=end:\n\t"
-        "s_or_b32 %[u0], %[t2], %[t3]"
-        : [A] "+s"(m.A), [C] "+s"(m.C), [CT] "+s"(m.CT), [cs] "+s"(cs), [cl] "+s"(cl), [cur] "+s"(m.cur),
-          [np] "+s"(m.np), [wc] "+s"(m.wc), [wci] "+s"(m.wci), [ctxv] "+v"(ctxv), [sl] "+s"(c.sl), [gl] "+s"(c.gl),
-          [rows] "+s"(c.rows), [vis] "+s"(c.vis), [nsig] "+s"(c.nsig), [csgn] "+s"(c.csgn), [carry] "+s"(c.carry),
-          [gcarry] "+s"(c.gcarry), [r] "=&s"(r), [r3] "=&s"(r3), [u0] "=&s"(u0), [u1] "=&s"(u1), [lab] "=&s"(lab),
-          [xr] "=&s"(xr), [d] "=&s"(d), [q] "=&s"(q), [t0] "=&s"(t0), [t1] "=&s"(t1), [t2] "=&s"(t2), [t3] "=&s"(t3),
-          [n] "=&s"(n)
-        : [len] "s"(m.len), [wp] "s"(m.words), [w0] "s"(w0), [w1] "s"(w1), [below] "s"(below), [zcv] "v"(zcv),
+        "s_or_b32 %[u0], %[t2], %[t3]\n\t"
+        "s_or_b32 %[u0], %[pw0], %[pw1]\n\t"
+        "s_or_b32 %[u0], %[u0], %[pw2]"
+        TC_STRIPE_OPERANDS
+        : [len] "s"(m.len), [wp] "s"(m.words), [rowm] "s"(rowm), [w] "s"(w), [vw0] "v"(vw0), [vw1] "v"(vw1),
+          [vw2] "v"(vw2), [zcv] "v"(zcv), [scv] "v"(scv), [trm] "v"(trm), [trl] "v"(trl)
+        : "scc", "m0");
+}
+
+// Cleanup over one stripe (T.800 D.3.4) in one scalar loop over the candidate
+// columns of cols (columns of vw0/vw1/vw2 and res as in spp_stripe; the
+// result is new significance | their signs << 8).  A column of four
+// candidate rows with an empty window starts in run-length mode (states strl
+// and stuni for the run-length and uniform contexts): a 0 ends the column, a
+// 1 is followed by the row (two uniform decisions) and its sign, and the rows
+// after it continue as zero-coding rows.
+__device__ __forceinline__ void cup_stripe(Mq& m, uint32_t& cs, uint32_t& cl, uint32_t& ctxv, uint32_t& strl,
+                                           uint32_t& stuni, uint64_t cols, uint32_t vw0, uint32_t vw1, uint32_t vw2,
+                                           uint32_t& res, uint32_t zcv, uint32_t scv, uint32_t trm, uint32_t trl) {
+    uint32_t x, xp, carry, gcarry, w0, w1, pw0, pw1, pw2, sl, gl, rows, vis, nsig, csgn, below;
+    uint32_t r, r3, u0, u1, lab, xr, d, q, t0, t1, t2, t3, n;
+    asm volatile(
+        "s_mov_b32 %[x], -2\n\t"
+        "s_mov_b32 %[below], 0\n\t"
+        "s_mov_b32 %[carry], 0\n\t"
+        "s_mov_b32 %[gcarry], 0\n\t"
+        "s_ff1_i32_b64 %[xp], %[cols]\n\t"
+        "s_max_i32 %[u0], %[xp], 0\n\t"
+        "v_readlane_b32 %[pw0], %[vw0], %[u0]\n\t"
+        "v_readlane_b32 %[pw1], %[vw1], %[u0]\n\t"
+        "v_readlane_b32 %[pw2], %[vw2], %[u0]\n"
+        "L%=col:\n\t"
+        "s_cmp_lt_i32 %[xp], 0\n\t"
+        "s_cbranch_scc1 L%=end\n\t"
+        "s_add_u32 %[u0], %[x], 1\n\t"
+        "s_cmp_eq_u32 %[xp], %[u0]\n\t"
+        "s_cselect_b32 %[t0], %[carry], 0\n\t"
+        "s_cselect_b32 %[t1], %[gcarry], 0\n\t"
+        "s_mov_b32 %[x], %[xp]\n\t"
+        "s_mov_b32 %[w0], %[pw0]\n\t"
+        "s_mov_b32 %[w1], %[pw1]\n\t"
+        "s_mov_b32 %[t2], %[pw2]\n\t"
+        "s_bitset0_b64 %[cols], %[x]\n\t"
+        "s_ff1_i32_b64 %[xp], %[cols]\n\t"
+        "s_max_i32 %[u0], %[xp], 0\n\t"
+        "v_readlane_b32 %[pw0], %[vw0], %[u0]\n\t"
+        "v_readlane_b32 %[pw1], %[vw1], %[u0]\n\t"
+        "v_readlane_b32 %[pw2], %[vw2], %[u0]\n\t"
+        "s_and_b32 %[sl], %[w0], 0x3ffff\n\t"
+        "s_or_b32 %[sl], %[sl], %[t0]\n\t"
+        "s_or_b32 %[gl], %[t2], %[t1]\n\t"
+        "s_bfe_u32 %[rows], %[w0], 0x40012\n\t"
+        "s_mov_b32 %[nsig], 0\n\t"
+        "s_mov_b32 %[csgn], 0\n\t"
+        "s_mov_b32 %[carry], 0\n\t"
+        "s_mov_b32 %[gcarry], 0\n\t"
+        "s_cmp_eq_u32 %[rows], 15\n\t"
+        "s_cbranch_scc0 L%=zrows\n\t"
+        "s_cmp_eq_u32 %[sl], 0\n\t"
+        "s_cbranch_scc0 L%=zrows\n\t"
+        TC_MQ_BODY("a", "strl", "L%=rl1")
+        "L%=rl1:\n\t"
+        "s_cmp_eq_u32 %[d], 0\n\t"
+        "s_cbranch_scc1 L%=col\n\t"
+        TC_MQ_BODY("b", "stuni", "L%=rl2")
+        "L%=rl2:\n\t"
+        "s_lshl_b32 %[r], %[d], 1\n\t"
+        TC_MQ_BODY("c", "stuni", "L%=rl3")
+        "L%=rl3:\n\t"
+        "s_or_b32 %[r], %[r], %[d]\n\t"
+        "s_lshl1_add_u32 %[r3], %[r], %[r]\n\t"
+        TC_SIGN_UPDATE("e", "L%=rl4")
+        "L%=rl4:\n\t"
+        "s_lshl_b32 %[u0], 2, %[r]\n\t"
+        "s_sub_u32 %[u0], %[u0], 1\n\t"
+        "s_andn2_b32 %[rows], %[rows], %[u0]\n"
+        "L%=zrows:\n\t"
+        "s_cmp_eq_u32 %[rows], 0\n\t"
+        "s_cbranch_scc1 L%=cend\n"
+        TC_ZC_ROWS("q", "L%=cend")
+        "L%=cend:\n\t"
+        "s_cmp_eq_u32 %[nsig], 0\n\t"
+        "s_cbranch_scc1 L%=col\n\t"
+        "s_lshl_b32 %[u1], %[csgn], 8\n\t"
+        "s_or_b32 %[u0], %[nsig], %[u1]\n\t"
+        "s_mov_b32 m0, %[x]\n\t"
+        "v_writelane_b32 %[res], %[u0], m0\n\t"
+        "s_branch L%=col\n"
+        // no lane read may still be in flight when the registers are handed back
+        "L%=end:\n\t"
+        "s_or_b32 %[u0], %[t2], %[t3]\n\t"
+        "s_or_b32 %[u0], %[pw0], %[pw1]\n\t"
+        "s_or_b32 %[u0], %[u0], %[pw2]"
+        TC_STRIPE_OPERANDS, [strl] "+s"(strl), [stuni] "+s"(stuni)
+        : [len] "s"(m.len), [wp] "s"(m.words), [vw0] "v"(vw0), [vw1] "v"(vw1), [vw2] "v"(vw2), [zcv] "v"(zcv),
           [scv] "v"(scv), [trm] "v"(trm), [trl] "v"(trl)
         : "scc", "m0");
 }
@@ -631,31 +700,13 @@ __device__ __forceinline__ uint32_t from_right(uint32_t v) {
 struct Blk {
     Mq m;
     uint32_t ctxv;            // lane l < 17: state of zero/sign/refinement context l
-    uint32_t cl, cs;          // context used last and its current state
+    // the context used last and its current state; the state goes back to
+    // lane cl of ctxv when a decision uses another context
+    uint32_t cl, cs;
     uint32_t st_rl, st_uni;
     uint32_t zcv, scv, trm, trl;
     uint64_t sig, sgn, pip, ref, ones;  // lane x: column x (bit y = row y)
     int w, h, cls;
-
-    // The state of the context used last stays in a scalar register (cl, cs)
-    // and goes back to its lane when another context is used.
-    __device__ __forceinline__ uint32_t dec_ctx(uint32_t lab) {
-        if (lab != cl) {
-            ctxv = wl(ctxv, cl, cs);
-            cs = rl(ctxv, lab);
-            cl = lab;
-        }
-        return mq_dec(m, cs, trm, trl);
-    }
-    // sign of the sample at stripe row r that just became significant (T.800
-    // D.3.2).  sl, gl: significance and sign windows (ColRows); the key holds
-    // (sign, significance) of the upper, left, right and lower neighbours, in
-    // bit pairs from bit 0.
-    __device__ __forceinline__ uint32_t dec_sign(uint32_t sl, uint32_t gl, int r) {
-        const uint32_t idx = ((sl >> (3 * r)) & 0xAAu) | ((gl >> (3 * r + 1)) & 0x55u);
-        const uint32_t e = (rl(scv, idx >> 2) >> ((idx & 3u) << 3)) & 0xFFu;
-        return dec_ctx(e & 15u) ^ (e >> 4);
-    }
 
     // Per-column words of a stripe for the significance and cleanup passes.
     // w0: significance window (18 bits) | candidate rows << 18 | significant
@@ -698,38 +749,8 @@ struct Blk {
         uint32_t vw0, vw1, vw2;
         uint64_t cols;
         column_words(y0, rowm, false, vw0, vw1, vw2, cols);
-        uint32_t res = 0;
-        int x = -1;
-        // new significance of column x: window bits, sign bits, rows
-        uint32_t carry = 0, gcarry = 0, crows = 0;
-        for (;;) {
-            int nx;
-            if (crows != 0u && x + 1 < w) {
-                nx = x + 1;
-            } else {
-                const uint64_t rem = x >= 63 ? 0ull : cols & (~0ull << (x + 1));
-                if (!rem) break;
-                nx = (int)__builtin_ctzll(rem);
-            }
-            const bool adj = nx == x + 1;
-            const uint32_t lc = adj ? carry : 0u, lr = adj ? crows : 0u, lg = adj ? gcarry : 0u;
-            x = nx;
-            const uint32_t w0 = rl(vw0, x), w1 = rl(vw1, x), w2 = rl(vw2, x);
-            const uint32_t s4 = (w0 >> 22) & 15u;
-            ColRows c;
-            c.sl = (w0 & 0x3FFFFu) | lc;
-            c.gl = w2 | lg;
-            // rows to code: the static candidates, and insignificant rows next to
-            // the left column's new significance; a row also joins when the row
-            // above it becomes significant
-            c.rows = ((w0 >> 18) & 15u) | ((lr | (lr << 1) | (lr >> 1)) & ~s4 & rowm);
-            c.vis = c.nsig = c.csgn = c.carry = c.gcarry = 0;
-            zc_rows(m, cs, cl, ctxv, c, w0, w1, ~s4 & rowm, zcv, scv, trm, trl);
-            carry = c.carry;
-            gcarry = c.gcarry;
-            crows = c.nsig;
-            if (c.vis) res = wl(res, (uint32_t)x, c.vis | (c.nsig << 4) | (c.csgn << 8));
-        }
+        uint32_t res = 0;  // lane x: coded rows | new significance << 4 | signs << 8
+        if (cols) spp_stripe(m, cs, cl, ctxv, cols, vw0, vw1, vw2, res, rowm, (uint32_t)w, zcv, scv, trm, trl);
         const uint64_t nv = (uint64_t)(res & 15u) << y0, ns = (uint64_t)((res >> 4) & 15u) << y0;
         pip |= nv;
         sig |= ns;
@@ -784,41 +805,8 @@ struct Blk {
         uint32_t vw0, vw1, vw2;
         uint64_t cols;
         column_words(y0, rowm, true, vw0, vw1, vw2, cols);
-        uint32_t res = 0;
-        int lastx = -2;
-        uint32_t carry = 0, gcarry = 0;
-        while (cols) {
-            const int x = (int)__builtin_ctzll(cols);
-            cols &= cols - 1;
-            const bool adj = x == lastx + 1;
-            const uint32_t lc = adj ? carry : 0u, lg = adj ? gcarry : 0u;
-            lastx = x;
-            carry = 0;
-            gcarry = 0;
-            const uint32_t w0 = rl(vw0, (uint32_t)x), w1 = rl(vw1, (uint32_t)x), w2 = rl(vw2, (uint32_t)x);
-            ColRows c;
-            c.sl = (w0 & 0x3FFFFu) | lc;
-            c.gl = w2 | lg;
-            c.rows = (w0 >> 18) & 15u;
-            c.vis = c.nsig = c.csgn = c.carry = c.gcarry = 0;
-            if (c.rows == 15u && c.sl == 0u) {  // run-length mode
-                if (!mq_dec(m, st_rl, trm, trl)) continue;
-                int r = (int)mq_dec(m, st_uni, trm, trl) << 1;
-                r |= (int)mq_dec(m, st_uni, trm, trl);
-                const uint32_t sg = dec_sign(c.sl, c.gl, r);
-                c.sl |= 1u << (3 * r + 4);
-                c.gl |= sg << (3 * r + 4);
-                c.nsig |= 1u << r;
-                c.csgn |= sg << r;
-                c.carry |= 1u << (3 * r + 3);
-                c.gcarry |= sg << (3 * r + 3);
-                c.rows &= ~((2u << r) - 1u);  // rows after r
-            }
-            zc_rows(m, cs, cl, ctxv, c, w0, w1, 0u, zcv, scv, trm, trl);
-            carry = c.carry;
-            gcarry = c.gcarry;
-            if (c.nsig) res = wl(res, (uint32_t)x, c.nsig | (c.csgn << 8));
-        }
+        uint32_t res = 0;  // lane x: new significance | signs << 8
+        if (cols) cup_stripe(m, cs, cl, ctxv, st_rl, st_uni, cols, vw0, vw1, vw2, res, zcv, scv, trm, trl);
         const uint64_t ns = (uint64_t)(res & 15u) << y0;
         sig |= ns;
         ones |= ns;
@@ -940,7 +928,7 @@ __global__ __launch_bounds__(64) void t1_stripe_kernel(const tc_cblk_desc* __res
     }
     uint32_t scv = 0;
     for (int b = 0; b < 4; b++) {
-        // key (Blk::dec_sign) -> the sign-coding table's index: significance
+        // key (TC_SIGN_UPDATE) -> the sign-coding table's index: significance
         // and sign of left, right, upper, lower neighbour, bits 0..7
         const uint32_t k = (uint32_t)(lane * 4 + b);
         const uint32_t i = ((k >> 3) & 1u) | (((k >> 2) & 1u) << 1) | (((k >> 5) & 1u) << 2) |
