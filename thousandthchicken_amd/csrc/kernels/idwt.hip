@@ -56,6 +56,14 @@ __device__ __forceinline__ int pse(int i, int i0, int i1) {
 __device__ __forceinline__ int floor_half(int v) { return v >> 1; }
 __device__ __forceinline__ int ceil_half(int v) { return (v + 1) >> 1; }
 
+// A work-group-uniform 64-bit value (a band offset) as a scalar register
+// value.  The lane reads keep the compiler from choosing between the band
+// offsets through a dynamically addressed copy of Src in scratch memory.
+__device__ __forceinline__ uint64_t uni64(uint64_t v) {
+    return (uint64_t)(uint32_t)__builtin_amdgcn_readfirstlane((int)(uint32_t)v) |
+           ((uint64_t)(uint32_t)__builtin_amdgcn_readfirstlane((int)(uint32_t)(v >> 32)) << 32);
+}
+
 // Source element of interleaved position (u, v) of the level described by
 // (u0,u1,v0,v1): picks LL/HL/LH/HH by parity and returns its element offset.
 struct Src {
@@ -199,13 +207,17 @@ __device__ __forceinline__ void tile_lift_inv(typename Lift<IS97>::T* tile, int 
 // with the reconstructed samples at tile[H+ty][H+tx].
 template <bool IS97, int TX, int TY>
 __device__ void tile_idwt(typename Lift<IS97>::T* tile, int pitch, const typename Lift<IS97>::T* __restrict__ src,
-                          const Src& s, int u0, int u1, int v0, int v1, int ox, int oy, bool copy_only,
+                          const Src& s_in, int u0, int u1, int v0, int v1, int ox, int oy, bool copy_only,
                           bool lift = true) {
     typedef typename Lift<IS97>::T T;
     constexpr int H = Lift<IS97>::H;
     constexpr int W = TX + 2 * H, HH = TY + 2 * H;
     const int tid = threadIdx.x;
     const bool h1 = (u1 - u0) == 1, v1x = (v1 - v0) == 1;
+    const uint64_t bll = uni64(s_in.ll), bhl = uni64(s_in.hl), blh = uni64(s_in.lh), bhh = uni64(s_in.hh);
+    const uint32_t llp = (uint32_t)__builtin_amdgcn_readfirstlane((int)s_in.llp);
+    const uint32_t bp = (uint32_t)__builtin_amdgcn_readfirstlane((int)s_in.bp);
+    const Src s{bll, bhl, blh, bhh, llp, bp};
     // interior tile: the extended window needs no mirroring, and every load of
     // a thread is issued before the first is used
     if (!copy_only && ox - H >= u0 && ox + TX + H <= u1 && oy - H >= v0 && oy + TY + H <= v1) {
@@ -221,8 +233,8 @@ __device__ void tile_idwt(typename Lift<IS97>::T* tile, int pitch, const typenam
                 const int ou = u & 1, ov = v & 1;
                 const int cx = ou ? floor_half(u) - fu0 : (u >> 1) - cu0;
                 const int cy = ov ? floor_half(v) - fv0 : (v >> 1) - cv0;
-                const uint64_t b = ov ? (ou ? s.hh : s.lh) : (ou ? s.hl : s.ll);
-                const uint32_t bpitch = (ou | ov) ? s.bp : s.llp;
+                const uint64_t b = ov ? (ou ? bhh : blh) : (ou ? bhl : bll);
+                const uint32_t bpitch = (ou | ov) ? bp : llp;
                 vals[k] = src[b + (uint64_t)cy * bpitch + cx];
             }
         }
