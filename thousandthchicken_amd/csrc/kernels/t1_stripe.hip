@@ -162,7 +162,15 @@ __device__ __forceinline__ void mq_mrp_stripe(Mq& m, uint32_t& cs, uint32_t& cl,
         "s_cselect_b32 %[t2], %[t2], %[t3]\n\t"
         "s_cselect_b32 %[t1], 0, 1\n\t"
         "s_mov_b32 %[A], %[q]\n\t"
-        "s_branch L%=_renorm\n"
+        "s_xor_b32 %[cs], %[t2], %[d]\n\t"
+        "s_xor_b32 %[d], %[d], %[t1]\n\t"
+        "s_flbit_i32_b32 %[n], %[A]\n\t"
+        "s_lshl1_add_u32 %[bits], %[bits], %[d]\n\t"
+        "s_sub_u32 %[CT], %[CT], %[n]\n\t"
+        "s_cbranch_scc1 L%=_refill\n\t"
+        "s_lshl_b32 %[C], %[C], %[n]\n\t"
+        "s_lshl_b32 %[A], %[A], %[n]\n\t"
+        "s_branch L%=_next\n"
         // another label, or the column's end
         "L%=_other:\n\t"
         "s_cmp_eq_u32 %[code], 15\n\t"
@@ -182,8 +190,32 @@ __device__ __forceinline__ void mq_mrp_stripe(Mq& m, uint32_t& cs, uint32_t& cl,
         "s_mov_b32 %[x], %[xn]\n\t"
         "s_mov_b32 %[iw], %[iwn]\n\t"
         "s_branch L%=_col\n"
-        // CT < n: RENORMD with BYTEIN as in TC_MQ_BODY
+        // CT < n: RENORMD with BYTEIN as in TC_MQ_BODY (TC_MQ_REFILL's common
+        // case in one straight run, anything else byte by byte)
         "L%=_refill:\n\t"
+        "s_cmp_ge_u32 %[np], %[len]\n\t"
+        "s_cbranch_scc1 L%=_slow\n\t"
+        "s_cmp_eq_u32 %[cur], 0xff\n\t"
+        "s_cbranch_scc1 L%=_slow\n\t"
+        "s_lshr_b32 %[t1], %[np], 2\n\t"
+        "s_cmp_eq_u32 %[t1], %[wci]\n\t"
+        "s_cbranch_scc0 L%=_slow\n\t"
+        "s_add_u32 %[t1], %[CT], 8\n\t"
+        "s_cmp_lt_i32 %[t1], 0\n\t"
+        "s_cbranch_scc1 L%=_slow\n\t"
+        "s_mov_b32 %[CT], %[t1]\n\t"
+        "s_and_b32 %[t1], %[np], 3\n\t"
+        "s_lshl_b32 %[t1], %[t1], 3\n\t"
+        "s_lshr_b32 %[t0], %[wc], %[t1]\n\t"
+        "s_and_b32 %[cur], %[t0], 0xff\n\t"
+        "s_lshl_b32 %[C], %[C], %[n]\n\t"
+        "s_sub_u32 %[t1], 16, %[CT]\n\t"
+        "s_lshl_b32 %[t0], %[cur], %[t1]\n\t"
+        "s_add_u32 %[C], %[C], %[t0]\n\t"
+        "s_lshl_b32 %[A], %[A], %[n]\n\t"
+        "s_add_u32 %[np], %[np], 1\n\t"
+        "s_branch L%=_next\n"
+        "L%=_slow:\n\t"
         "s_add_u32 %[CT], %[CT], %[n]\n"
         "L%=_loop:\n\t"
         "s_cmp_eq_u32 %[CT], 0\n\t"
@@ -256,6 +288,37 @@ __device__ __forceinline__ void mq_mrp_stripe(Mq& m, uint32_t& cs, uint32_t& cl,
 // LPS (the LPS one with SWITCH in bit 0); both are read first so their
 // latency overlaps the interval arithmetic.  CT is reduced first and restored
 // on the refill path; the codeword dword load waits for itself.
+// RENORMD across a byte boundary (CT < n; CT already reduced by n, so it
+// holds CT - n), common case in one straight run: the byte at BP is in the
+// cached dword, the byte before it is not 0xFF and one byte covers the rest
+// of the shift (n - CT <= 8).  BYTEIN's C += B << 8 lands before the last
+// n - CT shifts, so C = (C << n) + (B << (8 + n - CT)) and CT = 8 - (n - CT).
+// Anything else continues at P "slow" (CT restored there).
+#define TC_MQ_REFILL(P, EXIT)                        \
+    "L%=" P "refill:\n\t"                             \
+    "s_cmp_ge_u32 %[np], %[len]\n\t"                  \
+    "s_cbranch_scc1 L%=" P "slow\n\t"                 \
+    "s_cmp_eq_u32 %[cur], 0xff\n\t"                   \
+    "s_cbranch_scc1 L%=" P "slow\n\t"                 \
+    "s_lshr_b32 %[t1], %[np], 2\n\t"                  \
+    "s_cmp_eq_u32 %[t1], %[wci]\n\t"                  \
+    "s_cbranch_scc0 L%=" P "slow\n\t"                 \
+    "s_add_u32 %[t1], %[CT], 8\n\t"                   \
+    "s_cmp_lt_i32 %[t1], 0\n\t"                       \
+    "s_cbranch_scc1 L%=" P "slow\n\t"                 \
+    "s_mov_b32 %[CT], %[t1]\n\t"                      \
+    "s_and_b32 %[t1], %[np], 3\n\t"                   \
+    "s_lshl_b32 %[t1], %[t1], 3\n\t"                  \
+    "s_lshr_b32 %[t0], %[wc], %[t1]\n\t"              \
+    "s_and_b32 %[cur], %[t0], 0xff\n\t"               \
+    "s_lshl_b32 %[C], %[C], %[n]\n\t"                 \
+    "s_sub_u32 %[t1], 16, %[CT]\n\t"                  \
+    "s_lshl_b32 %[t0], %[cur], %[t1]\n\t"             \
+    "s_add_u32 %[C], %[C], %[t0]\n\t"                 \
+    "s_lshl_b32 %[A], %[A], %[n]\n\t"                 \
+    "s_add_u32 %[np], %[np], 1\n\t"                   \
+    "s_branch " EXIT "\n"
+
 #define TC_MQ_BODY(P, ST, EXIT)                      \
     "s_bfe_u32 %[t0], %[" ST "], 0x60001\n\t"             \
     "s_and_b32 %[q], %[" ST "], 0xffff0000\n\t"           \
@@ -285,8 +348,16 @@ __device__ __forceinline__ void mq_mrp_stripe(Mq& m, uint32_t& cs, uint32_t& cl,
     "s_cselect_b32 %[t2], %[t2], %[t3]\n\t"           \
     "s_cselect_b32 %[t1], 0, 1\n\t"                   \
     "s_mov_b32 %[A], %[q]\n\t"                        \
-    "s_branch L%=" P "renorm\n"                       \
-    "L%=" P "refill:\n\t"                             \
+    "s_xor_b32 %[" ST "], %[t2], %[d]\n\t"            \
+    "s_xor_b32 %[d], %[d], %[t1]\n\t"                 \
+    "s_flbit_i32_b32 %[n], %[A]\n\t"                  \
+    "s_sub_u32 %[CT], %[CT], %[n]\n\t"                \
+    "s_cbranch_scc1 L%=" P "refill\n\t"               \
+    "s_lshl_b32 %[C], %[C], %[n]\n\t"                 \
+    "s_lshl_b32 %[A], %[A], %[n]\n\t"                 \
+    "s_branch " EXIT "\n"                             \
+    TC_MQ_REFILL(P, EXIT)                             \
+    "L%=" P "slow:\n\t"                               \
     "s_add_u32 %[CT], %[CT], %[n]\n"                  \
     "L%=" P "loop:\n\t"                               \
     "s_cmp_eq_u32 %[CT], 0\n\t"                       \
