@@ -101,200 +101,13 @@ __device__ __forceinline__ void mq_init(Mq& m, const uint32_t* words, uint32_t l
     m.A = 0x80000000u;
 }
 
-// The refinement decisions of one stripe (T.800 D.3.3) in one scalar loop.
-// cols: the columns holding candidates (not empty); lane x of stream: column
-// x's candidate labels in scan order, 4 bits each (label - 14), ended by 15.
-// Column x's decisions go to lane x of res, the first one highest.  Each
-// decision is DECODE as in TC_MQ_BODY below; a label other than the one used
-// last swaps the context states through the lane array ctxv (cl/cs as in Blk;
-// cl is held as label - 14 inside).  The next column's label word is read
-// while the current column decodes.  The common path, a renormalising
-// decision with bits left in C, runs straight to the loop's one taken branch;
-// CT is reduced first and restored on the refill path.
-__device__ __forceinline__ void mq_mrp_stripe(Mq& m, uint32_t& cs, uint32_t& cl, uint32_t& ctxv, uint64_t cols,
-                                              uint32_t stream, uint32_t& res, uint32_t trm, uint32_t trl) {
-    uint32_t x, xn, iw, iwn, bits, code, d, q, t0, t1, t2, t3, n;
-    asm volatile(
-        "s_sub_u32 %[cl], %[cl], 14\n\t"
-        "s_ff1_i32_b64 %[x], %[cols]\n\t"
-        "v_readlane_b32 %[iw], %[stream], %[x]\n"
-        "L%=_col:\n\t"
-        "s_bitset0_b64 %[cols], %[x]\n\t"
-        "s_ff1_i32_b64 %[xn], %[cols]\n\t"
-        "s_max_i32 %[xn], %[xn], 0\n\t"
-        "s_mov_b32 %[bits], 0\n\t"
-        "v_readlane_b32 %[iwn], %[stream], %[xn]\n"
-        "L%=_next:\n\t"
-        "s_and_b32 %[code], %[iw], 15\n\t"
-        "s_lshr_b32 %[iw], %[iw], 4\n\t"
-        "s_cmp_eq_u32 %[code], %[cl]\n\t"
-        "s_cbranch_scc0 L%=_other\n"
-        "L%=_dec:\n\t"
-        "s_bfe_u32 %[t0], %[cs], 0x60001\n\t"
-        "s_and_b32 %[q], %[cs], 0xffff0000\n\t"
-        "s_and_b32 %[d], %[cs], 1\n\t"
-        "s_sub_u32 %[A], %[A], %[q]\n\t"
-        "v_readlane_b32 %[t2], %[trm], %[t0]\n\t"
-        "v_readlane_b32 %[t3], %[trl], %[t0]\n\t"
-        "s_cmp_lt_u32 %[C], %[q]\n\t"
-        "s_cbranch_scc1 L%=_lower\n\t"
-        "s_sub_u32 %[C], %[C], %[q]\n\t"
-        "s_cmp_lt_i32 %[A], 0\n\t"
-        "s_cbranch_scc1 L%=_fast\n\t"
-        "s_cmp_lt_u32 %[A], %[q]\n\t"
-        "s_cselect_b32 %[t2], %[t3], %[t2]\n\t"
-        "s_cselect_b32 %[t1], 1, 0\n"
-        "L%=_renorm:\n\t"
-        "s_xor_b32 %[cs], %[t2], %[d]\n\t"
-        "s_xor_b32 %[d], %[d], %[t1]\n\t"
-        "s_flbit_i32_b32 %[n], %[A]\n\t"
-        "s_lshl1_add_u32 %[bits], %[bits], %[d]\n\t"
-        "s_sub_u32 %[CT], %[CT], %[n]\n\t"
-        "s_cbranch_scc1 L%=_refill\n\t"
-        "s_lshl_b32 %[C], %[C], %[n]\n\t"
-        "s_lshl_b32 %[A], %[A], %[n]\n\t"
-        "s_branch L%=_next\n"
-        "L%=_fast:\n\t"
-        "s_lshl1_add_u32 %[bits], %[bits], %[d]\n\t"
-        "s_branch L%=_next\n"
-        "L%=_lower:\n\t"
-        "s_cmp_lt_u32 %[A], %[q]\n\t"
-        "s_cselect_b32 %[t2], %[t2], %[t3]\n\t"
-        "s_cselect_b32 %[t1], 0, 1\n\t"
-        "s_mov_b32 %[A], %[q]\n\t"
-        "s_xor_b32 %[cs], %[t2], %[d]\n\t"
-        "s_xor_b32 %[d], %[d], %[t1]\n\t"
-        "s_flbit_i32_b32 %[n], %[A]\n\t"
-        "s_lshl1_add_u32 %[bits], %[bits], %[d]\n\t"
-        "s_sub_u32 %[CT], %[CT], %[n]\n\t"
-        "s_cbranch_scc1 L%=_refill\n\t"
-        "s_lshl_b32 %[C], %[C], %[n]\n\t"
-        "s_lshl_b32 %[A], %[A], %[n]\n\t"
-        "s_branch L%=_next\n"
-        // another label, or the column's end
-        "L%=_other:\n\t"
-        "s_cmp_eq_u32 %[code], 15\n\t"
-        "s_cbranch_scc1 L%=_colend\n\t"
-        "s_add_u32 %[t0], %[cl], 14\n\t"
-        "s_mov_b32 m0, %[t0]\n\t"
-        "v_writelane_b32 %[ctxv], %[cs], m0\n\t"
-        "s_add_u32 %[t0], %[code], 14\n\t"
-        "s_mov_b32 %[cl], %[code]\n\t"
-        "v_readlane_b32 %[cs], %[ctxv], %[t0]\n\t"
-        "s_branch L%=_dec\n"
-        "L%=_colend:\n\t"
-        "s_mov_b32 m0, %[x]\n\t"
-        "v_writelane_b32 %[res], %[bits], m0\n\t"
-        "s_cmp_eq_u64 %[cols], 0\n\t"
-        "s_cbranch_scc1 L%=_end\n\t"
-        "s_mov_b32 %[x], %[xn]\n\t"
-        "s_mov_b32 %[iw], %[iwn]\n\t"
-        "s_branch L%=_col\n"
-        // CT < n: RENORMD with BYTEIN as in TC_MQ_BODY (TC_MQ_REFILL's common
-        // case in one straight run, anything else byte by byte)
-        "L%=_refill:\n\t"
-        "s_cmp_ge_u32 %[np], %[len]\n\t"
-        "s_cbranch_scc1 L%=_slow\n\t"
-        "s_cmp_eq_u32 %[cur], 0xff\n\t"
-        "s_cbranch_scc1 L%=_slow\n\t"
-        "s_lshr_b32 %[t1], %[np], 2\n\t"
-        "s_cmp_eq_u32 %[t1], %[wci]\n\t"
-        "s_cbranch_scc0 L%=_slow\n\t"
-        "s_add_u32 %[t1], %[CT], 8\n\t"
-        "s_cmp_lt_i32 %[t1], 0\n\t"
-        "s_cbranch_scc1 L%=_slow\n\t"
-        "s_mov_b32 %[CT], %[t1]\n\t"
-        "s_and_b32 %[t1], %[np], 3\n\t"
-        "s_lshl_b32 %[t1], %[t1], 3\n\t"
-        "s_lshr_b32 %[t0], %[wc], %[t1]\n\t"
-        "s_and_b32 %[cur], %[t0], 0xff\n\t"
-        "s_lshl_b32 %[C], %[C], %[n]\n\t"
-        "s_sub_u32 %[t1], 16, %[CT]\n\t"
-        "s_lshl_b32 %[t0], %[cur], %[t1]\n\t"
-        "s_add_u32 %[C], %[C], %[t0]\n\t"
-        "s_lshl_b32 %[A], %[A], %[n]\n\t"
-        "s_add_u32 %[np], %[np], 1\n\t"
-        "s_branch L%=_next\n"
-        "L%=_slow:\n\t"
-        "s_add_u32 %[CT], %[CT], %[n]\n"
-        "L%=_loop:\n\t"
-        "s_cmp_eq_u32 %[CT], 0\n\t"
-        "s_cbranch_scc0 L%=_shift\n\t"
-        "s_cmp_ge_u32 %[np], %[len]\n\t"
-        "s_mov_b32 %[t0], 0xff\n\t"
-        "s_cbranch_scc1 L%=_got\n\t"
-        "s_lshr_b32 %[t1], %[np], 2\n\t"
-        "s_cmp_eq_u32 %[t1], %[wci]\n\t"
-        "s_cbranch_scc1 L%=_cached\n\t"
-        "s_mov_b32 %[wci], %[t1]\n\t"
-        "s_lshl_b32 %[t1], %[t1], 2\n\t"
-        "s_load_dword %[wc], %[wp], %[t1]\n\t"
-        "s_waitcnt lgkmcnt(0)\n"
-        "L%=_cached:\n\t"
-        "s_and_b32 %[t1], %[np], 3\n\t"
-        "s_lshl_b32 %[t1], %[t1], 3\n\t"
-        "s_lshr_b32 %[t0], %[wc], %[t1]\n\t"
-        "s_and_b32 %[t0], %[t0], 0xff\n"
-        "L%=_got:\n\t"
-        "s_cmp_eq_u32 %[cur], 0xff\n\t"
-        "s_cbranch_scc0 L%=_plain\n\t"
-        "s_cmp_gt_u32 %[t0], 0x8f\n\t"
-        "s_cbranch_scc0 L%=_stuff\n\t"
-        "s_add_u32 %[C], %[C], 0xff00\n\t"
-        "s_mov_b32 %[CT], 8\n\t"
-        "s_branch L%=_shift\n"
-        "L%=_stuff:\n\t"
-        "s_add_u32 %[np], %[np], 1\n\t"
-        "s_mov_b32 %[cur], %[t0]\n\t"
-        "s_lshl_b32 %[t0], %[t0], 9\n\t"
-        "s_add_u32 %[C], %[C], %[t0]\n\t"
-        "s_mov_b32 %[CT], 7\n\t"
-        "s_branch L%=_shift\n"
-        "L%=_plain:\n\t"
-        "s_add_u32 %[np], %[np], 1\n\t"
-        "s_mov_b32 %[cur], %[t0]\n\t"
-        "s_lshl_b32 %[t0], %[t0], 8\n\t"
-        "s_add_u32 %[C], %[C], %[t0]\n\t"
-        "s_mov_b32 %[CT], 8\n"
-        "L%=_shift:\n\t"
-        "s_min_u32 %[t1], %[n], %[CT]\n\t"
-        "s_lshl_b32 %[C], %[C], %[t1]\n\t"
-        "s_lshl_b32 %[A], %[A], %[t1]\n\t"
-        "s_sub_u32 %[CT], %[CT], %[t1]\n\t"
-        "s_sub_u32 %[n], %[n], %[t1]\n\t"
-        "s_cmp_eq_u32 %[n], 0\n\t"
-        "s_cbranch_scc0 L%=_loop\n\t"
-        "s_branch L%=_next\n"
-        // no lane read may still be in flight when the registers are handed back
-        "L%=_end:\n\t"
-        "s_or_b32 %[t2], %[t2], %[t3]\n\t"
-        "s_or_b32 %[t2], %[t2], %[iwn]\n\t"
-        "s_add_u32 %[cl], %[cl], 14"
-        : [A] "+s"(m.A), [C] "+s"(m.C), [CT] "+s"(m.CT), [cs] "+s"(cs), [cl] "+s"(cl), [cur] "+s"(m.cur),
-          [np] "+s"(m.np), [wc] "+s"(m.wc), [wci] "+s"(m.wci), [cols] "+s"(cols), [ctxv] "+v"(ctxv),
-          [res] "+v"(res), [x] "=&s"(x), [xn] "=&s"(xn), [iw] "=&s"(iw), [iwn] "=&s"(iwn), [bits] "=&s"(bits),
-          [code] "=&s"(code), [d] "=&s"(d), [q] "=&s"(q), [t0] "=&s"(t0), [t1] "=&s"(t1), [t2] "=&s"(t2),
-          [t3] "=&s"(t3), [n] "=&s"(n)
-        : [len] "s"(m.len), [wp] "s"(m.words), [stream] "v"(stream), [trm] "v"(trm), [trl] "v"(trl)
-        : "scc", "m0");
-}
-
-// DECODE (C.3.2) with the conditional exchanges (C.3.3), the successor state
-// and RENORMD/BYTEIN (C.3.3/C.3.4), RENORMD shifting min(needed, CT) bits per
-// step (the same bits as one at a time), over the operands of the asm block
-// that uses it: label prefix P, context state operand ST (Qe << 16 | I << 1 |
-// MPS, updated in place), the decision in d; every path ends with a branch to
-// EXIT.  trm/trl: lane I holds the packed successor state of I after an MPS /
-// LPS (the LPS one with SWITCH in bit 0); both are read first so their
-// latency overlaps the interval arithmetic.  CT is reduced first and restored
-// on the refill path; the codeword dword load waits for itself.
 // RENORMD across a byte boundary (CT < n; CT already reduced by n, so it
 // holds CT - n), common case in one straight run: the byte at BP is in the
 // cached dword, the byte before it is not 0xFF and one byte covers the rest
 // of the shift (n - CT <= 8).  BYTEIN's C += B << 8 lands before the last
 // n - CT shifts, so C = (C << n) + (B << (8 + n - CT)) and CT = 8 - (n - CT).
 // Anything else continues at P "slow" (CT restored there).
-#define TC_MQ_REFILL(P, EXIT)                        \
+#define TC_MQ_REFILL(P, TAIL, EXIT)                  \
     "L%=" P "refill:\n\t"                             \
     "s_cmp_ge_u32 %[np], %[len]\n\t"                  \
     "s_cbranch_scc1 L%=" P "slow\n\t"                 \
@@ -317,9 +130,19 @@ __device__ __forceinline__ void mq_mrp_stripe(Mq& m, uint32_t& cs, uint32_t& cl,
     "s_add_u32 %[C], %[C], %[t0]\n\t"                 \
     "s_lshl_b32 %[A], %[A], %[n]\n\t"                 \
     "s_add_u32 %[np], %[np], 1\n\t"                   \
-    "s_branch " EXIT "\n"
+    TAIL "s_branch " EXIT "\n"
 
-#define TC_MQ_BODY(P, ST, EXIT)                      \
+// DECODE (C.3.2) with the conditional exchanges (C.3.3), the successor state
+// and RENORMD/BYTEIN (C.3.3/C.3.4), RENORMD shifting min(needed, CT) bits per
+// step (the same bits as one at a time), over the operands of the asm block
+// that uses it: label prefix P, context state operand ST (Qe << 16 | I << 1 |
+// MPS, updated in place), the decision in d.  A decision without
+// renormalisation branches to FAST; every other path runs TAIL and branches
+// to EXIT.  trm/trl: lane I holds the packed successor state of I after an MPS /
+// LPS (the LPS one with SWITCH in bit 0); both are read first so their
+// latency overlaps the interval arithmetic.  CT is reduced first and restored
+// on the refill path; the codeword dword load waits for itself.
+#define TC_MQ_DECIDE(P, ST, FAST, TAIL, EXIT)                \
     "s_bfe_u32 %[t0], %[" ST "], 0x60001\n\t"             \
     "s_and_b32 %[q], %[" ST "], 0xffff0000\n\t"           \
     "s_and_b32 %[d], %[" ST "], 1\n\t"                    \
@@ -330,7 +153,7 @@ __device__ __forceinline__ void mq_mrp_stripe(Mq& m, uint32_t& cs, uint32_t& cl,
     "s_cbranch_scc1 L%=" P "lower\n\t"                \
     "s_sub_u32 %[C], %[C], %[q]\n\t"                  \
     "s_cmp_lt_i32 %[A], 0\n\t"                        \
-    "s_cbranch_scc1 " EXIT "\n\t"                     \
+    "s_cbranch_scc1 " FAST "\n\t"                     \
     "s_cmp_lt_u32 %[A], %[q]\n\t"                     \
     "s_cselect_b32 %[t2], %[t3], %[t2]\n\t"           \
     "s_cselect_b32 %[t1], 1, 0\n"                     \
@@ -342,7 +165,7 @@ __device__ __forceinline__ void mq_mrp_stripe(Mq& m, uint32_t& cs, uint32_t& cl,
     "s_cbranch_scc1 L%=" P "refill\n\t"               \
     "s_lshl_b32 %[C], %[C], %[n]\n\t"                 \
     "s_lshl_b32 %[A], %[A], %[n]\n\t"                 \
-    "s_branch " EXIT "\n"                             \
+    TAIL "s_branch " EXIT "\n"                             \
     "L%=" P "lower:\n\t"                              \
     "s_cmp_lt_u32 %[A], %[q]\n\t"                     \
     "s_cselect_b32 %[t2], %[t2], %[t3]\n\t"           \
@@ -355,8 +178,8 @@ __device__ __forceinline__ void mq_mrp_stripe(Mq& m, uint32_t& cs, uint32_t& cl,
     "s_cbranch_scc1 L%=" P "refill\n\t"               \
     "s_lshl_b32 %[C], %[C], %[n]\n\t"                 \
     "s_lshl_b32 %[A], %[A], %[n]\n\t"                 \
-    "s_branch " EXIT "\n"                             \
-    TC_MQ_REFILL(P, EXIT)                             \
+    TAIL "s_branch " EXIT "\n"                             \
+    TC_MQ_REFILL(P, TAIL, EXIT)                             \
     "L%=" P "slow:\n\t"                               \
     "s_add_u32 %[CT], %[CT], %[n]\n"                  \
     "L%=" P "loop:\n\t"                               \
@@ -406,7 +229,98 @@ __device__ __forceinline__ void mq_mrp_stripe(Mq& m, uint32_t& cs, uint32_t& cl,
     "s_sub_u32 %[n], %[n], %[t1]\n\t"                 \
     "s_cmp_eq_u32 %[n], 0\n\t"                        \
     "s_cbranch_scc0 L%=" P "loop\n\t"                 \
-    "s_branch " EXIT "\n"
+    TAIL "s_branch " EXIT "\n"
+#define TC_MQ_BODY(P, ST, EXIT) TC_MQ_DECIDE(P, ST, EXIT, "", EXIT)
+
+// The refinement decisions of one stripe (T.800 D.3.3) in one scalar loop.
+// cols: the columns holding candidates (not empty); lane x of stream: column
+// x's candidate labels in scan order, 4 bits each (label - 14), ended by 15,
+// and, when all of them are the same label c, c | 4 << 24 | (count - 1) << 28.
+// Column x's decisions go to lane x of res, the first one highest.  A label
+// other than the one used last swaps the context states through the lane
+// array ctxv (cl/cs as in Blk; cl is held as label - 14 inside); a column
+// whose candidates all use the context used last runs a counted loop without
+// label tests (after swapping that label in when it is not the one used
+// last).  The next column's label word is read while the current column
+// decodes.
+#define TC_MRP_BIT "s_lshl1_add_u32 %[bits], %[bits], %[d]\n\t"
+#define TC_MRP_COUNT TC_MRP_BIT "s_sub_u32 %[cnt], %[cnt], 1\n\ts_cbranch_scc0 L%=_fast\n\t"
+__device__ __forceinline__ void mq_mrp_stripe(Mq& m, uint32_t& cs, uint32_t& cl, uint32_t& ctxv, uint64_t cols,
+                                              uint32_t stream, uint32_t& res, uint32_t trm, uint32_t trl) {
+    uint32_t x, xn, iw, iwn, bits, code, cnt, d, q, t0, t1, t2, t3, n;
+    asm volatile(
+        "s_sub_u32 %[cl], %[cl], 14\n\t"
+        "s_ff1_i32_b64 %[x], %[cols]\n\t"
+        "v_readlane_b32 %[iw], %[stream], %[x]\n"
+        "L%=_col:\n\t"
+        "s_bitset0_b64 %[cols], %[x]\n\t"
+        "s_ff1_i32_b64 %[xn], %[cols]\n\t"
+        "s_max_i32 %[xn], %[xn], 0\n\t"
+        "s_mov_b32 %[bits], 0\n\t"
+        "v_readlane_b32 %[iwn], %[stream], %[xn]\n\t"
+        "s_bfe_u32 %[t0], %[iw], 0x30018\n\t"
+        "s_or_b32 %[t1], %[cl], 4\n\t"
+        "s_cmp_eq_u32 %[t0], %[t1]\n\t"
+        "s_cbranch_scc1 L%=_cnt\n\t"
+        "s_bitcmp1_b32 %[iw], 26\n\t"
+        "s_cbranch_scc0 L%=_next\n\t"
+        // one label for the whole column, not the one used last: swap it in
+        "s_and_b32 %[t0], %[t0], 3\n\t"
+        "s_add_u32 %[t1], %[cl], 14\n\t"
+        "s_mov_b32 m0, %[t1]\n\t"
+        "v_writelane_b32 %[ctxv], %[cs], m0\n\t"
+        "s_mov_b32 %[cl], %[t0]\n\t"
+        "s_add_u32 %[t1], %[t0], 14\n\t"
+        "v_readlane_b32 %[cs], %[ctxv], %[t1]\n"
+        "L%=_cnt:\n\t"
+        "s_bfe_u32 %[cnt], %[iw], 0x2001c\n"
+        "L%=_fast:\n\t"
+        TC_MQ_DECIDE("f", "cs", "L%=ffx", TC_MRP_COUNT, "L%=_colend")
+        "L%=ffx:\n\t"
+        TC_MRP_COUNT
+        "s_branch L%=_colend\n"
+        "L%=_next:\n\t"
+        "s_and_b32 %[code], %[iw], 15\n\t"
+        "s_lshr_b32 %[iw], %[iw], 4\n\t"
+        "s_cmp_eq_u32 %[code], %[cl]\n\t"
+        "s_cbranch_scc0 L%=_other\n"
+        "L%=_dec:\n\t"
+        TC_MQ_DECIDE("g", "cs", "L%=gfx", TC_MRP_BIT, "L%=_next")
+        "L%=gfx:\n\t"
+        TC_MRP_BIT
+        "s_branch L%=_next\n"
+        // another label, or the column's end
+        "L%=_other:\n\t"
+        "s_cmp_eq_u32 %[code], 15\n\t"
+        "s_cbranch_scc1 L%=_colend\n\t"
+        "s_add_u32 %[t0], %[cl], 14\n\t"
+        "s_mov_b32 m0, %[t0]\n\t"
+        "v_writelane_b32 %[ctxv], %[cs], m0\n\t"
+        "s_add_u32 %[t0], %[code], 14\n\t"
+        "s_mov_b32 %[cl], %[code]\n\t"
+        "v_readlane_b32 %[cs], %[ctxv], %[t0]\n\t"
+        "s_branch L%=_dec\n"
+        "L%=_colend:\n\t"
+        "s_mov_b32 m0, %[x]\n\t"
+        "v_writelane_b32 %[res], %[bits], m0\n\t"
+        "s_cmp_eq_u64 %[cols], 0\n\t"
+        "s_cbranch_scc1 L%=_end\n\t"
+        "s_mov_b32 %[x], %[xn]\n\t"
+        "s_mov_b32 %[iw], %[iwn]\n\t"
+        "s_branch L%=_col\n"
+        // no lane read may still be in flight when the registers are handed back
+        "L%=_end:\n\t"
+        "s_or_b32 %[t2], %[t2], %[t3]\n\t"
+        "s_or_b32 %[t2], %[t2], %[iwn]\n\t"
+        "s_add_u32 %[cl], %[cl], 14"
+        : [A] "+s"(m.A), [C] "+s"(m.C), [CT] "+s"(m.CT), [cs] "+s"(cs), [cl] "+s"(cl), [cur] "+s"(m.cur),
+          [np] "+s"(m.np), [wc] "+s"(m.wc), [wci] "+s"(m.wci), [cols] "+s"(cols), [ctxv] "+v"(ctxv),
+          [res] "+v"(res), [x] "=&s"(x), [xn] "=&s"(xn), [iw] "=&s"(iw), [iwn] "=&s"(iwn), [bits] "=&s"(bits),
+          [code] "=&s"(code), [cnt] "=&s"(cnt), [d] "=&s"(d), [q] "=&s"(q), [t0] "=&s"(t0), [t1] "=&s"(t1),
+          [t2] "=&s"(t2), [t3] "=&s"(t3), [n] "=&s"(n)
+        : [len] "s"(m.len), [wp] "s"(m.words), [stream] "v"(stream), [trm] "v"(trm), [trl] "v"(trl)
+        : "scc", "m0");
+}
 
 // The sign of stripe row r (r3 = 3r), which just became significant (T.800
 // D.3.2): the sign-coding entry (label | XOR bit << 4) of the 8-bit key that
@@ -843,16 +757,22 @@ struct Blk {
         for (int r = 0; r < 4; r++) nz |= (((slice >> (3 * r)) & 0x1EFu) != 0u ? 1u : 0u) << r;
         const uint32_t cand = (int)threadIdx.x < w ? s4 & ~p4 & rowm : 0u;
         // the candidates' labels in scan order, 4 bits each (label - 14: 2 if
-        // refined before, else 1 with a significant neighbour, else 0), 15 ends
-        uint32_t stream = 0;
+        // refined before, else 1 with a significant neighbour, else 0), 15 ends;
+        // one label for all of them also as label | 4 << 24 | (count - 1) << 28
+        uint32_t stream = 0, first = 0;
+        bool same = true;
         int k = 0;
 #pragma unroll
         for (int r = 0; r < 4; r++)
             if ((cand >> r) & 1u) {
-                stream |= (((r4 >> r) & 1u) ? 2u : ((nz >> r) & 1u)) << (4 * k);
+                const uint32_t code = ((r4 >> r) & 1u) ? 2u : ((nz >> r) & 1u);
+                if (k == 0) first = code;
+                same = same && code == first;
+                stream |= code << (4 * k);
                 k++;
             }
         stream |= 15u << (4 * k);
+        if (k > 0 && same) stream |= (first | 4u | ((uint32_t)(k - 1) << 4)) << 24;
         uint64_t cols = __ballot(cand != 0u);
         uint32_t res = 0;  // lane x: column x's decisions, first candidate highest
         if (cols) mq_mrp_stripe(m, cs, cl, ctxv, cols, stream, res, trm, trl);
