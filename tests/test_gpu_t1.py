@@ -98,3 +98,65 @@ def test_t1_kernels_on_random_blocks(kernel, small):
         for x in bufs:
             x.free()
         L.tc_stream_destroy(stream)
+
+
+def _edge_blocks(seed):
+    """Dense and sparse blocks of edge shapes (single rows and columns, heights
+    that are not a multiple of the 4-row stripe) and of up to 19 magnitude
+    bits, full and truncated."""
+    rng = np.random.default_rng(seed)
+    L = synth._enc_lib()
+    out = []
+    shapes = [(1, 1), (1, 64), (64, 1), (2, 3), (3, 2), (5, 7), (64, 3), (3, 64), (63, 61), (64, 64)]
+    for w, h in shapes:
+        for bits, zero_frac, keep in ((19, 0.0, 999), (18, 0.5, 999), (12, 0.9, 999), (19, 0.0, 20), (8, 0.3, 7)):
+            a = rng.integers(-(1 << bits) + 1, 1 << bits, (h, w)).astype(np.int32)
+            a[rng.uniform(size=(h, w)) < zero_frac] = 0
+            orient = int(rng.integers(0, 4))
+            buf = np.zeros(1 << 20, np.uint8)
+            nb = ctypes.c_int(0)
+            n_ = L.tce_t1_encode_truncated(a.ctypes.data, w, h, orient, keep, buf.ctypes.data, buf.size,
+                                           ctypes.byref(nb))
+            if nb.value == 0:
+                continue
+            out.append(dict(w=w, h=h, orient=orient, numbps=nb.value, npasses=min(keep, 3 * nb.value - 2),
+                            data=bytes(buf[:n_]), src=a))
+    return out
+
+
+def test_stripe_kernel_edge_shapes_and_deep_blocks():
+    blocks = _edge_blocks(90)
+    assert max(b["numbps"] for b in blocks) >= 19
+    print(f"stripe edge blocks={len(blocks)}", flush=True)
+    dev = tca.default_device()
+    L = tca.lib()
+    stream = ctypes.c_void_p()
+    tca.check(L.tc_stream_create(dev.handle, ctypes.byref(stream)), "stream")
+    arena = bytearray()
+    descs = np.zeros(len(blocks), tca.CBLK_DESC)
+    pitch = 64
+    for i, b in enumerate(blocks):
+        arena += bytes((-len(arena)) % 4)
+        descs[i] = (len(arena), len(b["data"]), i * pitch * 64, pitch, b["w"], b["h"], b["numbps"], b["npasses"],
+                    1 if b["orient"] == 1 else (2 if b["orient"] == 3 else 0), 0, 1.0)
+        arena += b["data"]
+    arena += bytes((-len(arena)) % 16) + b"\xff" * 64
+    arena = np.frombuffer(bytes(arena), np.uint8)
+    n_out = len(blocks) * pitch * 64
+    bufs = [tca.DeviceBuffer(dev, max(x, 256)) for x in (arena.nbytes, descs.nbytes, n_out * 4)]
+    try:
+        bufs[0].upload(stream, arena)
+        bufs[1].upload(stream, descs)
+        tca.check(L.tc_buf_fill32(stream, bufs[2].handle, 0, 0, n_out), "fill")
+        tca.check(L.tc_launch_t1_decode_stripe(stream, 0, 64, bufs[1].ptr, None, len(blocks), bufs[0].ptr,
+                                               arena.nbytes, bufs[2].ptr), "launch")
+        out = bufs[2].download(stream, np.int32, n_out).reshape(len(blocks), 64, pitch)
+        for i, b in enumerate(blocks):
+            got = out[i, :b["h"], :b["w"]]
+            assert np.array_equal(got, _expected(b)), (i, b["w"], b["h"], b["orient"], b["numbps"], b["npasses"])
+            if b["npasses"] == 3 * b["numbps"] - 2:
+                assert np.array_equal(got, b["src"])
+    finally:
+        for x in bufs:
+            x.free()
+        L.tc_stream_destroy(stream)
