@@ -64,6 +64,15 @@ __device__ __forceinline__ uint64_t uni64(uint64_t v) {
            ((uint64_t)(uint32_t)__builtin_amdgcn_readfirstlane((int)(uint32_t)(v >> 32)) << 32);
 }
 
+// Work-group ids go to the 8 XCDs round robin, and each XCD has its own L2.
+// This bijection of [0, gridDim.x) gives XCD k (ids k, k+8, ...) one
+// contiguous run of work items, so neighbouring tiles, which share halo
+// lines, are fetched through the same L2.
+__device__ __forceinline__ uint32_t xcd_run_index() {
+    const uint32_t n = gridDim.x, b = blockIdx.x, k = b & 7u, q = n >> 3, r = n & 7u;
+    return k * q + (k < r ? k : r) + (b >> 3);
+}
+
 // Source element of interleaved position (u, v) of the level described by
 // (u0,u1,v0,v1): picks LL/HL/LH/HH by parity and returns its element offset.
 struct Src {
@@ -295,8 +304,9 @@ __global__ __launch_bounds__(NT) void idwt_level_kernel(const tc_dwt_job* __rest
     constexpr int H = Lift<IS97>::H;
     constexpr int PITCH = TX + 2 * H + 1;
     __shared__ T tile[(TY + 2 * H) * PITCH];
-    const tc_dwt_job& J = jobs[wg_job[blockIdx.x]];
-    int t = (int)blockIdx.x - J.wg_base;
+    const uint32_t wg = xcd_run_index();
+    const tc_dwt_job& J = jobs[wg_job[wg]];
+    int t = (int)wg - J.wg_base;
     int ox = J.u0 + (t % J.tiles_x) * TX, oy = J.v0 + (t / J.tiles_x) * TY;
     Src s{J.ll_off, J.hl_off, J.lh_off, J.hh_off, J.ll_pitch, J.band_pitch};
     tile_idwt<IS97, TX, TY>(tile, PITCH, base, s, J.u0, J.u1, J.v0, J.v1, ox, oy, false);
@@ -323,8 +333,9 @@ __global__ __launch_bounds__(NT) void idwt_final_kernel(const tc_final_job* __re
     constexpr int PITCH = TX + 2 * H + 1;
     constexpr int TS = (TY + 2 * H) * PITCH;
     __shared__ T tile[NC * TS];
-    const tc_final_job& J = jobs[wg_job[blockIdx.x]];
-    int t = (int)blockIdx.x - J.wg_base;
+    const uint32_t wg = xcd_run_index();
+    const tc_final_job& J = jobs[wg_job[wg]];
+    int t = (int)wg - J.wg_base;
     int ox = J.wx0 + (t % J.tiles_x) * TX, oy = J.wy0 + (t / J.tiles_x) * TY;
 #pragma unroll
     for (int c = 0; c < NC; c++) {
