@@ -52,6 +52,15 @@ __device__ __forceinline__ int pse(int i, int i0, int i1) {
 __device__ __forceinline__ int floor_half(int v) { return v >> 1; }
 __device__ __forceinline__ int ceil_half(int v) { return (v + 1) >> 1; }
 
+// Work-group ids go to the 8 XCDs round robin, and each XCD has its own L2.
+// This bijection of [0, gridDim.x) gives XCD k (ids k, k+8, ...) one
+// contiguous run of work items, so neighbouring tiles, which share halo
+// lines, are fetched through the same L2.
+__device__ __forceinline__ uint32_t xcd_run_index() {
+    const uint32_t n = gridDim.x, b = blockIdx.x, k = b & 7u, q = n >> 3, r = n & 7u;
+    return k * q + (k < r ? k : r) + (b >> 3);
+}
+
 template <bool IS97>
 struct FLift;
 
@@ -235,8 +244,9 @@ __global__ __launch_bounds__(NT) void fdwt_level_kernel(const tc_fdwt_job* __res
     constexpr int W = TX + 2 * H, HH = TY + 2 * H;
     constexpr int PITCH = W + 1;
     __shared__ T tile[HH * PITCH];
-    const tc_fdwt_job& J = jobs[wg_job[blockIdx.x]];
-    const int t = (int)blockIdx.x - J.wg_base;
+    const uint32_t wg = xcd_run_index();
+    const tc_fdwt_job& J = jobs[wg_job[wg]];
+    const int t = (int)wg - J.wg_base;
     const int ox = J.u0 + (t % J.tiles_x) * TX, oy = J.v0 + (t / J.tiles_x) * TY;
     const bool h1 = (J.u1 - J.u0) == 1, v1x = (J.v1 - J.v0) == 1;
     if (ox - H >= J.u0 && ox + TX + H <= J.u1 && oy - H >= J.v0 && oy + TY + H <= J.v1) {
@@ -272,8 +282,9 @@ __global__ __launch_bounds__(NT) void fdwt_first_kernel(const tc_fdwt_first_job*
     constexpr int PITCH = W + 1;
     constexpr int TS = HH * PITCH;
     __shared__ T tile[NC * TS];
-    const tc_fdwt_first_job& J = jobs[wg_job[blockIdx.x]];
-    const int t = (int)blockIdx.x - J.wg_base;
+    const uint32_t wg = xcd_run_index();
+    const tc_fdwt_first_job& J = jobs[wg_job[wg]];
+    const int t = (int)wg - J.wg_base;
     const int ox = J.u0 + (t % J.tiles_x) * TX, oy = J.v0 + (t / J.tiles_x) * TY;
     const bool h1 = (J.u1 - J.u0) == 1, v1x = (J.v1 - J.v0) == 1;
     const bool interior = ox - H >= J.u0 && ox + TX + H <= J.u1 && oy - H >= J.v0 && oy + TY + H <= J.v1;
