@@ -5,21 +5,24 @@
 // the MQ decoder of Annex C (C.3).
 //
 // Mapping.  The MQ decoder and the scan run on the scalar unit: registers,
-// context states of the refinement, run-length and uniform contexts, and the
-// loop state are wave-uniform.  Block state is column-major in vector
-// registers: lane x holds 64-bit masks of column x (bit y = row y) for
-// significance, sign, visited-in-this-plane, refined-before and the current
-// plane's 1-bits.  For each stripe and pass the 64 lanes build, in parallel,
-// a per-column word (the 3-column x 6-row significance window, candidate rows,
-// neighbour signs, refinement labels); the scalar loop then visits only the
-// columns that hold candidates, reading each column word with one v_readlane,
-// carries new significance from column x to x+1 in scalar registers, and
-// leaves its per-column results in one lane write.  After the stripe the lanes
-// fold the results back into the masks.  The zero-coding and sign-coding
-// tables, the 17 zero/sign/refinement context states and the MQ successor
-// table are 64-entry lane arrays.  The MQ decoder's renormalising path is one
-// scalar block that shifts by min(needed, CT) per step (T.800 C.3.3/C.3.4
-// semantics, bit-exact), so the register state never moves between paths.
+// context states in use, the run-length and uniform contexts and the loop
+// state are wave-uniform.  Block state is column-major in vector registers:
+// lane x holds 64-bit masks of column x (bit y = row y) for significance,
+// sign, visited-in-this-plane, refined-before and the current plane's 1-bits.
+// For each stripe and pass the 64 lanes build, in parallel, per-column words
+// (the 3-column x 6-row significance and sign windows, candidate rows,
+// zero-coding or refinement labels); one scalar loop written in assembly
+// then decodes the whole stripe: it visits only the columns that hold
+// candidates (and the column after new significance), reads each column's
+// words with v_readlane ahead of use, carries new significance from column x
+// to x+1 in scalar registers, and leaves its per-column results in one lane
+// write.  After the stripe the lanes fold the results back into the masks.
+// The zero-coding and sign-coding tables, the 17 zero/sign/refinement
+// context states and the MQ successor table are 64-entry lane arrays.  The
+// MQ decoder is one block of scalar instructions (T.800 C.3 semantics,
+// bit-exact) inside each loop, so its registers never move between paths;
+// one wave pays about 6 cycles per scalar instruction and 14-21 per branch,
+// so the loops are laid out for few branches on the common paths.
 #include <hip/hip_runtime.h>
 #include <stdint.h>
 
