@@ -85,9 +85,11 @@ tc_status tc_decoder_destroy(tc_decoder dec);
  * for blocks up to 32x32 (small_blocks = 1) or up to 64x64 (small_blocks = 0).
  * 1 runs one wavefront per block (tc_launch_t1_decode_stripe / _wave), 8..64 one
  * lane per block (tc_launch_t1_decode), TC_BPW_LOCKSTEP 64 blocks per wavefront
- * in lockstep and TC_BPW_LOCKSTEP32 32 (tc_launch_t1_decode_lockstep). */
+ * in lockstep, TC_BPW_LOCKSTEP32 32 and TC_BPW_LOCKSTEP16 16
+ * (tc_launch_t1_decode_lockstep). */
 #define TC_BPW_LOCKSTEP (-64)
 #define TC_BPW_LOCKSTEP32 (-32)
+#define TC_BPW_LOCKSTEP16 (-16)
 tc_status tc_decoder_set_blocks_per_wave(tc_decoder dec, int small_blocks, int bpw);
 
 /* Resolution-reduced decode (applies from the next prepare or decode): the
@@ -122,7 +124,7 @@ tc_status tc_decoder_set_t1_kernel(tc_decoder dec, int kind);
 
 /* Launch configuration of a decoder (applies from the next prepare). */
 typedef struct {
-    int32_t bpw_small, bpw_large; /* per block class: 0 auto, 1, 8..64, TC_BPW_LOCKSTEP or TC_BPW_LOCKSTEP32 */
+    int32_t bpw_small, bpw_large; /* per block class: 0 auto, 1, 8..64, TC_BPW_LOCKSTEP, _LOCKSTEP32 or _LOCKSTEP16 */
     int32_t t1_kernel;            /* one-wave-per-block kernel: 0 stripe, 1 row-mask */
     int32_t dwt_shape;            /* inverse-DWT tile shape, see tc_idwt_tile_shape */
     float t1_ms, dwt_ms;          /* stage times the autotuner measured for this choice */

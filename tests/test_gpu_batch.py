@@ -17,7 +17,7 @@ def _streams():
     return [open(os.path.join(GOLDEN, n), "rb").read() for n in names]
 
 
-@pytest.mark.parametrize("bpw", [0, 1, 8, 16, 32, 64])
+@pytest.mark.parametrize("bpw", [0, 1, 8, 16, 32, 64, -16])
 def test_batch_equals_single(gpu_decoder, bpw):
     streams = _streams()
     print(f"batch of {len(streams)} bpw={bpw}", flush=True)

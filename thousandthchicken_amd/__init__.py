@@ -13,10 +13,11 @@ import numpy as np
 
 _HERE = os.path.dirname(os.path.abspath(__file__))
 LIB_PATH = os.path.join(_HERE, "libtc.so")
-# Packing values: 64 / 32 code-blocks per wavefront decoded in lockstep
-# (TC_BPW_LOCKSTEP, TC_BPW_LOCKSTEP32).
+# Packing values: 64 / 32 / 16 code-blocks per wavefront decoded in lockstep
+# (TC_BPW_LOCKSTEP, TC_BPW_LOCKSTEP32, TC_BPW_LOCKSTEP16).
 BPW_LOCKSTEP = -64
 BPW_LOCKSTEP32 = -32
+BPW_LOCKSTEP16 = -16
 
 _lib = None
 
