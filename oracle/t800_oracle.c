@@ -299,7 +299,10 @@ static void decode_sign(t1_t *t, mqd_t *m, int x, int y) {
     h = h > 0 ? 1 : (h < 0 ? -1 : 0);
     v = v > 0 ? 1 : (v < 0 ? -1 : 0);
     int lab = or_sign_ctx(h, v, &xr);
-    t->sgn[i] = (uint8_t)(mq_decode(m, lab) ^ xr);
+    /* in a raw (BYPASS) segment the sign is the bit itself, without the
+     * sign-context XOR (T.800 D.6) */
+    int bit = mq_decode(m, lab);
+    t->sgn[i] = (uint8_t)(m->raw ? bit : bit ^ xr);
 }
 
 static void become_sig(t1_t *t, mqd_t *m, int x, int y, int p) {

@@ -9,6 +9,16 @@
    2.5.4 (bundled with Pillow 12.2.0) from seeded images, with the source
    image and OpenJPEG's own decode of each, an independent decoder's output
    for the oracle and the GPU path to match.
+3. pkg_*.j2k with pkg_*.npz: codestreams written by this package's encoder
+   with the features OpenJPEG's encoder cannot be asked for through Pillow
+   (code-block styles BYPASS/RESET/TERMALL/CAUSAL/PTERM/SEGSYM alone and
+   combined, PPT and PPM packed packet headers, sub-sampled components),
+   each with OpenJPEG's decode of it ("openjpeg") and the encoder settings
+   ("meta", JSON).  OpenJPEG returns sub-sampled components upsampled to the
+   full grid by repetition (sample (x, y) of component c is component c's
+   sample (x // XRsiz, y // YRsiz)); the layouts chosen avoid the one
+   OpenJPEG treats as sYCC (full-size first component, sub-sampled second and
+   third), which it colour-converts.
 """
 import io
 import json
@@ -101,8 +111,56 @@ def codestreams():
         print(name, len(cs))
 
 
+# name, (h, w, c, bits), seed, synth.encode options
+_PKG_IMG = (64, 72, 3, 8)
+PKG_CASES = []
+for _sty in (1, 2, 4, 8, 16, 32, 1 | 4, 1 | 8, 63):
+    PKG_CASES.append((f"sty{_sty}_ll", _PKG_IMG, 51, dict(levels=3, cb=16, cbsty=_sty)))
+    PKG_CASES.append((f"sty{_sty}_ly", _PKG_IMG, 51, dict(levels=3, cb=16, cbsty=_sty, lossy=True, bpp=12.0)))
+PKG_CASES += [
+    # 16-bit samples: more bit-planes, so most passes of most blocks are raw under BYPASS
+    ("sty1_ll_gray16", (56, 60, 1, 16), 52, dict(levels=3, cb=32, cbsty=1)),
+    ("sty63_ll_gray16", (56, 60, 1, 16), 52, dict(levels=3, cb=32, cbsty=63)),
+    ("ppt_ll", _PKG_IMG, 53, dict(levels=3, cb=16, packed="ppt", packed_max=100)),
+    ("ppt_ly_tiles", _PKG_IMG, 53, dict(levels=3, cb=16, packed="ppt", packed_max=100, tile=(40, 32),
+                                        lossy=True, bpp=2.0)),
+    ("ppm_ll_tiles", _PKG_IMG, 53, dict(levels=3, cb=16, packed="ppm", packed_max=100, tile=(40, 32))),
+    ("ppm_ly", _PKG_IMG, 53, dict(levels=3, cb=16, packed="ppm", packed_max=100, lossy=True, bpp=2.0)),
+    ("ppm_sty63_ly_tiles", _PKG_IMG, 54, dict(levels=3, cb=16, packed="ppm", packed_max=64, tile=(40, 32),
+                                              cbsty=63, lossy=True, bpp=12.0)),
+    ("sub_21_11_12_ll", _PKG_IMG, 55, dict(levels=3, cb=16, subsampling=[(2, 1), (1, 1), (1, 2)])),
+    ("sub_12_21_22_ll_tiles", _PKG_IMG, 55, dict(levels=3, cb=16, subsampling=[(1, 2), (2, 1), (2, 2)],
+                                                 tile=(40, 32))),
+    ("sub_22_ly", _PKG_IMG, 55, dict(levels=3, cb=16, subsampling=[(2, 2)] * 3, lossy=True, bpp=3.0)),
+]
+
+
+def package_streams():
+    from PIL import Image
+    from thousandthchicken_amd import synth
+    for name, (h, w, c, bits), seed, kw in PKG_CASES:
+        a = synth.image(h, w, c, seed, bits)
+        src = a[..., 0] if c == 1 else a
+        cs = synth.encode(src, bits=bits, **kw)
+        dec = np.asarray(Image.open(io.BytesIO(cs)))
+        if not kw.get("lossy"):
+            sub = kw.get("subsampling") or [(1, 1)] * c
+            for k, (dx, dy) in enumerate(sub):
+                plane = dec if c == 1 else dec[..., k]
+                comp = src if c == 1 else src[::dy, ::dx, k]
+                up = comp[np.arange(h)[:, None] // dy, np.arange(w)[None, :] // dx]
+                assert np.array_equal(plane, up), (name, k)
+        meta = dict(kw, shape=[h, w, c], bits=bits, seed=seed)
+        with open(os.path.join(HERE, f"pkg_{name}.j2k"), "wb") as f:
+            f.write(cs)
+        np.savez_compressed(os.path.join(HERE, f"pkg_{name}.npz"), openjpeg=dec, meta=np.array(json.dumps(meta)))
+        print(name, len(cs))
+
+
 if __name__ == "__main__":
     if os.path.isdir(REF):
         with open(os.path.join(HERE, "reference_tables.json"), "w") as f:
             json.dump(tables(), f)
-    codestreams()
+    if "--package-only" not in sys.argv:
+        codestreams()
+    package_streams()

@@ -90,7 +90,8 @@ def test_autotune_keeps_outputs(gpu_decoder):
     before = [dec.fetch(i) for i in range(len(streams))]
     best = dec.autotune(reps=1)
     print("autotune:", best.as_dict(), flush=True)
-    assert best.bpw_small in (1, 8, 16, 32, 64) and best.bpw_large in (1, 8, 16, 32, 64)
+    packings = (1, 8, 16, 32, 64, tca.BPW_LOCKSTEP, tca.BPW_LOCKSTEP32, tca.BPW_LOCKSTEP16)
+    assert best.bpw_small in packings and best.bpw_large in packings
     assert 0 <= best.dwt_shape < 4 and best.t1_ms > 0
     dec.run()
     dec.sync()

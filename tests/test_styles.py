@@ -2,9 +2,11 @@
 TERMALL (4), CAUSAL (8), PTERM (16) and SEGSYM (32) in the oracle and the host
 parser.
 
-No encoder available here writes these styles, so the oracle's styled tier-1 is
-pinned by properties that tie it to the style-free decode, which the reference's
-fixtures pin:
+OpenJPEG's encoder cannot be asked for these styles through Pillow, but its
+decoder reads them: tests/test_openjpeg_pins.py checks the oracle against
+OpenJPEG's decode of styled codestreams written by this package's encoder
+(every style alone and combined, BYPASS raw segments included).  Here the
+styled tier-1 is tied, block by block, to the style-free decode:
 - every full-pass styled block decodes to its source coefficients;
 - BYPASS, RESET, TERMALL, PTERM and SEGSYM change how passes are coded (raw
   bits, context resets, terminated segments), never which bits a pass carries,
@@ -17,9 +19,7 @@ fixtures pin:
   SEGSYM, fails (wrong coefficients or a corrupt-segment error), so the flags
   are not ignored.
 - a BYPASS or TERMALL block decoded as one segment, without its segment
-  lengths, decodes wrongly.
-Parity for the styles is therefore partially pinned: the properties above, not
-a third-party vector."""
+  lengths, decodes wrongly."""
 import ctypes
 
 import numpy as np

@@ -404,7 +404,8 @@ void t1_encode(const int32_t* coef, int w, int h, int pitch, int orient, double 
         V = V > 0 ? 1 : (V < 0 ? -1 : 0);
         int xr;
         int lab = sign_label(H, V, &xr);
-        enc(sgn[i] ^ xr, lab);
+        // a raw (BYPASS) segment carries the sign bit itself: no context, no XOR (T.800 D.6)
+        enc(raw_now ? sgn[i] : (sgn[i] ^ xr), lab);
     };
     // distortion reduction of revealing bit p of magnitude m (midpoint reconstruction)
     auto dsig = [&](uint32_t m, int p) {

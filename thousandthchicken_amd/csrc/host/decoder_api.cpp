@@ -293,22 +293,36 @@ tc_status tc_decoder_autotune(tc_decoder d, int reps, tc_tuning* best) {
         return Status::Ok;
     };
     // start from what the prepared plan uses; a class without blocks is not tuned
-    Tuning cur = d->pipe->tuning();
+    const Tuning start = d->pipe->tuning();
+    Tuning cur = start;
     const bool has[2] = {d->pipe->blocks_per_wave(0) != 0, d->pipe->blocks_per_wave(1) != 0};
+    const bool styled[2] = {d->pipe->styled(0), d->pipe->styled(1)};
     cur.bpw32 = has[0] ? d->pipe->blocks_per_wave(0) : 1;
     cur.bpw64 = has[1] ? d->pipe->blocks_per_wave(1) : 1;
+    // on any failure the batch is prepared again with the tuning it came with,
+    // so the decoder is left as it was (or reports why it cannot be)
+    auto fail = [&](Status s) -> tc_status {
+        const std::string why = err.empty() ? "autotune run failed" : err;
+        d->pipe->set_tuning(start);
+        std::string e2;
+        Status r = reprepare(d, e2);
+        return report(s, r == Status::Ok ? why : why + "; preparing again with the starting tuning failed too: " + e2);
+    };
     float t1 = 0, dw = 0, best_t1, best_dwt;
     Status s = measure(cur, &best_t1, &best_dwt);
-    if (s != Status::Ok) return report(s, err.empty() ? "autotune run failed" : err);
+    if (s != Status::Ok) return fail(s);
     const int cand[] = {1, 8, 16, 32, 64, TC_BPW_LOCKSTEP, TC_BPW_LOCKSTEP32, TC_BPW_LOCKSTEP16};
     for (int cls = 0; cls < 2; cls++) {
         if (!has[cls]) continue;
         for (int b : cand) {
+            // styled code-blocks decode only under a lockstep packing; the planner
+            // would run any other request as lockstep anyway
+            if (styled[cls] && b > 0) continue;
             Tuning t = cur;
             (cls == 0 ? t.bpw32 : t.bpw64) = b;
             if ((cls == 0 ? cur.bpw32 : cur.bpw64) == b) continue;
             s = measure(t, &t1, &dw);
-            if (s != Status::Ok) return report(s, err.empty() ? "autotune run failed" : err);
+            if (s != Status::Ok) return fail(s);
             if (t1 < best_t1) {
                 best_t1 = t1;
                 cur = t;
@@ -322,7 +336,7 @@ tc_status tc_decoder_autotune(tc_decoder d, int reps, tc_tuning* best) {
         Tuning t = cur;
         t.dwt_shape = sh;
         s = measure(t, &t1, &dw);
-        if (s != Status::Ok) return report(s, err.empty() ? "autotune run failed" : err);
+        if (s != Status::Ok) return fail(s);
         if (dw < best_dwt) {
             best_dwt = dw;
             cur = t;
@@ -330,9 +344,12 @@ tc_status tc_decoder_autotune(tc_decoder d, int reps, tc_tuning* best) {
     }
     d->pipe->set_tuning(cur);
     s = reprepare(d, err);
-    if (s != Status::Ok) return report(s, err);
+    if (s != Status::Ok) return fail(s);
     if (best) {
         tc_decoder_get_tuning(d, best);
+        // the packings the plan runs (a class without blocks keeps its setting)
+        if (has[0]) best->bpw_small = d->pipe->blocks_per_wave(0);
+        if (has[1]) best->bpw_large = d->pipe->blocks_per_wave(1);
         best->t1_ms = best_t1;
         best->dwt_ms = best_dwt;
     }
@@ -342,6 +359,8 @@ tc_status tc_decoder_autotune(tc_decoder d, int reps, tc_tuning* best) {
 tc_status tc_decoder_prepare(tc_decoder d, int n, const uint8_t* const* cs, const size_t* lens) {
     if (!d || n < 0 || (n > 0 && (!cs || !lens))) return report(Status::InvalidArgument, "tc_decoder_prepare: args");
     d->pipe->sync();
+    // the old plan goes first: a prepare that fails leaves nothing to run
+    d->pipe->invalidate();
     d->data.clear();
     d->imgs.clear();
     for (int i = 0; i < n; i++)
@@ -391,6 +410,8 @@ tc_status tc_decoder_prepare(tc_decoder d, int n, const uint8_t* const* cs, cons
 
 tc_status tc_decoder_run(tc_decoder d, int timed) {
     if (!d) return report(Status::InvalidArgument, "tc_decoder_run: null");
+    if (!d->pipe->ready())
+        return report(Status::InvalidArgument, "tc_decoder_run: nothing prepared (the last prepare failed or none ran)");
     Status s = d->pipe->run(timed != 0);
     return report(s, s == Status::Ok ? "" : std::string("launch failed: ") + tc_last_error());
 }
@@ -433,6 +454,8 @@ tc_status tc_decoder_capture_graph(tc_decoder d) {
 
 tc_status tc_decoder_launch_graph(tc_decoder d) {
     if (!d) return report(Status::InvalidArgument, "tc_decoder_launch_graph: null");
+    if (!d->pipe->ready())
+        return report(Status::InvalidArgument, "tc_decoder_launch_graph: nothing prepared (the last prepare failed)");
     Status s = d->pipe->launch_graph();
     return report(s, s == Status::Ok ? "" : std::string("graph launch failed: ") + tc_last_error());
 }

@@ -130,12 +130,32 @@ Status Pipeline::ensure_pinned(size_t bytes, std::string& err) {
     return Status::Ok;
 }
 
+void Pipeline::invalidate() {
+    ready_ = false;
+    bpw_[0] = bpw_[1] = 0;
+    styled_[0] = styled_[1] = false;
+    drop_graph();
+    groups_.clear();
+    levels_.clear();
+    finals_.clear();
+    images_.clear();
+}
+
 Status Pipeline::prepare(const std::vector<const Image*>& imgs, std::string& err) {
+    invalidate();
+    Status s = prepare_plan(imgs, err);
+    if (s == Status::Ok)
+        ready_ = true;
+    else
+        invalidate();
+    return s;
+}
+
+Status Pipeline::prepare_plan(const std::vector<const Image*>& imgs, std::string& err) {
     if (!stream_) {
         err = "no stream";
         return Status::Device;
     }
-    drop_graph();
     tc_stream_sync(stream_);
     shape_ = tuning_.dwt_shape;
     const int TX = idwt_tile_w(shape_), TY = idwt_tile_h(shape_);
@@ -226,6 +246,7 @@ Status Pipeline::prepare(const std::vector<const Image*>& imgs, std::string& err
             scratch_total += align_up(tc_t1_lockstep_scratch_bytes(g.small ? 1 : 0, -g.bpw, g.n, g.max_passes), 256);
         }
         bpw_[k] = g.bpw;
+        styled_[k] = style != 0;
         groups_.push_back(g);
     }
 
@@ -437,10 +458,13 @@ Status Pipeline::prepare(const std::vector<const Image*>& imgs, std::string& err
 }
 
 Status Pipeline::run(bool timed) {
+    if (!ready_) return Status::InvalidArgument;
     uint8_t* up = (uint8_t*)tc_buf_device_ptr(upload_);
     void* base = tc_buf_device_ptr(samples_);
     void* out = host_out_ ? host_dev_ : tc_buf_device_ptr(out_);
     if (!up || !base || !out) return Status::InvalidArgument;
+    for (const Group& g : groups_)
+        if (g.bpw < 0 && (!scratch_ || tc_buf_size(scratch_) <= g.scratch_off)) return Status::InvalidArgument;
     timed_ = timed;
     tc_status s = TC_SUCCESS;
     if (timed) tc_event_record(ev_[0], stream_);
@@ -507,6 +531,10 @@ StageTimes Pipeline::times() const {
 }
 
 Status Pipeline::fetch(int i, void* dst, size_t cap, std::string& err) {
+    if (!ready_) {
+        err = "nothing prepared (the last prepare failed or none ran)";
+        return Status::InvalidArgument;
+    }
     if (i < 0 || i >= (int)images_.size()) {
         err = "image index out of range";
         return Status::InvalidArgument;
@@ -532,14 +560,21 @@ Status Pipeline::fetch(int i, void* dst, size_t cap, std::string& err) {
 }
 
 const void* Pipeline::output_ptr(int i) const {
+    if (!ready_ || i < 0 || i >= (int)images_.size()) return nullptr;
     if (host_out_) return (const uint8_t*)host_buf_ + images_[i].off;
     return (const uint8_t*)tc_buf_device_ptr(out_) + images_[i].off;
 }
 
-size_t Pipeline::output_bytes(int i) const { return images_[i].bytes; }
+size_t Pipeline::output_bytes(int i) const {
+    return ready_ && i >= 0 && i < (int)images_.size() ? images_[i].bytes : 0;
+}
 
 Status Pipeline::capture_graph(std::string& err) {
     drop_graph();
+    if (!ready_) {
+        err = "nothing prepared (the last prepare failed or none ran)";
+        return Status::InvalidArgument;
+    }
     if (tc_graph_begin(stream_) != TC_SUCCESS) {
         err = tc_last_error();
         return Status::Device;
@@ -557,7 +592,7 @@ Status Pipeline::capture_graph(std::string& err) {
 }
 
 Status Pipeline::launch_graph() {
-    if (!graph_) return Status::InvalidArgument;
+    if (!ready_ || !graph_) return Status::InvalidArgument;
     return tc_graph_launch(graph_, stream_) == TC_SUCCESS ? Status::Ok : Status::Device;
 }
 

@@ -46,8 +46,13 @@ class Pipeline {
     Pipeline(const Pipeline&) = delete;
     Pipeline& operator=(const Pipeline&) = delete;
 
-    // Builds the work lists for `imgs` and uploads them (stream ordered).
+    // Builds the work lists for `imgs` and uploads them (stream ordered).  A
+    // failed prepare leaves no plan: run, capture_graph, fetch and the output
+    // queries then fail with InvalidArgument until a prepare succeeds.
     Status prepare(const std::vector<const Image*>& imgs, std::string& err);
+    // Drops the current plan (as a failed prepare does).
+    void invalidate();
+    bool ready() const { return ready_; }
     // Enqueues every stage on the resident data.  With events, per-stage
     // times become available after sync().
     Status run(bool timed = false);
@@ -58,7 +63,7 @@ class Pipeline {
     const void* output_ptr(int i) const;
     size_t output_bytes(int i) const;
     int output_sample_bytes(int i) const { return images_[i].bps; }
-    int num_images() const { return (int)images_.size(); }
+    int num_images() const { return ready_ ? (int)images_.size() : 0; }
     StageTimes times() const;
     tc_stream stream() const { return stream_; }
     void set_tuning(const Tuning& t) { tuning_ = t; }
@@ -72,7 +77,10 @@ class Pipeline {
     int64_t sample_count() const { return nsamples_; }
     int64_t t1_sample_count() const { return t1_samples_; }
     int64_t pixel_count() const { return npixels_; }
+    // packing the prepared plan runs for class cls (0: blocks up to 32x32), 0 if empty
     int blocks_per_wave(int cls) const { return bpw_[cls]; }
+    // the class holds code-blocks with a style (only lockstep packings decode them)
+    bool styled(int cls) const { return styled_[cls]; }
     // bytes moved by the HBM-bound stages (algorithmic): dwt levels and final
     int64_t dwt_bytes() const { return dwt_bytes_; }
     int64_t final_bytes() const { return final_bytes_; }
@@ -109,6 +117,7 @@ class Pipeline {
         int bps = 1;
     };
 
+    Status prepare_plan(const std::vector<const Image*>& imgs, std::string& err);
     Status ensure(tc_buf& b, size_t bytes, const char* what, std::string& err);
     Status ensure_pinned(size_t bytes, std::string& err);
 
@@ -129,6 +138,7 @@ class Pipeline {
     int64_t t1_samples_ = 0;
     int64_t ncblk_ = 0, cw_bytes_ = 0, nsamples_ = 0, npixels_ = 0, dwt_bytes_ = 0, final_bytes_ = 0;
     int bpw_[2] = {0, 0};
+    bool styled_[2] = {false, false};
     Tuning tuning_;
     int shape_ = 0;  // tile shape the prepared job lists were built for
     tc_event ev_[5] = {nullptr, nullptr, nullptr, nullptr, nullptr};
@@ -136,6 +146,7 @@ class Pipeline {
     tc_event fork_ = nullptr, join_ = nullptr;
     bool timed_ = false;
     tc_graph graph_ = nullptr;
+    bool ready_ = false;  // a prepare succeeded and nothing has invalidated it since
 };
 
 }  // namespace tc

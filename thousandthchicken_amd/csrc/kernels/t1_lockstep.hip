@@ -324,7 +324,8 @@ __device__ __forceinline__ void stripe_symbols(LdsLock<M>& L, int lane, LaneMq& 
                     }
                 }
             } else if (pend == PEND_SC) {
-                const uint32_t sg = bit ^ sxor;
+                // a raw (BYPASS) segment carries the sign bit itself (T.800 D.6)
+                const uint32_t sg = bit ^ (m.raw ? 0u : sxor);
                 if (sg) lr(RX + r + 1) |= bx;
                 *at = pbit | (sg << 31);  // the word's first bits: the plane was zero here
                 pend = PEND_NONE;

@@ -162,6 +162,7 @@ def encode(img, bits=None, levels=5, cb=64, tile=None, mct=None, lossy=False, bp
     TERMALL 4, CAUSAL 8, PTERM 16, SEGSYM 32).  packed: None, "ppt" or "ppm" -- packet headers in
     the packets, in PPT segments of the tile-part headers or in PPM segments of
     the main header; packed_max caps each segment's payload (0: 65532).
+    cb: code-block side, or (width, height) for non-square blocks.
     subsampling: per component (XRsiz, YRsiz); component c is then coded from
     the samples img[::YRsiz, ::XRsiz, c] (no colour transform)."""
     import ctypes
@@ -184,8 +185,9 @@ def encode(img, bits=None, levels=5, cb=64, tile=None, mct=None, lossy=False, bp
     target = (bpp * w * h / 8.0) if (lossy and bpp) else 0.0
     out = ctypes.c_void_p()
     n = ctypes.c_size_t()
-    cbe = int(cb).bit_length() - 1
-    rc = _enc_lib().tce_encode(a.ctypes.data, w, h, c, bits, levels, cbe, cbe, tw, th, 1 if mct else 0,
+    cbw, cbh = (cb, cb) if np.isscalar(cb) else cb
+    cbew, cbeh = int(cbw).bit_length() - 1, int(cbh).bit_length() - 1
+    rc = _enc_lib().tce_encode(a.ctypes.data, w, h, c, bits, levels, cbew, cbeh, tw, th, 1 if mct else 0,
                                1 if lossy else 0, target, base_step, threads, cbsty,
                                {None: 0, "ppt": 1, "ppm": 2}[packed], packed_max,
                                None if sub is None else sub.ctypes.data, ctypes.byref(out),
