@@ -130,40 +130,11 @@ Status Pipeline::ensure_pinned(size_t bytes, std::string& err) {
     return Status::Ok;
 }
 
-void Pipeline::invalidate() {
-    ready_ = false;
-    bpw_[0] = bpw_[1] = 0;
-    styled_[0] = styled_[1] = false;
-    drop_graph();
-    groups_.clear();
-    levels_.clear();
-    finals_.clear();
-    images_.clear();
-}
-
-Status Pipeline::prepare(const std::vector<const Image*>& imgs, std::string& err) {
-    invalidate();
-    Status s = prepare_plan(imgs, err);
-    if (s == Status::Ok)
-        ready_ = true;
-    else
-        invalidate();
-    return s;
-}
-
-Status Pipeline::prepare_plan(const std::vector<const Image*>& imgs, std::string& err) {
-    if (!stream_) {
-        err = "no stream";
-        return Status::Device;
-    }
-    tc_stream_sync(stream_);
-    shape_ = tuning_.dwt_shape;
-    const int TX = idwt_tile_w(shape_), TY = idwt_tile_h(shape_);
-    groups_.clear();
-    levels_.clear();
-    finals_.clear();
-    images_.clear();
-    ncblk_ = cw_bytes_ = nsamples_ = npixels_ = dwt_bytes_ = final_bytes_ = 0;
+Status build_plan(const std::vector<const Image*>& imgs, const Tuning& tuning, int cus, Plan& P, std::string& err) {
+    P = Plan();
+    P.shape = tuning.dwt_shape;
+    P.t1_kernel = tuning.t1_kernel;
+    const int TX = idwt_tile_w(P.shape), TY = idwt_tile_h(P.shape);
 
     // ---- sample arena layout: coefficient planes, then work buffers ----
     struct TCPos {
@@ -196,7 +167,7 @@ Status Pipeline::prepare_plan(const std::vector<const Image*>& imgs, std::string
                 }
             }
     }
-    coef_elems_ = coef;
+    P.coef_elems = coef;
 
     // ---- codewords and tier-1 descriptors ----
     uint64_t cw_end = 0;
@@ -212,25 +183,24 @@ Status Pipeline::prepare_plan(const std::vector<const Image*>& imgs, std::string
             if (st != Status::Ok) return st;
         }
     }
-    cw_bytes_ = (int64_t)cw_end;
-    cw_size_ = ((cw_end + 15) & ~(uint64_t)15) + 64;  // slack of 0xFF for look-ahead reads
-    ncblk_ = (int64_t)descs.size();
-    t1_samples_ = 0;
-    for (const auto& d : descs) t1_samples_ += (int64_t)d.w * d.h;
+    P.cw_bytes = (int64_t)cw_end;
+    P.cw_size = ((cw_end + 15) & ~(uint64_t)15) + 64;  // slack of 0xFF for look-ahead reads
+    P.ncblk = (int64_t)descs.size();
+    P.t1_samples = 0;
+    for (const auto& d : descs) P.t1_samples += (int64_t)d.w * d.h;
 
     Blob blob;
-    blob.base = align_up(cw_size_, 256);
-    descs_off_ = put(blob, descs);
-    int cus = device_cu_count(dev_);
+    blob.base = align_up(P.cw_size, 256);
+    P.descs_off = put(blob, descs);
     uint64_t scratch_total = 0;
     for (int k = 0; k < 2; k++) {
         auto& v = cls_idx[k];
-        bpw_[k] = 0;
+        P.bpw[k] = 0;
         if (v.empty()) continue;
         sort_for_packing(v, descs);
-        Group g;
+        Plan::Group g;
         g.small = k == 0;
-        g.bpw = pick_blocks_per_wave((int)v.size(), g.small, cus, tuning_);
+        g.bpw = pick_blocks_per_wave((int)v.size(), g.small, cus, tuning);
         g.n = (int)v.size();
         int style = 0;
         for (uint32_t i : v) {
@@ -245,9 +215,9 @@ Status Pipeline::prepare_plan(const std::vector<const Image*>& imgs, std::string
             g.scratch_off = scratch_total;
             scratch_total += align_up(tc_t1_lockstep_scratch_bytes(g.small ? 1 : 0, -g.bpw, g.n, g.max_passes), 256);
         }
-        bpw_[k] = g.bpw;
-        styled_[k] = style != 0;
-        groups_.push_back(g);
+        P.bpw[k] = g.bpw;
+        P.styled[k] = style != 0;
+        P.groups.push_back(g);
     }
 
     // ---- inverse DWT levels ----
@@ -267,25 +237,25 @@ Status Pipeline::prepare_plan(const std::vector<const Image*>& imgs, std::string
                     if (n < 2) continue;
                     const Resolution& R = tc.res[tc.nlev - n + 1];
                     if (R.width() <= 0 || R.height() <= 0) continue;
-                    const TCPos& P = pos[i][t][c];
-                    tc_dwt_job j = level_job(tc, n, P.plane, P.work, false, 0, 0, TX, TY);
+                    const TCPos& Q = pos[i][t][c];
+                    tc_dwt_job j = level_job(tc, n, Q.plane, Q.work, false, 0, 0, TX, TY);
                     int irr = tc.reversible ? 0 : 1;
                     int tiles_y = (R.height() + TY - 1) / TY;
                     j.wg_base = (int32_t)maps[irr].size();
                     uint32_t jid = (uint32_t)jobs[irr].size();
                     for (int q = 0; q < j.tiles_x * tiles_y; q++) maps[irr].push_back(jid);
                     jobs[irr].push_back(j);
-                    dwt_bytes_ += (int64_t)R.width() * R.height() * 8;
+                    P.dwt_bytes += (int64_t)R.width() * R.height() * 8;
                 }
         }
         for (int irr = 0; irr < 2; irr++) {
             if (jobs[irr].empty()) continue;
-            LevelLaunch L;
+            Plan::LevelLaunch L;
             L.irr = irr != 0;
             L.jobs_off = put(blob, jobs[irr]);
             L.map_off = put(blob, maps[irr]);
             L.nwg = (int)maps[irr].size();
-            levels_.push_back(L);
+            P.levels.push_back(L);
         }
     }
 
@@ -299,16 +269,16 @@ Status Pipeline::prepare_plan(const std::vector<const Image*>& imgs, std::string
     auto key_index = [](int irr, int nc, int ob) { return irr * 4 + (nc == 3 ? 2 : 0) + (ob == 2 ? 1 : 0); };
     for (size_t i = 0; i < imgs.size(); i++) {
         const Image& im = *imgs[i];
-        ImageOut io;
+        Plan::ImageOut io;
         io.bps = im.max_prec() > 8 ? 2 : 1;
         io.off = align_up(out_bytes_total, 256);
         const int ow = im.out_width(), oh = im.out_height();
         const bool planar = im.planar();
         io.bytes = im.out_bytes();
         out_bytes_total = io.off + io.bytes;
-        images_.push_back(io);
-        npixels_ += (int64_t)ow * oh;
-        nsamples_ += (int64_t)(io.bytes / io.bps);
+        P.images.push_back(io);
+        P.npixels += (int64_t)ow * oh;
+        P.nsamples += (int64_t)(io.bytes / io.bps);
         uint64_t out_elem_base = io.off / io.bps;
         // planar output (sub-sampled components): each component's plane, in turn
         std::vector<uint64_t> cbase(im.ncomp, 0);
@@ -354,23 +324,23 @@ Status Pipeline::prepare_plan(const std::vector<const Image*>& imgs, std::string
                 for (size_t k = 0; k < set.size(); k++) {
                     int c = set[k];
                     const TileComp& tc = T.comps[c];
-                    const TCPos& P = pos[i][t][c];
+                    const TCPos& Q = pos[i][t][c];
                     if (tc.nlev == 0) {
-                        f.ll_off[k] = f.hl_off[k] = f.lh_off[k] = f.hh_off[k] = P.plane;
+                        f.ll_off[k] = f.hl_off[k] = f.lh_off[k] = f.hh_off[k] = Q.plane;
                         f.ll_pitch[k] = f.band_pitch[k] = (uint32_t)tc.width();
                     } else {
                         const Resolution& Rl = tc.res[tc.nlev - 1];
                         int lw = Rl.width(), lh = Rl.height();
                         if (tc.nlev == 1) {
-                            f.ll_off[k] = P.plane;
+                            f.ll_off[k] = Q.plane;
                             f.ll_pitch[k] = (uint32_t)tc.width();
                         } else {
-                            f.ll_off[k] = P.work[(tc.nlev - 2) % 2];
+                            f.ll_off[k] = Q.work[(tc.nlev - 2) % 2];
                             f.ll_pitch[k] = (uint32_t)lw;
                         }
-                        f.hl_off[k] = P.plane + lw;
-                        f.lh_off[k] = P.plane + (uint64_t)lh * tc.width();
-                        f.hh_off[k] = P.plane + (uint64_t)lh * tc.width() + lw;
+                        f.hl_off[k] = Q.plane + lw;
+                        f.lh_off[k] = Q.plane + (uint64_t)lh * tc.width();
+                        f.hh_off[k] = Q.plane + (uint64_t)lh * tc.width() + lw;
                         f.band_pitch[k] = (uint32_t)tc.width();
                     }
                     int prec = im.prec[c];
@@ -404,7 +374,7 @@ Status Pipeline::prepare_plan(const std::vector<const Image*>& imgs, std::string
                 uint32_t jid = (uint32_t)fjobs[key].size();
                 for (int q = 0; q < f.tiles_x * tiles_y; q++) fmaps[key].push_back(jid);
                 fjobs[key].push_back(f);
-                final_bytes_ += (int64_t)(jx1 - jx0) * (jy1 - jy0) * nc * (4 + io.bps);
+                P.final_bytes += (int64_t)(jx1 - jx0) * (jy1 - jy0) * nc * (4 + io.bps);
             }
         }
     }
@@ -413,110 +383,162 @@ Status Pipeline::prepare_plan(const std::vector<const Image*>& imgs, std::string
             for (int ob : {1, 2}) {
                 int key = key_index(irr, nc, ob);
                 if (fjobs[key].empty()) continue;
-                FinalLaunch F;
+                Plan::FinalLaunch F;
                 F.irr = irr != 0;
                 F.nc = nc;
                 F.ob = ob;
                 F.jobs_off = put(blob, fjobs[key]);
                 F.map_off = put(blob, fmaps[key]);
                 F.nwg = (int)fmaps[key].size();
-                finals_.push_back(F);
+                P.finals.push_back(F);
             }
 
-    // ---- device buffers and upload ----
-    const uint64_t total = blob.base + blob.v.size();
-    Status st = ensure(upload_, total, "descriptors", err);
-    if (st == Status::Ok) st = ensure(samples_, std::max<uint64_t>(work, 1) * 4, "sample planes", err);
-    if (st == Status::Ok && !host_out_) st = ensure(out_, std::max<uint64_t>(out_bytes_total, 1), "output images", err);
-    if (st == Status::Ok && host_out_ && host_cap_ < out_bytes_total) {
+    P.cw_end = cw_end;
+    P.copies = std::move(copies);
+    P.blob_base = blob.base;
+    P.blob = std::move(blob.v);
+    P.sample_elems = work;
+    P.scratch_bytes = scratch_total;
+    P.out_bytes = out_bytes_total;
+    return Status::Ok;
+}
+
+void stage_upload(const Plan& P, uint8_t* stage) {
+    // alignment gaps between codewords are never read as data; the slack after
+    // the last one reads as 0xFF
+    std::memset(stage + P.cw_end, 0xFF, P.blob_base - P.cw_end);
+    gather(stage, P.copies, P.cw_end);
+    if (!P.blob.empty()) std::memcpy(stage + P.blob_base, P.blob.data(), P.blob.size());
+}
+
+tc_status launch_plan(const Plan& P, const PlanRegions& R, int cus, tc_stream stream, tc_stream side, tc_event fork_ev,
+                      tc_event join_ev, tc_event* ev) {
+    uint8_t* up = R.upload;
+    void* base = R.samples;
+    if (!up || !base || !R.out) return TC_ERR_INVALID_ARG;
+    for (const Plan::Group& g : P.groups)
+        if (g.bpw < 0 && (!R.scratch || R.scratch_bytes <= g.scratch_off)) return TC_ERR_INVALID_ARG;
+    tc_status s = TC_SUCCESS;
+    if (ev) tc_event_record(ev[0], stream);
+    s = tc_launch_fill32(stream, base, 0u, P.coef_elems);
+    if (ev) tc_event_record(ev[1], stream);
+    // Tier-1: one launch per block class.  With two classes the second runs on
+    // the side stream, so each launch's tail overlaps the other's work.
+    const bool fork = P.groups.size() == 2 && side && fork_ev && join_ev;
+    if (fork && s == TC_SUCCESS) {
+        s = tc_event_record(fork_ev, stream);
+        if (s == TC_SUCCESS) s = tc_stream_wait_event(side, fork_ev);
+    }
+    for (size_t gi = 0; gi < P.groups.size(); gi++) {
+        if (s != TC_SUCCESS) break;
+        const Plan::Group& g = P.groups[gi];
+        tc_stream st = (fork && gi == 1) ? side : stream;
+        if (g.bpw < 0)
+            s = tc_launch_t1_decode_lockstep(st, g.small ? 1 : 0, -g.bpw, up + P.descs_off,
+                                             (const uint32_t*)(up + g.order_off), g.n, g.max_passes, up, P.cw_size,
+                                             R.scratch + g.scratch_off, R.scratch_bytes - g.scratch_off, base);
+        else if (g.bpw == 1 && P.t1_kernel == 0 && g.max_planes <= TC_T1_STRIPE_MAX_PLANES(g.small))
+            s = tc_launch_t1_decode_stripe(st, g.small ? 1 : 0, std::min(g.n, cus * 32), up + P.descs_off,
+                                           (const uint32_t*)(up + g.order_off), g.n, up, P.cw_size, base);
+        else if (g.bpw == 1)
+            s = tc_launch_t1_decode_wave(st, g.small ? 1 : 0, std::min(g.n, cus * 16), up + P.descs_off,
+                                         (const uint32_t*)(up + g.order_off), g.n, up, P.cw_size, base);
+        else
+            s = tc_launch_t1_decode(st, g.small ? 1 : 0, g.bpw, up + P.descs_off, (const uint32_t*)(up + g.order_off),
+                                    g.n, up, base);
+    }
+    if (fork && s == TC_SUCCESS) {
+        s = tc_event_record(join_ev, side);
+        if (s == TC_SUCCESS) s = tc_stream_wait_event(stream, join_ev);
+    }
+    if (ev) tc_event_record(ev[2], stream);
+    for (const Plan::LevelLaunch& L : P.levels) {
+        if (s != TC_SUCCESS) break;
+        s = tc_launch_idwt_level(stream, L.irr ? 1 : 0, P.shape, up + L.jobs_off, (const uint32_t*)(up + L.map_off),
+                                 L.nwg, base);
+    }
+    if (ev) tc_event_record(ev[3], stream);
+    for (const Plan::FinalLaunch& F : P.finals) {
+        if (s != TC_SUCCESS) break;
+        s = tc_launch_idwt_final(stream, F.irr ? 1 : 0, P.shape, F.nc, F.ob, up + F.jobs_off,
+                                 (const uint32_t*)(up + F.map_off), F.nwg, base, R.out);
+    }
+    if (ev) tc_event_record(ev[4], stream);
+    return s;
+}
+
+// ---- Pipeline: resident buffers around a plan ----
+
+void Pipeline::invalidate() {
+    ready_ = false;
+    drop_graph();
+    plan_ = Plan();
+}
+
+size_t Pipeline::device_bytes_bound(const Plan& P, bool host_out) {
+    auto grown = [](uint64_t bytes) -> size_t { return std::max<size_t>(2 * bytes, 1 << 20); };
+    size_t b = grown(P.upload_bytes()) + grown(std::max<uint64_t>(P.sample_elems, 1) * 4);
+    if (!host_out) b += grown(std::max<uint64_t>(P.out_bytes, 1));
+    if (P.scratch_bytes) b += grown(P.scratch_bytes);
+    return b;
+}
+
+Status Pipeline::prepare(const std::vector<const Image*>& imgs, std::string& err) {
+    invalidate();
+    if (!stream_) {
+        err = "no stream";
+        return Status::Device;
+    }
+    tc_stream_sync(stream_);
+    Status st = build_plan(imgs, tuning_, device_cu_count(dev_), plan_, err);
+    const uint64_t total = plan_.upload_bytes();
+    if (st == Status::Ok) st = ensure(upload_, total, "descriptors", err);
+    if (st == Status::Ok) st = ensure(samples_, std::max<uint64_t>(plan_.sample_elems, 1) * 4, "sample planes", err);
+    if (st == Status::Ok && !host_out_) st = ensure(out_, std::max<uint64_t>(plan_.out_bytes, 1), "output images", err);
+    if (st == Status::Ok && host_out_ && host_cap_ < plan_.out_bytes) {
         tc_stream_sync(stream_);
         if (host_buf_) tc_host_free_pinned(host_buf_);
         host_buf_ = host_dev_ = nullptr;
         host_cap_ = 0;
-        const size_t want = std::max<size_t>(out_bytes_total, 4096);
+        const size_t want = std::max<size_t>(plan_.out_bytes, 4096);
         if (tc_host_alloc_pinned(want, &host_buf_) != TC_SUCCESS || !(host_dev_ = host_device_ptr(host_buf_))) {
             err = std::string("pinned output: ") + tc_last_error();
-            return Status::OutOfMemory;
+            st = Status::OutOfMemory;
+        } else {
+            host_cap_ = want;
         }
-        host_cap_ = want;
     }
-    if (st == Status::Ok && scratch_total) st = ensure(scratch_, scratch_total, "tier-1 state", err);
+    if (st == Status::Ok && plan_.scratch_bytes) st = ensure(scratch_, plan_.scratch_bytes, "tier-1 state", err);
     if (st == Status::Ok) st = ensure_pinned(total, err);
-    if (st != Status::Ok) return st;
-    uint8_t* stage = (uint8_t*)pinned_;
-    // alignment gaps between codewords are never read as data; the slack after
-    // the last one reads as 0xFF
-    std::memset(stage + cw_end, 0xFF, blob.base - cw_end);
-    gather(stage, copies, cw_end);
-    if (!blob.v.empty()) std::memcpy(stage + blob.base, blob.v.data(), blob.v.size());
-    tc_status ts = tc_buf_copy_h2d(stream_, upload_, 0, pinned_, total);
-    if (ts != TC_SUCCESS) {
-        err = tc_last_error();
-        return from_tc(ts);
+    if (st == Status::Ok) {
+        stage_upload(plan_, (uint8_t*)pinned_);
+        tc_status ts = tc_buf_copy_h2d(stream_, upload_, 0, pinned_, total);
+        if (ts != TC_SUCCESS) {
+            err = tc_last_error();
+            st = from_tc(ts);
+        }
     }
+    if (st != Status::Ok) {
+        invalidate();
+        return st;
+    }
+    ready_ = true;
     return Status::Ok;
 }
 
 Status Pipeline::run(bool timed) {
     if (!ready_) return Status::InvalidArgument;
-    uint8_t* up = (uint8_t*)tc_buf_device_ptr(upload_);
-    void* base = tc_buf_device_ptr(samples_);
-    void* out = host_out_ ? host_dev_ : tc_buf_device_ptr(out_);
-    if (!up || !base || !out) return Status::InvalidArgument;
-    for (const Group& g : groups_)
-        if (g.bpw < 0 && (!scratch_ || tc_buf_size(scratch_) <= g.scratch_off)) return Status::InvalidArgument;
+    PlanRegions R;
+    R.upload = (uint8_t*)tc_buf_device_ptr(upload_);
+    R.samples = tc_buf_device_ptr(samples_);
+    R.out = host_out_ ? host_dev_ : tc_buf_device_ptr(out_);
+    R.scratch = (uint8_t*)tc_buf_device_ptr(scratch_);
+    R.scratch_bytes = tc_buf_size(scratch_);
     timed_ = timed;
-    tc_status s = TC_SUCCESS;
-    if (timed) tc_event_record(ev_[0], stream_);
-    s = tc_launch_fill32(stream_, base, 0u, coef_elems_);
-    if (timed) tc_event_record(ev_[1], stream_);
-    // Tier-1: one launch per block class.  With two classes the second runs on
-    // the side stream, so each launch's tail overlaps the other's work.
-    const bool fork = groups_.size() == 2 && side_ && fork_ && join_;
-    if (fork && s == TC_SUCCESS) {
-        s = tc_event_record(fork_, stream_);
-        if (s == TC_SUCCESS) s = tc_stream_wait_event(side_, fork_);
-    }
-    for (size_t gi = 0; gi < groups_.size(); gi++) {
-        if (s != TC_SUCCESS) break;
-        const Group& g = groups_[gi];
-        tc_stream st = (fork && gi == 1) ? side_ : stream_;
-        if (g.bpw < 0)
-            s = tc_launch_t1_decode_lockstep(st, g.small ? 1 : 0, -g.bpw, up + descs_off_,
-                                             (const uint32_t*)(up + g.order_off),
-                                             g.n, g.max_passes, up, cw_size_, (uint8_t*)tc_buf_device_ptr(scratch_) + g.scratch_off,
-                                             tc_buf_size(scratch_) - g.scratch_off, base);
-        else if (g.bpw == 1 && tuning_.t1_kernel == 0 && g.max_planes <= TC_T1_STRIPE_MAX_PLANES(g.small))
-            s = tc_launch_t1_decode_stripe(st, g.small ? 1 : 0, std::min(g.n, device_cu_count(dev_) * 32),
-                                           up + descs_off_, (const uint32_t*)(up + g.order_off), g.n, up, cw_size_,
-                                           base);
-        else if (g.bpw == 1)
-            s = tc_launch_t1_decode_wave(st, g.small ? 1 : 0, std::min(g.n, device_cu_count(dev_) * 16),
-                                         up + descs_off_, (const uint32_t*)(up + g.order_off), g.n, up, cw_size_,
-                                         base);
-        else
-            s = tc_launch_t1_decode(st, g.small ? 1 : 0, g.bpw, up + descs_off_,
-                                    (const uint32_t*)(up + g.order_off), g.n, up, base);
-    }
-    if (fork && s == TC_SUCCESS) {
-        s = tc_event_record(join_, side_);
-        if (s == TC_SUCCESS) s = tc_stream_wait_event(stream_, join_);
-    }
-    if (timed) tc_event_record(ev_[2], stream_);
-    for (const LevelLaunch& L : levels_) {
-        if (s != TC_SUCCESS) break;
-        s = tc_launch_idwt_level(stream_, L.irr ? 1 : 0, shape_, up + L.jobs_off, (const uint32_t*)(up + L.map_off), L.nwg, base);
-    }
-    if (timed) tc_event_record(ev_[3], stream_);
-    for (const FinalLaunch& F : finals_) {
-        if (s != TC_SUCCESS) break;
-        s = tc_launch_idwt_final(stream_, F.irr ? 1 : 0, shape_, F.nc, F.ob, up + F.jobs_off, (const uint32_t*)(up + F.map_off),
-                                 F.nwg, base, out);
-    }
-    if (timed) tc_event_record(ev_[4], stream_);
+    tc_status s = launch_plan(plan_, R, device_cu_count(dev_), stream_, side_, fork_, join_, timed ? ev_ : nullptr);
+    if (s == TC_ERR_INVALID_ARG) return Status::InvalidArgument;
     return s == TC_SUCCESS ? Status::Ok : Status::Device;
 }
-
 Status Pipeline::sync() { return tc_stream_sync(stream_) == TC_SUCCESS ? Status::Ok : Status::Device; }
 
 StageTimes Pipeline::times() const {
@@ -535,11 +557,11 @@ Status Pipeline::fetch(int i, void* dst, size_t cap, std::string& err) {
         err = "nothing prepared (the last prepare failed or none ran)";
         return Status::InvalidArgument;
     }
-    if (i < 0 || i >= (int)images_.size()) {
+    if (i < 0 || i >= (int)plan_.images.size()) {
         err = "image index out of range";
         return Status::InvalidArgument;
     }
-    const ImageOut& o = images_[i];
+    const Plan::ImageOut& o = plan_.images[i];
     if (cap < o.bytes) {
         err = "output buffer too small";
         return Status::Capacity;
@@ -560,13 +582,13 @@ Status Pipeline::fetch(int i, void* dst, size_t cap, std::string& err) {
 }
 
 const void* Pipeline::output_ptr(int i) const {
-    if (!ready_ || i < 0 || i >= (int)images_.size()) return nullptr;
-    if (host_out_) return (const uint8_t*)host_buf_ + images_[i].off;
-    return (const uint8_t*)tc_buf_device_ptr(out_) + images_[i].off;
+    if (!ready_ || i < 0 || i >= (int)plan_.images.size()) return nullptr;
+    if (host_out_) return (const uint8_t*)host_buf_ + plan_.images[i].off;
+    return (const uint8_t*)tc_buf_device_ptr(out_) + plan_.images[i].off;
 }
 
 size_t Pipeline::output_bytes(int i) const {
-    return ready_ && i >= 0 && i < (int)images_.size() ? images_[i].bytes : 0;
+    return ready_ && i >= 0 && i < (int)plan_.images.size() ? plan_.images[i].bytes : 0;
 }
 
 Status Pipeline::capture_graph(std::string& err) {
