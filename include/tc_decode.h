@@ -153,6 +153,27 @@ tc_status tc_decoder_launch_graph(tc_decoder dec);
 tc_status tc_decoder_get_stats(tc_decoder dec, tc_decoder_stats *stats);
 tc_stream tc_decoder_stream(tc_decoder dec);
 
+/* Per-image decode that allocates no memory on the device or pinned host
+ * memory.  No counterpart in the reference: it is for callers that cannot
+ * allocate (serving loops that must not call hipMalloc, harnesses that forbid
+ * it).  Everything the kernels need beyond the codestream and the output --
+ * the uploaded codewords and descriptors, the coefficient planes, the
+ * inverse-DWT work buffers and the tier-1 state -- lives in one static device
+ * array of TC_NOALLOC_SCRATCH_BYTES per GPU, which the library holds from its
+ * first use on that GPU until the process ends.  The decoded image goes to
+ * out_device, device memory of out_cap bytes that the caller owns, in
+ * tc_decoder_decode's layout (info, when not NULL, receives the image's
+ * description).  The call is blocking: the stream has finished the decode when
+ * it returns.  An image whose working memory exceeds the array fails with
+ * TC_ERR_CAPACITY and launches nothing.  Every call on one GPU shares that
+ * GPU's array and leaves its contents behind (each call clears what must
+ * start at zero); calls that overlapped on one GPU would race over it, so the
+ * library serialises them per GPU with a lock held for the whole call.
+ * Host-side parsing and planning use ordinary heap memory. */
+#define TC_NOALLOC_SCRATCH_BYTES ((size_t)256 << 20)
+tc_status tc_decode_noalloc(tc_stream stream, const uint8_t *cs, size_t len, void *out_device, size_t out_cap,
+                            tc_image_info *info);
+
 /* Decodes by running the drop-in stage classes one after the other, as
  * TC/Decoder.cpp:153-180 does (CoefficientCoder::decode_tile,
  * Quantizer::dequantize_tile, DWT::iwt per tile, then Preprocessor), with each

@@ -101,6 +101,7 @@ def lib():
         "tc_decoder_get_stats": (i32, [vp, P(DecoderStats)]),
         "tc_decoder_stream": (vp, [vp]),
         "tc_decode_staged": (i32, [vp, ctypes.c_char_p, sz, vp, sz]),
+        "tc_decode_noalloc": (i32, [vp, ctypes.c_char_p, sz, vp, sz, P(ImageInfo)]),
         "tc_stream_create": (i32, [vp, P(vp)]),
         "tc_server_create": (i32, [vp, vp, P(vp)]),
         "tc_server_destroy": (i32, [vp]),

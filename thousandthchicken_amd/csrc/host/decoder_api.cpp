@@ -1,7 +1,10 @@
 // C ABI of the decode path (include/tc_decode.h).
+#include <hip/hip_runtime.h>
+
 #include <algorithm>
 #include <cstring>
 #include <memory>
+#include <mutex>
 #include <string>
 #include <thread>
 #include <vector>
@@ -89,6 +92,45 @@ uint32_t fnv1a(const uint8_t* p, size_t n, uint32_t h) {
     }
     return h;
 }
+
+// ---- working memory of tc_decode_noalloc ----
+//
+// One static device array serves every no-allocation decode on a GPU.  Host
+// code must not pass the array's name to a kernel or a copy: on the host it is
+// only a stand-in for the symbol.  Its device address, which differs per GPU,
+// comes from hipGetSymbolAddress once per device (with that device current)
+// and is the only handle used afterwards.
+constexpr size_t kNoallocBytes = TC_NOALLOC_SCRATCH_BYTES;
+constexpr int kMaxDevices = 64;
+
+}  // namespace
+
+static __device__ __align__(256) uint8_t scratch[kNoallocBytes];
+
+namespace {
+
+struct NoallocDevice {
+    std::once_flag once;
+    hipError_t err = hipSuccess;
+    uint8_t* ptr = nullptr;  // this GPU's copy of `scratch`
+    std::mutex use;          // held for a whole call: calls share the array
+};
+NoallocDevice g_noalloc[kMaxDevices];
+
+uint64_t align256(uint64_t v) { return (v + 255) / 256 * 256; }
+
+// Makes `ordinal` the current device for the lifetime of the guard.
+struct DeviceGuard {
+    int prev = -1;
+    hipError_t err = hipSuccess;
+    explicit DeviceGuard(int ordinal) {
+        if (hipGetDevice(&prev) != hipSuccess) prev = -1;
+        err = hipSetDevice(ordinal);
+    }
+    ~DeviceGuard() {
+        if (prev >= 0) (void)hipSetDevice(prev);
+    }
+};
 
 }  // namespace
 
@@ -444,6 +486,62 @@ tc_status tc_decoder_decode(tc_decoder d, const uint8_t* cs, size_t len, void* o
     s = tc_decoder_run(d, 0);
     if (s) return s;
     return tc_decoder_fetch(d, 0, out, cap);
+}
+
+tc_status tc_decode_noalloc(tc_stream stream, const uint8_t* cs, size_t len, void* out_device, size_t out_cap,
+                            tc_image_info* info) {
+    if (!stream || !cs || !out_device) return report(Status::InvalidArgument, "tc_decode_noalloc: null argument");
+    tc_device dev = tc_stream_device(stream);
+    const int ord = device_ordinal(dev);
+    if (ord < 0 || ord >= kMaxDevices) return report(Status::InvalidArgument, "tc_decode_noalloc: device ordinal");
+    DeviceGuard guard(ord);
+    if (guard.err != hipSuccess) return hip_fail(guard.err, "hipSetDevice");
+    NoallocDevice& D = g_noalloc[ord];
+    std::call_once(D.once, [&D] { D.err = hipGetSymbolAddress((void**)&D.ptr, HIP_SYMBOL(scratch)); });
+    if (D.err != hipSuccess || !D.ptr)
+        return hip_fail(D.err != hipSuccess ? D.err : hipErrorInvalidSymbol, "hipGetSymbolAddress(scratch)");
+
+    Image im;
+    std::string err;
+    Status s = parse_codestream(cs, len, im, err);
+    if (s == Status::Ok && im.planar() && im.ncomp > 4) {
+        s = Status::Unsupported;
+        err = "sub-sampled images of more than four components are not supported";
+    }
+    if (s != Status::Ok) return report(s, err);
+    Plan P;
+    s = build_plan({&im}, Tuning(), device_cu_count(dev), P, err);
+    if (s != Status::Ok) return report(s, err);
+    // the array holds the upload region, then the sample planes, then the
+    // tier-1 state, each 256-byte aligned
+    const uint64_t smp_off = align256(P.upload_bytes());
+    const uint64_t scr_off = align256(smp_off + std::max<uint64_t>(P.sample_elems, 1) * 4);
+    const uint64_t need = scr_off + P.scratch_bytes;
+    if (need > kNoallocBytes)
+        return report(Status::Capacity, "tc_decode_noalloc: the image needs " + std::to_string(need) +
+                                            " bytes of device working memory; the static array holds " +
+                                            std::to_string(kNoallocBytes));
+    if (out_cap < P.out_bytes)
+        return report(Status::Capacity, "tc_decode_noalloc: output needs " + std::to_string(P.out_bytes) + " bytes");
+    std::vector<uint8_t> stage(P.upload_bytes());
+    stage_upload(P, stage.data());
+    PlanRegions R;
+    R.upload = D.ptr;
+    R.samples = D.ptr + smp_off;
+    R.out = out_device;
+    R.scratch = P.scratch_bytes ? D.ptr + scr_off : nullptr;
+    R.scratch_bytes = P.scratch_bytes;
+    hipStream_t hs = stream_handle(stream);
+    std::lock_guard<std::mutex> lock(D.use);
+    hipError_t e = hipMemcpyAsync(R.upload, stage.data(), stage.size(), hipMemcpyHostToDevice, hs);
+    if (e != hipSuccess) return hip_fail(e, "hipMemcpyAsync (upload)");
+    tc_status ts = launch_plan(P, R, device_cu_count(dev), stream, nullptr, nullptr, nullptr, nullptr);
+    // the staging vector and the array stay in use until the stream drains
+    e = hipStreamSynchronize(hs);
+    if (ts != TC_SUCCESS) return ts == TC_ERR_INVALID_ARG ? report(Status::InvalidArgument, "launch_plan") : ts;
+    if (e != hipSuccess) return hip_fail(e, "hipStreamSynchronize");
+    if (info) fill_info(im, info);
+    return TC_SUCCESS;
 }
 
 tc_status tc_decoder_capture_graph(tc_decoder d) {
