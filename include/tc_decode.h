@@ -40,6 +40,13 @@ typedef struct {
      * components are interleaved per pixel */
     int32_t planar;
     int32_t comp_width[4], comp_height[4];
+    /* 1 when the input was a JP2 file (T.800 Annex I), whose image header the
+     * decoder checked against the codestream; colourspace is its first usable
+     * colour specification: the enumerated colour space (16 sRGB, 17
+     * greyscale, 18 sYCC), -1 for an ICC profile, 0 when there is none (and
+     * for a raw codestream).  The decoder returns the decoded components as
+     * they are: no colour-space conversion is applied. */
+    int32_t jp2, colourspace;
 } tc_image_info;
 
 /* tier-2 view of one code-block, for checking the parser */
@@ -60,7 +67,14 @@ typedef struct {
     int64_t t1_samples; /* coefficients of the code-blocks tier-1 decodes (blocks with coding passes) */
 } tc_decoder_stats;
 
-/* Main-header information (SIZ/COD/QCD) of a raw codestream or JP2 file. */
+/* Main-header information (SIZ/COD/QCD) of a raw codestream or JP2 file.
+ * A JP2 file (signature box first) is read as the reference's box reader
+ * does (TC/boxes.c:142-307): the File Type box must name 'jp2 ', the JP2
+ * Header box must start with an Image Header box and precede the codestream
+ * box.  Beyond the reference, the image header (size, component count, bit
+ * depths, compression type 7) must agree with SIZ (else TC_ERR_GEOMETRY), and
+ * files with palette (pclr) or component-mapping (cmap) boxes fail with
+ * TC_ERR_UNSUPPORTED.  Every decode entry point applies the same checks. */
 tc_status tc_image_probe(const uint8_t *cs, size_t len, tc_image_info *info);
 
 /* Same for the image decoded with its `reduce` finest DWT levels discarded

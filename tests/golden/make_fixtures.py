@@ -19,6 +19,9 @@
    sample (x // XRsiz, y // YRsiz)); the layouts chosen avoid the one
    OpenJPEG treats as sYCC (full-size first component, sub-sampled second and
    third), which it colour-converts.
+4. jp2_*.jp2 with jp2_*.npz: JP2 files (signature, File Type, JP2 Header with
+   Image Header and Colour Specification, Contiguous Codestream boxes) written
+   by OpenJPEG through Pillow, with the source image and OpenJPEG's decode.
 """
 import io
 import json
@@ -157,10 +160,39 @@ def package_streams():
         print(name, len(cs))
 
 
+# JP2 files (T.800 Annex I boxes) written by OpenJPEG through Pillow: name,
+# (h, w, c, bits), seed, Pillow options
+JP2_CASES = [
+    ("rgb_ll", (48, 64, 3, 8), 56, dict(num_resolutions=3, mct=1)),
+    ("gray_ly", (40, 56, 1, 8), 57, dict(num_resolutions=3, irreversible=True, quality_mode="rates",
+                                         quality_layers=[8])),
+    ("gray16_ll", (36, 44, 1, 16), 58, dict(num_resolutions=3)),
+]
+
+
+def jp2_files():
+    from PIL import Image
+    from thousandthchicken_amd import synth
+    for name, (h, w, c, bits), seed, kw in JP2_CASES:
+        a = synth.image(h, w, c, seed, bits)
+        src = a[..., 0] if c == 1 else a
+        b = io.BytesIO()
+        mode = "I;16" if bits == 16 else ("RGB" if c == 3 else "L")
+        Image.fromarray(src, mode).save(b, "JPEG2000", no_jp2=False, **kw)
+        data = b.getvalue()
+        dec = np.asarray(Image.open(io.BytesIO(data)))
+        with open(os.path.join(HERE, f"jp2_{name}.jp2"), "wb") as f:
+            f.write(data)
+        np.savez_compressed(os.path.join(HERE, f"jp2_{name}.npz"), source=src, openjpeg=dec)
+        print(name, len(data))
+
+
 if __name__ == "__main__":
     if os.path.isdir(REF):
         with open(os.path.join(HERE, "reference_tables.json"), "w") as f:
             json.dump(tables(), f)
-    if "--package-only" not in sys.argv:
+    if "--package-only" not in sys.argv and "--jp2-only" not in sys.argv:
         codestreams()
-    package_streams()
+    if "--jp2-only" not in sys.argv:
+        package_streams()
+    jp2_files()

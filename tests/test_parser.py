@@ -34,11 +34,15 @@ def test_probe_equals_oracle_info(name):
 
 def test_jp2_wrapper_is_accepted():
     cs = open(os.path.join(GOLDEN, "openjpeg_ll_rgb64.j2k"), "rb").read()
-    # minimal JP2: signature box, ftyp box, jp2c box
+    # minimal JP2: signature box, ftyp box, jp2h box holding only the image
+    # header (64 x 64, 3 components of 8 bits, compression type 7), jp2c box
     sig = bytes([0, 0, 0, 12]) + b"jP  " + bytes([0x0D, 0x0A, 0x87, 0x0A])
     ftyp = (20).to_bytes(4, "big") + b"ftyp" + b"jp2 " + bytes(4) + b"jp2 "
+    ihdr = (22).to_bytes(4, "big") + b"ihdr" + (64).to_bytes(4, "big") + (64).to_bytes(4, "big") + \
+        bytes([0, 3, 7, 7, 1, 0])
+    jp2h = (8 + len(ihdr)).to_bytes(4, "big") + b"jp2h" + ihdr
     jp2c = (8 + len(cs)).to_bytes(4, "big") + b"jp2c" + cs
-    f = sig + ftyp + jp2c
+    f = sig + ftyp + jp2h + jp2c
     assert [_key(c) for c in tca.list_cblks(f)] == [_key(c) for c in tca.list_cblks(cs)]
 
 

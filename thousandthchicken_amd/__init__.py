@@ -33,7 +33,8 @@ class ImageInfo(ctypes.Structure):
         "x0", "y0", "width", "height", "ncomp", "max_prec", "bytes_per_sample", "reversible",
         "nlevels", "ntx", "nty", "cbw", "cbh", "nlayers", "mct", "progression")] + [
         ("prec", ctypes.c_int32 * 4), ("sgnd", ctypes.c_int32 * 4), ("planar", ctypes.c_int32),
-        ("comp_width", ctypes.c_int32 * 4), ("comp_height", ctypes.c_int32 * 4)]
+        ("comp_width", ctypes.c_int32 * 4), ("comp_height", ctypes.c_int32 * 4),
+        ("jp2", ctypes.c_int32), ("colourspace", ctypes.c_int32)]
 
 
 class CblkInfo(ctypes.Structure):

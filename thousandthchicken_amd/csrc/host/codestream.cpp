@@ -356,36 +356,184 @@ Status read_segment(Reader& r, uint32_t marker, int ncomp, Params& P, std::vecto
     return Status::Ok;
 }
 
-// Locates the codestream inside a JP2 file (signature box, then the 'jp2c'
-// box, as TC/boxes.c:245-307 does for the reference), or returns the input.
-bool locate_codestream(const uint8_t* d, size_t n, const uint8_t** cs, size_t* cl) {
-    static const uint8_t sig[12] = {0, 0, 0, 12, 'j', 'P', ' ', ' ', 0x0D, 0x0A, 0x87, 0x0A};
-    if (n >= 12 && std::memcmp(d, sig, 12) == 0) {
-        size_t pos = 0;
-        while (pos + 8 <= n) {
-            uint64_t L = ((uint64_t)d[pos] << 24) | ((uint64_t)d[pos + 1] << 16) | ((uint64_t)d[pos + 2] << 8) | d[pos + 3];
-            size_t hdr = 8;
-            if (L == 1) {
-                if (pos + 16 > n) return false;
-                L = 0;
-                for (int i = 0; i < 8; i++) L = (L << 8) | d[pos + 8 + i];
-                hdr = 16;
-            } else if (L == 0) {
-                L = n - pos;
-            }
-            if (L < hdr) return false;
-            if (std::memcmp(d + pos + 4, "jp2c", 4) == 0) {
-                *cs = d + pos + hdr;
-                *cl = std::min<uint64_t>(L, n - pos) - hdr;
-                return true;
-            }
-            pos += (size_t)L;
-        }
-        return false;
+// ---- JP2 file format (T.800 Annex I) ----
+//
+// The reference reads the signature, File Type, JP2 Header (Image Header
+// first) and Contiguous Codestream boxes in TC/boxes.c:142-307: it rejects a
+// brand other than 'jp2 ', takes the image size and component count from
+// 'ihdr', and decodes the 'jp2c' payload.  Here the same boxes are read and
+// checked against the codestream: 'ihdr' (and 'bpcc') must agree with SIZ,
+// the compression type must be 7, and the colour specification is reported.
+// Palettes ('pclr') and component mappings ('cmap') would map the decoded
+// components to other channels, which this decoder does not do: such files
+// are rejected rather than returned unmapped.
+
+struct Jp2Header {
+    bool have_ihdr = false;
+    uint32_t height = 0, width = 0;
+    int nc = 0, bpc = 0, compression = 0;
+    std::vector<int> bpcc;  // bits per component box (BPC 255)
+    int colourspace = 0;    // EnumCS of the first usable 'colr', -1 for an ICC profile, 0 none
+};
+
+uint32_t be32(const uint8_t* p) { return ((uint32_t)p[0] << 24) | ((uint32_t)p[1] << 16) | ((uint32_t)p[2] << 8) | p[3]; }
+
+// One box at pos of [0, n): its type, payload [*off, *off + *len) and the
+// position after it.  A length of 0 runs to the end of the file (last box).
+bool next_box(const uint8_t* d, size_t n, size_t pos, uint32_t* type, size_t* off, size_t* len, size_t* end) {
+    if (pos + 8 > n) return false;
+    uint64_t L = be32(d + pos);
+    size_t hdr = 8;
+    if (L == 1) {
+        if (pos + 16 > n) return false;
+        L = ((uint64_t)be32(d + pos + 8) << 32) | be32(d + pos + 12);
+        hdr = 16;
+    } else if (L == 0) {
+        L = n - pos;
     }
-    *cs = d;
-    *cl = n;
+    if (L < hdr) return false;
+    *type = be32(d + pos + 4);
+    *off = pos + hdr;
+    *len = (size_t)std::min<uint64_t>(L, n - pos) - hdr;  // a truncated last box keeps what is there
+    *end = pos + (size_t)std::min<uint64_t>(L, n - pos);
     return true;
+}
+
+constexpr uint32_t box_id(const char (&t)[5]) {
+    return ((uint32_t)(uint8_t)t[0] << 24) | ((uint32_t)(uint8_t)t[1] << 16) | ((uint32_t)(uint8_t)t[2] << 8) |
+           (uint32_t)(uint8_t)t[3];
+}
+
+Status read_jp2_header(const uint8_t* d, size_t n, Jp2Header& H, std::string& err) {
+    size_t pos = 0;
+    bool first = true, have_colr = false;
+    while (pos < n) {
+        uint32_t t;
+        size_t off, len, end;
+        if (!next_box(d, n, pos, &t, &off, &len, &end)) {
+            err = "JP2 header box: malformed sub-box";
+            return Status::BadMarker;
+        }
+        const uint8_t* p = d + off;
+        if (first && t != box_id("ihdr")) {
+            err = "JP2 header box: the image header box must come first";
+            return Status::BadMarker;
+        }
+        first = false;
+        if (t == box_id("ihdr")) {
+            if (len < 14) {
+                err = "JP2 image header box too short";
+                return Status::Truncated;
+            }
+            H.have_ihdr = true;
+            H.height = be32(p);
+            H.width = be32(p + 4);
+            H.nc = (p[8] << 8) | p[9];
+            H.bpc = p[10];
+            H.compression = p[11];
+        } else if (t == box_id("bpcc")) {
+            H.bpcc.assign(p, p + len);
+        } else if (t == box_id("colr")) {
+            // the first colour specification of a method this reader knows is used (I.5.3.3)
+            if (!have_colr && len >= 3 && (p[0] == 1 || p[0] == 2)) {
+                if (p[0] == 1) {
+                    if (len < 7) {
+                        err = "JP2 colour specification box too short";
+                        return Status::Truncated;
+                    }
+                    H.colourspace = (int)be32(p + 3);
+                } else {
+                    H.colourspace = -1;
+                }
+                have_colr = true;
+            }
+        } else if (t == box_id("pclr") || t == box_id("cmap")) {
+            err = std::string("JP2 ") + (t == box_id("pclr") ? "palette (pclr)" : "component mapping (cmap)") +
+                  " boxes are not supported: the decoded components would need mapping to other channels";
+            return Status::Unsupported;
+        }
+        pos = end;
+    }
+    if (!H.have_ihdr) {
+        err = "JP2 header box without an image header box";
+        return Status::BadMarker;
+    }
+    if (H.compression != 7) {
+        err = "JP2 image header: compression type " + std::to_string(H.compression) + " (only 7, T.800, is defined)";
+        return Status::Unsupported;
+    }
+    return Status::Ok;
+}
+
+// Finds the codestream: inside a JP2 file (signature box, File Type box, JP2
+// Header box, then 'jp2c'), or the input itself when it does not start with
+// the JP2 signature.
+Status locate_codestream(const uint8_t* d, size_t n, const uint8_t** cs, size_t* cl, bool* is_jp2, Jp2Header& H,
+                         std::string& err) {
+    static const uint8_t sig[12] = {0, 0, 0, 12, 'j', 'P', ' ', ' ', 0x0D, 0x0A, 0x87, 0x0A};
+    *is_jp2 = n >= 12 && std::memcmp(d, sig, 12) == 0;
+    if (!*is_jp2) {
+        *cs = d;
+        *cl = n;
+        return Status::Ok;
+    }
+    size_t pos = 12;
+    uint32_t t;
+    size_t off, len, end;
+    // File Type box right after the signature (I.5.2): brand or compatibility list 'jp2 '
+    if (!next_box(d, n, pos, &t, &off, &len, &end) || t != box_id("ftyp") || len < 8) {
+        err = "JP2 file: the File Type box must follow the signature box";
+        return Status::BadMarker;
+    }
+    bool jp2 = be32(d + off) == box_id("jp2 ");
+    for (size_t k = 8; k + 4 <= len; k += 4) jp2 = jp2 || be32(d + off + k) == box_id("jp2 ");
+    if (!jp2) {
+        err = "JP2 file: neither the brand nor the compatibility list names 'jp2 '";
+        return Status::Unsupported;
+    }
+    pos = end;
+    bool have_header = false;
+    while (next_box(d, n, pos, &t, &off, &len, &end)) {
+        if (t == box_id("jp2h")) {
+            if (have_header) {
+                err = "JP2 file: more than one JP2 header box";
+                return Status::BadMarker;
+            }
+            Status s = read_jp2_header(d + off, len, H, err);
+            if (s != Status::Ok) return s;
+            have_header = true;
+        } else if (t == box_id("jp2c")) {
+            if (!have_header) {
+                err = "JP2 file: the JP2 header box must precede the codestream box";
+                return Status::BadMarker;
+            }
+            *cs = d + off;
+            *cl = len;
+            return Status::Ok;
+        }
+        pos = end;
+    }
+    err = "JP2 file without a contiguous codestream box";
+    return Status::BadMarker;
+}
+
+// The JP2 image header must describe the codestream's image (I.5.3.1, I.5.3.2).
+Status check_jp2_header(const Jp2Header& H, const Image& img, std::string& err) {
+    auto bits_ok = [&](int bpc, int c) { return (bpc & 0x7F) + 1 == img.prec[c] && (bpc >> 7) == img.sgnd[c]; };
+    std::string what;
+    if ((int64_t)H.width != img.width() || (int64_t)H.height != img.height())
+        what = "size " + std::to_string(H.width) + "x" + std::to_string(H.height) + " against " +
+               std::to_string(img.width()) + "x" + std::to_string(img.height());
+    else if (H.nc != img.ncomp)
+        what = std::to_string(H.nc) + " components against " + std::to_string(img.ncomp);
+    else if (H.bpc == 255 && (int)H.bpcc.size() != img.ncomp)
+        what = "a bits-per-component box of " + std::to_string(H.bpcc.size()) + " entries";
+    else
+        for (int c = 0; c < img.ncomp && what.empty(); c++)
+            if (!bits_ok(H.bpc == 255 ? H.bpcc[c] : H.bpc, c)) what = "bit depth of component " + std::to_string(c);
+    if (what.empty()) return Status::Ok;
+    err = "JP2 image header disagrees with the codestream (SIZ): " + what;
+    return Status::BadGeometry;
 }
 
 Status parse_main_header(Reader& r, Image& img, Params& P, std::string& err) {
@@ -846,17 +994,24 @@ Status read_packets(Image& img, int t, const Params& P, TileState& bstate, const
 
 Status parse_impl(const uint8_t* data, size_t len, Image& img, std::string& err, bool packets) {
     img = Image();
-    const uint8_t* cs;
-    size_t cl;
-    if (!data || !locate_codestream(data, len, &cs, &cl)) {
-        err = "no codestream found";
-        return Status::BadMarker;
+    const uint8_t* cs = nullptr;
+    size_t cl = 0;
+    if (!data) {
+        err = "no codestream";
+        return Status::InvalidArgument;
     }
+    bool is_jp2 = false;
+    Jp2Header jh;
+    Status s = locate_codestream(data, len, &cs, &cl, &is_jp2, jh, err);
+    if (s != Status::Ok) return s;
     img.cs = cs;
     img.cs_len = cl;
+    img.jp2 = is_jp2 ? 1 : 0;
+    img.colourspace = jh.colourspace;
     Reader r(cs, cl);
     Params main;
-    Status s = parse_main_header(r, img, main, err);
+    s = parse_main_header(r, img, main, err);
+    if (s == Status::Ok && is_jp2) s = check_jp2_header(jh, img, err);
     img.wx0 = img.x0;
     img.wy0 = img.y0;
     img.wx1 = img.x1;

@@ -104,6 +104,11 @@ struct Image {
     std::vector<std::vector<uint32_t>> termlens;
     const uint8_t* cs = nullptr;  // codestream (inside a JP2 wrapper if any)
     size_t cs_len = 0;
+    // JP2 file format (T.800 Annex I): 1 when the codestream came in a JP2
+    // file; the colour space of its first usable colour specification box
+    // (enumerated EnumCS: 16 sRGB, 17 greyscale, 18 sYCC; -1 an ICC profile;
+    // 0 none)
+    int jp2 = 0, colourspace = 0;
     // output window on the reference grid: the whole image unless
     // window_image narrowed it
     int wx0 = 0, wy0 = 0, wx1 = 0, wy1 = 0;

@@ -83,6 +83,8 @@ void fill_info(const Image& im, tc_image_info* info) {
         info->comp_height[c] = y1 - y0;
     }
     info->planar = im.planar() ? 1 : 0;
+    info->jp2 = im.jp2;
+    info->colourspace = im.colourspace;
 }
 
 uint32_t fnv1a(const uint8_t* p, size_t n, uint32_t h) {
