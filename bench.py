@@ -1,25 +1,38 @@
 """Decode throughput benchmark (decoded Mpixel/s on one MI355X).
 
-Headline workload: one 2048x2048 3-component 8-bit image from the seeded
-recipe (seed 2), coded lossy with ICT + 5-level 9/7, expounded quantization,
-64x64 code-blocks, one tile, one layer truncated to about 1.0 bpp (the
-package's encoder).  A step decodes it from the device-resident codestream to
-the interleaved 8-bit image in HBM: fill, tier-1, inverse DWT levels, fused
-final level (dequantization, ICT, DC shift, rounding, packing).
+Headline (`value`): the mixed batch of 256 images -- sides log-uniform in
+[256, 2048] per axis, 1 or 3 components, 8-bit, half lossless (RCT + 5/3,
+5 levels) and half lossy (ICT + 9/7 at 0.5-2 bpp), 32x32 or 64x64 code-blocks,
+image seeds 1000-1255, coded by the package's encoder -- decoded with one
+launch per stage over the whole batch and replayed from a captured graph,
+codestreams resident in HBM.  A step decodes the whole batch: fill, tier-1
+(one launch per code-block size class, the two side by side), the inverse-DWT
+levels and the fused final level.  Outputs are checked against per-image
+decode first (lossless identical, lossy within 1 LSB).
 
-Beside the headline line the same JSON object carries (unless --quick):
-  per_image  the same image through the per-image entry (host codestream in,
-             host image out, blocking), one call after the other;
-  batched    the mixed batch of 256 images (and the 3840x2160 12-bit tiled
-             image) decoded with one launch per stage, eager and from a
-             captured graph, against the same images decoded one by one;
-  served     Poisson arrivals (90% mixed-batch images, 10% 7680x4320 16-bit)
-             at several offered loads through the decode server, with p50/p99
-             latency, against serial per-image decode of the same requests.
+Beside it the line carries (unless --quick):
+  per_image  the 512x512 RGB lossless image, the 2048x2048 RGB lossy image and
+             the 7680x4320 16-bit lossless image through the per-image entry
+             (host codestream in, host image out, blocking): Mpixel/s, p50 and
+             p99 latency, and the decode time with the codestream resident;
+  accuracy   the 2048x2048 lossy image and the batch's lossy images against the
+             FP64 oracle: max |gpu - rounded FP64|, and max and RMS error
+             against the unrounded FP64 samples;
+  batched    the 256-image batch eager and from the graph against the same
+             images decoded one by one, and the 3840x2160 12-bit image (twelve
+             1024x1024 tiles) batched against per image;
+  served     Poisson arrivals (90% batch images, 10% 7680x4320 16-bit) at
+             100..5000 req/s, each load for at least 5 s or 3000 requests,
+             offered against achieved rate, p50/p99 latency, against serial
+             per-image decode of the same mix;
+  reduced, forward, region: the other decode forms;
+  cpu_baseline: the C oracle on a bounded sample of the batch's images, on all
+             host cores (one image per thread) and on one core.
 
 Prints one JSON line.
 """
 import argparse
+import concurrent.futures as cf
 import ctypes
 import json
 import os
@@ -38,52 +51,83 @@ def log(msg):
     print(msg, file=sys.stderr, flush=True)
 
 
-def workload(name):
-    from thousandthchicken_amd import workloads
-    if name == "cfg1_lossless_512":
-        return workloads.cfg1(), workloads.source_image("cfg1")
-    if name == "cfg2_lossy_2048":
-        return workloads.cfg2(), None
-    if name == "cfg3_lossless_4k_tiled":
-        return workloads.cfg3(), workloads.source_image("cfg3")
-    raise SystemExit(f"unknown workload {name}")
+def pct(a, q):
+    return round(float(np.percentile(np.asarray(a), q)) * 1e3, 3)
 
 
-def cpu_baseline(cs, pixels, budget_s=20.0):
-    """The oracle (scalar C, one core) decoding the same codestream."""
+def cpu_threads():
+    # the GPU box gives one GPU a share of 16 host cores
+    return max(1, min(16, os.cpu_count() or 1))
+
+
+def cpu_baseline(streams, budget_s=8.0):
+    """The oracle (scalar C) decoding the batch's images in order: on one core
+    until budget_s has passed, then the same images on all host cores (one
+    image per thread; the oracle's C calls release the GIL)."""
     from oracle import oracle
+
+    def px_of(cs):
+        a = oracle.decode(cs)
+        return a.shape[1] * a.shape[2]
     t0 = time.perf_counter()
-    n = 0
-    while True:
-        oracle.decode(cs)
+    n = px = 0
+    while time.perf_counter() - t0 < budget_s and n < len(streams):
+        px += px_of(streams[n])
         n += 1
-        el = time.perf_counter() - t0
-        if el > budget_s / 4 or n >= 3:
-            break
-    return {"value": round(pixels * n / el / 1e6, 3), "unit": "Mpixel/s", "cores": 1, "kind": "port",
-            "sample": f"{n} full decode(s) of the benchmark codestream by the C oracle, {el:.1f} s"}
+    one = px / (time.perf_counter() - t0) / 1e6
+    sample = streams[:n]
+    P = cpu_threads()
+    reps = max(1, int(round(P * 0.75)))  # about budget_s again on P cores
+    work = sample * reps
+    t0 = time.perf_counter()
+    with cf.ThreadPoolExecutor(P) as ex:
+        tot = sum(ex.map(px_of, work))
+    allc = tot / (time.perf_counter() - t0) / 1e6
+    return {"value": round(allc, 3), "unit": "Mpixel/s", "cores": P, "kind": "port",
+            "sample": f"the first {n} images of the 256-image batch, {px / 1e6:.1f} Mpixel, decoded by the C oracle "
+                      f"{reps} times over on {P} threads",
+            "one_core": {"value": round(one, 3), "unit": "Mpixel/s", "cores": 1,
+                         "sample": f"the same {n} images once on one core"}}
 
 
-def check_output(out, cs, src, what):
-    if src is not None:
-        if not np.array_equal(out.reshape(src.shape), src):
-            raise SystemExit(f"{what}: decode is not bit-exact")
-        return
+def lossy_errors(out_hwc, cs):
+    """(max |gpu - rounded FP64|, sum of squared errors, max |error|, samples)
+    of a lossy decode against the FP64 oracle's unrounded samples (clamped)."""
     from oracle import oracle
-    ref = oracle.decode(cs).transpose(1, 2, 0)
-    dmax = int(np.abs(out.astype(np.int32) - ref).max())
-    if dmax > 1:
-        raise SystemExit(f"{what}: decode differs from the oracle by {dmax} LSB")
+    import thousandthchicken_amd as tca
+    ref, ref_f = oracle.decode(cs, want_float=True)
+    ref = ref.transpose(1, 2, 0).astype(np.int32)
+    hi = (1 << tca.probe(cs).max_prec) - 1
+    err = out_hwc.reshape(ref.shape).astype(np.int32) - np.clip(ref_f.transpose(1, 2, 0), 0, hi)
+    return int(np.abs(out_hwc.reshape(ref.shape).astype(np.int32) - ref).max()), float((err * err).sum()), \
+        float(np.abs(err).max()), err.size
 
 
-def stage_times(dec, reps=10):
-    acc = np.zeros(5)
+def error_summary(parts):
+    n = sum(p[3] for p in parts)
+    return {"max_lsb_vs_rounded_fp64": max(p[0] for p in parts),
+            "max_abs_vs_fp64": round(max(p[2] for p in parts), 4),
+            "rms_vs_fp64": round((sum(p[1] for p in parts) / n) ** 0.5, 4), "samples": n}
+
+
+def check_equal_source(out, src, what):
+    if not np.array_equal(out.reshape(src.shape), src):
+        raise SystemExit(f"{what}: decode is not bit-exact")
+
+
+def stage_times(dec, reps=5):
+    keys = ("fill_ms", "t1_ms", "dwt_ms", "final_ms", "total_ms")
+    acc = {k: 0.0 for k in keys}
+    cls = [0.0, 0.0]
     for _ in range(reps):
         dec.run(timed=True)
         dec.sync()
         s = dec.stats()
-        acc += [s.fill_ms, s.t1_ms, s.dwt_ms, s.final_ms, s.total_ms]
-    return acc / reps
+        for k in keys:
+            acc[k] += getattr(s, k) / reps
+        for c in range(2):
+            cls[c] += s.t1_class_ms[c] / reps
+    return acc, cls
 
 
 def timed_runs(dec, steps, graph=False):
@@ -95,19 +139,23 @@ def timed_runs(dec, steps, graph=False):
     return (time.perf_counter() - t0) / steps
 
 
-def t1_kernel_name(stats, args):
-    bpw = stats.bpw_large or stats.bpw_small
+def t1_kernel_name(bpw, small, args):
+    """The tier-1 kernel a block class runs, as rocprof names it."""
     if bpw < 0:
-        return "t1_lockstep_kernel"
+        return "t1_lockstep_kernel<unsigned int, 32>" if small else "t1_lockstep_kernel<unsigned long, 64>"
     if bpw > 1:
         return f"t1_decode_kernel<{bpw}>"
     return "t1_stripe_kernel" if args.t1_kernel == 0 else "t1_wave_kernel"
 
 
+def kernel_tag(name):
+    return name.replace("<unsigned int, 32>", "_32").replace("<unsigned long, 64>", "_64").split("<")[0]
+
+
 def traffic_record(kernel, workload):
-    """HBM bytes per tier-1 launch measured with FETCH_SIZE / WRITE_SIZE
+    """HBM bytes per launch measured with FETCH_SIZE / WRITE_SIZE
     (profiles/traffic_<kernel>_<workload>.json)."""
-    path = os.path.join(ROOT, "profiles", f"traffic_{kernel.split('<')[0]}_{workload}.json")
+    path = os.path.join(ROOT, "profiles", f"traffic_{kernel_tag(kernel)}_{workload}.json")
     try:
         with open(path) as f:
             return json.load(f)
@@ -123,85 +171,29 @@ def tuned_decoder(tca, args):
     return dec
 
 
-def bench_batched(tca, steps, args):
-    from thousandthchicken_amd import workloads
-    res = {}
-    t = time.perf_counter()
-    streams = workloads.cfg4(threads=16)
-    log(f"mixed batch generated in {time.perf_counter() - t:.1f} s")
-    lay = workloads.cfg4_layout()
-    px = 0
-    serial_dec = tca.Decoder()
-    outs = []
-    for cs in streams[:4]:
-        serial_dec.decode(cs)
-    t0 = time.perf_counter()
-    for cs in streams:
-        o = serial_dec.decode(cs)
-        outs.append(o)
-        px += o.shape[0] * o.shape[1]
-    serial_s = time.perf_counter() - t0
-    serial_dec.close()
-    dec = tuned_decoder(tca, args)
-    dec.prepare(streams)
-    t0 = time.perf_counter()
-    for _ in range(3):
-        dec.prepare(streams)
-    prep_ms = (time.perf_counter() - t0) / 3 * 1e3
-    tuning4 = None if args.no_autotune else dec.autotune(reps=1).as_dict()
-    dec.run()
-    dec.sync()
-    lossy_diff = 0
-    for i in range(len(streams)):
-        d = int(np.abs(dec.fetch(i).astype(np.int32) - outs[i]).max())
-        if d > (0 if lay[i]["lossless"] else 1):
-            raise SystemExit(f"mixed batch image {i}: batched differs from per-image by {d}")
-        lossy_diff = max(lossy_diff, d)
-    st = stage_times(dec, 3)
-    eager = timed_runs(dec, steps)
-    dec.capture_graph()
-    dec.launch_graph()
-    graph = timed_runs(dec, steps, graph=True)
-    s = dec.stats()
-    res["cfg4_mixed_256"] = {
-        "images": len(streams), "mpixel": round(px / 1e6, 2), "cblks": s.cblks,
-        "mpixel_s_eager": round(px / eager / 1e6, 1), "mpixel_s_graph": round(px / graph / 1e6, 1),
-        "ms_per_batch": round(graph * 1e3, 3),
-        "prepare_ms": round(prep_ms, 1),
-        "stages_ms": {"fill": round(st[0], 3), "tier1": round(st[1], 3), "idwt_levels": round(st[2], 3),
-                      "final_level": round(st[3], 3)},
-        "per_image_serial_mpixel_s": round(px / serial_s / 1e6, 1),
-        "speedup_vs_per_image": round(serial_s / graph, 2),
-        "blocks_per_wave": {"small": s.bpw_small, "large": s.bpw_large},
-        "autotuned": tuning4,
-        "outputs": "lossless identical to per-image decode",
-        "lossy_max_diff_vs_per_image": lossy_diff}
-    dec.close()
-    # one large tiled image (12-bit, 1024x1024 tiles, 32x32 blocks)
-    cs, src = workload("cfg3_lossless_4k_tiled")
-    dec = tuned_decoder(tca, args)
+def per_image_leg(tca, cs, src, calls, what, resident_steps=10):
+    """Per-image entry (host in, host out), one call after the other: latency
+    percentiles and Mpixel/s; then the resident decode time of the same image."""
+    dec = tca.Decoder()
+    out = dec.decode(cs)
+    if src is not None:
+        check_equal_source(out, src, what)
+    lat = []
+    for _ in range(calls):
+        t = time.perf_counter()
+        dec.decode(cs)
+        lat.append(time.perf_counter() - t)
     dec.prepare([cs])
-    tuning3 = None if args.no_autotune else dec.autotune(reps=2).as_dict()
     dec.run()
-    dec.sync()
-    check_output(dec.fetch(0), cs, src, "cfg3")
-    dec.capture_graph()
-    dec.launch_graph()
-    graph = timed_runs(dec, steps, graph=True)
-    one = tca.Decoder()
-    one.decode(cs)
-    t0 = time.perf_counter()
-    for _ in range(3):
-        one.decode(cs)
-    per = (time.perf_counter() - t0) / 3
-    s = dec.stats()
-    res["cfg3_lossless_4k_tiled"] = {
-        "mpixel_s_graph": round(3840 * 2160 / graph / 1e6, 1), "ms": round(graph * 1e3, 3), "cblks": s.cblks,
-        "per_image_mpixel_s": round(3840 * 2160 / per / 1e6, 1), "bit_exact": True,
-        "autotuned": tuning3}
+    res_s = timed_runs(dec, resident_steps)
     dec.close()
-    one.close()
-    return res
+    i = tca.probe(cs)
+    px = i.width * i.height
+    return {"width": i.width, "height": i.height, "components": i.ncomp, "bits": i.max_prec,
+            "reversible": bool(i.reversible), "calls": calls,
+            "mpixel_s": round(px / float(np.mean(lat)) / 1e6, 2), "p50_ms": pct(lat, 50), "p99_ms": pct(lat, 99),
+            "resident_ms": round(res_s * 1e3, 3), "resident_mpixel_s": round(px / res_s / 1e6, 1),
+            "bit_exact": src is not None}, out
 
 
 def bench_reduced(tca, cs, steps):
@@ -327,89 +319,129 @@ def bench_region(tca, steps):
     return res
 
 
-def bench_served(tca, loads, n_req, args):
-    from thousandthchicken_amd import serve, workloads
-    t = time.perf_counter()
-    small = workloads.cfg4(threads=16)
-    large = [workloads.cfg5_large(5), workloads.cfg5_large(6)]
-    log(f"serving pool generated in {time.perf_counter() - t:.1f} s")
-    order = serve.mix(small, large, n_req, 0.1, seed=11)
-    seq = [small[i] if i >= 0 else large[-1 - i] for i in order]
-    keep = set(range(0, n_req, max(1, n_req // 16)))
-    dec = tca.Decoder()
-    for cs in seq[:3]:
-        dec.decode(cs)
-    ser = serve.serial(dec, seq)
-    ref = {i: dec.decode(seq[i]) for i in keep}
-    dec.close()
-    out = {"requests": n_req, "large_fraction": 0.1, "serial_per_image": ser, "loads": []}
-    # one server for every load, warmed with one untimed pass so that its
-    # resident buffers have reached their working size
-    srv = tca.Server(slots=3, max_batch=256, bpw_small=args.bpw_small, bpw_large=args.bpw_large)
-    serve.run(srv, seq, max(loads), seed=1)
-    for rate in loads:
-        srv.reset_stats()
-        r = serve.run(srv, seq, rate, seed=int(rate), keep=keep)
-        for i, img in r.pop("images").items():
-            d = int(np.abs(img.astype(np.int32) - ref[i]).max())
-            if d > (0 if tca.probe(seq[i]).reversible else 1):
-                raise SystemExit(f"served request {i} differs from its per-image decode by {d}")
-        r["speedup_vs_serial"] = round(r["mpixel_s"] / ser["mpixel_s"], 2)
-        out["loads"].append(r)
-        log(f"served {rate} req/s: {r}")
-    srv.close()
-    out["warm"] = "one untimed pass of the same requests before the timed loads"
-    out["outputs"] = (f"{len(keep)} sampled requests checked against per-image decode "
-                      "(lossless identical, lossy within 1 LSB)")
-    return out
-
-
 def profile_mode(tca, args):
-    """A few steps of the workload plus a device copy of known size, for counters."""
-    if args.workload == "cfg4_mixed_256":
-        from thousandthchicken_amd import workloads
-        streams = workloads.cfg4(threads=16)
-    else:
-        streams = [workload(args.workload)[0]]
+    """A few steps of the workload (under the packing given on the command line,
+    no autotuning, so that the trace holds only the decode's own dispatches)
+    plus one device copy of known size, for scaling the traffic counters."""
+    from thousandthchicken_amd import workloads
+    make = {"cfg1_lossless_512": lambda: [workloads.cfg1()], "cfg2_lossy_2048": lambda: [workloads.cfg2()],
+            "cfg3_lossless_4k_tiled": lambda: [workloads.cfg3()], "cfg4_mixed_256": lambda: workloads.cfg4(threads=16),
+            "cfg5_large_16bit": lambda: [workloads.cfg5_large(5)]}
+    streams = make[args.workload]()
     dec = tuned_decoder(tca, args)
     dec.prepare(streams)
     for _ in range(args.steps):
         dec.run()
     dec.sync()
+    s = dec.stats()
     n = 256 << 20
     L = tca.lib()
     a, b = ctypes.c_void_p(), ctypes.c_void_p()
     tca.check(L.tc_buf_alloc(dec.device.handle, n, ctypes.byref(a)))
     tca.check(L.tc_buf_alloc(dec.device.handle, n, ctypes.byref(b)))
     st = L.tc_decoder_stream(dec.handle)
-    tca.check(L.tc_buf_fill32(st, a, 0, 1, n // 4))
-    for _ in range(3):
-        tca.check(L.tc_buf_copy_d2d(st, b, 0, a, 0, n))
+    # the copied bytes' values do not matter to the counters: no fill kernel
+    # runs here, so the decode's own fill dispatches stay alone in the trace
+    tca.check(L.tc_buf_copy_d2d(st, b, 0, a, 0, n))
     tca.check(L.tc_stream_sync(st))
     L.tc_buf_free(a)
     L.tc_buf_free(b)
-    print(json.dumps({"profile": args.workload, "steps": args.steps, "copy_bytes": n}), flush=True)
+    print(json.dumps({"profile": args.workload, "steps": args.steps, "copy_bytes": n,
+                      "blocks_per_wave": {"small": s.bpw_small, "large": s.bpw_large},
+                      "t1_class_bytes": list(s.t1_class_bytes)}), flush=True)
+
+
+def bench_cfg3(tca, steps, args):
+    """The 3840x2160 12-bit image (twelve 1024x1024 tiles, 32x32 blocks):
+    batched from a graph against the per-image entry; bit-exact first."""
+    from thousandthchicken_amd import workloads
+    cs, src = workloads.cfg3(), workloads.source_image("cfg3")
+    dec = tuned_decoder(tca, args)
+    dec.prepare([cs])
+    tuning = None if args.no_autotune else dec.autotune(reps=2).as_dict()
+    dec.run()
+    dec.sync()
+    check_equal_source(dec.fetch(0), src, "cfg3")
+    dec.capture_graph()
+    dec.launch_graph()
+    graph = timed_runs(dec, steps, graph=True)
+    s = dec.stats()
+    dec.close()
+    one = tca.Decoder()
+    one.decode(cs)
+    t0 = time.perf_counter()
+    for _ in range(3):
+        one.decode(cs)
+    per = (time.perf_counter() - t0) / 3
+    one.close()
+    return {"mpixel_s_graph": round(3840 * 2160 / graph / 1e6, 1), "ms": round(graph * 1e3, 3), "cblks": s.cblks,
+            "per_image_mpixel_s": round(3840 * 2160 / per / 1e6, 1), "bit_exact": True, "autotuned": tuning}
+
+
+def bench_served(tca, small, large, loads, args, serial_requests=300):
+    """Poisson loads through the server.  Each load offers min(5 s x rate,
+    3000) requests, so it lasts at least 5 s or 3000 requests; a load counts as
+    sustained when the achieved rate is within 5% of the offered rate."""
+    from thousandthchicken_amd import serve
+    n_max = max(min(int(5 * r), 3000) for r in loads)
+    order = serve.mix(small, large, n_max, 0.1, seed=11)
+    seq = [small[i] if i >= 0 else large[-1 - i] for i in order]
+    keep = set(range(0, n_max, max(1, n_max // 24)))
+    dec = tca.Decoder()
+    for cs in seq[:3]:
+        dec.decode(cs)
+    ser = serve.serial(dec, seq[:serial_requests])
+    ref = {i: dec.decode(seq[i]) for i in keep}
+    dec.close()
+    out = {"large_fraction": 0.1, "serial_per_image": ser, "loads": []}
+    srv = tca.Server(slots=args.slots, max_batch=256, bpw_small=args.bpw_small, bpw_large=args.bpw_large)
+    serve.run(srv, seq[:300], max(loads), seed=1)  # warm: resident buffers reach their working size
+    best = None
+    for rate in loads:
+        n = min(int(5 * rate), 3000)
+        srv.reset_stats()
+        r = serve.run(srv, seq[:n], rate, seed=int(rate), keep={i for i in keep if i < n})
+        for i, img in r.pop("images").items():
+            d = int(np.abs(img.astype(np.int32) - ref[i].reshape(img.shape).astype(np.int32)).max())
+            if d > (0 if tca.probe(seq[i]).reversible else 1):
+                raise SystemExit(f"served request {i} differs from its per-image decode by {d}")
+        r["speedup_vs_serial"] = round(r["mpixel_s"] / ser["mpixel_s"], 2)
+        r["sustained"] = r["achieved_rps"] >= 0.95 * rate
+        out["loads"].append(r)
+        log(f"served {rate} req/s: {r}")
+        if r["sustained"]:
+            best = r
+    srv.close()
+    out["max_sustained"] = None if best is None else {
+        k: best[k] for k in ("offered_rps", "achieved_rps", "mpixel_s", "p50_ms", "p99_ms", "speedup_vs_serial")}
+    out["peak"] = max(out["loads"], key=lambda r: r["mpixel_s"])["mpixel_s"]
+    out["warm"] = "one untimed pass of 300 requests before the timed loads"
+    out["outputs"] = (f"{len(keep)} sampled requests checked against per-image decode "
+                      "(lossless identical, lossy within 1 LSB)")
+    return out
 
 
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
-    ap.add_argument("--steps", type=int, default=50)
+    ap.add_argument("--steps", type=int, default=20)
     ap.add_argument("--warmup", type=int, default=5)
-    ap.add_argument("--workload", default="cfg2_lossy_2048",
-                    choices=["cfg1_lossless_512", "cfg2_lossy_2048", "cfg3_lossless_4k_tiled", "cfg4_mixed_256"],
-                    help="headline workload (cfg4_mixed_256: --profile only)")
-    ap.add_argument("--quick", action="store_true", help="headline workload only (no batched/served)")
+    ap.add_argument("--workload", default="cfg4_mixed_256",
+                    choices=["cfg1_lossless_512", "cfg2_lossy_2048", "cfg3_lossless_4k_tiled", "cfg4_mixed_256",
+                             "cfg5_large_16bit"],
+                    help="workload of --profile")
+    ap.add_argument("--quick", action="store_true", help="headline only")
     ap.add_argument("--no-served", action="store_true", help="skip the serving runs")
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--profile", action="store_true", help="workload steps and a calibration copy only")
     ap.add_argument("--forward-only", action="store_true",
                     help="only the GPU forward transform (encoder front end) leg")
     ap.add_argument("--region-only", action="store_true", help="only the region-decode leg")
-    ap.add_argument("--loads", default="50,100,1000,5000", help="offered loads (req/s) of the serving run")
-    ap.add_argument("--requests", type=int, default=300, help="requests per serving run")
-    ap.add_argument("--bpw-large", type=int, default=0, help="blocks per wave, blocks up to 64x64 (0 auto, -64/-32 lockstep)")
-    ap.add_argument("--bpw-small", type=int, default=0, help="blocks per wave, blocks up to 32x32 (0 auto, -64/-32 lockstep)")
+    ap.add_argument("--served-only", action="store_true", help="only the serving leg")
+    ap.add_argument("--loads", default="100,250,500,1000,2000,5000", help="offered loads (req/s) of the serving run")
+    ap.add_argument("--slots", type=int, default=3, help="server slots")
+    ap.add_argument("--bpw-large", type=int, default=0, help="blocks per wave, blocks up to 64x64 (0 auto, -64/-32/-16 lockstep)")
+    ap.add_argument("--bpw-small", type=int, default=0, help="blocks per wave, blocks up to 32x32 (0 auto, -64/-32/-16 lockstep)")
     ap.add_argument("--t1-kernel", type=int, default=0, help="one-wave-per-block kernel: 0 stripe-column, 1 row-mask")
     ap.add_argument("--no-autotune", action="store_true", help="keep the default packing and DWT tile shape")
     args = ap.parse_args()
@@ -419,6 +451,7 @@ def main():
         sys.exit(2)
 
     import thousandthchicken_amd as tca
+    from thousandthchicken_amd import workloads
     if args.profile:
         return profile_mode(tca, args)
     if args.forward_only:
@@ -427,102 +460,124 @@ def main():
     if args.region_only:
         print(json.dumps({"region": bench_region(tca, args.steps)}), flush=True)
         return
+    t = time.perf_counter()
+    streams = workloads.cfg4(threads=16)
+    log(f"256-image batch generated in {time.perf_counter() - t:.1f} s")
+    if args.served_only:
+        large = [workloads.cfg5_large(5), workloads.cfg5_large(6)]
+        print(json.dumps({"served": bench_served(tca, streams, large,
+                                                 [float(x) for x in args.loads.split(",") if x], args)}), flush=True)
+        return
+    lay = workloads.cfg4_layout()
 
-    cs, src = workload(args.workload)
-    info = tca.probe(cs)
-    pixels = info.width * info.height
+    # ---- headline: the 256-image batch, one launch per stage, from a graph ----
+    serial_dec = tca.Decoder()
+    for cs in streams[:4]:
+        serial_dec.decode(cs)
+    outs, px = [], 0
+    t0 = time.perf_counter()
+    for cs in streams:
+        o = serial_dec.decode(cs)
+        outs.append(o)
+        px += o.shape[0] * o.shape[1]
+    serial_s = time.perf_counter() - t0
+    serial_dec.close()
     dec = tuned_decoder(tca, args)
-    dec.prepare([cs])
-    tuning = None if args.no_autotune else dec.autotune(reps=2).as_dict()
-    for _ in range(args.warmup):
-        dec.run()
+    dec.prepare(streams)
+    t0 = time.perf_counter()
+    for _ in range(3):
+        dec.prepare(streams)
+    prep_ms = (time.perf_counter() - t0) / 3 * 1e3
+    tuning = None if args.no_autotune else dec.autotune(reps=1).as_dict()
+    dec.run()
     dec.sync()
-    check_output(dec.fetch(0), cs, src, args.workload)
-
-    step = timed_runs(dec, args.steps)
-    ms = step * 1e3
-    st = stage_times(dec)
+    batch_outs = [dec.fetch(i) for i in range(len(streams))]
+    for i in range(len(streams)):
+        d = int(np.abs(batch_outs[i].astype(np.int32) - outs[i].astype(np.int32)).max())
+        if d > (0 if lay[i]["lossless"] else 1):
+            raise SystemExit(f"batch image {i}: batched differs from per-image by {d}")
+    dec.capture_graph()
+    for _ in range(args.warmup):
+        dec.launch_graph()
+    graph = timed_runs(dec, args.steps, graph=True)
+    eager = timed_runs(dec, max(3, args.steps // 4))
+    st, cls_ms = stage_times(dec)
     s = dec.stats()
-    # tier-1 algorithmic bytes per launch: codeword bytes read + 4 B per coefficient written
-    # (the coefficients of the blocks that have coding passes; the rest stay as filled)
-    t1_bytes = s.codeword_bytes + 4 * s.t1_samples
-    t1_gbs = t1_bytes / (st[1] * 1e-3) / 1e9
-    dwt_gbs = (s.dwt_bytes + s.final_bytes) / ((st[2] + st[3]) * 1e-3) / 1e9 if st[2] + st[3] > 0 else 0.0
-    tr = traffic_record(t1_kernel_name(s, args), args.workload)
-    traffic = tr["hbm_bytes_per_launch"] if tr else None
-
-    # plain per-image entry (host in, host out), one call after the other
-    per = tca.Decoder()
-    per.decode(cs)
-    t0 = time.perf_counter()
-    n_per = max(5, args.steps // 5)
-    for _ in range(n_per):
-        per.decode(cs)
-    per_s = (time.perf_counter() - t0) / n_per
-    per.close()
-    # the same with zero-copy output: the final kernel writes pinned host memory
-    # and the caller reads it in place (codestream still uploaded per call)
-    zc = tca.Decoder()
-    zc.set_host_output(True)
-    zc.prepare([cs])
-    zc.run()
-    zc.sync()
-    if not np.array_equal(zc.output_view(0), dec.fetch(0)):
-        raise SystemExit("zero-copy host output differs from the device output")
-    t0 = time.perf_counter()
-    for _ in range(n_per):
-        zc.prepare([cs])
-        zc.run()
-        zc.sync()
-        zc.output_view(0)
-    zc_s = (time.perf_counter() - t0) / n_per
-    zc.close()
-
+    dec.close()
+    # the dominant kernel: the longer of the two tier-1 launches
+    c = 1 if cls_ms[1] >= cls_ms[0] else 0
+    bpw = s.bpw_large if c == 1 else s.bpw_small
+    kname = t1_kernel_name(bpw, c == 0, args)
+    t1_bytes = s.t1_class_bytes[c]
+    t1_gbs = t1_bytes / (cls_ms[c] * 1e-3) / 1e9
+    dwt_gbs = (s.dwt_bytes + s.final_bytes) / ((st["dwt_ms"] + st["final_ms"]) * 1e-3) / 1e9
+    tr = traffic_record(kname, "cfg4_mixed_256")
     line = {
         "metric": "decoded Mpixel/s on 1 MI355X",
-        "value": round(pixels / step / 1e6, 2),
+        "value": round(px / graph / 1e6, 1),
         "unit": "Mpixel/s",
         "n_gpus": 1,
         "steps": args.steps,
         "warmup": args.warmup,
-        "ms_per_step": round(ms, 4),
+        "ms_per_step": round(graph * 1e3, 4),
         "higher_is_better": True,
         "scaling": "weak",
         "vs_baseline": None,
         "dtype": "int32+fp32",
         "data": "synthetic (seeded recipe, coded by the package encoder)",
-        "config": {"workload": args.workload, "width": info.width, "height": info.height,
-                   "components": info.ncomp, "reversible": bool(info.reversible), "levels": info.nlevels,
-                   "codeblock": info.cbw, "codestream_bytes": len(cs),
-                   "bpp": round(len(cs) * 8 / pixels, 3), "inputs": "codestream resident in HBM"},
-        "stages_ms": {"fill": round(st[0], 4), "tier1": round(st[1], 4), "idwt_levels": round(st[2], 4),
-                      "final_level": round(st[3], 4), "total": round(st[4], 4)},
-        "roofline": {"bound": "hbm", "kernel": t1_kernel_name(s, args), "achieved": round(t1_gbs, 2),
-                     "peak": HBM_PEAK_GBS, "unit": "GB/s", "frac": round(t1_gbs / HBM_PEAK_GBS, 5),
-                     "traffic": traffic, "algorithmic_bytes": t1_bytes,
-                     "note": "tier-1 is bound by the serial MQ symbol chain of the heaviest code-block, "
-                             "not by HBM; bytes = codewords + 4 B per coefficient"},
+        "config": {"workload": "cfg4_mixed_256", "images": len(streams), "mpixel": round(px / 1e6, 2),
+                   "codestream_bytes": sum(map(len, streams)), "cblks": s.cblks,
+                   "mix": "sides log-uniform 256-2048, 1 or 3 components, 8-bit, half lossless 5/3, "
+                          "half lossy 9/7 0.5-2 bpp, 32x32 or 64x64 blocks",
+                   "form": "batched, one launch per stage, graph replay", "inputs": "codestreams resident in HBM"},
+        "stages_ms": {"fill": round(st["fill_ms"], 3), "tier1": round(st["t1_ms"], 3),
+                      "tier1_32x32_class": round(cls_ms[0], 3), "tier1_64x64_class": round(cls_ms[1], 3),
+                      "idwt_levels": round(st["dwt_ms"], 3), "final_level": round(st["final_ms"], 3),
+                      "total_eager": round(st["total_ms"], 3)},
+        "roofline": {"bound": "hbm", "kernel": kname, "achieved": round(t1_gbs, 2), "peak": HBM_PEAK_GBS,
+                     "unit": "GB/s", "frac": round(t1_gbs / HBM_PEAK_GBS, 5),
+                     "traffic": tr["hbm_bytes_per_launch"] if tr else None, "algorithmic_bytes": t1_bytes,
+                     "launch_ms": round(cls_ms[c], 3),
+                     "note": "tier-1 is bound by the serial MQ symbol chains of its blocks, not by HBM; bytes = "
+                             "codeword bytes + 4 B per coefficient of the class's blocks, over the launch's own "
+                             "time (HIP events on the stream it runs on)"},
         "roofline_dwt": {"bound": "hbm", "achieved": round(dwt_gbs, 2), "peak": HBM_PEAK_GBS, "unit": "GB/s",
                          "frac": round(dwt_gbs / HBM_PEAK_GBS, 5)},
         "blocks_per_wave": {"small": s.bpw_small, "large": s.bpw_large},
         "autotuned": tuning,
-        "cblks": s.cblks,
-        "per_image": {"mpixel_s": round(pixels / per_s / 1e6, 2), "ms": round(per_s * 1e3, 3),
-                      "note": "host codestream in, host image out, PCIe included"},
-        "per_image_host_output": {"mpixel_s": round(pixels / zc_s / 1e6, 2), "ms": round(zc_s * 1e3, 3),
-                                  "note": "same, the final kernel writing pinned host memory (no output copy)"},
+        "batched": {"cfg4_mixed_256": {
+            "mpixel_s_graph": round(px / graph / 1e6, 1), "mpixel_s_eager": round(px / eager / 1e6, 1),
+            "ms_per_batch": round(graph * 1e3, 3), "prepare_ms": round(prep_ms, 1),
+            "per_image_serial_mpixel_s": round(px / serial_s / 1e6, 1),
+            "speedup_vs_per_image": round(serial_s / graph, 2),
+            "outputs": "lossless identical to per-image decode, lossy within 1 LSB"}},
     }
-    dec.close()
     if not args.quick:
-        line["reduced"] = bench_reduced(tca, cs, max(5, args.steps // 2))
+        lossy = [i for i in range(len(streams)) if not lay[i]["lossless"]]
+        with cf.ThreadPoolExecutor(cpu_threads()) as ex:
+            parts = list(ex.map(lambda i: lossy_errors(batch_outs[i], streams[i]), lossy))
+        acc = {"cfg4_lossy_images": dict(error_summary(parts), images=len(lossy))}
+        del batch_outs, outs
+        per = {}
+        per["cfg1_lossless_512"], _ = per_image_leg(tca, workloads.cfg1(), workloads.source_image("cfg1"), 50, "cfg1")
+        cs2 = workloads.cfg2()
+        per["cfg2_lossy_2048"], out2 = per_image_leg(tca, cs2, None, 30, "cfg2")
+        acc["cfg2_lossy_2048"] = error_summary([lossy_errors(out2, cs2)])
+        if acc["cfg2_lossy_2048"]["max_lsb_vs_rounded_fp64"] > 1:
+            raise SystemExit("cfg2 differs from the FP64 oracle by more than 1 LSB")
+        large = [workloads.cfg5_large(5), workloads.cfg5_large(6)]
+        per["cfg5_large_16bit"], _ = per_image_leg(tca, large[0], workloads.source_image("cfg5_large"), 10,
+                                                   "cfg5 large", resident_steps=5)
+        line["per_image"] = per
+        line["accuracy"] = acc
+        line["batched"]["cfg3_lossless_4k_tiled"] = bench_cfg3(tca, max(3, args.steps // 4), args)
+        line["reduced"] = bench_reduced(tca, cs2, max(5, args.steps // 2))
         line["forward"] = bench_forward(tca, max(5, args.steps // 2))
-        line["region"] = bench_region(tca, max(5, args.steps // 5))
-        line["batched"] = bench_batched(tca, max(3, args.steps // 5), args)
-        loads = [float(x) for x in args.loads.split(",") if x]
+        line["region"] = bench_region(tca, max(5, args.steps // 4))
         if not args.no_served:
-            line["served"] = bench_served(tca, loads, args.requests, args)
+            line["served"] = bench_served(tca, streams, large, [float(x) for x in args.loads.split(",") if x], args)
     if not args.no_cpu_baseline:
-        line["cpu_baseline"] = cpu_baseline(cs, pixels)
+        line["cpu_baseline"] = cpu_baseline(streams)
     print(json.dumps(line), flush=True)
 
 

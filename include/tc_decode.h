@@ -65,6 +65,11 @@ typedef struct {
     int32_t images;
     int64_t device_bytes;
     int64_t t1_samples; /* coefficients of the code-blocks tier-1 decodes (blocks with coding passes) */
+    /* per block class (0: blocks up to 32x32, 1: larger): the tier-1 launch's
+     * time on its own stream in the last timed run, and its algorithmic bytes
+     * (codeword bytes + 4 B per coefficient of the blocks it decodes) */
+    float t1_class_ms[2];
+    int64_t t1_class_bytes[2];
 } tc_decoder_stats;
 
 /* Main-header information (SIZ/COD/QCD) of a raw codestream or JP2 file.

@@ -48,7 +48,8 @@ class DecoderStats(ctypes.Structure):
         (n, ctypes.c_int64) for n in ("cblks", "codeword_bytes", "samples", "pixels", "dwt_bytes",
                                       "final_bytes")] + [
         ("bpw_small", ctypes.c_int32), ("bpw_large", ctypes.c_int32), ("images", ctypes.c_int32),
-        ("device_bytes", ctypes.c_int64), ("t1_samples", ctypes.c_int64)]
+        ("device_bytes", ctypes.c_int64), ("t1_samples", ctypes.c_int64),
+        ("t1_class_ms", ctypes.c_float * 2), ("t1_class_bytes", ctypes.c_int64 * 2)]
 
 
 class DeviceProps(ctypes.Structure):

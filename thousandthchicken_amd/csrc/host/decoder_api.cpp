@@ -580,6 +580,10 @@ tc_status tc_decoder_get_stats(tc_decoder d, tc_decoder_stats* st) {
     st->bpw_large = d->pipe->blocks_per_wave(1);
     st->images = d->pipe->num_images();
     st->device_bytes = (int64_t)d->pipe->device_bytes();
+    for (int c = 0; c < 2; c++) {
+        st->t1_class_ms[c] = t.t1_class_ms[c];
+        st->t1_class_bytes[c] = d->pipe->t1_class_bytes(c);
+    }
     return TC_SUCCESS;
 }
 

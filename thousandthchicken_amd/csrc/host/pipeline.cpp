@@ -188,6 +188,8 @@ Status build_plan(const std::vector<const Image*>& imgs, const Tuning& tuning, i
     P.ncblk = (int64_t)descs.size();
     P.t1_samples = 0;
     for (const auto& d : descs) P.t1_samples += (int64_t)d.w * d.h;
+    for (int k = 0; k < 2; k++)
+        for (uint32_t i : cls_idx[k]) P.t1_class_bytes[k] += (int64_t)descs[i].data_len + 4 * (int64_t)descs[i].w * descs[i].h;
 
     Blob blob;
     blob.base = align_up(P.cw_size, 256);
@@ -433,6 +435,8 @@ tc_status launch_plan(const Plan& P, const PlanRegions& R, int cus, tc_stream st
         if (s != TC_SUCCESS) break;
         const Plan::Group& g = P.groups[gi];
         tc_stream st = (fork && gi == 1) ? side : stream;
+        tc_event* gev = ev ? ev + 5 + 2 * (g.small ? 0 : 1) : nullptr;
+        if (gev) tc_event_record(gev[0], st);
         if (g.bpw < 0)
             s = tc_launch_t1_decode_lockstep(st, g.small ? 1 : 0, -g.bpw, up + P.descs_off,
                                              (const uint32_t*)(up + g.order_off), g.n, g.max_passes, up, P.cw_size,
@@ -446,6 +450,7 @@ tc_status launch_plan(const Plan& P, const PlanRegions& R, int cus, tc_stream st
         else
             s = tc_launch_t1_decode(st, g.small ? 1 : 0, g.bpw, up + P.descs_off, (const uint32_t*)(up + g.order_off),
                                     g.n, up, base);
+        if (gev) tc_event_record(gev[1], st);
     }
     if (fork && s == TC_SUCCESS) {
         s = tc_event_record(join_ev, side);
@@ -549,6 +554,10 @@ StageTimes Pipeline::times() const {
     tc_event_elapsed_ms(ev_[2], ev_[3], &t.dwt_ms);
     tc_event_elapsed_ms(ev_[3], ev_[4], &t.final_ms);
     tc_event_elapsed_ms(ev_[0], ev_[4], &t.total_ms);
+    for (const Plan::Group& g : plan_.groups) {
+        const int c = g.small ? 0 : 1;
+        tc_event_elapsed_ms(ev_[5 + 2 * c], ev_[6 + 2 * c], &t.t1_class_ms[c]);
+    }
     return t;
 }
 

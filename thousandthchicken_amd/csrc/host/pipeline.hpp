@@ -41,6 +41,8 @@ int pick_blocks_per_wave(int nblocks, bool small_blocks, int cu_count, const Tun
 
 struct StageTimes {
     float fill_ms = 0, t1_ms = 0, dwt_ms = 0, final_ms = 0, total_ms = 0;
+    // each tier-1 launch on its own stream, by block class (0: blocks up to 32x32)
+    float t1_class_ms[2] = {0, 0};
 };
 
 // The device work of a batch: launches, their descriptor offsets in the upload
@@ -88,6 +90,9 @@ struct Plan {
     uint64_t scratch_bytes = 0;    // tier-1 state of the lockstep packings
     uint64_t out_bytes = 0;        // all output images
     int bpw[2] = {0, 0};           // packing per class (0: no blocks)
+    // tier-1 algorithmic bytes per class: codeword bytes read + 4 B per
+    // coefficient of the blocks that have coding passes
+    int64_t t1_class_bytes[2] = {0, 0};
     bool styled[2] = {false, false};
     int64_t ncblk = 0, cw_bytes = 0, nsamples = 0, npixels = 0, dwt_bytes = 0, final_bytes = 0, t1_samples = 0;
 
@@ -110,8 +115,9 @@ struct PlanRegions {
 };
 
 // Enqueues the plan on `stream`.  With a side stream and fork/join events the
-// second tier-1 class runs beside the first; with ev (five events) the stage
-// boundaries are recorded on `stream`.
+// second tier-1 class runs beside the first; with ev (nine events) the stage
+// boundaries are recorded on `stream` (ev[0..4]) and each tier-1 launch is
+// bracketed on the stream it runs on (ev[5..8], two per block class).
 tc_status launch_plan(const Plan& P, const PlanRegions& R, int cus, tc_stream stream, tc_stream side, tc_event fork,
                       tc_event join, tc_event* ev);
 
@@ -159,6 +165,7 @@ class Pipeline {
     bool styled(int cls) const { return plan_.styled[cls]; }
     // bytes moved by the HBM-bound stages (algorithmic): dwt levels and final
     int64_t dwt_bytes() const { return plan_.dwt_bytes; }
+    int64_t t1_class_bytes(int cls) const { return plan_.t1_class_bytes[cls]; }
     int64_t final_bytes() const { return plan_.final_bytes; }
     // Capture run() as a graph and replay it (batched form).
     Status capture_graph(std::string& err);
@@ -186,7 +193,7 @@ class Pipeline {
     size_t pinned_cap_ = 0;
     Plan plan_;
     Tuning tuning_;
-    tc_event ev_[5] = {nullptr, nullptr, nullptr, nullptr, nullptr};
+    tc_event ev_[9] = {};
     tc_stream side_ = nullptr;           // second tier-1 class runs here, beside the first
     tc_event fork_ = nullptr, join_ = nullptr;
     bool timed_ = false;
