@@ -116,18 +116,19 @@ def check_equal_source(out, src, what):
 
 
 def stage_times(dec, reps=5):
+    """Mean stage times of timed runs, and each tier-1 launch's time on its own stream."""
     keys = ("fill_ms", "t1_ms", "dwt_ms", "final_ms", "total_ms")
     acc = {k: 0.0 for k in keys}
-    cls = [0.0, 0.0]
+    launch = [0.0] * 4
     for _ in range(reps):
         dec.run(timed=True)
         dec.sync()
         s = dec.stats()
         for k in keys:
             acc[k] += getattr(s, k) / reps
-        for c in range(2):
-            cls[c] += s.t1_class_ms[c] / reps
-    return acc, cls
+        for g in range(s.t1_launches):
+            launch[g] += s.t1_launch_ms[g] / reps
+    return acc, launch
 
 
 def timed_runs(dec, steps, graph=False):
@@ -140,16 +141,28 @@ def timed_runs(dec, steps, graph=False):
 
 
 def t1_kernel_name(bpw, small, args):
-    """The tier-1 kernel a block class runs, as rocprof names it."""
+    """The tier-1 kernel a launch runs, as rocprof names it."""
     if bpw < 0:
         return "t1_lockstep_kernel<unsigned int, 32>" if small else "t1_lockstep_kernel<unsigned long, 64>"
     if bpw > 1:
         return f"t1_decode_kernel<{bpw}>"
-    return "t1_stripe_kernel" if args.t1_kernel == 0 else "t1_wave_kernel"
+    if args.t1_kernel == 0:
+        return "t1_stripe_kernel<unsigned int, 32, 32>" if small else "t1_stripe_kernel<unsigned long, 20, 64>"
+    return "t1_wave_kernel"
 
 
 def kernel_tag(name):
-    return name.replace("<unsigned int, 32>", "_32").replace("<unsigned long, 64>", "_64").split("<")[0]
+    """File-name form of a kernel name: t1_lockstep_kernel_64, t1_stripe_kernel_32, ..."""
+    base = name.split("<")[0]
+    if "<" not in name:
+        return base
+    return f"{base}_{name.rstrip('>').split(',')[-1].strip()}"
+
+
+def t1_launches(s, ms, args):
+    return [{"kernel": t1_kernel_name(s.t1_launch_bpw[g], s.t1_launch_class[g] == 0, args),
+             "blocks": s.t1_launch_blocks[g], "blocks_per_wave": s.t1_launch_bpw[g],
+             "bytes": s.t1_launch_bytes[g], "ms": round(ms[g], 3)} for g in range(s.t1_launches)]
 
 
 def traffic_record(kernel, workload):
@@ -501,15 +514,15 @@ def main():
         dec.launch_graph()
     graph = timed_runs(dec, args.steps, graph=True)
     eager = timed_runs(dec, max(3, args.steps // 4))
-    st, cls_ms = stage_times(dec)
+    st, launch_ms = stage_times(dec)
     s = dec.stats()
     dec.close()
-    # the dominant kernel: the longer of the two tier-1 launches
-    c = 1 if cls_ms[1] >= cls_ms[0] else 0
-    bpw = s.bpw_large if c == 1 else s.bpw_small
-    kname = t1_kernel_name(bpw, c == 0, args)
-    t1_bytes = s.t1_class_bytes[c]
-    t1_gbs = t1_bytes / (cls_ms[c] * 1e-3) / 1e9
+    launches = t1_launches(s, launch_ms, args)
+    # the dominant kernel: the longest tier-1 launch
+    dom = max(launches, key=lambda x: x["ms"])
+    kname = dom["kernel"]
+    t1_bytes = dom["bytes"]
+    t1_gbs = t1_bytes / (dom["ms"] * 1e-3) / 1e9
     dwt_gbs = (s.dwt_bytes + s.final_bytes) / ((st["dwt_ms"] + st["final_ms"]) * 1e-3) / 1e9
     tr = traffic_record(kname, "cfg4_mixed_256")
     line = {
@@ -531,19 +544,20 @@ def main():
                           "half lossy 9/7 0.5-2 bpp, 32x32 or 64x64 blocks",
                    "form": "batched, one launch per stage, graph replay", "inputs": "codestreams resident in HBM"},
         "stages_ms": {"fill": round(st["fill_ms"], 3), "tier1": round(st["t1_ms"], 3),
-                      "tier1_32x32_class": round(cls_ms[0], 3), "tier1_64x64_class": round(cls_ms[1], 3),
                       "idwt_levels": round(st["dwt_ms"], 3), "final_level": round(st["final_ms"], 3),
                       "total_eager": round(st["total_ms"], 3)},
+        "tier1_launches": launches,
         "roofline": {"bound": "hbm", "kernel": kname, "achieved": round(t1_gbs, 2), "peak": HBM_PEAK_GBS,
                      "unit": "GB/s", "frac": round(t1_gbs / HBM_PEAK_GBS, 5),
                      "traffic": tr["hbm_bytes_per_launch"] if tr else None, "algorithmic_bytes": t1_bytes,
-                     "launch_ms": round(cls_ms[c], 3),
+                     "launch_ms": dom["ms"], "blocks": dom["blocks"],
                      "note": "tier-1 is bound by the serial MQ symbol chains of its blocks, not by HBM; bytes = "
-                             "codeword bytes + 4 B per coefficient of the class's blocks, over the launch's own "
+                             "codeword bytes + 4 B per coefficient of the launch's blocks, over the launch's own "
                              "time (HIP events on the stream it runs on)"},
         "roofline_dwt": {"bound": "hbm", "achieved": round(dwt_gbs, 2), "peak": HBM_PEAK_GBS, "unit": "GB/s",
                          "frac": round(dwt_gbs / HBM_PEAK_GBS, 5)},
         "blocks_per_wave": {"small": s.bpw_small, "large": s.bpw_large},
+        "heavy_threshold": {"small": s.heavy_small, "large": s.heavy_large},
         "autotuned": tuning,
         "batched": {"cfg4_mixed_256": {
             "mpixel_s_graph": round(px / graph / 1e6, 1), "mpixel_s_eager": round(px / eager / 1e6, 1),

@@ -65,11 +65,17 @@ typedef struct {
     int32_t images;
     int64_t device_bytes;
     int64_t t1_samples; /* coefficients of the code-blocks tier-1 decodes (blocks with coding passes) */
-    /* per block class (0: blocks up to 32x32, 1: larger): the tier-1 launch's
-     * time on its own stream in the last timed run, and its algorithmic bytes
-     * (codeword bytes + 4 B per coefficient of the blocks it decodes) */
-    float t1_class_ms[2];
-    int64_t t1_class_bytes[2];
+    /* The tier-1 launches of the plan (at most four: per block class, the
+     * one-wave-per-block launch of its heaviest blocks, then its main launch),
+     * each with its block class (0: blocks up to 32x32, 1: larger), packing
+     * (blocks per wave as in tc_decoder_set_blocks_per_wave), block count,
+     * algorithmic bytes (codeword bytes + 4 B per coefficient of its blocks)
+     * and its time on the stream it ran on in the last timed run. */
+    int32_t t1_launches;
+    int32_t t1_launch_class[4], t1_launch_bpw[4];
+    int64_t t1_launch_blocks[4], t1_launch_bytes[4];
+    float t1_launch_ms[4];
+    int32_t heavy_small, heavy_large; /* heavy-block thresholds the plan applied (0: none) */
 } tc_decoder_stats;
 
 /* Main-header information (SIZ/COD/QCD) of a raw codestream or JP2 file.
@@ -147,14 +153,19 @@ typedef struct {
     int32_t t1_kernel;            /* one-wave-per-block kernel: 0 stripe, 1 row-mask */
     int32_t dwt_shape;            /* inverse-DWT tile shape, see tc_idwt_tile_shape */
     float t1_ms, dwt_ms;          /* stage times the autotuner measured for this choice */
+    /* per block class packed in lockstep: blocks of more estimated MQ symbols
+     * than this decode one wavefront per block in a launch of their own, beside
+     * the lockstep launch (0: no split, -1: the default) */
+    int32_t heavy_small, heavy_large;
 } tc_tuning;
 
 tc_status tc_decoder_get_tuning(tc_decoder dec, tc_tuning *out);
 tc_status tc_decoder_set_tuning(tc_decoder dec, const tc_tuning *in);
 
 /* Autotuner: times the prepared batch under candidate code-block-to-wave
- * packings (per block class) and inverse-DWT tile shapes, keeps the fastest,
- * and re-prepares with it.  *best (may be NULL) receives the choice. */
+ * packings (per block class), heavy-block thresholds and inverse-DWT tile
+ * shapes, keeps the fastest, and re-prepares with it.  *best (may be NULL)
+ * receives the choice. */
 tc_status tc_decoder_autotune(tc_decoder dec, int reps, tc_tuning *best);
 
 /* Per-image decode: parse, upload, decode, download (blocking). */

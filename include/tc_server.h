@@ -5,7 +5,10 @@
  * No counterpart in the reference (its Decoder decodes one file per call,
  * TC/Decoder.cpp:94-216).  Requests are parsed when submitted; a scheduler
  * thread admits queued requests into the next batch of a free slot while the
- * batch's device-memory footprint fits the slot's share of the budget, and
+ * batch's device-memory footprint (an upper estimate of the coefficient
+ * planes, inverse-DWT work buffers, tier-1 state, codewords, descriptors and
+ * output, times the 1.25 growth of the slot's buffers) fits the slot's share
+ * of the budget, and
  * launches it (one launch per stage for the whole batch) on the slot's own
  * stream, so batches of different slots overlap on the device.  Device buffers
  * of every slot stay resident between batches.  Results are copied back with
@@ -51,6 +54,8 @@ typedef struct {
     double plan_s;            /* summed over batches: parsing, planning and staging (host) */
     double device_s;          /* launch to results in host memory */
     double copy_s;            /* publishing results (they stay in pinned memory until read) */
+    uint64_t slot_device_peak; /* most device bytes one slot's resident buffers held after a batch
+                                  (a slot holding more than its share of the budget releases them) */
 } tc_server_stats;
 
 tc_status tc_server_create(tc_device dev, const tc_server_config *cfg, tc_server *out);

@@ -80,6 +80,40 @@ def test_admission_by_footprint(streams):
     srv.close()
 
 
+@pytest.mark.parametrize("bpw", [0, tca.BPW_LOCKSTEP16])
+def test_slot_device_bytes_within_share(bpw):
+    # requests of mixed sizes; under lockstep-16 packing every block holds
+    # tier-1 state in the scratch buffer, the largest share of the footprint
+    reqs = workloads.cfg4(n=24, scale=0.35) + [synth.encode(synth.image(700, 900, 1, 78, 16), bits=16, levels=5,
+                                                            cb=64)]
+    fp = []
+    for cs in reqs:
+        # alone, the resident buffers of the slot stay within the request's
+        # estimated footprint, grown by the slot's 1.25 factor (1 MiB floor per buffer)
+        srv = tca.Server(slots=1, max_batch=1, bpw_small=bpw, bpw_large=bpw)
+        rid = srv.submit(cs)
+        assert srv.wait(rid, 60.0)
+        f = srv.info(rid).footprint
+        fp.append(f)
+        srv.release(rid)
+        held = srv.stats().slot_device_peak
+        srv.close()
+        assert 0 < held <= 1.25 * f + (4 << 20), (held, f)
+    share = 3 * max(fp)
+    print(f"slot share {share >> 20} MiB, footprints {min(fp) >> 20}..{max(fp) >> 20} MiB", flush=True)
+    srv = tca.Server(slots=2, max_batch=256, memory_budget=2 * share, bpw_small=bpw, bpw_large=bpw)
+    ids = [srv.submit(cs) for cs in reqs * 2]
+    sizes = []
+    for rid in ids:
+        assert srv.wait(rid, 60.0)
+        sizes.append(srv.info(rid).batch_size)
+        srv.release(rid)
+    st = srv.stats()
+    srv.close()
+    assert max(sizes) > 1
+    assert st.slot_device_peak <= share, (st.slot_device_peak, share)
+
+
 def test_bad_request_fails_alone(streams):
     srv = tca.Server(slots=2)
     good = srv.submit(streams[0])

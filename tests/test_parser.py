@@ -72,3 +72,18 @@ def test_unsupported_features_are_reported():
     with pytest.raises(tca.TCError) as e:
         tca.probe(bytes(cs))
     assert e.value.status == 4
+
+
+@pytest.mark.parametrize("kw", [dict(tile=(256, 256)), dict(tile=(200, 160), cbsty=5, lossy=True, bpp=12.0),
+                                dict(tile=(256, 192), packed="ppt")])
+def test_tiles_parsed_in_parallel_equal_oracle(kw):
+    # more than a megabyte of packet data in many tiles: the parser reads the
+    # tiles' packets on several threads (one per megabyte, up to eight) and
+    # merges them in tile order; the table must equal the oracle's serial parse
+    from thousandthchicken_amd import synth
+    src = synth.image(1600, 1800, 3, 81)
+    cs = synth.encode(src, levels=4, cb=32, **kw)
+    assert len(cs) > (2 << 20)
+    a = [_key(c) for c in oracle.list_cblks(cs)]
+    b = [_key(c) for c in tca.list_cblks(cs)]
+    assert a == b

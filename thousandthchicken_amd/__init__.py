@@ -49,7 +49,9 @@ class DecoderStats(ctypes.Structure):
                                       "final_bytes")] + [
         ("bpw_small", ctypes.c_int32), ("bpw_large", ctypes.c_int32), ("images", ctypes.c_int32),
         ("device_bytes", ctypes.c_int64), ("t1_samples", ctypes.c_int64),
-        ("t1_class_ms", ctypes.c_float * 2), ("t1_class_bytes", ctypes.c_int64 * 2)]
+        ("t1_launches", ctypes.c_int32), ("t1_launch_class", ctypes.c_int32 * 4), ("t1_launch_bpw", ctypes.c_int32 * 4),
+        ("t1_launch_blocks", ctypes.c_int64 * 4), ("t1_launch_bytes", ctypes.c_int64 * 4),
+        ("t1_launch_ms", ctypes.c_float * 4), ("heavy_small", ctypes.c_int32), ("heavy_large", ctypes.c_int32)]
 
 
 class DeviceProps(ctypes.Structure):
@@ -280,7 +282,8 @@ assert CBLK_DESC.itemsize == 32
 
 class Tuning(ctypes.Structure):
     _fields_ = [("bpw_small", ctypes.c_int32), ("bpw_large", ctypes.c_int32), ("t1_kernel", ctypes.c_int32),
-                ("dwt_shape", ctypes.c_int32), ("t1_ms", ctypes.c_float), ("dwt_ms", ctypes.c_float)]
+                ("dwt_shape", ctypes.c_int32), ("t1_ms", ctypes.c_float), ("dwt_ms", ctypes.c_float),
+                ("heavy_small", ctypes.c_int32), ("heavy_large", ctypes.c_int32)]
 
     def as_dict(self):
         return {k: (round(getattr(self, k), 4) if k.endswith("_ms") else getattr(self, k)) for k, _ in self._fields_}
@@ -303,7 +306,7 @@ class ServerStats(ctypes.Structure):
     _fields_ = [(n, ctypes.c_uint64) for n in ("submitted", "completed", "failed", "batches", "pixels")] + [
         ("busy_s", ctypes.c_double), ("peak_bytes", ctypes.c_uint64), ("queued", ctypes.c_int32),
         ("in_flight", ctypes.c_int32), ("plan_s", ctypes.c_double), ("device_s", ctypes.c_double),
-        ("copy_s", ctypes.c_double)]
+        ("copy_s", ctypes.c_double), ("slot_device_peak", ctypes.c_uint64)]
 
 
 class Server:

@@ -236,12 +236,18 @@ tc_status tc_buf_copy_rect(tc_stream s, tc_buf dst, tc_buf src, const size_t so[
     if (!ssp) ssp = region[1] * srp;
     if (!drp) drp = region[0];
     if (!dsp) dsp = region[1] * drp;
+    if (region[0] > srp || region[0] > drp) return fail(TC_ERR_INVALID_ARG, "tc_buf_copy_rect: row wider than pitch");
+    // every slice is checked before the first is copied: a bad region copies nothing
+    const size_t last = (region[1] ? region[1] - 1 : 0);
     for (size_t z = 0; z < region[2]; z++) {
         size_t sb = (so[2] + z) * ssp + so[1] * srp + so[0];
         size_t db = (doo[2] + z) * dsp + doo[1] * drp + doo[0];
-        size_t last = (region[1] ? region[1] - 1 : 0);
         if (!in_range(src, sb, last * srp + region[0]) || !in_range(dst, db, last * drp + region[0]))
             return fail(TC_ERR_INVALID_ARG, "tc_buf_copy_rect: region outside buffer");
+    }
+    for (size_t z = 0; z < region[2] && region[0] && region[1]; z++) {
+        size_t sb = (so[2] + z) * ssp + so[1] * srp + so[0];
+        size_t db = (doo[2] + z) * dsp + doo[1] * drp + doo[0];
         tc_status st = hip_fail(hipMemcpy2DAsync((char*)dst->p + db, drp, (char*)src->p + sb, srp, region[0],
                                                  region[1], hipMemcpyDeviceToDevice, s->s),
                                 "hipMemcpy2DAsync");

@@ -2,6 +2,7 @@
 #include "codestream.hpp"
 
 #include <algorithm>
+#include <thread>
 #include <array>
 #include <tuple>
 #include <cmath>
@@ -789,9 +790,18 @@ int ilog2(int v) {
 // Reads the packet of (component c, resolution r, precinct q, layer l): its
 // body at pos within the tile data ranges, its header there too or, for
 // packed headers (PPM/PPT), in *ph at hpos.
+// Where the packets of one tile record what they carry: the codeword segments
+// (in codestream order) and the BYPASS / TERMALL segment lengths, whose index
+// a code-block's `term` holds.  Tiles parse into their own and are merged in
+// tile order.
+struct PacketOut {
+    std::vector<Segment> segs;
+    std::vector<std::vector<uint32_t>> termlens;
+};
+
 Status read_packet(Image& img, int t, int c, int r, int q, int layer, const Params& P,
                    TileState& bstate, const std::vector<Range>& ranges, size_t& ri,
-                   size_t& pos, const std::vector<uint8_t>* ph, size_t& hpos, std::string& err) {
+                   size_t& pos, const std::vector<uint8_t>* ph, size_t& hpos, PacketOut& out, std::string& err) {
     const TileComp& tc = img.tiles[t].comps[c];
     const Resolution& R = tc.res[r];
     ResState& RS = bstate[c][r];
@@ -886,16 +896,16 @@ Status read_packet(Image& img, int t, int c, int r, int q, int layer, const Para
         CodeBlock& cb = img.cblks[e.cb];
         if (multi) {
             if (cb.term < 0) {
-                cb.term = (int)img.termlens.size();
-                img.termlens.emplace_back();
+                cb.term = (int)out.termlens.size();
+                out.termlens.emplace_back();
             }
-            std::vector<uint32_t>& tl = img.termlens[cb.term];
+            std::vector<uint32_t>& tl = out.termlens[cb.term];
             if ((int)tl.size() <= e.seg) tl.resize(e.seg + 1, 0);
             tl[e.seg] += e.len;
         }
         if (e.len) {
             if (cb.nseg == 0) cb.seg0_off = (uint32_t)pos;
-            img.segs.push_back(Segment{e.cb, (uint32_t)pos, e.len});
+            out.segs.push_back(Segment{e.cb, (uint32_t)pos, e.len});
             cb.nseg++;
             cb.nbytes += e.len;
         }
@@ -905,7 +915,7 @@ Status read_packet(Image& img, int t, int c, int r, int q, int layer, const Para
 }
 
 Status read_packets(Image& img, int t, const Params& P, TileState& bstate, const std::vector<Range>& ranges,
-                    const std::vector<uint8_t>* ph, std::string& err) {
+                    const std::vector<uint8_t>* ph, PacketOut& out, std::string& err) {
     if (ranges.empty() && !ph) return Status::Ok;
     size_t ri = 0, pos = ranges.empty() ? 0 : ranges[0].off, hpos = 0;
     const Tile& T = img.tiles[t];
@@ -918,7 +928,7 @@ Status read_packets(Image& img, int t, const Params& P, TileState& bstate, const
         ResState& RS = bstate[c][r];
         if (q >= RS.pw * RS.ph || RS.nextl[q] != l) return Status::Ok;
         RS.nextl[q]++;
-        return read_packet(img, t, c, r, q, l, P, bstate, ranges, ri, pos, ph, hpos, err);
+        return read_packet(img, t, c, r, q, l, P, bstate, ranges, ri, pos, ph, hpos, out, err);
     };
     // RPCL, PCRL and CPRL (B.12.1.3-B.12.1.5) step through reference-grid
     // positions and emit a precinct where it starts: at a multiple of its size
@@ -992,7 +1002,7 @@ Status read_packets(Image& img, int t, const Params& P, TileState& bstate, const
     return volume(Poc{0, 0, P.nlayers, maxr, C, P.prog});
 }
 
-Status parse_impl(const uint8_t* data, size_t len, Image& img, std::string& err, bool packets) {
+Status parse_impl(const uint8_t* data, size_t len, Image& img, std::string& err, bool packets, int threads) {
     img = Image();
     const uint8_t* cs = nullptr;
     size_t cl = 0;
@@ -1125,13 +1135,46 @@ Status parse_impl(const uint8_t* data, size_t len, Image& img, std::string& err,
         if (end > r.pos()) ranges[I].push_back(Range{r.pos(), end - r.pos()});
         r.seek(end);
     }
+    // geometry of every tile (its code-blocks take consecutive indices), then
+    // the packets of each tile -- independent of the other tiles' (B.9), so
+    // several tiles parse at once -- merged in tile order
+    std::vector<TileState> bs(nt);
+    std::vector<size_t> cb0(nt + 1, 0);
     for (int t = 0; t < nt; t++) {
-        const Params& P = seen[t] ? tp[t] : main;
-        TileState bs;
-        s = build_tile(img, t, P, bs, err);
+        cb0[t] = img.cblks.size();
+        s = build_tile(img, t, seen[t] ? tp[t] : main, bs[t], err);
         if (s != Status::Ok) return s;
-        s = read_packets(img, t, P, bs, ranges[t], packed[t] ? &hdrs[t] : nullptr, err);
-        if (s != Status::Ok) return s;
+    }
+    cb0[nt] = img.cblks.size();
+    std::vector<PacketOut> outs(nt);
+    std::vector<Status> st(nt, Status::Ok);
+    std::vector<std::string> errs(nt);
+    auto work = [&](int k, int n) {
+        for (int t = k; t < nt; t += n)
+            st[t] = read_packets(img, t, seen[t] ? tp[t] : main, bs[t], ranges[t], packed[t] ? &hdrs[t] : nullptr,
+                                 outs[t], errs[t]);
+    };
+    // a thread is worth starting for about a megabyte of packet data
+    const int nthr = std::max(1, std::min<int>({threads, nt, (int)(1 + cl / (1u << 20))}));
+    if (nthr <= 1) {
+        work(0, 1);
+    } else {
+        std::vector<std::thread> th;
+        for (int k = 1; k < nthr; k++) th.emplace_back(work, k, nthr);
+        work(0, nthr);
+        for (auto& x : th) x.join();
+    }
+    for (int t = 0; t < nt; t++)
+        if (st[t] != Status::Ok) {
+            err = errs[t];
+            return st[t];
+        }
+    for (int t = 0; t < nt; t++) {
+        const int off = (int)img.termlens.size();
+        for (size_t k = cb0[t]; k < cb0[t + 1]; k++)
+            if (img.cblks[k].term >= 0) img.cblks[k].term += off;
+        for (auto& tl : outs[t].termlens) img.termlens.push_back(std::move(tl));
+        img.segs.insert(img.segs.end(), outs[t].segs.begin(), outs[t].segs.end());
     }
     // each BYPASS / TERMALL block's segment lengths end with their count (the
     // table the tier-1 kernel reads before the codeword)
@@ -1163,12 +1206,12 @@ bool Image::all_reversible() const {
     return true;
 }
 
-Status parse_codestream(const uint8_t* data, size_t len, Image& img, std::string& err) {
-    return parse_impl(data, len, img, err, true);
+Status parse_codestream(const uint8_t* data, size_t len, Image& img, std::string& err, int threads) {
+    return parse_impl(data, len, img, err, true, threads);
 }
 
 Status probe_codestream(const uint8_t* data, size_t len, Image& img, std::string& err) {
-    return parse_impl(data, len, img, err, false);
+    return parse_impl(data, len, img, err, false, 1);
 }
 
 Status reduce_image(Image& img, int k, std::string& err) {

@@ -153,7 +153,9 @@ struct Image {
 
 // Parses a raw codestream or a JP2 file.  The image keeps a pointer into
 // `data`, which must outlive it.
-Status parse_codestream(const uint8_t* data, size_t len, Image& img, std::string& err);
+// Parses the whole codestream.  threads > 1 reads the packets of up to that
+// many tiles at once (one thread per megabyte of codestream at most).
+Status parse_codestream(const uint8_t* data, size_t len, Image& img, std::string& err, int threads = 1);
 
 // Main header only (SIZ/COD/QCD), no packets.
 Status probe_codestream(const uint8_t* data, size_t len, Image& img, std::string& err);

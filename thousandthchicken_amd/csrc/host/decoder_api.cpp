@@ -163,7 +163,7 @@ tc_status tc_image_list_cblks(const uint8_t* cs, size_t len, tc_cblk_info* out, 
     if (!cs || !count) return report(Status::InvalidArgument, "tc_image_list_cblks: null");
     Image im;
     std::string err;
-    Status s = parse_codestream(cs, len, im, err);
+    Status s = parse_codestream(cs, len, im, err, 8);
     if (s != Status::Ok) return report(s, err);
     // crc over each block's segments in order
     std::vector<uint32_t> crc(im.cblks.size(), 2166136261u);
@@ -284,6 +284,8 @@ void to_tuning(const tc_tuning& in, Tuning& t) {
     t.bpw64 = in.bpw_large;
     t.t1_kernel = in.t1_kernel;
     t.dwt_shape = in.dwt_shape;
+    t.heavy32 = in.heavy_small;
+    t.heavy64 = in.heavy_large;
 }
 
 }  // namespace
@@ -296,6 +298,8 @@ tc_status tc_decoder_get_tuning(tc_decoder d, tc_tuning* out) {
     out->bpw_large = t.bpw64;
     out->t1_kernel = t.t1_kernel;
     out->dwt_shape = t.dwt_shape;
+    out->heavy_small = t.heavy32;
+    out->heavy_large = t.heavy64;
     return TC_SUCCESS;
 }
 
@@ -303,7 +307,7 @@ tc_status tc_decoder_set_tuning(tc_decoder d, const tc_tuning* in) {
     int nshape = 0;
     tc_idwt_tile_shape(0, nullptr, nullptr, &nshape);
     if (!d || !in || !valid_bpw(in->bpw_small) || !valid_bpw(in->bpw_large) || in->t1_kernel < 0 || in->t1_kernel > 1 ||
-        in->dwt_shape < 0 || in->dwt_shape >= nshape)
+        in->dwt_shape < 0 || in->dwt_shape >= nshape || in->heavy_small < -1 || in->heavy_large < -1)
         return report(Status::InvalidArgument, "tc_decoder_set_tuning: value out of range");
     Tuning t = d->pipe->tuning();
     to_tuning(*in, t);
@@ -373,6 +377,27 @@ tc_status tc_decoder_autotune(tc_decoder d, int reps, tc_tuning* best) {
             }
         }
     }
+    // heavy-block thresholds of a lockstep class (estimated symbols; 0: no split)
+    for (int cls = 0; cls < 2; cls++) {
+        if (!has[cls] || (cls == 0 ? cur.bpw32 : cur.bpw64) >= 0) continue;
+        static const int cand_small[] = {0, 3000, 4500, 6000, 8000};
+        static const int cand_large[] = {0, 10000, 13000, 16000, 20000, 25000};
+        const int* c = cls == 0 ? cand_small : cand_large;
+        const int nc = cls == 0 ? 5 : 6;
+        int& slot_cur = cls == 0 ? cur.heavy32 : cur.heavy64;
+        const int now = slot_cur < 0 ? default_heavy(cls == 0) : slot_cur;
+        for (int q = 0; q < nc; q++) {
+            if (c[q] == now) continue;
+            Tuning t = cur;
+            (cls == 0 ? t.heavy32 : t.heavy64) = c[q];
+            s = measure(t, &t1, &dw);
+            if (s != Status::Ok) return fail(s);
+            if (t1 < best_t1) {
+                best_t1 = t1;
+                cur = t;
+            }
+        }
+    }
     int nshape = 0;
     tc_idwt_tile_shape(0, nullptr, nullptr, &nshape);
     for (int sh = 0; sh < nshape; sh++) {
@@ -394,6 +419,8 @@ tc_status tc_decoder_autotune(tc_decoder d, int reps, tc_tuning* best) {
         // the packings the plan runs (a class without blocks keeps its setting)
         if (has[0]) best->bpw_small = d->pipe->blocks_per_wave(0);
         if (has[1]) best->bpw_large = d->pipe->blocks_per_wave(1);
+        best->heavy_small = d->pipe->plan().heavy[0];
+        best->heavy_large = d->pipe->plan().heavy[1];
         best->t1_ms = best_t1;
         best->dwt_ms = best_dwt;
     }
@@ -416,12 +443,14 @@ tc_status tc_decoder_prepare(tc_decoder d, int n, const uint8_t* const* cs, cons
     // copy and parse the codestreams on up to eight threads (tier-2 of each
     // image is independent); the resolution / region rewrites follow
     auto work = [&](int k, int nt) {
+        // the threads not needed across images read the tiles of each image
+        const int tile_threads = std::max(1, 8 / nt);
         for (int i = k; i < n; i += nt) {
             d->data[i].assign(cs[i], cs[i] + lens[i]);
             d->imgs[i].reset(new Image());
             Image& im = *d->imgs[i];
             std::string& err = errs[i];
-            Status s = parse_codestream(d->data[i].data(), d->data[i].size(), im, err);
+            Status s = parse_codestream(d->data[i].data(), d->data[i].size(), im, err, tile_threads);
             if (s == Status::Ok) s = reduce_image(im, d->reduce, err);
             if (s == Status::Ok && d->region[2] > 0 && d->region[3] > 0)
                 s = window_image(im, d->region[0], d->region[1], d->region[2], d->region[3], err);
@@ -505,7 +534,7 @@ tc_status tc_decode_noalloc(tc_stream stream, const uint8_t* cs, size_t len, voi
 
     Image im;
     std::string err;
-    Status s = parse_codestream(cs, len, im, err);
+    Status s = parse_codestream(cs, len, im, err, 4);
     if (s == Status::Ok && im.planar() && im.ncomp > 4) {
         s = Status::Unsupported;
         err = "sub-sampled images of more than four components are not supported";
@@ -537,7 +566,7 @@ tc_status tc_decode_noalloc(tc_stream stream, const uint8_t* cs, size_t len, voi
     std::lock_guard<std::mutex> lock(D.use);
     hipError_t e = hipMemcpyAsync(R.upload, stage.data(), stage.size(), hipMemcpyHostToDevice, hs);
     if (e != hipSuccess) return hip_fail(e, "hipMemcpyAsync (upload)");
-    tc_status ts = launch_plan(P, R, device_cu_count(dev), stream, nullptr, nullptr, nullptr, nullptr);
+    tc_status ts = launch_plan(P, R, device_cu_count(dev), stream, nullptr, nullptr);
     // the staging vector and the array stay in use until the stream drains
     e = hipStreamSynchronize(hs);
     if (ts != TC_SUCCESS) return ts == TC_ERR_INVALID_ARG ? report(Status::InvalidArgument, "launch_plan") : ts;
@@ -580,10 +609,17 @@ tc_status tc_decoder_get_stats(tc_decoder d, tc_decoder_stats* st) {
     st->bpw_large = d->pipe->blocks_per_wave(1);
     st->images = d->pipe->num_images();
     st->device_bytes = (int64_t)d->pipe->device_bytes();
-    for (int c = 0; c < 2; c++) {
-        st->t1_class_ms[c] = t.t1_class_ms[c];
-        st->t1_class_bytes[c] = d->pipe->t1_class_bytes(c);
+    const Plan& P = d->pipe->plan();
+    st->t1_launches = (int32_t)std::min<size_t>(P.groups.size(), kMaxT1Launches);
+    for (int g = 0; g < st->t1_launches; g++) {
+        st->t1_launch_class[g] = P.groups[g].small ? 0 : 1;
+        st->t1_launch_bpw[g] = P.groups[g].bpw;
+        st->t1_launch_blocks[g] = P.groups[g].n;
+        st->t1_launch_bytes[g] = P.groups[g].bytes;
+        st->t1_launch_ms[g] = t.t1_launch_ms[g];
     }
+    st->heavy_small = P.heavy[0];
+    st->heavy_large = P.heavy[1];
     return TC_SUCCESS;
 }
 

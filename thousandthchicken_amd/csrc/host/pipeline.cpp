@@ -73,22 +73,32 @@ int pick_blocks_per_wave(int n, bool small, int cus, const Tuning& t) {
     return n >= 8 * 64 * c ? TC_BPW_LOCKSTEP : TC_BPW_LOCKSTEP32;
 }
 
+// Heavy-block threshold (estimated MQ symbols) when none is set: none.  On
+// the 256-image mixed batch every split measured slower than none (64x64
+// class, 16 blocks per lockstep wave: 35.5 ms unsplit, 37-45 ms with the
+// blocks above 8 000-25 000 symbols on their own launch), because the
+// one-wave-per-block waves compete with the lockstep waves for the scalar
+// unit and LDS of their compute units; the autotuner still tries thresholds.
+int default_heavy(bool) { return 0; }
+
 Pipeline::Pipeline(tc_device dev) : dev_(dev) {
     if (tc_stream_create(dev, &stream_) != TC_SUCCESS) stream_ = nullptr;
-    if (tc_stream_create(dev, &side_) != TC_SUCCESS) side_ = nullptr;
+    for (auto& sd : ls_.side)
+        if (tc_stream_create(dev, &sd) != TC_SUCCESS) sd = nullptr;
     for (auto& e : ev_) tc_event_create(&e);
-    tc_event_create(&fork_);
-    tc_event_create(&join_);
+    tc_event_create(&ls_.fork);
+    for (auto& e : ls_.join) tc_event_create(&e);
 }
 
 Pipeline::~Pipeline() {
     if (stream_) tc_stream_sync(stream_);
     drop_graph();
-    if (side_) tc_stream_sync(side_);
+    for (auto& sd : ls_.side)
+        if (sd) tc_stream_sync(sd);
     for (auto& e : ev_) tc_event_destroy(e);
-    tc_event_destroy(fork_);
-    tc_event_destroy(join_);
-    tc_stream_destroy(side_);
+    tc_event_destroy(ls_.fork);
+    for (auto& e : ls_.join) tc_event_destroy(e);
+    for (auto& sd : ls_.side) tc_stream_destroy(sd);
     tc_buf_free(upload_);
     tc_buf_free(samples_);
     tc_buf_free(out_);
@@ -105,7 +115,8 @@ Status Pipeline::ensure(tc_buf& b, size_t bytes, const char* what, std::string& 
         tc_buf_free(b);
         b = nullptr;
     }
-    size_t want = std::max<size_t>(2 * bytes, 1 << 20);  // grow geometrically: reallocation syncs the device
+    // grow geometrically: reallocation syncs the device
+    size_t want = std::max<size_t>((size_t)((double)bytes * growth_), 1 << 20);
     tc_status s = tc_buf_alloc(dev_, want, &b);
     if (s != TC_SUCCESS) {
         err = std::string("device allocation of ") + what + " failed: " + tc_last_error();
@@ -188,39 +199,70 @@ Status build_plan(const std::vector<const Image*>& imgs, const Tuning& tuning, i
     P.ncblk = (int64_t)descs.size();
     P.t1_samples = 0;
     for (const auto& d : descs) P.t1_samples += (int64_t)d.w * d.h;
-    for (int k = 0; k < 2; k++)
-        for (uint32_t i : cls_idx[k]) P.t1_class_bytes[k] += (int64_t)descs[i].data_len + 4 * (int64_t)descs[i].w * descs[i].h;
 
     Blob blob;
     blob.base = align_up(P.cw_size, 256);
     P.descs_off = put(blob, descs);
     uint64_t scratch_total = 0;
-    for (int k = 0; k < 2; k++) {
-        auto& v = cls_idx[k];
-        P.bpw[k] = 0;
-        if (v.empty()) continue;
-        sort_for_packing(v, descs);
+    auto group_of = [&](const std::vector<uint32_t>& v, bool small, int bpw, bool heavy) {
         Plan::Group g;
-        g.small = k == 0;
-        g.bpw = pick_blocks_per_wave((int)v.size(), g.small, cus, tuning);
+        g.small = small;
+        g.bpw = bpw;
+        g.heavy = heavy;
         g.n = (int)v.size();
-        int style = 0;
         for (uint32_t i : v) {
             g.max_planes = std::max<int>(g.max_planes, descs[i].numbps);
             g.max_passes = std::max<int>(g.max_passes, descs[i].npasses);
-            style |= descs[i].mode & TC_MODE_STYLE;
+            g.bytes += (int64_t)descs[i].data_len + 4 * (int64_t)descs[i].w * descs[i].h;
         }
-        // only the lockstep kernel decodes the code-block styles
-        if (style && g.bpw >= 0) g.bpw = g.n >= 8 * 64 * std::max(cus, 1) ? TC_BPW_LOCKSTEP : TC_BPW_LOCKSTEP32;
         g.order_off = put(blob, v);
         if (g.bpw < 0) {  // lockstep, -bpw blocks per wave
             g.scratch_off = scratch_total;
             scratch_total += align_up(tc_t1_lockstep_scratch_bytes(g.small ? 1 : 0, -g.bpw, g.n, g.max_passes), 256);
         }
-        P.bpw[k] = g.bpw;
+        return g;
+    };
+    std::vector<Plan::Group> heavy_groups, main_groups;
+    for (int k = 0; k < 2; k++) {
+        auto& v = cls_idx[k];
+        P.bpw[k] = 0;
+        if (v.empty()) continue;
+        const bool small = k == 0;
+        int bpw = pick_blocks_per_wave((int)v.size(), small, cus, tuning);
+        int style = 0;
+        for (uint32_t i : v) style |= descs[i].mode & TC_MODE_STYLE;
+        // only the lockstep kernel decodes the code-block styles
+        if (style && bpw >= 0) bpw = (int)v.size() >= 8 * 64 * std::max(cus, 1) ? TC_BPW_LOCKSTEP : TC_BPW_LOCKSTEP32;
+        // a lockstep class sends its longest symbol chains to one wave per block
+        int thr = small ? tuning.heavy32 : tuning.heavy64;
+        if (thr < 0) thr = default_heavy(small);
+        std::vector<uint32_t> heavy;
+        if (bpw < 0 && thr > 0) {
+            std::vector<uint32_t> light;
+            for (uint32_t i : v) {
+                const tc_cblk_desc& d = descs[i];
+                const bool h = !(d.mode & TC_MODE_STYLE) && d.numbps <= TC_T1_STRIPE_MAX_PLANES(small) &&
+                               estimated_symbols(d) > thr;
+                (h ? heavy : light).push_back(i);
+            }
+            v.swap(light);
+            // longest chains first: they start first and bound the launch
+            std::stable_sort(heavy.begin(), heavy.end(), [&](uint32_t a, uint32_t b) {
+                return estimated_symbols(descs[a]) > estimated_symbols(descs[b]);
+            });
+        }
+        if (!heavy.empty()) heavy_groups.push_back(group_of(heavy, small, 1, true));
+        if (!v.empty()) {
+            sort_for_packing(v, descs);
+            main_groups.push_back(group_of(v, small, bpw, false));
+        }
+        P.bpw[k] = v.empty() ? 1 : bpw;
+        P.heavy[k] = heavy.empty() ? 0 : thr;
         P.styled[k] = style != 0;
-        P.groups.push_back(g);
     }
+    // heavy launches go first, so that their waves are dispatched first
+    for (auto& g : heavy_groups) P.groups.push_back(g);
+    for (auto& g : main_groups) P.groups.push_back(g);
 
     // ---- inverse DWT levels ----
     int maxlev = 0;
@@ -413,29 +455,33 @@ void stage_upload(const Plan& P, uint8_t* stage) {
     if (!P.blob.empty()) std::memcpy(stage + P.blob_base, P.blob.data(), P.blob.size());
 }
 
-tc_status launch_plan(const Plan& P, const PlanRegions& R, int cus, tc_stream stream, tc_stream side, tc_event fork_ev,
-                      tc_event join_ev, tc_event* ev) {
+tc_status launch_plan(const Plan& P, const PlanRegions& R, int cus, tc_stream stream, const LaunchStreams* S,
+                      tc_event* ev) {
     uint8_t* up = R.upload;
     void* base = R.samples;
     if (!up || !base || !R.out) return TC_ERR_INVALID_ARG;
+    if (P.groups.size() > (size_t)kMaxT1Launches) return TC_ERR_INVALID_ARG;
     for (const Plan::Group& g : P.groups)
         if (g.bpw < 0 && (!R.scratch || R.scratch_bytes <= g.scratch_off)) return TC_ERR_INVALID_ARG;
     tc_status s = TC_SUCCESS;
     if (ev) tc_event_record(ev[0], stream);
     s = tc_launch_fill32(stream, base, 0u, P.coef_elems);
     if (ev) tc_event_record(ev[1], stream);
-    // Tier-1: one launch per block class.  With two classes the second runs on
-    // the side stream, so each launch's tail overlaps the other's work.
-    const bool fork = P.groups.size() == 2 && side && fork_ev && join_ev;
+    // Tier-1: launch g > 0 runs on side stream g - 1, so that every launch's
+    // waves share the machine with the others' (a short launch's tail does not
+    // idle the GPU while a long one still runs)
+    const int ng = (int)P.groups.size();
+    bool fork = S && S->fork && ng > 1;
+    for (int g = 1; fork && g < ng; g++) fork = S->side[g - 1] && S->join[g - 1];
     if (fork && s == TC_SUCCESS) {
-        s = tc_event_record(fork_ev, stream);
-        if (s == TC_SUCCESS) s = tc_stream_wait_event(side, fork_ev);
+        s = tc_event_record(S->fork, stream);
+        for (int g = 1; s == TC_SUCCESS && g < ng; g++) s = tc_stream_wait_event(S->side[g - 1], S->fork);
     }
-    for (size_t gi = 0; gi < P.groups.size(); gi++) {
+    for (int gi = 0; gi < ng; gi++) {
         if (s != TC_SUCCESS) break;
         const Plan::Group& g = P.groups[gi];
-        tc_stream st = (fork && gi == 1) ? side : stream;
-        tc_event* gev = ev ? ev + 5 + 2 * (g.small ? 0 : 1) : nullptr;
+        tc_stream st = (fork && gi > 0) ? S->side[gi - 1] : stream;
+        tc_event* gev = ev ? ev + 5 + 2 * gi : nullptr;
         if (gev) tc_event_record(gev[0], st);
         if (g.bpw < 0)
             s = tc_launch_t1_decode_lockstep(st, g.small ? 1 : 0, -g.bpw, up + P.descs_off,
@@ -452,9 +498,9 @@ tc_status launch_plan(const Plan& P, const PlanRegions& R, int cus, tc_stream st
                                     g.n, up, base);
         if (gev) tc_event_record(gev[1], st);
     }
-    if (fork && s == TC_SUCCESS) {
-        s = tc_event_record(join_ev, side);
-        if (s == TC_SUCCESS) s = tc_stream_wait_event(stream, join_ev);
+    for (int g = 1; fork && s == TC_SUCCESS && g < ng; g++) {
+        s = tc_event_record(S->join[g - 1], S->side[g - 1]);
+        if (s == TC_SUCCESS) s = tc_stream_wait_event(stream, S->join[g - 1]);
     }
     if (ev) tc_event_record(ev[2], stream);
     for (const Plan::LevelLaunch& L : P.levels) {
@@ -480,12 +526,13 @@ void Pipeline::invalidate() {
     plan_ = Plan();
 }
 
-size_t Pipeline::device_bytes_bound(const Plan& P, bool host_out) {
-    auto grown = [](uint64_t bytes) -> size_t { return std::max<size_t>(2 * bytes, 1 << 20); };
-    size_t b = grown(P.upload_bytes()) + grown(std::max<uint64_t>(P.sample_elems, 1) * 4);
-    if (!host_out) b += grown(std::max<uint64_t>(P.out_bytes, 1));
-    if (P.scratch_bytes) b += grown(P.scratch_bytes);
-    return b;
+void Pipeline::release_buffers() {
+    if (stream_) tc_stream_sync(stream_);
+    invalidate();
+    for (tc_buf* b : {&upload_, &samples_, &out_, &scratch_}) {
+        tc_buf_free(*b);
+        *b = nullptr;
+    }
 }
 
 Status Pipeline::prepare(const std::vector<const Image*>& imgs, std::string& err) {
@@ -540,7 +587,7 @@ Status Pipeline::run(bool timed) {
     R.scratch = (uint8_t*)tc_buf_device_ptr(scratch_);
     R.scratch_bytes = tc_buf_size(scratch_);
     timed_ = timed;
-    tc_status s = launch_plan(plan_, R, device_cu_count(dev_), stream_, side_, fork_, join_, timed ? ev_ : nullptr);
+    tc_status s = launch_plan(plan_, R, device_cu_count(dev_), stream_, &ls_, timed ? ev_ : nullptr);
     if (s == TC_ERR_INVALID_ARG) return Status::InvalidArgument;
     return s == TC_SUCCESS ? Status::Ok : Status::Device;
 }
@@ -554,10 +601,8 @@ StageTimes Pipeline::times() const {
     tc_event_elapsed_ms(ev_[2], ev_[3], &t.dwt_ms);
     tc_event_elapsed_ms(ev_[3], ev_[4], &t.final_ms);
     tc_event_elapsed_ms(ev_[0], ev_[4], &t.total_ms);
-    for (const Plan::Group& g : plan_.groups) {
-        const int c = g.small ? 0 : 1;
-        tc_event_elapsed_ms(ev_[5 + 2 * c], ev_[6 + 2 * c], &t.t1_class_ms[c]);
-    }
+    for (size_t g = 0; g < plan_.groups.size() && g < (size_t)kMaxT1Launches; g++)
+        tc_event_elapsed_ms(ev_[5 + 2 * g], ev_[6 + 2 * g], &t.t1_launch_ms[g]);
     return t;
 }
 

@@ -33,7 +33,26 @@ struct Tuning {
     int bpw32 = 0;  // blocks per wave for blocks up to 32x32 (0 = pick automatically)
     int t1_kernel = 0;  // one-wave-per-block kernel: 0 stripe-column form, 1 row-mask form
     int dwt_shape = 0;  // inverse-DWT output tile shape (tc_idwt_tile_shape)
+    // A class packed in lockstep sends its blocks of more estimated MQ
+    // symbols than this to a one-wave-per-block launch of their own, whose
+    // waves decode a long symbol chain several times faster than a lockstep
+    // lane does (0: no split; -1: the default rule, default_heavy).
+    int heavy64 = -1, heavy32 = -1;
 };
+
+// Estimated MQ symbols of a code-block from its descriptor: a least-squares
+// fit over the 250 000 blocks of the 256-image mixed batch (correlation 0.996
+// with the oracle's counts).
+inline int64_t estimated_symbols(const tc_cblk_desc& d) {
+    return (int64_t)(6.3 * d.data_len + 7.5 * d.npasses * ((double)d.w * d.h / 64.0)) + 200;
+}
+
+// Heavy-block threshold used when Tuning says -1.
+int default_heavy(bool small_blocks);
+
+// Tier-1 launches of a plan: at most two per class (the heavy blocks' one
+// wave per block launch, then the class's main launch).
+constexpr int kMaxT1Launches = 4;
 
 // Chooses blocks-per-wave from the block count and the machine size, by a rule
 // taken from the autotuner's choices; a fixed value in Tuning overrides it.
@@ -41,8 +60,8 @@ int pick_blocks_per_wave(int nblocks, bool small_blocks, int cu_count, const Tun
 
 struct StageTimes {
     float fill_ms = 0, t1_ms = 0, dwt_ms = 0, final_ms = 0, total_ms = 0;
-    // each tier-1 launch on its own stream, by block class (0: blocks up to 32x32)
-    float t1_class_ms[2] = {0, 0};
+    // each tier-1 launch, timed on the stream it runs on (order of Plan::groups)
+    float t1_launch_ms[kMaxT1Launches] = {0, 0, 0, 0};
 };
 
 // The device work of a batch: launches, their descriptor offsets in the upload
@@ -56,6 +75,8 @@ struct Plan {
         uint64_t order_off = 0;
         uint64_t scratch_off = 0;  // lockstep packing: its state area in the tier-1 scratch region
         int n = 0;
+        bool heavy = false;  // the class's heaviest blocks, split off its lockstep launch
+        int64_t bytes = 0;   // algorithmic bytes: codeword bytes + 4 B per coefficient
     };
     struct LevelLaunch {
         bool irr = false;
@@ -90,9 +111,7 @@ struct Plan {
     uint64_t scratch_bytes = 0;    // tier-1 state of the lockstep packings
     uint64_t out_bytes = 0;        // all output images
     int bpw[2] = {0, 0};           // packing per class (0: no blocks)
-    // tier-1 algorithmic bytes per class: codeword bytes read + 4 B per
-    // coefficient of the blocks that have coding passes
-    int64_t t1_class_bytes[2] = {0, 0};
+    int heavy[2] = {0, 0};         // heavy-block threshold applied per class (0: none)
     bool styled[2] = {false, false};
     int64_t ncblk = 0, cw_bytes = 0, nsamples = 0, npixels = 0, dwt_bytes = 0, final_bytes = 0, t1_samples = 0;
 
@@ -114,12 +133,20 @@ struct PlanRegions {
     size_t scratch_bytes = 0;
 };
 
-// Enqueues the plan on `stream`.  With a side stream and fork/join events the
-// second tier-1 class runs beside the first; with ev (nine events) the stage
-// boundaries are recorded on `stream` (ev[0..4]) and each tier-1 launch is
-// bracketed on the stream it runs on (ev[5..8], two per block class).
-tc_status launch_plan(const Plan& P, const PlanRegions& R, int cus, tc_stream stream, tc_stream side, tc_event fork,
-                      tc_event join, tc_event* ev);
+// Streams and events for running a plan's tier-1 launches side by side.
+struct LaunchStreams {
+    tc_stream side[kMaxT1Launches - 1] = {};  // launch g > 0 runs on side[g - 1]
+    tc_event fork = nullptr;
+    tc_event join[kMaxT1Launches - 1] = {};
+};
+
+// Enqueues the plan on `stream`.  With side streams (S, may be null) the
+// tier-1 launches run side by side, the heavy blocks' launches first; with
+// ev (5 + 2 * kMaxT1Launches events) the stage boundaries are recorded on
+// `stream` (ev[0..4]) and each tier-1 launch is bracketed on the stream it
+// runs on (ev[5 + 2g], ev[6 + 2g]).
+tc_status launch_plan(const Plan& P, const PlanRegions& R, int cus, tc_stream stream, const LaunchStreams* S,
+                      tc_event* ev);
 
 class Pipeline {
   public:
@@ -165,7 +192,7 @@ class Pipeline {
     bool styled(int cls) const { return plan_.styled[cls]; }
     // bytes moved by the HBM-bound stages (algorithmic): dwt levels and final
     int64_t dwt_bytes() const { return plan_.dwt_bytes; }
-    int64_t t1_class_bytes(int cls) const { return plan_.t1_class_bytes[cls]; }
+    const Plan& plan() const { return plan_; }
     int64_t final_bytes() const { return plan_.final_bytes; }
     // Capture run() as a graph and replay it (batched form).
     Status capture_graph(std::string& err);
@@ -173,10 +200,13 @@ class Pipeline {
     void drop_graph();
     // device memory the pipeline holds (its four buffers, as allocated)
     size_t device_bytes() const;
-    // the most device memory a pipeline holds after preparing P, counting the
-    // geometric growth of its buffers (each grows to twice what it is asked
-    // for) and the 1 MiB floor of each
-    static size_t device_bytes_bound(const Plan& P, bool host_out);
+    // A buffer that must grow is allocated this many times the size asked
+    // for (default 2: few reallocations; each one synchronises the stream).
+    void set_growth(double factor) { growth_ = factor < 1.0 ? 1.0 : factor; }
+    // Frees the device buffers (after the stream drains); the next prepare
+    // allocates again.  The current plan is dropped.
+    void release_buffers();
+
 
   private:
     Status ensure(tc_buf& b, size_t bytes, const char* what, std::string& err);
@@ -193,12 +223,12 @@ class Pipeline {
     size_t pinned_cap_ = 0;
     Plan plan_;
     Tuning tuning_;
-    tc_event ev_[9] = {};
-    tc_stream side_ = nullptr;           // second tier-1 class runs here, beside the first
-    tc_event fork_ = nullptr, join_ = nullptr;
+    tc_event ev_[5 + 2 * kMaxT1Launches] = {};
+    LaunchStreams ls_;  // tier-1 launches after the first run on these, side by side
     bool timed_ = false;
     tc_graph graph_ = nullptr;
     bool ready_ = false;  // a prepare succeeded and nothing has invalidated it since
+    double growth_ = 2.0;
 };
 
 }  // namespace tc

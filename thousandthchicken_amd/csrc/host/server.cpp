@@ -51,24 +51,46 @@ struct Slot {
     std::thread worker;
 };
 
-// Device bytes a request needs in a pipeline: coefficient planes and work
-// buffers (4 B per element), the packed output, codewords and descriptors.
-// From the main header: tile 0's geometry stands for every tile (tile 0 is a
-// full-size tile), and one descriptor per 16 codeword bytes bounds the blocks.
+// Upper estimate of the device bytes a request needs in a pipeline, from its
+// main header:
+// - coefficient planes: 4 B per sample of every component over the whole
+//   image (the tiles partition it), plus one 64-element alignment per
+//   tile-component;
+// - inverse-DWT work buffers: per tile-component at most two quarter-tiles of
+//   the nominal tile size (the level-1 low band, ping-pong), 4 B each;
+// - tier-1 state of a lockstep packing, whichever the planner picks: per
+//   code-block (4 + passes / 3) rows of the block height, 8 B per row for
+//   64-wide blocks and 4 B for narrower ones, with passes bounded by the
+//   bit-planes (precision + 2 guard bits + levels + 2), blocks counted over
+//   the image with partial blocks at every band edge;
+// - the packed output, the codestream (codewords) and 40 B of descriptor per
+//   16 codeword bytes.
 uint64_t footprint_of(const Image& im) {
+    if (im.tiles.empty() || im.tiles[0].comps.empty()) return (uint64_t)im.cs_len * 4 + (1u << 20);
+    const uint64_t ntiles = (uint64_t)std::max(1, im.ntx * im.nty);
     uint64_t f = 0;
-    if (!im.tiles.empty())
-        for (const auto& c : im.tiles[0].comps) {
-            uint64_t sz[2];
-            work_sizes(c, sz);
-            f += ((uint64_t)c.width() * c.height() + sz[0] + sz[1] + 192) * 4;
-        }
-    f *= (uint64_t)std::max<size_t>(im.tiles.size(), 1);
+    for (int c = 0; c < im.ncomp; c++) {
+        int x0, y0, x1, y1;
+        im.comp_rect(c, x0, y0, x1, y1);
+        const uint64_t samples = (uint64_t)std::max(0, x1 - x0) * std::max(0, y1 - y0);
+        const TileComp& tc0 = im.tiles[0].comps[c];
+        const uint64_t tw = (uint64_t)(im.tw + im.dx[c] - 1) / im.dx[c], th = (uint64_t)(im.th + im.dy[c] - 1) / im.dy[c];
+        const uint64_t work = tc0.nlev > 1 ? 2 * ((tw + 1) / 2) * ((th + 1) / 2) : 0;
+        f += (samples + ntiles * (64 + work + 128)) * 4;
+        const Band& B = tc0.res[tc0.nlev].band[0];
+        const uint64_t cbw = (uint64_t)1 << B.xcb, cbh = (uint64_t)1 << B.ycb;
+        const uint64_t blocks = samples / (cbw * cbh) + ntiles * (uint64_t)(3 * tc0.nlev + 1) * (tw / cbw + th / cbh + 2);
+        const uint64_t passes = 3 * (uint64_t)(im.prec[c] + 2 + tc0.nlev + 2);
+        f += blocks * (4 + passes / 3) * cbh * (cbw > 32 ? 8 : 4);
+    }
     f += im.out_bytes();
     f += im.cs_len + im.cs_len / 16 * 40 + 4096;
     return f;
 }
 
+// Resident buffers of a slot's pipeline grow by this factor when they must
+// (admission charges it).
+constexpr double kSlotGrowth = 1.25;
 }  // namespace
 
 // One worker thread per slot: it takes the next batch from the queue (FIFO,
@@ -182,6 +204,7 @@ struct tc_server_s {
 
     void work(Slot& s) {
         const uint64_t slot_budget = cfg.memory_budget / (uint64_t)std::max(1, cfg.slots);
+        const uint64_t batch_budget = (uint64_t)((double)slot_budget / kSlotGrowth);
         std::unique_lock<std::mutex> lk(mu);
         for (;;) {
             idle_workers++;
@@ -201,7 +224,7 @@ struct tc_server_s {
             while (!queue.empty() && (int)s.batch.size() < cfg.max_batch) {
                 Request* r = queue.front();
                 if (!s.batch.empty() &&
-                    (fp + r->footprint > slot_budget || taken >= share || taken + r->cs_len > cfg.max_batch_bytes))
+                    (fp + r->footprint > batch_budget || taken >= share || taken + r->cs_len > cfg.max_batch_bytes))
                     break;
                 fp += r->footprint;
                 taken += r->cs_len;
@@ -236,7 +259,13 @@ struct tc_server_s {
             const tc_status e = execute(s);
             const double tc0 = now();
             if (e != TC_SUCCESS) tc_stream_sync(s.pipe->stream());
+            // a slot keeps its buffers for the next batch unless they outgrew
+            // its share of the budget (a request larger than the share runs
+            // alone and its buffers go with it)
+            const uint64_t held = s.pipe->device_bytes();
+            if (held > slot_budget) s.pipe->release_buffers();
             lk.lock();
+            st.slot_device_peak = std::max<uint64_t>(st.slot_device_peak, held);
             if (e == TC_SUCCESS)
                 for (size_t i = 0; i < s.batch.size(); i++) {
                     Request* r = s.batch[i];
@@ -291,6 +320,7 @@ tc_status tc_server_create(tc_device dev, const tc_server_config* cfg, tc_server
             delete s;
             return fail(TC_ERR_DEVICE, "tc_server_create: stream/event creation failed");
         }
+        sl.pipe->set_growth(kSlotGrowth);
         Tuning t;
         t.bpw32 = s->cfg.bpw_small;
         t.bpw64 = s->cfg.bpw_large;

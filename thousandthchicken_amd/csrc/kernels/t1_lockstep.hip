@@ -353,11 +353,12 @@ __device__ __forceinline__ void stripe_symbols(LdsLock<M>& L, int lane, LaneMq& 
 }
 
 // g: the wave's state rows [mask][row][lane]; lg: its refinement logs
-// [pass][row][lane], one per refinement pass (pass k = 3j + 2 writes log j).
+// [pass][row][lane], one per refinement pass (pass k = 3j + 2 writes log j);
+// a row holds one entry per block of the wave (stride bpw).
 template <typename M, int MAXH>
 __device__ void decode_lockstep(LdsLock<M>& L, const tc_cblk_desc& d, bool active, int lane,
                                 const uint32_t* __restrict__ words, uint32_t nwords, uint32_t* __restrict__ coef,
-                                M* __restrict__ g, M* __restrict__ lg) {
+                                M* __restrict__ g, M* __restrict__ lg, int bpw) {
     constexpr int NB = (int)sizeof(M) * 8;
     const int w = active ? d.w : 0, h = active ? d.h : 0;
     const int numbps = active ? d.numbps : 0, npasses = active ? d.npasses : 0;
@@ -391,7 +392,7 @@ __device__ void decode_lockstep(LdsLock<M>& L, const tc_cblk_desc& d, bool activ
         lane_init(m);
     }
 
-    auto gat = [&](int mask, int row) -> M& { return g[(mask * MAXH + row) * kLaneCount + lane]; };
+    auto gat = [&](int mask, int row) -> M& { return g[(mask * MAXH + row) * bpw + lane]; };
     auto lr = [&](int i) -> M& { return L.rows[i][lane]; };
 
     M carry_s = 0, carry_x = 0;  // rows y0-1 of the previous stripe
@@ -509,11 +510,11 @@ __device__ void decode_lockstep(LdsLock<M>& L, const tc_cblk_desc& d, bool activ
                     carry_s = lr(RS + 4);
                     carry_x = lr(RX + 4);
                 } else {
-                    M* const lj = lg + (size_t)((k - 2) / 3) * MAXH * kLaneCount;
-                    lj[(y0 + 0) * kLaneCount + lane] = o0;
-                    lj[(y0 + 1) * kLaneCount + lane] = o1;
-                    lj[(y0 + 2) * kLaneCount + lane] = o2;
-                    lj[(y0 + 3) * kLaneCount + lane] = o3;
+                    M* const lj = lg + (size_t)((k - 2) / 3) * MAXH * bpw;
+                    lj[(y0 + 0) * bpw + lane] = o0;
+                    lj[(y0 + 1) * bpw + lane] = o1;
+                    lj[(y0 + 2) * bpw + lane] = o2;
+                    lj[(y0 + 3) * bpw + lane] = o3;
                 }
             }
         }
@@ -542,7 +543,7 @@ __device__ void decode_lockstep(LdsLock<M>& L, const tc_cblk_desc& d, bool activ
 // 32 loads are in flight per lane.
 template <typename M, int MAXH>
 __device__ void convert_lockstep_block(int lane, int j, const tc_cblk_desc& d, uint32_t* __restrict__ coef,
-                                       const M* __restrict__ lg) {
+                                       const M* __restrict__ lg, int bpw) {
     constexpr int RU = 4, TU = 8;
     const int numbps = d.numbps, np = d.npasses;
     if (numbps <= 0 || np <= 0) return;
@@ -577,7 +578,7 @@ __device__ void convert_lockstep_block(int lane, int j, const tc_cblk_desc& d, u
 #pragma unroll
                 for (int u = 0; u < TU; u++) {
                     const int t = t0 + u < nlog ? t0 + u : nlog - 1;  // clamped: duplicate loads are harmless
-                    r[q][u] = lg[((size_t)t * MAXH + yc) * kLaneCount + j];
+                    r[q][u] = lg[((size_t)t * MAXH + yc) * bpw + j];
                 }
             }
 #pragma unroll
@@ -642,9 +643,11 @@ __global__ __launch_bounds__(64) void t1_lockstep_kernel(const tc_cblk_desc* __r
         atomicMax(&L.hmax, (int)d.h);
     }
     __syncthreads();
-    M* const g = scratch + (size_t)blockIdx.x * (4 + nlog_max) * MAXH * kLaneCount;
-    M* const lg = g + 4 * MAXH * kLaneCount;
-    decode_lockstep<M, MAXH>(L, d, active, lane, reinterpret_cast<const uint32_t*>(arena), arena_words, coef, g, lg);
+    // lanes past bpw hold no block: their state rows are never touched
+    M* const g = scratch + (size_t)blockIdx.x * (4 + nlog_max) * MAXH * bpw;
+    M* const lg = g + 4 * MAXH * bpw;
+    decode_lockstep<M, MAXH>(L, d, active, lane, reinterpret_cast<const uint32_t*>(arena), arena_words, coef, g, lg,
+                             bpw);
 
     // make every lane's stores visible, and drop stale L1 lines, before the
     // cooperative read-back
@@ -652,7 +655,7 @@ __global__ __launch_bounds__(64) void t1_lockstep_kernel(const tc_cblk_desc* __r
     __syncthreads();
     for (int j = 0; j < bpw && base + j < n; j++) {
         const tc_cblk_desc dj = descs[order ? order[base + j] : (uint32_t)(base + j)];
-        convert_lockstep_block<M, MAXH>(lane, j, dj, coef, lg);
+        convert_lockstep_block<M, MAXH>(lane, j, dj, coef, lg, bpw);
     }
 }
 
@@ -672,11 +675,11 @@ static hipError_t launch_lockstep_variant(hipStream_t s, const tc_cblk_desc* des
 // Scratch bytes the lockstep kernel needs for n blocks of one size class with
 // at most max_passes coding passes each, bpw blocks per wave: per wave, four
 // state masks and one refinement log per refinement pass, each MAXH rows of
-// 64 lanes.
+// bpw entries.
 size_t t1_lockstep_scratch_bytes(bool cls32, int n, int max_passes, int bpw) {
     const size_t waves = (size_t)((n + bpw - 1) / bpw);
     const size_t rows = (size_t)(4 + max_passes / 3);
-    return cls32 ? waves * rows * 32 * kLaneCount * sizeof(uint32_t) : waves * rows * 64 * kLaneCount * sizeof(uint64_t);
+    return cls32 ? waves * rows * 32 * (size_t)bpw * sizeof(uint32_t) : waves * rows * 64 * (size_t)bpw * sizeof(uint64_t);
 }
 
 hipError_t launch_t1_lockstep(hipStream_t s, bool cls32, int bpw, const tc_cblk_desc* descs, const uint32_t* order,
