@@ -361,7 +361,8 @@ def profile_mode(tca, args):
     L.tc_buf_free(b)
     print(json.dumps({"profile": args.workload, "steps": args.steps, "copy_bytes": n,
                       "blocks_per_wave": {"small": s.bpw_small, "large": s.bpw_large},
-                      "t1_class_bytes": list(s.t1_class_bytes)}), flush=True)
+                      "t1_launches": [{"blocks_per_wave": s.t1_launch_bpw[g], "blocks": s.t1_launch_blocks[g],
+                                       "bytes": s.t1_launch_bytes[g]} for g in range(s.t1_launches)]}), flush=True)
 
 
 def bench_cfg3(tca, steps, args):

@@ -283,7 +283,7 @@ assert CBLK_DESC.itemsize == 32
 class Tuning(ctypes.Structure):
     _fields_ = [("bpw_small", ctypes.c_int32), ("bpw_large", ctypes.c_int32), ("t1_kernel", ctypes.c_int32),
                 ("dwt_shape", ctypes.c_int32), ("t1_ms", ctypes.c_float), ("dwt_ms", ctypes.c_float),
-                ("heavy_small", ctypes.c_int32), ("heavy_large", ctypes.c_int32)]
+                ("heavy_small", ctypes.c_int32), ("heavy_large", ctypes.c_int32), ("heavy_waves", ctypes.c_int32)]
 
     def as_dict(self):
         return {k: (round(getattr(self, k), 4) if k.endswith("_ms") else getattr(self, k)) for k, _ in self._fields_}

@@ -286,6 +286,7 @@ void to_tuning(const tc_tuning& in, Tuning& t) {
     t.dwt_shape = in.dwt_shape;
     t.heavy32 = in.heavy_small;
     t.heavy64 = in.heavy_large;
+    t.heavy_waves_per_cu = in.heavy_waves > 0 ? in.heavy_waves : 2;
 }
 
 }  // namespace
@@ -300,6 +301,7 @@ tc_status tc_decoder_get_tuning(tc_decoder d, tc_tuning* out) {
     out->dwt_shape = t.dwt_shape;
     out->heavy_small = t.heavy32;
     out->heavy_large = t.heavy64;
+    out->heavy_waves = t.heavy_waves_per_cu;
     return TC_SUCCESS;
 }
 
@@ -307,7 +309,8 @@ tc_status tc_decoder_set_tuning(tc_decoder d, const tc_tuning* in) {
     int nshape = 0;
     tc_idwt_tile_shape(0, nullptr, nullptr, &nshape);
     if (!d || !in || !valid_bpw(in->bpw_small) || !valid_bpw(in->bpw_large) || in->t1_kernel < 0 || in->t1_kernel > 1 ||
-        in->dwt_shape < 0 || in->dwt_shape >= nshape || in->heavy_small < -1 || in->heavy_large < -1)
+        in->dwt_shape < 0 || in->dwt_shape >= nshape || in->heavy_small < -1 || in->heavy_large < -1 ||
+        in->heavy_waves < 0 || in->heavy_waves > 32)
         return report(Status::InvalidArgument, "tc_decoder_set_tuning: value out of range");
     Tuning t = d->pipe->tuning();
     to_tuning(*in, t);

@@ -145,6 +145,7 @@ Status build_plan(const std::vector<const Image*>& imgs, const Tuning& tuning, i
     P = Plan();
     P.shape = tuning.dwt_shape;
     P.t1_kernel = tuning.t1_kernel;
+    P.heavy_waves_per_cu = std::max(1, tuning.heavy_waves_per_cu);
     const int TX = idwt_tile_w(P.shape), TY = idwt_tile_h(P.shape);
 
     // ---- sample arena layout: coefficient planes, then work buffers ----
@@ -488,8 +489,10 @@ tc_status launch_plan(const Plan& P, const PlanRegions& R, int cus, tc_stream st
                                              (const uint32_t*)(up + g.order_off), g.n, g.max_passes, up, P.cw_size,
                                              R.scratch + g.scratch_off, R.scratch_bytes - g.scratch_off, base);
         else if (g.bpw == 1 && P.t1_kernel == 0 && g.max_planes <= TC_T1_STRIPE_MAX_PLANES(g.small))
-            s = tc_launch_t1_decode_stripe(st, g.small ? 1 : 0, std::min(g.n, cus * 32), up + P.descs_off,
-                                           (const uint32_t*)(up + g.order_off), g.n, up, P.cw_size, base);
+            s = tc_launch_t1_decode_stripe(st, g.small ? 1 : 0,
+                                           std::min(g.n, cus * (g.heavy ? P.heavy_waves_per_cu : 32)),
+                                           up + P.descs_off, (const uint32_t*)(up + g.order_off), g.n, up, P.cw_size,
+                                           base);
         else if (g.bpw == 1)
             s = tc_launch_t1_decode_wave(st, g.small ? 1 : 0, std::min(g.n, cus * 16), up + P.descs_off,
                                          (const uint32_t*)(up + g.order_off), g.n, up, P.cw_size, base);

@@ -38,6 +38,10 @@ struct Tuning {
     // waves decode a long symbol chain several times faster than a lockstep
     // lane does (0: no split; -1: the default rule, default_heavy).
     int heavy64 = -1, heavy32 = -1;
+    // waves per compute unit of the heavy blocks' launch (each wave decodes
+    // its share of them one after the other), so that they leave room on
+    // every compute unit for the lockstep waves
+    int heavy_waves_per_cu = 2;
 };
 
 // Estimated MQ symbols of a code-block from its descriptor: a least-squares
@@ -67,6 +71,7 @@ struct StageTimes {
 // The device work of a batch: launches, their descriptor offsets in the upload
 // region, and the sizes of the four device regions it needs.
 struct Plan {
+    int heavy_waves_per_cu = 2;
     struct Group {  // one t1 launch
         bool small = false;
         int bpw = 1;
