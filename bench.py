@@ -144,8 +144,6 @@ def t1_kernel_name(bpw, small, args):
     """The tier-1 kernel a launch runs, as rocprof names it."""
     if bpw < 0:
         return "t1_lockstep_kernel<unsigned int, 32>" if small else "t1_lockstep_kernel<unsigned long, 64>"
-    if bpw > 1:
-        return f"t1_decode_kernel<{bpw}>"
     if args.t1_kernel == 0:
         return "t1_stripe_kernel<unsigned int, 32, 32>" if small else "t1_stripe_kernel<unsigned long, 20, 64>"
     return "t1_wave_kernel"

@@ -17,6 +17,8 @@ class CoefficientCoder : public DeviceKernel {
     explicit CoefficientCoder(KernelInitInfoBase initInfo);
     virtual ~CoefficientCoder(void);
     void decode_tile(type_tile* tile);           // TC/CoefficientCoder.cpp:62-88
+    // packing of the tile's blocks (0 automatic, 1 one wave per block, or
+    // TC_BPW_LOCKSTEP / _LOCKSTEP32 / _LOCKSTEP16 from tc_decode.h)
     void set_blocks_per_wave(int small_blocks, int bpw) { (small_blocks ? bpw32_ : bpw64_) = bpw; }
     float last_kernel_ms() const { return last_ms_; }
 

@@ -106,10 +106,9 @@ tc_status tc_image_cblk_data(const uint8_t *cs, size_t len, int index, uint8_t *
 tc_status tc_decoder_create(tc_device dev, tc_decoder *out);
 tc_status tc_decoder_destroy(tc_decoder dec);
 
-/* Code-block-to-wave packing: blocks per wave (1, 8, 16, 32, 64; 0 = automatic)
- * for blocks up to 32x32 (small_blocks = 1) or up to 64x64 (small_blocks = 0).
- * 1 runs one wavefront per block (tc_launch_t1_decode_stripe / _wave), 8..64 one
- * lane per block (tc_launch_t1_decode), TC_BPW_LOCKSTEP 64 blocks per wavefront
+/* Code-block-to-wave packing for blocks up to 32x32 (small_blocks = 1) or up to
+ * 64x64 (small_blocks = 0): 0 automatic, 1 one wavefront per block
+ * (tc_launch_t1_decode_stripe / _wave), TC_BPW_LOCKSTEP 64 blocks per wavefront
  * in lockstep, TC_BPW_LOCKSTEP32 32 and TC_BPW_LOCKSTEP16 16
  * (tc_launch_t1_decode_lockstep). */
 #define TC_BPW_LOCKSTEP (-64)
@@ -149,7 +148,7 @@ tc_status tc_decoder_set_t1_kernel(tc_decoder dec, int kind);
 
 /* Launch configuration of a decoder (applies from the next prepare). */
 typedef struct {
-    int32_t bpw_small, bpw_large; /* per block class: 0 auto, 1, 8..64, TC_BPW_LOCKSTEP, _LOCKSTEP32 or _LOCKSTEP16 */
+    int32_t bpw_small, bpw_large; /* per block class: 0 auto, 1, TC_BPW_LOCKSTEP, _LOCKSTEP32 or _LOCKSTEP16 */
     int32_t t1_kernel;            /* one-wave-per-block kernel: 0 stripe, 1 row-mask */
     int32_t dwt_shape;            /* inverse-DWT tile shape, see tc_idwt_tile_shape */
     float t1_ms, dwt_ms;          /* stage times the autotuner measured for this choice */

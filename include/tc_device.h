@@ -114,16 +114,12 @@ tc_status tc_graph_destroy(tc_graph g);
 
 /* ---- typed kernel launches (DeviceKernel::enqueue, TC/DeviceKernel.cpp:180-197) ---- */
 
-/* Tier-1 decode of n code-blocks (g_decode, TC/coefficient_coder.cl:778).
- * descs: device array of tc_cblk_desc (thousandthchicken_amd/csrc/tc_kernels.h),
- * order: device array of n indices into descs (packing order) or NULL,
- * arena: codewords, coef: zero-filled coefficient planes (int32 or float).
- * small_blocks: every block fits 32x32; blocks_per_wave: 8, 16, 32 or 64 (one lane per block). */
-tc_status tc_launch_t1_decode(tc_stream s, int small_blocks, int blocks_per_wave, const void *descs,
-                              const uint32_t *order, int n, const uint8_t *arena, void *coef);
-
-/* Same work with one wavefront per code-block (latency-bound launches with few
- * blocks): grid work-groups loop over the n blocks in order.  Codewords must
+/* Tier-1 decode of n code-blocks (g_decode, TC/coefficient_coder.cl:778) with
+ * one wavefront per code-block (latency-bound launches with few blocks): grid
+ * work-groups loop over the n blocks in order.  descs: device array of
+ * tc_cblk_desc (thousandthchicken_amd/csrc/tc_kernels.h), order: device array
+ * of n indices into descs (packing order) or NULL, arena: codewords, coef:
+ * zero-filled coefficient planes (int32 or float).  Codewords must
  * start at 4-byte aligned offsets; arena_bytes bounds every codeword read.
  * small_blocks: every block is at most 32 wide.  Writes every coefficient of each
  * decoded block (no zero-fill needed for them). */
@@ -140,7 +136,7 @@ tc_status tc_launch_t1_decode_wave(tc_stream s, int small_blocks, int grid, cons
 tc_status tc_launch_t1_decode_stripe(tc_stream s, int small_blocks, int grid, const void *descs, const uint32_t *order,
                                      int n, const uint8_t *arena, size_t arena_bytes, void *coef);
 
-/* Same work as tc_launch_t1_decode with blocks_per_wave (64, 32 or 16)
+/* Same work as tc_launch_t1_decode_wave with blocks_per_wave (64, 32 or 16)
  * code-blocks per wavefront decoded in lockstep, one MQ symbol per lane per
  * step (throughput form for large batches; fewer than 64 per wave makes more
  * waves for launches with few blocks per CU).  Blocks that share a wavefront

@@ -6,7 +6,7 @@ import thousandthchicken_amd as tca
 
 def test_library_exports_every_declared_symbol():
     names = tca.exported_symbols()
-    assert "tc_launch_t1_decode" in names and "tc_decoder_decode" in names
+    assert "tc_launch_t1_decode_lockstep" in names and "tc_decoder_decode" in names
     L = ctypes.CDLL(tca.LIB_PATH)
     missing = [n for n in names if not hasattr(L, n)]
     assert not missing, missing

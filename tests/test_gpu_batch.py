@@ -18,7 +18,7 @@ def _streams():
     return [open(os.path.join(GOLDEN, n), "rb").read() for n in names]
 
 
-@pytest.mark.parametrize("bpw", [0, 1, 8, 16, 32, 64, -16])
+@pytest.mark.parametrize("bpw", [0, 1, -64, -32, -16])
 def test_batch_equals_single(gpu_decoder, bpw):
     streams = _streams()
     print(f"batch of {len(streams)} bpw={bpw}", flush=True)
@@ -91,7 +91,7 @@ def test_autotune_keeps_outputs(gpu_decoder):
     before = [dec.fetch(i) for i in range(len(streams))]
     best = dec.autotune(reps=1)
     print("autotune:", best.as_dict(), flush=True)
-    packings = (1, 8, 16, 32, 64, tca.BPW_LOCKSTEP, tca.BPW_LOCKSTEP32, tca.BPW_LOCKSTEP16)
+    packings = (1, tca.BPW_LOCKSTEP, tca.BPW_LOCKSTEP32, tca.BPW_LOCKSTEP16)
     assert best.bpw_small in packings and best.bpw_large in packings
     assert 0 <= best.dwt_shape < 4 and best.t1_ms > 0
     dec.run()

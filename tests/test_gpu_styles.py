@@ -95,7 +95,7 @@ def test_styled_codestreams_fused_and_staged(gpu_decoder, case):
     _check(tca.decode_staged(cs), cs, lossy, src)
 
 
-@pytest.mark.parametrize("bpw", [0, 1, 32])
+@pytest.mark.parametrize("bpw", [0, 1, tca.BPW_LOCKSTEP32])
 def test_styled_and_plain_codestreams_in_one_batch(bpw):
     cases = _streams()
     plain = synth.image(100, 120, 3, 77)

@@ -1,6 +1,7 @@
-"""Every tier-1 kernel (one wave per block, one lane per block, 64 blocks per
-wave in lockstep) through its typed C-ABI launch on random code-blocks of every shape, orientation, bit
-depth and truncation point, against the oracle's tier-1 decode."""
+"""Every tier-1 kernel (one wave per block in stripe-column and row-mask form,
+64 or 16 blocks per wave in lockstep) through its typed C-ABI launch on random
+code-blocks of every shape, orientation, bit depth and truncation point,
+against the oracle's tier-1 decode."""
 import ctypes
 
 import numpy as np
@@ -41,7 +42,7 @@ def _expected(b):
     return np.where(sign.astype(bool), -v, v).astype(np.int32)
 
 
-@pytest.mark.parametrize("kernel", ["stripe", "wave", "lockstep", "lockstep_unaligned", "lockstep16", 8, 16, 32, 64])
+@pytest.mark.parametrize("kernel", ["stripe", "wave", "lockstep", "lockstep_unaligned", "lockstep16"])
 @pytest.mark.parametrize("small", [True, False])
 def test_t1_kernels_on_random_blocks(kernel, small):
     lim = 32 if small else 64
@@ -80,12 +81,9 @@ def test_t1_kernels_on_random_blocks(kernel, small):
             st = L.tc_launch_t1_decode_lockstep(stream, 1 if small else 0, lbpw, bufs[1].ptr, None, len(blocks),
                                                 max_passes, bufs[0].ptr, arena.nbytes, bufs[3].ptr, scratch,
                                                 bufs[2].ptr)
-        elif kernel == "wave":
+        else:
             st = L.tc_launch_t1_decode_wave(stream, 1 if small else 0, 512, bufs[1].ptr, None, len(blocks),
                                             bufs[0].ptr, arena.nbytes, bufs[2].ptr)
-        else:
-            st = L.tc_launch_t1_decode(stream, 1 if small else 0, kernel, bufs[1].ptr, None, len(blocks),
-                                       bufs[0].ptr, bufs[2].ptr)
         tca.check(st, "launch")
         out = bufs[2].download(stream, np.int32, n_out).reshape(len(blocks), 64, pitch)
         for i, b in enumerate(blocks):

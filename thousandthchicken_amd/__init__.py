@@ -133,7 +133,6 @@ def lib():
         "tc_buf_copy_d2h": (i32, [vp, vp, vp, sz, sz]),
         "tc_buf_fill32": (i32, [vp, vp, sz, ctypes.c_uint32, sz]),
         "tc_buf_copy_d2d": (i32, [vp, vp, sz, vp, sz, sz]),
-        "tc_launch_t1_decode": (i32, [vp, i32, i32, vp, vp, i32, vp, vp]),
         "tc_launch_t1_decode_wave": (i32, [vp, i32, i32, vp, vp, i32, vp, sz, vp]),
         "tc_launch_t1_decode_stripe": (i32, [vp, i32, i32, vp, vp, i32, vp, sz, vp]),
         "tc_t1_lockstep_scratch_bytes": (sz, [i32, i32, i32, i32]),

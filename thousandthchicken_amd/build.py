@@ -16,7 +16,6 @@ OBJ = os.path.join(HERE, "_objs")
 
 SOURCES = [
     "device.hip",
-    "kernels/t1_decode.hip",
     "kernels/t1_wave.hip",
     "kernels/t1_stripe.hip",
     "kernels/t1_lockstep.hip",

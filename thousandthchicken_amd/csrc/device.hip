@@ -13,8 +13,6 @@
 #include "tc_internal.hpp"
 
 namespace tc {
-hipError_t launch_t1(hipStream_t s, bool cls32, int g, const tc_cblk_desc* descs, const uint32_t* order, int n,
-                     const uint8_t* arena, uint32_t* coef);
 hipError_t launch_t1_wave(hipStream_t s, bool narrow, int grid, const tc_cblk_desc* descs, const uint32_t* order,
                           int n, const uint8_t* arena, uint32_t arena_bytes, uint32_t* coef);
 hipError_t launch_t1_stripe(hipStream_t s, bool small, int grid, const tc_cblk_desc* descs, const uint32_t* order,
@@ -376,16 +374,6 @@ tc_status tc_graph_destroy(tc_graph g) {
     (void)hipGraphDestroy(g->g);
     delete g;
     return TC_SUCCESS;
-}
-
-tc_status tc_launch_t1_decode(tc_stream s, int small_blocks, int g, const void* descs, const uint32_t* order, int n,
-                              const uint8_t* arena, void* coef) {
-    if (!s || (n > 0 && (!descs || !arena || !coef)) || n < 0) return fail(TC_ERR_INVALID_ARG, "tc_launch_t1_decode");
-    if (g != 8 && g != 16 && g != 32 && g != 64)
-        return fail(TC_ERR_INVALID_ARG, "tc_launch_t1_decode: blocks_per_wave must be 8, 16, 32 or 64");
-    return hip_fail(launch_t1(s->s, small_blocks != 0, g, (const tc_cblk_desc*)descs, order, n, arena,
-                              (uint32_t*)coef),
-                    "t1_decode launch");
 }
 
 tc_status tc_launch_t1_decode_wave(tc_stream s, int small_blocks, int grid, const void* descs, const uint32_t* order,

@@ -227,15 +227,14 @@ tc_status tc_decoder_destroy(tc_decoder d) {
 }
 
 static bool valid_bpw(int b) {
-    return b == 0 || b == 1 || b == 8 || b == 16 || b == 32 || b == 64 || b == TC_BPW_LOCKSTEP ||
-           b == TC_BPW_LOCKSTEP32 || b == TC_BPW_LOCKSTEP16;
+    return b == 0 || b == 1 || b == TC_BPW_LOCKSTEP || b == TC_BPW_LOCKSTEP32 || b == TC_BPW_LOCKSTEP16;
 }
 
 tc_status tc_decoder_set_blocks_per_wave(tc_decoder d, int small, int bpw) {
     if (!d || !valid_bpw(bpw))
         return report(Status::InvalidArgument,
-                      "blocks per wave must be 0 (auto), 1 (one wave per block), 8, 16, 32, 64, TC_BPW_LOCKSTEP, "
-                      "TC_BPW_LOCKSTEP32 or TC_BPW_LOCKSTEP16");
+                      "blocks per wave must be 0 (auto), 1 (one wave per block), TC_BPW_LOCKSTEP, TC_BPW_LOCKSTEP32 "
+                      "or TC_BPW_LOCKSTEP16");
     Tuning t = d->pipe->tuning();
     (small ? t.bpw32 : t.bpw64) = bpw;
     d->pipe->set_tuning(t);
@@ -362,7 +361,7 @@ tc_status tc_decoder_autotune(tc_decoder d, int reps, tc_tuning* best) {
     float t1 = 0, dw = 0, best_t1, best_dwt;
     Status s = measure(cur, &best_t1, &best_dwt);
     if (s != Status::Ok) return fail(s);
-    const int cand[] = {1, 8, 16, 32, 64, TC_BPW_LOCKSTEP, TC_BPW_LOCKSTEP32, TC_BPW_LOCKSTEP16};
+    const int cand[] = {1, TC_BPW_LOCKSTEP, TC_BPW_LOCKSTEP32, TC_BPW_LOCKSTEP16};
     for (int cls = 0; cls < 2; cls++) {
         if (!has[cls]) continue;
         for (int b : cand) {

@@ -493,12 +493,9 @@ tc_status launch_plan(const Plan& P, const PlanRegions& R, int cus, tc_stream st
                                            std::min(g.n, cus * (g.heavy ? P.heavy_waves_per_cu : 32)),
                                            up + P.descs_off, (const uint32_t*)(up + g.order_off), g.n, up, P.cw_size,
                                            base);
-        else if (g.bpw == 1)
+        else
             s = tc_launch_t1_decode_wave(st, g.small ? 1 : 0, std::min(g.n, cus * 16), up + P.descs_off,
                                          (const uint32_t*)(up + g.order_off), g.n, up, P.cw_size, base);
-        else
-            s = tc_launch_t1_decode(st, g.small ? 1 : 0, g.bpw, up + P.descs_off, (const uint32_t*)(up + g.order_off),
-                                    g.n, up, base);
         if (gev) tc_event_record(gev[1], st);
     }
     for (int g = 1; fork && s == TC_SUCCESS && g < ng; g++) {

@@ -19,6 +19,7 @@
 #include "../../../include/tc/DWTForward97.h"
 #include "codestream.hpp"
 #include "forward.hpp"
+#include "pipeline.hpp"
 #include "plan_util.hpp"
 
 namespace tc {
@@ -66,7 +67,7 @@ struct ImageState {
 // ---------------------------------------------------------------- CoefficientCoder
 
 CoefficientCoder::CoefficientCoder(KernelInitInfoBase initInfo)
-    : DeviceKernel(KernelInitInfo(initInfo, "t1_decode.hip", "t1_decode")) {}
+    : DeviceKernel(KernelInitInfo(initInfo, "t1_lockstep.hip", "t1_decode")) {}
 
 CoefficientCoder::~CoefficientCoder(void) {
     tc_buf_free(d_upload_);
@@ -132,30 +133,30 @@ void CoefficientCoder::decode_tile(type_tile* tile) {
     for (int k = 0; k < 2; k++) {
         if (cls[k].empty()) continue;
         int n = (int)cls[k].size();
-        int fixed = k == 0 ? bpw32_ : bpw64_;
-        int per = n / std::max(cus, 1), g = 1;
-        for (int c : {8, 16, 32, 64})
-            if (per >= c) g = c;
-        if (fixed) g = fixed;
+        // the fused path's rule: lockstep from 64 blocks per CU up, else one wave per block
+        Tuning tn;
+        tn.bpw32 = bpw32_;
+        tn.bpw64 = bpw64_;
+        const int g = pick_blocks_per_wave(n, k == 0, cus, tn);
         const uint32_t* order = (const uint32_t*)(up + (k == 0 ? off_o0 : off_o1));
         int style = 0, max_passes = 0;
         for (uint32_t i : cls[k]) {
             style |= descs[i].mode & TC_MODE_STYLE;
             max_passes = std::max<int>(max_passes, descs[i].npasses);
         }
-        if (style) {
-            // only the lockstep kernel decodes the code-block styles
-            const size_t sb = tc_t1_lockstep_scratch_bytes(k == 0, 32, n, max_passes);
+        if (style || g < 0) {
+            // lockstep (only it decodes the code-block styles), -g blocks per wave
+            const int bpw = g < 0 ? -g : 32;
+            const size_t sb = tc_t1_lockstep_scratch_bytes(k == 0, bpw, n, max_passes);
             ensure(queue_, S->dev, d_scratch_, sb);
-            check(tc_launch_t1_decode_lockstep(queue_, k == 0, 32, up + off_desc, order, n, max_passes, up, cw_bytes,
+            check(tc_launch_t1_decode_lockstep(queue_, k == 0, bpw, up + off_desc, order, n, max_passes, up, cw_bytes,
                                                tc_buf_device_ptr(d_scratch_), tc_buf_size(d_scratch_), base),
                   "t1 launch");
-        } else if (g == 1)
+        } else {
             check(tc_launch_t1_decode_wave(queue_, k == 0, std::min(n, cus * 16), up + off_desc, order, n, up, cw_bytes,
                                            base),
                   "t1 launch");
-        else
-            check(tc_launch_t1_decode(queue_, k == 0, g, up + off_desc, order, n, up, base), "t1 launch");
+        }
     }
     tc_event_record(e1, queue_);
     tc_event_sync(e1);

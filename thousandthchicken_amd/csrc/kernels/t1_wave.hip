@@ -2,7 +2,7 @@
 // code-blocks (one image at a time), where the serial MQ symbol chain of each
 // block bounds the run time.
 //
-// Reference behaviour as in t1_decode.hip (g_decode, TC/coefficient_coder.cl:
+// Reference behaviour as in t1_lockstep.hip (g_decode, TC/coefficient_coder.cl:
 // 778-819 with the repaired setters and the pass count honoured).
 //
 // Mapping.  The MQ registers, the position in the block and every decision
