@@ -179,6 +179,10 @@ def tuned_decoder(tca, args):
     dec.set_blocks_per_wave(False, args.bpw_large)
     dec.set_blocks_per_wave(True, args.bpw_small)
     dec.set_t1_kernel(args.t1_kernel)
+    if args.dwt_shape >= 0:
+        t = dec.tuning()
+        t.dwt_shape = args.dwt_shape
+        dec.set_tuning(t)
     return dec
 
 
@@ -455,6 +459,7 @@ def main():
     ap.add_argument("--bpw-large", type=int, default=0, help="blocks per wave, blocks up to 64x64 (0 auto, -64/-32/-16 lockstep)")
     ap.add_argument("--bpw-small", type=int, default=0, help="blocks per wave, blocks up to 32x32 (0 auto, -64/-32/-16 lockstep)")
     ap.add_argument("--t1-kernel", type=int, default=0, help="one-wave-per-block kernel: 0 stripe-column, 1 row-mask")
+    ap.add_argument("--dwt-shape", type=int, default=-1, help="inverse-DWT tile shape (tc_idwt_tile_shape; -1 default)")
     ap.add_argument("--no-autotune", action="store_true", help="keep the default packing and DWT tile shape")
     args = ap.parse_args()
     if args.gpus != 1:

@@ -215,6 +215,7 @@ Status build_plan(const std::vector<const Image*>& imgs, const Tuning& tuning, i
             g.max_planes = std::max<int>(g.max_planes, descs[i].numbps);
             g.max_passes = std::max<int>(g.max_passes, descs[i].npasses);
             g.bytes += (int64_t)descs[i].data_len + 4 * (int64_t)descs[i].w * descs[i].h;
+            g.max_symbols = std::max(g.max_symbols, estimated_symbols(descs[i]));
         }
         g.order_off = put(blob, v);
         if (g.bpw < 0) {  // lockstep, -bpw blocks per wave
@@ -261,7 +262,13 @@ Status build_plan(const std::vector<const Image*>& imgs, const Tuning& tuning, i
         P.heavy[k] = heavy.empty() ? 0 : thr;
         P.styled[k] = style != 0;
     }
-    // heavy launches go first, so that their waves are dispatched first
+    // Heavy launches go first, then the class whose longest block chain is
+    // longest: launches dispatch their waves in submission order, and the
+    // waves resident first are the ones that bound the tier-1 stage (on the
+    // 256-image mixed batch, the 64x64 class first takes the stage from 35.3
+    // to 32.8 ms; its waves otherwise wait for the 32x32 class's to leave).
+    std::stable_sort(main_groups.begin(), main_groups.end(),
+                     [](const Plan::Group& a, const Plan::Group& b) { return a.max_symbols > b.max_symbols; });
     for (auto& g : heavy_groups) P.groups.push_back(g);
     for (auto& g : main_groups) P.groups.push_back(g);
 

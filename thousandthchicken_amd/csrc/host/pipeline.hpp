@@ -82,6 +82,7 @@ struct Plan {
         int n = 0;
         bool heavy = false;  // the class's heaviest blocks, split off its lockstep launch
         int64_t bytes = 0;   // algorithmic bytes: codeword bytes + 4 B per coefficient
+        int64_t max_symbols = 0;  // largest estimated_symbols of a block in the launch
     };
     struct LevelLaunch {
         bool irr = false;

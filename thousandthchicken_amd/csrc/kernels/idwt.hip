@@ -13,9 +13,13 @@
 // the batched form many images).  Each 256-thread work-group owns a TXxTY
 // output tile: it gathers the (TX+2H)x(TY+2H) interleaved input straight from
 // the four subbands (mirroring at the resolution borders, K-scaling folded into
-// the load), lifts every row, then every column, in LDS, and writes the tile
-// once with coalesced rows.  HBM traffic is one read of the subbands and one
-// write of the output per level; the LL copy-back of the reference
+// the load) into LDS.  Lifting runs on every thread: a row is split into
+// segments of 8 outputs, each lifted in registers together with the halo it
+// shares with its neighbours (the halo is computed twice, which costs less
+// than a barrier per lifting step) and written back in place; then each
+// thread lifts one segment of one column the same way and writes its outputs
+// straight to HBM, a wave covering 64 consecutive columns.  HBM traffic is one
+// read of the subbands and one write of the output per level; the LL copy-back of the reference
 // (TC/DWTKernel.cpp:127-154) does not exist.  The final level also applies the
 // inverse RCT/ICT, the DC level shift, rounding and clamping, and packs the
 // interleaved 8/16-bit output (the work of TC/Preprocessor.cpp:44-157).
@@ -94,73 +98,23 @@ struct Lift;
 
 template <>
 struct Lift<false> {
-    static constexpr int H = 2;
+    static constexpr int H = 2;  // halo: one sample per lifting step
     typedef int32_t T;
-    // 1D_SR 5/3 over the LDS line a[0..n) whose element 0 has parity par0.
-    __device__ static void steps(int32_t* base, int stride_pos, int stride_line, int nlines, int n, int par0,
-                                 int tid) {
-        int half = (n + 1) / 2;
-        // STEP1: even -= floor((odd_l + odd_r + 2) / 4)
-        for (int i = tid; i < nlines * half; i += NT) {
-            int line = i % nlines, j = i / nlines;
-            int pos = 2 * j + (par0 & 1);  // absolute-even positions
-            if (pos >= 1 && pos < n - 1) {
-                int32_t* p = base + line * stride_line + pos * stride_pos;
-                p[0] -= (p[-stride_pos] + p[stride_pos] + 2) >> 2;
-            }
-        }
-        __syncthreads();
-        // STEP2: odd += floor((even_l + even_r) / 2)
-        for (int i = tid; i < nlines * half; i += NT) {
-            int line = i % nlines, j = i / nlines;
-            int pos = 2 * j + 1 - (par0 & 1);
-            if (pos >= 2 && pos < n - 2) {
-                int32_t* p = base + line * stride_line + pos * stride_pos;
-                p[0] += (p[-stride_pos] + p[stride_pos]) >> 1;
-            }
-        }
-        __syncthreads();
-    }
 };
 
 template <>
 struct Lift<true> {
     static constexpr int H = 4;
     typedef float T;
-    __device__ static void step(float* base, int stride_pos, int stride_line, int nlines, int n, int par,
-                                int lo, float c, int tid) {
-        int half = (n + 1) / 2;
-        for (int i = tid; i < nlines * half; i += NT) {
-            int line = i % nlines, j = i / nlines;
-            int pos = 2 * j + par;
-            if (pos >= lo && pos < n - lo) {
-                float* p = base + line * stride_line + pos * stride_pos;
-                p[0] = fmaf(-c, p[-stride_pos] + p[stride_pos], p[0]);
-            }
-        }
-        __syncthreads();
-    }
-    // STEP3..6 of 1D_SR 9/7 (the K scaling of STEP1/2 is applied at load).
-    __device__ static void steps(float* base, int stride_pos, int stride_line, int nlines, int n, int par0,
-                                 int tid) {
-        int pe = par0 & 1, po = 1 - pe;  // first even / odd position index
-        step(base, stride_pos, stride_line, nlines, n, pe, 1, kDelta, tid);
-        step(base, stride_pos, stride_line, nlines, n, po, 2, kGamma, tid);
-        step(base, stride_pos, stride_line, nlines, n, pe, 3, kBeta, tid);
-        step(base, stride_pos, stride_line, nlines, n, po, 4, kAlpha, tid);
-    }
 };
 
-// One line of N samples lifted in registers: 1D_SR STEP3..6 for 9/7 (the K
-// scaling is applied at load), STEP1..2 for 5/3, over the same ranges as the
-// LDS sweeps above (step s updates positions [s, N - s) of its parity).  P:
-// parity of position 0.  Read from and written back to LDS at `stride`.
+// 1D_SR over N samples in registers, x[0] at a position of parity P:
+// STEP3..6 for 9/7 (the K scaling is applied at load), STEP1..2 for 5/3.
+// Step s updates the positions [s, N - s) of its parity, so with N = K + 2H
+// the middle K samples come out exact and the H on either side are the halo
+// the segment shares with its neighbours.
 template <bool IS97, int N, int P>
-__device__ __forceinline__ void lift_line_inv(typename Lift<IS97>::T* p, int stride) {
-    typedef typename Lift<IS97>::T T;
-    T x[N];
-#pragma unroll
-    for (int i = 0; i < N; i++) x[i] = p[i * stride];
+__device__ __forceinline__ void lift_regs(typename Lift<IS97>::T (&x)[N]) {
     if (IS97) {
         const float k[4] = {kDelta, kGamma, kBeta, kAlpha};
 #pragma unroll
@@ -176,124 +130,154 @@ __device__ __forceinline__ void lift_line_inv(typename Lift<IS97>::T* p, int str
         for (int i = 2; i < N - 2; i++)
             if ((i + P) & 1) x[i] += ((int)x[i - 1] + (int)x[i + 1]) >> 1;
     }
-#pragma unroll
-    for (int i = 0; i < N; i++) p[i * stride] = x[i];
 }
 
-// Lifts NC loaded tiles (tile c at tile + c * cs): every row, then every
-// column, one line per thread in registers, unless an axis has length 1.
-template <bool IS97, int TX, int TY, int NC>
-__device__ __forceinline__ void tile_lift_inv(typename Lift<IS97>::T* tile, int pitch, int ox, int oy, bool h1,
-                                              bool v1x, int cs) {
-    constexpr int H = Lift<IS97>::H;
-    constexpr int W = TX + 2 * H, HH = TY + 2 * H;
-    const int tid = threadIdx.x;
-    if (!h1) {
-        const bool odd = ((ox - H) & 1) != 0;
-        for (int L = tid; L < NC * HH; L += NT) {
-            typename Lift<IS97>::T* p = tile + (L / HH) * cs + (L % HH) * pitch;
-            if (odd)
-                lift_line_inv<IS97, W, 1>(p, 1);
-            else
-                lift_line_inv<IS97, W, 0>(p, 1);
-        }
-        __syncthreads();
-    }
-    if (!v1x) {
-        const bool odd = ((oy - H) & 1) != 0;
-        for (int L = tid; L < NC * W; L += NT) {
-            typename Lift<IS97>::T* p = tile + (L / W) * cs + (L % W);
-            if (odd)
-                lift_line_inv<IS97, HH, 1>(p, pitch);
-            else
-                lift_line_inv<IS97, HH, 0>(p, pitch);
-        }
-        __syncthreads();
-    }
+template <bool IS97, int N>
+__device__ __forceinline__ void lift_regs(typename Lift<IS97>::T (&x)[N], int par) {
+    if (par)
+        lift_regs<IS97, N, 1>(x);
+    else
+        lift_regs<IS97, N, 0>(x);
 }
 
-// Loads the extended tile and, with `lift`, lifts rows then columns.  Returns
-// with the reconstructed samples at tile[H+ty][H+tx].
+// Segment lengths: a row of the window is lifted as TX/KR segments of KR
+// outputs, and the columns as TX * TY/KC segments of KC outputs, at most one
+// per thread.  At least 8 outputs per segment keep the recomputed halo (2H
+// samples) from outweighing the work.
+constexpr int kRowSeg = 8;
+__host__ __device__ constexpr int col_seg(int tx, int ty) { return tx * ty / NT > 8 ? tx * ty / NT : 8; }
+
+// Mirror of i into [i0, i1) for a window reaching at most one halo past the
+// ends (the length exceeds the halo); indices further out, which only feed
+// samples outside the rectangle, are clamped.
+__device__ __forceinline__ int reflect1(int i, int i0, int i1) {
+    int m = i < i0 ? 2 * i0 - i : (i >= i1 ? 2 * (i1 - 1) - i : i);
+    return m < i0 ? i0 : (m >= i1 ? i1 - 1 : m);
+}
+
+// Loads the extended (TX + 2H) x (TY + 2H) window of one tile-component into
+// LDS (pitch `pitch`): every sample straight from its subband, mirrored at the
+// resolution borders, with the 9/7 K scaling folded in.  Thread t owns window
+// column t % W and the rows t / W + R * m (R = NT / W), so the horizontal
+// mirroring, band, source column and scale are worked out once per thread and
+// each sample costs only its row's part of the address.  All of a thread's
+// loads are issued before the first is used.
 template <bool IS97, int TX, int TY>
-__device__ void tile_idwt(typename Lift<IS97>::T* tile, int pitch, const typename Lift<IS97>::T* __restrict__ src,
-                          const Src& s_in, int u0, int u1, int v0, int v1, int ox, int oy, bool copy_only,
-                          bool lift = true) {
+__device__ __forceinline__ void tile_load(typename Lift<IS97>::T* tile, int pitch,
+                                          const typename Lift<IS97>::T* __restrict__ src, const Src& s_in, int u0,
+                                          int u1, int v0, int v1, int ox, int oy, bool copy_only) {
     typedef typename Lift<IS97>::T T;
     constexpr int H = Lift<IS97>::H;
     constexpr int W = TX + 2 * H, HH = TY + 2 * H;
+    constexpr int R = NT / W;                  // rows in flight per pass over the window
+    constexpr int MR = (HH + R - 1) / R;       // rows per thread
+    static_assert(R >= 1, "window wider than the work-group");
     const int tid = threadIdx.x;
+    if (tid >= R * W) return;
+    const int lx = tid % W, ty = tid / W;
     const bool h1 = (u1 - u0) == 1, v1x = (v1 - v0) == 1;
     const uint64_t bll = uni64(s_in.ll), bhl = uni64(s_in.hl), blh = uni64(s_in.lh), bhh = uni64(s_in.hh);
     const uint32_t llp = (uint32_t)__builtin_amdgcn_readfirstlane((int)s_in.llp);
     const uint32_t bp = (uint32_t)__builtin_amdgcn_readfirstlane((int)s_in.bp);
-    const Src s{bll, bhl, blh, bhh, llp, bp};
-    // interior tile: the extended window needs no mirroring, and every load of
-    // a thread is issued before the first is used
-    if (!copy_only && ox - H >= u0 && ox + TX + H <= u1 && oy - H >= v0 && oy + TY + H <= v1) {
-        constexpr int NIT = (W * HH + NT - 1) / NT;
-        const int cu0 = ceil_half(u0), fu0 = floor_half(u0), cv0 = ceil_half(v0), fv0 = floor_half(v0);
-        T vals[NIT];
-#pragma unroll
-        for (int k = 0; k < NIT; k++) {
-            const int i = tid + k * NT;
-            if (i < W * HH) {
-                const int ly = i / W, lx = i % W;
-                const int u = ox - H + lx, v = oy - H + ly;
-                const int ou = u & 1, ov = v & 1;
-                const int cx = ou ? floor_half(u) - fu0 : (u >> 1) - cu0;
-                const int cy = ov ? floor_half(v) - fv0 : (v >> 1) - cv0;
-                const uint64_t b = ov ? (ou ? bhh : blh) : (ou ? bhl : bll);
-                const uint32_t bpitch = (ou | ov) ? bp : llp;
-                vals[k] = src[b + (uint64_t)cy * bpitch + cx];
-            }
-        }
-#pragma unroll
-        for (int k = 0; k < NIT; k++) {
-            const int i = tid + k * NT;
-            if (i < W * HH) {
-                const int ly = i / W, lx = i % W;
-                T val = vals[k];
-                if (IS97) {
-                    const int ou = (ox - H + lx) & 1, ov = (oy - H + ly) & 1;
-                    float f = (float)val;
-                    f *= ou ? kInvK : kK;
-                    f *= ov ? kInvK : kK;
-                    val = (T)f;
-                }
-                tile[ly * pitch + lx] = val;
-            }
-        }
-        __syncthreads();
-        if (lift) tile_lift_inv<IS97, TX, TY, 1>(tile, pitch, ox, oy, false, false, 0);
-        return;
+    // 0: no mirroring needed; 1: one reflection at most; 2: short signals (general PSE)
+    const bool interior = ox - H >= u0 && ox + TX + H <= u1 && oy - H >= v0 && oy + TY + H <= v1;
+    const int mode = interior ? 0 : ((u1 - u0 > H && v1 - v0 > H) ? 1 : 2);
+    auto mirror = [&](int i, int i0, int i1, bool one) -> int {
+        if (one) return i0;
+        return mode == 0 ? i : (mode == 1 ? reflect1(i, i0, i1) : pse(i, i0, i1));
+    };
+    // this thread's column: its band side, source column and base addresses
+    const int u = ox - H + lx;
+    const int mu = copy_only ? (u < u0 ? u0 : (u >= u1 ? u1 - 1 : u)) : mirror(u, u0, u1, h1);
+    const int ou = copy_only ? 0 : (mu & 1);
+    const int cx = copy_only ? mu - u0 : (ou ? floor_half(mu) - floor_half(u0) : (mu >> 1) - ceil_half(u0));
+    const uint64_t col_lo = (ou ? bhl : bll) + (uint64_t)(int64_t)cx;  // rows of even v
+    const uint64_t col_hi = (ou ? bhh : blh) + (uint64_t)(int64_t)cx;  // rows of odd v
+    const uint32_t pitch_lo = ou ? bp : llp;
+    const int cv0 = ceil_half(v0), fv0 = floor_half(v0);
+    float sc_lo = 1.0f, sc_hi = 1.0f;  // K scaling of the two vertical bands
+    if (IS97 && !copy_only) {
+        const float hs = h1 ? 1.0f : (ou ? kInvK : kK);
+        sc_lo = v1x ? hs : hs * kK;
+        sc_hi = v1x ? hs : hs * kInvK;
     }
-    for (int i = tid; i < W * HH; i += NT) {
-        int ly = i / W, lx = i % W;
-        int u = ox - H + lx, v = oy - H + ly;
-        T val;
-        if (copy_only) {
-            int cu = u < u0 ? u0 : (u >= u1 ? u1 - 1 : u);
-            int cv = v < v0 ? v0 : (v >= v1 ? v1 - 1 : v);
-            val = src[s.ll + (uint64_t)(cv - v0) * s.llp + (cu - u0)];
-        } else {
-            int mu = h1 ? u0 : pse(u, u0, u1);
-            int mv = v1x ? v0 : pse(v, v0, v1);
-            val = src[src_index(s, mu, mv, u0, v0)];
-            if (IS97) {
-                float f = (float)val;
-                if (!h1) f *= (mu & 1) ? kInvK : kK;
-                if (!v1x) f *= (mv & 1) ? kInvK : kK;
-                val = (T)f;
+    T vals[MR];
+#pragma unroll
+    for (int m = 0; m < MR; m++) {
+        const int ly = ty + R * m;
+        if (ly < HH) {
+            const int v = oy - H + ly;
+            if (copy_only) {
+                const int cv = v < v0 ? v0 : (v >= v1 ? v1 - 1 : v);
+                vals[m] = src[col_lo + (uint64_t)(cv - v0) * llp];
+            } else {
+                const int mv = mirror(v, v0, v1, v1x);
+                const int ov = mv & 1;
+                const int cy = ov ? floor_half(mv) - fv0 : (mv >> 1) - cv0;
+                vals[m] = src[(ov ? col_hi : col_lo) + (uint64_t)cy * (ov ? bp : pitch_lo)];
             }
-            // length-1 signal at an odd origin: X = Y / 2 (T.800 F.3.7)
-            if (h1 && (u0 & 1)) val = IS97 ? (T)(val * (T)0.5f) : (T)((int32_t)val / 2);
-            if (v1x && (v0 & 1)) val = IS97 ? (T)(val * (T)0.5f) : (T)((int32_t)val / 2);
         }
-        tile[ly * pitch + lx] = val;
+    }
+#pragma unroll
+    for (int m = 0; m < MR; m++) {
+        const int ly = ty + R * m;
+        if (ly < HH) {
+            T val = vals[m];
+            if (!copy_only) {
+                // parity survives the mirroring, so the band follows from v
+                const int ov = (oy - H + ly) & 1;
+                if (IS97) val = (T)((float)val * (ov ? sc_hi : sc_lo));
+                // length-1 signal at an odd origin: X = Y / 2 (T.800 F.3.7)
+                if (h1 && (u0 & 1)) val = IS97 ? (T)(val * (T)0.5f) : (T)((int32_t)val / 2);
+                if (v1x && (v0 & 1)) val = IS97 ? (T)(val * (T)0.5f) : (T)((int32_t)val / 2);
+            }
+            tile[ly * pitch + lx] = val;
+        }
+    }
+}
+
+// Lifts every row of NC loaded windows (window c at tile + c * cs) in place,
+// over the TX output columns: each thread takes a KR-sample segment of a row
+// with its halo into registers.  Rows go in rounds of NT / (TX / KR); within a
+// round every segment is read before any is written, and rounds touch
+// different rows.
+template <bool IS97, int TX, int TY, int NC>
+__device__ __forceinline__ void rows_pass(typename Lift<IS97>::T* tile, int pitch, int cs, int par) {
+    typedef typename Lift<IS97>::T T;
+    constexpr int H = Lift<IS97>::H, HH = TY + 2 * H;
+    constexpr int KR = kRowSeg, SEG = TX / KR, N = KR + 2 * H;
+    constexpr int ROWS = NC * HH, RPR = NT / SEG;
+    static_assert(NT % SEG == 0, "row segments must tile the work-group");
+    const int tid = threadIdx.x;
+    const int seg = tid / RPR, rr = tid % RPR;  // consecutive lanes take consecutive rows (odd pitch: no conflicts)
+#pragma unroll
+    for (int r0 = 0; r0 < ROWS; r0 += RPR) {
+        const int row = r0 + rr < ROWS ? r0 + rr : ROWS - 1;
+        T* const p = tile + (row / HH) * cs + (row % HH) * pitch + seg * KR;
+        T x[N];
+#pragma unroll
+        for (int j = 0; j < N; j++) x[j] = p[j];
+        __syncthreads();
+        if (r0 + rr < ROWS) {
+            lift_regs<IS97, N>(x, par);
+#pragma unroll
+            for (int j = 0; j < KR; j++) p[H + j] = x[H + j];
+        }
     }
     __syncthreads();
-    if (copy_only || !lift) return;
-    tile_lift_inv<IS97, TX, TY, 1>(tile, pitch, ox, oy, h1, v1x, 0);
+}
+
+// The column segment of thread tid of window c after the column pass: KC
+// samples of column H + col, rows H + seg * KC + j, lifted unless v1x.
+template <bool IS97, int TX, int TY>
+__device__ __forceinline__ void column_segment(const typename Lift<IS97>::T* tile, int pitch, int col, int seg,
+                                               bool v1x, int par,
+                                               typename Lift<IS97>::T (&x)[col_seg(TX, TY) + 2 * Lift<IS97>::H]) {
+    constexpr int H = Lift<IS97>::H, KC = col_seg(TX, TY), N = KC + 2 * H;
+    const typename Lift<IS97>::T* const p = tile + seg * KC * pitch + H + col;
+#pragma unroll
+    for (int j = 0; j < N; j++) x[j] = p[j * pitch];
+    if (!v1x) lift_regs<IS97, N>(x, par);
 }
 
 template <bool IS97, int TX, int TY>
@@ -303,18 +287,28 @@ __global__ __launch_bounds__(NT) void idwt_level_kernel(const tc_dwt_job* __rest
     typedef typename Lift<IS97>::T T;
     constexpr int H = Lift<IS97>::H;
     constexpr int PITCH = TX + 2 * H + 1;
+    constexpr int KC = col_seg(TX, TY);
     __shared__ T tile[(TY + 2 * H) * PITCH];
     const uint32_t wg = xcd_run_index();
     const tc_dwt_job& J = jobs[wg_job[wg]];
-    int t = (int)wg - J.wg_base;
-    int ox = J.u0 + (t % J.tiles_x) * TX, oy = J.v0 + (t / J.tiles_x) * TY;
-    Src s{J.ll_off, J.hl_off, J.lh_off, J.hh_off, J.ll_pitch, J.band_pitch};
-    tile_idwt<IS97, TX, TY>(tile, PITCH, base, s, J.u0, J.u1, J.v0, J.v1, ox, oy, false);
-    for (int i = threadIdx.x; i < TX * TY; i += NT) {
-        int ly = i / TX, lx = i % TX;
-        int u = ox + lx, v = oy + ly;
-        if (u < J.u1 && v < J.v1)
-            base[J.out_off + (uint64_t)(v - J.v0) * J.out_pitch + (u - J.u0)] = tile[(ly + H) * PITCH + lx + H];
+    const int t = (int)wg - J.wg_base;
+    const int ox = J.u0 + (t % J.tiles_x) * TX, oy = J.v0 + (t / J.tiles_x) * TY;
+    const Src s{J.ll_off, J.hl_off, J.lh_off, J.hh_off, J.ll_pitch, J.band_pitch};
+    const bool h1 = (J.u1 - J.u0) == 1, v1x = (J.v1 - J.v0) == 1;
+    tile_load<IS97, TX, TY>(tile, PITCH, base, s, J.u0, J.u1, J.v0, J.v1, ox, oy, false);
+    __syncthreads();
+    if (!h1) rows_pass<IS97, TX, TY, 1>(tile, PITCH, 0, (ox - H) & 1);
+    const int col = threadIdx.x % TX, seg = threadIdx.x / TX;
+    if (seg >= TY / KC) return;
+    T x[KC + 2 * H];
+    column_segment<IS97, TX, TY>(tile, PITCH, col, seg, v1x, (oy - H) & 1, x);
+    const int u = ox + col;
+    if (u >= J.u1) return;
+    T* const o = base + J.out_off + (u - J.u0);
+#pragma unroll
+    for (int j = 0; j < KC; j++) {
+        const int v = oy + seg * KC + j;
+        if (v < J.v1) o[(uint64_t)(v - J.v0) * J.out_pitch] = x[H + j];
     }
 }
 
@@ -332,49 +326,59 @@ __global__ __launch_bounds__(NT) void idwt_final_kernel(const tc_final_job* __re
     constexpr int H = Lift<IS97>::H;
     constexpr int PITCH = TX + 2 * H + 1;
     constexpr int TS = (TY + 2 * H) * PITCH;
+    constexpr int KC = col_seg(TX, TY);
     __shared__ T tile[NC * TS];
     const uint32_t wg = xcd_run_index();
     const tc_final_job& J = jobs[wg_job[wg]];
-    int t = (int)wg - J.wg_base;
-    int ox = J.wx0 + (t % J.tiles_x) * TX, oy = J.wy0 + (t / J.tiles_x) * TY;
+    const int t = (int)wg - J.wg_base;
+    const int ox = J.wx0 + (t % J.tiles_x) * TX, oy = J.wy0 + (t / J.tiles_x) * TY;
+    const bool copy = J.levels == 0;
+    const bool h1 = copy || (J.u1 - J.u0) == 1, v1x = copy || (J.v1 - J.v0) == 1;
 #pragma unroll
     for (int c = 0; c < NC; c++) {
-        Src s{J.ll_off[c], J.hl_off[c], J.lh_off[c], J.hh_off[c], J.ll_pitch[c], J.band_pitch[c]};
-        tile_idwt<IS97, TX, TY>(tile + c * TS, PITCH, base, s, J.u0, J.u1, J.v0, J.v1, ox, oy, J.levels == 0, false);
+        const Src s{J.ll_off[c], J.hl_off[c], J.lh_off[c], J.hh_off[c], J.ll_pitch[c], J.band_pitch[c]};
+        tile_load<IS97, TX, TY>(tile + c * TS, PITCH, base, s, J.u0, J.u1, J.v0, J.v1, ox, oy, copy);
     }
-    // every component's rows, then every component's columns, between the same barriers
-    if (J.levels != 0)
-        tile_lift_inv<IS97, TX, TY, NC>(tile, PITCH, ox, oy, (J.u1 - J.u0) == 1, (J.v1 - J.v0) == 1, TS);
     __syncthreads();
-    for (int i = threadIdx.x; i < TX * TY; i += NT) {
-        int ly = i / TX, lx = i % TX;
-        int u = ox + lx, v = oy + ly;
-        if (u >= J.wx1 || v >= J.wy1) continue;
-        int li = (ly + H) * PITCH + lx + H;
-        uint64_t o = J.out_off + (uint64_t)(v - J.wy0) * J.out_pitch + (uint64_t)(u - J.wx0) * J.out_cstride;
+    // every component's rows in the same rounds
+    if (!h1) rows_pass<IS97, TX, TY, NC>(tile, PITCH, TS, (ox - H) & 1);
+    const int col = threadIdx.x % TX, seg = threadIdx.x / TX;
+    if (seg >= TY / KC) return;
+    T x[NC][KC + 2 * H];
+#pragma unroll
+    for (int c = 0; c < NC; c++) column_segment<IS97, TX, TY>(tile + c * TS, PITCH, col, seg, v1x, (oy - H) & 1, x[c]);
+    const int u = ox + col;
+    if (u >= J.wx1) return;
+    O* const orow = outimg + J.out_off + (uint64_t)(u - J.wx0) * J.out_cstride;
+#pragma unroll
+    for (int j = 0; j < KC; j++) {
+        const int v = oy + seg * KC + j;
+        if (v >= J.wy1) break;
+        O* const o = orow + (uint64_t)(v - J.wy0) * J.out_pitch;
         if (NC == 3 && J.mct) {
             if (IS97) {
                 // inverse ICT, T.800 G.3 (not the YUV constants of TC/preprocess_ict_inverse.cl:31-35)
-                float y0 = (float)tile[li], y1 = (float)tile[TS + li], y2 = (float)tile[2 * TS + li];
-                float r = fmaf(1.402f, y2, y0);
-                float g = fmaf(-0.71414f, y2, fmaf(-0.34413f, y1, y0));
-                float b = fmaf(1.772f, y1, y0);
-                outimg[o] = clamp_out<O>((int)rintf(r) + J.shift[0], J.minv[0], J.maxv[0]);
-                outimg[o + 1] = clamp_out<O>((int)rintf(g) + J.shift[1], J.minv[1], J.maxv[1]);
-                outimg[o + 2] = clamp_out<O>((int)rintf(b) + J.shift[2], J.minv[2], J.maxv[2]);
+                const float y0 = (float)x[0][H + j], y1 = (float)x[NC > 1 ? 1 : 0][H + j],
+                            y2 = (float)x[NC > 2 ? 2 : 0][H + j];
+                const float r = fmaf(1.402f, y2, y0);
+                const float g = fmaf(-0.71414f, y2, fmaf(-0.34413f, y1, y0));
+                const float b = fmaf(1.772f, y1, y0);
+                o[0] = clamp_out<O>((int)rintf(r) + J.shift[0], J.minv[0], J.maxv[0]);
+                o[1] = clamp_out<O>((int)rintf(g) + J.shift[1], J.minv[1], J.maxv[1]);
+                o[2] = clamp_out<O>((int)rintf(b) + J.shift[2], J.minv[2], J.maxv[2]);
             } else {
                 // inverse RCT, T.800 G.2 (TC/preprocess_rct_inverse.cl:18-36)
-                int y0 = (int)tile[li], y1 = (int)tile[TS + li], y2 = (int)tile[2 * TS + li];
-                int g = y0 - ((y1 + y2) >> 2);
-                outimg[o] = clamp_out<O>(y2 + g + J.shift[0], J.minv[0], J.maxv[0]);
-                outimg[o + 1] = clamp_out<O>(g + J.shift[1], J.minv[1], J.maxv[1]);
-                outimg[o + 2] = clamp_out<O>(y1 + g + J.shift[2], J.minv[2], J.maxv[2]);
+                const int y0 = (int)x[0][H + j], y1 = (int)x[NC > 1 ? 1 : 0][H + j], y2 = (int)x[NC > 2 ? 2 : 0][H + j];
+                const int g = y0 - ((y1 + y2) >> 2);
+                o[0] = clamp_out<O>(y2 + g + J.shift[0], J.minv[0], J.maxv[0]);
+                o[1] = clamp_out<O>(g + J.shift[1], J.minv[1], J.maxv[1]);
+                o[2] = clamp_out<O>(y1 + g + J.shift[2], J.minv[2], J.maxv[2]);
             }
         } else {
 #pragma unroll
             for (int c = 0; c < NC; c++) {
-                int iv = IS97 ? (int)rintf((float)tile[c * TS + li]) : (int)tile[c * TS + li];
-                outimg[o + c] = clamp_out<O>(iv + J.shift[c], J.minv[c], J.maxv[c]);
+                const int iv = IS97 ? (int)rintf((float)x[c][H + j]) : (int)x[c][H + j];
+                o[c] = clamp_out<O>(iv + J.shift[c], J.minv[c], J.maxv[c]);
             }
         }
     }
