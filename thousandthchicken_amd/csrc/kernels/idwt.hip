@@ -26,6 +26,8 @@
 #include <hip/hip_runtime.h>
 #include <stdint.h>
 
+#include <type_traits>
+
 #include "../tc_kernels.h"
 
 namespace tc {
@@ -132,14 +134,6 @@ __device__ __forceinline__ void lift_regs(typename Lift<IS97>::T (&x)[N]) {
     }
 }
 
-template <bool IS97, int N>
-__device__ __forceinline__ void lift_regs(typename Lift<IS97>::T (&x)[N], int par) {
-    if (par)
-        lift_regs<IS97, N, 1>(x);
-    else
-        lift_regs<IS97, N, 0>(x);
-}
-
 // Segment lengths: a row of the window is lifted as TX/KR segments of KR
 // outputs, and the columns as TX * TY/KC segments of KC outputs, at most one
 // per thread.  At least 8 outputs per segment keep the recomputed halo (2H
@@ -202,6 +196,29 @@ __device__ __forceinline__ void tile_load(typename Lift<IS97>::T* tile, int pitc
         sc_hi = v1x ? hs : hs * kInvK;
     }
     T vals[MR];
+    if (mode == 0 && !copy_only) {
+        // Interior: window rows ty + R*m of one parity are consecutive rows of
+        // one vertical band, so each load is a base plus a fixed multiple of
+        // the band pitch.  Row m = 0 fixes one parity, m = 1 (R odd) the other.
+        const int va = oy - H + ty, qa = va & 1, qb = qa ^ (R & 1);
+        const int vb = va + R;
+        const uint64_t base_a = (qa ? col_hi : col_lo) +
+                                (uint64_t)(int64_t)((qa ? floor_half(va) - fv0 : (va >> 1) - cv0)) * (qa ? bp : pitch_lo);
+        const uint64_t base_b = (qb ? col_hi : col_lo) +
+                                (uint64_t)(int64_t)((qb ? floor_half(vb) - fv0 : (vb >> 1) - cv0)) * (qb ? bp : pitch_lo);
+        const uint64_t pa = qa ? bp : pitch_lo, pb = qb ? bp : pitch_lo;
+#pragma unroll
+        for (int m = 0; m < MR; m++) {
+            if (ty + R * m < HH) {
+                // rows m and m' share a parity when R * (m - m') is even; R * m / 2 band rows past row 0 (or
+                // R * (m - 1) / 2 past row 1)
+                if (((R * m) & 1) == 0)
+                    vals[m] = src[base_a + (uint64_t)(R * m / 2) * pa];
+                else
+                    vals[m] = src[base_b + (uint64_t)(R * (m - 1) / 2) * pb];
+            }
+        }
+    } else {
 #pragma unroll
     for (int m = 0; m < MR; m++) {
         const int ly = ty + R * m;
@@ -217,6 +234,7 @@ __device__ __forceinline__ void tile_load(typename Lift<IS97>::T* tile, int pitc
                 vals[m] = src[(ov ? col_hi : col_lo) + (uint64_t)cy * (ov ? bp : pitch_lo)];
             }
         }
+    }
     }
 #pragma unroll
     for (int m = 0; m < MR; m++) {
@@ -240,9 +258,9 @@ __device__ __forceinline__ void tile_load(typename Lift<IS97>::T* tile, int pitc
 // over the TX output columns: each thread takes a KR-sample segment of a row
 // with its halo into registers.  Rows go in rounds of NT / (TX / KR); within a
 // round every segment is read before any is written, and rounds touch
-// different rows.
-template <bool IS97, int TX, int TY, int NC>
-__device__ __forceinline__ void rows_pass(typename Lift<IS97>::T* tile, int pitch, int cs, int par) {
+// different rows.  P: parity of window column 0.
+template <bool IS97, int TX, int TY, int NC, int P>
+__device__ __forceinline__ void rows_pass(typename Lift<IS97>::T* tile, int pitch, int cs) {
     typedef typename Lift<IS97>::T T;
     constexpr int H = Lift<IS97>::H, HH = TY + 2 * H;
     constexpr int KR = kRowSeg, SEG = TX / KR, N = KR + 2 * H;
@@ -259,7 +277,7 @@ __device__ __forceinline__ void rows_pass(typename Lift<IS97>::T* tile, int pitc
         for (int j = 0; j < N; j++) x[j] = p[j];
         __syncthreads();
         if (r0 + rr < ROWS) {
-            lift_regs<IS97, N>(x, par);
+            lift_regs<IS97, N, P>(x);
 #pragma unroll
             for (int j = 0; j < KR; j++) p[H + j] = x[H + j];
         }
@@ -268,16 +286,17 @@ __device__ __forceinline__ void rows_pass(typename Lift<IS97>::T* tile, int pitc
 }
 
 // The column segment of thread tid of window c after the column pass: KC
-// samples of column H + col, rows H + seg * KC + j, lifted unless v1x.
-template <bool IS97, int TX, int TY>
+// samples of column H + col, rows H + seg * KC + j, lifted unless v1x.  P:
+// parity of window row 0.
+template <bool IS97, int TX, int TY, int P>
 __device__ __forceinline__ void column_segment(const typename Lift<IS97>::T* tile, int pitch, int col, int seg,
-                                               bool v1x, int par,
+                                               bool v1x,
                                                typename Lift<IS97>::T (&x)[col_seg(TX, TY) + 2 * Lift<IS97>::H]) {
     constexpr int H = Lift<IS97>::H, KC = col_seg(TX, TY), N = KC + 2 * H;
     const typename Lift<IS97>::T* const p = tile + seg * KC * pitch + H + col;
 #pragma unroll
     for (int j = 0; j < N; j++) x[j] = p[j * pitch];
-    if (!v1x) lift_regs<IS97, N>(x, par);
+    if (!v1x) lift_regs<IS97, N, P>(x);
 }
 
 template <bool IS97, int TX, int TY>
@@ -297,24 +316,119 @@ __global__ __launch_bounds__(NT) void idwt_level_kernel(const tc_dwt_job* __rest
     const bool h1 = (J.u1 - J.u0) == 1, v1x = (J.v1 - J.v0) == 1;
     tile_load<IS97, TX, TY>(tile, PITCH, base, s, J.u0, J.u1, J.v0, J.v1, ox, oy, false);
     __syncthreads();
-    if (!h1) rows_pass<IS97, TX, TY, 1>(tile, PITCH, 0, (ox - H) & 1);
+    // the window's parities pick one of two compiled forms of each pass
+    if (!h1) {
+        if ((ox - H) & 1) {
+            asm volatile("; odd window columns");
+            rows_pass<IS97, TX, TY, 1, 1>(tile, PITCH, 0);
+        } else {
+            rows_pass<IS97, TX, TY, 1, 0>(tile, PITCH, 0);
+        }
+    }
     const int col = threadIdx.x % TX, seg = threadIdx.x / TX;
     if (seg >= TY / KC) return;
-    T x[KC + 2 * H];
-    column_segment<IS97, TX, TY>(tile, PITCH, col, seg, v1x, (oy - H) & 1, x);
-    const int u = ox + col;
-    if (u >= J.u1) return;
-    T* const o = base + J.out_off + (u - J.u0);
+    // each parity form runs its column pass and its stores on its own (a join
+    // between them makes the compiler merge the register arrays through
+    // dynamically indexed moves)
+    auto finish = [&](auto parity) {
+        T x[KC + 2 * H];
+        column_segment<IS97, TX, TY, decltype(parity)::value>(tile, PITCH, col, seg, v1x, x);
+        const int u = ox + col;
+        if (u >= J.u1) return;
+        const int vs = oy + seg * KC;
+        const int nrows = J.v1 - vs;  // rows of the segment inside the rectangle
+        T* o = base + J.out_off + (u - J.u0) + (uint64_t)(vs - J.v0) * J.out_pitch;
+        if (nrows >= KC) {
 #pragma unroll
-    for (int j = 0; j < KC; j++) {
-        const int v = oy + seg * KC + j;
-        if (v < J.v1) o[(uint64_t)(v - J.v0) * J.out_pitch] = x[H + j];
+            for (int j = 0; j < KC; j++, o += J.out_pitch) *o = x[H + j];
+        } else {
+#pragma unroll
+            for (int j = 0; j < KC; j++, o += J.out_pitch)
+                if (j < nrows) *o = x[H + j];
+        }
+    };
+    if ((oy - H) & 1) {
+        asm volatile("; odd window rows");
+        finish(std::integral_constant<int, 1>());
+    } else {
+        finish(std::integral_constant<int, 0>());
     }
 }
 
+// min of max: one v_med3_i32
 template <typename O>
 __device__ __forceinline__ O clamp_out(int v, int lo, int hi) {
-    return (O)(v < lo ? lo : (v > hi ? hi : v));
+    return (O)min(max(v, lo), hi);
+}
+
+// Column pass of the final level for this thread's segment of every
+// component, then the inverse colour transform, DC shift, rounding, clamping
+// and packing of its KC output pixels.  P: parity of window row 0.
+template <bool IS97, int NC, typename O, int TX, int TY, int P>
+__device__ __forceinline__ void final_columns(const tc_final_job& J, const typename Lift<IS97>::T* tile, int ox,
+                                              int oy, int col, int seg, bool v1x, O* __restrict__ outimg) {
+    typedef typename Lift<IS97>::T T;
+    constexpr int H = Lift<IS97>::H;
+    constexpr int PITCH = TX + 2 * H + 1;
+    constexpr int TS = (TY + 2 * H) * PITCH;
+    constexpr int KC = col_seg(TX, TY);
+    T x[NC][KC + 2 * H];
+#pragma unroll
+    for (int c = 0; c < NC; c++) column_segment<IS97, TX, TY, P>(tile + c * TS, PITCH, col, seg, v1x, x[c]);
+    const int u = ox + col;
+    if (u >= J.wx1) return;
+    const int vs = oy + seg * KC;
+    const int nrows = J.wy1 - vs;  // rows of the segment inside the window (>= KC for all but the last tiles)
+    O* const o0 = outimg + J.out_off + (uint64_t)(u - J.wx0) * J.out_cstride + (uint64_t)(vs - J.wy0) * J.out_pitch;
+    // one copy of the store loop per (colour transform, whole segment) case,
+    // so that neither test runs per pixel
+    auto emit = [&](auto mct, auto whole) {
+        O* o = o0;
+#pragma unroll
+        for (int j = 0; j < KC; j++, o += J.out_pitch) {
+            if (!decltype(whole)::value && j >= nrows) continue;
+            if (decltype(mct)::value) {
+                if (IS97) {
+                    // inverse ICT, T.800 G.3 (not the YUV constants of TC/preprocess_ict_inverse.cl:31-35)
+                    const float y0 = (float)x[0][H + j], y1 = (float)x[NC > 1 ? 1 : 0][H + j],
+                                y2 = (float)x[NC > 2 ? 2 : 0][H + j];
+                    const float r = fmaf(1.402f, y2, y0);
+                    const float g = fmaf(-0.71414f, y2, fmaf(-0.34413f, y1, y0));
+                    const float b = fmaf(1.772f, y1, y0);
+                    o[0] = clamp_out<O>((int)rintf(r) + J.shift[0], J.minv[0], J.maxv[0]);
+                    o[1] = clamp_out<O>((int)rintf(g) + J.shift[1], J.minv[1], J.maxv[1]);
+                    o[2] = clamp_out<O>((int)rintf(b) + J.shift[2], J.minv[2], J.maxv[2]);
+                } else {
+                    // inverse RCT, T.800 G.2 (TC/preprocess_rct_inverse.cl:18-36)
+                    const int y0 = (int)x[0][H + j], y1 = (int)x[NC > 1 ? 1 : 0][H + j],
+                              y2 = (int)x[NC > 2 ? 2 : 0][H + j];
+                    const int g = y0 - ((y1 + y2) >> 2);
+                    o[0] = clamp_out<O>(y2 + g + J.shift[0], J.minv[0], J.maxv[0]);
+                    o[1] = clamp_out<O>(g + J.shift[1], J.minv[1], J.maxv[1]);
+                    o[2] = clamp_out<O>(y1 + g + J.shift[2], J.minv[2], J.maxv[2]);
+                }
+            } else {
+#pragma unroll
+                for (int c = 0; c < NC; c++) {
+                    const int iv = IS97 ? (int)rintf((float)x[c][H + j]) : (int)x[c][H + j];
+                    o[c] = clamp_out<O>(iv + J.shift[c], J.minv[c], J.maxv[c]);
+                }
+            }
+        }
+    };
+    typedef std::integral_constant<bool, true> yes;
+    typedef std::integral_constant<bool, false> no;
+    if (NC == 3 && J.mct) {
+        if (nrows >= KC)
+            emit(std::integral_constant<bool, NC == 3>(), yes());
+        else
+            emit(std::integral_constant<bool, NC == 3>(), no());
+    } else {
+        if (nrows >= KC)
+            emit(no(), yes());
+        else
+            emit(no(), no());
+    }
 }
 
 template <bool IS97, int NC, typename O, int TX, int TY>
@@ -341,46 +455,24 @@ __global__ __launch_bounds__(NT) void idwt_final_kernel(const tc_final_job* __re
     }
     __syncthreads();
     // every component's rows in the same rounds
-    if (!h1) rows_pass<IS97, TX, TY, NC>(tile, PITCH, TS, (ox - H) & 1);
+    if (!h1) {
+        if ((ox - H) & 1) {
+            asm volatile("; odd window columns");
+            rows_pass<IS97, TX, TY, NC, 1>(tile, PITCH, TS);
+        } else {
+            rows_pass<IS97, TX, TY, NC, 0>(tile, PITCH, TS);
+        }
+    }
     const int col = threadIdx.x % TX, seg = threadIdx.x / TX;
     if (seg >= TY / KC) return;
-    T x[NC][KC + 2 * H];
-#pragma unroll
-    for (int c = 0; c < NC; c++) column_segment<IS97, TX, TY>(tile + c * TS, PITCH, col, seg, v1x, (oy - H) & 1, x[c]);
-    const int u = ox + col;
-    if (u >= J.wx1) return;
-    O* const orow = outimg + J.out_off + (uint64_t)(u - J.wx0) * J.out_cstride;
-#pragma unroll
-    for (int j = 0; j < KC; j++) {
-        const int v = oy + seg * KC + j;
-        if (v >= J.wy1) break;
-        O* const o = orow + (uint64_t)(v - J.wy0) * J.out_pitch;
-        if (NC == 3 && J.mct) {
-            if (IS97) {
-                // inverse ICT, T.800 G.3 (not the YUV constants of TC/preprocess_ict_inverse.cl:31-35)
-                const float y0 = (float)x[0][H + j], y1 = (float)x[NC > 1 ? 1 : 0][H + j],
-                            y2 = (float)x[NC > 2 ? 2 : 0][H + j];
-                const float r = fmaf(1.402f, y2, y0);
-                const float g = fmaf(-0.71414f, y2, fmaf(-0.34413f, y1, y0));
-                const float b = fmaf(1.772f, y1, y0);
-                o[0] = clamp_out<O>((int)rintf(r) + J.shift[0], J.minv[0], J.maxv[0]);
-                o[1] = clamp_out<O>((int)rintf(g) + J.shift[1], J.minv[1], J.maxv[1]);
-                o[2] = clamp_out<O>((int)rintf(b) + J.shift[2], J.minv[2], J.maxv[2]);
-            } else {
-                // inverse RCT, T.800 G.2 (TC/preprocess_rct_inverse.cl:18-36)
-                const int y0 = (int)x[0][H + j], y1 = (int)x[NC > 1 ? 1 : 0][H + j], y2 = (int)x[NC > 2 ? 2 : 0][H + j];
-                const int g = y0 - ((y1 + y2) >> 2);
-                o[0] = clamp_out<O>(y2 + g + J.shift[0], J.minv[0], J.maxv[0]);
-                o[1] = clamp_out<O>(g + J.shift[1], J.minv[1], J.maxv[1]);
-                o[2] = clamp_out<O>(y1 + g + J.shift[2], J.minv[2], J.maxv[2]);
-            }
-        } else {
-#pragma unroll
-            for (int c = 0; c < NC; c++) {
-                const int iv = IS97 ? (int)rintf((float)x[c][H + j]) : (int)x[c][H + j];
-                o[c] = clamp_out<O>(iv + J.shift[c], J.minv[c], J.maxv[c]);
-            }
-        }
+    // each parity form runs its column pass and its stores on its own: with a
+    // join between them the compiler merges the two register arrays through
+    // dynamically indexed moves
+    if ((oy - H) & 1) {
+        asm volatile("; odd window rows");
+        final_columns<IS97, NC, O, TX, TY, 1>(J, tile, ox, oy, col, seg, v1x, outimg);
+    } else {
+        final_columns<IS97, NC, O, TX, TY, 0>(J, tile, ox, oy, col, seg, v1x, outimg);
     }
 }
 
