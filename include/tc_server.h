@@ -27,13 +27,15 @@ extern "C" {
 typedef struct tc_server_s *tc_server;
 
 typedef struct {
-    int32_t slots;            /* concurrent batches (streams), default 3 */
+    int32_t slots;            /* concurrent batches (streams), default 2 */
     int32_t max_batch;        /* most requests per batch, default 256 */
     uint64_t memory_budget;   /* device bytes for all slots together, default 32 GiB */
     int32_t bpw_small, bpw_large; /* tier-1 packing (0: automatic) */
     int32_t dwt_shape;            /* inverse-DWT tile shape (tc_idwt_tile_shape) */
     uint64_t max_batch_bytes;     /* most codestream bytes per batch (a larger request
-                                     runs alone), default 256 MiB */
+                                     runs alone), default 1 GiB: batches that large
+                                     hold several 8K 16-bit images, whose longest
+                                     block chains then bound one batch, not each */
 } tc_server_config;
 
 typedef struct {

@@ -311,8 +311,8 @@ class ServerStats(ctypes.Structure):
 class Server:
     """Continuous-batching decode server over the C ABI (include/tc_server.h)."""
 
-    def __init__(self, device: "Device" = None, slots=3, max_batch=256, memory_budget=32 << 30, bpw_small=0,
-                 bpw_large=0, dwt_shape=0, max_batch_bytes=256 << 20):
+    def __init__(self, device: "Device" = None, slots=2, max_batch=256, memory_budget=32 << 30, bpw_small=0,
+                 bpw_large=0, dwt_shape=0, max_batch_bytes=1 << 30):
         self.device = device or default_device()
         cfg = ServerConfig(slots, max_batch, memory_budget, bpw_small, bpw_large, dwt_shape, max_batch_bytes)
         h = ctypes.c_void_p()

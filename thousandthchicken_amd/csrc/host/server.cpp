@@ -304,10 +304,10 @@ tc_status tc_server_create(tc_device dev, const tc_server_config* cfg, tc_server
     if (!dev || !out) return fail(TC_ERR_INVALID_ARG, "tc_server_create: null");
     auto s = new tc_server_s();
     s->dev = dev;
-    s->cfg.slots = cfg && cfg->slots > 0 ? cfg->slots : 3;
+    s->cfg.slots = cfg && cfg->slots > 0 ? cfg->slots : 2;
     s->cfg.max_batch = cfg && cfg->max_batch > 0 ? cfg->max_batch : 256;
     s->cfg.memory_budget = cfg && cfg->memory_budget ? cfg->memory_budget : (32ull << 30);
-    s->cfg.max_batch_bytes = cfg && cfg->max_batch_bytes ? cfg->max_batch_bytes : (256ull << 20);
+    s->cfg.max_batch_bytes = cfg && cfg->max_batch_bytes ? cfg->max_batch_bytes : (1ull << 30);
     s->cfg.bpw_small = cfg ? cfg->bpw_small : 0;
     s->cfg.bpw_large = cfg ? cfg->bpw_large : 0;
     s->cfg.dwt_shape = cfg ? cfg->dwt_shape : 0;

@@ -82,6 +82,12 @@ def run(server, streams, rate, seed=0, keep=None):
     span = max(x.done_s for x in infos) - min(x.submit_s for x in infos)
     bs = np.array([x.batch_size for x in infos])
     failed = sum(1 for x in infos if x.status != 0)
+    # latency of the second and the last fifth of the requests (in arrival
+    # order): a server that keeps up holds it flat, a saturated one's queue
+    # makes it grow with time
+    q = max(1, n // 5)
+    lat_early = float(np.median(lat[q:2 * q])) if n >= 2 * q else float(np.median(lat))
+    lat_late = float(np.median(lat[n - q:]))
     stt = server.stats()
     return {
         "server_s": {"busy": round(stt.busy_s, 3), "plan": round(stt.plan_s, 3), "device": round(stt.device_s, 3),
@@ -94,6 +100,8 @@ def run(server, streams, rate, seed=0, keep=None):
         "p50_ms": round(float(np.percentile(lat, 50)) * 1e3, 3),
         "p99_ms": round(float(np.percentile(lat, 99)) * 1e3, 3),
         "p50_queue_ms": round(float(np.percentile(wait, 50)) * 1e3, 3),
+        "p50_ms_second_fifth": round(lat_early * 1e3, 3),
+        "p50_ms_last_fifth": round(lat_late * 1e3, 3),
         "mean_batch": round(float(bs.mean()), 2),
         "span_s": round(span, 3),
         "images": kept,
