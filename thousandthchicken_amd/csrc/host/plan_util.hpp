@@ -30,8 +30,9 @@ struct CwCopy {
 Status plan_tile_cblks(const Image& im, int t, const uint64_t* plane, uint64_t& cw_size, std::vector<CwCopy>& copies,
                        std::vector<tc_cblk_desc>& descs, std::vector<uint32_t> (&cls)[2], std::string& err);
 
-// Orders a class of blocks for packing into waves: similar pass counts and
-// shapes side by side, heaviest first.
+// Orders a class of blocks for packing into waves: equal pass counts, then
+// equal stripe counts, then similar codeword lengths side by side, heaviest
+// first.
 void sort_for_packing(std::vector<uint32_t>& idx, const std::vector<tc_cblk_desc>& descs);
 
 // Work-buffer sizes (elements) of the two ping-pong buffers of a tile-component.
