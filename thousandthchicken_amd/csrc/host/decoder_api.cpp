@@ -362,22 +362,33 @@ tc_status tc_decoder_autotune(tc_decoder d, int reps, tc_tuning* best) {
     Status s = measure(cur, &best_t1, &best_dwt);
     if (s != Status::Ok) return fail(s);
     const int cand[] = {1, TC_BPW_LOCKSTEP, TC_BPW_LOCKSTEP32, TC_BPW_LOCKSTEP16};
-    for (int cls = 0; cls < 2; cls++) {
-        if (!has[cls]) continue;
-        for (int b : cand) {
-            // styled code-blocks decode only under a lockstep packing; the planner
-            // would run any other request as lockstep anyway
-            if (styled[cls] && b > 0) continue;
-            Tuning t = cur;
-            (cls == 0 ? t.bpw32 : t.bpw64) = b;
-            if ((cls == 0 ? cur.bpw32 : cur.bpw64) == b) continue;
-            s = measure(t, &t1, &dw);
-            if (s != Status::Ok) return fail(s);
-            if (t1 < best_t1) {
-                best_t1 = t1;
-                cur = t;
+    // The two classes' launches run side by side and the stage lasts as long
+    // as the longer one, so a class tuned while the other one is the longer
+    // launch sees only noise: a second round re-tunes each class with the
+    // other at its new packing.
+    for (int round = 0; round < 2; round++) {
+        const Tuning before = cur;
+        for (int cls = 0; cls < 2; cls++) {
+            if (!has[cls]) continue;
+            for (int b : cand) {
+                // styled code-blocks decode only under a lockstep packing; the planner
+                // would run any other request as lockstep anyway
+                if (styled[cls] && b > 0) continue;
+                Tuning t = cur;
+                (cls == 0 ? t.bpw32 : t.bpw64) = b;
+                if ((cls == 0 ? cur.bpw32 : cur.bpw64) == b) continue;
+                s = measure(t, &t1, &dw);
+                if (s != Status::Ok) return fail(s);
+                if (t1 < best_t1) {
+                    best_t1 = t1;
+                    cur = t;
+                }
             }
         }
+        if (cur.bpw32 == before.bpw32 && cur.bpw64 == before.bpw64) break;
+        // the incumbent's time was measured beside the other class's old packing
+        s = measure(cur, &best_t1, &dw);
+        if (s != Status::Ok) return fail(s);
     }
     // heavy-block thresholds of a lockstep class (estimated symbols; 0: no split)
     for (int cls = 0; cls < 2; cls++) {
