@@ -33,6 +33,7 @@ struct Request {
     std::vector<uint8_t> out;     // the result, once moved out of pinned memory
     const uint8_t* pin = nullptr;  // the result in a slot's pinned buffer
     bool done = false;
+    bool moving = false;           // a worker is copying the result out of pinned memory
 };
 
 struct Slot {
@@ -251,9 +252,20 @@ struct tc_server_s {
                 if (it != reqs.end() && it->second->pin) unread.push_back(it->second.get());
             }
             s.holders[s.cur].clear();
-            for (Request* r : unread) {  // results nobody read since the batch before last
-                r->out.assign(r->pin, r->pin + r->out_bytes);
-                r->pin = nullptr;
+            // results nobody read since the batch before last leave the buffer
+            // before it is reused; the copy runs without the lock (so that
+            // submissions and other slots do not wait for it), and readers of
+            // these requests wait for it to finish
+            for (Request* r : unread) r->moving = true;
+            if (!unread.empty()) {
+                lk.unlock();
+                for (Request* r : unread) r->out.assign(r->pin, r->pin + r->out_bytes);
+                lk.lock();
+                for (Request* r : unread) {
+                    r->pin = nullptr;
+                    r->moving = false;
+                }
+                cv_done.notify_all();
             }
             lk.unlock();
             const tc_status e = execute(s);
@@ -426,31 +438,53 @@ tc_status tc_server_wait(tc_server s, uint64_t id, double timeout_s) {
 
 tc_status tc_server_result(tc_server s, uint64_t id, void* out, size_t cap, tc_request_info* info) {
     if (!s) return fail(TC_ERR_INVALID_ARG, "tc_server_result: null");
-    std::lock_guard<std::mutex> lk(s->mu);
+    std::unique_ptr<Request> dead;  // destroyed after the lock is released
+    std::unique_lock<std::mutex> lk(s->mu);
     auto it = s->reqs.find(id);
     if (it == s->reqs.end()) return fail(TC_ERR_NOT_FOUND, "tc_server_result: unknown request");
+    if (out) {
+        while (it->second->moving) {
+            s->cv_done.wait(lk);
+            it = s->reqs.find(id);
+            if (it == s->reqs.end()) return fail(TC_ERR_NOT_FOUND, "tc_server_result: unknown request");
+        }
+    }
     Request* r = it->second.get();
     if (!r->done) return fail(TC_ERR_BUSY, "tc_server_result: not finished");
     if (info) *info = r->info;
     if (!out) return TC_SUCCESS;
     if (r->info.status != TC_SUCCESS) {
         tc_status st = r->info.status;
+        dead = std::move(it->second);
         s->reqs.erase(it);
+        lk.unlock();
         return fail(st, "request failed");
     }
     if (cap < r->out_bytes) return fail(TC_ERR_CAPACITY, "tc_server_result: output buffer too small");
+    // a result still in pinned memory is copied under the lock: the slot may
+    // reuse that buffer as soon as the request is gone
     std::memcpy(out, r->pin ? r->pin : r->out.data(), r->out_bytes);
+    dead = std::move(it->second);
     s->reqs.erase(it);
+    lk.unlock();
     return TC_SUCCESS;
 }
 
 tc_status tc_server_release(tc_server s, uint64_t id) {
     if (!s) return fail(TC_ERR_INVALID_ARG, "tc_server_release: null");
-    std::lock_guard<std::mutex> lk(s->mu);
+    std::unique_ptr<Request> dead;  // its codestream copy, parse and result are freed after the lock
+    std::unique_lock<std::mutex> lk(s->mu);
     auto it = s->reqs.find(id);
     if (it == s->reqs.end()) return fail(TC_ERR_NOT_FOUND, "tc_server_release: unknown request");
     if (!it->second->done) return fail(TC_ERR_BUSY, "tc_server_release: not finished");
+    while (it->second->moving) {
+        s->cv_done.wait(lk);
+        it = s->reqs.find(id);
+        if (it == s->reqs.end()) return fail(TC_ERR_NOT_FOUND, "tc_server_release: unknown request");
+    }
+    dead = std::move(it->second);
     s->reqs.erase(it);
+    lk.unlock();
     return TC_SUCCESS;
 }
 
