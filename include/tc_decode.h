@@ -156,7 +156,7 @@ typedef struct {
      * than this decode one wavefront per block in a launch of their own, beside
      * the lockstep launch (0: no split, -1: the default) */
     int32_t heavy_small, heavy_large;
-    int32_t heavy_waves;          /* waves per compute unit of that launch (0: 2) */
+    int32_t heavy_waves;          /* waves per compute unit of that launch (0: 4) */
 } tc_tuning;
 
 tc_status tc_decoder_get_tuning(tc_decoder dec, tc_tuning *out);

@@ -285,7 +285,7 @@ void to_tuning(const tc_tuning& in, Tuning& t) {
     t.dwt_shape = in.dwt_shape;
     t.heavy32 = in.heavy_small;
     t.heavy64 = in.heavy_large;
-    t.heavy_waves_per_cu = in.heavy_waves > 0 ? in.heavy_waves : 2;
+    t.heavy_waves_per_cu = in.heavy_waves > 0 ? in.heavy_waves : 4;
 }
 
 }  // namespace

@@ -73,12 +73,11 @@ int pick_blocks_per_wave(int n, bool small, int cus, const Tuning& t) {
     return n >= 8 * 64 * c ? TC_BPW_LOCKSTEP : TC_BPW_LOCKSTEP32;
 }
 
-// Heavy-block threshold (estimated MQ symbols) when none is set: none.  On
-// the 256-image mixed batch every split measured slower than none (64x64
-// class, 16 blocks per lockstep wave: 35.5 ms unsplit, 37-45 ms with the
-// blocks above 8 000-25 000 symbols on their own launch), because the
-// one-wave-per-block waves compete with the lockstep waves for the scalar
-// unit and LDS of their compute units; the autotuner still tries thresholds.
+// Heavy-block threshold (estimated MQ symbols) when none is set: none.  Which
+// split helps depends on the batch: on the 256-image mixed batch the blocks
+// above 20 000 symbols gain 4 % (their launch last), those above 16 000 lose
+// 40 %, and the one-wave-per-block waves compete with the lockstep waves for
+// their compute units' scalar unit and LDS; the autotuner tries thresholds.
 int default_heavy(bool) { return 0; }
 
 Pipeline::Pipeline(tc_device dev) : dev_(dev) {
@@ -262,15 +261,19 @@ Status build_plan(const std::vector<const Image*>& imgs, const Tuning& tuning, i
         P.heavy[k] = heavy.empty() ? 0 : thr;
         P.styled[k] = style != 0;
     }
-    // Heavy launches go first, then the class whose longest block chain is
-    // longest: launches dispatch their waves in submission order, and the
-    // waves resident first are the ones that bound the tier-1 stage (on the
-    // 256-image mixed batch, the 64x64 class first takes the stage from 35.3
-    // to 32.8 ms; its waves otherwise wait for the 32x32 class's to leave).
+    // The class whose longest block chain is longest goes first: launches
+    // dispatch their waves in submission order, and the waves resident first
+    // are the ones that bound the tier-1 stage (on the 256-image mixed batch,
+    // the 64x64 class first takes the stage from 35.3 to 32.8 ms; its waves
+    // otherwise wait for the 32x32 class's to leave).  Heavy blocks' launches
+    // go last: their one-wave-per-block waves are short beside the lockstep
+    // waves and fill the compute units those leave (first, they slowed the
+    // lockstep waves and made the stage vary by 30 %; last, blocks above 20 000
+    // estimated symbols take it from 30.1 to 29.0 ms on that batch).
     std::stable_sort(main_groups.begin(), main_groups.end(),
                      [](const Plan::Group& a, const Plan::Group& b) { return a.max_symbols > b.max_symbols; });
-    for (auto& g : heavy_groups) P.groups.push_back(g);
     for (auto& g : main_groups) P.groups.push_back(g);
+    for (auto& g : heavy_groups) P.groups.push_back(g);
 
     // ---- inverse DWT levels ----
     int maxlev = 0;

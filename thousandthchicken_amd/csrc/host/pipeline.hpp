@@ -41,7 +41,7 @@ struct Tuning {
     // waves per compute unit of the heavy blocks' launch (each wave decodes
     // its share of them one after the other), so that they leave room on
     // every compute unit for the lockstep waves
-    int heavy_waves_per_cu = 2;
+    int heavy_waves_per_cu = 4;
 };
 
 // Estimated MQ symbols of a code-block from its descriptor: a least-squares
@@ -71,7 +71,7 @@ struct StageTimes {
 // The device work of a batch: launches, their descriptor offsets in the upload
 // region, and the sizes of the four device regions it needs.
 struct Plan {
-    int heavy_waves_per_cu = 2;
+    int heavy_waves_per_cu = 4;
     struct Group {  // one t1 launch
         bool small = false;
         int bpw = 1;
@@ -147,7 +147,7 @@ struct LaunchStreams {
 };
 
 // Enqueues the plan on `stream`.  With side streams (S, may be null) the
-// tier-1 launches run side by side, the heavy blocks' launches first; with
+// tier-1 launches run side by side, the heavy blocks' launches last; with
 // ev (5 + 2 * kMaxT1Launches events) the stage boundaries are recorded on
 // `stream` (ev[0..4]) and each tier-1 launch is bracketed on the stream it
 // runs on (ev[5 + 2g], ev[6 + 2g]).
