@@ -281,22 +281,21 @@ __device__ __forceinline__ void stripe_symbols(LdsLock<M>& L, int lane, LaneMq& 
             }
             const M bx = (M)1 << x;
             uint32_t lab, sxor = 0;
+            M s1 = 0, x1 = 0;  // TYPE 0/2: significance and sign rows r+1 as read for the labels
             if (TYPE == 1) {
                 lab = ((enib >> r) & 1u) ? 16u : 14u + ((pnib >> r) & 1u);
-            } else if (pend == PEND_NONE && !rl) {
-                const uint32_t w9 = col3<M>(lr(RS + r), x) | (col3<M>(lr(RS + r + 1), x) << 3) |
-                                    (col3<M>(lr(RS + r + 2), x) << 6);
-                lab = L.zc[cls][w9];
-            } else if (pend == PEND_SC) {
-                const uint32_t w9 = col3<M>(lr(RS + r), x) | (col3<M>(lr(RS + r + 1), x) << 3) |
-                                    (col3<M>(lr(RS + r + 2), x) << 6);
-                const uint32_t xw = col3<M>(lr(RX + r), x) | (col3<M>(lr(RX + r + 1), x) << 3) |
-                                    (col3<M>(lr(RX + r + 2), x) << 6);
-                const uint32_t e = L.sc[(w9 & 0xAAu) | ((xw & 0xAAu) >> 1)];
-                lab = e & 15u;
-                sxor = e >> 4;
             } else {
-                lab = rl ? 17u : 18u;  // run-length or uniform
+                // Zero-coding and sign-coding labels from one read of the
+                // staged rows, for every lane: nearly every step has lanes of
+                // both kinds, and two branches would read the rows twice.
+                s1 = lr(RS + r + 1);
+                x1 = lr(RX + r + 1);
+                const uint32_t w9 = col3<M>(lr(RS + r), x) | (col3<M>(s1, x) << 3) | (col3<M>(lr(RS + r + 2), x) << 6);
+                const uint32_t xw = col3<M>(lr(RX + r), x) | (col3<M>(x1, x) << 3) | (col3<M>(lr(RX + r + 2), x) << 6);
+                const uint32_t lz = L.zc[cls][w9];
+                const uint32_t e = L.sc[(w9 & 0xAAu) | ((xw & 0xAAu) >> 1)];
+                lab = pend == PEND_SC ? (e & 15u) : (pend == PEND_NONE ? (rl ? 17u : lz) : 18u);  // 17 run-length, 18 uniform
+                sxor = e >> 4;
             }
             uint32_t st = L.ctx[lab][lane];
             const uint32_t bit = m.raw ? lane_raw(m) : lane_decode(m, st, L.trans);
@@ -304,46 +303,50 @@ __device__ __forceinline__ void stripe_symbols(LdsLock<M>& L, int lane, LaneMq& 
             uint32_t* const at = out + (uint32_t)r * pitch + x;
             if (TYPE == 1) {
                 onib |= bit << r;
-            } else if (pend == PEND_NONE && !rl) {
-                // zero coding
-                if (bit) {
-                    lr(RS + r + 1) |= bx;
-                    pend = PEND_SC;
-                    if (TYPE == 2) sany |= bx;
-                    if (TYPE == 0) {
-                        // neighbours later in scan order may now be coded in this pass
-                        // (the row below joins this column's nibble too; c0..c3 record all)
-                        cur |= enib & (2u << r) & 15u;
-                        const M nx = bx << 1;
-                        const M a0 = ((r <= 1) ? nx : (M)0) & e0;
-                        const M a1 = (((r <= 2) ? nx : (M)0) | ((r == 0) ? bx : (M)0)) & e1;
-                        const M a2 = (((r >= 1) ? nx : (M)0) | ((r == 1) ? bx : (M)0)) & e2;
-                        const M a3 = (((r >= 2) ? nx : (M)0) | ((r == 2) ? bx : (M)0)) & e3;
-                        c0 |= a0, c1 |= a1, c2 |= a2, c3 |= a3;
-                        cany |= (a0 | a1 | a2 | a3) & nx;
-                    }
-                }
-            } else if (pend == PEND_SC) {
+            } else {
+                // The decision's effect on the staged rows, for every lane:
+                // rows r+1 are written back (changed where a zero-coding 1
+                // or a sign 1 sets their bit) from the values read for the
+                // labels, without a branch.
+                const bool zc = pend == PEND_NONE && !rl;
+                const bool sig = zc && bit != 0u;
+                const bool sgn = pend == PEND_SC;
                 // a raw (BYPASS) segment carries the sign bit itself (T.800 D.6)
                 const uint32_t sg = bit ^ (m.raw ? 0u : sxor);
-                if (sg) lr(RX + r + 1) |= bx;
-                *at = pbit | (sg << 31);  // the word's first bits: the plane was zero here
-                pend = PEND_NONE;
-            } else if (TYPE == 2) {
-                if (rl) {
-                    if (bit)
-                        pend = PEND_UNI1;
-                    else
-                        cur = 0;  // the whole column stays insignificant
-                } else if (pend == PEND_UNI1) {
-                    r = (int)(bit << 1);
-                    pend = PEND_UNI2;
-                } else {
-                    r |= (int)bit;
-                    cur &= ~((2u << r) - 1u);  // rows above r were coded as insignificant
-                    lr(RS + r + 1) |= bx;
-                    sany |= bx;
-                    pend = PEND_SC;
+                lr(RS + r + 1) = sig ? (s1 | bx) : s1;
+                lr(RX + r + 1) = (sgn && sg) ? (x1 | bx) : x1;
+                if (TYPE == 2) sany |= sig ? bx : (M)0;
+                if (sgn) *at = pbit | (sg << 31);  // the word's first bits: the plane was zero here
+                const uint32_t pend0 = pend;
+                pend = sig ? PEND_SC : (sgn ? PEND_NONE : pend);
+                if (TYPE == 0 && sig) {
+                    // neighbours later in scan order may now be coded in this pass
+                    // (the row below joins this column's nibble too; c0..c3 record all)
+                    cur |= enib & (2u << r) & 15u;
+                    const M nx = bx << 1;
+                    const M a0 = ((r <= 1) ? nx : (M)0) & e0;
+                    const M a1 = (((r <= 2) ? nx : (M)0) | ((r == 0) ? bx : (M)0)) & e1;
+                    const M a2 = (((r >= 1) ? nx : (M)0) | ((r == 1) ? bx : (M)0)) & e2;
+                    const M a3 = (((r >= 2) ? nx : (M)0) | ((r == 2) ? bx : (M)0)) & e3;
+                    c0 |= a0, c1 |= a1, c2 |= a2, c3 |= a3;
+                    cany |= (a0 | a1 | a2 | a3) & nx;
+                }
+                if (TYPE == 2 && !zc && !sgn) {
+                    if (rl) {
+                        if (bit)
+                            pend = PEND_UNI1;
+                        else
+                            cur = 0;  // the whole column stays insignificant
+                    } else if (pend0 == PEND_UNI1) {
+                        r = (int)(bit << 1);
+                        pend = PEND_UNI2;
+                    } else {
+                        r |= (int)bit;
+                        cur &= ~((2u << r) - 1u);  // rows above r were coded as insignificant
+                        lr(RS + r + 1) |= bx;
+                        sany |= bx;
+                        pend = PEND_SC;
+                    }
                 }
             }
             work = pend != PEND_NONE || cur != 0 || cany != 0;
