@@ -42,12 +42,9 @@ def _expected(b):
     return np.where(sign.astype(bool), -v, v).astype(np.int32)
 
 
-@pytest.mark.parametrize("kernel", ["stripe", "wave", "lockstep", "lockstep_unaligned", "lockstep16"])
-@pytest.mark.parametrize("small", [True, False])
-def test_t1_kernels_on_random_blocks(kernel, small):
-    lim = 32 if small else 64
-    blocks = _blocks(7 if small else 8, 300, lim, lim)
-    print(f"t1 kernel={kernel} small={small} blocks={len(blocks)}", flush=True)
+def _decode(kernel, small, blocks):
+    """Decodes `blocks` with one tier-1 kernel through its C-ABI launch;
+    returns the 64x64 output window of each block."""
     dev = tca.default_device()
     L = tca.lib()
     stream = ctypes.c_void_p()
@@ -85,17 +82,67 @@ def test_t1_kernels_on_random_blocks(kernel, small):
             st = L.tc_launch_t1_decode_wave(stream, 1 if small else 0, 512, bufs[1].ptr, None, len(blocks),
                                             bufs[0].ptr, arena.nbytes, bufs[2].ptr)
         tca.check(st, "launch")
-        out = bufs[2].download(stream, np.int32, n_out).reshape(len(blocks), 64, pitch)
-        for i, b in enumerate(blocks):
-            got = out[i, :b["h"], :b["w"]]
-            exp = _expected(b)
-            assert np.array_equal(got, exp), (i, b["w"], b["h"], b["orient"], b["numbps"], b["npasses"])
-            if b["npasses"] == 3 * b["numbps"] - 2:
-                assert np.array_equal(got, b["src"])
+        return bufs[2].download(stream, np.int32, n_out).reshape(len(blocks), 64, pitch)
     finally:
         for x in bufs:
             x.free()
         L.tc_stream_destroy(stream)
+
+
+@pytest.mark.parametrize("kernel", ["stripe", "wave", "lockstep", "lockstep_unaligned", "lockstep16"])
+@pytest.mark.parametrize("small", [True, False])
+def test_t1_kernels_on_random_blocks(kernel, small):
+    lim = 32 if small else 64
+    blocks = _blocks(7 if small else 8, 300, lim, lim)
+    print(f"t1 kernel={kernel} small={small} blocks={len(blocks)}", flush=True)
+    out = _decode(kernel, small, blocks)
+    for i, b in enumerate(blocks):
+        got = out[i, :b["h"], :b["w"]]
+        exp = _expected(b)
+        assert np.array_equal(got, exp), (i, b["w"], b["h"], b["orient"], b["numbps"], b["npasses"])
+        if b["npasses"] == 3 * b["numbps"] - 2:
+            assert np.array_equal(got, b["src"])
+
+
+def _byte_blocks(seed, n, lim):
+    """Code-blocks whose codewords are byte strings rather than encoder
+    output, weighted towards what the MQ byte-in treats specially: 0xFF bytes,
+    0x80-0x8F after 0xFF (a carry into the stuffed bit), markers (above 0x8F
+    after 0xFF) and codewords that end early."""
+    rng = np.random.default_rng(seed)
+    out = []
+    for i in range(n):
+        w, h = int(rng.integers(1, lim + 1)), int(rng.integers(1, lim + 1))
+        numbps = int(rng.integers(1, 18))
+        npasses = int(rng.integers(1, 3 * numbps - 1))
+        n_bytes = int(rng.integers(0, 400))
+        d = rng.integers(0, 256, n_bytes).astype(np.uint8)
+        mode = i % 4
+        for j in range(1, n_bytes):
+            if mode >= 1 and rng.uniform() < (0.15 if mode == 1 else 0.3):
+                d[j] = 0xFF
+            if d[j - 1] == 0xFF:
+                if mode == 2 or (mode == 3 and rng.uniform() < 0.7):
+                    d[j] = 0x80 + int(rng.integers(0, 16))
+                elif mode == 1 and d[j] > 0x8F:
+                    d[j] &= 0x7F
+        out.append(dict(w=w, h=h, orient=int(rng.integers(0, 4)), numbps=numbps, npasses=npasses,
+                        data=d.tobytes()))
+    return out
+
+
+@pytest.mark.parametrize("kernel", ["stripe", "wave", "lockstep", "lockstep_unaligned", "lockstep16"])
+@pytest.mark.parametrize("small", [True, False])
+def test_t1_kernels_on_byte_strings(kernel, small):
+    """Every kernel's MQ byte-in against T.800's on arbitrary codeword bytes:
+    the decoded coefficients equal the oracle's bit for bit."""
+    lim = 32 if small else 64
+    blocks = _byte_blocks(11 if small else 12, 160, lim)
+    print(f"t1 bytes kernel={kernel} small={small} blocks={len(blocks)}", flush=True)
+    out = _decode(kernel, small, blocks)
+    for i, b in enumerate(blocks):
+        assert np.array_equal(out[i, :b["h"], :b["w"]], _expected(b)), (i, b["w"], b["h"], b["numbps"],
+                                                                         b["npasses"], len(b["data"]))
 
 
 def _edge_blocks(seed):

@@ -42,10 +42,13 @@
 // (so no context of the stripe sees it), SEGSYM decodes the four uniform
 // symbols that follow each cleanup pass.
 //
-// Codewords are staged through a per-lane ring of 8 dwords in LDS.  The ring
-// is topped up in a wave-uniform step whenever some lane has fewer than four
-// bytes left in it (one symbol consumes at most two), so no global load is in
-// flight across symbol steps and the MQ byte-in reads LDS only.
+// Codewords are staged through a per-lane ring of 8 dwords in LDS, topped up
+// (for every lane of the branch running short) before a lane reads past it,
+// so no global load is in flight across symbol steps.  The MQ decoder keeps
+// T.800's C register in the upper half of a 64-bit register and takes the
+// codeword four bytes at a time into the lower half, so renormalisation is
+// one shift, with no byte-in inside it (lane_fill states why this equals the
+// byte-wise BYTEIN).
 #include <hip/hip_runtime.h>
 #include <stdint.h>
 
@@ -83,17 +86,18 @@ struct LdsLock {
 
 // Per-lane MQ decoder reading its codeword from the LDS ring.
 struct LaneMq {
-    uint32_t A, C;
-    int CT;
-    uint32_t bp, len, cur;
-    uint32_t raw;    // 1: the current codeword segment is raw (BYPASS), C/CT unused
+    uint32_t A;
+    uint64_t C;      // MQ: T.800's C register in bits 63..32, codeword bytes taken ahead below
+    int lsb;         // MQ: bit of C holding the lowest bit of the last byte taken
+    int CT;          // raw segments: bits of `cur` still unread
+    uint32_t bp, len, cur;  // cur: the byte at bp, the last one taken (0xFF past a marker or the end)
+    uint32_t raw;    // 1: the current codeword segment is raw (BYPASS), C unused
     uint32_t base;   // arena byte offset of the codeword
     uint32_t rend;   // arena byte offset one past the ring's last byte
     const uint32_t* ring;  // this lane's column of the ring (stride kLaneCount)
 };
 
-// Codeword byte i (i <= bp + 3 while the ring holds four bytes past bp);
-// past the end the decoder sees 0xFF.
+// Codeword byte i (the ring must hold it); past the end the decoder sees 0xFF.
 __device__ __forceinline__ uint32_t lane_byte(const LaneMq& m, uint32_t i) {
     const uint32_t a = m.base + i;
     const uint32_t b = (m.ring[((a >> 2) & 7) * kLaneCount] >> ((a & 3) * 8)) & 0xFFu;
@@ -122,37 +126,79 @@ __device__ __forceinline__ void lane_refill(LaneMq& m, uint32_t* ring_col, const
     m.rend = (q0 + 8) * 4;
 }
 
-// BYTEIN, T.800 Figure C.19, with selects: after 0xFF a byte above 0x8F is a
-// marker (feed 1-bits, stay), any other byte enters 7 bits; otherwise 8 bits.
-// Applied only where `when` holds.
-__device__ __forceinline__ void lane_bytein(LaneMq& m, bool when = true) {
-    const uint32_t b1 = lane_byte(m, m.bp + 1);
-    const bool ff = m.cur == 0xFFu;
-    const bool stay = ff && b1 > 0x8Fu;
-    const uint32_t add = stay ? 0xFF00u : (b1 << (ff ? 9 : 8));
-    m.C += when ? add : 0u;
-    m.CT = when ? ((ff && !stay) ? 7 : 8) : m.CT;
-    const bool adv = when && !stay;
-    m.bp += adv ? 1u : 0u;
-    m.cur = adv ? b1 : m.cur;
+// Takes codeword bytes into C below the last one (called while lsb >= 32).
+//
+// T.800's BYTEIN (Figure C.19) adds the byte after B at 8 bits below it, or
+// 7 after B = 0xFF; after 0xFF a byte above 0x8F is a marker, which feeds
+// 0xFF at 8-bit spacing without advancing, as does the end of the codeword.
+// Taking a byte earlier than BYTEIN would adds it into bits that are still
+// zero, so the bits compared with Qe (63..48) are unchanged, with one
+// exception: a byte of 0x80-0x8F after 0xFF overlaps the 0xFF's lowest bit
+// and carries through it.  BYTEIN takes that byte when the 0xFF's lowest bit
+// has been shifted past bit 48, so it is held back until then.  The common
+// case (no 0xFF among the last byte and the next three, all four inside the
+// codeword) takes four bytes with one add; after a marker or the end, 32
+// one-bits.  The rest goes byte by byte.  Bytes up to lsb <= 48 are always
+// held when Qe is compared, so every decision equals T.800's.
+__device__ __forceinline__ void lane_fill(LaneMq& m, uint32_t* ring_col, const uint32_t* __restrict__ words,
+                                         uint32_t nwords) {
+    if (__ballot(m.rend - (m.base + m.bp) < 8u)) {
+        if (m.rend - (m.base + m.bp) < 16u) lane_refill(m, ring_col, words, nwords);
+    }
+    const uint32_t a = m.base + m.bp + 1;
+    const uint32_t w0 = m.ring[((a >> 2) & 7) * kLaneCount];
+    const uint32_t w1 = m.ring[(((a >> 2) + 1) & 7) * kLaneCount];
+    const uint32_t le = __builtin_amdgcn_alignbyte(w1, w0, a & 3);  // bytes bp+1.. bp+4, the first lowest
+    const uint32_t nq = ~((le << 8) | m.cur);                       // the last byte and the next three, inverted
+    const bool noff = ((nq - 0x01010101u) & ~nq & 0x80808080u) == 0u;
+    const bool inside = m.bp + 4 < m.len;
+    const bool marker = m.cur == 0xFFu && (m.bp + 1 >= m.len || (le & 0xFFu) > 0x8Fu);
+    if ((noff && inside) || marker) {
+        const uint32_t word = marker ? 0xFFFFFFFFu : __builtin_bswap32(le);
+        m.lsb -= 32;
+        m.C += (uint64_t)word << m.lsb;
+        m.bp += marker ? 0u : 4u;
+        m.cur = marker ? 0xFFu : (le >> 24);
+    } else {
+        for (int i = 0; i < 4; i++) {
+            const uint32_t b = lane_byte(m, m.bp + 1);
+            if (m.cur == 0xFFu) {
+                if (b > 0x8Fu) {  // marker: 0xFF, no advance
+                    m.lsb -= 8;
+                    m.C += (uint64_t)0xFFu << m.lsb;
+                    continue;
+                }
+                if (b >= 0x80u && m.lsb <= 48) break;  // carry byte: not before BYTEIN would take it
+                m.lsb -= 7;
+            } else {
+                m.lsb -= 8;
+            }
+            m.C += (uint64_t)b << m.lsb;
+            m.bp += 1;
+            m.cur = b;
+        }
+    }
 }
 
-// INITDEC, T.800 Figure C.20 (the ring must hold the first bytes).
-__device__ __forceinline__ void lane_init(LaneMq& m) {
+// INITDEC, T.800 Figure C.20 (the ring must hold the first bytes): the first
+// byte goes to bits 55..62 (after INITDEC's shift by 7), the rest follow.
+__device__ __forceinline__ void lane_init(LaneMq& m, uint32_t* ring_col, const uint32_t* __restrict__ words,
+                                         uint32_t nwords) {
     m.bp = 0;
     m.cur = lane_byte(m, 0);
-    m.C = m.cur << 16;
-    lane_bytein(m);
-    m.C <<= 7;
-    m.CT -= 7;
+    m.C = (uint64_t)m.cur << 55;
+    m.lsb = 55;
+    lane_fill(m, ring_col, words, nwords);
     m.A = 0x8000;
 }
 
 // One raw bit (BYPASS segments, T.800 D.6): MSB first; the byte after 0xFF
 // holds 7 bits below a stuffed 0; past the segment the bytes read 0xFF.  cur
 // is the byte at bp and CT the bits of it still unread.
-__device__ __forceinline__ uint32_t lane_raw(LaneMq& m) {
+__device__ __forceinline__ uint32_t lane_raw(LaneMq& m, uint32_t* ring_col, const uint32_t* __restrict__ words,
+                                            uint32_t nwords) {
     if (m.CT == 0) {
+        if (m.rend - (m.base + m.bp) < 8u) lane_refill(m, ring_col, words, nwords);
         const uint32_t prev = m.cur;
         m.bp += 1;
         m.cur = lane_byte(m, m.bp);
@@ -173,8 +219,8 @@ __device__ __forceinline__ bool lock_raw_pass(uint32_t style, int k) {
 }
 
 // DECODE, T.800 Figures C.15-C.18, on a packed context state
-// (Qe | I << 16 | MPS << 22); RENORMD shifts by whole runs up to the next
-// BYTEIN.
+// (Qe | I << 16 | MPS << 22), with the codeword taken ahead by lane_fill
+// (lsb <= 48 on entry): RENORMD is one shift.
 __device__ __forceinline__ uint32_t lane_decode(LaneMq& m, uint32_t& st, const uint64_t* trans) {
     // Nearly every step has some lane that renormalises, so the exchange is
     // computed for all lanes with selects rather than behind a branch.
@@ -182,8 +228,9 @@ __device__ __forceinline__ uint32_t lane_decode(LaneMq& m, uint32_t& st, const u
     const uint32_t mps = (st >> 22) & 1u;
     const uint64_t e = trans[(st >> 16) & 63];
     uint32_t A = m.A - qe;
-    const bool lps = (m.C >> 16) < qe;
-    m.C = lps ? m.C : m.C - (qe << 16);
+    const uint32_t chi = (uint32_t)(m.C >> 32);
+    const bool lps = (chi >> 16) < qe;
+    m.C = ((uint64_t)(lps ? chi : chi - (qe << 16)) << 32) | (uint32_t)m.C;
     const bool renorm = lps || A < 0x8000u;
     const bool take_mps = lps == (A < qe);  // LPS exchange: A < Qe gives MPS; MPS exchange: the reverse
     const uint32_t e2 = (uint32_t)(e >> 32);
@@ -195,16 +242,10 @@ __device__ __forceinline__ uint32_t lane_decode(LaneMq& m, uint32_t& st, const u
     const uint32_t d = (renorm && !take_mps) ? (mps ^ 1u) : mps;
     st = renorm ? st2 : st;
     A = lps ? qe : A;
-    int n = __clz(A) - 16;  // 0 unless renormalising
-    while (n > 0) {
-        lane_bytein(m, m.CT == 0);
-        const int s = n < m.CT ? n : m.CT;
-        A <<= s;
-        m.C <<= s;
-        m.CT -= s;
-        n -= s;
-    }
-    m.A = A;
+    const int n = __clz(A) - 16;  // 0 unless renormalising
+    m.A = A << n;
+    m.C <<= n;
+    m.lsb += n;
     return d;
 }
 
@@ -259,11 +300,9 @@ __device__ __forceinline__ void stripe_symbols(LdsLock<M>& L, int lane, LaneMq& 
     };
     int x = 0, r = 0;
     while (__ballot(work)) {
-        // top up the rings of lanes running short before anyone reads past them
-        if (__ballot(work && m.rend - (m.base + m.bp) < 4u)) {
-            if (work && m.rend - (m.base + m.bp) < 16u) lane_refill(m, ring_col, words, nwords);
-        }
         if (work) {
+            // codeword bytes for the lanes whose register runs low
+            if (!m.raw && m.lsb >= 32) lane_fill(m, ring_col, words, nwords);
             bool rl = false;
             if (pend == PEND_NONE) {
                 if (cur == 0) {
@@ -298,7 +337,7 @@ __device__ __forceinline__ void stripe_symbols(LdsLock<M>& L, int lane, LaneMq& 
                 sxor = e >> 4;
             }
             uint32_t st = L.ctx[lab][lane];
-            const uint32_t bit = m.raw ? lane_raw(m) : lane_decode(m, st, L.trans);
+            const uint32_t bit = m.raw ? lane_raw(m, ring_col, words, nwords) : lane_decode(m, st, L.trans);
             L.ctx[lab][lane] = st;
             uint32_t* const at = out + (uint32_t)r * pitch + x;
             if (TYPE == 1) {
@@ -392,8 +431,9 @@ __device__ void decode_lockstep(LdsLock<M>& L, const tc_cblk_desc& d, bool activ
     }
     if (active) {
         lane_refill(m, ring_col, words, nwords);
-        lane_init(m);
+        lane_init(m, ring_col, words, nwords);
     }
+    asm volatile("" : "+v"(m.len));  // the length's load is waited for here, not in the symbol loop
 
     auto gat = [&](int mask, int row) -> M& { return g[(mask * MAXH + row) * bpw + lane]; };
     auto lr = [&](int i) -> M& { return L.rows[i][lane]; };
@@ -410,6 +450,10 @@ __device__ void decode_lockstep(LdsLock<M>& L, const tc_cblk_desc& d, bool activ
             m.base += m.len;
             seg += 1;
             m.len = seg < nseg ? words[tab + seg] : 0u;
+            // Consume the loaded length here: otherwise the wait for its
+            // load lands on its first use, inside the symbol loop, where it
+            // runs every step and waits for the coefficient stores in flight.
+            asm volatile("" : "+v"(m.len));
             m.bp = 0;
             lane_refill(m, ring_col, words, nwords);
             if (lock_raw_pass(style, k)) {
@@ -418,7 +462,7 @@ __device__ void decode_lockstep(LdsLock<M>& L, const tc_cblk_desc& d, bool activ
                 m.CT = 8;
             } else {
                 m.raw = 0;
-                lane_init(m);
+                lane_init(m, ring_col, words, nwords);
             }
         }
         for (int y0 = 0; y0 < hmax; y0 += 4) {
@@ -524,8 +568,8 @@ __device__ void decode_lockstep(LdsLock<M>& L, const tc_cblk_desc& d, bool activ
         if (pass_on && type == 2 && (style & TC_MODE_SEGSYM)) {
             // segmentation symbol: four uniform-context decisions after each
             // cleanup pass (decoded to keep the MQ state in step, not checked)
-            if (m.rend - (m.base + m.bp) < 16u) lane_refill(m, ring_col, words, nwords);
             for (int s4 = 0; s4 < 4; s4++) {
+                if (m.lsb >= 32) lane_fill(m, ring_col, words, nwords);
                 uint32_t st = L.ctx[18][lane];
                 (void)lane_decode(m, st, L.trans);
                 L.ctx[18][lane] = st;
