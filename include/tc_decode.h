@@ -165,7 +165,8 @@ tc_status tc_decoder_set_tuning(tc_decoder dec, const tc_tuning *in);
 /* Autotuner: times the prepared batch under candidate code-block-to-wave
  * packings (per block class, in up to two rounds, since the two classes'
  * launches share the tier-1 stage), heavy-block thresholds and inverse-DWT
- * tile shapes, keeps the fastest, and re-prepares with it.  *best (may be
+ * tile shapes, keeps the fastest (a candidate must beat the incumbent by more
+ * than 1 %, the run-to-run spread), and re-prepares with it.  *best (may be
  * NULL) receives the choice. */
 tc_status tc_decoder_autotune(tc_decoder dec, int reps, tc_tuning *best);
 

@@ -359,6 +359,11 @@ tc_status tc_decoder_autotune(tc_decoder d, int reps, tc_tuning* best) {
         return report(s, r == Status::Ok ? why : why + "; preparing again with the starting tuning failed too: " + e2);
     };
     float t1 = 0, dw = 0, best_t1, best_dwt;
+    // A candidate replaces the incumbent only when it is faster by more than
+    // the run-to-run spread of one measurement (about 1 %): near-equal
+    // packings otherwise win by noise, and a heavy-block split chosen that
+    // way costs its extra launch for nothing.
+    constexpr float kBetter = 0.99f;
     Status s = measure(cur, &best_t1, &best_dwt);
     if (s != Status::Ok) return fail(s);
     const int cand[] = {1, TC_BPW_LOCKSTEP, TC_BPW_LOCKSTEP32, TC_BPW_LOCKSTEP16};
@@ -379,7 +384,7 @@ tc_status tc_decoder_autotune(tc_decoder d, int reps, tc_tuning* best) {
                 if ((cls == 0 ? cur.bpw32 : cur.bpw64) == b) continue;
                 s = measure(t, &t1, &dw);
                 if (s != Status::Ok) return fail(s);
-                if (t1 < best_t1) {
+                if (t1 < best_t1 * kBetter) {
                     best_t1 = t1;
                     cur = t;
                 }
@@ -405,7 +410,7 @@ tc_status tc_decoder_autotune(tc_decoder d, int reps, tc_tuning* best) {
             (cls == 0 ? t.heavy32 : t.heavy64) = c[q];
             s = measure(t, &t1, &dw);
             if (s != Status::Ok) return fail(s);
-            if (t1 < best_t1) {
+            if (t1 < best_t1 * kBetter) {
                 best_t1 = t1;
                 cur = t;
             }
@@ -419,7 +424,7 @@ tc_status tc_decoder_autotune(tc_decoder d, int reps, tc_tuning* best) {
         t.dwt_shape = sh;
         s = measure(t, &t1, &dw);
         if (s != Status::Ok) return fail(s);
-        if (dw < best_dwt) {
+        if (dw < best_dwt * kBetter) {
             best_dwt = dw;
             cur = t;
         }
