@@ -413,7 +413,8 @@ def bench_served(tca, small, large, loads, args, serial_requests=300):
     out = {"large_fraction": 0.1, "serial_per_image": ser, "loads": []}
     srv = tca.Server(slots=args.slots, max_batch=args.max_batch, bpw_small=args.bpw_small, bpw_large=args.bpw_large,
                      max_batch_bytes=args.batch_mib << 20)
-    serve.run(srv, seq[:300], max(loads), seed=1)  # warm: resident buffers reach their working size
+    # warm: the slots' resident device buffers and pinned buffers reach their working size
+    serve.run(srv, seq[:1000], max(loads), seed=1)
     best = None
     for rate in loads:
         n = min(int(5 * rate), 3000)
@@ -438,7 +439,7 @@ def bench_served(tca, small, large, loads, args, serial_requests=300):
     out["max_sustained"] = None if best is None else {
         k: best[k] for k in ("offered_rps", "achieved_rps", "mpixel_s", "p50_ms", "p99_ms", "speedup_vs_serial")}
     out["peak"] = max(out["loads"], key=lambda r: r["mpixel_s"])["mpixel_s"]
-    out["warm"] = "one untimed pass of 300 requests before the timed loads"
+    out["warm"] = "one untimed pass of 1000 requests before the timed loads"
     out["outputs"] = (f"{len(keep)} sampled requests checked against per-image decode "
                       "(lossless identical, lossy within 1 LSB)")
     return out

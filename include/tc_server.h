@@ -51,7 +51,8 @@ typedef struct {
     uint64_t submitted, completed, failed, batches;
     uint64_t pixels;          /* decoded pixels */
     double busy_s;            /* time with at least one batch on the device */
-    uint64_t peak_bytes;      /* largest admitted footprint of all slots */
+    uint64_t peak_bytes;      /* largest footprint of the batches holding device buffers at once
+                                 (a slot plans its next batch while one runs: that one is not counted) */
     int32_t queued, in_flight;
     double plan_s;            /* summed over batches: parsing, planning and staging (host) */
     double device_s;          /* launch to results in host memory */

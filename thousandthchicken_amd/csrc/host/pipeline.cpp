@@ -545,6 +545,10 @@ void Pipeline::release_buffers() {
     }
 }
 
+Status Pipeline::plan_batch(const std::vector<const Image*>& imgs, Plan& P, std::string& err) const {
+    return build_plan(imgs, tuning_, device_cu_count(dev_), P, err);
+}
+
 Status Pipeline::prepare(const std::vector<const Image*>& imgs, std::string& err) {
     invalidate();
     if (!stream_) {
@@ -552,9 +556,26 @@ Status Pipeline::prepare(const std::vector<const Image*>& imgs, std::string& err
         return Status::Device;
     }
     tc_stream_sync(stream_);
-    Status st = build_plan(imgs, tuning_, device_cu_count(dev_), plan_, err);
+    Plan P;
+    Status st = plan_batch(imgs, P, err);
+    if (st == Status::Ok) st = ensure_pinned(P.upload_bytes(), err);
+    if (st == Status::Ok) {
+        stage_upload(P, (uint8_t*)pinned_);
+        st = adopt(std::move(P), pinned_, err);
+    }
+    if (st != Status::Ok) invalidate();
+    return st;
+}
+
+Status Pipeline::adopt(Plan&& P, const void* staged, std::string& err) {
+    invalidate();
+    if (!stream_) {
+        err = "no stream";
+        return Status::Device;
+    }
+    plan_ = std::move(P);
     const uint64_t total = plan_.upload_bytes();
-    if (st == Status::Ok) st = ensure(upload_, total, "descriptors", err);
+    Status st = ensure(upload_, total, "descriptors", err);
     if (st == Status::Ok) st = ensure(samples_, std::max<uint64_t>(plan_.sample_elems, 1) * 4, "sample planes", err);
     if (st == Status::Ok && !host_out_) st = ensure(out_, std::max<uint64_t>(plan_.out_bytes, 1), "output images", err);
     if (st == Status::Ok && host_out_ && host_cap_ < plan_.out_bytes) {
@@ -571,10 +592,8 @@ Status Pipeline::prepare(const std::vector<const Image*>& imgs, std::string& err
         }
     }
     if (st == Status::Ok && plan_.scratch_bytes) st = ensure(scratch_, plan_.scratch_bytes, "tier-1 state", err);
-    if (st == Status::Ok) st = ensure_pinned(total, err);
     if (st == Status::Ok) {
-        stage_upload(plan_, (uint8_t*)pinned_);
-        tc_status ts = tc_buf_copy_h2d(stream_, upload_, 0, pinned_, total);
+        tc_status ts = tc_buf_copy_h2d(stream_, upload_, 0, staged, total);
         if (ts != TC_SUCCESS) {
             err = tc_last_error();
             st = from_tc(ts);

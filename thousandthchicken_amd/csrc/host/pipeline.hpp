@@ -165,6 +165,13 @@ class Pipeline {
     // failed prepare leaves no plan: run, capture_graph, fetch and the output
     // queries then fail with InvalidArgument until a prepare succeeds.
     Status prepare(const std::vector<const Image*>& imgs, std::string& err);
+    // The two halves of prepare, for callers that plan the next batch while
+    // the current one runs: plan_batch is host work only (it touches no device
+    // state); adopt makes the buffers fit P and enqueues the upload of its
+    // staged region (stage_upload's P.upload_bytes() bytes at `staged`, which
+    // must stay unchanged until the stream has passed the copy).
+    Status plan_batch(const std::vector<const Image*>& imgs, Plan& P, std::string& err) const;
+    Status adopt(Plan&& P, const void* staged, std::string& err);
     // Drops the current plan (as a failed prepare does).
     void invalidate();
     bool ready() const { return ready_; }

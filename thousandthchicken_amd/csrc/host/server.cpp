@@ -36,19 +36,37 @@ struct Request {
     bool moving = false;           // a worker is copying the result out of pinned memory
 };
 
+// A batch of one slot: its requests, where their results go in the slot's
+// pinned result buffer, its plan and the host copy of its upload region.
+struct Batch {
+    std::vector<Request*> reqs;
+    std::vector<Request*> rejected;  // failed to parse
+    std::vector<size_t> offs;
+    uint64_t footprint = 0;
+    int buf = 0;                     // pinned result buffer (0 or 1)
+    size_t out_total = 0;            // bytes of the results in it
+    Plan plan;
+    tc_status status = TC_SUCCESS;
+    double t_plan = 0, t_launch = 0;
+};
+
 struct Slot {
     std::unique_ptr<Pipeline> pipe;
-    std::vector<Request*> batch;
-    tc_event ev = nullptr;
+    tc_event ev = nullptr;     // the running batch's results are in pinned memory
+    tc_event kdone = nullptr;  // the running batch's kernels have finished
     // Results land in one of two pinned buffers and stay there until read; a
     // buffer is reused every other batch, after moving out what is unread.
     void* pinned[2] = {nullptr, nullptr};
     size_t pinned_cap[2] = {0, 0};
     std::vector<uint64_t> holders[2];  // request ids with results in each buffer
     int cur = 0;
-    std::vector<size_t> offs;
-    std::vector<Request*> rejected;  // of the current batch: failed to parse
-    uint64_t footprint = 0;
+    // Host copy of the upload region.  The next batch is staged into it once
+    // the running batch's upload has left (up_done; long before its kernels
+    // end).
+    void* staging = nullptr;
+    size_t staging_cap = 0;
+    tc_event up_done = nullptr;
+    bool up_pending = false;
     std::thread worker;
 };
 
@@ -95,10 +113,11 @@ constexpr double kSlotGrowth = 1.25;
 }  // namespace
 
 // One worker thread per slot: it takes the next batch from the queue (FIFO,
-// within the slot's share of the memory budget), prepares and launches it on
-// the slot's stream, waits for it, and copies the results out.  Host work of
-// one slot (planning, staging, result copies) overlaps the device work of the
-// others, and batches of different slots run concurrently on the device.
+// within the slot's share of the memory budget), plans it while the slot's
+// previous batch runs, then launches it on the slot's stream once that one
+// has finished.  Host work (parsing, planning, staging) overlaps the slot's
+// own device work and that of the other slots, and batches of different
+// slots run concurrently on the device.
 // Threads one slot uses to parse a large batch's codestreams.
 constexpr int kParseThreads = 4;
 
@@ -117,6 +136,15 @@ struct tc_server_s {
     double busy_since = -1;
     int busy_slots = 0;
     uint64_t inflight_bytes = 0;
+    // Kernels of different slots run one batch after the other: each batch's
+    // first kernel waits for the last kernel of the batch launched before it
+    // (on whichever slot), while uploads, result copies and host work of the
+    // slots overlap them.  Two batches whose kernels shared the device took
+    // longer together than one after the other: each is bound by its longest
+    // tier-1 chains, which slow down when another batch's waves share the
+    // compute units.
+    std::mutex compute_mu;
+    tc_event last_compute = nullptr;  // kdone of the slot that launched kernels last
     int idle_workers = 0;
 
     double now() const {
@@ -124,35 +152,31 @@ struct tc_server_s {
     }
 
     // caller holds mu
-    void slot_idle(Slot& s) {
-        inflight_bytes -= s.footprint;
-        s.footprint = 0;
-        s.batch.clear();
+    void batch_done(Batch& b) {
+        inflight_bytes -= b.footprint;
+        b.footprint = 0;
         if (--busy_slots == 0 && busy_since >= 0) {
             st.busy_s += now() - busy_since;
             busy_since = -1;
         }
     }
 
-    // Runs a batch on slot s without the lock: plan, upload, launch, copy the
-    // results to pinned memory, wait.
-    tc_status execute(Slot& s) {
-        std::vector<const Image*> imgs;
-        size_t total = 0;
-        s.offs.clear();
+    // Host half of a batch, without the lock and while the slot's previous
+    // batch runs on the device: parse (on a few threads when the batch is
+    // large; a request whose packets do not parse fails alone), plan, and
+    // gather the upload region into the batch's staging buffer.
+    tc_status plan_host(Slot& s, Batch& b) {
         const double tp = now();
-        // full parse, on a few threads when the batch is large; a request whose
-        // packets do not parse fails alone
-        std::vector<char> good(s.batch.size(), 0);
+        std::vector<char> good(b.reqs.size(), 0);
         size_t bytes = 0;
-        for (Request* r : s.batch) bytes += r->cs_len;
-        const int nt = (int)std::min<size_t>({(size_t)kParseThreads, s.batch.size(), 1 + bytes / (4u << 20)});
+        for (Request* r : b.reqs) bytes += r->cs_len;
+        const int nt = (int)std::min<size_t>({(size_t)kParseThreads, b.reqs.size(), 1 + bytes / (4u << 20)});
         auto parse_part = [&](int k) {
             std::string perr;
-            for (size_t i = (size_t)k; i < s.batch.size(); i += (size_t)nt)
+            for (size_t i = (size_t)k; i < b.reqs.size(); i += (size_t)nt)
                 // (sub-sampled images, whose output is planar, are not served)
-                good[i] = parse_codestream(s.batch[i]->cs, s.batch[i]->cs_len, s.batch[i]->img, perr) == Status::Ok &&
-                          !s.batch[i]->img.planar();
+                good[i] = parse_codestream(b.reqs[i]->cs, b.reqs[i]->cs_len, b.reqs[i]->img, perr) == Status::Ok &&
+                          !b.reqs[i]->img.planar();
         };
         if (nt <= 1) {
             parse_part(0);
@@ -163,150 +187,236 @@ struct tc_server_s {
             for (auto& t : th) t.join();
         }
         std::vector<Request*> ok;
-        for (size_t i = 0; i < s.batch.size(); i++) (good[i] ? ok : s.rejected).push_back(s.batch[i]);
-        s.batch.swap(ok);
-        if (s.batch.empty()) return TC_SUCCESS;
-        for (Request* r : s.batch) {
+        for (size_t i = 0; i < b.reqs.size(); i++) (good[i] ? ok : b.rejected).push_back(b.reqs[i]);
+        b.reqs.swap(ok);
+        b.offs.clear();
+        if (b.reqs.empty()) return TC_SUCCESS;
+        std::vector<const Image*> imgs;
+        size_t total = 0;
+        for (Request* r : b.reqs) {
             imgs.push_back(&r->img);
-            s.offs.push_back(total);
+            b.offs.push_back(total);
             total += (r->out_bytes + 255) & ~(size_t)255;
         }
         std::string err;
-        if (s.pipe->prepare(imgs, err) != Status::Ok) {
+        if (s.pipe->plan_batch(imgs, b.plan, err) != Status::Ok) {
             set_error(err);
             return TC_ERR_DEVICE;
         }
-        void*& pin = s.pinned[s.cur];
-        if (!pin || s.pinned_cap[s.cur] < total) {
-            if (pin) tc_host_free_pinned(pin);
-            pin = nullptr;
-            s.pinned_cap[s.cur] = 0;
-            const size_t want = std::max<size_t>(2 * total, 16 << 20);
-            tc_status e = tc_host_alloc_pinned(want, &pin);
+        b.out_total = total;
+        // Pinned memory for the upload.  Allocating and freeing pinned memory
+        // stalls the whole device, so this buffer and the result buffers
+        // start large (the upload at the batch byte cap), grow by doubling
+        // and are freed only with the server.  The staging buffer is replaced
+        // or rewritten only once the running batch's upload has left.
+        const size_t up = b.plan.upload_bytes();
+        if (s.up_pending) {
+            tc_status e = tc_event_sync(s.up_done);
             if (e) return e;
-            s.pinned_cap[s.cur] = want;
+            s.up_pending = false;
         }
-        const double tr = now();
-        if (s.pipe->run(false) != Status::Ok) return TC_ERR_DEVICE;
-        hipStream_t hs = stream_handle(s.pipe->stream());
-        for (size_t i = 0; i < s.batch.size(); i++) {
-            hipError_t e = hipMemcpyAsync((uint8_t*)pin + s.offs[i], s.pipe->output_ptr((int)i),
-                                          s.batch[i]->out_bytes, hipMemcpyDeviceToHost, hs);
-            if (e != hipSuccess) return hip_fail(e, "result copy");
+        if (!s.staging || s.staging_cap < up) {
+            if (s.staging) tc_host_free_pinned(s.staging);
+            s.staging = nullptr;
+            const size_t want = std::max<size_t>({2 * up, 2 * s.staging_cap, (size_t)cfg.max_batch_bytes + (64u << 20)});
+            s.staging_cap = 0;
+            tc_status e = tc_host_alloc_pinned(want, &s.staging);
+            if (e) return e;
+            s.staging_cap = want;
         }
-        tc_status e = tc_event_record(s.ev, s.pipe->stream());
-        if (!e) e = tc_event_sync(s.ev);
-        const double td = now();
+        stage_upload(b.plan, (uint8_t*)s.staging);
         std::lock_guard<std::mutex> lk(mu);
-        st.plan_s += tr - tp;
-        st.device_s += td - tr;
-        return e;
+        st.plan_s += now() - tp;
+        return TC_SUCCESS;
     }
 
-    void work(Slot& s) {
-        const uint64_t slot_budget = cfg.memory_budget / (uint64_t)std::max(1, cfg.slots);
-        const uint64_t batch_budget = (uint64_t)((double)slot_budget / kSlotGrowth);
-        std::unique_lock<std::mutex> lk(mu);
-        for (;;) {
-            idle_workers++;
-            cv_work.wait(lk, [&] { return stop || !queue.empty(); });
-            idle_workers--;
-            if (queue.empty()) break;  // stopping and drained
-            // admission: FIFO while the batch fits the slot's memory share and
-            // the per-batch byte cap (which also bounds the staging buffers, so
-            // that they stop growing once warm), and at most this worker's share
-            // of the queued codestream bytes when other slots are idle (so that
-            // large requests spread over slots and one slot's host work overlaps
-            // another's device work)
-            uint64_t queued = 0;
-            for (const Request* r : queue) queued += r->cs_len;
-            const uint64_t share = std::min<uint64_t>(queued / (uint64_t)(idle_workers + 1), cfg.max_batch_bytes);
-            uint64_t fp = 0, taken = 0;
-            while (!queue.empty() && (int)s.batch.size() < cfg.max_batch) {
-                Request* r = queue.front();
-                if (!s.batch.empty() &&
-                    (fp + r->footprint > batch_budget || taken >= share || taken + r->cs_len > cfg.max_batch_bytes))
-                    break;
-                fp += r->footprint;
-                taken += r->cs_len;
-                s.batch.push_back(r);
-                queue.pop_front();
+    // Device half, once the slot's previous batch has finished: fit the
+    // buffers, upload, launch, copy the results to pinned memory and record
+    // the slot's event (nothing waits here).
+    tc_status launch(Slot& s, Batch& b) {
+        b.t_launch = now();
+        if (b.reqs.empty()) return TC_SUCCESS;
+        vacate(s, b.buf);
+        void*& pin = s.pinned[b.buf];
+        if (!pin || s.pinned_cap[b.buf] < b.out_total) {
+            if (pin) tc_host_free_pinned(pin);
+            pin = nullptr;
+            const size_t want = std::max<size_t>({2 * b.out_total, 2 * s.pinned_cap[b.buf], (size_t)256 << 20});
+            s.pinned_cap[b.buf] = 0;
+            tc_status e = tc_host_alloc_pinned(want, &pin);
+            if (e) return e;
+            s.pinned_cap[b.buf] = want;
+        }
+        std::string err;
+        if (s.pipe->adopt(std::move(b.plan), s.staging, err) != Status::Ok) {
+            set_error(err);
+            return TC_ERR_DEVICE;
+        }
+        if (tc_event_record(s.up_done, s.pipe->stream()) == TC_SUCCESS) s.up_pending = true;
+        {
+            std::lock_guard<std::mutex> g(compute_mu);
+            if (last_compute && tc_stream_wait_event(s.pipe->stream(), last_compute) != TC_SUCCESS) return TC_ERR_DEVICE;
+            if (s.pipe->run(false) != Status::Ok) return TC_ERR_DEVICE;
+            tc_status e = tc_event_record(s.kdone, s.pipe->stream());
+            if (e != TC_SUCCESS) return e;
+            last_compute = s.kdone;
+        }
+        hipStream_t hs = stream_handle(s.pipe->stream());
+        for (size_t i = 0; i < b.reqs.size(); i++) {
+            hipError_t e = hipMemcpyAsync((uint8_t*)s.pinned[b.buf] + b.offs[i], s.pipe->output_ptr((int)i),
+                                          b.reqs[i]->out_bytes, hipMemcpyDeviceToHost, hs);
+            if (e != hipSuccess) return hip_fail(e, "result copy");
+        }
+        return tc_event_record(s.ev, s.pipe->stream());
+    }
+
+    // Waits for a launched batch and publishes its results.
+    void finish(Slot& s, Batch& b) {
+        tc_status e = b.status;
+        if (e == TC_SUCCESS && !b.reqs.empty()) e = tc_event_sync(s.ev);
+        const double tc0 = now();
+        if (e != TC_SUCCESS) tc_stream_sync(s.pipe->stream());
+        // a slot keeps its buffers for the next batch unless they outgrew its
+        // share of the budget (a request larger than the share runs alone and
+        // its buffers go with it)
+        const uint64_t held = s.pipe->device_bytes();
+        if (held > slot_budget()) s.pipe->release_buffers();
+        std::lock_guard<std::mutex> lk(mu);
+        if (!b.reqs.empty()) st.device_s += tc0 - b.t_launch;
+        st.slot_device_peak = std::max<uint64_t>(st.slot_device_peak, held);
+        if (e == TC_SUCCESS)
+            for (size_t i = 0; i < b.reqs.size(); i++) {
+                Request* r = b.reqs[i];
+                r->pin = (const uint8_t*)s.pinned[b.buf] + b.offs[i];
+                s.holders[b.buf].push_back(r->id);
             }
-            const uint64_t seq = ++batch_seq;
-            const double t = now();
-            for (Request* r : s.batch) {
-                r->info.start_s = t;
-                r->info.batch = seq;
-                r->info.batch_size = (int32_t)s.batch.size();
-            }
-            s.footprint = fp;
-            inflight_bytes += fp;
-            st.peak_bytes = std::max(st.peak_bytes, inflight_bytes);
-            if (busy_slots++ == 0) busy_since = t;
-            st.batches++;
-            if (!queue.empty()) cv_work.notify_one();  // another slot can start on the rest
-            s.cur ^= 1;
-            std::vector<Request*> unread;
-            for (uint64_t id : s.holders[s.cur]) {
-                auto it = reqs.find(id);
-                if (it != reqs.end() && it->second->pin) unread.push_back(it->second.get());
-            }
-            s.holders[s.cur].clear();
-            // results nobody read since the batch before last leave the buffer
-            // before it is reused; the copy runs without the lock (so that
-            // submissions and other slots do not wait for it), and readers of
-            // these requests wait for it to finish
-            for (Request* r : unread) r->moving = true;
-            if (!unread.empty()) {
-                lk.unlock();
-                for (Request* r : unread) r->out.assign(r->pin, r->pin + r->out_bytes);
-                lk.lock();
-                for (Request* r : unread) {
-                    r->pin = nullptr;
-                    r->moving = false;
-                }
-                cv_done.notify_all();
-            }
-            lk.unlock();
-            const tc_status e = execute(s);
-            const double tc0 = now();
-            if (e != TC_SUCCESS) tc_stream_sync(s.pipe->stream());
-            // a slot keeps its buffers for the next batch unless they outgrew
-            // its share of the budget (a request larger than the share runs
-            // alone and its buffers go with it)
-            const uint64_t held = s.pipe->device_bytes();
-            if (held > slot_budget) s.pipe->release_buffers();
-            lk.lock();
-            st.slot_device_peak = std::max<uint64_t>(st.slot_device_peak, held);
-            if (e == TC_SUCCESS)
-                for (size_t i = 0; i < s.batch.size(); i++) {
-                    Request* r = s.batch[i];
-                    r->pin = (const uint8_t*)s.pinned[s.cur] + s.offs[i];
-                    s.holders[s.cur].push_back(r->id);
-                }
-            st.copy_s += now() - tc0;
-            const double td = now();
-            for (Request* r : s.rejected) {
-                r->info.done_s = td;
-                r->info.status = TC_ERR_UNSUPPORTED;
-                r->done = true;
+        const double td = now();
+        st.copy_s += td - tc0;
+        for (Request* r : b.rejected) {
+            r->info.done_s = td;
+            r->info.status = TC_ERR_UNSUPPORTED;
+            r->done = true;
+            st.failed++;
+        }
+        for (Request* r : b.reqs) {
+            r->info.done_s = td;
+            r->info.status = e;
+            r->done = true;
+            if (e == TC_SUCCESS) {
+                st.completed++;
+                st.pixels += (uint64_t)r->img.width() * r->img.height();
+            } else {
                 st.failed++;
             }
-            s.rejected.clear();
-            for (Request* r : s.batch) {
-                r->info.done_s = td;
-                r->info.status = e;
-                r->done = true;
-                if (e == TC_SUCCESS) {
-                    st.completed++;
-                    st.pixels += (uint64_t)r->img.width() * r->img.height();
-                } else {
-                    st.failed++;
-                }
-            }
-            slot_idle(s);
-            cv_done.notify_all();
         }
+        batch_done(b);
+        cv_done.notify_all();
+    }
+
+    uint64_t slot_budget() const { return cfg.memory_budget / (uint64_t)std::max(1, cfg.slots); }
+
+    // Takes the next batch from the queue into b (caller holds mu, queue not
+    // empty): FIFO while the batch fits the slot's memory share and the
+    // per-batch byte cap (which also bounds the staging buffers, so that they
+    // stop growing once warm), and at most this worker's share of the queued
+    // codestream bytes when other slots are idle (so that large requests
+    // spread over slots and one slot's host work overlaps another's device
+    // work).
+    void admit(Slot& s, Batch& b) {
+        const uint64_t batch_budget = (uint64_t)((double)slot_budget() / kSlotGrowth);
+        uint64_t queued = 0;
+        for (const Request* r : queue) queued += r->cs_len;
+        const uint64_t share = std::min<uint64_t>(queued / (uint64_t)(idle_workers + 1), cfg.max_batch_bytes);
+        uint64_t fp = 0, taken = 0;
+        while (!queue.empty() && (int)b.reqs.size() < cfg.max_batch) {
+            Request* r = queue.front();
+            if (!b.reqs.empty() &&
+                (fp + r->footprint > batch_budget || taken >= share || taken + r->cs_len > cfg.max_batch_bytes))
+                break;
+            fp += r->footprint;
+            taken += r->cs_len;
+            b.reqs.push_back(r);
+            queue.pop_front();
+        }
+        const uint64_t seq = ++batch_seq;
+        const double t = now();
+        for (Request* r : b.reqs) {
+            r->info.start_s = t;
+            r->info.batch = seq;
+            r->info.batch_size = (int32_t)b.reqs.size();
+        }
+        b.footprint = fp;  // charged to inflight_bytes while the batch holds the slot's device buffers
+        if (busy_slots++ == 0) busy_since = t;
+        st.batches++;
+        if (!queue.empty()) cv_work.notify_one();  // another slot can start on the rest
+        s.cur ^= 1;
+        b.buf = s.cur;
+    }
+
+    // Results still unread in pinned buffer k leave it before the buffer is
+    // written again (caller does not hold mu): they are copied out without
+    // the lock, and readers of these requests wait for the copy.  This runs
+    // when the batch that reuses the buffer launches, one whole batch after
+    // the results were published, so readers have usually taken them.
+    void vacate(Slot& s, int k) {
+        std::unique_lock<std::mutex> lk(mu);
+        std::vector<Request*> unread;
+        for (uint64_t id : s.holders[k]) {
+            auto it = reqs.find(id);
+            if (it != reqs.end() && it->second->pin) unread.push_back(it->second.get());
+        }
+        s.holders[k].clear();
+        if (unread.empty()) return;
+        for (Request* r : unread) r->moving = true;
+        lk.unlock();
+        for (Request* r : unread) r->out.assign(r->pin, r->pin + r->out_bytes);
+        lk.lock();
+        for (Request* r : unread) {
+            r->pin = nullptr;
+            r->moving = false;
+        }
+        cv_done.notify_all();
+    }
+
+    // One worker per slot.  Its batches are pipelined: while batch n runs on
+    // the slot's stream, the worker admits batch n+1 and does its host work
+    // (parse, plan, gather), then waits for batch n, publishes it and
+    // launches n+1.  So the slot's device stream is fed without a gap for
+    // host work, and several slots still overlap with each other.
+    void work(Slot& s) {
+        std::unique_ptr<Batch> running;  // launched, not yet finished
+        std::unique_lock<std::mutex> lk(mu);
+        for (;;) {
+            if (!running) {
+                idle_workers++;
+                cv_work.wait(lk, [&] { return stop || !queue.empty(); });
+                idle_workers--;
+                if (queue.empty()) break;  // stopping and drained
+            }
+            std::unique_ptr<Batch> next;
+            if (!queue.empty()) {
+                next.reset(new Batch());
+                admit(s, *next);
+            }
+            lk.unlock();
+            if (next) next->status = plan_host(s, *next);
+            if (running) {
+                finish(s, *running);
+                running.reset();
+            }
+            lk.lock();
+            if (next) {
+                // a slot's batches take its device buffers one after the other
+                inflight_bytes += next->footprint;
+                st.peak_bytes = std::max(st.peak_bytes, inflight_bytes);
+                lk.unlock();
+                if (next->status == TC_SUCCESS) next->status = launch(s, *next);
+                running = std::move(next);
+                lk.lock();
+            }
+        }
+        lk.unlock();
+        if (running) finish(s, *running);
     }
 };
 
@@ -328,7 +438,8 @@ tc_status tc_server_create(tc_device dev, const tc_server_config* cfg, tc_server
     s->slots.resize(s->cfg.slots);
     for (Slot& sl : s->slots) {
         sl.pipe.reset(new Pipeline(dev));
-        if (!sl.pipe->stream() || tc_event_create(&sl.ev) != TC_SUCCESS) {
+        if (!sl.pipe->stream() || tc_event_create(&sl.ev) != TC_SUCCESS || tc_event_create(&sl.kdone) != TC_SUCCESS ||
+            tc_event_create(&sl.up_done) != TC_SUCCESS) {
             delete s;
             return fail(TC_ERR_DEVICE, "tc_server_create: stream/event creation failed");
         }
@@ -356,8 +467,11 @@ tc_status tc_server_destroy(tc_server s) {
     for (Slot& sl : s->slots) {
         if (sl.pipe) tc_stream_sync(sl.pipe->stream());
         tc_event_destroy(sl.ev);
+        tc_event_destroy(sl.kdone);
         for (void* p : sl.pinned)
             if (p) tc_host_free_pinned(p);
+        if (sl.staging) tc_host_free_pinned(sl.staging);
+        tc_event_destroy(sl.up_done);
     }
     delete s;
     return TC_SUCCESS;
