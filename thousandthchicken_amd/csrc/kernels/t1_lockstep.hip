@@ -299,7 +299,9 @@ __device__ __forceinline__ void stripe_symbols(LdsLock<M>& L, int lane, LaneMq& 
         onib = 0;
     };
     int x = 0, r = 0;
-    while (__ballot(work)) {
+    // (a do-while, tested at the bottom: with the test at the top the compiler
+    // copied the lane state twice per step in the refinement loop)
+    if (__ballot(work)) do {
         if (work) {
             // codeword bytes for the lanes whose register runs low
             if (!m.raw && m.lsb >= 32) lane_fill(m, ring_col, words, nwords);
@@ -390,7 +392,7 @@ __device__ __forceinline__ void stripe_symbols(LdsLock<M>& L, int lane, LaneMq& 
             }
             work = pend != PEND_NONE || cur != 0 || cany != 0;
         }
-    }
+    } while (__ballot(work));
     if (TYPE == 1) flush(x);
 }
 
