@@ -87,6 +87,8 @@ Pipeline::Pipeline(tc_device dev) : dev_(dev) {
     for (auto& e : ev_) tc_event_create(&e);
     tc_event_create(&ls_.fork);
     for (auto& e : ls_.join) tc_event_create(&e);
+    tc_event_create(&ls_.dwt_fork);
+    tc_event_create(&ls_.dwt_join);
 }
 
 Pipeline::~Pipeline() {
@@ -97,6 +99,8 @@ Pipeline::~Pipeline() {
     for (auto& e : ev_) tc_event_destroy(e);
     tc_event_destroy(ls_.fork);
     for (auto& e : ls_.join) tc_event_destroy(e);
+    tc_event_destroy(ls_.dwt_fork);
+    tc_event_destroy(ls_.dwt_join);
     for (auto& sd : ls_.side) tc_stream_destroy(sd);
     tc_buf_free(upload_);
     tc_buf_free(samples_);
@@ -513,16 +517,32 @@ tc_status launch_plan(const Plan& P, const PlanRegions& R, int cus, tc_stream st
         if (s == TC_SUCCESS) s = tc_stream_wait_event(stream, S->join[g - 1]);
     }
     if (ev) tc_event_record(ev[2], stream);
+    // The 5/3 and 9/7 tile-components form two independent chains (their
+    // levels, then their final levels): the 9/7 chain runs on a side stream
+    // beside the 5/3 one, so that the small coarse-level launches and the
+    // final launches' tails of one filter overlap the other's.
+    bool irr_any[2] = {false, false};
+    for (const Plan::LevelLaunch& L : P.levels) irr_any[L.irr ? 1 : 0] = true;
+    for (const Plan::FinalLaunch& F : P.finals) irr_any[F.irr ? 1 : 0] = true;
+    const bool split = S && S->side[0] && S->dwt_fork && S->dwt_join && irr_any[0] && irr_any[1];
+    if (split && s == TC_SUCCESS) {
+        s = tc_event_record(S->dwt_fork, stream);
+        if (s == TC_SUCCESS) s = tc_stream_wait_event(S->side[0], S->dwt_fork);
+    }
     for (const Plan::LevelLaunch& L : P.levels) {
         if (s != TC_SUCCESS) break;
-        s = tc_launch_idwt_level(stream, L.irr ? 1 : 0, P.shape, up + L.jobs_off, (const uint32_t*)(up + L.map_off),
-                                 L.nwg, base);
+        s = tc_launch_idwt_level(split && L.irr ? S->side[0] : stream, L.irr ? 1 : 0, P.shape, up + L.jobs_off,
+                                 (const uint32_t*)(up + L.map_off), L.nwg, base);
     }
-    if (ev) tc_event_record(ev[3], stream);
+    if (ev) tc_event_record(ev[3], stream);  // (split: the 5/3 chain's levels)
     for (const Plan::FinalLaunch& F : P.finals) {
         if (s != TC_SUCCESS) break;
-        s = tc_launch_idwt_final(stream, F.irr ? 1 : 0, P.shape, F.nc, F.ob, up + F.jobs_off,
-                                 (const uint32_t*)(up + F.map_off), F.nwg, base, R.out);
+        s = tc_launch_idwt_final(split && F.irr ? S->side[0] : stream, F.irr ? 1 : 0, P.shape, F.nc, F.ob,
+                                 up + F.jobs_off, (const uint32_t*)(up + F.map_off), F.nwg, base, R.out);
+    }
+    if (split && s == TC_SUCCESS) {
+        s = tc_event_record(S->dwt_join, S->side[0]);
+        if (s == TC_SUCCESS) s = tc_stream_wait_event(stream, S->dwt_join);
     }
     if (ev) tc_event_record(ev[4], stream);
     return s;

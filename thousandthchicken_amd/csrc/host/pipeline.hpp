@@ -144,10 +144,12 @@ struct LaunchStreams {
     tc_stream side[kMaxT1Launches - 1] = {};  // launch g > 0 runs on side[g - 1]
     tc_event fork = nullptr;
     tc_event join[kMaxT1Launches - 1] = {};
+    tc_event dwt_fork = nullptr, dwt_join = nullptr;  // the 9/7 DWT chain on side[0]
 };
 
 // Enqueues the plan on `stream`.  With side streams (S, may be null) the
-// tier-1 launches run side by side, the heavy blocks' launches last; with
+// tier-1 launches run side by side, the heavy blocks' launches last, and the
+// 9/7 inverse-DWT chain runs beside the 5/3 one; with
 // ev (5 + 2 * kMaxT1Launches events) the stage boundaries are recorded on
 // `stream` (ev[0..4]) and each tier-1 launch is bracketed on the stream it
 // runs on (ev[5 + 2g], ev[6 + 2g]).
