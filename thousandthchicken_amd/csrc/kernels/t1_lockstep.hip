@@ -279,7 +279,9 @@ __device__ __forceinline__ uint32_t nibble(M a0, M a1, M a2, M a3, int x) {
 // with a significant neighbour (label 15; the rest take 14); the decoded
 // 1-bits are returned as row masks in o0..o3.  TYPE 2: sany is the OR of the
 // six staged significance rows (run-length test).  out points at row y0.
-template <typename M, int TYPE>
+// RAW: some lane of the wave is in a raw (BYPASS) segment; without it the
+// raw paths are not compiled into the symbol loop.
+template <typename M, int TYPE, bool RAW>
 __device__ __forceinline__ void stripe_symbols(LdsLock<M>& L, int lane, LaneMq& m, uint32_t* ring_col,
                                                const uint32_t* __restrict__ words, uint32_t nwords, bool work, M& c0,
                                                M& c1, M& c2, M& c3, M e0, M e1, M e2, M e3, M p0, M p1, M p2, M p3,
@@ -304,7 +306,8 @@ __device__ __forceinline__ void stripe_symbols(LdsLock<M>& L, int lane, LaneMq& 
     if (__ballot(work)) do {
         if (work) {
             // codeword bytes for the lanes whose register runs low
-            if (!m.raw && m.lsb >= 32) lane_fill(m, ring_col, words, nwords);
+            const bool raw = RAW && m.raw;
+            if (!raw && m.lsb >= 32) lane_fill(m, ring_col, words, nwords);
             bool rl = false;
             if (pend == PEND_NONE) {
                 if (cur == 0) {
@@ -339,7 +342,7 @@ __device__ __forceinline__ void stripe_symbols(LdsLock<M>& L, int lane, LaneMq& 
                 sxor = e >> 4;
             }
             uint32_t st = L.ctx[lab][lane];
-            const uint32_t bit = m.raw ? lane_raw(m, ring_col, words, nwords) : lane_decode(m, st, L.trans);
+            const uint32_t bit = raw ? lane_raw(m, ring_col, words, nwords) : lane_decode(m, st, L.trans);
             L.ctx[lab][lane] = st;
             uint32_t* const at = out + (uint32_t)r * pitch + x;
             if (TYPE == 1) {
@@ -353,7 +356,7 @@ __device__ __forceinline__ void stripe_symbols(LdsLock<M>& L, int lane, LaneMq& 
                 const bool sig = zc && bit != 0u;
                 const bool sgn = pend == PEND_SC;
                 // a raw (BYPASS) segment carries the sign bit itself (T.800 D.6)
-                const uint32_t sg = bit ^ (m.raw ? 0u : sxor);
+                const uint32_t sg = bit ^ (raw ? 0u : sxor);
                 lr(RS + r + 1) = sig ? (s1 | bx) : s1;
                 lr(RX + r + 1) = (sgn && sg) ? (x1 | bx) : x1;
                 if (TYPE == 2) sany |= sig ? bx : (M)0;
@@ -532,18 +535,25 @@ __device__ void decode_lockstep(LdsLock<M>& L, const tc_cblk_desc& d, bool activ
             }
 
             const bool work = on && ((c0 | c1 | c2 | c3) != 0);
-            if (type == 0)
-                stripe_symbols<M, 0>(L, lane, m, ring_col, words, nwords, work, c0, c1, c2, c3, e0, e1, e2, e3, p0,
-                                     p1, p2, p3, o0, o1, o2, o3, sany, cls, out + (uint32_t)y0 * pitch, pitch,
-                                     pbit);
-            else if (type == 1)
-                stripe_symbols<M, 1>(L, lane, m, ring_col, words, nwords, work, c0, c1, c2, c3, e0, e1, e2, e3, p0,
-                                     p1, p2, p3, o0, o1, o2, o3, sany, cls, out + (uint32_t)y0 * pitch, pitch,
-                                     pbit);
-            else
-                stripe_symbols<M, 2>(L, lane, m, ring_col, words, nwords, work, c0, c1, c2, c3, e0, e1, e2, e3, p0,
-                                     p1, p2, p3, o0, o1, o2, o3, sany, cls, out + (uint32_t)y0 * pitch, pitch,
-                                     pbit);
+            // raw segments occur in significance and refinement passes only
+            const bool anyraw = type != 2 && __ballot(work && m.raw) != 0;
+#define TC_STRIPE(T, R)                                                                                              \
+    stripe_symbols<M, T, R>(L, lane, m, ring_col, words, nwords, work, c0, c1, c2, c3, e0, e1, e2, e3, p0, p1, p2, p3, \
+                            o0, o1, o2, o3, sany, cls, out + (uint32_t)y0 * pitch, pitch, pbit)
+            if (type == 0) {
+                if (anyraw)
+                    TC_STRIPE(0, true);
+                else
+                    TC_STRIPE(0, false);
+            } else if (type == 1) {
+                if (anyraw)
+                    TC_STRIPE(1, true);
+                else
+                    TC_STRIPE(1, false);
+            } else {
+                TC_STRIPE(2, false);
+            }
+#undef TC_STRIPE
 
             if (on) {
                 if (type != 1) {
