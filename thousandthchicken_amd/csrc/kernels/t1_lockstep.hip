@@ -596,14 +596,17 @@ __device__ void decode_lockstep(LdsLock<M>& L, const tc_cblk_desc& d, bool activ
 // Finishes block j of the wave: merges its refinement logs (log entry
 // [j'][row][j] holds plane numbps-2-j') into the words its significance
 // stores left, then rebuilds and dequantizes each coefficient as
-// convert_block does (midpoint of the last coded plane).  The 64 lanes cover
-// 64/span rows per step, span = w rounded up to a power of two; each lane
-// takes four rows at a time and loads eight logs of each together, so about
-// 32 loads are in flight per lane.
+// convert_block does (midpoint of the last coded plane).  The block's rows go
+// in groups whose logs fit the wave's LDS stage area (cap entries): the
+// group's log rows are loaded into LDS with one load per lane per entry
+// batch, then the 64 lanes cover 64/span rows per step (span = w rounded up
+// to a power of two), each lane with RU rows in flight, reading the log bits
+// from LDS.  (Reading the logs from global memory per row group had cost one
+// memory round trip per four rows: 0.4 ms of a 16-block wave.)
 template <typename M, int MAXH>
 __device__ void convert_lockstep_block(int lane, int j, const tc_cblk_desc& d, uint32_t* __restrict__ coef,
-                                       const M* __restrict__ lg, int bpw) {
-    constexpr int RU = 4, TU = 8;
+                                       const M* __restrict__ lg, int bpw, M* stage, int cap) {
+    constexpr int RU = 16;
     const int numbps = d.numbps, np = d.npasses;
     if (numbps <= 0 || np <= 0) return;
     const int kl = np - 1;
@@ -618,48 +621,56 @@ __device__ void convert_lockstep_block(int lane, int j, const tc_cblk_desc& d, u
     while (span < w) span <<= 1;
     const int rows_per = 64 / span;
     const int lx = lane % span, ly = lane / span;
-    if (lx >= w) return;
-    for (int y0 = ly; y0 < h; y0 += RU * rows_per) {
-        uint32_t v[RU], mag[RU];
-        int yy[RU];
-#pragma unroll
-        for (int q = 0; q < RU; q++) {
-            yy[q] = y0 + q * rows_per;
-            const int yc = yy[q] < h ? yy[q] : y0;  // rows past the block re-read row y0 (in bounds, unused)
-            v[q] = out[(uint32_t)yc * d.out_pitch + lx];
-            mag[q] = v[q] & 0x7FFFFFFFu;
+    // rows per group: a multiple of rows_per whose logs fit the stage area
+    int G = h;
+    if (nlog > 0 && nlog * G > cap) G = cap / nlog;
+    if (G > rows_per) G -= G % rows_per;
+    if (G < 1) G = 1;
+    for (int g0 = 0; g0 < h; g0 += G) {
+        const int gn = min(G, h - g0);
+        __syncthreads();  // the previous group's reads are done
+        for (int i = lane; i < nlog * gn; i += 64) {
+            const int t = i / gn, y = g0 + i % gn;
+            stage[i] = lg[((size_t)t * MAXH + y) * bpw + j];
         }
-        for (int t0 = 0; t0 < nlog; t0 += TU) {
-            M r[RU][TU];
+        __syncthreads();
+        if (lx >= w) continue;
+        for (int y0 = g0 + ly; y0 < g0 + gn; y0 += RU * rows_per) {
+            uint32_t v[RU], mag[RU];
+            int yy[RU];
 #pragma unroll
             for (int q = 0; q < RU; q++) {
-                const int yc = yy[q] < h ? yy[q] : y0;
+                yy[q] = y0 + q * rows_per;
+                const int yc = yy[q] < g0 + gn ? yy[q] : y0;  // rows past the group re-read row y0 (in bounds, unused)
+                v[q] = out[(uint32_t)yc * d.out_pitch + lx];
+            }
+            int yo[RU];
 #pragma unroll
-                for (int u = 0; u < TU; u++) {
-                    const int t = t0 + u < nlog ? t0 + u : nlog - 1;  // clamped: duplicate loads are harmless
-                    r[q][u] = lg[((size_t)t * MAXH + yc) * bpw + j];
+            for (int q = 0; q < RU; q++) {
+                yo[q] = (yy[q] < g0 + gn ? yy[q] : y0) - g0;
+                mag[q] = v[q] & 0x7FFFFFFFu;
+            }
+            for (int t = 0; t < nlog; t++) {
+                const M* const st = stage + t * gn;
+                const int sh = numbps - 2 - t;
+#pragma unroll
+                for (int q = 0; q < RU; q++) mag[q] |= (uint32_t)((st[yo[q]] >> lx) & 1u) << sh;
+            }
+#pragma unroll
+            for (int q = 0; q < RU; q++) {
+                if (yy[q] >= g0 + gn || v[q] == 0) continue;
+                const int qp = (tlast == 0 && mag[q] != sigonly) ? plast + 1 : plast;
+                const bool neg = (v[q] >> 31) != 0;
+                uint32_t res;
+                if ((d.mode & TC_MODE_IRREVERSIBLE) == 0) {
+                    const int32_t iv = (int32_t)(mag[q] + (qp > 0 ? (1u << (qp - 1)) : 0u));
+                    res = (uint32_t)(neg ? -iv : iv);
+                } else {
+                    const float f = ((float)mag[q] + ldexpf(0.5f, qp)) * d.delta;
+                    res = __float_as_uint(neg ? -f : f);
                 }
+                out[(uint32_t)yy[q] * d.out_pitch + lx] = res;
             }
-#pragma unroll
-            for (int q = 0; q < RU; q++)
-#pragma unroll
-                for (int u = 0; u < TU; u++)
-                    if (t0 + u < nlog) mag[q] |= (uint32_t)((r[q][u] >> lx) & 1u) << (numbps - 2 - t0 - u);
-        }
-#pragma unroll
-        for (int q = 0; q < RU; q++) {
-            if (yy[q] >= h || v[q] == 0) continue;
-            const int qp = (tlast == 0 && mag[q] != sigonly) ? plast + 1 : plast;
-            const bool neg = (v[q] >> 31) != 0;
-            uint32_t res;
-            if ((d.mode & TC_MODE_IRREVERSIBLE) == 0) {
-                const int32_t iv = (int32_t)(mag[q] + (qp > 0 ? (1u << (qp - 1)) : 0u));
-                res = (uint32_t)(neg ? -iv : iv);
-            } else {
-                const float f = ((float)mag[q] + ldexpf(0.5f, qp)) * d.delta;
-                res = __float_as_uint(neg ? -f : f);
-            }
-            out[(uint32_t)yy[q] * d.out_pitch + lx] = res;
         }
     }
 }
@@ -714,7 +725,7 @@ __global__ __launch_bounds__(64) void t1_lockstep_kernel(const tc_cblk_desc* __r
     __syncthreads();
     for (int j = 0; j < bpw && base + j < n; j++) {
         const tc_cblk_desc dj = descs[order ? order[base + j] : (uint32_t)(base + j)];
-        convert_lockstep_block<M, MAXH>(lane, j, dj, coef, lg, bpw);
+        convert_lockstep_block<M, MAXH>(lane, j, dj, coef, lg, bpw, &L.rows[0][0], NROWS * kLaneCount);
     }
 }
 
