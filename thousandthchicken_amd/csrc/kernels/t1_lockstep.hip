@@ -470,8 +470,34 @@ __device__ void decode_lockstep(LdsLock<M>& L, const tc_cblk_desc& d, bool activ
                 lane_init(m, ring_col, words, nwords);
             }
         }
+        // With 32-column masks the state rows of a stripe are loaded while
+        // the stripe before it decodes: rows y0+4.. belong to the next
+        // stripe, which this pass has not reached, so they hold the previous
+        // pass's state.  (The 32x32 class's launch 21.1 -> 20.2 ms on the
+        // mixed batch; with 64-column masks the held rows take the kernel
+        // from 114 to 190 VGPRs and the 64x64 launch gained nothing, so it
+        // loads each stripe's rows when the stripe starts.)
+        constexpr bool kAhead = sizeof(M) == 4;
+        M pS[5], pX[5], pP[4], pR[4];
+        auto fetch = [&](int ys) {
+#pragma unroll
+            for (int r = 0; r < 5; r++) {
+                // stripe-causal contexts take the row below the stripe as insignificant
+                const bool ok = pass_on && k > 0 && ys + r < h && !(r == 4 && (style & TC_MODE_CAUSAL));
+                pS[r] = ok ? gat(GS, ys + r) : (M)0;
+                pX[r] = (ok && type != 1) ? gat(GX, ys + r) : (M)0;
+            }
+#pragma unroll
+            for (int r = 0; r < 4; r++) {
+                const bool ok = pass_on && k > 0 && ys + r < h;
+                pP[r] = (ok && type != 0) ? gat(GP, ys + r) : (M)0;
+                pR[r] = (ok && type == 1) ? gat(GR, ys + r) : (M)0;
+            }
+        };
+        if (kAhead) fetch(0);
         for (int y0 = 0; y0 < hmax; y0 += 4) {
             const bool on = pass_on && y0 < h;
+            if (!kAhead) fetch(y0);
             M c0 = 0, c1 = 0, c2 = 0, c3 = 0;  // candidate columns per stripe row
             M e0 = 0, e1 = 0, e2 = 0, e3 = 0;  // sigprop: rows that may still join; magref: label-16 candidates
             M p0 = 0, p1 = 0, p2 = 0, p3 = 0;  // magref: label-15 candidates
@@ -483,16 +509,13 @@ __device__ void decode_lockstep(LdsLock<M>& L, const tc_cblk_desc& d, bool activ
                 X[0] = (y0 > 0 && type != 1) ? carry_x : (M)0;
 #pragma unroll
                 for (int r = 0; r < 5; r++) {
-                    // stripe-causal contexts take the row below the stripe as insignificant
-                    const bool ok = k > 0 && y0 + r < h && !(r == 4 && (style & TC_MODE_CAUSAL));
-                    S[r + 1] = ok ? gat(GS, y0 + r) : (M)0;
-                    X[r + 1] = (ok && type != 1) ? gat(GX, y0 + r) : (M)0;
+                    S[r + 1] = pS[r];
+                    X[r + 1] = pX[r];
                 }
 #pragma unroll
                 for (int r = 0; r < 4; r++) {
-                    const bool ok = k > 0 && y0 + r < h;
-                    P[r] = (ok && type != 0) ? gat(GP, y0 + r) : (M)0;
-                    R[r] = (ok && type == 1) ? gat(GR, y0 + r) : (M)0;
+                    P[r] = pP[r];
+                    R[r] = pR[r];
                 }
                 if (type != 1) {
 #pragma unroll
@@ -534,6 +557,7 @@ __device__ void decode_lockstep(LdsLock<M>& L, const tc_cblk_desc& d, bool activ
                 }
             }
 
+            if (kAhead && y0 + 4 < hmax) fetch(y0 + 4);
             const bool work = on && ((c0 | c1 | c2 | c3) != 0);
             // raw segments occur in significance and refinement passes only
             const bool anyraw = type != 2 && __ballot(work && m.raw) != 0;
