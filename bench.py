@@ -7,8 +7,14 @@ image seeds 1000-1255, coded by the package's encoder -- decoded with one
 launch per stage over the whole batch and replayed from a captured graph,
 codestreams resident in HBM.  A step decodes the whole batch: fill, tier-1
 (one launch per code-block size class, the two side by side), the inverse-DWT
-levels and the fused final level.  Outputs are checked against per-image
-decode first (lossless identical, lossy within 1 LSB).
+levels and the fused final level.  Steady state keeps several batches in
+flight: copies of the prepared batch on their own pipelines and streams,
+launched round robin, so that one batch's tier-1 tail (its longest lockstep
+waves, which leave most compute units idle) overlaps the next batch's work;
+the number in flight and the tier-1 packing are picked by measurement, and
+one batch alone is reported beside it.  Outputs are checked against
+per-image decode first, and every in-flight copy's outputs again after
+timing (lossless identical, lossy within 1 LSB).
 
 Beside it the line carries (unless --quick):
   per_image  the 512x512 RGB lossless image, the 2048x2048 RGB lossy image and
@@ -138,6 +144,89 @@ def timed_runs(dec, steps, graph=False):
         dec.launch_graph() if graph else dec.run()
     dec.sync()
     return (time.perf_counter() - t0) / steps
+
+
+def in_flight_leg(tca, streams, outs, lossless, px, args, tuned):
+    """The batch decoded back to back with D batches in flight: D resident
+    copies of the prepared batch, each its own pipeline (buffers, streams,
+    captured graph), launched round robin.  A step is still one whole decode of
+    the batch; the copies fill the compute units that one batch's longest
+    tier-1 waves leave idle.  With more than one in flight, the tier-1 packing
+    is picked for throughput (fewer wave-steps) rather than for one batch's
+    latency.  Every copy's outputs are checked against per-image decode."""
+    depths = sorted({int(x) for x in args.in_flight.split(",") if x and int(x) >= 1})
+    cands = [(tuned["bpw_small"], tuned["bpw_large"])] if tuned else [(args.bpw_small, args.bpw_large)]
+    if not args.no_autotune:
+        cands += [(-64, -16), (-64, -32), (-64, -64)]
+    cands = list(dict.fromkeys(cands))
+    dmax = max(depths)
+    tried, best = [], None
+    for bs, bl in cands:
+        decs = []
+        for _ in range(dmax):
+            d = tca.Decoder()
+            d.set_blocks_per_wave(True, bs)
+            d.set_blocks_per_wave(False, bl)
+            if tuned and tuned.get("dwt_shape") is not None:
+                t = d.tuning()
+                t.dwt_shape = tuned["dwt_shape"]
+                d.set_tuning(t)
+            d.prepare(streams)
+            d.capture_graph()
+            decs.append(d)
+        for depth in depths:
+            for k in range(2 * depth):
+                decs[k % depth].launch_graph()
+            for d in decs:
+                d.sync()
+            t0 = time.perf_counter()
+            for k in range(args.steps):
+                decs[k % depth].launch_graph()
+            for d in decs[:depth]:
+                d.sync()
+            ms = (time.perf_counter() - t0) / args.steps * 1e3
+            r = {"bpw_small": bs, "bpw_large": bl, "in_flight": depth, "ms_per_batch": round(ms, 3),
+                 "mpixel_s": round(px / ms / 1e3, 1)}
+            tried.append(r)
+            log(f"in flight: {r}")
+            if best is None or ms < best["ms_per_batch"]:
+                best = dict(r)
+        for d in decs:
+            d.close()
+    # the chosen configuration again, every copy's outputs checked
+    decs = []
+    for _ in range(best["in_flight"]):
+        d = tca.Decoder()
+        d.set_blocks_per_wave(True, best["bpw_small"])
+        d.set_blocks_per_wave(False, best["bpw_large"])
+        if tuned and tuned.get("dwt_shape") is not None:
+            t = d.tuning()
+            t.dwt_shape = tuned["dwt_shape"]
+            d.set_tuning(t)
+        d.prepare(streams)
+        d.capture_graph()
+        decs.append(d)
+    depth = len(decs)
+    for k in range(args.warmup * depth):
+        decs[k % depth].launch_graph()
+    for d in decs:
+        d.sync()
+    t0 = time.perf_counter()
+    for k in range(args.steps):
+        decs[k % depth].launch_graph()
+    for d in decs:
+        d.sync()
+    sec = (time.perf_counter() - t0) / args.steps
+    for j, d in enumerate(decs):
+        for i in range(len(streams)):
+            diff = int(np.abs(d.fetch(i).astype(np.int32) - outs[i].astype(np.int32)).max())
+            if diff > (0 if lossless[i] else 1):
+                raise SystemExit(f"in-flight copy {j}, image {i}: differs from per-image decode by {diff}")
+    for d in decs:
+        d.close()
+    best["ms_per_batch"] = round(sec * 1e3, 3)
+    best["mpixel_s"] = round(px / sec / 1e6, 1)
+    return best, tried, sec
 
 
 def t1_kernel_name(bpw, small, args):
@@ -412,7 +501,7 @@ def bench_served(tca, small, large, loads, args, serial_requests=300):
     dec.close()
     out = {"large_fraction": 0.1, "serial_per_image": ser, "loads": []}
     srv = tca.Server(slots=args.slots, max_batch=args.max_batch, bpw_small=args.bpw_small, bpw_large=args.bpw_large,
-                     max_batch_bytes=args.batch_mib << 20)
+                     max_batch_bytes=args.batch_mib << 20, overlap_kernels=args.overlap)
     # warm: the slots' resident device buffers and pinned buffers reach their working size
     serve.run(srv, seq[:1000], max(loads), seed=1)
     best = None
@@ -425,12 +514,14 @@ def bench_served(tca, small, large, loads, args, serial_requests=300):
             if d > (0 if tca.probe(seq[i]).reversible else 1):
                 raise SystemExit(f"served request {i} differs from its per-image decode by {d}")
         r["speedup_vs_serial"] = round(r["mpixel_s"] / ser["mpixel_s"], 2)
-        # keeping up: the whole run's rate is within 5 % of the offer, or the
-        # latency of the last fifth of the requests has not grown past 1.5x
-        # that of the second fifth (the run's drain tail alone costs a few %
-        # of a 5 s run)
+        # keeping up: the whole run's rate is within 5 % of the offer, or
+        # within 20 % of it with the latency of the last fifth of the
+        # requests not grown past 1.5x that of the second fifth (a short run's
+        # drain tail, the last batches' latency after the last arrival, costs
+        # up to ~15 % of the achieved rate at 2000 req/s)
         r["sustained"] = bool(r["achieved_rps"] >= 0.95 * rate or
-                              r["p50_ms_last_fifth"] <= 1.5 * r["p50_ms_second_fifth"])
+                              (r["achieved_rps"] >= 0.8 * rate and
+                               r["p50_ms_last_fifth"] <= 1.5 * r["p50_ms_second_fifth"]))
         out["loads"].append(r)
         log(f"served {rate} req/s: {r}")
         if r["sustained"]:
@@ -466,12 +557,22 @@ def main():
     ap.add_argument("--slots", type=int, default=2, help="server slots")
     ap.add_argument("--max-batch", type=int, default=256, help="server: most requests per batch")
     ap.add_argument("--batch-mib", type=int, default=1024, help="server: most codestream MiB per batch")
+    ap.add_argument("--overlap", action="store_true", help="server: kernels of different slots may run together")
     ap.add_argument("--bpw-large", type=int, default=0, help="blocks per wave, blocks up to 64x64 (0 auto, -64/-32/-16 lockstep)")
     ap.add_argument("--bpw-small", type=int, default=0, help="blocks per wave, blocks up to 32x32 (0 auto, -64/-32/-16 lockstep)")
     ap.add_argument("--t1-kernel", type=int, default=0, help="one-wave-per-block kernel: 0 stripe-column, 1 row-mask")
     ap.add_argument("--dwt-shape", type=int, default=-1, help="inverse-DWT tile shape (tc_idwt_tile_shape; -1 default)")
     ap.add_argument("--no-autotune", action="store_true", help="keep the default packing and DWT tile shape")
+    ap.add_argument("--in-flight", default="1,2,4,8",
+                    help="batches in flight tried for the headline (each its own pipeline and stream)")
+    ap.add_argument("--hw-queues", type=int, default=16,
+                    help="GPU_MAX_HW_QUEUES for this process (set before HIP starts; 0 leaves it)")
     args = ap.parse_args()
+    if args.hw_queues > 0:
+        # several pipelines in flight use several streams each; with HIP's
+        # default of 4 hardware queues, streams share queues and their work
+        # serialises behind each other
+        os.environ["GPU_MAX_HW_QUEUES"] = str(min(args.hw_queues, 32))
     if args.gpus != 1:
         print(f"bench.py measures one GPU: this build has no multi-GPU path (asked for {args.gpus})",
               file=sys.stderr)
@@ -531,6 +632,12 @@ def main():
     st, launch_ms = stage_times(dec)
     s = dec.stats()
     dec.close()
+    lossless = [lay[i]["lossless"] for i in range(len(streams))]
+    flight, flight_tried, flight_s = in_flight_leg(tca, streams, outs, lossless, px, args, tuning)
+    if flight_s > graph:  # one in flight (the latency-tuned graph) stays the faster form
+        flight = {"bpw_small": s.bpw_small, "bpw_large": s.bpw_large, "in_flight": 1,
+                  "ms_per_batch": round(graph * 1e3, 3), "mpixel_s": round(px / graph / 1e6, 1)}
+        flight_s = graph
     launches = t1_launches(s, launch_ms, args)
     # the dominant kernel: the longest tier-1 launch
     dom = max(launches, key=lambda x: x["ms"])
@@ -541,12 +648,12 @@ def main():
     tr = traffic_record(kname, "cfg4_mixed_256")
     line = {
         "metric": "decoded Mpixel/s on 1 MI355X",
-        "value": round(px / graph / 1e6, 1),
+        "value": round(px / flight_s / 1e6, 1),
         "unit": "Mpixel/s",
         "n_gpus": 1,
         "steps": args.steps,
         "warmup": args.warmup,
-        "ms_per_step": round(graph * 1e3, 4),
+        "ms_per_step": round(flight_s * 1e3, 4),
         "higher_is_better": True,
         "scaling": "weak",
         "vs_baseline": None,
@@ -556,7 +663,15 @@ def main():
                    "codestream_bytes": sum(map(len, streams)), "cblks": s.cblks,
                    "mix": "sides log-uniform 256-2048, 1 or 3 components, 8-bit, half lossless 5/3, "
                           "half lossy 9/7 0.5-2 bpp, 32x32 or 64x64 blocks",
-                   "form": "batched, one launch per stage, graph replay", "inputs": "codestreams resident in HBM"},
+                   "form": "batched, one launch per stage per batch, graph replay, "
+                           f"{flight['in_flight']} batch(es) in flight on their own pipelines and streams",
+                   "in_flight": flight["in_flight"], "inputs": "codestreams resident in HBM"},
+        "in_flight": {"chosen": flight, "tried": flight_tried,
+                      "hw_queues": os.environ.get("GPU_MAX_HW_QUEUES"),
+                      "note": "a step is one whole decode of the batch; copies of the prepared batch on their own "
+                              "pipelines are launched round robin, so one batch's tier-1 tail overlaps the next "
+                              "batch's work; every copy's outputs are checked against per-image decode. "
+                              "stages_ms, tier1_launches and roofline below are one batch alone"},
         "stages_ms": {"fill": round(st["fill_ms"], 3), "tier1": round(st["t1_ms"], 3),
                       "idwt_levels": round(st["dwt_ms"], 3), "final_level": round(st["final_ms"], 3),
                       "total_eager": round(st["total_ms"], 3)},
@@ -574,10 +689,12 @@ def main():
         "heavy_threshold": {"small": s.heavy_small, "large": s.heavy_large},
         "autotuned": tuning,
         "batched": {"cfg4_mixed_256": {
+            "mpixel_s_in_flight": round(px / flight_s / 1e6, 1),
             "mpixel_s_graph": round(px / graph / 1e6, 1), "mpixel_s_eager": round(px / eager / 1e6, 1),
-            "ms_per_batch": round(graph * 1e3, 3), "prepare_ms": round(prep_ms, 1),
+            "ms_per_batch_one_in_flight": round(graph * 1e3, 3), "prepare_ms": round(prep_ms, 1),
             "per_image_serial_mpixel_s": round(px / serial_s / 1e6, 1),
-            "speedup_vs_per_image": round(serial_s / graph, 2),
+            "speedup_vs_per_image": round(serial_s / flight_s, 2),
+            "speedup_one_in_flight_vs_per_image": round(serial_s / graph, 2),
             "outputs": "lossless identical to per-image decode, lossy within 1 LSB"}},
     }
     if not args.quick:

@@ -36,6 +36,9 @@ typedef struct {
                                      runs alone), default 1 GiB: batches that large
                                      hold several 8K 16-bit images, whose longest
                                      block chains then bound one batch, not each */
+    int32_t overlap_kernels;      /* 1: the kernels of different slots' batches may run at the
+                                     same time; 0 (default): one batch's kernels after another's,
+                                     with uploads, result copies and host work overlapped */
 } tc_server_config;
 
 typedef struct {

@@ -291,7 +291,7 @@ class Tuning(ctypes.Structure):
 class ServerConfig(ctypes.Structure):
     _fields_ = [("slots", ctypes.c_int32), ("max_batch", ctypes.c_int32), ("memory_budget", ctypes.c_uint64),
                 ("bpw_small", ctypes.c_int32), ("bpw_large", ctypes.c_int32), ("dwt_shape", ctypes.c_int32),
-                ("max_batch_bytes", ctypes.c_uint64)]
+                ("max_batch_bytes", ctypes.c_uint64), ("overlap_kernels", ctypes.c_int32)]
 
 
 class RequestInfo(ctypes.Structure):
@@ -312,9 +312,10 @@ class Server:
     """Continuous-batching decode server over the C ABI (include/tc_server.h)."""
 
     def __init__(self, device: "Device" = None, slots=2, max_batch=256, memory_budget=32 << 30, bpw_small=0,
-                 bpw_large=0, dwt_shape=0, max_batch_bytes=1 << 30):
+                 bpw_large=0, dwt_shape=0, max_batch_bytes=1 << 30, overlap_kernels=False):
         self.device = device or default_device()
-        cfg = ServerConfig(slots, max_batch, memory_budget, bpw_small, bpw_large, dwt_shape, max_batch_bytes)
+        cfg = ServerConfig(slots, max_batch, memory_budget, bpw_small, bpw_large, dwt_shape, max_batch_bytes,
+                           1 if overlap_kernels else 0)
         h = ctypes.c_void_p()
         check(lib().tc_server_create(self.device.handle, ctypes.byref(cfg), ctypes.byref(h)), "tc_server_create")
         self.handle = h

@@ -255,7 +255,9 @@ struct tc_server_s {
         if (tc_event_record(s.up_done, s.pipe->stream()) == TC_SUCCESS) s.up_pending = true;
         {
             std::lock_guard<std::mutex> g(compute_mu);
-            if (last_compute && tc_stream_wait_event(s.pipe->stream(), last_compute) != TC_SUCCESS) return TC_ERR_DEVICE;
+            if (!cfg.overlap_kernels && last_compute &&
+                tc_stream_wait_event(s.pipe->stream(), last_compute) != TC_SUCCESS)
+                return TC_ERR_DEVICE;
             if (s.pipe->run(false) != Status::Ok) return TC_ERR_DEVICE;
             tc_status e = tc_event_record(s.kdone, s.pipe->stream());
             if (e != TC_SUCCESS) return e;
@@ -433,6 +435,7 @@ tc_status tc_server_create(tc_device dev, const tc_server_config* cfg, tc_server
     s->cfg.bpw_small = cfg ? cfg->bpw_small : 0;
     s->cfg.bpw_large = cfg ? cfg->bpw_large : 0;
     s->cfg.dwt_shape = cfg ? cfg->dwt_shape : 0;
+    s->cfg.overlap_kernels = cfg ? cfg->overlap_kernels : 0;
     std::memset(&s->st, 0, sizeof s->st);
     s->t0 = std::chrono::steady_clock::now();
     s->slots.resize(s->cfg.slots);
