@@ -432,11 +432,16 @@ def profile_mode(tca, args):
             "cfg3_lossless_4k_tiled": lambda: [workloads.cfg3()], "cfg4_mixed_256": lambda: workloads.cfg4(threads=16),
             "cfg5_large_16bit": lambda: [workloads.cfg5_large(5)]}
     streams = make[args.workload]()
-    dec = tuned_decoder(tca, args)
-    dec.prepare(streams)
-    for _ in range(args.steps):
-        dec.run()
-    dec.sync()
+    decs = []
+    for _ in range(max(1, args.profile_in_flight)):
+        d = tuned_decoder(tca, args)
+        d.prepare(streams)
+        decs.append(d)
+    dec = decs[0]
+    for k in range(args.steps * len(decs)):
+        decs[k % len(decs)].run()
+    for d in decs:
+        d.sync()
     s = dec.stats()
     n = 256 << 20
     L = tca.lib()
@@ -553,7 +558,7 @@ def main():
                     help="only the GPU forward transform (encoder front end) leg")
     ap.add_argument("--region-only", action="store_true", help="only the region-decode leg")
     ap.add_argument("--served-only", action="store_true", help="only the serving leg")
-    ap.add_argument("--loads", default="100,250,500,1000,2000,5000", help="offered loads (req/s) of the serving run")
+    ap.add_argument("--loads", default="100,250,500,1000,2000,3000,5000", help="offered loads (req/s) of the serving run")
     ap.add_argument("--slots", type=int, default=2, help="server slots")
     ap.add_argument("--max-batch", type=int, default=256, help="server: most requests per batch")
     ap.add_argument("--batch-mib", type=int, default=1024, help="server: most codestream MiB per batch")
@@ -565,6 +570,8 @@ def main():
     ap.add_argument("--no-autotune", action="store_true", help="keep the default packing and DWT tile shape")
     ap.add_argument("--in-flight", default="1,2,4,8",
                     help="batches in flight tried for the headline (each its own pipeline and stream)")
+    ap.add_argument("--profile-in-flight", type=int, default=1,
+                    help="--profile: batches in flight (each its own pipeline, launched round robin)")
     ap.add_argument("--hw-queues", type=int, default=16,
                     help="GPU_MAX_HW_QUEUES for this process (set before HIP starts; 0 leaves it)")
     args = ap.parse_args()
