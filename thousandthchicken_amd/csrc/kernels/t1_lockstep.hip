@@ -264,6 +264,16 @@ __device__ __forceinline__ uint32_t nibble(M a0, M a1, M a2, M a3, int x) {
            (((uint32_t)(a3 >> x) & 1u) << 3);
 }
 
+// Bits x and x+1 of four row masks as two column nibbles (row 0 in bit 0):
+// n0 for column x, n1 for column x+1 (0 past the mask).
+template <typename M>
+__device__ __forceinline__ void nibble2(M a0, M a1, M a2, M a3, int x, uint32_t& n0, uint32_t& n1) {
+    const uint32_t t0 = (uint32_t)(a0 >> x), t1 = (uint32_t)(a1 >> x), t2 = (uint32_t)(a2 >> x),
+                   t3 = (uint32_t)(a3 >> x);
+    n0 = (t0 & 1u) | ((t1 & 1u) << 1) | ((t2 & 1u) << 2) | ((t3 & 1u) << 3);
+    n1 = ((t0 >> 1) & 1u) | (t1 & 2u) | ((t2 << 1) & 4u) | ((t3 << 2) & 8u);
+}
+
 // Decodes one stripe of one pass for every lane with work, one symbol per
 // lane per step.  TYPE: 0 significance propagation (T.800 D.3.1), 1 magnitude
 // refinement (D.3.3), 2 cleanup with run-length mode (D.3.4).
@@ -290,6 +300,7 @@ __device__ __forceinline__ void stripe_symbols(LdsLock<M>& L, int lane, LaneMq& 
     auto lr = [&](int i) -> M& { return L.rows[i][lane]; };
     M cany = c0 | c1 | c2 | c3;
     uint32_t pend = PEND_NONE, cur = 0, enib = 0, pnib = 0;
+    uint32_t enx = 0;  // TYPE 0: rows of column x+1 that may still join
     uint32_t onib = 0;  // TYPE 1: refinement 1-bits of the current column
     // moves the current column's refinement 1-bits into the row masks
     auto flush = [&](int col) {
@@ -316,7 +327,8 @@ __device__ __forceinline__ void stripe_symbols(LdsLock<M>& L, int lane, LaneMq& 
                     x = (int)__builtin_ctzll((unsigned long long)cany);
                     cany &= cany - 1;
                     cur = nibble<M>(c0, c1, c2, c3, x);
-                    if (TYPE != 2) enib = nibble<M>(e0, e1, e2, e3, x);
+                    if (TYPE == 0) nibble2<M>(e0, e1, e2, e3, x, enib, enx);
+                    if (TYPE == 1) enib = nibble<M>(e0, e1, e2, e3, x);
                     if (TYPE == 1) pnib = nibble<M>(p0, p1, p2, p3, x);
                     if (TYPE == 2) rl = cur == 15u && col3<M>(sany, x) == 0;
                 }
@@ -364,16 +376,17 @@ __device__ __forceinline__ void stripe_symbols(LdsLock<M>& L, int lane, LaneMq& 
                 const uint32_t pend0 = pend;
                 pend = sig ? PEND_SC : (sgn ? PEND_NONE : pend);
                 if (TYPE == 0 && sig) {
-                    // neighbours later in scan order may now be coded in this pass
-                    // (the row below joins this column's nibble too; c0..c3 record all)
-                    cur |= enib & (2u << r) & 15u;
-                    const M nx = bx << 1;
-                    const M a0 = ((r <= 1) ? nx : (M)0) & e0;
-                    const M a1 = (((r <= 2) ? nx : (M)0) | ((r == 0) ? bx : (M)0)) & e1;
-                    const M a2 = (((r >= 1) ? nx : (M)0) | ((r == 1) ? bx : (M)0)) & e2;
-                    const M a3 = (((r >= 2) ? nx : (M)0) | ((r == 2) ? bx : (M)0)) & e3;
-                    c0 |= a0, c1 |= a1, c2 |= a2, c3 |= a3;
-                    cany |= (a0 | a1 | a2 | a3) & nx;
+                    // neighbours later in scan order may now be coded in this
+                    // pass: the row below in this column (it joins the column's
+                    // nibble) and rows r-1..r+1 of column x+1; c0..c3 record all
+                    const uint32_t add0 = enib & (2u << r) & 15u;
+                    const uint32_t add1 = enx & (((7u << r) >> 1) & 15u);
+                    cur |= add0;
+                    c0 |= (M)((add0 & 1u) | ((add1 & 1u) << 1)) << x;
+                    c1 |= (M)(((add0 >> 1) & 1u) | (add1 & 2u)) << x;
+                    c2 |= (M)(((add0 >> 2) & 1u) | ((add1 >> 1) & 2u)) << x;
+                    c3 |= (M)(((add0 >> 3) & 1u) | ((add1 >> 2) & 2u)) << x;
+                    cany |= add1 ? (bx << 1) : (M)0;
                 }
                 if (TYPE == 2 && !zc && !sgn) {
                     if (rl) {
