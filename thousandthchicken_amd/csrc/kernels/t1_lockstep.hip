@@ -25,7 +25,7 @@
 // [wave][mask][row][lane] (each row load or store of the 64 lanes is one
 // contiguous 512-byte access) and only the rows of the current stripe and
 // its two halo rows are staged into LDS ([row][lane], conflict-free) while
-// the stripe is decoded.  Context states (19 per lane) and the MQ / zero-coding
+// the stripe is decoded.  Context states (19 bytes per lane) and the MQ / zero-coding
 // / sign-coding tables also live in LDS.
 //
 // Output.  Scattered read-modify-writes of single bits would saturate the
@@ -73,10 +73,23 @@ constexpr int RS = 0, RX = 6, NROWS = 12;
 // Masks of the global scratch state.
 constexpr int GS = 0, GX = 1, GP = 2, GR = 3;
 
+// A context state is one byte, I | MPS << 6 (T.800 C.2); tq[I] holds
+// Qe(I) | NMPS(I) << 16 | NLPS(I) << 22 | SWITCH(I) << 28.  Each lane's 19
+// states sit in a 20-byte record (lane-major: the 64 lanes' accesses fall in
+// 64 different banks).  Bytes instead of packed 32-bit states take 3.6 KB off
+// a wave's LDS, and the waves resident per compute unit are bound by LDS.
+constexpr int kCtxStride = 20;
+
+__device__ __forceinline__ uint32_t tq_entry(int i) {
+    return (uint32_t)kQe[i] | ((uint32_t)kNmps[i] << 16) | ((uint32_t)kNlps[i] << 22) |
+           ((i == 0 || i == 6 || i == 14) ? (1u << 28) : 0u);
+}
+__device__ __forceinline__ uint8_t ctx_init(int c) { return (uint8_t)(c == 0 ? 4 : (c == 17 ? 3 : (c == 18 ? 46 : 0))); }
+
 template <typename M>
 struct LdsLock {
-    uint64_t trans[48];
-    uint32_t ctx[kLockLabels][kLaneCount];
+    uint32_t tq[48];
+    uint8_t ctx[kLaneCount][kCtxStride];
     M rows[NROWS][kLaneCount];
     uint32_t ring[8][kLaneCount];  // codeword dwords, slot = dword index mod 8
     uint8_t zc[3][512];  // label by 3x3 significance window (centre ignored)
@@ -218,25 +231,23 @@ __device__ __forceinline__ bool lock_raw_pass(uint32_t style, int k) {
     return (style & TC_MODE_BYPASS) && k >= 10 && (k - 1) % 3 != 2;
 }
 
-// DECODE, T.800 Figures C.15-C.18, on a packed context state
-// (Qe | I << 16 | MPS << 22), with the codeword taken ahead by lane_fill
-// (lsb <= 48 on entry): RENORMD is one shift.
-__device__ __forceinline__ uint32_t lane_decode(LaneMq& m, uint32_t& st, const uint64_t* trans) {
+// DECODE, T.800 Figures C.15-C.18, on a context state byte (I | MPS << 6),
+// with the codeword taken ahead by lane_fill (lsb <= 48 on entry): RENORMD is
+// one shift.
+__device__ __forceinline__ uint32_t lane_decode(LaneMq& m, uint32_t& st, const uint32_t* tq) {
     // Nearly every step has some lane that renormalises, so the exchange is
     // computed for all lanes with selects rather than behind a branch.
-    const uint32_t qe = st & 0xFFFFu;
-    const uint32_t mps = (st >> 22) & 1u;
-    const uint64_t e = trans[(st >> 16) & 63];
+    const uint32_t t = tq[st & 63u];
+    const uint32_t qe = t & 0xFFFFu;
+    const uint32_t mps = (st >> 6) & 1u;
     uint32_t A = m.A - qe;
     const uint32_t chi = (uint32_t)(m.C >> 32);
     const bool lps = (chi >> 16) < qe;
     m.C = ((uint64_t)(lps ? chi : chi - (qe << 16)) << 32) | (uint32_t)m.C;
     const bool renorm = lps || A < 0x8000u;
     const bool take_mps = lps == (A < qe);  // LPS exchange: A < Qe gives MPS; MPS exchange: the reverse
-    const uint32_t e2 = (uint32_t)(e >> 32);
-    const uint32_t nm = mps ^ ((e2 >> 22) & 1u);
-    const uint32_t smps = ((uint32_t)e & 0x3FFFFFu) | (mps << 22);
-    const uint32_t slps = (e2 & 0x3FFFFFu) | (nm << 22);
+    const uint32_t smps = ((t >> 16) & 63u) | (mps << 6);
+    const uint32_t slps = ((t >> 22) & 63u) | ((mps ^ (t >> 28)) << 6);
     const uint32_t pick = take_mps ? ~0u : 0u;  // bitwise select keeps this off the branch path
     const uint32_t st2 = (smps & pick) | (slps & ~pick);
     const uint32_t d = (renorm && !take_mps) ? (mps ^ 1u) : mps;
@@ -353,9 +364,9 @@ __device__ __forceinline__ void stripe_symbols(LdsLock<M>& L, int lane, LaneMq& 
                 lab = pend == PEND_SC ? (e & 15u) : (pend == PEND_NONE ? (rl ? 17u : lz) : 18u);  // 17 run-length, 18 uniform
                 sxor = e >> 4;
             }
-            uint32_t st = L.ctx[lab][lane];
-            const uint32_t bit = raw ? lane_raw(m, ring_col, words, nwords) : lane_decode(m, st, L.trans);
-            L.ctx[lab][lane] = st;
+            uint32_t st = L.ctx[lane][lab];
+            const uint32_t bit = raw ? lane_raw(m, ring_col, words, nwords) : lane_decode(m, st, L.tq);
+            L.ctx[lane][lab] = (uint8_t)st;
             uint32_t* const at = out + (uint32_t)r * pitch + x;
             if (TYPE == 1) {
                 onib |= bit << r;
@@ -428,8 +439,7 @@ __device__ void decode_lockstep(LdsLock<M>& L, const tc_cblk_desc& d, bool activ
     uint32_t* const out = coef + (active ? d.out_off : 0);
     const uint32_t pitch = active ? d.out_pitch : 0;
 
-    for (int c = 0; c < kLockLabels; c++)
-        L.ctx[c][lane] = pack_state(c == 0 ? 4 : (c == 17 ? 3 : (c == 18 ? 46 : 0)), 0);
+    for (int c = 0; c < kLockLabels; c++) L.ctx[lane][c] = ctx_init(c);
     LaneMq m;
     uint32_t* const ring_col = &L.ring[0][lane];
     m.ring = ring_col;
@@ -619,14 +629,13 @@ __device__ void decode_lockstep(LdsLock<M>& L, const tc_cblk_desc& d, bool activ
             // cleanup pass (decoded to keep the MQ state in step, not checked)
             for (int s4 = 0; s4 < 4; s4++) {
                 if (m.lsb >= 32) lane_fill(m, ring_col, words, nwords);
-                uint32_t st = L.ctx[18][lane];
-                (void)lane_decode(m, st, L.trans);
-                L.ctx[18][lane] = st;
+                uint32_t st = L.ctx[lane][18];
+                (void)lane_decode(m, st, L.tq);
+                L.ctx[lane][18] = (uint8_t)st;
             }
         }
         if (pass_on && (style & TC_MODE_RESET))
-            for (int c = 0; c < kLockLabels; c++)
-                L.ctx[c][lane] = pack_state(c == 0 ? 4 : (c == 17 ? 3 : (c == 18 ? 46 : 0)), 0);
+            for (int c = 0; c < kLockLabels; c++) L.ctx[lane][c] = ctx_init(c);
     }
 }
 
@@ -720,7 +729,7 @@ __global__ __launch_bounds__(64) void t1_lockstep_kernel(const tc_cblk_desc* __r
                                                          int nlog_max, int bpw) {
     __shared__ LdsLock<M> L;
     const int lane = threadIdx.x;
-    for (int i = lane; i < 47; i += 64) L.trans[i] = trans_entry(i);
+    for (int i = lane; i < 47; i += 64) L.tq[i] = tq_entry(i);
     for (int i = lane; i < 3 * 512; i += 64) {
         const uint32_t w9 = (uint32_t)(i & 511);
         L.zc[i >> 9][w9] = (uint8_t)zc_label(i >> 9, (w9 & 0xFu) | ((w9 >> 1) & 0xF0u));
