@@ -91,7 +91,7 @@ struct LdsLock {
     uint32_t tq[48];
     uint8_t ctx[kLaneCount][kCtxStride];
     M rows[NROWS][kLaneCount];
-    uint32_t ring[8][kLaneCount];  // codeword dwords, slot = dword index mod 8
+    uint32_t ring[9][kLaneCount];  // codeword dwords, slot = dword index mod 8; slot 8 repeats slot 0
     uint8_t zc[3][512];  // label by 3x3 significance window (centre ignored)
     uint8_t sc[256];     // sign label | xor<<4 by (sig,sign) of up,left,right,down
     int kmax, hmax;
@@ -129,13 +129,17 @@ __device__ __forceinline__ void lane_refill(LaneMq& m, uint32_t* ring_col, const
     u32x4 v[2];
 #pragma unroll
     for (int j = 0; j < 2; j++) v[j] = __builtin_nontemporal_load(src + j);
+    // q0 is a multiple of 4, so the first four dwords fill the ring half
+    // starting at slot q0 & 4 and the next four the other half
 #pragma unroll
     for (int j = 0; j < 2; j++) {
-        ring_col[((q0 + 4 * j + 0) & 7) * kLaneCount] = v[j].x;
-        ring_col[((q0 + 4 * j + 1) & 7) * kLaneCount] = v[j].y;
-        ring_col[((q0 + 4 * j + 2) & 7) * kLaneCount] = v[j].z;
-        ring_col[((q0 + 4 * j + 3) & 7) * kLaneCount] = v[j].w;
+        uint32_t* const h = ring_col + (((q0 & 4u) ^ (4u * j)) * kLaneCount);
+        h[0 * kLaneCount] = v[j].x;
+        h[1 * kLaneCount] = v[j].y;
+        h[2 * kLaneCount] = v[j].z;
+        h[3 * kLaneCount] = v[j].w;
     }
+    ring_col[8 * kLaneCount] = (q0 & 4u) ? v[1].x : v[0].x;  // slot 0 again
     m.rend = (q0 + 8) * 4;
 }
 
@@ -159,8 +163,9 @@ __device__ __forceinline__ void lane_fill(LaneMq& m, uint32_t* ring_col, const u
         if (m.rend - (m.base + m.bp) < 16u) lane_refill(m, ring_col, words, nwords);
     }
     const uint32_t a = m.base + m.bp + 1;
-    const uint32_t w0 = m.ring[((a >> 2) & 7) * kLaneCount];
-    const uint32_t w1 = m.ring[(((a >> 2) + 1) & 7) * kLaneCount];
+    const uint32_t* const rw = m.ring + ((a >> 2) & 7) * kLaneCount;  // slots s and s + 1 (slot 8 = slot 0)
+    const uint32_t w0 = rw[0];
+    const uint32_t w1 = rw[kLaneCount];
     const uint32_t le = __builtin_amdgcn_alignbyte(w1, w0, a & 3);  // bytes bp+1.. bp+4, the first lowest
     const uint32_t nq = ~((le << 8) | m.cur);                       // the last byte and the next three, inverted
     const bool noff = ((nq - 0x01010101u) & ~nq & 0x80808080u) == 0u;
