@@ -297,10 +297,11 @@ __device__ __forceinline__ void nibble2(M a0, M a1, M a2, M a3, int x, uint32_t&
 // c0..c3 hold the candidate columns of the stripe's rows.  A lane walks them
 // in scan order with a column cursor: `cany` holds the columns still to visit;
 // taking the lowest one loads its rows as a nibble `cur`, whose set bits are
-// then popped one per step.  TYPE 0: e0..e3 are the rows that may still join;
-// a coefficient turning significant adds its later neighbours (the row below
-// in this column through `cur`, three rows of the next column through c0..c3
-// and `cany`), so on return c0..c3 hold every position coded in the pass.
+// then popped one per step.  TYPE 0: a coefficient turning significant adds
+// its later neighbours that are inside the block (rows vrows of the stripe,
+// columns below wcols) and not significant (the row below in this column
+// through `cur`, three rows of the next column through c0..c3 and `cany`), so
+// on return c0..c3 hold every position coded in the pass.
 // TYPE 1: e0..e3 mark candidates refined before (label 16), p0..p3 the others
 // with a significant neighbour (label 15; the rest take 14); the decoded
 // 1-bits are returned as row masks in o0..o3.  TYPE 2: sany is the OR of the
@@ -312,11 +313,11 @@ __device__ __forceinline__ void stripe_symbols(LdsLock<M>& L, int lane, LaneMq& 
                                                const uint32_t* __restrict__ words, uint32_t nwords, bool work, M& c0,
                                                M& c1, M& c2, M& c3, M e0, M e1, M e2, M e3, M p0, M p1, M p2, M p3,
                                                M& o0, M& o1, M& o2, M& o3, M& sany, uint32_t cls,
-                                               uint32_t* __restrict__ out, uint32_t pitch, uint32_t pbit) {
+                                               uint32_t* __restrict__ out, uint32_t pitch, uint32_t pbit,
+                                               uint32_t vrows, int wcols) {
     auto lr = [&](int i) -> M& { return L.rows[i][lane]; };
     M cany = c0 | c1 | c2 | c3;
     uint32_t pend = PEND_NONE, cur = 0, enib = 0, pnib = 0;
-    uint32_t enx = 0;  // TYPE 0: rows of column x+1 that may still join
     uint32_t onib = 0;  // TYPE 1: refinement 1-bits of the current column
     // moves the current column's refinement 1-bits into the row masks
     auto flush = [&](int col) {
@@ -343,7 +344,6 @@ __device__ __forceinline__ void stripe_symbols(LdsLock<M>& L, int lane, LaneMq& 
                     x = (int)__builtin_ctzll((unsigned long long)cany);
                     cany &= cany - 1;
                     cur = nibble<M>(c0, c1, c2, c3, x);
-                    if (TYPE == 0) nibble2<M>(e0, e1, e2, e3, x, enib, enx);
                     if (TYPE == 1) enib = nibble<M>(e0, e1, e2, e3, x);
                     if (TYPE == 1) pnib = nibble<M>(p0, p1, p2, p3, x);
                     if (TYPE == 2) rl = cur == 15u && col3<M>(sany, x) == 0;
@@ -354,6 +354,7 @@ __device__ __forceinline__ void stripe_symbols(LdsLock<M>& L, int lane, LaneMq& 
             const M bx = (M)1 << x;
             uint32_t lab, sxor = 0;
             M s1 = 0, x1 = 0;  // TYPE 0/2: significance and sign rows r+1 as read for the labels
+            uint32_t w9 = 0;
             if (TYPE == 1) {
                 lab = ((enib >> r) & 1u) ? 16u : 14u + ((pnib >> r) & 1u);
             } else {
@@ -362,7 +363,7 @@ __device__ __forceinline__ void stripe_symbols(LdsLock<M>& L, int lane, LaneMq& 
                 // both kinds, and two branches would read the rows twice.
                 s1 = lr(RS + r + 1);
                 x1 = lr(RX + r + 1);
-                const uint32_t w9 = col3<M>(lr(RS + r), x) | (col3<M>(s1, x) << 3) | (col3<M>(lr(RS + r + 2), x) << 6);
+                w9 = col3<M>(lr(RS + r), x) | (col3<M>(s1, x) << 3) | (col3<M>(lr(RS + r + 2), x) << 6);
                 const uint32_t xw = col3<M>(lr(RX + r), x) | (col3<M>(x1, x) << 3) | (col3<M>(lr(RX + r + 2), x) << 6);
                 const uint32_t lz = L.zc[cls][w9];
                 const uint32_t e = L.sc[(w9 & 0xAAu) | ((xw & 0xAAu) >> 1)];
@@ -394,9 +395,13 @@ __device__ __forceinline__ void stripe_symbols(LdsLock<M>& L, int lane, LaneMq& 
                 if (TYPE == 0 && sig) {
                     // neighbours later in scan order may now be coded in this
                     // pass: the row below in this column (it joins the column's
-                    // nibble) and rows r-1..r+1 of column x+1; c0..c3 record all
-                    const uint32_t add0 = enib & (2u << r) & 15u;
-                    const uint32_t add1 = enx & (((7u << r) >> 1) & 15u);
+                    // nibble) and rows r-1..r+1 of column x+1, where inside the
+                    // block and not significant; c0..c3 record all.  Those
+                    // positions come later in the scan, so the window read
+                    // for the labels holds their significance at stripe entry.
+                    const uint32_t add0 = ((~w9 >> 7) & 1u) << (r + 1) & vrows & 15u;
+                    const uint32_t n1 = ((w9 >> 2) & 1u) | ((w9 >> 4) & 2u) | ((w9 >> 6) & 4u);  // column x+1, rows r-1..r+1
+                    const uint32_t add1 = x + 1 < wcols ? (((~n1 & 7u) << r) >> 1) & vrows & 15u : 0u;
                     cur |= add0;
                     c0 |= (M)((add0 & 1u) | ((add1 & 1u) << 1)) << x;
                     c1 |= (M)(((add0 >> 1) & 1u) | (add1 & 2u)) << x;
@@ -587,11 +592,12 @@ __device__ void decode_lockstep(LdsLock<M>& L, const tc_cblk_desc& d, bool activ
 
             if (kAhead && y0 + 4 < hmax) fetch(y0 + 4);
             const bool work = on && ((c0 | c1 | c2 | c3) != 0);
+            const uint32_t vrows = (y0 < h ? 1u : 0u) | (y0 + 1 < h ? 2u : 0u) | (y0 + 2 < h ? 4u : 0u) | (y0 + 3 < h ? 8u : 0u);
             // raw segments occur in significance and refinement passes only
             const bool anyraw = type != 2 && __ballot(work && m.raw) != 0;
 #define TC_STRIPE(T, R)                                                                                              \
     stripe_symbols<M, T, R>(L, lane, m, ring_col, words, nwords, work, c0, c1, c2, c3, e0, e1, e2, e3, p0, p1, p2, p3, \
-                            o0, o1, o2, o3, sany, cls, out + (uint32_t)y0 * pitch, pitch, pbit)
+                            o0, o1, o2, o3, sany, cls, out + (uint32_t)y0 * pitch, pitch, pbit, vrows, w)
             if (type == 0) {
                 if (anyraw)
                     TC_STRIPE(0, true);
