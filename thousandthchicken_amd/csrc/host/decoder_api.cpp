@@ -429,6 +429,14 @@ tc_status tc_decoder_autotune(tc_decoder d, int reps, tc_tuning* best) {
             s = measure(t, &t1, &dw);
             if (s != Status::Ok) return fail(s);
             if (t1 < best_t1 * kBetter) {
+                // one of several candidates beating the incumbent once is
+                // often noise: it must do so again
+                float again = 0, dw2 = 0;
+                s = measure(t, &again, &dw2);
+                if (s != Status::Ok) return fail(s);
+                t1 = std::max(t1, again);
+            }
+            if (t1 < best_t1 * kBetter) {
                 best_t1 = t1;
                 cur = t;
             }
@@ -442,6 +450,12 @@ tc_status tc_decoder_autotune(tc_decoder d, int reps, tc_tuning* best) {
         t.dwt_shape = sh;
         s = measure(t, &t1, &dw);
         if (s != Status::Ok) return fail(s);
+        if (dw < best_dwt * kBetter) {
+            float t1b = 0, again = 0;
+            s = measure(t, &t1b, &again);
+            if (s != Status::Ok) return fail(s);
+            dw = std::max(dw, again);
+        }
         if (dw < best_dwt * kBetter) {
             best_dwt = dw;
             cur = t;
