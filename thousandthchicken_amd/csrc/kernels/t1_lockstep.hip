@@ -73,22 +73,31 @@ constexpr int RS = 0, RX = 6, NROWS = 12;
 // Masks of the global scratch state.
 constexpr int GS = 0, GX = 1, GP = 2, GR = 3;
 
-// A context state is one byte, I | MPS << 6 (T.800 C.2); tq[I] holds
-// Qe(I) | NMPS(I) << 16 | NLPS(I) << 22 | SWITCH(I) << 28.  Each lane's 19
-// states sit in a 20-byte record (lane-major: the 64 lanes' accesses fall in
-// 64 different banks).  Bytes instead of packed 32-bit states take 3.6 KB off
-// a wave's LDS, and the waves resident per compute unit are bound by LDS.
+// A context state (T.800 C.2: index I and MPS) is one byte, st = I + 47 MPS,
+// which is also its entry in tq: Qe(I), the two state bytes a
+// renormalisation can lead to (MPS path NMPS(I) + 47 MPS in bits 16-22, LPS
+// path NLPS(I) + 47 (MPS ^ SWITCH(I)) in bits 23-29) and MPS in bit 30, so
+// the update is two field extracts.  The 94 entries fall in different LDS
+// banks except entries 64 apart (two different states), where I | MPS << 6
+// had put both MPS values of every index in one bank.  Each lane's 19 states
+// sit in a 20-byte record (lane-major: the 64 lanes' accesses fall in 64
+// different banks).  Bytes instead of packed 32-bit states take 3.6 KB off a
+// wave's LDS, and the waves resident per compute unit are bound by LDS.
 constexpr int kCtxStride = 20;
+constexpr int kStates = 47;
 
-__device__ __forceinline__ uint32_t tq_entry(int i) {
-    return (uint32_t)kQe[i] | ((uint32_t)kNmps[i] << 16) | ((uint32_t)kNlps[i] << 22) |
-           ((i == 0 || i == 6 || i == 14) ? (1u << 28) : 0u);
+__device__ __forceinline__ uint32_t tq_entry(int st) {
+    const uint32_t mps = st >= kStates ? 1u : 0u;
+    const int i = st - (int)mps * kStates;
+    const uint32_t sw = (i == 0 || i == 6 || i == 14) ? 1u : 0u;
+    return (uint32_t)kQe[i] | (((uint32_t)kNmps[i] + mps * kStates) << 16) |
+           (((uint32_t)kNlps[i] + (mps ^ sw) * kStates) << 23) | (mps << 30);
 }
 __device__ __forceinline__ uint8_t ctx_init(int c) { return (uint8_t)(c == 0 ? 4 : (c == 17 ? 3 : (c == 18 ? 46 : 0))); }
 
 template <typename M>
 struct LdsLock {
-    uint32_t tq[48];
+    uint32_t tq[2 * kStates];
     uint8_t ctx[kLaneCount][kCtxStride];
     M rows[NROWS][kLaneCount];
     uint32_t ring[9][kLaneCount];  // codeword dwords, slot = dword index mod 8; slot 8 repeats slot 0
@@ -236,23 +245,23 @@ __device__ __forceinline__ bool lock_raw_pass(uint32_t style, int k) {
     return (style & TC_MODE_BYPASS) && k >= 10 && (k - 1) % 3 != 2;
 }
 
-// DECODE, T.800 Figures C.15-C.18, on a context state byte (I | MPS << 6),
+// DECODE, T.800 Figures C.15-C.18, on a context state byte (I + 47 MPS),
 // with the codeword taken ahead by lane_fill (lsb <= 48 on entry): RENORMD is
 // one shift.
 __device__ __forceinline__ uint32_t lane_decode(LaneMq& m, uint32_t& st, const uint32_t* tq) {
     // Nearly every step has some lane that renormalises, so the exchange is
     // computed for all lanes with selects rather than behind a branch.
-    const uint32_t t = tq[st & 63u];
+    const uint32_t t = tq[st];
     const uint32_t qe = t & 0xFFFFu;
-    const uint32_t mps = (st >> 6) & 1u;
+    const uint32_t mps = (t >> 30) & 1u;
     uint32_t A = m.A - qe;
     const uint32_t chi = (uint32_t)(m.C >> 32);
     const bool lps = (chi >> 16) < qe;
     m.C = ((uint64_t)(lps ? chi : chi - (qe << 16)) << 32) | (uint32_t)m.C;
     const bool renorm = lps || A < 0x8000u;
     const bool take_mps = lps == (A < qe);  // LPS exchange: A < Qe gives MPS; MPS exchange: the reverse
-    const uint32_t smps = ((t >> 16) & 63u) | (mps << 6);
-    const uint32_t slps = ((t >> 22) & 63u) | ((mps ^ (t >> 28)) << 6);
+    const uint32_t smps = (t >> 16) & 127u;
+    const uint32_t slps = (t >> 23) & 127u;
     const uint32_t pick = take_mps ? ~0u : 0u;  // bitwise select keeps this off the branch path
     const uint32_t st2 = (smps & pick) | (slps & ~pick);
     const uint32_t d = (renorm && !take_mps) ? (mps ^ 1u) : mps;
@@ -740,7 +749,7 @@ __global__ __launch_bounds__(64) void t1_lockstep_kernel(const tc_cblk_desc* __r
                                                          int nlog_max, int bpw) {
     __shared__ LdsLock<M> L;
     const int lane = threadIdx.x;
-    for (int i = lane; i < 47; i += 64) L.tq[i] = tq_entry(i);
+    for (int i = lane; i < 2 * kStates; i += 64) L.tq[i] = tq_entry(i);
     for (int i = lane; i < 3 * 512; i += 64) {
         const uint32_t w9 = (uint32_t)(i & 511);
         L.zc[i >> 9][w9] = (uint8_t)zc_label(i >> 9, (w9 & 0xFu) | ((w9 >> 1) & 0xF0u));
