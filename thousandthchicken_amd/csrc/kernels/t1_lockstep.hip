@@ -74,9 +74,9 @@ constexpr int RS = 0, RX = 6, NROWS = 12;
 constexpr int GS = 0, GX = 1, GP = 2, GR = 3;
 
 // A context state (T.800 C.2: index I and MPS) is one byte, st = I + 47 MPS,
-// which is also its entry in tq: Qe(I), the two state bytes a
-// renormalisation can lead to (MPS path NMPS(I) + 47 MPS in bits 16-22, LPS
-// path NLPS(I) + 47 (MPS ^ SWITCH(I)) in bits 23-29) and MPS in bit 30, so
+// which is also its entry in tq: Qe(I) in bits 16-31, the two state bytes a
+// renormalisation can lead to (MPS path NMPS(I) + 47 MPS in bits 0-6, LPS
+// path NLPS(I) + 47 (MPS ^ SWITCH(I)) in bits 7-13) and MPS in bit 14, so
 // the update is two field extracts.  The 94 entries fall in different LDS
 // banks except entries 64 apart (two different states), where I | MPS << 6
 // had put both MPS values of every index in one bank.  Each lane's 19 states
@@ -90,8 +90,8 @@ __device__ __forceinline__ uint32_t tq_entry(int st) {
     const uint32_t mps = st >= kStates ? 1u : 0u;
     const int i = st - (int)mps * kStates;
     const uint32_t sw = (i == 0 || i == 6 || i == 14) ? 1u : 0u;
-    return (uint32_t)kQe[i] | (((uint32_t)kNmps[i] + mps * kStates) << 16) |
-           (((uint32_t)kNlps[i] + (mps ^ sw) * kStates) << 23) | (mps << 30);
+    return ((uint32_t)kQe[i] << 16) | ((uint32_t)kNmps[i] + mps * kStates) |
+           (((uint32_t)kNlps[i] + (mps ^ sw) * kStates) << 7) | (mps << 14);
 }
 __device__ __forceinline__ uint8_t ctx_init(int c) { return (uint8_t)(c == 0 ? 4 : (c == 17 ? 3 : (c == 18 ? 46 : 0))); }
 
@@ -108,7 +108,7 @@ struct LdsLock {
 
 // Per-lane MQ decoder reading its codeword from the LDS ring.
 struct LaneMq {
-    uint32_t A;
+    uint32_t A;      // MQ: T.800's A register in bits 31..16
     uint64_t C;      // MQ: T.800's C register in bits 63..32, codeword bytes taken ahead below
     int lsb;         // MQ: bit of C holding the lowest bit of the last byte taken
     int CT;          // raw segments: bits of `cur` still unread
@@ -216,7 +216,7 @@ __device__ __forceinline__ void lane_init(LaneMq& m, uint32_t* ring_col, const u
     m.C = (uint64_t)m.cur << 55;
     m.lsb = 55;
     lane_fill(m, ring_col, words, nwords);
-    m.A = 0x8000;
+    m.A = 0x8000u << 16;
 }
 
 // One raw bit (BYPASS segments, T.800 D.6): MSB first; the byte after 0xFF
@@ -251,23 +251,25 @@ __device__ __forceinline__ bool lock_raw_pass(uint32_t style, int k) {
 __device__ __forceinline__ uint32_t lane_decode(LaneMq& m, uint32_t& st, const uint32_t* tq) {
     // Nearly every step has some lane that renormalises, so the exchange is
     // computed for all lanes with selects rather than behind a branch.
+    // A and Qe are held in bits 31..16, so C's top half compares with Qe
+    // directly and the renormalisation shift is the leading zero count.
     const uint32_t t = tq[st];
-    const uint32_t qe = t & 0xFFFFu;
-    const uint32_t mps = (t >> 30) & 1u;
+    const uint32_t qe = t & 0xFFFF0000u;
+    const uint32_t mps = (t >> 14) & 1u;
     uint32_t A = m.A - qe;
     const uint32_t chi = (uint32_t)(m.C >> 32);
-    const bool lps = (chi >> 16) < qe;
-    m.C = ((uint64_t)(lps ? chi : chi - (qe << 16)) << 32) | (uint32_t)m.C;
-    const bool renorm = lps || A < 0x8000u;
+    const bool lps = chi < qe;
+    m.C = ((uint64_t)(lps ? chi : chi - qe) << 32) | (uint32_t)m.C;
+    const bool renorm = lps || A < 0x80000000u;
     const bool take_mps = lps == (A < qe);  // LPS exchange: A < Qe gives MPS; MPS exchange: the reverse
-    const uint32_t smps = (t >> 16) & 127u;
-    const uint32_t slps = (t >> 23) & 127u;
+    const uint32_t smps = t & 127u;
+    const uint32_t slps = (t >> 7) & 127u;
     const uint32_t pick = take_mps ? ~0u : 0u;  // bitwise select keeps this off the branch path
     const uint32_t st2 = (smps & pick) | (slps & ~pick);
     const uint32_t d = (renorm && !take_mps) ? (mps ^ 1u) : mps;
     st = renorm ? st2 : st;
     A = lps ? qe : A;
-    const int n = __clz(A) - 16;  // 0 unless renormalising
+    const int n = __builtin_clz(A);  // 0 unless renormalising (A > 0: Qe >= 1 and A - Qe > 0)
     m.A = A << n;
     m.C <<= n;
     m.lsb += n;
