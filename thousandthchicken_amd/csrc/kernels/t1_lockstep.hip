@@ -284,6 +284,19 @@ __device__ __forceinline__ uint32_t col3(M m, int x) {
     return ((uint32_t)(m >> sh) << (x == 0 ? 1 : 0)) & 7u;
 }
 
+// The positions of one stripe row that a significance pass coded: not
+// significant at stripe entry, inside the block (valid), and with a
+// significant neighbour when the scan reached them.  A neighbour earlier in
+// the scan (in the row above, the one to the left, the row below's left one)
+// counts with its state after the stripe (*_end), a later one (the row
+// above's right one, the one to the right, the row below's centre and right)
+// with its state at entry (*0).  Column x of a mask is bit x.
+template <typename M>
+__device__ __forceinline__ M sigprop_coded(M up_end, M up0, M row_end, M row0, M dn_end, M dn0, M valid) {
+    return ~row0 & valid &
+           ((up_end << 1) | up_end | (up0 >> 1) | (row_end << 1) | (row0 >> 1) | (dn_end << 1) | dn0 | (dn0 >> 1));
+}
+
 // Bit x of four row masks as a 4-bit column nibble (row 0 in bit 0).
 template <typename M>
 __device__ __forceinline__ uint32_t nibble(M a0, M a1, M a2, M a3, int x) {
@@ -310,9 +323,12 @@ __device__ __forceinline__ void nibble2(M a0, M a1, M a2, M a3, int x, uint32_t&
 // taking the lowest one loads its rows as a nibble `cur`, whose set bits are
 // then popped one per step.  TYPE 0: a coefficient turning significant adds
 // its later neighbours that are inside the block (rows vrows of the stripe,
-// columns below wcols) and not significant (the row below in this column
-// through `cur`, three rows of the next column through c0..c3 and `cany`), so
-// on return c0..c3 hold every position coded in the pass.
+// columns below wcols) and not significant: the row below in this column to
+// `cur`, three rows of the next column to the nibble `nxt` that joins that
+// column's rows when the walk reaches it (and the column to `cany`, so the
+// walk's next column is that one).  c0..c3 are not changed; the positions
+// coded in the pass are found from the stripe's rows afterwards
+// (sigprop_coded).
 // TYPE 1: e0..e3 mark candidates refined before (label 16), p0..p3 the others
 // with a significant neighbour (label 15; the rest take 14); the decoded
 // 1-bits are returned as row masks in o0..o3.  TYPE 2: sany is the OR of the
@@ -329,6 +345,7 @@ __device__ __forceinline__ void stripe_symbols(LdsLock<M>& L, int lane, LaneMq& 
     auto lr = [&](int i) -> M& { return L.rows[i][lane]; };
     M cany = c0 | c1 | c2 | c3;
     uint32_t pend = PEND_NONE, cur = 0, enib = 0, pnib = 0;
+    uint32_t nxt = 0;  // TYPE 0: rows of the next column that joined
     uint32_t onib = 0;  // TYPE 1: refinement 1-bits of the current column
     // moves the current column's refinement 1-bits into the row masks
     auto flush = [&](int col) {
@@ -355,6 +372,10 @@ __device__ __forceinline__ void stripe_symbols(LdsLock<M>& L, int lane, LaneMq& 
                     x = (int)__builtin_ctzll((unsigned long long)cany);
                     cany &= cany - 1;
                     cur = nibble<M>(c0, c1, c2, c3, x);
+                    if (TYPE == 0) {
+                        cur |= nxt;
+                        nxt = 0;
+                    }
                     if (TYPE == 1) enib = nibble<M>(e0, e1, e2, e3, x);
                     if (TYPE == 1) pnib = nibble<M>(p0, p1, p2, p3, x);
                     if (TYPE == 2) rl = cur == 15u && col3<M>(sany, x) == 0;
@@ -406,18 +427,16 @@ __device__ __forceinline__ void stripe_symbols(LdsLock<M>& L, int lane, LaneMq& 
                 if (TYPE == 0 && sig) {
                     // neighbours later in scan order may now be coded in this
                     // pass: the row below in this column (it joins the column's
-                    // nibble) and rows r-1..r+1 of column x+1, where inside the
-                    // block and not significant; c0..c3 record all.  Those
-                    // positions come later in the scan, so the window read
-                    // for the labels holds their significance at stripe entry.
+                    // nibble) and rows r-1..r+1 of column x+1 (they join the
+                    // next column's), where inside the block and not
+                    // significant.  Those positions come later in the scan, so
+                    // the window read for the labels holds their significance
+                    // at stripe entry.
                     const uint32_t add0 = ((~w9 >> 7) & 1u) << (r + 1) & vrows & 15u;
                     const uint32_t n1 = ((w9 >> 2) & 1u) | ((w9 >> 4) & 2u) | ((w9 >> 6) & 4u);  // column x+1, rows r-1..r+1
                     const uint32_t add1 = x + 1 < wcols ? (((~n1 & 7u) << r) >> 1) & vrows & 15u : 0u;
                     cur |= add0;
-                    c0 |= (M)((add0 & 1u) | ((add1 & 1u) << 1)) << x;
-                    c1 |= (M)(((add0 >> 1) & 1u) | (add1 & 2u)) << x;
-                    c2 |= (M)(((add0 >> 2) & 1u) | ((add1 >> 1) & 2u)) << x;
-                    c3 |= (M)(((add0 >> 3) & 1u) | ((add1 >> 2) & 2u)) << x;
+                    nxt |= add1;
                     cany |= add1 ? (bx << 1) : (M)0;
                 }
                 if (TYPE == 2 && !zc && !sgn) {
@@ -543,7 +562,7 @@ __device__ void decode_lockstep(LdsLock<M>& L, const tc_cblk_desc& d, bool activ
             const bool on = pass_on && y0 < h;
             if (!kAhead) fetch(y0);
             M c0 = 0, c1 = 0, c2 = 0, c3 = 0;  // candidate columns per stripe row
-            M e0 = 0, e1 = 0, e2 = 0, e3 = 0;  // sigprop: rows that may still join; magref: label-16 candidates
+            M e0 = 0, e1 = 0, e2 = 0, e3 = 0;  // sigprop: significance at entry; magref: label-16 candidates
             M p0 = 0, p1 = 0, p2 = 0, p3 = 0;  // magref: label-15 candidates
             M sany = 0;                        // OR of the six significance rows
             M o0 = 0, o1 = 0, o2 = 0, o3 = 0;  // magref: 1-bits decoded in the stripe
@@ -574,8 +593,8 @@ __device__ void decode_lockstep(LdsLock<M>& L, const tc_cblk_desc& d, bool activ
                 for (int r = 0; r < 4; r++) {
                     const M vr = (y0 + r < h) ? VM : (M)0;
                     if (type == 0) {
-                        er[r] = ~S[r + 1] & vr;
-                        cr[r] = er[r] & spread<M>(S[r] | S[r + 1] | S[r + 2]);
+                        er[r] = S[r + 1];  // significance at entry, for sigprop_coded
+                        cr[r] = ~S[r + 1] & vr & spread<M>(S[r] | S[r + 1] | S[r + 2]);
                     } else if (type == 1) {
                         cr[r] = S[r + 1] & ~P[r];
                         er[r] = cr[r] & R[r];
@@ -614,6 +633,15 @@ __device__ void decode_lockstep(LdsLock<M>& L, const tc_cblk_desc& d, bool activ
                     TC_STRIPE(0, true);
                 else
                     TC_STRIPE(0, false);
+                // c0..c3 become the positions coded in the pass; the rows
+                // above and below the stripe have not changed
+                const M E0 = lr(RS + 0), E1 = lr(RS + 1), E2 = lr(RS + 2), E3 = lr(RS + 3), E4 = lr(RS + 4),
+                        E5 = lr(RS + 5);
+                auto vrow = [&](int r) -> M { return (y0 + r < h) ? VM : (M)0; };
+                c0 = sigprop_coded<M>(E0, E0, E1, e0, E2, e1, vrow(0));
+                c1 = sigprop_coded<M>(E1, e0, E2, e1, E3, e2, vrow(1));
+                c2 = sigprop_coded<M>(E2, e1, E3, e2, E4, e3, vrow(2));
+                c3 = sigprop_coded<M>(E3, e2, E4, e3, E5, E5, vrow(3));
             } else if (type == 1) {
                 if (anyraw)
                     TC_STRIPE(1, true);
