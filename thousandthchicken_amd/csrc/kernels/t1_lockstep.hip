@@ -65,11 +65,13 @@ constexpr int kLaneCount = 64;
 enum : uint32_t { PEND_NONE = 0, PEND_SC = 1, PEND_UNI1 = 2, PEND_UNI2 = 3 };
 
 // Staged stripe rows, index into LdsLock::rows: significance rows y0-1..y0+4,
-// then sign rows y0-1..y0+4.  A refinement pass stages nothing: significance
+// then sign rows y0-1..y0+4.  A refinement pass stages no state: significance
 // is fixed during it, so its context labels are formed as masks at stripe
-// entry.  The coded-in-this-pass rows are never read inside a pass and stay
-// in registers.
+// entry; the same rows then hold its decoded 1-bits (RO), its label-16
+// candidates (RE) and label-15 candidates (RP), rows y0..y0+3 each.  The
+// coded-in-this-pass rows are never read inside a pass and stay in registers.
 constexpr int RS = 0, RX = 6, NROWS = 12;
+constexpr int RO = 0, RE = 4, RP = 8;
 // Masks of the global scratch state.
 constexpr int GS = 0, GX = 1, GP = 2, GR = 3;
 
@@ -297,6 +299,12 @@ __device__ __forceinline__ M sigprop_coded(M up_end, M up0, M row_end, M row0, M
            ((up_end << 1) | up_end | (up0 >> 1) | (row_end << 1) | (row0 >> 1) | (dn_end << 1) | dn0 | (dn0 >> 1));
 }
 
+// OR into a lane's own LDS word (no return value: one LDS instruction).
+__device__ __forceinline__ void lds_or(uint32_t* p, uint32_t v) { (void)atomicOr(p, v); }
+__device__ __forceinline__ void lds_or(uint64_t* p, uint64_t v) {
+    (void)atomicOr(reinterpret_cast<unsigned long long*>(p), (unsigned long long)v);
+}
+
 // Bit x of four row masks as a 4-bit column nibble (row 0 in bit 0).
 template <typename M>
 __device__ __forceinline__ uint32_t nibble(M a0, M a1, M a2, M a3, int x) {
@@ -329,9 +337,10 @@ __device__ __forceinline__ void nibble2(M a0, M a1, M a2, M a3, int x, uint32_t&
 // walk's next column is that one).  c0..c3 are not changed; the positions
 // coded in the pass are found from the stripe's rows afterwards
 // (sigprop_coded).
-// TYPE 1: e0..e3 mark candidates refined before (label 16), p0..p3 the others
-// with a significant neighbour (label 15; the rest take 14); the decoded
-// 1-bits are returned as row masks in o0..o3.  TYPE 2: sany is the OR of the
+// TYPE 1: the staged rows RE mark candidates refined before (label 16), RP
+// the others with a significant neighbour (label 15; the rest take 14); each
+// decoded 1-bit is ORed into the staged rows RO, returned as row masks in
+// o0..o3.  TYPE 2: sany is the OR of the
 // six staged significance rows (run-length test).  out points at row y0.
 // RAW: some lane of the wave is in a raw (BYPASS) segment; without it the
 // raw paths are not compiled into the symbol loop.
@@ -344,18 +353,8 @@ __device__ __forceinline__ void stripe_symbols(LdsLock<M>& L, int lane, LaneMq& 
                                                uint32_t vrows, int wcols) {
     auto lr = [&](int i) -> M& { return L.rows[i][lane]; };
     M cany = c0 | c1 | c2 | c3;
-    uint32_t pend = PEND_NONE, cur = 0, enib = 0, pnib = 0;
+    uint32_t pend = PEND_NONE, cur = 0;
     uint32_t nxt = 0;  // TYPE 0: rows of the next column that joined
-    uint32_t onib = 0;  // TYPE 1: refinement 1-bits of the current column
-    // moves the current column's refinement 1-bits into the row masks
-    auto flush = [&](int col) {
-        const M bc = (M)1 << col;
-        o0 |= (onib & 1u) ? bc : (M)0;
-        o1 |= (onib & 2u) ? bc : (M)0;
-        o2 |= (onib & 4u) ? bc : (M)0;
-        o3 |= (onib & 8u) ? bc : (M)0;
-        onib = 0;
-    };
     int x = 0, r = 0;
     // (a do-while, tested at the bottom: with the test at the top the compiler
     // copied the lane state twice per step in the refinement loop)
@@ -368,7 +367,6 @@ __device__ __forceinline__ void stripe_symbols(LdsLock<M>& L, int lane, LaneMq& 
             if (pend == PEND_NONE) {
                 if (cur == 0) {
                     // next candidate column
-                    if (TYPE == 1) flush(x);
                     x = (int)__builtin_ctzll((unsigned long long)cany);
                     cany &= cany - 1;
                     cur = nibble<M>(c0, c1, c2, c3, x);
@@ -376,8 +374,6 @@ __device__ __forceinline__ void stripe_symbols(LdsLock<M>& L, int lane, LaneMq& 
                         cur |= nxt;
                         nxt = 0;
                     }
-                    if (TYPE == 1) enib = nibble<M>(e0, e1, e2, e3, x);
-                    if (TYPE == 1) pnib = nibble<M>(p0, p1, p2, p3, x);
                     if (TYPE == 2) rl = cur == 15u && col3<M>(sany, x) == 0;
                 }
                 r = __builtin_ctz(cur);
@@ -388,7 +384,7 @@ __device__ __forceinline__ void stripe_symbols(LdsLock<M>& L, int lane, LaneMq& 
             M s1 = 0, x1 = 0;  // TYPE 0/2: significance and sign rows r+1 as read for the labels
             uint32_t w9 = 0;
             if (TYPE == 1) {
-                lab = ((enib >> r) & 1u) ? 16u : 14u + ((pnib >> r) & 1u);
+                lab = 14u + ((uint32_t)(lr(RP + r) >> x) & 1u) + (((uint32_t)(lr(RE + r) >> x) & 1u) << 1);
             } else {
                 // Zero-coding and sign-coding labels from one read of the
                 // staged rows, for every lane: nearly every step has lanes of
@@ -407,7 +403,7 @@ __device__ __forceinline__ void stripe_symbols(LdsLock<M>& L, int lane, LaneMq& 
             L.ctx[lane][lab] = (uint8_t)st;
             uint32_t* const at = out + (uint32_t)r * pitch + x;
             if (TYPE == 1) {
-                onib |= bit << r;
+                if (bit) lds_or(&lr(RO + r), bx);
             } else {
                 // The decision's effect on the staged rows, for every lane:
                 // rows r+1 are written back (changed where a zero-coding 1
@@ -460,7 +456,12 @@ __device__ __forceinline__ void stripe_symbols(LdsLock<M>& L, int lane, LaneMq& 
             work = pend != PEND_NONE || cur != 0 || cany != 0;
         }
     } while (__ballot(work));
-    if (TYPE == 1) flush(x);
+    if (TYPE == 1) {
+        o0 = lr(RO + 0);
+        o1 = lr(RO + 1);
+        o2 = lr(RO + 2);
+        o3 = lr(RO + 3);
+    }
 }
 
 // g: the wave's state rows [mask][row][lane]; lg: its refinement logs
@@ -614,6 +615,9 @@ __device__ void decode_lockstep(LdsLock<M>& L, const tc_cblk_desc& d, bool activ
                         pr[r] = cr[r] & ~R[r] & nb;
                         // after the pass every coefficient significant before it has been refined
                         gat(GR, y0 + r) = cr[r];
+                        lr(RO + r) = (M)0;
+                        lr(RE + r) = er[r];
+                        lr(RP + r) = pr[r];
                     }
                     p0 = pr[0], p1 = pr[1], p2 = pr[2], p3 = pr[3];
                     carry_s = S[4];
