@@ -154,7 +154,8 @@ __device__ __forceinline__ void lane_refill(LaneMq& m, uint32_t* ring_col, const
     m.rend = (q0 + 8) * 4;
 }
 
-// Takes codeword bytes into C below the last one (called while lsb >= 32).
+// Takes codeword bytes into C below the last one, as many whole bytes as
+// there is room for (lsb >= 8 on entry; at most four, lsb >= 0 after).
 //
 // T.800's BYTEIN (Figure C.19) adds the byte after B at 8 bits below it, or
 // 7 after B = 0xFF; after 0xFF a byte above 0x8F is a marker, which feeds
@@ -164,9 +165,9 @@ __device__ __forceinline__ void lane_refill(LaneMq& m, uint32_t* ring_col, const
 // exception: a byte of 0x80-0x8F after 0xFF overlaps the 0xFF's lowest bit
 // and carries through it.  BYTEIN takes that byte when the 0xFF's lowest bit
 // has been shifted past bit 48, so it is held back until then.  The common
-// case (no 0xFF among the last byte and the next three, all four inside the
-// codeword) takes four bytes with one add; after a marker or the end, 32
-// one-bits.  The rest goes byte by byte.  Bytes up to lsb <= 48 are always
+// case (k = min(4, lsb / 8) bytes, no 0xFF among the last byte and the next
+// k - 1, all k inside the codeword) takes them with one add; after a marker
+// or the end, 8k one-bits.  The rest goes byte by byte.  Bytes up to lsb <= 48 are always
 // held when Qe is compared, so every decision equals T.800's.
 __device__ __forceinline__ void lane_fill(LaneMq& m, uint32_t* ring_col, const uint32_t* __restrict__ words,
                                          uint32_t nwords) {
@@ -178,18 +179,25 @@ __device__ __forceinline__ void lane_fill(LaneMq& m, uint32_t* ring_col, const u
     const uint32_t w0 = rw[0];
     const uint32_t w1 = rw[kLaneCount];
     const uint32_t le = __builtin_amdgcn_alignbyte(w1, w0, a & 3);  // bytes bp+1.. bp+4, the first lowest
+    const uint32_t k = min(4u, (uint32_t)m.lsb >> 3);              // bytes there is room for
     const uint32_t nq = ~((le << 8) | m.cur);                       // the last byte and the next three, inverted
-    const bool noff = ((nq - 0x01010101u) & ~nq & 0x80808080u) == 0u;
-    const bool inside = m.bp + 4 < m.len;
+    // a zero byte among the low k of nq (borrows only run upwards, so the
+    // test is exact for the low bytes)
+    const uint32_t lowk = (uint32_t)((1ull << (8 * k)) - 1u);
+    const bool noff = ((nq - 0x01010101u) & ~nq & 0x80808080u & lowk) == 0u;
+    const bool inside = m.bp + k < m.len;
     const bool marker = m.cur == 0xFFu && (m.bp + 1 >= m.len || (le & 0xFFu) > 0x8Fu);
     if ((noff && inside) || marker) {
-        const uint32_t word = marker ? 0xFFFFFFFFu : __builtin_bswap32(le);
-        m.lsb -= 32;
-        m.C += (uint64_t)word << m.lsb;
-        m.bp += marker ? 0u : 4u;
-        m.cur = marker ? 0xFFu : (le >> 24);
+        // the first k bytes, placed below bit lsb
+        const uint32_t word =
+            (marker ? 0xFFFFFFFFu : __builtin_bswap32(le)) & (uint32_t)(0xFFFFFFFF00000000ull >> (8 * k));
+        m.C += ((uint64_t)word << 32) >> (64 - m.lsb);
+        m.lsb -= (int)(8 * k);
+        m.bp += marker ? 0u : k;
+        m.cur = marker ? 0xFFu : ((le >> (8 * k - 8)) & 0xFFu);
     } else {
         for (int i = 0; i < 4; i++) {
+            if (m.lsb < 8) break;  // no room for another byte
             const uint32_t b = lane_byte(m, m.bp + 1);
             if (m.cur == 0xFFu) {
                 if (b > 0x8Fu) {  // marker: 0xFF, no advance
@@ -360,9 +368,13 @@ __device__ __forceinline__ void stripe_symbols(LdsLock<M>& L, int lane, LaneMq& 
     // copied the lane state twice per step in the refinement loop)
     if (__ballot(work)) do {
         if (work) {
-            // codeword bytes for the lanes whose register runs low
+            // Codeword bytes: whenever one lane's register runs low, every
+            // lane tops up, so the lanes' fills fall on the same steps and
+            // the wave runs the fill far less often than once per lane.
             const bool raw = RAW && m.raw;
-            if (!raw && m.lsb >= 32) lane_fill(m, ring_col, words, nwords);
+            if (__ballot(!raw && m.lsb >= 32)) {
+                if (!raw && m.lsb >= 8) lane_fill(m, ring_col, words, nwords);
+            }
             bool rl = false;
             if (pend == PEND_NONE) {
                 if (cur == 0) {
