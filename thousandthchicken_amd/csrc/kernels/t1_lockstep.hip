@@ -417,17 +417,15 @@ __device__ __forceinline__ void stripe_symbols(LdsLock<M>& L, int lane, LaneMq& 
             if (TYPE == 1) {
                 if (bit) lds_or(&lr(RO + r), bx);
             } else {
-                // The decision's effect on the staged rows, for every lane:
-                // rows r+1 are written back (changed where a zero-coding 1
-                // or a sign 1 sets their bit) from the values read for the
-                // labels, without a branch.
+                // The decision's effect on the staged rows: a zero-coding 1
+                // or a sign 1 sets its bit with one LDS OR.
                 const bool zc = pend == PEND_NONE && !rl;
                 const bool sig = zc && bit != 0u;
                 const bool sgn = pend == PEND_SC;
                 // a raw (BYPASS) segment carries the sign bit itself (T.800 D.6)
                 const uint32_t sg = bit ^ (raw ? 0u : sxor);
-                lr(RS + r + 1) = sig ? (s1 | bx) : s1;
-                lr(RX + r + 1) = (sgn && sg) ? (x1 | bx) : x1;
+                if (sig) lds_or(&lr(RS + r + 1), bx);
+                if (sgn && sg) lds_or(&lr(RX + r + 1), bx);
                 if (TYPE == 2) sany |= sig ? bx : (M)0;
                 if (sgn) *at = pbit | (sg << 31);  // the word's first bits: the plane was zero here
                 const uint32_t pend0 = pend;
@@ -459,7 +457,7 @@ __device__ __forceinline__ void stripe_symbols(LdsLock<M>& L, int lane, LaneMq& 
                     } else {
                         r |= (int)bit;
                         cur &= ~((2u << r) - 1u);  // rows above r were coded as insignificant
-                        lr(RS + r + 1) |= bx;
+                        lds_or(&lr(RS + r + 1), bx);
                         sany |= bx;
                         pend = PEND_SC;
                     }
