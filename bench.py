@@ -263,6 +263,35 @@ def traffic_record(kernel, workload):
         return None
 
 
+def valu_record(workload):
+    """Wave-wide VALU instructions of one batch's tier-1 launches at a given
+    packing, counted with SQ_INSTS_VALU (profiles/valu_<workload>.json)."""
+    try:
+        with open(os.path.join(ROOT, "profiles", f"valu_{workload}.json")) as f:
+            return json.load(f)
+    except (OSError, ValueError):
+        return None
+
+
+def roofline_valu(flight, workload):
+    """Tier-1's VALU issue rate in the chosen steady state against the
+    machine's, when the counted packing is the chosen one (else None).  The
+    time per batch also holds the DWT stages, so this understates tier-1's own
+    rate."""
+    rec = valu_record(workload)
+    if rec is None or flight["in_flight"] < 2 or rec.get("packing") != {"bpw_small": flight["bpw_small"],
+                                                                        "bpw_large": flight["bpw_large"]}:
+        return None
+    achieved = rec["valu_tier1_per_batch"] / (flight["ms_per_batch"] * 1e-3)
+    peak = rec["peak_wave_instructions_per_s"]
+    return {"bound": "valu", "kernel": "t1_lockstep_kernel (both block classes)",
+            "achieved": round(achieved / 1e9, 1), "peak": round(peak / 1e9, 1), "unit": "G wave-instructions/s",
+            "frac": round(achieved / peak, 4), "valu_per_batch": rec["valu_tier1_per_batch"],
+            "source": f"profiles/valu_{workload}.json",
+            "note": "tier-1 VALU instructions of one batch (counted) over the measured time per batch in flight; "
+                    "peak: 1024 SIMDs x 2.4 GHz, one wave-wide instruction per 4 cycles"}
+
+
 def tuned_decoder(tca, args):
     dec = tca.Decoder()
     dec.set_blocks_per_wave(False, args.bpw_large)
@@ -690,6 +719,7 @@ def main():
                      "note": "tier-1 is bound by the serial MQ symbol chains of its blocks, not by HBM; bytes = "
                              "codeword bytes + 4 B per coefficient of the launch's blocks, over the launch's own "
                              "time (HIP events on the stream it runs on)"},
+        "roofline_valu": roofline_valu(flight, "cfg4_mixed_256"),
         "roofline_dwt": {"bound": "hbm", "achieved": round(dwt_gbs, 2), "peak": HBM_PEAK_GBS, "unit": "GB/s",
                          "frac": round(dwt_gbs / HBM_PEAK_GBS, 5)},
         "blocks_per_wave": {"small": s.bpw_small, "large": s.bpw_large},
