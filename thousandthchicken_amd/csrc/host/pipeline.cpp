@@ -258,7 +258,7 @@ Status build_plan(const std::vector<const Image*>& imgs, const Tuning& tuning, i
         }
         if (!heavy.empty()) heavy_groups.push_back(group_of(heavy, small, 1, true));
         if (!v.empty()) {
-            sort_for_packing(v, descs);
+            sort_for_packing(v, descs, small);
             main_groups.push_back(group_of(v, small, bpw, false));
         }
         P.bpw[k] = v.empty() ? 1 : bpw;

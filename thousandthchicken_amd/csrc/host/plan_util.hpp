@@ -31,9 +31,9 @@ Status plan_tile_cblks(const Image& im, int t, const uint64_t* plane, uint64_t& 
                        std::vector<tc_cblk_desc>& descs, std::vector<uint32_t> (&cls)[2], std::string& err);
 
 // Orders a class of blocks for packing into waves: equal pass counts, then
-// equal stripe counts, then similar codeword lengths side by side, heaviest
-// first.
-void sort_for_packing(std::vector<uint32_t>& idx, const std::vector<tc_cblk_desc>& descs);
+// equal stripe counts (then, for the class up to 32x32, equal widths), then
+// similar codeword lengths side by side, heaviest first.
+void sort_for_packing(std::vector<uint32_t>& idx, const std::vector<tc_cblk_desc>& descs, bool small);
 
 // Work-buffer sizes (elements) of the two ping-pong buffers of a tile-component.
 void work_sizes(const TileComp& tc, uint64_t sz[2]);

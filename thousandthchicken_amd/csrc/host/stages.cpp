@@ -117,7 +117,7 @@ void CoefficientCoder::decode_tile(type_tile* tile) {
     std::vector<uint8_t> blob(total, 0);
     std::memcpy(blob.data(), cw.data(), cw.size());
     std::memcpy(blob.data() + off_desc, descs.data(), descs.size() * sizeof(tc_cblk_desc));
-    for (int k = 0; k < 2; k++) sort_for_packing(cls[k], descs);
+    for (int k = 0; k < 2; k++) sort_for_packing(cls[k], descs, k == 0);
     if (!cls[0].empty()) std::memcpy(blob.data() + off_o0, cls[0].data(), cls[0].size() * 4);
     if (!cls[1].empty()) std::memcpy(blob.data() + off_o1, cls[1].data(), cls[1].size() * 4);
     ensure(queue_, S->dev, d_upload_, total);
