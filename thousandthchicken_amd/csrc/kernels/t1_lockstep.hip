@@ -368,11 +368,12 @@ __device__ __forceinline__ void stripe_symbols(LdsLock<M>& L, int lane, LaneMq& 
     // copied the lane state twice per step in the refinement loop)
     if (__ballot(work)) do {
         if (work) {
-            // Codeword bytes: whenever one lane's register runs low, every
-            // lane tops up, so the lanes' fills fall on the same steps and
-            // the wave runs the fill far less often than once per lane.
+            // Codeword bytes: whenever one lane's register runs low (a
+            // decision needs the bits down to 48), every lane tops up, so the
+            // lanes' fills fall on the same steps and the wave runs the fill
+            // far less often than once per lane.
             const bool raw = RAW && m.raw;
-            if (__ballot(!raw && m.lsb >= 32)) {
+            if (__ballot(!raw && m.lsb > 48)) {
                 if (!raw && m.lsb >= 8) lane_fill(m, ring_col, words, nwords);
             }
             bool rl = false;
